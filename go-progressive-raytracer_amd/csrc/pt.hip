@@ -1,0 +1,584 @@
+// pt.hip — wavefront path tracer and device-resident progressive film.
+//
+// Estimator: PathTracingIntegrator.rayColorRecursive (pkg/integrator/path_tracing.go:35-234)
+// unrolled into bounce stages over compacted path queues:
+//   camera -> [extend (closest hit) -> shade (emission, BSDF sample, NEE, MIS, RR) -> shadow (any hit)]*
+// The recursion's return value is linear in the child radiance, so each level's factor
+// (Russian-roulette compensation, specular attenuation or att*cos*w/pdf including the MIS
+// weight applied to the whole recursive radiance, :172-184) is folded into a path weight
+// `beta`; the Russian-roulette throughput `thr` is tracked separately exactly as the
+// recursion passes it (:178, :91). Film accumulation follows PixelStats (pkg/renderer/stats.go)
+// with Kahan-compensated float32 sums; the per-pixel order of additions is fixed, so a
+// render is bit-reproducible independent of scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "prt_internal.h"
+#include "shading.h"
+#include "traverse.h"
+
+namespace prt {
+
+constexpr int kBlock = 256;
+constexpr int kLds = 16;
+
+struct PathBuffers {
+    float4* o;      // origin
+    float4* d;      // direction (unnormalised as the reference passes scatter.Scattered)
+    float4* beta;   // path weight
+    float4* thr;    // Russian-roulette throughput
+    float4* L;      // radiance gathered so far
+    float4* hit;    // t, prim (bits), u, v
+    float4* so;     // shadow ray origin, tmin
+    float4* sd;     // shadow ray direction, tmax
+    float4* sc;     // shadow contribution
+    uint32_t* key;
+    uint32_t* pixel;
+    uint32_t* queue[2];
+    uint32_t* squeue;
+    uint32_t* counters;  // [0],[1]: path queues, [2]: shadow queue
+};
+
+struct PassParams {
+    int32_t width, height;
+    int32_t px0, npix;      // pixel range of this chunk (linear pixel ids)
+    int32_t spp;            // samples per pixel in this chunk
+    int32_t sample_base;    // first sample index
+    int32_t max_depth, rr_min;
+    uint64_t seed;
+};
+
+// wave-aggregated append to a queue
+__device__ __forceinline__ void queue_push(uint32_t* q, uint32_t* counter, uint32_t value, bool active) {
+    uint64_t mask = __ballot(active);
+    if (mask == 0) return;
+    uint32_t lane = threadIdx.x & 63u;
+    uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (active) {
+        uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+        q[base + rank] = value;
+    }
+}
+
+// Russian roulette at the start of bounce level `b` (ApplyRussianRoulette, :193-220).
+// Returns false if the path terminates; multiplies beta by the compensation otherwise.
+__device__ __forceinline__ bool russian_roulette(const PassParams& pp, uint32_t key, int b, V3 thr, V3& beta) {
+    float u = draw(key, (uint32_t)(b + 1) * 16u);
+    if (b < pp.rr_min) return true;
+    float q = fminf(0.95f, fmaxf(0.5f, lum(thr)));
+    if (u > q) return false;
+    beta = beta * (1.0f / q);
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, PathBuffers pb) {
+    const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
+    uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+    bool active = false;
+    if (slot < n) {
+        uint32_t s_local = slot / (uint32_t)pp.npix;
+        uint32_t pix = (uint32_t)pp.px0 + slot % (uint32_t)pp.npix;
+        int i = (int)(pix % (uint32_t)pp.width), j = (int)(pix / (uint32_t)pp.width);
+        uint32_t key = path_key(pp.seed, pix, (uint32_t)pp.sample_base + s_local);
+        // GetRay(i, j, Get2D(), Get2D()) (tile_renderer.go:56)
+        float l0 = draw(key, 0), l1 = draw(key, 1), j0 = draw(key, 2), j1 = draw(key, 3);
+        V3 o, d;
+        camera_ray(sc.cam, i, j, l0, l1, j0, j1, o, d);
+        V3 beta = v3(1, 1, 1);
+        if (pp.max_depth > 0) active = russian_roulette(pp, key, 0, v3(1, 1, 1), beta);
+        pb.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
+        pb.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
+        pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
+        pb.thr[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pb.key[slot] = key;
+        pb.pixel[slot] = pix;
+    }
+    queue_push(pb.queue[0], &pb.counters[0], slot, active);
+}
+
+template <bool ANY, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, int bounce, uint2* gstack,
+                                                  uint32_t gstride) {
+    __shared__ uint32_t s_node[kLds * kBlock];
+    __shared__ float s_t[kLds * kBlock];
+    TStack<kLds, kBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kBlock + threadIdx.x;
+    st.gstride = gstride;
+    const uint32_t cur = (uint32_t)bounce & 1u;
+    const uint32_t count = ANY ? pb.counters[2] : pb.counters[cur];
+    const uint32_t* q = ANY ? pb.squeue : pb.queue[cur];
+    if (!ANY && blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.counters[cur ^ 1u] = 0;  // next bounce's queue (its last reader finished earlier)
+        pb.counters[2] = 0;         // shadow queue of this bounce
+    }
+    const uint32_t stride = gridDim.x * kBlock;
+    // uniform trip count per wave so every lane reaches the loop exit together
+    for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < count; base += stride) {
+        uint32_t i = base + (threadIdx.x & 63u);
+        if (i >= count) continue;
+        uint32_t slot = q[i];
+        float4 o, d;
+        if (ANY) {
+            o = pb.so[slot];
+            d = pb.sd[slot];
+        } else {
+            o = pb.o[slot];
+            d = pb.d[slot];
+        }
+        TRay r;
+        r.ox = o.x;
+        r.oy = o.y;
+        r.oz = o.z;
+        r.dx = d.x;
+        r.dy = d.y;
+        r.dz = d.z;
+        r.tmin = ANY ? o.w : 0.001f;
+        r.tmax = ANY ? d.w : __int_as_float(0x7f800000);
+        THit h;
+        bool hit = trace<ANY, MIXED, false, kLds, kBlock>(sc, r, h, st, nullptr);
+        if (ANY) {
+            if (!hit) {
+                float4 c = pb.sc[slot];
+                float4 l = pb.L[slot];
+                pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+            }
+        } else {
+            pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, PathBuffers pb, int b) {
+    const uint32_t cur = (uint32_t)b & 1u;
+    const uint32_t count = pb.counters[cur];
+    const uint32_t* q = pb.queue[cur];
+    const uint32_t stride = gridDim.x * kBlock;
+    const float kInfF = __int_as_float(0x7f800000);
+    for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < count; base += stride) {
+        uint32_t i = base + (threadIdx.x & 63u);
+        bool cont = false, shadow = false;
+        uint32_t slot = 0;
+        if (i < count) {
+            slot = q[i];
+            float4 o4 = pb.o[slot], d4 = pb.d[slot], h4 = pb.hit[slot];
+            float4 b4 = pb.beta[slot], t4 = pb.thr[slot], l4 = pb.L[slot];
+            uint32_t key = pb.key[slot];
+            V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+            V3 beta = v3(b4.x, b4.y, b4.z), thr = v3(t4.x, t4.y, t4.z), L = v3(l4.x, l4.y, l4.z);
+            THit h;
+            h.t = h4.x;
+            h.prim = __float_as_uint(h4.y);
+            h.u = h4.z;
+            h.v = h4.w;
+            const uint32_t pos = (uint32_t)(b + 1) * 16u;
+            if (h.prim == 0xFFFFFFFFu) {
+                // miss: sum of infinite-light emission (:49-60)
+                V3 e = v3(0, 0, 0);
+                for (int li = 0; li < sc.num_lights; li++)
+                    if (light_is_infinite(sc.lights[li].type)) e = e + light_emit(sc, sc.lights[li], d);
+                L = L + mulv(beta, e);
+            } else {
+                Surf s = make_surf(sc, o, d, h);
+                const DevMaterial& M = sc.mats[s.mat];
+                if (M.type == 3 && s.front) L = L + mulv(beta, v3(M.emission));  // getEmittedLight :100-105
+                ScatterOut so;
+                if (scatter(sc, s.mat, d, s, key, pos + 1, so)) {
+                    V3 nd = so.dir;
+                    if (so.pdf <= 0.0f) {  // specular (:88-97)
+                        beta = mulv(beta, so.att);
+                        thr = mulv(thr, so.att);
+                        cont = true;
+                    } else {
+                        // direct lighting (:116-157), light draws at positions 8..10
+                        if (sc.num_lights > 0) {
+                            float prob;
+                            int li = select_light(sc, draw(key, pos + 8), prob);
+                            LSample ls = light_sample(sc, sc.lights[li], s.p, s.n, draw(key, pos + 9), draw(key, pos + 10));
+                            ls.pdf *= prob;
+                            float cosine = dot(ls.dir, s.n);
+                            if (lum(ls.emission) > 0.0f && ls.pdf > 0.0f && cosine > 0.0f) {
+                                bool delta;
+                                float mp = mat_pdf(sc, s.mat, d, ls.dir, s.n, delta);
+                                if (!delta) {
+                                    float w = power_heuristic(ls.pdf, mp);
+                                    V3 f = brdf(sc, s.mat, d, ls.dir, s, 0);
+                                    V3 c = mulv(beta, mulv(f, ls.emission) * (cosine * w / ls.pdf));
+                                    if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+                                        float tmax = ls.dist - 0.001f;
+                                        pb.so[slot] = make_float4(s.p.x, s.p.y, s.p.z, 0.001f);
+                                        pb.sd[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, tmax);
+                                        pb.sc[slot] = make_float4(c.x, c.y, c.z, 0.f);
+                                        shadow = true;
+                                    }
+                                }
+                            }
+                        }
+                        // indirect lighting (:160-189)
+                        V3 sd = normalize(nd);
+                        float cosine = dot(sd, s.n);
+                        if (cosine > 0.0f) {
+                            float lp = total_light_pdf(sc, s.p, s.n, sd);
+                            float w = power_heuristic(so.pdf, lp);
+                            thr = mulv(thr, so.att) * (cosine / so.pdf);
+                            beta = mulv(beta, so.att) * (cosine * w / so.pdf);
+                            cont = true;
+                        }
+                    }
+                    if (cont) {
+                        o = s.p;
+                        d = nd;
+                    }
+                }
+            }
+            // next level: depth limit (:37-39) and Russian roulette (:42-45)
+            if (cont) {
+                if (b + 1 >= pp.max_depth) cont = false;
+                else cont = russian_roulette(pp, key, b + 1, thr, beta);
+            }
+            pb.L[slot] = make_float4(L.x, L.y, L.z, 0.f);
+            if (cont) {
+                pb.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
+                pb.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
+                pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
+                pb.thr[slot] = make_float4(thr.x, thr.y, thr.z, 0.f);
+            }
+            (void)kInfF;
+        }
+        queue_push(pb.queue[cur ^ 1u], &pb.counters[cur ^ 1u], slot, cont);
+        queue_push(pb.squeue, &pb.counters[2], slot, shadow);
+    }
+}
+
+// Film (PixelStats.AddSample, stats.go:41-47): per pixel, the chunk's samples are summed in
+// sample order, then added to the running sums with Kahan compensation.
+struct FilmBuffers {
+    float4* sum;   // rgb sum, luminance sum
+    float4* comp;  // Kahan compensation of sum
+    float2* sq;    // luminance^2 sum, compensation
+    uint32_t* count;
+};
+
+__device__ __forceinline__ void kahan(float& s, float& c, float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_film(PassParams pp, PathBuffers pb, FilmBuffers fb) {
+    int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= pp.npix) return;
+    float r = 0, g = 0, b = 0, l = 0, l2 = 0;
+    float cr = 0, cg = 0, cb = 0, cl = 0, cl2 = 0;
+    for (int s = 0; s < pp.spp; s++) {
+        float4 v = pb.L[(size_t)s * pp.npix + p];
+        float lv = 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z;
+        kahan(r, cr, v.x);
+        kahan(g, cg, v.y);
+        kahan(b, cb, v.z);
+        kahan(l, cl, lv);
+        kahan(l2, cl2, lv * lv);
+    }
+    int px = pp.px0 + p;
+    float4 S = fb.sum[px], C = fb.comp[px];
+    float2 Q = fb.sq[px];
+    kahan(S.x, C.x, r);
+    kahan(S.y, C.y, g);
+    kahan(S.z, C.z, b);
+    kahan(S.w, C.w, l);
+    kahan(Q.x, Q.y, l2);
+    fb.sum[px] = S;
+    fb.comp[px] = C;
+    fb.sq[px] = Q;
+    fb.count[px] += (uint32_t)pp.spp;
+}
+
+}  // namespace prt
+
+using namespace prt;
+
+struct prt_film {
+    const prt_scene* scene = nullptr;
+    prt_render_params params{};
+    int32_t width = 0, height = 0;
+    int64_t npix = 0;
+    int64_t capacity = 0;  // path slots
+    int32_t spp_done = 0;
+    std::vector<void*> bufs;
+    PathBuffers pb{};
+    FilmBuffers fb{};
+    uint2* gstack = nullptr;
+    uint32_t gslots = 0;
+    int blocks = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int film_alloc(prt_film* f, size_t bytes, void** out) {
+    void* p = nullptr;
+    PRT_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    f->bufs.push_back(p);
+    *out = p;
+    return PRT_OK;
+}
+
+// Renders `spp` samples per pixel for every pixel, starting at sample index spp_done.
+int render_samples(prt_film* f, int32_t spp, float* ms) {
+    const prt_scene* s = f->scene;
+    PRT_HIP(hipSetDevice(s->device));
+    const DevScene& sc = s->dev;
+    hipStream_t st = f->stream;
+    PRT_HIP(hipEventRecord(f->ev0, st));
+    int64_t done = 0;
+    while (done < spp) {
+        // chunking depends only on the film's own size, so renders are reproducible
+        int64_t per = f->capacity / f->npix;
+        int32_t chunk_spp = (int32_t)std::max<int64_t>(1, std::min<int64_t>(per, spp - done));
+        int64_t pix_step = per >= 1 ? f->npix : f->capacity;
+        for (int64_t p0 = 0; p0 < f->npix; p0 += pix_step) {
+            PassParams pp;
+            pp.width = f->width;
+            pp.height = f->height;
+            pp.px0 = (int32_t)p0;
+            pp.npix = (int32_t)std::min<int64_t>(pix_step, f->npix - p0);
+            pp.spp = chunk_spp;
+            pp.sample_base = f->spp_done + (int32_t)done;
+            pp.max_depth = f->params.max_depth;
+            pp.rr_min = f->params.rr_min_bounces;
+            pp.seed = f->params.seed;
+            int64_t n = (int64_t)pp.npix * pp.spp;
+            PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 4 * sizeof(uint32_t), st));
+            int cam_blocks = (int)((n + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL(k_camera, dim3(cam_blocks), dim3(kBlock), 0, st, sc, pp, f->pb);
+            for (int b = 0; b < pp.max_depth; b++) {
+                if (sc.mixed)
+                    hipLaunchKernelGGL((k_trace<false, true>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
+                                       f->gstack, f->gslots);
+                else
+                    hipLaunchKernelGGL((k_trace<false, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
+                                       f->gstack, f->gslots);
+                hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
+                if (sc.mixed)
+                    hipLaunchKernelGGL((k_trace<true, true>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
+                                       f->gstack, f->gslots);
+                else
+                    hipLaunchKernelGGL((k_trace<true, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
+                                       f->gstack, f->gslots);
+            }
+            int film_blocks = (pp.npix + kBlock - 1) / kBlock;
+            hipLaunchKernelGGL(k_film, dim3(film_blocks), dim3(kBlock), 0, st, pp, f->pb, f->fb);
+            PRT_HIP(hipGetLastError());
+        }
+        done += chunk_spp;
+    }
+    PRT_HIP(hipEventRecord(f->ev1, st));
+    PRT_HIP(hipEventSynchronize(f->ev1));
+    float t = 0;
+    PRT_HIP(hipEventElapsedTime(&t, f->ev0, f->ev1));
+    if (ms) *ms = t;
+    f->spp_done += spp;
+    return PRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prt_film_create(const prt_scene* scene, const prt_render_params* p, prt_film** out) {
+    if (!scene || !p || !out) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (p->integrator != PRT_INTEGRATOR_PATH && p->integrator != PRT_INTEGRATOR_BDPT)
+        return fail(PRT_ERR_INVALID_ARGUMENT, "unknown integrator");
+    if (p->integrator == PRT_INTEGRATOR_BDPT) return fail(PRT_ERR_UNSUPPORTED, "BDPT is not built yet");
+    if (p->max_depth < 0 || p->max_depth > 1024) return fail(PRT_ERR_INVALID_ARGUMENT, "max_depth out of range");
+    if (p->max_passes < 1 || p->max_samples_per_pixel < 1 || p->initial_samples < 0)
+        return fail(PRT_ERR_INVALID_ARGUMENT, "bad pass schedule");
+    int32_t w = p->width > 0 ? p->width : scene->image_width;
+    int32_t h = p->height > 0 ? p->height : scene->image_height;
+    if (w != scene->image_width || h != scene->image_height)
+        return fail(PRT_ERR_INVALID_ARGUMENT, "film size must match the camera image size");
+    if (w <= 0 || h <= 0) return fail(PRT_ERR_INVALID_ARGUMENT, "empty image");
+    PRT_HIP(hipSetDevice(scene->device));
+    prt_film* f = new prt_film();
+    f->scene = scene;
+    f->params = *p;
+    f->width = w;
+    f->height = h;
+    f->npix = (int64_t)w * h;
+    f->capacity = std::max<int64_t>(std::min<int64_t>(f->npix * 64, (int64_t)1 << 23), std::min<int64_t>(f->npix, (int64_t)1 << 23));
+    int rc;
+    auto bail = [&](int code) {
+        prt_film_destroy(f);
+        return code;
+    };
+    const int64_t C = f->capacity;
+    void* ptr;
+#define FALLOC(field, type, count)                                                      \
+    if ((rc = film_alloc(f, sizeof(type) * (size_t)(count), &ptr)) != PRT_OK) return bail(rc); \
+    field = (type*)ptr;
+    FALLOC(f->pb.o, float4, C);
+    FALLOC(f->pb.d, float4, C);
+    FALLOC(f->pb.beta, float4, C);
+    FALLOC(f->pb.thr, float4, C);
+    FALLOC(f->pb.L, float4, C);
+    FALLOC(f->pb.hit, float4, C);
+    FALLOC(f->pb.so, float4, C);
+    FALLOC(f->pb.sd, float4, C);
+    FALLOC(f->pb.sc, float4, C);
+    FALLOC(f->pb.key, uint32_t, C);
+    FALLOC(f->pb.pixel, uint32_t, C);
+    FALLOC(f->pb.queue[0], uint32_t, C);
+    FALLOC(f->pb.queue[1], uint32_t, C);
+    FALLOC(f->pb.squeue, uint32_t, C);
+    FALLOC(f->pb.counters, uint32_t, 4);
+    FALLOC(f->fb.sum, float4, f->npix);
+    FALLOC(f->fb.comp, float4, f->npix);
+    FALLOC(f->fb.sq, float2, f->npix);
+    FALLOC(f->fb.count, uint32_t, f->npix);
+#undef FALLOC
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace<false, true>, kBlock, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    per_cu = std::min(per_cu, 8);
+    f->blocks = std::max(1, scene->num_cus) * per_cu;
+    f->gslots = (uint32_t)f->blocks * kBlock;
+    size_t depth = (size_t)std::max(1, scene->max_stack - kLds + 1);
+    if ((rc = film_alloc(f, depth * f->gslots * sizeof(uint2), &ptr)) != PRT_OK) return bail(rc);
+    f->gstack = (uint2*)ptr;
+    hipError_t e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    e = hipEventCreate(&f->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&f->ev1);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
+    if ((rc = prt_film_reset(f)) != PRT_OK) return bail(rc);
+    *out = f;
+    return PRT_OK;
+}
+
+int prt_film_reset(prt_film* f) {
+    if (!f) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL film");
+    PRT_HIP(hipSetDevice(f->scene->device));
+    PRT_HIP(hipMemsetAsync(f->fb.sum, 0, sizeof(float4) * f->npix, f->stream));
+    PRT_HIP(hipMemsetAsync(f->fb.comp, 0, sizeof(float4) * f->npix, f->stream));
+    PRT_HIP(hipMemsetAsync(f->fb.sq, 0, sizeof(float2) * f->npix, f->stream));
+    PRT_HIP(hipMemsetAsync(f->fb.count, 0, sizeof(uint32_t) * f->npix, f->stream));
+    PRT_HIP(hipStreamSynchronize(f->stream));
+    f->spp_done = 0;
+    return PRT_OK;
+}
+
+int prt_film_destroy(prt_film* f) {
+    if (!f) return PRT_OK;
+    (void)hipSetDevice(f->scene->device);
+    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    for (void* p : f->bufs) (void)hipFree(p);
+    if (f->ev0) (void)hipEventDestroy(f->ev0);
+    if (f->ev1) (void)hipEventDestroy(f->ev1);
+    if (f->stream) (void)hipStreamDestroy(f->stream);
+    delete f;
+    return PRT_OK;
+}
+
+static void fill_stats(const prt_film* f, int32_t target, float ms, prt_pass_stats* st) {
+    if (!st) return;
+    st->total_pixels = f->npix;
+    st->total_samples = f->npix * (int64_t)f->spp_done;
+    st->average_samples = (double)f->spp_done;
+    st->max_samples = target;
+    st->min_samples = f->spp_done;
+    st->max_samples_used = f->spp_done;
+    st->target_samples = target;
+    st->device_ms = ms;
+}
+
+int prt_render_samples(prt_film* f, int32_t spp, prt_pass_stats* st) {
+    if (!f) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL film");
+    if (spp < 0) return fail(PRT_ERR_INVALID_ARGUMENT, "negative spp");
+    float ms = 0;
+    if (spp > 0) {
+        int rc = render_samples(f, spp, &ms);
+        if (rc != PRT_OK) return rc;
+    }
+    fill_stats(f, f->spp_done, ms, st);
+    return PRT_OK;
+}
+
+int prt_render_pass(prt_film* f, int32_t pass, prt_pass_stats* st) {
+    if (!f) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL film");
+    int32_t target = 0;
+    int rc = prt_samples_for_pass(&f->params, pass, &target);
+    if (rc != PRT_OK) return rc;
+    float ms = 0;
+    int32_t need = target - f->spp_done;
+    if (need > 0) {
+        rc = render_samples(f, need, &ms);
+        if (rc != PRT_OK) return rc;
+    }
+    fill_stats(f, target, ms, st);
+    return PRT_OK;
+}
+
+int prt_film_read(const prt_film* f, float* mean_rgb, float* lum_sum, float* lum_sq_sum, uint32_t* counts) {
+    if (!f) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL film");
+    PRT_HIP(hipSetDevice(f->scene->device));
+    std::vector<float4> sum(f->npix);
+    std::vector<float2> sq(f->npix);
+    std::vector<uint32_t> cnt(f->npix);
+    PRT_HIP(hipMemcpy(sum.data(), f->fb.sum, sizeof(float4) * f->npix, hipMemcpyDeviceToHost));
+    PRT_HIP(hipMemcpy(sq.data(), f->fb.sq, sizeof(float2) * f->npix, hipMemcpyDeviceToHost));
+    PRT_HIP(hipMemcpy(cnt.data(), f->fb.count, sizeof(uint32_t) * f->npix, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < f->npix; i++) {
+        if (mean_rgb) {
+            float inv = cnt[i] ? 1.0f / (float)cnt[i] : 0.0f;  // GetColor: zero when no samples
+            mean_rgb[3 * i + 0] = cnt[i] ? sum[i].x * inv : 0.f;
+            mean_rgb[3 * i + 1] = cnt[i] ? sum[i].y * inv : 0.f;
+            mean_rgb[3 * i + 2] = cnt[i] ? sum[i].z * inv : 0.f;
+        }
+        if (lum_sum) lum_sum[i] = sum[i].w;
+        if (lum_sq_sum) lum_sq_sum[i] = sq[i].x;
+        if (counts) counts[i] = cnt[i];
+    }
+    return PRT_OK;
+}
+
+int prt_film_read_rgba8(const prt_film* f, int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint8_t* rgba) {
+    if (!f || !rgba) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (x0 < 0 || y0 < 0 || x1 > f->width || y1 > f->height || x0 > x1 || y0 > y1)
+        return fail(PRT_ERR_INVALID_ARGUMENT, "rectangle outside the image");
+    std::vector<float> mean(3 * f->npix);
+    std::vector<uint32_t> cnt(f->npix);
+    int rc = prt_film_read(f, mean.data(), nullptr, nullptr, cnt.data());
+    if (rc != PRT_OK) return rc;
+    const int tw = x1 - x0;
+    for (int y = y0; y < y1; y++)
+        for (int x = x0; x < x1; x++) {
+            int64_t i = (int64_t)y * f->width + x;
+            uint8_t* o = rgba + 4 * ((int64_t)(y - y0) * tw + (x - x0));
+            for (int c = 0; c < 3; c++) {  // vec3ToColor: gamma 2, clamp, uint8 truncation
+                double v = std::sqrt((double)mean[3 * i + c]);
+                v = std::max(0.0, std::min(1.0, v));
+                o[c] = (uint8_t)(255.0 * v);
+            }
+            o[3] = 255;
+        }
+    return PRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,705 @@
+// scene.hip — scene flattening/upload (replaces Scene.Preprocess, pkg/scene/scene.go:45-78) and
+// the batched closest-hit / any-hit entry points (replace BVH.Hit, pkg/geometry/bvh.go:183-239).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "bvh_build.h"
+#include "prt_internal.h"
+#include "traverse.h"
+
+namespace prt {
+
+namespace {
+
+struct D3 {
+    double x, y, z;
+};
+D3 d3(const double* p) { return {p[0], p[1], p[2]}; }
+D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double len(D3 a) { return std::sqrt(dot(a, a)); }
+D3 normalize(D3 a) {
+    double l = len(a);
+    if (l == 0) return {0, 0, 0};
+    return {a.x / l, a.y / l, a.z / l};
+}
+double comp(D3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// Vec3.Rotate X->Y->Z (pkg/core/vec3.go:162-198)
+D3 rotate(D3 v, D3 r) {
+    if (r.x == 0 && r.y == 0 && r.z == 0) return v;
+    if (r.x != 0) {
+        double c = std::cos(r.x), s = std::sin(r.x);
+        v = {v.x, v.y * c - v.z * s, v.y * s + v.z * c};
+    }
+    if (r.y != 0) {
+        double c = std::cos(r.y), s = std::sin(r.y);
+        v = {v.x * c + v.z * s, v.y, -v.x * s + v.z * c};
+    }
+    if (r.z != 0) {
+        double c = std::cos(r.z), s = std::sin(r.z);
+        v = {v.x * c - v.y * s, v.x * s + v.y * c, v.z};
+    }
+    return v;
+}
+
+struct QuadF64 {
+    D3 corner, u, v;
+    int mat;
+};
+
+// Quad.BoundingBox with the +-0.001 slab for axis-aligned quads (quad.go:44-77,183-210).
+PrimBounds quad_bounds(const QuadF64& q) {
+    D3 n = normalize(cross(q.u, q.v));
+    D3 cs[4] = {q.corner, q.corner + q.u, q.corner + q.v, q.corner + q.u + q.v};
+    PrimBounds b;
+    for (int a = 0; a < 3; a++) {
+        b.lo[a] = b.hi[a] = comp(cs[0], a);
+        for (int k = 1; k < 4; k++) {
+            b.lo[a] = std::min(b.lo[a], comp(cs[k], a));
+            b.hi[a] = std::max(b.hi[a], comp(cs[k], a));
+        }
+    }
+    const double thr = 0.9999, tol = 0.0001;
+    int al = -1;
+    if (std::fabs(n.x) > thr && std::fabs(n.y) < tol && std::fabs(n.z) < tol) al = 0;
+    else if (std::fabs(n.y) > thr && std::fabs(n.x) < tol && std::fabs(n.z) < tol) al = 1;
+    else if (std::fabs(n.z) > thr && std::fabs(n.x) < tol && std::fabs(n.y) < tol) al = 2;
+    if (al >= 0) {
+        double f = comp(cs[0], al);
+        b.lo[al] = f - 0.001;
+        b.hi[al] = f + 0.001;
+    }
+    return b;
+}
+
+void grow(PrimBounds& w, const PrimBounds& b) {
+    for (int a = 0; a < 3; a++) {
+        w.lo[a] = std::min(w.lo[a], b.lo[a]);
+        w.hi[a] = std::max(w.hi[a], b.hi[a]);
+    }
+}
+
+float4 f4(double x, double y, double z, float w) { return make_float4((float)x, (float)y, (float)z, w); }
+float bits(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// NewCamera (pkg/geometry/camera.go:70-170), float64 then rounded to float32.
+DevCamera make_camera(const prt_camera& c, int& height) {
+    const double pi = 3.14159265358979323846;
+    D3 C = d3(c.center), L = d3(c.look_at), U = d3(c.up);
+    D3 w = normalize(C - L);
+    D3 u = normalize(cross(w, U));
+    D3 v = cross(w, u);
+    double focus = c.focus_distance;
+    if (focus <= 0) focus = len(C - L);
+    double lr = c.aperture / 2;
+    double la = pi * lr * lr;
+    if (c.aperture == 0) la = 1.0;
+    int W = c.width;
+    int H = (int)((double)W / c.aspect_ratio);
+    double theta = c.vfov * pi / 180.0;
+    double hh = std::tan(theta / 2.0);
+    double vh = 2.0 * hh * focus, vw = vh * c.aspect_ratio;
+    double vh1 = 2.0 * std::tan(theta / 2.0);
+    double plane = (vh1 * c.aspect_ratio) * vh1;
+    D3 VU = u * vw, VV = v * vh;
+    D3 du = VU * (1.0 / (double)W), dv = VV * (1.0 / (double)H);
+    D3 ul = C - w * focus - VU * 0.5 - VV * 0.5;
+    D3 p00 = ul + (du + dv) * 0.5;
+    D3 disk_u = u * (c.aperture / 2), disk_v = v * (c.aperture / 2);
+    D3 fwd = normalize(L - C);
+    D3 corner = normalize(D3{-vw / 2, -vh / 2, -focus});
+    double ctw = dot(corner, D3{0, 0, -1});
+    DevCamera k{};
+    auto put = [](float* o, D3 a) {
+        o[0] = (float)a.x;
+        o[1] = (float)a.y;
+        o[2] = (float)a.z;
+    };
+    put(k.center, C);
+    put(k.pixel00, p00);
+    put(k.du, du);
+    put(k.dv, dv);
+    put(k.disk_u, disk_u);
+    put(k.disk_v, disk_v);
+    put(k.u, u);
+    put(k.v, v);
+    put(k.w, w);
+    put(k.forward, fwd);
+    k.lens_radius = (float)lr;
+    k.lens_area = (float)la;
+    k.vw = (float)vw;
+    k.vh = (float)vh;
+    k.plane_area = (float)plane;
+    k.cos_total_width = (float)ctw;
+    k.focus = (float)focus;
+    k.width = W;
+    k.height = H;
+    k.has_lens = len(disk_u) > 0 ? 1 : 0;
+    height = H;
+    return k;
+}
+
+}  // namespace
+
+int scene_alloc(prt_scene* s, size_t bytes, const void* src, void** out) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 16;
+    PRT_HIP(hipMalloc(&p, bytes));
+    s->bufs.push_back({p, bytes});
+    s->info.device_bytes += (int64_t)bytes;
+    if (src) PRT_HIP(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    *out = p;
+    return PRT_OK;
+}
+
+int persistent_blocks(const prt_scene* s, int block, int per_cu) {
+    (void)block;
+    return std::max(1, s->num_cus) * per_cu;
+}
+
+// ---- intersection kernels -------------------------------------------------------------
+constexpr int kTraceBlock = 256;
+constexpr int kTraceLds = 16;
+
+struct ScratchState {
+    std::mutex mu;
+    hipEvent_t last = nullptr;
+};
+
+template <bool ANY, bool MIXED, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_intersect(DevScene sc, const prt_ray* __restrict__ rays,
+                                                           int64_t n, prt_hit* __restrict__ hits,
+                                                           uint8_t* __restrict__ occluded, uint32_t* cnt_nodes,
+                                                           uint32_t* cnt_boxes, uint32_t* cnt_prims, uint2* gstack,
+                                                           uint32_t gstride, unsigned long long* work) {
+    __shared__ uint32_t s_node[kTraceLds * kTraceBlock];
+    __shared__ float s_t[kTraceLds * kTraceBlock];
+    TStack<kTraceLds, kTraceBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kTraceBlock + threadIdx.x;
+    st.gstride = gstride;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(work, 64ull);
+        base = __shfl(base, 0, 64);
+        if (base >= (unsigned long long)n) break;
+        int64_t i = (int64_t)base + lane;
+        if (i < n) {
+            const float4* rp = reinterpret_cast<const float4*>(rays + i);
+            float4 a = rp[0], b = rp[1];
+            TRay r{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            THit h;
+            TCounts c{0, 0, 0};
+            bool hit = trace<ANY, MIXED, COUNT, kTraceLds, kTraceBlock>(sc, r, h, st, &c);
+            if (COUNT) {
+                cnt_nodes[i] = c.nodes;
+                cnt_boxes[i] = c.boxes;
+                cnt_prims[i] = c.prims;
+            } else if (ANY) {
+                occluded[i] = hit ? 1 : 0;
+            } else {
+                prt_hit o;
+                o.t = h.t;
+                o.prim_id = hit ? (int32_t)__float_as_uint(sc.prims[3 * (size_t)h.prim + 2].w) : -1;
+                o.u = hit ? h.u : 0.0f;
+                o.v = hit ? h.v : 0.0f;
+                reinterpret_cast<float4*>(hits)[i] =
+                    make_float4(o.t, __int_as_float(o.prim_id), o.u, o.v);
+            }
+        }
+    }
+}
+
+}  // namespace prt
+
+using namespace prt;
+
+namespace {
+std::mutex g_scratch_mu;
+}
+
+static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* rays, int64_t n, void* out0,
+                            uint32_t* c1, uint32_t* c2, uint32_t flags, void* stream) {
+    if (!scene) return fail(PRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n < 0) return fail(PRT_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (n == 0) return PRT_OK;
+    if (!rays || !out0) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL buffer");
+    if (flags > PRT_BUFFERS_DEVICE) return fail(PRT_ERR_INVALID_ARGUMENT, "unknown flags");
+    PRT_HIP(hipSetDevice(scene->device));
+    hipStream_t st = (hipStream_t)stream;
+    const bool dev = flags == PRT_BUFFERS_DEVICE;
+    const size_t out_elem = mode == 0 ? sizeof(prt_hit) : (mode == 1 ? 1 : 4);
+    void* d_rays = (void*)rays;
+    void* d_out0 = out0;
+    uint32_t* d_c1 = c1;
+    uint32_t* d_c2 = c2;
+    std::vector<void*> temps;
+    auto cleanup = [&]() {
+        for (void* p : temps) (void)hipFree(p);
+    };
+    if (!dev) {
+        hipError_t e;
+        e = hipMalloc(&d_rays, (size_t)n * sizeof(prt_ray));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(rays)");
+        temps.push_back(d_rays);
+        e = hipMalloc(&d_out0, (size_t)n * out_elem);
+        if (e != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc(out)"); }
+        temps.push_back(d_out0);
+        if (mode == 2) {
+            e = hipMalloc((void**)&d_c1, (size_t)n * 4);
+            if (e != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc(counts)"); }
+            temps.push_back(d_c1);
+            e = hipMalloc((void**)&d_c2, (size_t)n * 4);
+            if (e != hipSuccess) { cleanup(); return hip_fail(e, "hipMalloc(counts)"); }
+            temps.push_back(d_c2);
+        }
+        e = hipMemcpyAsync(d_rays, rays, (size_t)n * sizeof(prt_ray), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) { cleanup(); return hip_fail(e, "hipMemcpyAsync(rays)"); }
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    // scene scratch (work counter + overflow stack) is shared: order launches on it
+    unsigned long long* work = (unsigned long long*)scene->bufs[0].p;
+    hipError_t e = hipMemsetAsync(work, 0, sizeof(unsigned long long), st);
+    if (e != hipSuccess) { cleanup(); return hip_fail(e, "hipMemsetAsync(work)"); }
+    int blocks = (int)(scene->gstack_slots / kTraceBlock);
+    const DevScene& sc = scene->dev;
+    uint2* gs = scene->gstack;
+    uint32_t gstride = scene->gstack_slots;
+    if (mode == 0) {
+        if (sc.mixed)
+            hipLaunchKernelGGL((k_intersect<false, true, false>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
+                               (prt_hit*)d_out0, nullptr, nullptr, nullptr, nullptr, gs, gstride, work);
+        else
+            hipLaunchKernelGGL((k_intersect<false, false, false>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
+                               (prt_hit*)d_out0, nullptr, nullptr, nullptr, nullptr, gs, gstride, work);
+    } else if (mode == 1) {
+        if (sc.mixed)
+            hipLaunchKernelGGL((k_intersect<true, true, false>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
+                               nullptr, (uint8_t*)d_out0, nullptr, nullptr, nullptr, gs, gstride, work);
+        else
+            hipLaunchKernelGGL((k_intersect<true, false, false>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
+                               nullptr, (uint8_t*)d_out0, nullptr, nullptr, nullptr, gs, gstride, work);
+    } else {
+        if (sc.mixed)
+            hipLaunchKernelGGL((k_intersect<false, true, true>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
+                               nullptr, nullptr, (uint32_t*)d_out0, d_c1, d_c2, gs, gstride, work);
+        else
+            hipLaunchKernelGGL((k_intersect<false, false, true>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
+                               nullptr, nullptr, (uint32_t*)d_out0, d_c1, d_c2, gs, gstride, work);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) { cleanup(); return hip_fail(e, "intersect kernel launch"); }
+    if (!dev) {
+        e = hipMemcpyAsync(out0, d_out0, (size_t)n * out_elem, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && mode == 2) e = hipMemcpyAsync(c1, d_c1, (size_t)n * 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && mode == 2) e = hipMemcpyAsync(c2, d_c2, (size_t)n * 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        cleanup();
+        if (e != hipSuccess) return hip_fail(e, "intersect copy-back");
+    } else {
+        // the shared scratch is reused by the next call: keep calls ordered on the device
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "intersect synchronize");
+    }
+    return PRT_OK;
+}
+
+extern "C" {
+
+int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
+    if (!d || !out) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PRT_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(PRT_ERR_INVALID_ARGUMENT, "device index out of range");
+    if (d->num_materials < 0 || d->num_spheres < 0 || d->num_quads < 0 || d->num_boxes < 0 ||
+        d->num_meshes < 0 || d->num_lights < 0)
+        return fail(PRT_ERR_INVALID_ARGUMENT, "negative count");
+    if (d->camera.width <= 0 || !(d->camera.aspect_ratio > 0))
+        return fail(PRT_ERR_INVALID_ARGUMENT, "camera width/aspect must be positive");
+    const int nm = d->num_materials;
+    auto bad_mat = [&](int m) { return m < 0 || m >= nm; };
+    for (int i = 0; i < nm; i++) {
+        const prt_material& m = d->materials[i];
+        if (m.type < PRT_MAT_LAMBERTIAN || m.type > PRT_MAT_LAYERED)
+            return fail(PRT_ERR_INVALID_ARGUMENT, "unknown material type");
+        if (m.type == PRT_MAT_LAYERED) {
+            if (bad_mat(m.outer) || bad_mat(m.inner) || m.outer == i || m.inner == i ||
+                d->materials[m.outer].type == PRT_MAT_LAYERED || d->materials[m.inner].type == PRT_MAT_LAYERED)
+                return fail(PRT_ERR_INVALID_ARGUMENT, "layered material needs two non-layered components");
+        }
+    }
+    PRT_HIP(hipSetDevice(device));
+    prt_scene* s = new prt_scene();
+    s->device = device;
+    int rc = PRT_OK;
+    auto bail = [&](int code) {
+        prt_scene_destroy(s);
+        return code;
+    };
+
+    // ---- gather primitives in global id order ----
+    std::vector<QuadF64> quads;
+    for (int i = 0; i < d->num_quads; i++) {
+        const prt_quad& q = d->quads[i];
+        if (bad_mat(q.material)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "quad material out of range"));
+        quads.push_back({d3(q.corner), d3(q.u), d3(q.v), q.material});
+    }
+    std::vector<PrimBounds> world_boxes;  // reference shape boxes (for world centre/radius)
+    for (int i = 0; i < d->num_boxes; i++) {  // NewBox / generateFaces box.go:34-89
+        const prt_box& b = d->boxes[i];
+        if (bad_mat(b.material)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "box material out of range"));
+        D3 c[8] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}};
+        PrimBounds wb;
+        for (int k = 0; k < 8; k++) {
+            c[k] = D3{c[k].x * b.size[0], c[k].y * b.size[1], c[k].z * b.size[2]};
+            c[k] = rotate(c[k], d3(b.rotation));
+            c[k] = c[k] + d3(b.center);
+            for (int a = 0; a < 3; a++) {
+                double v = comp(c[k], a);
+                if (k == 0) wb.lo[a] = wb.hi[a] = v;
+                wb.lo[a] = std::min(wb.lo[a], v);
+                wb.hi[a] = std::max(wb.hi[a], v);
+            }
+        }
+        int m = b.material;
+        quads.push_back({c[4], c[5] - c[4], c[7] - c[4], m});
+        quads.push_back({c[1], c[0] - c[1], c[2] - c[1], m});
+        quads.push_back({c[5], c[1] - c[5], c[6] - c[5], m});
+        quads.push_back({c[0], c[4] - c[0], c[3] - c[0], m});
+        quads.push_back({c[3], c[7] - c[3], c[2] - c[3], m});
+        quads.push_back({c[4], c[0] - c[4], c[5] - c[4], m});
+        world_boxes.push_back(wb);
+    }
+    int64_t ntri = 0;
+    for (int i = 0; i < d->num_meshes; i++) {
+        const prt_mesh& m = d->meshes[i];
+        if (m.num_faces < 0 || m.num_vertices < 0 || (m.num_faces > 0 && (!m.faces || !m.vertices)))
+            return bail(fail(PRT_ERR_INVALID_ARGUMENT, "bad mesh arrays"));
+        if (!m.face_materials && m.num_faces > 0 && bad_mat(m.material))
+            return bail(fail(PRT_ERR_INVALID_ARGUMENT, "mesh material out of range"));
+        ntri += m.num_faces;
+    }
+    const int64_t nsph = d->num_spheres;
+    const int64_t nq = (int64_t)quads.size();
+    const int64_t nprim = ntri + nsph + nq;
+    if (nprim >= (1ll << 28)) return bail(fail(PRT_ERR_UNSUPPORTED, "more than 2^28 primitives"));
+
+    std::vector<PrimBounds> bounds(nprim);
+    std::vector<float4> rec(3 * nprim);
+    PrimBounds world;
+    for (int a = 0; a < 3; a++) {
+        world.lo[a] = INFINITY;
+        world.hi[a] = -INFINITY;
+    }
+    bool world_any = false;
+    int64_t gid = 0;
+    for (int i = 0; i < d->num_meshes; i++) {
+        const prt_mesh& m = d->meshes[i];
+        PrimBounds mb;
+        for (int a = 0; a < 3; a++) {
+            mb.lo[a] = INFINITY;
+            mb.hi[a] = -INFINITY;
+        }
+        for (int64_t f = 0; f < m.num_faces; f++, gid++) {
+            const int32_t* fi = m.faces + 3 * f;
+            if (fi[0] < 0 || fi[1] < 0 || fi[2] < 0 || fi[0] >= m.num_vertices || fi[1] >= m.num_vertices ||
+                fi[2] >= m.num_vertices)
+                return bail(fail(PRT_ERR_INVALID_ARGUMENT, "face index out of bounds"));
+            int mat = m.face_materials ? m.face_materials[f] : m.material;
+            if (bad_mat(mat)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "face material out of range"));
+            const double* p0 = m.vertices + 3 * fi[0];
+            const double* p1 = m.vertices + 3 * fi[1];
+            const double* p2 = m.vertices + 3 * fi[2];
+            PrimBounds& b = bounds[gid];
+            for (int a = 0; a < 3; a++) {
+                b.lo[a] = std::min(p0[a], std::min(p1[a], p2[a]));
+                b.hi[a] = std::max(p0[a], std::max(p1[a], p2[a]));
+            }
+            grow(mb, b);
+            rec[3 * gid + 0] = f4(p0[0], p0[1], p0[2], bits(kPrimTriangle << 30));
+            rec[3 * gid + 1] = f4(p1[0], p1[1], p1[2], bits((uint32_t)mat));
+            rec[3 * gid + 2] = f4(p2[0], p2[1], p2[2], bits((uint32_t)gid));
+        }
+        if (m.num_faces > 0) {
+            grow(world, mb);
+            world_any = true;
+        }
+    }
+    for (int64_t i = 0; i < nsph; i++, gid++) {
+        const prt_sphere& sp = d->spheres[i];
+        if (bad_mat(sp.material)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "sphere material out of range"));
+        double r = sp.radius, ar = std::fabs(r);
+        PrimBounds& b = bounds[gid];
+        PrimBounds ref;  // reference box c-r .. c+r (inverted for r < 0, sphere.go:83-90)
+        for (int a = 0; a < 3; a++) {
+            b.lo[a] = sp.center[a] - ar;
+            b.hi[a] = sp.center[a] + ar;
+            ref.lo[a] = sp.center[a] - r;
+            ref.hi[a] = sp.center[a] + r;
+        }
+        grow(world, ref);
+        world_any = true;
+        rec[3 * gid + 0] = f4(sp.center[0], sp.center[1], sp.center[2], bits((kPrimSphere << 30) | (uint32_t)i));
+        rec[3 * gid + 1] = make_float4((float)r, 0.f, 0.f, bits((uint32_t)sp.material));
+        rec[3 * gid + 2] = make_float4(0.f, 0.f, 0.f, bits((uint32_t)gid));
+    }
+    std::vector<QuadExt> qext(std::max<int64_t>(nq, 1));
+    for (int64_t i = 0; i < nq; i++, gid++) {
+        const QuadF64& q = quads[i];
+        bounds[gid] = quad_bounds(q);
+        if (i < d->num_quads) {
+            grow(world, bounds[gid]);
+            world_any = true;
+        }
+        D3 n = normalize(cross(q.u, q.v));
+        double D = dot(n, q.corner);
+        D3 w = n * (1.0 / dot(n, cross(q.u, q.v)));
+        qext[i] = QuadExt{(float)n.x, (float)n.y, (float)n.z, (float)D, (float)w.x, (float)w.y, (float)w.z, 0.f};
+        rec[3 * gid + 0] = f4(q.corner.x, q.corner.y, q.corner.z, bits((kPrimQuad << 30) | (uint32_t)i));
+        rec[3 * gid + 1] = f4(q.u.x, q.u.y, q.u.z, bits((uint32_t)q.mat));
+        rec[3 * gid + 2] = f4(q.v.x, q.v.y, q.v.z, bits((uint32_t)gid));
+    }
+    for (auto& wb : world_boxes) {
+        grow(world, wb);
+        world_any = true;
+    }
+    // world centre / radius of the reference BVH root box (bvh.go:38-48)
+    if (world_any) {
+        for (int a = 0; a < 3; a++) s->info.world_center[a] = 0.5 * (world.lo[a] + world.hi[a]);
+        double dx = world.hi[0] - s->info.world_center[0], dy = world.hi[1] - s->info.world_center[1],
+               dz = world.hi[2] - s->info.world_center[2];
+        s->info.world_radius = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+
+    // ---- BVH ----
+    BVHBuildResult bvh = build_bvh4(bounds, 4);
+    std::vector<float4> ordered(3 * std::max<int64_t>(nprim, 1));
+    s->orig_id.resize(nprim);
+    for (int64_t k = 0; k < nprim; k++) {
+        uint32_t i = bvh.order[k];
+        ordered[3 * k + 0] = rec[3 * i + 0];
+        ordered[3 * k + 1] = rec[3 * i + 1];
+        ordered[3 * k + 2] = rec[3 * i + 2];
+        s->orig_id[k] = i;
+    }
+    s->info.num_primitives = nprim;
+    s->info.num_triangles = ntri;
+    s->info.bvh_nodes = (int64_t)bvh.nodes.size();
+    s->info.bvh_depth = bvh.depth;
+    s->info.max_stack = bvh.max_stack;
+    s->max_stack = bvh.max_stack;
+
+    // ---- device upload ----
+    hipDeviceProp_t prop;
+    PRT_HIP(hipGetDeviceProperties(&prop, device));
+    s->num_cus = prop.multiProcessorCount;
+    void* p;
+    // bufs[0]: work counter (kept first: the intersection launcher uses it)
+    if ((rc = scene_alloc(s, 256, nullptr, &p)) != PRT_OK) return bail(rc);
+    if ((rc = scene_alloc(s, bvh.nodes.size() * sizeof(Node4), bvh.nodes.data(), &p)) != PRT_OK) return bail(rc);
+    s->dev.nodes = (const Node4*)p;
+    if ((rc = scene_alloc(s, ordered.size() * sizeof(float4), ordered.data(), &p)) != PRT_OK) return bail(rc);
+    s->dev.prims = (const float4*)p;
+    if ((rc = scene_alloc(s, qext.size() * sizeof(QuadExt), qext.data(), &p)) != PRT_OK) return bail(rc);
+    s->dev.quads = (const QuadExt*)p;
+    s->dev.mixed = (nsph + nq) > 0 ? 1 : 0;
+    s->dev.max_stack = bvh.max_stack;
+
+    // materials (float32 copies)
+    std::vector<DevMaterial> mats(std::max(nm, 1));
+    for (int i = 0; i < nm; i++) {
+        const prt_material& m = d->materials[i];
+        DevMaterial& o = mats[i];
+        o.type = m.type;
+        o.outer = m.outer;
+        o.inner = m.inner;
+        for (int a = 0; a < 3; a++) {
+            o.albedo[a] = (float)m.albedo[a];
+            o.emission[a] = (float)m.emission[a];
+        }
+        o.fuzz = (float)std::min(1.0, std::max(0.0, m.fuzz));  // NewMetal clamps (metal.go:19-29)
+        o.ior = (float)m.ior;
+    }
+    if ((rc = scene_alloc(s, mats.size() * sizeof(DevMaterial), mats.data(), &p)) != PRT_OK) return bail(rc);
+    s->dev.mats = (const DevMaterial*)p;
+
+    // lights
+    const int nl = d->num_lights;
+    std::vector<DevLight> lights(std::max(nl, 1));
+    std::vector<float> weights(std::max(nl, 1), 0.f);
+    int ninf = 0, nenv = 0;
+    for (int i = 0; i < nl; i++) {
+        const prt_light& L = d->lights[i];
+        DevLight& o = lights[i];
+        std::memset(&o, 0, sizeof(o));
+        o.type = L.type;
+        for (int a = 0; a < 3; a++) {
+            o.emission[a] = (float)L.emission[a];
+            o.top[a] = (float)L.top[a];
+            o.bottom[a] = (float)L.bottom[a];
+        }
+        if (L.type == PRT_LIGHT_QUAD) {
+            if (L.shape < 0 || L.shape >= d->num_quads)
+                return bail(fail(PRT_ERR_INVALID_ARGUMENT, "quad light shape out of range"));
+            const prt_quad& q = d->quads[L.shape];
+            D3 u = d3(q.u), v = d3(q.v), c = d3(q.corner);
+            D3 n = normalize(cross(u, v));
+            D3 w = n * (1.0 / dot(n, cross(u, v)));
+            for (int a = 0; a < 3; a++) {
+                o.corner[a] = (float)comp(c, a);
+                o.u[a] = (float)comp(u, a);
+                o.v[a] = (float)comp(v, a);
+                o.n[a] = (float)comp(n, a);
+                o.W[a] = (float)comp(w, a);
+            }
+            o.D = (float)dot(n, c);
+            o.area = (float)len(cross(u, v));  // NewQuadLight quad_light.go:19-29
+            o.mat = q.material;
+        } else if (L.type == PRT_LIGHT_SPHERE) {
+            if (L.shape < 0 || L.shape >= d->num_spheres)
+                return bail(fail(PRT_ERR_INVALID_ARGUMENT, "sphere light shape out of range"));
+            const prt_sphere& sp = d->spheres[L.shape];
+            for (int a = 0; a < 3; a++) o.center[a] = (float)sp.center[a];
+            o.radius = (float)sp.radius;
+            o.mat = sp.material;
+        } else if (L.type == PRT_LIGHT_UNIFORM_INFINITE || L.type == PRT_LIGHT_GRADIENT_INFINITE) {
+            ninf++;
+        } else if (L.type == PRT_LIGHT_ENVMAP) {
+            ninf++;
+            nenv++;
+            if (nenv > 1) return bail(fail(PRT_ERR_UNSUPPORTED, "at most one environment map light"));
+            if (!L.env_rgb || L.env_width <= 0 || L.env_height <= 0)
+                return bail(fail(PRT_ERR_INVALID_ARGUMENT, "environment map needs pixels"));
+            // piecewise-constant 2D distribution over luminance * sin(theta), built in float64
+            const int W = L.env_width, H = L.env_height;
+            const double pi = 3.14159265358979323846;
+            std::vector<float> func((size_t)W * H), col((size_t)H * (W + 1)), row(H + 1);
+            std::vector<double> rs(H), cc(W + 1);
+            for (int y = 0; y < H; y++) {
+                double st = std::sin((y + 0.5) / H * pi);
+                cc[0] = 0;
+                for (int x = 0; x < W; x++) {
+                    const float* px = L.env_rgb + ((size_t)y * W + x) * 3;
+                    double l = 0.2126 * (double)px[0] + 0.7152 * (double)px[1] + 0.0722 * (double)px[2];
+                    double f = std::max(l, 0.0) * st;
+                    func[(size_t)y * W + x] = (float)f;
+                    cc[x + 1] = cc[x] + f;
+                }
+                rs[y] = cc[W];
+                for (int x = 0; x <= W; x++)
+                    col[(size_t)y * (W + 1) + x] = (float)(rs[y] > 0 ? cc[x] / rs[y] : (double)x / W);
+                col[(size_t)y * (W + 1) + W] = 1.0f;
+            }
+            double acc = 0;
+            std::vector<double> rc64(H + 1);
+            rc64[0] = 0;
+            for (int y = 0; y < H; y++) rc64[y + 1] = rc64[y] + rs[y];
+            acc = rc64[H];
+            for (int y = 0; y <= H; y++) row[y] = (float)(acc > 0 ? rc64[y] / acc : (double)y / H);
+            row[H] = 1.0f;
+            s->dev.env.width = W;
+            s->dev.env.height = H;
+            s->dev.env.integral = (float)(acc / ((double)W * H));
+            if ((rc = scene_alloc(s, (size_t)W * H * 3 * 4, L.env_rgb, &p)) != PRT_OK) return bail(rc);
+            s->dev.env.rgb = (const float*)p;
+            if ((rc = scene_alloc(s, func.size() * 4, func.data(), &p)) != PRT_OK) return bail(rc);
+            s->dev.env.func = (const float*)p;
+            if ((rc = scene_alloc(s, row.size() * 4, row.data(), &p)) != PRT_OK) return bail(rc);
+            s->dev.env.row_cdf = (const float*)p;
+            if ((rc = scene_alloc(s, col.size() * 4, col.data(), &p)) != PRT_OK) return bail(rc);
+            s->dev.env.col_cdf = (const float*)p;
+        } else {
+            return bail(fail(PRT_ERR_UNSUPPORTED, "unknown light type"));
+        }
+        if ((L.type == PRT_LIGHT_QUAD || L.type == PRT_LIGHT_SPHERE) && bad_mat(o.mat))
+            return bail(fail(PRT_ERR_INVALID_ARGUMENT, "light material out of range"));
+    }
+    // light selection probabilities (NewUniformLightSampler / NewWeightedLightSampler)
+    if (nl > 0) {
+        double tot = 0;
+        if (d->light_weights)
+            for (int i = 0; i < nl; i++) {
+                if (d->light_weights[i] < 0) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "negative light weight"));
+                tot += d->light_weights[i];
+            }
+        for (int i = 0; i < nl; i++)
+            weights[i] = (float)((d->light_weights && tot > 0) ? d->light_weights[i] / tot : 1.0 / nl);
+    }
+    if ((rc = scene_alloc(s, lights.size() * sizeof(DevLight), lights.data(), &p)) != PRT_OK) return bail(rc);
+    s->dev.lights = (const DevLight*)p;
+    if ((rc = scene_alloc(s, weights.size() * 4, weights.data(), &p)) != PRT_OK) return bail(rc);
+    s->dev.light_weights = (const float*)p;
+    s->dev.num_lights = nl;
+    s->dev.num_infinite = ninf;
+    for (int a = 0; a < 3; a++) s->dev.world_center[a] = (float)s->info.world_center[a];
+    s->dev.world_radius = (float)s->info.world_radius;
+    int H = 0;
+    s->dev.cam = make_camera(d->camera, H);
+    s->image_width = d->camera.width;
+    s->image_height = H;
+    s->aspect = d->camera.aspect_ratio;
+
+    // overflow stack for the intersection kernels: one slot per resident thread
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_intersect<false, true, false>,
+                                                     kTraceBlock, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    per_cu = std::min(per_cu, 8);
+    uint32_t slots = (uint32_t)(s->num_cus * per_cu * kTraceBlock);
+    size_t depth = (size_t)std::max(1, s->max_stack - kTraceLds + 1);
+    if ((rc = scene_alloc(s, depth * slots * sizeof(uint2), nullptr, &p)) != PRT_OK) return bail(rc);
+    s->gstack = (uint2*)p;
+    s->gstack_slots = slots;
+    *out = s;
+    return PRT_OK;
+}
+
+int prt_scene_info_get(const prt_scene* scene, prt_scene_info* info) {
+    if (!scene || !info) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *info = scene->info;
+    return PRT_OK;
+}
+
+int prt_scene_destroy(prt_scene* scene) {
+    if (!scene) return PRT_OK;
+    (void)hipSetDevice(scene->device);
+    for (auto& b : scene->bufs) (void)hipFree(b.p);
+    delete scene;
+    return PRT_OK;
+}
+
+int prt_intersect_closest(const prt_scene* scene, const prt_ray* rays, int64_t n, prt_hit* hits, uint32_t flags,
+                          void* stream) {
+    return launch_intersect(scene, 0, rays, n, hits, nullptr, nullptr, flags, stream);
+}
+
+int prt_intersect_any(const prt_scene* scene, const prt_ray* rays, int64_t n, uint8_t* occluded, uint32_t flags,
+                      void* stream) {
+    return launch_intersect(scene, 1, rays, n, occluded, nullptr, nullptr, flags, stream);
+}
+
+int prt_traversal_counts(const prt_scene* scene, const prt_ray* rays, int64_t n, uint32_t* node_visits,
+                         uint32_t* box_tests, uint32_t* prim_tests, uint32_t flags, void* stream) {
+    if (!box_tests || !prim_tests) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL buffer");
+    return launch_intersect(scene, 2, rays, n, node_visits, box_tests, prim_tests, flags, stream);
+}
+
+}  // extern "C"

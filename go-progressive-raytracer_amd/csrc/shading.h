@@ -1,0 +1,513 @@
+// shading.h — float32 device versions of the reference's camera, BSDF and light routines.
+// Each function cites the reference routine it follows (paths relative to the reference tree).
+#pragma once
+#include "prt_common.h"
+#include "traverse.h"
+
+namespace prt {
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 v3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 mulv(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) {  // Vec3.Normalize: zero stays zero
+    float l = length(a);
+    if (l == 0.0f) return {0, 0, 0};
+    return {a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ float lum(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+constexpr float kPiF = 3.14159265358979323846f;
+
+// ---- warps, pkg/core/sampling.go ------------------------------------------------------
+__device__ __forceinline__ V3 sample_cosine_hemisphere(V3 n, float s0, float s1) {  // :42-67
+    float a = 2.0f * kPiF * s0;
+    float r = sqrtf(s1);
+    float sa, ca;
+    sincosf(a, &sa, &ca);
+    float x = r * ca, y = r * sa, zc = sqrtf(1.0f - s1);
+    V3 nt = fabsf(n.x) > 0.1f ? v3(0, 1, 0) : v3(1, 0, 0);
+    V3 t = normalize(cross(nt, n));
+    V3 b = cross(n, t);
+    return t * x + b * y + n * zc;
+}
+__device__ __forceinline__ V3 sample_point_in_unit_disk(float s0, float s1) {  // :108-126
+    float ox = 2.0f * s0 - 1.0f, oy = 2.0f * s1 - 1.0f;
+    if (ox == 0.0f && oy == 0.0f) return {0, 0, 0};
+    float theta, r;
+    if (fabsf(ox) > fabsf(oy)) {
+        r = ox;
+        theta = kPiF / 4.0f * (oy / ox);
+    } else {
+        r = oy;
+        theta = kPiF / 2.0f - kPiF / 4.0f * (ox / oy);
+    }
+    float st, ct;
+    sincosf(theta, &st, &ct);
+    return {r * ct, r * st, 0};
+}
+__device__ __forceinline__ V3 sample_point_in_unit_sphere(float s0, float s1, float s2) {  // :130-147
+    float r = cbrtf(s0);
+    float phi = 2.0f * kPiF * s1;
+    float ct = 2.0f * s2 - 1.0f;
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    return {r * st * cp, r * st * sp, r * ct};
+}
+
+// ---- camera, pkg/geometry/camera.go:173-190 --------------------------------------------
+__device__ __forceinline__ void camera_ray(const DevCamera& c, int i, int j, float l0, float l1, float j0, float j1,
+                                           V3& o, V3& d) {
+    float fx = (float)i + (j0 - 0.5f), fy = (float)j + (j1 - 0.5f);
+    V3 ps = v3(c.pixel00) + v3(c.du) * fx + v3(c.dv) * fy;
+    o = v3(c.center);
+    if (c.has_lens) {
+        V3 p = sample_point_in_unit_disk(l0, l1);
+        o = o + v3(c.disk_u) * p.x + v3(c.disk_v) * p.y;
+    }
+    d = normalize(ps - o);
+}
+
+// ---- surface interaction ----------------------------------------------------------------
+struct Surf {
+    V3 p, n;
+    bool front;
+    int mat;
+};
+
+// Hit record of a primitive hit (SetFaceNormal, material/interfaces.go:58-65).
+__device__ __forceinline__ Surf make_surf(const DevScene& sc, V3 o, V3 d, const THit& h) {
+    const float4* pp = sc.prims + 3 * (size_t)h.prim;
+    float4 a = pp[0], b = pp[1];
+    uint32_t tag = __float_as_uint(a.w);
+    uint32_t kind = prim_kind(tag);
+    Surf s;
+    s.p = o + d * h.t;
+    s.mat = (int)__float_as_uint(b.w);
+    V3 outward;
+    if (kind == kPrimTriangle) {
+        float4 c = pp[2];
+        V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+        outward = normalize(cross(v1 - v0, v2 - v0));  // triangle.go:92-96
+    } else if (kind == kPrimSphere) {
+        outward = (s.p - v3(a.x, a.y, a.z)) * (1.0f / b.x);  // sphere.go:63, sign of r kept
+    } else {
+        const QuadExt& q = sc.quads[prim_side(tag)];
+        outward = v3(q.nx, q.ny, q.nz);
+    }
+    s.front = dot(d, outward) < 0.0f;
+    s.n = s.front ? outward : outward * -1.0f;
+    return s;
+}
+
+// ---- materials, pkg/material ------------------------------------------------------------
+struct ScatterOut {
+    V3 dir, att;
+    float pdf;
+};
+
+__device__ __forceinline__ V3 reflect(V3 v, V3 n) { return v - n * (2.0f * dot(v, n)); }
+__device__ __forceinline__ V3 refract(V3 uv, V3 n, float eta) {  // dielectric.go:155-161
+    float ct = fminf(-dot(uv, n), 1.0f);
+    V3 perp = (uv + n * ct) * eta;
+    V3 par = n * (-sqrtf(fabsf(1.0f - dot(perp, perp))));
+    return perp + par;
+}
+__device__ __forceinline__ float reflectance(float c, float ratio) {  // dielectric.go:164-172
+    float r0 = (1.0f - ratio) / (1.0f + ratio);
+    r0 = r0 * r0;
+    float x = 1.0f - c;
+    float x2 = x * x;
+    return r0 + (1.0f - r0) * (x2 * x2 * x);
+}
+__device__ __forceinline__ bool is_reflection_path(V3 in, V3 out, V3 n) {  // layered.go:110-119
+    V3 inc = in * -1.0f;
+    V3 refl = inc - n * (2.0f * dot(inc, n));
+    return length(out - refl) < 0.1f;
+}
+
+// Scatter of a non-layered material; draws from stream positions base+k.
+__device__ __forceinline__ bool scatter_basic(const DevMaterial& M, V3 din, const Surf& s, uint32_t key, uint32_t pos,
+                                              uint32_t& used, ScatterOut& out) {
+    switch (M.type) {
+        case 0: {  // Lambertian, lambertian.go:25-50
+            float u0 = draw(key, pos), u1 = draw(key, pos + 1);
+            used = 2;
+            V3 d = sample_cosine_hemisphere(s.n, u0, u1);
+            float ct = dot(normalize(d), s.n);
+            if (ct < 0.0f) ct = 0.0f;
+            out.dir = d;
+            out.pdf = ct / kPiF;
+            out.att = v3(M.albedo) * (1.0f / kPiF);
+            return true;
+        }
+        case 1: {  // Metal, metal.go:38-59
+            V3 r = reflect(normalize(din), s.n);
+            used = 0;
+            if (M.fuzz > 0.0f) {
+                float u0 = draw(key, pos), u1 = draw(key, pos + 1), u2 = draw(key, pos + 2);
+                used = 3;
+                r = r + sample_point_in_unit_sphere(u0, u1, u2) * M.fuzz;
+            }
+            out.dir = r;
+            out.att = v3(M.albedo);
+            out.pdf = 0.0f;
+            return dot(r, s.n) > 0.0f;
+        }
+        case 2: {  // Dielectric, dielectric.go:20-60
+            float ratio = s.front ? 1.0f / M.ior : M.ior;
+            V3 ud = normalize(din);
+            float ct = fminf(-dot(ud, s.n), 1.0f);
+            float st = sqrtf(1.0f - ct * ct);
+            bool cannot = ratio * st > 1.0f;
+            used = 0;
+            bool refl = cannot;
+            if (!cannot) {
+                used = 1;
+                refl = reflectance(ct, ratio) > draw(key, pos);
+            }
+            out.dir = refl ? reflect(ud, s.n) : refract(ud, s.n, ratio);
+            out.att = v3(1, 1, 1);
+            out.pdf = 0.0f;
+            return true;
+        }
+        default:  // Emissive never scatters (emissive.go:24-27)
+            used = 0;
+            return false;
+    }
+}
+
+// Material.Scatter including Layered (layered.go:24-69). Draws start at pos.
+__device__ __forceinline__ bool scatter(const DevScene& sc, int m, V3 din, const Surf& s, uint32_t key, uint32_t pos,
+                                        ScatterOut& out) {
+    const DevMaterial& M = sc.mats[m];
+    uint32_t used = 0;
+    if (M.type != 4) return scatter_basic(M, din, s, key, pos, used, out);
+    ScatterOut o;
+    if (!scatter_basic(sc.mats[M.outer], din, s, key, pos, used, o)) return false;
+    V3 sd = normalize(o.dir);
+    if (!(dot(sd, s.n) < 0.0f)) {
+        out = o;
+        return true;
+    }
+    ScatterOut i;
+    uint32_t used2 = 0;
+    if (!scatter_basic(sc.mats[M.inner], sd, s, key, pos + used, used2, i)) {
+        out = o;
+        return true;
+    }
+    out.dir = i.dir;
+    out.att = mulv(o.att, i.att);
+    out.pdf = i.pdf;
+    return true;
+}
+
+__device__ __forceinline__ V3 brdf_basic(const DevMaterial& M, V3 in, V3 outd, const Surf& s, int mode) {
+    switch (M.type) {
+        case 0:  // lambertian.go:53-63
+            if (dot(outd, s.n) <= 0.0f) return {0, 0, 0};
+            return v3(M.albedo) * (1.0f / kPiF);
+        case 1: {  // metal.go:62-74
+            V3 r = reflect(in * -1.0f, s.n);
+            if (length(outd - r) < 0.001f) return v3(M.albedo);
+            return {0, 0, 0};
+        }
+        case 2: {  // dielectric.go:63-143
+            V3 wi = normalize(in), wo = normalize(outd), n = normalize(s.n);
+            float wiN = dot(wi, n), woN = dot(wo, n);
+            bool same = (wiN > 0.0f) == (woN > 0.0f);
+            float eta = s.front ? 1.0f / M.ior : M.ior;
+            if (!same) {
+                V3 r = reflect(wi, n);
+                if (length(wo - r) < 0.001f) {
+                    float f = reflectance(fabsf(wiN), eta);
+                    return {f, f, f};
+                }
+            } else {
+                V3 rr = refract(wi, n, eta);
+                if (length(wo - rr) < 0.001f) {
+                    float tr = 1.0f - reflectance(fabsf(wiN), eta);
+                    if (mode == 0) {
+                        float etap = s.front ? M.ior : 1.0f / M.ior;
+                        tr /= etap * etap;
+                    }
+                    return {tr, tr, tr};
+                }
+            }
+            return {0, 0, 0};
+        }
+        default:
+            return {0, 0, 0};
+    }
+}
+__device__ __forceinline__ V3 brdf(const DevScene& sc, int m, V3 in, V3 outd, const Surf& s, int mode) {
+    const DevMaterial& M = sc.mats[m];
+    if (M.type != 4) return brdf_basic(M, in, outd, s, mode);
+    if (is_reflection_path(in, outd, s.n)) return brdf_basic(sc.mats[M.outer], in, outd, s, mode);  // layered.go:73-87
+    return brdf_basic(sc.mats[M.inner], in, outd, s, mode);
+}
+__device__ __forceinline__ float pdf_basic(const DevMaterial& M, V3 outd, V3 n, bool& delta) {
+    if (M.type == 0) {  // lambertian.go:66-72
+        delta = false;
+        float ct = dot(outd, n);
+        return ct <= 0.0f ? 0.0f : ct / kPiF;
+    }
+    delta = (M.type == 1 || M.type == 2);  // metal/dielectric are delta; emissive is not
+    return 0.0f;
+}
+__device__ __forceinline__ float mat_pdf(const DevScene& sc, int m, V3 in, V3 outd, V3 n, bool& delta) {
+    const DevMaterial& M = sc.mats[m];
+    if (M.type != 4) return pdf_basic(M, outd, n, delta);
+    bool d2;
+    float p = is_reflection_path(in, outd, n) ? pdf_basic(sc.mats[M.outer], outd, n, d2)
+                                              : pdf_basic(sc.mats[M.inner], outd, n, d2);
+    delta = false;  // layered.go:90-104
+    return p;
+}
+
+// ---- lights, pkg/lights ------------------------------------------------------------------
+struct LSample {
+    V3 dir, emission;
+    float dist, pdf;
+};
+
+// Emit of the light's material with no hit record (emissive.go:30-37); zero if not emissive
+__device__ __forceinline__ V3 emissive(const DevScene& sc, int m) {
+    const DevMaterial& M = sc.mats[m];
+    return M.type == 3 ? v3(M.emission) : v3(0, 0, 0);
+}
+
+// sphere test for light routines (sphere.go:27-81), returns t or -1
+__device__ __forceinline__ float sphere_hit_t(V3 c, float rad, V3 o, V3 d, float tmin, float tmax) {
+    V3 oc = o - c;
+    float A = dot(d, d), hb = dot(oc, d), C = dot(oc, oc) - rad * rad;
+    float disc = hb * hb - A * C;
+    if (disc < 0.0f) return -1.0f;
+    float sq = sqrtf(disc);
+    float root = (-hb - sq) / A;
+    if (root < tmin || root > tmax) {
+        root = (-hb + sq) / A;
+        if (root < tmin || root > tmax) return -1.0f;
+    }
+    return root;
+}
+
+__device__ __forceinline__ void env_lookup(const DevEnv& e, V3 d, int& x, int& y) {
+    d = normalize(d);
+    float cy = fmaxf(-1.0f, fminf(1.0f, d.y));
+    float theta = acosf(cy);
+    float phi = atan2f(d.z, d.x);
+    if (phi < 0.0f) phi += 2.0f * kPiF;
+    x = min((int)(phi / (2.0f * kPiF) * e.width), e.width - 1);
+    y = min((int)(theta / kPiF * e.height), e.height - 1);
+    x = max(x, 0);
+    y = max(y, 0);
+}
+__device__ __forceinline__ V3 env_eval(const DevEnv& e, V3 d) {
+    int x, y;
+    env_lookup(e, d, x, y);
+    const float* p = e.rgb + ((size_t)y * e.width + x) * 3;
+    return v3(p[0], p[1], p[2]);
+}
+__device__ __forceinline__ float env_pdf(const DevEnv& e, V3 d) {
+    if (!(e.integral > 0.0f)) return 0.0f;
+    int x, y;
+    env_lookup(e, d, x, y);
+    float st = sinf(((float)y + 0.5f) / e.height * kPiF);
+    if (st <= 0.0f) return 0.0f;
+    return (e.func[(size_t)y * e.width + x] / e.integral) / (2.0f * kPiF * kPiF * st);
+}
+__device__ __forceinline__ int find_interval(const float* cdf, int n, float u) {
+    int lo = 0, hi = n;
+    while (lo + 1 < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ V3 env_sample(const DevEnv& e, float s0, float s1, float& pdf) {
+    int y = find_interval(e.row_cdf, e.height, s0);
+    float dy = e.row_cdf[y + 1] - e.row_cdf[y];
+    float fy = dy > 0.0f ? (s0 - e.row_cdf[y]) / dy : 0.5f;
+    const float* cc = e.col_cdf + (size_t)y * (e.width + 1);
+    int x = find_interval(cc, e.width, s1);
+    float dx = cc[x + 1] - cc[x];
+    float fx = dx > 0.0f ? (s1 - cc[x]) / dx : 0.5f;
+    float u = ((float)x + fx) / e.width, v = ((float)y + fy) / e.height;
+    float theta = v * kPiF, phi = u * 2.0f * kPiF;
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);
+    sincosf(phi, &sp, &cp);
+    float sc = sinf(((float)y + 0.5f) / e.height * kPiF);
+    pdf = (sc > 0.0f && e.integral > 0.0f) ? (e.func[(size_t)y * e.width + x] / e.integral) / (2.0f * kPiF * kPiF * sc)
+                                           : 0.0f;
+    return v3(st * cp, ct, st * sp);
+}
+
+__device__ __forceinline__ bool light_is_infinite(int type) { return type >= PRT_LIGHT_UNIFORM_INFINITE; }
+
+// Light.Emit of an infinite light for a ray leaving the scene
+__device__ __forceinline__ V3 light_emit(const DevScene& sc, const DevLight& L, V3 d) {
+    if (L.type == PRT_LIGHT_UNIFORM_INFINITE) return v3(L.emission);  // uniform_infinite_light.go:124-126
+    if (L.type == PRT_LIGHT_GRADIENT_INFINITE) {                       // gradient_infinite_light.go:133-137
+        V3 n = normalize(d);
+        float t = 0.5f * (n.y + 1.0f);
+        return v3(L.bottom) * (1.0f - t) + v3(L.top) * t;
+    }
+    if (L.type == PRT_LIGHT_ENVMAP) return env_eval(sc.env, d);
+    return {0, 0, 0};
+}
+
+__device__ __forceinline__ LSample sphere_sample_uniform(const DevScene& sc, const DevLight& L, V3 p, float s0, float s1) {
+    float z = 1.0f - 2.0f * s0;  // sphere_light.go:52-77
+    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sp, cp;
+    sincosf(2.0f * kPiF * s1, &sp, &cp);
+    V3 ld = v3(r * cp, r * sp, z);
+    V3 pt = v3(L.center) + ld * L.radius;
+    V3 dir = pt - p;
+    LSample ls;
+    ls.dist = length(dir);
+    ls.dir = normalize(dir);
+    ls.pdf = 1.0f / (4.0f * kPiF * L.radius * L.radius);
+    ls.emission = emissive(sc, L.mat);
+    return ls;
+}
+
+// Light.Sample (interfaces.go:22-25) for every supported light type
+__device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLight& L, V3 p, V3 n, float s0, float s1) {
+    LSample ls;
+    if (L.type == PRT_LIGHT_QUAD) {  // quad_light.go:36-84
+        V3 spt = v3(L.corner) + v3(L.u) * s0 + v3(L.v) * s1;
+        V3 tl = spt - p;
+        float dist = length(tl);
+        V3 dir = tl * (1.0f / dist);
+        V3 qn = v3(L.n);
+        float ct = fabsf(dot(qn, dir * -1.0f));
+        ls.dir = dir;
+        ls.dist = dist;
+        if (ct < 1e-8f) {
+            ls.emission = {0, 0, 0};
+            ls.pdf = 0.0f;
+            return ls;
+        }
+        ls.pdf = (1.0f / L.area) * dist * dist / ct;
+        ls.emission = dot(dir, qn) < 0.0f ? emissive(sc, L.mat) : v3(0, 0, 0);
+        return ls;
+    }
+    if (L.type == PRT_LIGHT_SPHERE) {  // sphere_light.go:37-141
+        V3 c = v3(L.center);
+        V3 tc = c - p;
+        float dc = length(tc);
+        if (dc <= L.radius) return sphere_sample_uniform(sc, L, p, s0, s1);
+        V3 w = normalize(tc);
+        V3 u = fabsf(w.x) > 0.1f ? v3(0, 1, 0) : v3(1, 0, 0);
+        u = normalize(cross(u, w));
+        V3 v = cross(w, u);
+        float stm = L.radius / dc;
+        float ctm = sqrtf(fmaxf(0.0f, 1.0f - stm * stm));
+        float ct = 1.0f - s0 * (1.0f - ctm);
+        float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        float sp, cp;
+        sincosf(2.0f * kPiF * s1, &sp, &cp);
+        V3 dir = u * (st * cp) + v * (st * sp) + w * ct;
+        float t = sphere_hit_t(c, L.radius, p, dir, 0.001f, __int_as_float(0x7f800000));
+        if (t < 0.0f) return sphere_sample_uniform(sc, L, p, s0, s1);
+        ls.dir = dir;
+        ls.dist = t;
+        ls.pdf = 1.0f / (2.0f * kPiF * (1.0f - ctm));
+        ls.emission = emissive(sc, L.mat);
+        return ls;
+    }
+    if (L.type == PRT_LIGHT_ENVMAP) {
+        float pdf;
+        V3 dir = env_sample(sc.env, s0, s1, pdf);
+        ls.dir = dir;
+        ls.dist = __int_as_float(0x7f800000);
+        ls.pdf = pdf;
+        ls.emission = env_eval(sc.env, dir);
+        return ls;
+    }
+    // uniform / gradient infinite: cosine hemisphere about the shading normal
+    V3 dir = sample_cosine_hemisphere(n, s0, s1);
+    ls.dir = dir;
+    ls.dist = __int_as_float(0x7f800000);
+    ls.pdf = dot(dir, n) / kPiF;
+    if (L.type == PRT_LIGHT_UNIFORM_INFINITE) {
+        ls.emission = v3(L.emission);
+    } else {
+        float t = 0.5f * (dir.y + 1.0f);
+        ls.emission = v3(L.bottom) * (1.0f - t) + v3(L.top) * t;
+    }
+    return ls;
+}
+
+// Light.PDF (interfaces.go:27-29)
+__device__ __forceinline__ float light_pdf(const DevScene& sc, const DevLight& L, V3 p, V3 n, V3 dir) {
+    if (L.type == PRT_LIGHT_QUAD) {  // quad_light.go:87-104: re-hit the quad from p
+        V3 qn = v3(L.n);
+        float den = dot(dir, qn);
+        if (fabsf(den) < 1e-8f) return 0.0f;
+        float t = (L.D - dot(p, qn)) / den;
+        if (t < 0.001f) return 0.0f;
+        V3 hv = (p + dir * t) - v3(L.corner);
+        float alpha = dot(v3(L.W), cross(hv, v3(L.v)));
+        float beta = dot(v3(L.W), cross(v3(L.u), hv));
+        if (alpha < 0.0f || alpha > 1.0f || beta < 0.0f || beta > 1.0f) return 0.0f;
+        float ct = fabsf(dot(qn, dir * -1.0f));
+        if (ct < 1e-8f) return 0.0f;
+        return (1.0f / L.area) * t * t / ct;
+    }
+    if (L.type == PRT_LIGHT_SPHERE) {  // sphere_light.go:144-165
+        V3 c = v3(L.center);
+        if (sphere_hit_t(c, L.radius, p, dir, 0.001f, __int_as_float(0x7f800000)) < 0.0f) return 0.0f;
+        float dc = length(c - p);
+        if (dc <= L.radius) return 1.0f / (4.0f * kPiF * L.radius * L.radius);
+        float stm = L.radius / dc;
+        float ctm = sqrtf(fmaxf(0.0f, 1.0f - stm * stm));
+        return 1.0f / (2.0f * kPiF * (1.0f - ctm));
+    }
+    if (L.type == PRT_LIGHT_ENVMAP) return env_pdf(sc.env, dir);
+    float ct = dot(dir, n);  // uniform/gradient: cosine hemisphere
+    return ct <= 0.0f ? 0.0f : ct / kPiF;
+}
+
+// WeightedLightSampler.SampleLight: CDF walk with u <= cumulative (weighted_light_sampler.go:82-96)
+__device__ __forceinline__ int select_light(const DevScene& sc, float u, float& prob) {
+    float cum = 0.0f;
+    for (int i = 0; i < sc.num_lights; i++) {
+        cum += sc.light_weights[i];
+        if (u <= cum) {
+            prob = sc.light_weights[i];
+            return i;
+        }
+    }
+    prob = sc.light_weights[sc.num_lights - 1];
+    return sc.num_lights - 1;
+}
+
+// CalculateLightPDF (light_sampling.go:11-25)
+__device__ __forceinline__ float total_light_pdf(const DevScene& sc, V3 p, V3 n, V3 dir) {
+    float tot = 0.0f;
+    for (int i = 0; i < sc.num_lights; i++) tot += light_pdf(sc, sc.lights[i], p, n, dir) * sc.light_weights[i];
+    return tot;
+}
+
+__device__ __forceinline__ float power_heuristic(float f, float g) {  // path_tracing.go:224-234
+    if (f == 0.0f) return 0.0f;
+    return (f * f) / (f * f + g * g);
+}
+
+}  // namespace prt

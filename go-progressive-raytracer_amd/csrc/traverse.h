@@ -1,0 +1,300 @@
+// traverse.h — device-side ray traversal of the 4-wide BVH (closest hit / any hit).
+//
+// Semantics follow BVH.Hit (pkg/geometry/bvh.go:183-239): a hit is any primitive hit with
+// t in [tmin, tmax] (inclusive), the closest one is reported. Triangles are two-sided
+// (triangle.go:107-175) and use the watertight test of Woop, Benthin and Wald (JCGT 2013)
+// in float32 with a float64 fallback on exact edge hits; boxes use the robust slab test
+// with the far distance widened by 1 + 2*gamma(3) (Ize 2013) so that float32 boxes never
+// reject a ray that the exact box would accept.
+#pragma once
+#include "prt_common.h"
+
+namespace prt {
+
+struct TRay {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+};
+
+struct THit {
+    float t;
+    uint32_t prim;  // index into the leaf-ordered primitive array, 0xFFFFFFFF on a miss
+    float u, v;
+};
+
+struct TCounts {
+    uint32_t nodes, boxes, prims;
+};
+
+__device__ __forceinline__ float sel3(float x, float y, float z, int k) {
+    return k == 0 ? x : (k == 1 ? y : z);
+}
+
+__device__ __forceinline__ uint32_t pick4(uint4 c, uint32_t i) {
+    return i == 0 ? c.x : (i == 1 ? c.y : (i == 2 ? c.z : c.w));
+}
+
+// Per-ray constants of the watertight triangle test.
+struct WRay {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ WRay make_wray(float dx, float dy, float dz) {
+    WRay w;
+    float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dkz = sel3(dx, dy, dz, kz);
+    if (dkz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    w.kx = kx;
+    w.ky = ky;
+    w.kz = kz;
+    w.Sz = 1.0f / dkz;
+    w.Sx = sel3(dx, dy, dz, kx) * w.Sz;
+    w.Sy = sel3(dx, dy, dz, ky) * w.Sz;
+    return w;
+}
+
+// Watertight two-sided ray/triangle test. On a hit with t in [tmin, tbest] writes t, u, v
+// (u, v = barycentric weights of v1, v2) and returns true.
+__device__ __forceinline__ bool tri_test(const TRay& r, const WRay& w, float4 a, float4 b, float4 c,
+                                         float tbest, float& t_out, float& u_out, float& v_out) {
+    float Ax = a.x - r.ox, Ay = a.y - r.oy, Az = a.z - r.oz;
+    float Bx = b.x - r.ox, By = b.y - r.oy, Bz = b.z - r.oz;
+    float Cx = c.x - r.ox, Cy = c.y - r.oy, Cz = c.z - r.oz;
+    float akx = sel3(Ax, Ay, Az, w.kx), aky = sel3(Ax, Ay, Az, w.ky), akz = sel3(Ax, Ay, Az, w.kz);
+    float bkx = sel3(Bx, By, Bz, w.kx), bky = sel3(Bx, By, Bz, w.ky), bkz = sel3(Bx, By, Bz, w.kz);
+    float ckx = sel3(Cx, Cy, Cz, w.kx), cky = sel3(Cx, Cy, Cz, w.ky), ckz = sel3(Cx, Cy, Cz, w.kz);
+    float ax = akx - w.Sx * akz, ay = aky - w.Sy * akz;
+    float bx = bkx - w.Sx * bkz, by = bky - w.Sy * bkz;
+    float cx = ckx - w.Sx * ckz, cy = cky - w.Sy * ckz;
+    float U = cx * by - cy * bx;
+    float V = ax * cy - ay * cx;
+    float W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double U64 = (double)cx * (double)by - (double)cy * (double)bx;
+        double V64 = (double)ax * (double)cy - (double)ay * (double)cx;
+        double W64 = (double)bx * (double)ay - (double)by * (double)ax;
+        U = (float)U64;
+        V = (float)V64;
+        W = (float)W64;
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float az2 = w.Sz * akz, bz2 = w.Sz * bkz, cz2 = w.Sz * ckz;
+    float T = U * az2 + V * bz2 + W * cz2;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= r.tmin && t <= tbest)) return false;
+    t_out = t;
+    u_out = V * rcp;
+    v_out = W * rcp;
+    return true;
+}
+
+// Sphere test (sphere.go:27-81): nearer root, else farther root, in [tmin, tbest].
+__device__ __forceinline__ bool sphere_test(const TRay& r, float4 a, float4 b, float tbest, float& t_out) {
+    float ocx = r.ox - a.x, ocy = r.oy - a.y, ocz = r.oz - a.z;
+    float rad = b.x;
+    float A = r.dx * r.dx + r.dy * r.dy + r.dz * r.dz;
+    float hb = ocx * r.dx + ocy * r.dy + ocz * r.dz;
+    float C = ocx * ocx + ocy * ocy + ocz * ocz - rad * rad;
+    float disc = hb * hb - A * C;
+    if (disc < 0.0f) return false;
+    float sq = sqrtf(disc);
+    float root = (-hb - sq) / A;
+    if (root < r.tmin || root > tbest) {
+        root = (-hb + sq) / A;
+        if (root < r.tmin || root > tbest) return false;
+    }
+    t_out = root;
+    return true;
+}
+
+// Quad test (quad.go:133-180).
+__device__ __forceinline__ bool quad_test(const TRay& r, float4 a, float4 b, float4 c, const QuadExt& q,
+                                          float tbest, float& t_out, float& u_out, float& v_out) {
+    float den = r.dx * q.nx + r.dy * q.ny + r.dz * q.nz;
+    if (fabsf(den) < 1e-8f) return false;
+    float t = (q.D - (r.ox * q.nx + r.oy * q.ny + r.oz * q.nz)) / den;
+    if (t < r.tmin || t > tbest) return false;
+    float px = r.ox + t * r.dx - a.x, py = r.oy + t * r.dy - a.y, pz = r.oz + t * r.dz - a.z;
+    // alpha = W . (hv x v), beta = W . (u x hv)
+    float hxv_x = py * c.z - pz * c.y, hxv_y = pz * c.x - px * c.z, hxv_z = px * c.y - py * c.x;
+    float uxh_x = b.y * pz - b.z * py, uxh_y = b.z * px - b.x * pz, uxh_z = b.x * py - b.y * px;
+    float alpha = q.wx * hxv_x + q.wy * hxv_y + q.wz * hxv_z;
+    float beta = q.wx * uxh_x + q.wy * uxh_y + q.wz * uxh_z;
+    if (alpha < 0.0f || alpha > 1.0f || beta < 0.0f || beta > 1.0f) return false;
+    t_out = t;
+    u_out = alpha;
+    v_out = beta;
+    return true;
+}
+
+constexpr float kBoxWiden = 1.0f + 2.0f * (3.0f * 5.9604645e-08f) / (1.0f - 3.0f * 5.9604645e-08f);
+
+// Traversal stack: the first LDS_DEPTH entries per lane live in LDS (lane-major so each lane
+// owns one bank column), deeper entries spill to a global array indexed by the thread slot.
+template <int LDS_DEPTH, int BLOCK>
+struct TStack {
+    uint32_t* s_node;  // [LDS_DEPTH][BLOCK]
+    float* s_t;        // [LDS_DEPTH][BLOCK]
+    uint2* g;          // [(max_stack - LDS_DEPTH)][gstride]
+    uint32_t lane;     // threadIdx.x
+    uint32_t gslot;    // global thread slot
+    uint32_t gstride;
+    __device__ __forceinline__ void push(int sp, uint32_t node, float t) {
+        if (sp < LDS_DEPTH) {
+            s_node[sp * BLOCK + lane] = node;
+            s_t[sp * BLOCK + lane] = t;
+        } else {
+            g[(size_t)(sp - LDS_DEPTH) * gstride + gslot] = make_uint2(node, __float_as_uint(t));
+        }
+    }
+    __device__ __forceinline__ void pop(int sp, uint32_t& node, float& t) {
+        if (sp < LDS_DEPTH) {
+            node = s_node[sp * BLOCK + lane];
+            t = s_t[sp * BLOCK + lane];
+        } else {
+            uint2 e = g[(size_t)(sp - LDS_DEPTH) * gstride + gslot];
+            node = e.x;
+            t = __uint_as_float(e.y);
+        }
+    }
+};
+
+// Closest hit (ANY = false) or any hit (ANY = true). Returns true on a hit.
+template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK>
+__device__ bool trace(const DevScene& sc, const TRay& r_in, THit& hit, TStack<LDS_DEPTH, BLOCK>& st,
+                      TCounts* cnt) {
+    TRay r = r_in;
+    if (!(r.tmin >= 0.0f)) r.tmin = 0.0f;
+    // reciprocal direction; zero components become huge so that 0 * inf never occurs
+    float idx = 1.0f / (fabsf(r.dx) > 1e-30f ? r.dx : copysignf(1e-30f, r.dx));
+    float idy = 1.0f / (fabsf(r.dy) > 1e-30f ? r.dy : copysignf(1e-30f, r.dy));
+    float idz = 1.0f / (fabsf(r.dz) > 1e-30f ? r.dz : copysignf(1e-30f, r.dz));
+    const bool px = idx >= 0.0f, py = idy >= 0.0f, pz = idz >= 0.0f;
+    WRay w = make_wray(r.dx, r.dy, r.dz);
+
+    float tbest = r.tmax;
+    uint32_t best = 0xFFFFFFFFu;
+    float bu = 0.0f, bv = 0.0f;
+    int sp = 0;
+    uint32_t cur = 0;  // root node
+    for (;;) {
+        if (!child_is_leaf(cur)) {
+            // ---- inner node: test the 4 child boxes ----
+            const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
+            float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+            if (COUNT) {
+                cnt->nodes++;
+                cnt->boxes += 4;
+            }
+            float nx0 = ((px ? lx.x : hx.x) - r.ox) * idx, fx0 = ((px ? hx.x : lx.x) - r.ox) * idx;
+            float nx1 = ((px ? lx.y : hx.y) - r.ox) * idx, fx1 = ((px ? hx.y : lx.y) - r.ox) * idx;
+            float nx2 = ((px ? lx.z : hx.z) - r.ox) * idx, fx2 = ((px ? hx.z : lx.z) - r.ox) * idx;
+            float nx3 = ((px ? lx.w : hx.w) - r.ox) * idx, fx3 = ((px ? hx.w : lx.w) - r.ox) * idx;
+            float ny0 = ((py ? ly.x : hy.x) - r.oy) * idy, fy0 = ((py ? hy.x : ly.x) - r.oy) * idy;
+            float ny1 = ((py ? ly.y : hy.y) - r.oy) * idy, fy1 = ((py ? hy.y : ly.y) - r.oy) * idy;
+            float ny2 = ((py ? ly.z : hy.z) - r.oy) * idy, fy2 = ((py ? hy.z : ly.z) - r.oy) * idy;
+            float ny3 = ((py ? ly.w : hy.w) - r.oy) * idy, fy3 = ((py ? hy.w : ly.w) - r.oy) * idy;
+            float nz0 = ((pz ? lz.x : hz.x) - r.oz) * idz, fz0 = ((pz ? hz.x : lz.x) - r.oz) * idz;
+            float nz1 = ((pz ? lz.y : hz.y) - r.oz) * idz, fz1 = ((pz ? hz.y : lz.y) - r.oz) * idz;
+            float nz2 = ((pz ? lz.z : hz.z) - r.oz) * idz, fz2 = ((pz ? hz.z : lz.z) - r.oz) * idz;
+            float nz3 = ((pz ? lz.w : hz.w) - r.oz) * idz, fz3 = ((pz ? hz.w : lz.w) - r.oz) * idz;
+            float tn0 = fmaxf(fmaxf(nx0, ny0), fmaxf(nz0, r.tmin));
+            float tn1 = fmaxf(fmaxf(nx1, ny1), fmaxf(nz1, r.tmin));
+            float tn2 = fmaxf(fmaxf(nx2, ny2), fmaxf(nz2, r.tmin));
+            float tn3 = fmaxf(fmaxf(nx3, ny3), fmaxf(nz3, r.tmin));
+            float tf0 = fminf(fminf(fx0, fy0), fz0) * kBoxWiden;
+            float tf1 = fminf(fminf(fx1, fy1), fz1) * kBoxWiden;
+            float tf2 = fminf(fminf(fx2, fy2), fz2) * kBoxWiden;
+            float tf3 = fminf(fminf(fx3, fy3), fz3) * kBoxWiden;
+            tf0 = fminf(tf0, tbest);
+            tf1 = fminf(tf1, tbest);
+            tf2 = fminf(tf2, tbest);
+            tf3 = fminf(tf3, tbest);
+            // sort keys: non-negative float bits with the slot in the low 2 bits (rounds the
+            // entry distance down, which only weakens culling)
+            uint32_t k0 = (tn0 <= tf0 && ch.x != kEmptyChild) ? ((__float_as_uint(tn0) & ~3u) | 0u) : 0xFFFFFFFFu;
+            uint32_t k1 = (tn1 <= tf1 && ch.y != kEmptyChild) ? ((__float_as_uint(tn1) & ~3u) | 1u) : 0xFFFFFFFFu;
+            uint32_t k2 = (tn2 <= tf2 && ch.z != kEmptyChild) ? ((__float_as_uint(tn2) & ~3u) | 2u) : 0xFFFFFFFFu;
+            uint32_t k3 = (tn3 <= tf3 && ch.w != kEmptyChild) ? ((__float_as_uint(tn3) & ~3u) | 3u) : 0xFFFFFFFFu;
+            // 4-element sorting network (ascending)
+            uint32_t t0 = min(k0, k1), t1 = max(k0, k1), t2 = min(k2, k3), t3 = max(k2, k3);
+            uint32_t s0 = min(t0, t2), s2 = max(t0, t2), s1 = min(t1, t3), s3 = max(t1, t3);
+            uint32_t m1 = min(s1, s2), m2 = max(s1, s2);
+            s1 = m1;
+            s2 = m2;
+            if (s0 == 0xFFFFFFFFu) goto pop;
+            if (s3 != 0xFFFFFFFFu) st.push(sp++, pick4(ch, s3 & 3u), __uint_as_float(s3 & ~3u));
+            if (s2 != 0xFFFFFFFFu) st.push(sp++, pick4(ch, s2 & 3u), __uint_as_float(s2 & ~3u));
+            if (s1 != 0xFFFFFFFFu) st.push(sp++, pick4(ch, s1 & 3u), __uint_as_float(s1 & ~3u));
+            cur = pick4(ch, s0 & 3u);
+            continue;
+        } else {
+            // ---- leaf: test its primitives ----
+            uint32_t first = leaf_first(cur), count = leaf_count(cur);
+            for (uint32_t k = 0; k < count; k++) {
+                const float4* pp = sc.prims + 3 * (size_t)(first + k);
+                float4 a = pp[0];
+                float t, u = 0.0f, v = 0.0f;
+                bool h;
+                if (COUNT) cnt->prims++;
+                if (!MIXED) {
+                    float4 b = pp[1], c = pp[2];
+                    h = tri_test(r, w, a, b, c, tbest, t, u, v);
+                } else {
+                    uint32_t tag = __float_as_uint(a.w);
+                    uint32_t kind = prim_kind(tag);
+                    float4 b = pp[1];
+                    if (kind == kPrimTriangle) {
+                        float4 c = pp[2];
+                        h = tri_test(r, w, a, b, c, tbest, t, u, v);
+                    } else if (kind == kPrimSphere) {
+                        h = sphere_test(r, a, b, tbest, t);
+                    } else {
+                        float4 c = pp[2];
+                        h = quad_test(r, a, b, c, sc.quads[prim_side(tag)], tbest, t, u, v);
+                    }
+                }
+                if (h) {
+                    tbest = t;
+                    best = first + k;
+                    bu = u;
+                    bv = v;
+                    if (ANY) {
+                        hit.t = t;
+                        hit.prim = best;
+                        hit.u = u;
+                        hit.v = v;
+                        return true;
+                    }
+                }
+            }
+        }
+    pop:
+        for (;;) {
+            if (sp == 0) goto done;
+            float tn;
+            st.pop(--sp, cur, tn);
+            if (tn <= tbest) break;  // entry distance beyond the current closest hit: cull
+        }
+    }
+done:
+    hit.t = best == 0xFFFFFFFFu ? __int_as_float(0x7f800000) : tbest;
+    hit.prim = best;
+    hit.u = bu;
+    hit.v = bv;
+    return best != 0xFFFFFFFFu;
+}
+
+}  // namespace prt

@@ -1,0 +1,487 @@
+"""ctypes binding of libprt.so (the C ABI in include/prt.h) and a host-side mirror of the
+reference's renderer interface for this path.
+
+Names follow the reference: ``ProgressiveRaytracer.render_pass`` mirrors
+``ProgressiveRaytracer.RenderPass`` (pkg/renderer/progressive.go:126-223),
+``get_samples_for_pass`` mirrors ``getSamplesForPass`` (progressive.go:98-123) and
+``Scene.hit`` mirrors the batched ``BVH.Hit`` (pkg/geometry/bvh.go:183-239).
+The library must be built (``python build.py``); importing this module without it
+raises immediately: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libprt.so")
+
+PRT_OK = 0
+ERRORS = {1: "invalid argument", 2: "device error", 3: "out of memory", 4: "no device",
+          5: "unsupported", 6: "cancelled"}
+
+MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_EMISSIVE, MAT_LAYERED = range(5)
+LIGHT_QUAD, LIGHT_SPHERE, LIGHT_UNIFORM_INFINITE, LIGHT_GRADIENT_INFINITE, LIGHT_ENVMAP = range(5)
+INTEGRATOR_PATH, INTEGRATOR_BDPT = 0, 1
+BUFFERS_HOST, BUFFERS_DEVICE = 0, 1
+
+
+class PrtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"prt error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+D3 = C.c_double * 3
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("outer", C.c_int32), ("inner", C.c_int32), ("pad", C.c_int32),
+                ("albedo", D3), ("fuzz", C.c_double), ("ior", C.c_double), ("emission", D3)]
+
+
+class Sphere(C.Structure):
+    _fields_ = [("center", D3), ("radius", C.c_double), ("material", C.c_int32), ("pad", C.c_int32)]
+
+
+class Quad(C.Structure):
+    _fields_ = [("corner", D3), ("u", D3), ("v", D3), ("material", C.c_int32), ("pad", C.c_int32)]
+
+
+class Box(C.Structure):
+    _fields_ = [("center", D3), ("size", D3), ("rotation", D3), ("material", C.c_int32), ("pad", C.c_int32)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("vertices", C.POINTER(C.c_double)), ("num_vertices", C.c_int64),
+                ("faces", C.POINTER(C.c_int32)), ("num_faces", C.c_int64),
+                ("face_materials", C.POINTER(C.c_int32)), ("material", C.c_int32), ("pad", C.c_int32)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_int32), ("shape", C.c_int32), ("emission", D3), ("top", D3), ("bottom", D3),
+                ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_int32), ("env_height", C.c_int32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("center", D3), ("look_at", D3), ("up", D3), ("width", C.c_int32), ("pad", C.c_int32),
+                ("aspect_ratio", C.c_double), ("vfov", C.c_double), ("aperture", C.c_double),
+                ("focus_distance", C.c_double)]
+
+
+class SceneDescC(C.Structure):
+    _fields_ = [("materials", C.POINTER(Material)), ("num_materials", C.c_int32),
+                ("spheres", C.POINTER(Sphere)), ("num_spheres", C.c_int32),
+                ("quads", C.POINTER(Quad)), ("num_quads", C.c_int32),
+                ("boxes", C.POINTER(Box)), ("num_boxes", C.c_int32),
+                ("meshes", C.POINTER(Mesh)), ("num_meshes", C.c_int32),
+                ("lights", C.POINTER(Light)), ("num_lights", C.c_int32),
+                ("light_weights", C.POINTER(C.c_double)),
+                ("camera", Camera)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("num_primitives", C.c_int64), ("num_triangles", C.c_int64), ("bvh_nodes", C.c_int64),
+                ("bvh_depth", C.c_int32), ("max_stack", C.c_int32), ("world_center", D3),
+                ("world_radius", C.c_double), ("device_bytes", C.c_int64)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("integrator", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+                ("max_depth", C.c_int32), ("rr_min_bounces", C.c_int32), ("tile_size", C.c_int32),
+                ("initial_samples", C.c_int32), ("max_samples_per_pixel", C.c_int32),
+                ("max_passes", C.c_int32), ("pad", C.c_int32), ("adaptive_min_samples", C.c_double),
+                ("adaptive_threshold", C.c_double), ("seed", C.c_uint64)]
+
+
+class PassStats(C.Structure):
+    _fields_ = [("total_pixels", C.c_int64), ("total_samples", C.c_int64), ("average_samples", C.c_double),
+                ("max_samples", C.c_int32), ("min_samples", C.c_int32), ("max_samples_used", C.c_int32),
+                ("target_samples", C.c_int32), ("device_ms", C.c_double)]
+
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<i4"), ("u", "<f4"), ("v", "<f4")])
+
+# every symbol include/prt.h and include/prt_workload.h declare
+EXPORTS = [
+    "prt_last_error", "prt_version", "prt_device_count", "prt_scene_create", "prt_scene_info_get",
+    "prt_scene_destroy", "prt_intersect_closest", "prt_intersect_any", "prt_traversal_counts",
+    "prt_samples_for_pass", "prt_film_create", "prt_render_pass", "prt_render_samples", "prt_film_read",
+    "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy",
+    "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libprt.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with go-progressive-raytracer_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "prt_last_error": (C.c_char_p, []),
+        "prt_version": (C.c_int, []),
+        "prt_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+        "prt_scene_create": (C.c_int, [C.c_int32, C.POINTER(SceneDescC), C.POINTER(vp)]),
+        "prt_scene_info_get": (C.c_int, [vp, C.POINTER(SceneInfo)]),
+        "prt_scene_destroy": (C.c_int, [vp]),
+        "prt_intersect_closest": (C.c_int, [vp, vp, C.c_int64, vp, C.c_uint32, vp]),
+        "prt_intersect_any": (C.c_int, [vp, vp, C.c_int64, vp, C.c_uint32, vp]),
+        "prt_traversal_counts": (C.c_int, [vp, vp, C.c_int64, vp, vp, vp, C.c_uint32, vp]),
+        "prt_samples_for_pass": (C.c_int, [C.POINTER(RenderParams), C.c_int32, C.POINTER(C.c_int32)]),
+        "prt_film_create": (C.c_int, [vp, C.POINTER(RenderParams), C.POINTER(vp)]),
+        "prt_render_pass": (C.c_int, [vp, C.c_int32, C.POINTER(PassStats)]),
+        "prt_render_samples": (C.c_int, [vp, C.c_int32, C.POINTER(PassStats)]),
+        "prt_film_read": (C.c_int, [vp, vp, vp, vp, vp]),
+        "prt_film_read_rgba8": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]),
+        "prt_film_reset": (C.c_int, [vp]),
+        "prt_film_destroy": (C.c_int, [vp]),
+        "prt_workload_icosphere": (C.c_int, [C.c_int32, C.c_double, vp, C.c_double, C.c_uint64, vp,
+                                             C.POINTER(C.c_int64), vp, C.POINTER(C.c_int64)]),
+        "prt_workload_rays": (C.c_int, [vp, vp, C.c_double, C.c_int64, C.c_uint64, vp]),
+        "prt_workload_sky": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.c_double, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != PRT_OK:
+        raise PrtError(rc, lib().prt_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def device_count():
+    n = C.c_int32(0)
+    check(lib().prt_device_count(C.byref(n)))
+    return n.value
+
+
+# ---------------------------------------------------------------------------------------
+class SceneDesc:
+    """Flattened scene (materials, shapes, lights, camera) in the reference's float64 terms.
+
+    Mirrors the pieces of ``scene.Scene`` (pkg/scene/scene.go:11-19) the path needs; the
+    ``add_*`` helpers follow Scene.AddSphereLight / AddQuadLight / AddGradientInfiniteLight
+    (scene.go:102-141).
+    """
+
+    def __init__(self):
+        self.materials = []
+        self.spheres = []
+        self.quads = []
+        self.boxes = []
+        self.meshes = []  # (vertices float64 (n,3), faces int32 (m,3), material, face_materials|None)
+        self.lights = []
+        self.light_weights = None
+        self.camera = None
+
+    # materials (pkg/material constructors)
+    def _mat(self, **kw):
+        m = Material()
+        m.type = kw.get("type", MAT_LAMBERTIAN)
+        m.outer, m.inner = kw.get("outer", -1), kw.get("inner", -1)
+        m.albedo = D3(*kw.get("albedo", (0, 0, 0)))
+        m.fuzz = kw.get("fuzz", 0.0)
+        m.ior = kw.get("ior", 1.0)
+        m.emission = D3(*kw.get("emission", (0, 0, 0)))
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def lambertian(self, albedo):
+        return self._mat(type=MAT_LAMBERTIAN, albedo=albedo)
+
+    def metal(self, albedo, fuzz):
+        return self._mat(type=MAT_METAL, albedo=albedo, fuzz=min(1.0, max(0.0, fuzz)))
+
+    def dielectric(self, ior):
+        return self._mat(type=MAT_DIELECTRIC, ior=ior)
+
+    def emissive(self, emission):
+        return self._mat(type=MAT_EMISSIVE, emission=emission)
+
+    def layered(self, outer, inner):
+        return self._mat(type=MAT_LAYERED, outer=outer, inner=inner)
+
+    # shapes
+    def sphere(self, center, radius, material):
+        s = Sphere()
+        s.center, s.radius, s.material = D3(*center), radius, material
+        self.spheres.append(s)
+        return len(self.spheres) - 1
+
+    def quad(self, corner, u, v, material):
+        q = Quad()
+        q.corner, q.u, q.v, q.material = D3(*corner), D3(*u), D3(*v), material
+        self.quads.append(q)
+        return len(self.quads) - 1
+
+    def box(self, center, size, rotation, material):
+        b = Box()
+        b.center, b.size, b.rotation, b.material = D3(*center), D3(*size), D3(*rotation), material
+        self.boxes.append(b)
+        return len(self.boxes) - 1
+
+    def mesh(self, vertices, faces, material, face_materials=None):
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+        fm = None if face_materials is None else np.ascontiguousarray(face_materials, dtype=np.int32)
+        self.meshes.append((v, f, material, fm))
+        return len(self.meshes) - 1
+
+    # lights
+    def add_sphere_light(self, center, radius, emission):
+        m = self.emissive(emission)
+        s = self.sphere(center, radius, m)
+        L = Light()
+        L.type, L.shape = LIGHT_SPHERE, s
+        self.lights.append(L)
+
+    def add_quad_light(self, corner, u, v, emission):
+        m = self.emissive(emission)
+        q = self.quad(corner, u, v, m)
+        L = Light()
+        L.type, L.shape = LIGHT_QUAD, q
+        self.lights.append(L)
+
+    def add_uniform_infinite_light(self, emission):
+        L = Light()
+        L.type, L.shape, L.emission = LIGHT_UNIFORM_INFINITE, -1, D3(*emission)
+        self.lights.append(L)
+
+    def add_gradient_infinite_light(self, top, bottom):
+        L = Light()
+        L.type, L.shape, L.top, L.bottom = LIGHT_GRADIENT_INFINITE, -1, D3(*top), D3(*bottom)
+        self.lights.append(L)
+
+    def add_envmap_light(self, rgb):
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        assert rgb.ndim == 3 and rgb.shape[2] == 3
+        L = Light()
+        L.type, L.shape = LIGHT_ENVMAP, -1
+        L.env_width, L.env_height = rgb.shape[1], rgb.shape[0]
+        L._rgb = rgb
+        self.lights.append(L)
+
+    def set_camera(self, center, look_at, up, width, aspect_ratio, vfov, aperture=0.0, focus_distance=0.0):
+        c = Camera()
+        c.center, c.look_at, c.up = D3(*center), D3(*look_at), D3(*up)
+        c.width, c.aspect_ratio, c.vfov = width, aspect_ratio, vfov
+        c.aperture, c.focus_distance = aperture, focus_distance
+        self.camera = c
+
+    @property
+    def image_size(self):
+        w = self.camera.width
+        return w, int(w / self.camera.aspect_ratio)
+
+    def to_c(self):
+        """Returns (prt_scene_desc, keepalive) — keep the second value alive while the first is used."""
+        keep = []
+
+        def arr(ctype, items):
+            if not items:
+                return None
+            a = (ctype * len(items))(*items)
+            keep.append(a)
+            return C.cast(a, C.POINTER(ctype))
+
+        meshes = []
+        for v, f, m, fm in self.meshes:
+            M = Mesh()
+            M.vertices = v.ctypes.data_as(C.POINTER(C.c_double))
+            M.num_vertices = v.shape[0]
+            M.faces = f.ctypes.data_as(C.POINTER(C.c_int32))
+            M.num_faces = f.shape[0]
+            M.face_materials = fm.ctypes.data_as(C.POINTER(C.c_int32)) if fm is not None else None
+            M.material = m
+            keep += [v, f, fm]
+            meshes.append(M)
+        for L in self.lights:
+            if L.type == LIGHT_ENVMAP:
+                L.env_rgb = L._rgb.ctypes.data_as(C.POINTER(C.c_float))
+                keep.append(L._rgb)
+        d = SceneDescC()
+        d.materials, d.num_materials = arr(Material, self.materials), len(self.materials)
+        d.spheres, d.num_spheres = arr(Sphere, self.spheres), len(self.spheres)
+        d.quads, d.num_quads = arr(Quad, self.quads), len(self.quads)
+        d.boxes, d.num_boxes = arr(Box, self.boxes), len(self.boxes)
+        d.meshes, d.num_meshes = arr(Mesh, meshes), len(meshes)
+        d.lights, d.num_lights = arr(Light, self.lights), len(self.lights)
+        if self.light_weights is not None:
+            w = np.ascontiguousarray(self.light_weights, dtype=np.float64)
+            keep.append(w)
+            d.light_weights = w.ctypes.data_as(C.POINTER(C.c_double))
+        d.camera = self.camera if self.camera is not None else Camera()
+        return d, keep
+
+
+def sampling_params(width, height, max_depth, rr_min_bounces, max_samples_per_pixel, max_passes=1,
+                    initial_samples=1, seed=0, integrator=INTEGRATOR_PATH, tile_size=64,
+                    adaptive_min_samples=1.0, adaptive_threshold=0.0):
+    """RenderParams = SamplingConfig (scene.go:22-30) + ProgressiveConfig (progressive.go:29-46)."""
+    p = RenderParams()
+    p.integrator, p.width, p.height = integrator, width, height
+    p.max_depth, p.rr_min_bounces, p.tile_size = max_depth, rr_min_bounces, tile_size
+    p.initial_samples, p.max_samples_per_pixel, p.max_passes = initial_samples, max_samples_per_pixel, max_passes
+    p.adaptive_min_samples, p.adaptive_threshold, p.seed = adaptive_min_samples, adaptive_threshold, seed
+    return p
+
+
+def get_samples_for_pass(params, pass_number):
+    t = C.c_int32(0)
+    check(lib().prt_samples_for_pass(C.byref(params), pass_number, C.byref(t)))
+    return t.value
+
+
+# ---------------------------------------------------------------------------------------
+class Scene:
+    """A scene resident on one GPU (prt_scene_create): BVH + primitives + materials + lights."""
+
+    def __init__(self, desc, device=0):
+        self.desc = desc
+        d, keep = desc.to_c()
+        h = C.c_void_p()
+        check(lib().prt_scene_create(device, C.byref(d), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().prt_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = SceneInfo()
+        check(lib().prt_scene_info_get(self.handle, C.byref(i)))
+        return i
+
+    def intersect_closest(self, rays):
+        """rays: structured array of RAY_DTYPE (host). Returns HIT_DTYPE array."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.empty(rays.shape[0], dtype=HIT_DTYPE)
+        check(lib().prt_intersect_closest(self.handle, _ptr(rays), rays.shape[0], _ptr(hits), BUFFERS_HOST, None))
+        return hits
+
+    def intersect_closest_device(self, rays_ptr, n, hits_ptr, stream=None):
+        check(lib().prt_intersect_closest(self.handle, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr),
+                                          BUFFERS_DEVICE, C.c_void_p(stream) if stream else None))
+
+    def intersect_any(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        occ = np.empty(rays.shape[0], dtype=np.uint8)
+        check(lib().prt_intersect_any(self.handle, _ptr(rays), rays.shape[0], _ptr(occ), BUFFERS_HOST, None))
+        return occ
+
+    def traversal_counts(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        n = rays.shape[0]
+        nodes, boxes, prims = (np.empty(n, np.uint32) for _ in range(3))
+        check(lib().prt_traversal_counts(self.handle, _ptr(rays), n, _ptr(nodes), _ptr(boxes), _ptr(prims),
+                                         BUFFERS_HOST, None))
+        return nodes, boxes, prims
+
+
+class ProgressiveRaytracer:
+    """Mirror of renderer.ProgressiveRaytracer (pkg/renderer/progressive.go:49-96) on one GPU."""
+
+    def __init__(self, scene, params):
+        self.scene = scene
+        self.params = params
+        h = C.c_void_p()
+        check(lib().prt_film_create(scene.handle, C.byref(params), C.byref(h)))
+        self.handle = h
+        self.width, self.height = scene.desc.image_size
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().prt_film_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_samples_for_pass(self, pass_number):
+        return get_samples_for_pass(self.params, pass_number)
+
+    def render_pass(self, pass_number):
+        """RenderPass: returns (rgba uint8 image (H,W,4), PassStats)."""
+        st = PassStats()
+        check(lib().prt_render_pass(self.handle, pass_number, C.byref(st)))
+        return self.image(), st
+
+    def render_samples(self, spp):
+        st = PassStats()
+        check(lib().prt_render_samples(self.handle, spp, C.byref(st)))
+        return st
+
+    def reset(self):
+        check(lib().prt_film_reset(self.handle))
+
+    def read(self):
+        """Per-pixel mean RGB (H,W,3), luminance sum, luminance^2 sum, counts."""
+        n = self.width * self.height
+        mean = np.empty((self.height, self.width, 3), np.float32)
+        ls = np.empty((self.height, self.width), np.float32)
+        lq = np.empty((self.height, self.width), np.float32)
+        cnt = np.empty((self.height, self.width), np.uint32)
+        check(lib().prt_film_read(self.handle, _ptr(mean), _ptr(ls), _ptr(lq), _ptr(cnt)))
+        assert mean.size == 3 * n
+        return mean, ls, lq, cnt
+
+    def image(self, x0=0, y0=0, x1=None, y1=None):
+        x1 = self.width if x1 is None else x1
+        y1 = self.height if y1 is None else y1
+        out = np.empty((y1 - y0, x1 - x0, 4), np.uint8)
+        check(lib().prt_film_read_rgba8(self.handle, x0, y0, x1, y1, _ptr(out)))
+        return out
+
+
+# ---------------------------------------------------------------------------------------
+def icosphere(frequency, radius=1.0, center=(0.0, 0.0, 0.0), amplitude=0.1, seed=0):
+    """Displaced geodesic icosphere: (vertices float64 (n,3), faces int32 (m,3))."""
+    nv, nf = C.c_int64(0), C.c_int64(0)
+    c = np.asarray(center, np.float64)
+    check(lib().prt_workload_icosphere(frequency, radius, _ptr(c), amplitude, seed, None, C.byref(nv), None,
+                                       C.byref(nf)))
+    v = np.empty((nv.value, 3), np.float64)
+    f = np.empty((nf.value, 3), np.int32)
+    check(lib().prt_workload_icosphere(frequency, radius, _ptr(c), amplitude, seed, _ptr(v), C.byref(nv), _ptr(f),
+                                       C.byref(nf)))
+    return v, f
+
+
+def random_rays(lo, hi, n, scale=2.0, seed=0):
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    rays = np.empty(n, RAY_DTYPE)
+    check(lib().prt_workload_rays(_ptr(lo), _ptr(hi), scale, n, seed, _ptr(rays)))
+    return rays
+
+
+def hdr_sky(width=1024, height=512, seed=0, sun_peak=1.0e4):
+    rgb = np.empty((height, width, 3), np.float32)
+    check(lib().prt_workload_sky(width, height, seed, sun_peak, _ptr(rgb)))
+    return rgb

@@ -1,0 +1,130 @@
+"""Scene constructors of the benchmark configurations, mirroring pkg/scene.
+
+Each returns (SceneDesc, dict of sampling settings). Coordinates, materials and lights follow
+the reference constructors cited in each docstring.
+"""
+import math
+
+import numpy as np
+
+import prtgpu as P
+
+
+def default_scene(pinhole=True, width=400):
+    """NewDefaultScene (pkg/scene/default_scene.go:11-95).
+
+    pinhole=True forces aperture 0: MergeCameraConfig only applies non-zero overrides
+    (pkg/geometry/camera.go:341), so the reference cannot zero it through an override.
+    """
+    s = P.SceneDesc()
+    s.set_camera(center=(0, 0.75, 2), look_at=(0, 0.5, -1), up=(0, 1, 0), width=width,
+                 aspect_ratio=16.0 / 9.0, vfov=40.0, aperture=0.0 if pinhole else 0.05, focus_distance=0.0)
+    green = s.lambertian((0.8 * 0.6, 0.8 * 0.6, 0.0))
+    blue = s.lambertian((0.1, 0.2, 0.5))
+    red = s.lambertian((0.65, 0.25, 0.2))
+    silver = s.metal((0.8, 0.8, 0.8), 0.0)
+    gold = s.metal((0.8, 0.6, 0.2), 0.3)
+    glass = s.dielectric(1.5)
+    coated = s.layered(glass, red)
+    # AddSphereLight first (default_scene.go:78-82), then the shapes in the reference order
+    s.add_sphere_light((30, 30.5, 15), 10, (15.0, 14.0, 13.0))
+    s.sphere((0, 0.5, -1), 0.5, coated)
+    s.sphere((-1, 0.5, -1), 0.5, silver)
+    s.sphere((1, 0.5, -1), 0.5, gold)
+    ground_quad(s, (0, 0, 0), 10000.0, green)
+    s.sphere((0.5, 0.25, -0.5), 0.25, glass)
+    s.sphere((-0.5, 0.25, -0.5), 0.25, glass)
+    s.sphere((-0.5, 0.25, -0.5), -0.24, glass)
+    s.sphere((-0.5, 0.25, -0.5), 0.20, blue)
+    s.add_gradient_infinite_light((0.5, 0.7, 1.0), (1.0, 1.0, 1.0))
+    cfg = dict(max_depth=50, rr_min_bounces=20, spp=64)
+    return s, cfg
+
+
+def ground_quad(s, center, size, material):
+    """NewGroundQuad (pkg/scene/scene.go:33-42)."""
+    corner = (center[0] - size / 2, center[1], center[2] - size / 2)
+    return s.quad(corner, (size, 0, 0), (0, 0, size), material)
+
+
+def cornell_scene(scale=1.0, width=400, light="reference", tall_angle_deg=-20.0, short_angle_deg=18.0):
+    """NewCornellScene(CornellBoxes, CornellQuadLight) (pkg/scene/cornell.go:32-257), uniformly
+    scaled by `scale`. light="reference": 130x105 quad, emission (18,15,8)*2.5;
+    light="metre": radiance 15 on the same (scaled) quad."""
+    k = scale
+    s = P.SceneDesc()
+    s.set_camera(center=(278 * k, 278 * k, -800 * k), look_at=(278 * k, 278 * k, 0), up=(0, 1, 0), width=width,
+                 aspect_ratio=1.0, vfov=40.0, aperture=0.0, focus_distance=0.0)
+    white = s.lambertian((0.73, 0.73, 0.73))
+    red = s.lambertian((0.65, 0.05, 0.05))
+    green = s.lambertian((0.12, 0.45, 0.15))
+    s.quad((0, 0, 0), (556 * k, 0, 0), (0, 0, 556 * k), white)          # floor
+    s.quad((0, 556 * k, 0), (556 * k, 0, 0), (0, 0, 556 * k), white)    # ceiling
+    s.quad((0, 0, 556 * k), (556 * k, 0, 0), (0, 556 * k, 0), white)    # back wall
+    s.quad((0, 0, 0), (0, 0, 556 * k), (0, 556 * k, 0), red)            # left wall
+    s.quad((556 * k, 0, 0), (0, 0, 556 * k), (0, 556 * k, 0), green)    # right wall
+    emission = (18.0 * 2.5, 15.0 * 2.5, 8.0 * 2.5) if light == "reference" else (15.0, 15.0, 15.0)
+    s.add_quad_light((213 * k, (556 - 0.001) * k, 227 * k), (130 * k, 0, 0), (0, 0, 105 * k), emission)
+    white_box = s.lambertian((0.73, 0.73, 0.73))
+    mirror = s.metal((0.9, 0.9, 0.9), 0.0)
+    s.box((370 * k, 82.5 * k, 169 * k), (82.5 * k, 82.5 * k, 82.5 * k), (0, short_angle_deg * math.pi / 180, 0),
+          white_box)
+    s.box((185 * k, 165 * k, 351 * k), (82.5 * k, 165 * k, 82.5 * k), (0, tall_angle_deg * math.pi / 180, 0), mirror)
+    cfg = dict(max_depth=5, rr_min_bounces=16, spp=150)
+    return s, cfg
+
+
+def cornell_metre(width=512):
+    """Metre-scale Cornell box with two boxes rotated +-18 degrees, radiance-15 quad light,
+    512x512, PT depth 8, 1024 spp (x0.001 of cornell.go's geometry)."""
+    s, _ = cornell_scene(scale=0.001, width=width, light="metre", tall_angle_deg=-18.0, short_angle_deg=18.0)
+    return s, dict(max_depth=8, rr_min_bounces=16, spp=1024)
+
+
+def mesh_scene(frequency=256, seed=0, width=512):
+    """Closest-hit workload: one displaced icosphere (20*f^2 triangles) at the origin."""
+    v, f = P.icosphere(frequency, radius=1.0, center=(0, 0, 0), amplitude=0.1, seed=seed)
+    s = P.SceneDesc()
+    m = s.lambertian((0.7, 0.7, 0.7))
+    s.mesh(v, f, m)
+    s.set_camera(center=(0, 0, 4), look_at=(0, 0, 0), up=(0, 1, 0), width=width, aspect_ratio=1.0, vfov=40.0)
+    return s, dict(vertices=v, faces=f)
+
+
+def mesh_sky_scene(frequency=300, seed=0, width=1024, sky=None):
+    """1.8M-triangle displaced glass icosphere (IOR 1.5) over a Lambertian ground, lit by a
+    1024x512 HDR sky environment light; 1024x1024, PT depth 16, 256 spp."""
+    v, f = P.icosphere(frequency, radius=1.0, center=(0, 1.15, 0), amplitude=0.1, seed=seed)
+    s = P.SceneDesc()
+    glass = s.dielectric(1.5)
+    ground = s.lambertian((0.5, 0.5, 0.5))
+    s.mesh(v, f, glass)
+    ground_quad(s, (0, 0, 0), 20.0, ground)
+    if sky is None:
+        sky = P.hdr_sky(1024, 512, seed=seed, sun_peak=1.0e4)
+    s.add_envmap_light(sky)
+    s.set_camera(center=(0, 1.6, 4.5), look_at=(0, 1.0, 0), up=(0, 1, 0), width=width, aspect_ratio=1.0,
+                 vfov=40.0)
+    return s, dict(max_depth=16, rr_min_bounces=5, spp=256)
+
+
+def empty_uniform_scene(width=32, emission=(1.0, 1.0, 1.0)):
+    """No geometry, uniform infinite light: every pixel equals the emission exactly
+    (progressive_integration_test.go:44-70)."""
+    s = P.SceneDesc()
+    s.add_uniform_infinite_light(emission)
+    s.set_camera(center=(0, 0, 0), look_at=(0, 0, -1), up=(0, 1, 0), width=width, aspect_ratio=1.0, vfov=45.0)
+    return s, dict(max_depth=5, rr_min_bounces=5, spp=4)
+
+
+def furnace_scene(albedo=0.5, width=32):
+    """White furnace: a Lambertian sphere of albedo a under a uniform sky of radiance 1. With a
+    single infinite light the light and BSDF pdfs are both cos/pi, so the reference estimator
+    weights direct and indirect lighting by 1/2 each; every bounce off the convex sphere escapes,
+    so each sample on the sphere is exactly a and each sky sample exactly 1."""
+    s = P.SceneDesc()
+    m = s.lambertian((albedo, albedo, albedo))
+    s.sphere((0, 0, -3), 1.0, m)
+    s.add_uniform_infinite_light((1.0, 1.0, 1.0))
+    s.set_camera(center=(0, 0, 0), look_at=(0, 0, -1), up=(0, 1, 0), width=width, aspect_ratio=1.0, vfov=60.0)
+    return s, dict(max_depth=50, rr_min_bounces=50, spp=16)

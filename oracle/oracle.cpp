@@ -1,0 +1,1477 @@
+// oracle.cpp — float64 CPU restatement of the reference path-tracing hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP library and the timed
+// CPU baseline ("port"); nothing in the product links, loads or calls it. Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it.
+//
+// Every routine follows the reference function cited beside it (paths relative to the
+// df07/go-progressive-raytracer tree): same branch structure, epsilons and estimator
+// quirks, float64 throughout (pkg/core/vec3.go:10). Randomness comes from the counter
+// stream of oracle_rng.h instead of Go's math/rand (which cannot be run here).
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../include/prt.h"
+#include "oracle_rng.h"
+
+namespace orc {
+
+static const double kPi = 3.14159265358979323846;
+static const double kInf = INFINITY;
+
+// ---- pkg/core/vec3.go -----------------------------------------------------------
+struct Vec3 {
+    double x = 0, y = 0, z = 0;
+    Vec3() {}
+    Vec3(double a, double b, double c) : x(a), y(b), z(c) {}
+    Vec3 operator+(const Vec3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    Vec3 operator-(const Vec3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    Vec3 operator*(double s) const { return {x * s, y * s, z * s}; }
+    Vec3 mul(const Vec3& o) const { return {x * o.x, y * o.y, z * o.z}; }  // MultiplyVec :113
+    double dot(const Vec3& o) const { return x * o.x + y * o.y + z * o.z; }
+    Vec3 cross(const Vec3& o) const {                                        // Cross :103
+        return {y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x};
+    }
+    double length() const { return std::sqrt(x * x + y * y + z * z); }
+    double length2() const { return x * x + y * y + z * z; }
+    Vec3 normalize() const {  // Normalize :93-99 (zero stays zero)
+        double l = length();
+        if (l == 0) return {0, 0, 0};
+        return {x / l, y / l, z / l};
+    }
+    Vec3 neg() const { return {-x, -y, -z}; }
+    double luminance() const { return 0.2126 * x + 0.7152 * y + 0.0722 * z; }  // :125
+    bool is_zero() const { return x == 0 && y == 0 && z == 0; }
+    double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+static Vec3 V(const double* p) { return {p[0], p[1], p[2]}; }
+struct Vec2 {
+    double x = 0, y = 0;
+};
+struct Ray {
+    Vec3 o, d;
+    Vec3 at(double t) const { return o + d * t; }  // Ray.At vec3.go:216
+};
+
+// Rotate X->Y->Z, pkg/core/vec3.go:162-198
+static Vec3 rotate(Vec3 v, Vec3 r) {
+    if (r.x == 0 && r.y == 0 && r.z == 0) return v;
+    Vec3 res = v;
+    if (r.x != 0) {
+        double c = std::cos(r.x), s = std::sin(r.x);
+        res = Vec3(res.x, res.y * c - res.z * s, res.y * s + res.z * c);
+    }
+    if (r.y != 0) {
+        double c = std::cos(r.y), s = std::sin(r.y);
+        res = Vec3(res.x * c + res.z * s, res.y, -res.x * s + res.z * c);
+    }
+    if (r.z != 0) {
+        double c = std::cos(r.z), s = std::sin(r.z);
+        res = Vec3(res.x * c - res.y * s, res.x * s + res.y * c, res.z);
+    }
+    return res;
+}
+
+// ---- Sampler over the counter stream (interface of pkg/core/sampling.go:10-14) ----
+struct Sampler {
+    uint32_t key = 0;
+    uint32_t slot = 0;
+    uint32_t k = 0;
+    void set_slot(uint32_t s) { slot = s; k = 0; }
+    double get1d() { return draw(key, slot * 16u + (k++)); }
+    Vec2 get2d() { Vec2 r; r.x = get1d(); r.y = get1d(); return r; }
+    Vec3 get3d() { Vec3 r; r.x = get1d(); r.y = get1d(); r.z = get1d(); return r; }
+};
+
+// ---- warps, pkg/core/sampling.go -----------------------------------------------
+static Vec3 sample_cosine_hemisphere(Vec3 n, Vec2 s) {  // :42-67
+    double a = 2.0 * kPi * s.x;
+    double z = s.y;
+    double r = std::sqrt(z);
+    double x = r * std::cos(a);
+    double y = r * std::sin(a);
+    double zc = std::sqrt(1.0 - z);
+    Vec3 nt = std::fabs(n.x) > 0.1 ? Vec3(0, 1, 0) : Vec3(1, 0, 0);
+    Vec3 t = nt.cross(n).normalize();
+    Vec3 b = n.cross(t);
+    return t * x + b * y + n * zc;
+}
+static Vec3 sample_on_unit_sphere(Vec2 s) {  // :97-104
+    double z = 1.0 - 2.0 * s.x;
+    double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+    double phi = 2.0 * kPi * s.y;
+    return {r * std::cos(phi), r * std::sin(phi), z};
+}
+static Vec3 sample_point_in_unit_disk(Vec2 s) {  // :108-126
+    double ox = 2 * s.x - 1, oy = 2 * s.y - 1;
+    if (ox == 0 && oy == 0) return {0, 0, 0};
+    double theta, r;
+    if (std::fabs(ox) > std::fabs(oy)) {
+        r = ox;
+        theta = kPi / 4 * (oy / ox);
+    } else {
+        r = oy;
+        theta = kPi / 2 - kPi / 4 * (ox / oy);
+    }
+    return {r * std::cos(theta), r * std::sin(theta), 0};
+}
+static Vec3 sample_point_in_unit_sphere(Vec3 s) {  // :130-147
+    double r = std::pow(s.x, 1.0 / 3.0);
+    double phi = 2 * kPi * s.y;
+    double ct = 2 * s.z - 1;
+    double st = std::sqrt(1 - ct * ct);
+    return {r * st * std::cos(phi), r * st * std::sin(phi), r * ct};
+}
+
+// ---- AABB, pkg/geometry/aabb.go -------------------------------------------------
+struct AABB {
+    Vec3 mn, mx;
+    bool hit(const Ray& ray, double tmin, double tmax) const {  // :43-95
+        for (int a = 0; a < 3; a++) {
+            double lo = mn[a], hi = mx[a], o = ray.o[a], d = ray.d[a];
+            if (std::fabs(d) < 1e-8) {
+                if (o < lo || o > hi) return false;
+                continue;
+            }
+            double inv = 1.0 / d;
+            double t1 = (lo - o) * inv, t2 = (hi - o) * inv;
+            if (t1 > t2) std::swap(t1, t2);
+            tmin = std::max(tmin, t1);
+            tmax = std::min(tmax, t2);
+            if (tmin > tmax) return false;
+        }
+        return true;
+    }
+    AABB uni(const AABB& o) const {  // Union :98-111
+        return {{std::min(mn.x, o.mn.x), std::min(mn.y, o.mn.y), std::min(mn.z, o.mn.z)},
+                {std::max(mx.x, o.mx.x), std::max(mx.y, o.mx.y), std::max(mx.z, o.mx.z)}};
+    }
+    Vec3 center() const { return (mn + mx) * 0.5; }
+    int longest_axis() const {  // :126-135
+        Vec3 s = mx - mn;
+        if (s.x > s.y && s.x > s.z) return 0;
+        if (s.y > s.z) return 1;
+        return 2;
+    }
+};
+static AABB aabb_from_points(const Vec3* p, int n) {  // NewAABBFromPoints :23-40
+    AABB b{p[0], p[0]};
+    for (int i = 1; i < n; i++) {
+        b.mn = {std::min(b.mn.x, p[i].x), std::min(b.mn.y, p[i].y), std::min(b.mn.z, p[i].z)};
+        b.mx = {std::max(b.mx.x, p[i].x), std::max(b.mx.y, p[i].y), std::max(b.mx.z, p[i].z)};
+    }
+    return b;
+}
+
+// ---- hit record, pkg/material/interfaces.go:48-65 -------------------------------
+struct Hit {
+    Vec3 p, n;
+    double t = 0;
+    bool front = false;
+    int mat = -1;
+    Vec2 uv;
+    int64_t prim = -1;
+    void set_face_normal(const Ray& r, Vec3 outward) {
+        front = r.d.dot(outward) < 0;
+        n = front ? outward : outward * -1.0;
+    }
+};
+
+struct Counters {
+    uint64_t box = 0, prim = 0;
+};
+
+struct Shape {
+    virtual ~Shape() {}
+    virtual bool hit(const Ray& r, double tmin, double tmax, Hit& h, Counters* c) const = 0;
+    virtual AABB bbox() const = 0;
+};
+
+// ---- Triangle, pkg/geometry/triangle.go:92-175 ------------------------------------
+struct Triangle : Shape {
+    Vec3 v0, v1, v2, normal;
+    AABB box;
+    int mat;
+    int64_t id;
+    Triangle(Vec3 a, Vec3 b, Vec3 c, int m, int64_t i) : v0(a), v1(b), v2(c), mat(m), id(i) {
+        normal = (v1 - v0).cross(v2 - v0).normalize();  // computeNormal :92-96
+        Vec3 p[3] = {v0, v1, v2};
+        box = aabb_from_points(p, 3);
+    }
+    bool hit(const Ray& r, double tmin, double tmax, Hit& h, Counters* c) const override {
+        if (c) c->prim++;
+        const double eps = 1e-8;
+        Vec3 e1 = v1 - v0, e2 = v2 - v0;
+        Vec3 hv = r.d.cross(e2);
+        double a = e1.dot(hv);
+        if (a > -eps && a < eps) return false;
+        double f = 1.0 / a;
+        Vec3 s = r.o - v0;
+        double u = f * s.dot(hv);
+        if (u < 0.0 || u > 1.0) return false;
+        Vec3 q = s.cross(e1);
+        double v = f * r.d.dot(q);
+        if (v < 0.0 || u + v > 1.0) return false;
+        double t = f * e2.dot(q);
+        if (t < tmin || t > tmax) return false;
+        h.t = t;
+        h.p = r.at(t);
+        h.mat = mat;
+        h.uv = {u, v};
+        h.prim = id;
+        h.set_face_normal(r, normal);
+        return true;
+    }
+    AABB bbox() const override { return box; }
+};
+
+// ---- Sphere, pkg/geometry/sphere.go:27-90 -----------------------------------------
+struct Sphere : Shape {
+    Vec3 c;
+    double r;
+    int mat;
+    int64_t id;
+    Sphere(Vec3 cc, double rr, int m, int64_t i) : c(cc), r(rr), mat(m), id(i) {}
+    bool hit(const Ray& ray, double tmin, double tmax, Hit& h, Counters* cnt) const override {
+        if (cnt) cnt->prim++;
+        Vec3 oc = ray.o - c;
+        double a = ray.d.dot(ray.d);
+        double hb = oc.dot(ray.d);
+        double cc = oc.dot(oc) - r * r;
+        double disc = hb * hb - a * cc;
+        if (disc < 0) return false;
+        double sq = std::sqrt(disc);
+        double root = (-hb - sq) / a;
+        if (root < tmin || root > tmax) {
+            root = (-hb + sq) / a;
+            if (root < tmin || root > tmax) return false;
+        }
+        h.t = root;
+        h.p = ray.at(root);
+        Vec3 outward = (h.p - c) * (1.0 / r);
+        double theta = std::acos(-outward.y);
+        double phi = std::atan2(-outward.z, outward.x) + kPi;
+        h.uv = {phi / (2.0 * kPi), theta / kPi};
+        h.mat = mat;
+        h.prim = id;
+        h.set_face_normal(ray, outward);
+        return true;
+    }
+    AABB bbox() const override {  // :83-90 (a negative radius yields an inverted box, as there)
+        Vec3 rv(r, r, r);
+        return {c - rv, c + rv};
+    }
+};
+
+// ---- Quad, pkg/geometry/quad.go ----------------------------------------------------
+struct Quad : Shape {
+    Vec3 corner, u, v, normal, w;
+    double D;
+    int mat;
+    int64_t id;
+    Quad(Vec3 c, Vec3 uu, Vec3 vv, int m, int64_t i) : corner(c), u(uu), v(vv), mat(m), id(i) {
+        normal = u.cross(v).normalize();  // NewQuad :98-121
+        D = normal.dot(corner);
+        Vec3 cr = u.cross(v);
+        w = normal * (1.0 / normal.dot(cr));
+    }
+    bool hit(const Ray& r, double tmin, double tmax, Hit& h, Counters* c) const override {
+        if (c) c->prim++;
+        double den = r.d.dot(normal);  // :133-180
+        if (std::fabs(den) < 1e-8) return false;
+        double t = (D - r.o.dot(normal)) / den;
+        if (t < tmin || t > tmax) return false;
+        Vec3 p = r.at(t);
+        Vec3 hv = p - corner;
+        double alpha = w.dot(hv.cross(v));
+        double beta = w.dot(u.cross(hv));
+        if (alpha < 0 || alpha > 1 || beta < 0 || beta > 1) return false;
+        h.t = t;
+        h.p = p;
+        h.uv = {alpha, beta};
+        h.mat = mat;
+        h.prim = id;
+        h.set_face_normal(r, normal);
+        return true;
+    }
+    AABB bbox() const override {  // BoundingBox :183-210 with the +-0.001 slab (:44-77)
+        Vec3 cs[4] = {corner, corner + u, corner + v, corner + u + v};
+        const double thr = 0.9999, tol = 0.0001, e = 0.001;
+        Vec3 n = normal;
+        int al = -1;
+        if (std::fabs(n.x) > thr && std::fabs(n.y) < tol && std::fabs(n.z) < tol) al = 0;
+        else if (std::fabs(n.y) > thr && std::fabs(n.x) < tol && std::fabs(n.z) < tol) al = 1;
+        else if (std::fabs(n.z) > thr && std::fabs(n.x) < tol && std::fabs(n.y) < tol) al = 2;
+        AABB b = aabb_from_points(cs, 4);
+        if (al >= 0) {
+            double fixed = cs[0][al];
+            double lo[3] = {b.mn.x, b.mn.y, b.mn.z}, hi[3] = {b.mx.x, b.mx.y, b.mx.z};
+            lo[al] = fixed - e;
+            hi[al] = fixed + e;
+            b.mn = {lo[0], lo[1], lo[2]};
+            b.mx = {hi[0], hi[1], hi[2]};
+        }
+        return b;
+    }
+};
+
+// ---- Box, pkg/geometry/box.go:34-140 ----------------------------------------------
+struct Box : Shape {
+    std::unique_ptr<Quad> faces[6];
+    AABB box;
+    Box(Vec3 center, Vec3 size, Vec3 rot, int m, int64_t first_id) {
+        Vec3 c[8] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                     {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+        for (auto& p : c) {
+            p = Vec3(p.x * size.x, p.y * size.y, p.z * size.z);
+            p = rotate(p, rot);
+            p = p + center;
+        }
+        faces[0].reset(new Quad(c[4], c[5] - c[4], c[7] - c[4], m, first_id + 0));
+        faces[1].reset(new Quad(c[1], c[0] - c[1], c[2] - c[1], m, first_id + 1));
+        faces[2].reset(new Quad(c[5], c[1] - c[5], c[6] - c[5], m, first_id + 2));
+        faces[3].reset(new Quad(c[0], c[4] - c[0], c[3] - c[0], m, first_id + 3));
+        faces[4].reset(new Quad(c[3], c[7] - c[3], c[2] - c[3], m, first_id + 4));
+        faces[5].reset(new Quad(c[4], c[0] - c[4], c[5] - c[4], m, first_id + 5));
+        box = aabb_from_points(c, 8);
+    }
+    bool hit(const Ray& r, double tmin, double tmax, Hit& h, Counters* cnt) const override {
+        bool any = false;
+        double closest = tmax;
+        for (auto& f : faces) {
+            Hit fh;
+            if (f->hit(r, tmin, closest, fh, cnt)) {
+                if (fh.t < closest) {  // strict, as box.go:131
+                    closest = fh.t;
+                    h = fh;
+                    any = true;
+                }
+            }
+        }
+        return any;
+    }
+    AABB bbox() const override { return box; }
+};
+
+// ---- midpoint BVH, pkg/geometry/bvh.go ----------------------------------------------
+struct BVHNode {
+    AABB box;
+    int left = -1, right = -1;
+    std::vector<const Shape*> shapes;  // leaf when non-empty or both children absent
+    bool leaf = false;
+};
+struct BVH : Shape {
+    std::vector<BVHNode> nodes;
+    int root = -1;
+    Vec3 center;
+    double radius = 0;
+    explicit BVH(std::vector<const Shape*> shapes) {  // NewBVH :26-55
+        if (shapes.empty()) {
+            center = {0, 0, 0};
+            radius = 0;
+            return;
+        }
+        nodes.reserve(2 * shapes.size() / 4 + 8);
+        root = build(shapes);
+        center = nodes[root].box.center();
+        radius = (nodes[root].box.mx - center).length();
+    }
+    int build(std::vector<const Shape*>& shapes) {  // buildBVH :64-109
+        AABB b = shapes[0]->bbox();
+        for (size_t i = 1; i < shapes.size(); i++) b = b.uni(shapes[i]->bbox());
+        int id = (int)nodes.size();
+        nodes.emplace_back();
+        nodes[id].box = b;
+        if (shapes.size() <= 8) {
+            nodes[id].leaf = true;
+            nodes[id].shapes = shapes;
+            return id;
+        }
+        int axis = b.longest_axis();  // findBestSplitSimple :112-133
+        double lo = b.mn[axis], hi = b.mx[axis];
+        if (hi <= lo) {
+            nodes[id].leaf = true;
+            nodes[id].shapes = shapes;
+            return id;
+        }
+        double split = (lo + hi) * 0.5;
+        std::vector<const Shape*> L, R;  // partitionShapesSimple :136-160
+        for (auto* s : shapes) {
+            if (s->bbox().center()[axis] < split) L.push_back(s);
+            else R.push_back(s);
+        }
+        if (L.empty() || R.empty()) {
+            nodes[id].leaf = true;
+            nodes[id].shapes = shapes;
+            return id;
+        }
+        std::vector<const Shape*>().swap(shapes);
+        int l = build(L);
+        int r = build(R);
+        nodes[id].left = l;
+        nodes[id].right = r;
+        return id;
+    }
+    bool hit_node(int ni, const Ray& r, double tmin, double tmax, Hit& h, Counters* c) const {
+        const BVHNode& n = nodes[ni];  // hitNode :191-239
+        if (c) c->box++;
+        if (!n.box.hit(r, tmin, tmax)) return false;
+        bool any = false;
+        double closest = tmax;
+        if (n.leaf) {
+            for (auto* s : n.shapes) {
+                Hit sh;
+                if (s->hit(r, tmin, closest, sh, c)) {
+                    any = true;
+                    closest = sh.t;
+                    h = sh;
+                }
+            }
+            return any;
+        }
+        Hit sh;
+        if (n.left >= 0 && hit_node(n.left, r, tmin, closest, sh, c)) {
+            any = true;
+            closest = sh.t;
+            h = sh;
+        }
+        if (n.right >= 0 && hit_node(n.right, r, tmin, closest, sh, c)) {
+            any = true;
+            closest = sh.t;
+            h = sh;
+        }
+        return any;
+    }
+    bool hit(const Ray& r, double tmin, double tmax, Hit& h, Counters* c) const override {
+        if (root < 0) return false;
+        return hit_node(root, r, tmin, tmax, h, c);
+    }
+    AABB bbox() const override { return root < 0 ? AABB{} : nodes[root].box; }
+};
+
+// ---- TriangleMesh, pkg/geometry/triangle_mesh.go:33-152 -----------------------------
+struct Mesh : Shape {
+    std::vector<std::unique_ptr<Triangle>> tris;
+    std::unique_ptr<BVH> bvh;
+    AABB box;
+    Mesh(const prt_mesh& m, int64_t first_id) {
+        tris.reserve(m.num_faces);
+        std::vector<const Shape*> sh;
+        sh.reserve(m.num_faces);
+        for (int64_t i = 0; i < m.num_faces; i++) {
+            const int32_t* f = m.faces + 3 * i;
+            int mat = m.face_materials ? m.face_materials[i] : m.material;
+            tris.emplace_back(new Triangle(V(m.vertices + 3 * f[0]), V(m.vertices + 3 * f[1]),
+                                           V(m.vertices + 3 * f[2]), mat, first_id + i));
+            sh.push_back(tris.back().get());
+        }
+        bvh.reset(new BVH(sh));
+        if (!tris.empty()) {
+            box = tris[0]->bbox();
+            for (size_t i = 1; i < tris.size(); i++) box = box.uni(tris[i]->bbox());
+        }
+    }
+    bool hit(const Ray& r, double tmin, double tmax, Hit& h, Counters* c) const override {
+        return bvh->hit(r, tmin, tmax, h, c);
+    }
+    AABB bbox() const override { return box; }
+};
+
+// ---- Camera, pkg/geometry/camera.go:70-190 ------------------------------------------
+struct Camera {
+    Vec3 center, pixel00, du, dv, disk_u, disk_v, u, v, w, forward;
+    double focus = 0, lens_radius = 0, lens_area = 1, vw = 0, vh = 0, plane_area = 0;
+    double cos_total_width = 0;
+    int width = 0, height = 0;
+    void init(const prt_camera& c) {
+        Vec3 C = V(c.center), L = V(c.look_at), U = V(c.up);
+        w = (C - L).normalize();
+        u = w.cross(U).normalize();
+        v = w.cross(u);
+        focus = c.focus_distance;
+        if (focus <= 0) focus = (C - L).length();
+        lens_radius = c.aperture / 2;
+        lens_area = kPi * lens_radius * lens_radius;
+        if (c.aperture == 0) lens_area = 1.0;
+        width = c.width;
+        height = (int)((double)width / c.aspect_ratio);
+        double theta = c.vfov * kPi / 180.0;
+        double hh = std::tan(theta / 2.0);
+        vh = 2.0 * hh * focus;
+        vw = vh * c.aspect_ratio;
+        double vh1 = 2.0 * std::tan(theta / 2.0);
+        plane_area = (vh1 * c.aspect_ratio) * vh1;
+        Vec3 VU = u * vw, VV = v * vh;
+        du = VU * (1.0 / (double)width);
+        dv = VV * (1.0 / (double)height);
+        Vec3 ul = C - w * focus - VU * 0.5 - VV * 0.5;
+        pixel00 = ul + (du + dv) * 0.5;
+        disk_u = u * (c.aperture / 2);
+        disk_v = v * (c.aperture / 2);
+        forward = (L - C).normalize();
+        Vec3 corner = Vec3(-vw / 2, -vh / 2, -focus).normalize();
+        cos_total_width = corner.dot(Vec3(0, 0, -1));
+        center = C;
+    }
+    Ray get_ray(int i, int j, Vec2 sp, Vec2 sj) const {  // GetRay :173-190
+        Vec3 jit(sj.x - 0.5, sj.y - 0.5, 0);
+        Vec3 ps = pixel00 + du * ((double)i + jit.x) + dv * ((double)j + jit.y);
+        Vec3 o = center;
+        if (disk_u.length() > 0) {
+            Vec3 p = sample_point_in_unit_disk(sp);
+            o = center + disk_u * p.x + disk_v * p.y;
+        }
+        return {o, (ps - o).normalize()};
+    }
+};
+
+// ---- materials, pkg/material ----------------------------------------------------------
+struct Material {
+    int type = 0, outer = -1, inner = -1;
+    Vec3 albedo, emission;
+    double fuzz = 0, ior = 1;
+};
+struct Scatter {
+    Ray in, out;
+    Vec3 att;
+    double pdf = 0;
+    bool specular() const { return pdf <= 0; }  // IsSpecular interfaces.go:42-44
+};
+
+static Vec3 reflect(Vec3 v, Vec3 n) { return v - n * (2 * v.dot(n)); }  // metal.go:87-90
+static Vec3 refract(Vec3 uv, Vec3 n, double eta) {                      // dielectric.go:155-161
+    double ct = std::min(-uv.dot(n), 1.0);
+    Vec3 perp = (uv + n * ct) * eta;
+    Vec3 par = n * (-std::sqrt(std::fabs(1.0 - perp.length2())));
+    return perp + par;
+}
+static double reflectance(double cosine, double ratio) {  // Schlick, dielectric.go:164-172
+    double r0 = (1 - ratio) / (1 + ratio);
+    r0 = r0 * r0;
+    return r0 + (1 - r0) * std::pow(1 - cosine, 5);
+}
+static bool is_reflection_path(Vec3 in, Vec3 out, Vec3 n) {  // layered.go:110-119
+    Vec3 inc = in.neg();
+    Vec3 refl = inc - n * (2 * inc.dot(n));
+    return (out - refl).length() < 0.1;
+}
+
+struct Scene;
+static bool mat_scatter(const Scene& S, int m, const Ray& in, const Hit& h, Sampler& smp, Scatter& out);
+static Vec3 mat_brdf(const Scene& S, int m, Vec3 in, Vec3 out, const Hit& h, int mode);
+static double mat_pdf(const Scene& S, int m, Vec3 in, Vec3 out, Vec3 n, bool& delta);
+
+// ---- lights, pkg/lights ------------------------------------------------------------------
+struct LightSample {
+    Vec3 p, n, dir, emission;
+    double dist = 0, pdf = 0;
+};
+struct EnvMap {
+    int w = 0, h = 0;
+    std::vector<double> rgb;       // h*w*3
+    std::vector<double> row_cdf;   // h+1
+    std::vector<double> col_cdf;   // h*(w+1)
+    std::vector<double> func;      // h*w, luminance * sin(theta)
+    double integral = 0;
+};
+struct Light {
+    int type = 0, shape = -1;
+    Vec3 emission, top, bottom;
+    // quad light (quad_light.go) / sphere light (sphere_light.go) geometry copies
+    std::unique_ptr<Quad> quad;
+    double area = 0;
+    std::unique_ptr<Sphere> sphere;
+    int mat = -1;
+    Vec3 world_center;
+    double world_radius = 0;
+    std::shared_ptr<EnvMap> env;
+};
+
+struct Scene {
+    std::vector<Material> mats;
+    std::vector<std::unique_ptr<Shape>> owned;
+    std::vector<const Shape*> shapes;
+    std::unique_ptr<BVH> bvh;
+    std::vector<Light> lights;
+    std::vector<double> weights;
+    Camera cam;
+    int64_t num_prims = 0;
+};
+
+static Vec3 emissive_emit(const Scene& S, int m, const Hit* h) {  // emissive.go:30-37
+    if (h && !h->front) return {0, 0, 0};
+    return S.mats[m].emission;
+}
+static bool is_emitter(const Scene& S, int m) { return m >= 0 && S.mats[m].type == PRT_MAT_EMISSIVE; }
+
+static bool mat_scatter(const Scene& S, int m, const Ray& in, const Hit& h, Sampler& smp, Scatter& out) {
+    const Material& M = S.mats[m];
+    switch (M.type) {
+        case PRT_MAT_LAMBERTIAN: {  // lambertian.go:25-50
+            Vec3 d = sample_cosine_hemisphere(h.n, smp.get2d());
+            double ct = d.normalize().dot(h.n);
+            if (ct < 0) ct = 0;
+            out.in = in;
+            out.out = {h.p, d};
+            out.pdf = ct / kPi;
+            out.att = M.albedo * (1.0 / kPi);
+            return true;
+        }
+        case PRT_MAT_METAL: {  // metal.go:38-59
+            Vec3 r = reflect(in.d.normalize(), h.n);
+            if (M.fuzz > 0) r = r + sample_point_in_unit_sphere(smp.get3d()) * M.fuzz;
+            out.in = in;
+            out.out = {h.p, r};
+            out.att = M.albedo;
+            out.pdf = 0;
+            return r.dot(h.n) > 0;
+        }
+        case PRT_MAT_DIELECTRIC: {  // dielectric.go:20-60
+            double ratio = h.front ? 1.0 / M.ior : M.ior;
+            Vec3 ud = in.d.normalize();
+            double ct = std::min(-ud.dot(h.n), 1.0);
+            double st = std::sqrt(1.0 - ct * ct);
+            bool cannot = ratio * st > 1.0;
+            Vec3 d;
+            if (cannot || reflectance(ct, ratio) > smp.get1d()) d = reflect(ud, h.n);
+            else d = refract(ud, h.n, ratio);
+            out.in = in;
+            out.out = {h.p, d};
+            out.att = {1, 1, 1};
+            out.pdf = 0;
+            return true;
+        }
+        case PRT_MAT_EMISSIVE:
+            return false;  // emissive.go:24-27
+        case PRT_MAT_LAYERED: {  // layered.go:24-69
+            Hit oh = h;
+            oh.mat = M.outer;
+            Scatter o;
+            if (!mat_scatter(S, M.outer, in, oh, smp, o)) return false;
+            Vec3 sd = o.out.d.normalize();
+            if (!(sd.dot(h.n) < 0)) {
+                out = o;
+                return true;
+            }
+            Ray ir{h.p, sd};
+            Hit ih = h;
+            ih.mat = M.inner;
+            Scatter i;
+            if (!mat_scatter(S, M.inner, ir, ih, smp, i)) {
+                out = o;
+                return true;
+            }
+            out.in = in;
+            out.out = i.out;
+            out.att = o.att.mul(i.att);
+            out.pdf = i.pdf;
+            return true;
+        }
+    }
+    return false;
+}
+
+static Vec3 mat_brdf(const Scene& S, int m, Vec3 in, Vec3 outd, const Hit& h, int mode) {
+    const Material& M = S.mats[m];
+    switch (M.type) {
+        case PRT_MAT_LAMBERTIAN: {  // lambertian.go:53-63
+            if (outd.dot(h.n) <= 0) return {0, 0, 0};
+            return M.albedo * (1.0 / kPi);
+        }
+        case PRT_MAT_METAL: {  // metal.go:62-74
+            Vec3 r = reflect(in.neg(), h.n);
+            if ((outd - r).length() < 0.001) return M.albedo;
+            return {0, 0, 0};
+        }
+        case PRT_MAT_DIELECTRIC: {  // dielectric.go:63-143
+            Vec3 wi = in.normalize(), wo = outd.normalize(), n = h.n.normalize();
+            double wiN = wi.dot(n), woN = wo.dot(n);
+            bool same = (wiN > 0) == (woN > 0);
+            if (!same) {
+                Vec3 r = reflect(wi, n);
+                if ((wo - r).length() < 0.001) {
+                    double ct = std::fabs(wiN);
+                    double eta = h.front ? 1.0 / M.ior : M.ior;
+                    double f = reflectance(ct, eta);
+                    return {f, f, f};
+                }
+            } else {
+                double eta = h.front ? 1.0 / M.ior : M.ior;
+                double ct = std::fabs(wiN);
+                Vec3 rr = refract(wi, n, eta);
+                if ((wo - rr).length() < 0.001) {
+                    double tr = 1.0 - reflectance(ct, eta);
+                    if (mode == 0) {
+                        double etap = h.front ? M.ior : 1.0 / M.ior;
+                        tr /= etap * etap;
+                    }
+                    return {tr, tr, tr};
+                }
+            }
+            return {0, 0, 0};
+        }
+        case PRT_MAT_EMISSIVE:
+            return {0, 0, 0};
+        case PRT_MAT_LAYERED:  // layered.go:73-87
+            if (is_reflection_path(in, outd, h.n)) return mat_brdf(S, M.outer, in, outd, h, mode);
+            return mat_brdf(S, M.inner, in, outd, h, mode);
+    }
+    return {0, 0, 0};
+}
+
+static double mat_pdf(const Scene& S, int m, Vec3 in, Vec3 outd, Vec3 n, bool& delta) {
+    const Material& M = S.mats[m];
+    switch (M.type) {
+        case PRT_MAT_LAMBERTIAN: {  // lambertian.go:66-72
+            delta = false;
+            double ct = outd.dot(n);
+            if (ct <= 0) return 0;
+            return ct / kPi;
+        }
+        case PRT_MAT_METAL:
+        case PRT_MAT_DIELECTRIC:
+            delta = true;
+            return 0;
+        case PRT_MAT_EMISSIVE:
+            delta = false;
+            return 0;
+        case PRT_MAT_LAYERED: {  // layered.go:90-104
+            bool d2;
+            double p;
+            if (is_reflection_path(in, outd, n)) p = mat_pdf(S, M.outer, in, outd, n, d2);
+            else p = mat_pdf(S, M.inner, in, outd, n, d2);
+            delta = false;
+            return p;
+        }
+    }
+    delta = false;
+    return 0;
+}
+
+// environment map (no reference counterpart; contract of pkg/lights/interfaces.go:17-43)
+static void env_dir_to_uv(const EnvMap& e, Vec3 d, int& x, int& y) {
+    d = d.normalize();
+    double cy = std::max(-1.0, std::min(1.0, d.y));
+    double theta = std::acos(cy);
+    double phi = std::atan2(d.z, d.x);
+    if (phi < 0) phi += 2 * kPi;
+    double u = phi / (2 * kPi), v = theta / kPi;
+    x = std::min((int)(u * e.w), e.w - 1);
+    y = std::min((int)(v * e.h), e.h - 1);
+    if (x < 0) x = 0;
+    if (y < 0) y = 0;
+}
+static Vec3 env_eval(const EnvMap& e, Vec3 d) {
+    int x, y;
+    env_dir_to_uv(e, d, x, y);
+    const double* p = &e.rgb[((size_t)y * e.w + x) * 3];
+    return {p[0], p[1], p[2]};
+}
+static double env_pdf(const EnvMap& e, Vec3 d) {
+    if (e.integral <= 0) return 0;
+    int x, y;
+    env_dir_to_uv(e, d, x, y);
+    double st = std::sin((y + 0.5) / e.h * kPi);
+    if (st <= 0) return 0;
+    double p = e.func[(size_t)y * e.w + x] / e.integral;  // density on the unit square
+    return p / (2 * kPi * kPi * st);
+}
+static int find_interval(const double* cdf, int n, double u) {  // largest i with cdf[i] <= u
+    int lo = 0, hi = n;                                          // cdf has n+1 entries
+    while (lo + 1 < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+static Vec3 env_sample(const EnvMap& e, Vec2 s, double& pdf) {
+    int y = find_interval(e.row_cdf.data(), e.h, s.x);
+    double dy = (e.row_cdf[y + 1] - e.row_cdf[y]);
+    double fy = dy > 0 ? (s.x - e.row_cdf[y]) / dy : 0.5;
+    const double* cc = &e.col_cdf[(size_t)y * (e.w + 1)];
+    int x = find_interval(cc, e.w, s.y);
+    double dx = cc[x + 1] - cc[x];
+    double fx = dx > 0 ? (s.y - cc[x]) / dx : 0.5;
+    double u = (x + fx) / e.w, v = (y + fy) / e.h;
+    double theta = v * kPi, phi = u * 2 * kPi;
+    double st = std::sin(theta);
+    Vec3 d(st * std::cos(phi), std::cos(theta), st * std::sin(phi));
+    double sc = std::sin((y + 0.5) / e.h * kPi);
+    pdf = (sc > 0 && e.integral > 0) ? (e.func[(size_t)y * e.w + x] / e.integral) / (2 * kPi * kPi * sc) : 0;
+    return d;
+}
+
+static Vec3 light_emit(const Scene& S, const Light& L, const Ray& r) {  // Light.Emit for infinite lights
+    switch (L.type) {
+        case PRT_LIGHT_UNIFORM_INFINITE: return L.emission;  // uniform_infinite_light.go:124-126
+        case PRT_LIGHT_GRADIENT_INFINITE: {                   // gradient_infinite_light.go:133-137
+            Vec3 d = r.d.normalize();
+            double t = 0.5 * (d.y + 1.0);
+            return L.bottom * (1.0 - t) + L.top * t;
+        }
+        case PRT_LIGHT_ENVMAP: return env_eval(*L.env, r.d);
+        default: return {0, 0, 0};
+    }
+}
+static bool light_is_infinite(const Light& L) {
+    return L.type == PRT_LIGHT_UNIFORM_INFINITE || L.type == PRT_LIGHT_GRADIENT_INFINITE ||
+           L.type == PRT_LIGHT_ENVMAP;
+}
+
+static LightSample sphere_sample_uniform(const Scene& S, const Light& L, Vec3 p, Vec2 s) {  // sphere_light.go:52-77
+    const Sphere& sp = *L.sphere;
+    double z = 1.0 - 2.0 * s.x;
+    double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+    double phi = 2.0 * kPi * s.y;
+    Vec3 ld(r * std::cos(phi), r * std::sin(phi), z);
+    Vec3 sp_pt = sp.c + ld * sp.r;
+    Vec3 dir = sp_pt - p;
+    LightSample ls;
+    ls.p = sp_pt;
+    ls.n = ld;
+    ls.dist = dir.length();
+    ls.dir = dir.normalize();
+    ls.pdf = 1.0 / (4.0 * kPi * sp.r * sp.r);
+    ls.emission = emissive_emit(S, L.mat, nullptr);
+    return ls;
+}
+
+static LightSample light_sample(const Scene& S, const Light& L, Vec3 p, Vec3 n, Vec2 s) {
+    LightSample ls;
+    switch (L.type) {
+        case PRT_LIGHT_QUAD: {  // quad_light.go:36-84
+            const Quad& q = *L.quad;
+            Vec3 sp = q.corner + q.u * s.x + q.v * s.y;
+            Vec3 tl = sp - p;
+            double dist = tl.length();
+            Vec3 dir = tl * (1.0 / dist);
+            double pdf = 1.0 / L.area;
+            double ct = std::fabs(q.normal.dot(dir * -1.0));
+            ls.p = sp;
+            ls.n = q.normal;
+            ls.dir = dir;
+            ls.dist = dist;
+            if (ct < 1e-8) {
+                ls.emission = {0, 0, 0};
+                ls.pdf = 0;
+                return ls;
+            }
+            ls.pdf = pdf * dist * dist / ct;
+            bool front = dir.dot(q.normal) < 0;
+            ls.emission = front ? emissive_emit(S, L.mat, nullptr) : Vec3(0, 0, 0);
+            return ls;
+        }
+        case PRT_LIGHT_SPHERE: {  // sphere_light.go:37-141
+            const Sphere& sp = *L.sphere;
+            Vec3 tc = sp.c - p;
+            double dc = tc.length();
+            if (dc <= sp.r) return sphere_sample_uniform(S, L, p, s);
+            Vec3 w = tc.normalize();
+            Vec3 u = std::fabs(w.x) > 0.1 ? Vec3(0, 1, 0) : Vec3(1, 0, 0);
+            u = u.cross(w).normalize();
+            Vec3 v = w.cross(u);
+            double stm = sp.r / dc;
+            double ctm = std::sqrt(std::max(0.0, 1.0 - stm * stm));
+            double ct = 1.0 - s.x * (1.0 - ctm);
+            double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+            double phi = 2.0 * kPi * s.y;
+            Vec3 dir = u * (st * std::cos(phi)) + v * (st * std::sin(phi)) + w * ct;
+            Ray r{p, dir};
+            Hit h;
+            if (!sp.hit(r, 0.001, kInf, h, nullptr)) return sphere_sample_uniform(S, L, p, s);
+            ls.p = h.p;
+            ls.n = h.n;
+            ls.dir = dir;
+            ls.dist = h.t;
+            ls.pdf = 1.0 / (2.0 * kPi * (1.0 - ctm));
+            ls.emission = emissive_emit(S, L.mat, nullptr);
+            return ls;
+        }
+        case PRT_LIGHT_UNIFORM_INFINITE:
+        case PRT_LIGHT_GRADIENT_INFINITE: {  // uniform_infinite_light.go:73-85, gradient :86-100
+            Vec3 dir = sample_cosine_hemisphere(n, s);
+            double ct = dir.dot(n);
+            ls.p = p + dir * 1e10;
+            ls.n = dir * -1.0;
+            ls.dir = dir;
+            ls.dist = kInf;
+            ls.pdf = ct / kPi;
+            if (L.type == PRT_LIGHT_UNIFORM_INFINITE) ls.emission = L.emission;
+            else {
+                double t = 0.5 * (dir.y + 1.0);
+                ls.emission = L.bottom * (1.0 - t) + L.top * t;
+            }
+            return ls;
+        }
+        case PRT_LIGHT_ENVMAP: {
+            double pdf;
+            Vec3 dir = env_sample(*L.env, s, pdf);
+            ls.p = p + dir * 1e10;
+            ls.n = dir * -1.0;
+            ls.dir = dir;
+            ls.dist = kInf;
+            ls.pdf = pdf;
+            ls.emission = env_eval(*L.env, dir);
+            return ls;
+        }
+    }
+    return ls;
+}
+
+static double light_pdf(const Scene& S, const Light& L, Vec3 p, Vec3 n, Vec3 dir) {
+    switch (L.type) {
+        case PRT_LIGHT_QUAD: {  // quad_light.go:87-104
+            Ray r{p, dir};
+            Hit h;
+            if (!L.quad->hit(r, 0.001, kInf, h, nullptr)) return 0;
+            double dist = h.t;
+            double ct = std::fabs(L.quad->normal.dot(dir * -1.0));
+            if (ct < 1e-8) return 0;
+            return (1.0 / L.area) * dist * dist / ct;
+        }
+        case PRT_LIGHT_SPHERE: {  // sphere_light.go:144-165
+            Ray r{p, dir};
+            Hit h;
+            const Sphere& sp = *L.sphere;
+            if (!sp.hit(r, 0.001, kInf, h, nullptr)) return 0;
+            double dc = (sp.c - p).length();
+            if (dc <= sp.r) return 1.0 / (4.0 * kPi * sp.r * sp.r);
+            double stm = sp.r / dc;
+            double ctm = std::sqrt(std::max(0.0, 1.0 - stm * stm));
+            return 1.0 / (2.0 * kPi * (1.0 - ctm));
+        }
+        case PRT_LIGHT_UNIFORM_INFINITE:
+        case PRT_LIGHT_GRADIENT_INFINITE: {
+            double ct = dir.dot(n);
+            if (ct <= 0) return 0;
+            return ct / kPi;
+        }
+        case PRT_LIGHT_ENVMAP: return env_pdf(*L.env, dir);
+    }
+    return 0;
+}
+
+// WeightedLightSampler.SampleLight, weighted_light_sampler.go:82-96
+static int select_light(const Scene& S, double u, double& prob) {
+    int n = (int)S.lights.size();
+    double cum = 0;
+    for (int i = 0; i < n; i++) {
+        cum += S.weights[i];
+        if (u <= cum) {
+            prob = S.weights[i];
+            return i;
+        }
+    }
+    prob = S.weights[n - 1];
+    return n - 1;
+}
+// CalculateLightPDF, light_sampling.go:11-25
+static double total_light_pdf(const Scene& S, Vec3 p, Vec3 n, Vec3 dir) {
+    if (S.lights.empty()) return 0;
+    double tot = 0;
+    for (size_t i = 0; i < S.lights.size(); i++) tot += light_pdf(S, S.lights[i], p, n, dir) * S.weights[i];
+    return tot;
+}
+
+// ---- path tracing, pkg/integrator/path_tracing.go ------------------------------------
+struct PTConfig {
+    int max_depth = 50, rr_min = 20;
+};
+
+static double power_heuristic(int nf, double fp, int ng, double gp) {  // :224-234
+    if (fp == 0) return 0;
+    double f = nf * fp, g = ng * gp;
+    return (f * f) / (f * f + g * g);
+}
+
+static Vec3 direct_light(const Scene& S, const Scatter& sc, const Hit& h, Sampler& smp) {  // :116-157
+    if (S.lights.empty()) return {0, 0, 0};
+    double prob;
+    int li = select_light(S, smp.get1d(), prob);
+    LightSample ls = light_sample(S, S.lights[li], h.p, h.n, smp.get2d());
+    ls.pdf *= prob;
+    if (ls.emission.luminance() <= 0 || ls.pdf <= 0) return {0, 0, 0};
+    Ray shadow{h.p, ls.dir};
+    Hit bh;
+    if (S.bvh->hit(shadow, 0.001, ls.dist - 0.001, bh, nullptr)) return {0, 0, 0};
+    double cosine = ls.dir.dot(h.n);
+    if (cosine <= 0) return {0, 0, 0};
+    bool delta;
+    double mp = mat_pdf(S, h.mat, sc.in.d, ls.dir, h.n, delta);
+    if (delta) return {0, 0, 0};
+    double w = power_heuristic(1, ls.pdf, 1, mp);
+    Vec3 f = mat_brdf(S, h.mat, sc.in.d, ls.dir, h, 0);
+    return f.mul(ls.emission) * (cosine * w / ls.pdf);
+}
+
+static Vec3 ray_color(const Scene& S, const PTConfig& cfg, const Ray& ray, Sampler& smp, int depth,
+                      Vec3 thr) {  // rayColorRecursive :35-85
+    if (depth <= 0) return {0, 0, 0};
+    int bounce = cfg.max_depth - depth;
+    smp.set_slot((uint32_t)bounce + 1);
+    // ApplyRussianRoulette :193-220 (the sample is drawn on every level)
+    double u = smp.get1d();
+    double rrc = 1.0;
+    if (bounce >= cfg.rr_min) {
+        double q = std::min(0.95, std::max(0.5, thr.luminance()));
+        if (u > q) return {0, 0, 0};
+        rrc = 1.0 / q;
+    }
+    Hit h;
+    if (!S.bvh->hit(ray, 0.001, kInf, h, nullptr)) {
+        Vec3 e;
+        for (auto& L : S.lights)
+            if (light_is_infinite(L)) e = e + light_emit(S, L, ray);
+        return e * rrc;
+    }
+    Vec3 le = is_emitter(S, h.mat) ? emissive_emit(S, h.mat, &h) : Vec3(0, 0, 0);  // getEmittedLight :100-105
+    Scatter sc;
+    if (!mat_scatter(S, h.mat, ray, h, smp, sc)) return le * rrc;
+    Vec3 cs;
+    if (sc.specular()) {  // calculateSpecularColor :88-97
+        Vec3 nt = thr.mul(sc.att);
+        Vec3 in = ray_color(S, cfg, sc.out, smp, depth - 1, nt);
+        cs = sc.att.mul(in);
+    } else {  // calculateDiffuseColor :100-106
+        smp.set_slot((uint32_t)bounce + 1);
+        smp.k = 8;  // NEE draws take stream positions 8..10 of this bounce
+        Vec3 dl = direct_light(S, sc, h, smp);
+        Vec3 il;
+        // CalculateIndirectLighting :160-189
+        if (sc.pdf > 0) {
+            Vec3 sd = sc.out.d.normalize();
+            double cosine = sd.dot(h.n);
+            if (cosine > 0) {
+                double lp = total_light_pdf(S, h.p, h.n, sd);
+                double w = power_heuristic(1, sc.pdf, 1, lp);
+                Vec3 nt = thr.mul(sc.att) * (cosine / sc.pdf);
+                Vec3 in = ray_color(S, cfg, sc.out, smp, depth - 1, nt);
+                il = (sc.att * (cosine * w / sc.pdf)).mul(in);
+            }
+        }
+        cs = dl + il;
+    }
+    return (le + cs) * rrc;
+}
+
+static void build_env(EnvMap& e, const prt_light& pl) {
+    e.w = pl.env_width;
+    e.h = pl.env_height;
+    size_t n = (size_t)e.w * e.h;
+    e.rgb.resize(n * 3);
+    for (size_t i = 0; i < n * 3; i++) e.rgb[i] = (double)pl.env_rgb[i];
+    e.func.resize(n);
+    e.col_cdf.resize((size_t)e.h * (e.w + 1));
+    e.row_cdf.resize(e.h + 1);
+    std::vector<double> row_sum(e.h);
+    for (int y = 0; y < e.h; y++) {
+        double st = std::sin((y + 0.5) / e.h * kPi);
+        double* cc = &e.col_cdf[(size_t)y * (e.w + 1)];
+        cc[0] = 0;
+        for (int x = 0; x < e.w; x++) {
+            const double* p = &e.rgb[((size_t)y * e.w + x) * 3];
+            double l = 0.2126 * p[0] + 0.7152 * p[1] + 0.0722 * p[2];
+            double f = std::max(l, 0.0) * st;
+            e.func[(size_t)y * e.w + x] = f;
+            cc[x + 1] = cc[x] + f;
+        }
+        row_sum[y] = cc[e.w];
+        if (row_sum[y] > 0) {
+            for (int x = 1; x <= e.w; x++) cc[x] /= row_sum[y];
+        } else {
+            for (int x = 1; x <= e.w; x++) cc[x] = (double)x / e.w;
+        }
+        cc[e.w] = 1.0;
+    }
+    e.row_cdf[0] = 0;
+    for (int y = 0; y < e.h; y++) e.row_cdf[y + 1] = e.row_cdf[y] + row_sum[y];
+    double tot = e.row_cdf[e.h];
+    for (int y = 1; y <= e.h; y++) e.row_cdf[y] = tot > 0 ? e.row_cdf[y] / tot : (double)y / e.h;
+    e.row_cdf[e.h] = 1.0;
+    // density on the unit square: func / (tot / (w*h))
+    e.integral = tot / ((double)e.w * e.h);
+}
+
+}  // namespace orc
+
+// =============================== C interface ===========================================
+using namespace orc;
+
+struct orc_scene {
+    Scene s;
+};
+
+extern "C" {
+
+orc_scene* orc_scene_create(const prt_scene_desc* d) {
+    auto* os = new orc_scene();
+    Scene& S = os->s;
+    for (int i = 0; i < d->num_materials; i++) {
+        const prt_material& pm = d->materials[i];
+        Material m;
+        m.type = pm.type;
+        m.outer = pm.outer;
+        m.inner = pm.inner;
+        m.albedo = V(pm.albedo);
+        m.emission = V(pm.emission);
+        m.fuzz = std::min(1.0, std::max(0.0, pm.fuzz));
+        m.ior = pm.ior;
+        S.mats.push_back(m);
+    }
+    int64_t id = 0;
+    for (int i = 0; i < d->num_meshes; i++) {
+        S.owned.emplace_back(new Mesh(d->meshes[i], id));
+        id += d->meshes[i].num_faces;
+    }
+    for (int i = 0; i < d->num_spheres; i++) {
+        const prt_sphere& p = d->spheres[i];
+        S.owned.emplace_back(new Sphere(V(p.center), p.radius, p.material, id++));
+    }
+    for (int i = 0; i < d->num_quads; i++) {
+        const prt_quad& q = d->quads[i];
+        S.owned.emplace_back(new Quad(V(q.corner), V(q.u), V(q.v), q.material, id++));
+    }
+    for (int i = 0; i < d->num_boxes; i++) {
+        const prt_box& b = d->boxes[i];
+        S.owned.emplace_back(new Box(V(b.center), V(b.size), V(b.rotation), b.material, id));
+        id += 6;
+    }
+    S.num_prims = id;
+    for (auto& o : S.owned) S.shapes.push_back(o.get());
+    S.bvh.reset(new BVH(S.shapes));  // Scene.Preprocess scene.go:45-48
+    for (int i = 0; i < d->num_lights; i++) {
+        const prt_light& pl = d->lights[i];
+        Light L;
+        L.type = pl.type;
+        L.shape = pl.shape;
+        L.emission = V(pl.emission);
+        L.top = V(pl.top);
+        L.bottom = V(pl.bottom);
+        L.world_center = S.bvh->center;
+        L.world_radius = S.bvh->radius;
+        if (pl.type == PRT_LIGHT_QUAD) {
+            const prt_quad& q = d->quads[pl.shape];
+            L.quad.reset(new Quad(V(q.corner), V(q.u), V(q.v), q.material, -1));
+            L.area = V(q.u).cross(V(q.v)).length();  // NewQuadLight quad_light.go:19-29
+            L.mat = q.material;
+        } else if (pl.type == PRT_LIGHT_SPHERE) {
+            const prt_sphere& sp = d->spheres[pl.shape];
+            L.sphere.reset(new Sphere(V(sp.center), sp.radius, sp.material, -1));
+            L.mat = sp.material;
+        } else if (pl.type == PRT_LIGHT_ENVMAP) {
+            L.env = std::make_shared<EnvMap>();
+            build_env(*L.env, pl);
+        }
+        S.lights.push_back(std::move(L));
+    }
+    int nl = d->num_lights;  // NewUniformLightSampler / NewWeightedLightSampler
+    S.weights.assign(nl, nl ? 1.0 / nl : 0.0);
+    if (d->light_weights && nl) {
+        double tot = 0;
+        for (int i = 0; i < nl; i++) tot += d->light_weights[i];
+        for (int i = 0; i < nl; i++) S.weights[i] = tot == 0 ? 1.0 / nl : d->light_weights[i] / tot;
+    }
+    S.cam.init(d->camera);
+    return os;
+}
+
+void orc_scene_destroy(orc_scene* s) { delete s; }
+
+void orc_scene_world(const orc_scene* s, double* center, double* radius, int64_t* nprims) {
+    center[0] = s->s.bvh->center.x;
+    center[1] = s->s.bvh->center.y;
+    center[2] = s->s.bvh->center.z;
+    *radius = s->s.bvh->radius;
+    *nprims = s->s.num_prims;
+}
+
+static void parallel_for(int64_t n, int nthreads, const std::function<void(int64_t, int64_t)>& fn);
+
+// Closest hit of each ray through the reference midpoint BVH (BVH.Hit).
+// out_t: +inf on miss; out_prim: -1 on miss. brute != 0 tests every primitive instead.
+void orc_intersect(const orc_scene* os, const prt_ray* rays, int64_t n, double* out_t, int64_t* out_prim,
+                   double* out_u, double* out_v, int32_t brute, int32_t nthreads, uint64_t* counters) {
+    const Scene& S = os->s;
+    std::atomic<uint64_t> nbox{0}, nprim{0};
+    parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+        Counters c;
+        for (int64_t i = b; i < e; i++) {
+            const prt_ray& pr = rays[i];
+            Ray r{{pr.org[0], pr.org[1], pr.org[2]}, {pr.dir[0], pr.dir[1], pr.dir[2]}};
+            Hit h;
+            bool hit = false;
+            if (brute) {
+                double closest = pr.tmax;
+                for (auto* sh : S.shapes) {
+                    const Mesh* m = dynamic_cast<const Mesh*>(sh);
+                    if (m) {
+                        for (auto& t : m->tris) {
+                            Hit th;
+                            if (t->hit(r, pr.tmin, closest, th, nullptr)) {
+                                hit = true;
+                                closest = th.t;
+                                h = th;
+                            }
+                        }
+                    } else {
+                        Hit th;
+                        if (sh->hit(r, pr.tmin, closest, th, nullptr)) {
+                            hit = true;
+                            closest = th.t;
+                            h = th;
+                        }
+                    }
+                }
+            } else {
+                hit = S.bvh->hit(r, pr.tmin, pr.tmax, h, counters ? &c : nullptr);
+            }
+            out_t[i] = hit ? h.t : INFINITY;
+            out_prim[i] = hit ? h.prim : -1;
+            if (out_u) out_u[i] = hit ? h.uv.x : 0;
+            if (out_v) out_v[i] = hit ? h.uv.y : 0;
+        }
+        nbox += c.box;
+        nprim += c.prim;
+    });
+    if (counters) {
+        counters[0] = nbox.load();
+        counters[1] = nprim.load();
+    }
+}
+
+// Path-traced radiance sums per pixel for samples [sample_offset, sample_offset + spp)
+// over the pixel rectangle [x0,x1)x[y0,y1) of the camera image. Output arrays are
+// (y1-y0)*(x1-x0) pixels: sum_rgb (x3), lum_sum, lum_sq_sum (PixelStats.AddSample, stats.go:41-47).
+int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int32_t sample_offset,
+               int32_t x0, int32_t y0, int32_t x1, int32_t y1, double* sum_rgb, double* lum_sum,
+               double* lum_sq, int32_t nthreads) {
+    const Scene& S = os->s;
+    if (p->integrator != PRT_INTEGRATOR_PATH) return PRT_ERR_UNSUPPORTED;
+    PTConfig cfg;
+    cfg.max_depth = p->max_depth;
+    cfg.rr_min = p->rr_min_bounces;
+    int W = S.cam.width;
+    int tw = x1 - x0, th = y1 - y0;
+    // 64x64 tiles over a worker pool, as pkg/renderer/worker_pool.go:49-131
+    int ts = p->tile_size > 0 ? p->tile_size : 64;
+    int ntx = (tw + ts - 1) / ts, nty = (th + ts - 1) / ts;
+    int ntiles = ntx * nty;
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        for (;;) {
+            int t = next.fetch_add(1);
+            if (t >= ntiles) break;
+            int tx = t % ntx, ty = t / ntx;
+            for (int j = y0 + ty * ts; j < std::min(y1, y0 + (ty + 1) * ts); j++)
+                for (int i = x0 + tx * ts; i < std::min(x1, x0 + (tx + 1) * ts); i++) {
+                    size_t o = (size_t)(j - y0) * tw + (i - x0);
+                    double sr = 0, sg = 0, sb = 0, ls = 0, lq = 0;
+                    for (int s = sample_offset; s < sample_offset + spp; s++) {
+                        Sampler smp;
+                        smp.key = path_key(p->seed, (uint32_t)(j * W + i), (uint32_t)s);
+                        smp.set_slot(0);
+                        Vec2 a = smp.get2d();
+                        Vec2 b = smp.get2d();
+                        Ray r = S.cam.get_ray(i, j, a, b);
+                        Vec3 c = ray_color(S, cfg, r, smp, cfg.max_depth, Vec3(1, 1, 1));
+                        sr += c.x;
+                        sg += c.y;
+                        sb += c.z;
+                        double l = c.luminance();
+                        ls += l;
+                        lq += l * l;
+                    }
+                    sum_rgb[o * 3 + 0] = sr;
+                    sum_rgb[o * 3 + 1] = sg;
+                    sum_rgb[o * 3 + 2] = sb;
+                    if (lum_sum) lum_sum[o] = ls;
+                    if (lum_sq) lum_sq[o] = lq;
+                }
+        }
+    };
+    int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    std::vector<std::thread> pool;
+    for (int i = 0; i < nt; i++) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    return PRT_OK;
+}
+
+// Radiance of individual samples: pixels[i] = (x, y), samples[i] = sample index.
+int orc_render_samples(const orc_scene* os, const prt_render_params* p, const int32_t* pixels,
+                       const int32_t* samples, int64_t n, double* out_rgb, int32_t nthreads) {
+    const Scene& S = os->s;
+    PTConfig cfg;
+    cfg.max_depth = p->max_depth;
+    cfg.rr_min = p->rr_min_bounces;
+    int W = S.cam.width;
+    parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; k++) {
+            int i = pixels[2 * k], j = pixels[2 * k + 1];
+            Sampler smp;
+            smp.key = path_key(p->seed, (uint32_t)(j * W + i), (uint32_t)samples[k]);
+            smp.set_slot(0);
+            Vec2 a = smp.get2d();
+            Vec2 bb = smp.get2d();
+            Ray r = S.cam.get_ray(i, j, a, bb);
+            Vec3 c = ray_color(S, cfg, r, smp, cfg.max_depth, Vec3(1, 1, 1));
+            out_rgb[3 * k] = c.x;
+            out_rgb[3 * k + 1] = c.y;
+            out_rgb[3 * k + 2] = c.z;
+        }
+    });
+    return PRT_OK;
+}
+
+// ---- unit-level entry points used by the golden-vector tests ------------------------
+double orc_draw(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t pos) {
+    return draw(path_key(seed, pixel, sample), pos);
+}
+void orc_sample_cosine_hemisphere(const double* n, const double* s, double* out) {
+    Vec3 r = sample_cosine_hemisphere(V(n), Vec2{s[0], s[1]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void orc_sample_on_unit_sphere(const double* s, double* out) {
+    Vec3 r = sample_on_unit_sphere(Vec2{s[0], s[1]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void orc_sample_point_in_unit_disk(const double* s, double* out) {
+    Vec3 r = sample_point_in_unit_disk(Vec2{s[0], s[1]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void orc_sample_point_in_unit_sphere(const double* s, double* out) {
+    Vec3 r = sample_point_in_unit_sphere(Vec3(s[0], s[1], s[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+double orc_power_heuristic(int32_t nf, double f, int32_t ng, double g) { return power_heuristic(nf, f, ng, g); }
+double orc_reflectance(double c, double r) { return reflectance(c, r); }
+// Single-shape intersection: kind 0 triangle (9 doubles), 1 sphere (4), 2 quad (9).
+// out: t, p(3), n(3), front, u, v.  Returns 1 on hit.
+int32_t orc_shape_hit(int32_t kind, const double* g, const double* o, const double* d, double tmin,
+                      double tmax, double* out) {
+    Ray r{V(o), V(d)};
+    Hit h;
+    bool hit = false;
+    if (kind == 0) hit = Triangle(V(g), V(g + 3), V(g + 6), 0, 0).hit(r, tmin, tmax, h, nullptr);
+    else if (kind == 1) hit = Sphere(V(g), g[3], 0, 0).hit(r, tmin, tmax, h, nullptr);
+    else if (kind == 2) hit = Quad(V(g), V(g + 3), V(g + 6), 0, 0).hit(r, tmin, tmax, h, nullptr);
+    if (!hit) return 0;
+    out[0] = h.t;
+    out[1] = h.p.x; out[2] = h.p.y; out[3] = h.p.z;
+    out[4] = h.n.x; out[5] = h.n.y; out[6] = h.n.z;
+    out[7] = h.front ? 1 : 0;
+    out[8] = h.uv.x; out[9] = h.uv.y;
+    return 1;
+}
+void orc_quad_bbox(const double* g, double* out) {
+    AABB b = Quad(V(g), V(g + 3), V(g + 6), 0, 0).bbox();
+    out[0] = b.mn.x; out[1] = b.mn.y; out[2] = b.mn.z; out[3] = b.mx.x; out[4] = b.mx.y; out[5] = b.mx.z;
+}
+int32_t orc_aabb_hit(const double* box, const double* o, const double* d, double tmin, double tmax) {
+    AABB b{V(box), V(box + 3)};
+    return b.hit(Ray{V(o), V(d)}, tmin, tmax) ? 1 : 0;
+}
+// Camera: GetRay for pixel (i,j) with lens and jitter samples; also image height.
+void orc_camera_get_ray(const orc_scene* os, int32_t i, int32_t j, const double* lens, const double* jit,
+                        double* o, double* d) {
+    Ray r = os->s.cam.get_ray(i, j, Vec2{lens[0], lens[1]}, Vec2{jit[0], jit[1]});
+    o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z;
+    d[0] = r.d.x; d[1] = r.d.y; d[2] = r.d.z;
+}
+int32_t orc_camera_height(const orc_scene* os) { return os->s.cam.height; }
+// Light queries on light `li` of the scene: Sample (out: dir3, dist, emission3, pdf), PDF.
+void orc_light_sample(const orc_scene* os, int32_t li, const double* p, const double* n, const double* s,
+                      double* out) {
+    LightSample ls = light_sample(os->s, os->s.lights[li], V(p), V(n), Vec2{s[0], s[1]});
+    out[0] = ls.dir.x; out[1] = ls.dir.y; out[2] = ls.dir.z; out[3] = ls.dist;
+    out[4] = ls.emission.x; out[5] = ls.emission.y; out[6] = ls.emission.z; out[7] = ls.pdf;
+}
+double orc_light_pdf(const orc_scene* os, int32_t li, const double* p, const double* n, const double* d) {
+    return light_pdf(os->s, os->s.lights[li], V(p), V(n), V(d));
+}
+// Material queries: scatter with explicit stream key (returns 1 if scattered;
+// out: dir3, att3, pdf), BRDF (mode 0 radiance / 1 importance), PDF (out delta).
+int32_t orc_material_scatter(const orc_scene* os, int32_t m, const double* ro, const double* rd, const double* hp,
+                             const double* hn, int32_t front, uint32_t key, double* out) {
+    Hit h;
+    h.p = V(hp);
+    h.n = V(hn);
+    h.front = front != 0;
+    h.mat = m;
+    Sampler smp;
+    smp.key = key;
+    smp.set_slot(1);
+    Scatter sc;
+    bool ok = mat_scatter(os->s, m, Ray{V(ro), V(rd)}, h, smp, sc);
+    out[0] = sc.out.d.x; out[1] = sc.out.d.y; out[2] = sc.out.d.z;
+    out[3] = sc.att.x; out[4] = sc.att.y; out[5] = sc.att.z; out[6] = sc.pdf;
+    return ok ? 1 : 0;
+}
+void orc_material_brdf(const orc_scene* os, int32_t m, const double* in, const double* outd, const double* hn,
+                       int32_t front, int32_t mode, double* out) {
+    Hit h;
+    h.n = V(hn);
+    h.front = front != 0;
+    Vec3 f = mat_brdf(os->s, m, V(in), V(outd), h, mode);
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+}
+double orc_material_pdf(const orc_scene* os, int32_t m, const double* in, const double* outd, const double* n,
+                        int32_t* delta) {
+    bool d;
+    double p = mat_pdf(os->s, m, V(in), V(outd), V(n), d);
+    *delta = d ? 1 : 0;
+    return p;
+}
+// Radiance of one explicit ray through the PT estimator (RayColor) with key/bounce slots.
+void orc_ray_color(const orc_scene* os, const prt_render_params* p, const double* o, const double* d,
+                   uint32_t key, double* out) {
+    PTConfig cfg;
+    cfg.max_depth = p->max_depth;
+    cfg.rr_min = p->rr_min_bounces;
+    Sampler smp;
+    smp.key = key;
+    Vec3 c = ray_color(os->s, cfg, Ray{V(o), V(d)}, smp, cfg.max_depth, Vec3(1, 1, 1));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+// Pass schedule, getSamplesForPass progressive.go:98-123
+int32_t orc_samples_for_pass(int32_t initial, int32_t max_spp, int32_t max_passes, int32_t pass) {
+    if (max_passes == 1) return max_spp;
+    if (pass == 1) return initial;
+    int32_t rem = max_spp - initial;
+    int32_t rp = max_passes - 1;
+    int32_t per = rem / rp;
+    int32_t t = initial + (pass - 1) * per;
+    if (pass == max_passes) t = max_spp;
+    return t;
+}
+
+}  // extern "C"
+
+static void parallel_for(int64_t n, int nthreads, const std::function<void(int64_t, int64_t)>& fn) {
+    int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    if (nt <= 1 || n < 1024) {
+        fn(0, n);
+        return;
+    }
+    const int64_t chunk = 256;
+    std::atomic<int64_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+        pool.emplace_back([&]() {
+            for (;;) {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                fn(b, std::min(n, b + chunk));
+            }
+        });
+    for (auto& t : pool) t.join();
+}

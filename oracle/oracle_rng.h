@@ -1,0 +1,36 @@
+// Counter-based RNG shared in definition (not in code) with the device path.
+// TEST INFRASTRUCTURE ONLY: part of the oracle; the product has its own copy of the
+// definition in go-progressive-raytracer_amd/csrc/prt_rng.h.
+//
+// Keyed by (seed, pixel, sample); a draw is addressed by a stream position
+// pos = slot * 16 + k where slot 0 is the camera sample and slot b+1 is bounce b.
+// The reference draws from a per-tile math/rand stream (pkg/core/sampling.go:17-39,
+// pkg/renderer/progressive.go:442-452); that stream cannot be reproduced here (no Go),
+// so the oracle and the device consume this counter stream in the same call order.
+#pragma once
+#include <cstdint>
+
+namespace orc {
+
+static inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+static inline uint32_t path_key(uint64_t seed, uint32_t pixel, uint32_t sample) {
+    uint32_t k = mix32((uint32_t)seed ^ mix32((uint32_t)(seed >> 32) + 0x9E3779B9U));
+    k = mix32(k ^ mix32(pixel + 0x632BE5ABU));
+    k = mix32(k ^ mix32(sample + 0x85EBCA6BU));
+    return k;
+}
+
+static inline double draw(uint32_t key, uint32_t pos) {
+    uint32_t h = mix32(key ^ mix32(pos * 0x9E3779B9U + 0x7F4A7C15U));
+    return (double)(h >> 8) * (1.0 / 16777216.0);
+}
+
+}  // namespace orc
