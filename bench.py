@@ -29,6 +29,8 @@ def main():
     ap.add_argument("--frequency", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this size (counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (this build runs on a single "
@@ -43,6 +45,16 @@ def main():
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if a.calibrate_copy_mib:
+        nb = a.calibrate_copy_mib << 20
+        src = D.DeviceBuffer.from_array(np.ones(nb // 4, np.float32))
+        dst = D.DeviceBuffer(nb)
+        for _ in range(2):
+            D.copy_d2d(dst, src, nb)
+        D.synchronize()
+        log(f"copied {nb} bytes device-to-device twice")
+        return
 
     t0 = time.time()
     desc, info = scenes.mesh_scene(frequency=a.frequency)

@@ -101,5 +101,9 @@ class Event:
         return ms.value
 
 
+def copy_d2d(dst, src, nbytes):
+    _ok(hip().hipMemcpy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, 3), "hipMemcpy D2D")
+
+
 def synchronize():
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
