@@ -29,6 +29,9 @@ def main():
     ap.add_argument("--frequency", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pt", action="store_true", help="skip the path-tracing throughput section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (time-to-first-pass) section")
+    ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (counter calibration) and exit")
     a = ap.parse_args()
@@ -125,6 +128,12 @@ def main():
                "sample": f"first {done} of the {a.rays} rays through the float64 oracle (reference midpoint BVH, "
                          f"leaf 8, Moller-Trumbore), {cores} threads, {t_c:.1f}s"}
 
+    secondary = {}
+    if not a.no_pt:
+        secondary.update(pt_section(a, log))
+    if not a.no_serve:
+        secondary["serving"] = serve_section(a, log)
+
     out = {
         "metric": "Mrays/s closest-hit",
         "value": round(value, 2),
@@ -145,8 +154,122 @@ def main():
                    "bvh_depth": int(si.bvh_depth)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
+
+
+def pt_section(a, log):
+    """Path-tracing samples/s on the three render configurations, each set next to the oracle
+    (the float64 restatement of the reference estimator) timed on a bounded crop with all host
+    threads. Both draw the same counter-RNG stream per (pixel, sample), so at equal samples per
+    pixel they have the same expected RMSE: samples/s is the equal-RMSE rate."""
+    import numpy as np
+
+    import prtgpu as P
+    import scenes
+
+    configs = [
+        ("C0 default scene 400x225, 64 spp, depth 50 (pinhole)", lambda: scenes.default_scene(), None),
+        ("C2 metre Cornell boxes 512x512, 1024 spp, depth 8", lambda: scenes.cornell_metre(), None),
+        ("C3 1.8M-triangle glass icosphere + HDR sky 1024x1024, 256 spp, depth 16",
+         lambda: scenes.mesh_sky_scene(), None),
+    ]
+    out = []
+    cores = min(os.cpu_count() or 1, 16)
+    for name, make, _ in configs:
+        desc, cfg = make()
+        w, h = desc.image_size
+        spp = cfg["spp"]
+        scene = P.Scene(desc, device=0)
+        params = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp, max_passes=1, seed=1)
+        pr = P.ProgressiveRaytracer(scene, params)
+        pr.render_samples(1)  # warm-up (kernels, first-touch)
+        pr.reset()
+        t0 = time.perf_counter()
+        st = pr.render_samples(spp)
+        wall = time.perf_counter() - t0
+        gpu_sps = w * h * spp / wall
+        # first-pass latency: one sample per pixel on a fresh film (reference pass 1, progressive.go:104-107)
+        pr2 = P.ProgressiveRaytracer(scene, P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp,
+                                                              max_passes=7, seed=2))
+        t0 = time.perf_counter()
+        pr2.render_pass(1)
+        ttfp_ms = (time.perf_counter() - t0) * 1e3
+        pr2.close()
+        entry = {"config": name, "samples_per_s": round(gpu_sps, 1), "render_s": round(wall, 3),
+                 "device_ms": round(st.device_ms, 2), "first_pass_ms": round(ttfp_ms, 2)}
+        if not a.no_cpu:
+            import oracle as O
+
+            os_ = O.OracleScene(desc)
+            # bounded sample: the whole image at a few samples per pixel (~1-5 s of CPU work)
+            c_spp = 4 if w * h <= 400 * 225 else 1
+            c0 = time.perf_counter()
+            os_.render(params, c_spp, 0, nthreads=cores)
+            ct = time.perf_counter() - c0
+            cpu_sps = w * h * c_spp / ct
+            entry["cpu_samples_per_s"] = round(cpu_sps, 1)
+            entry["cpu_sample"] = f"full {w}x{h} image, {c_spp} spp, {cores} threads, {ct:.2f}s"
+            entry["speedup_equal_rmse"] = round(gpu_sps / cpu_sps, 1)
+        log(f"{name}: {entry}")
+        out.append(entry)
+        pr.close()
+        scene.close()
+    return {"pt": out, "cores": cores}
+
+
+def serve_section(a, log):
+    """Poisson arrivals (rate 20/s, seed 0) of progressive path-tracing requests mixing the three
+    render scenes x {256x256, 512x512, 1280x720, 1920x1080} x spp log-uniform in [16, 1024], 7 passes,
+    through the device-resident serving loop (HBM budget 64 GiB, up to 32 active requests).
+    Time to first pass = submit -> pass 1 (1 spp) readable, per request."""
+    import numpy as np
+
+    import prtgpu as P
+    import scenes
+
+    rng = np.random.default_rng(0)
+    made = [scenes.default_scene(), scenes.cornell_metre(), scenes.mesh_sky_scene()]
+    sc = [(P.Scene(d), cfg) for d, cfg in made]
+    sizes = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    n = a.serve_requests
+    arrivals = np.cumsum(rng.exponential(1 / 20.0, n))
+    picks = rng.integers(0, 3, n)
+    res = rng.integers(0, 4, n)
+    spps = np.exp(rng.uniform(np.log(16), np.log(1024), n)).astype(int)
+    srv = P.Server(hbm_budget_bytes=64 << 30, max_active=32)
+    ids, meta = [], {}
+    t0 = time.perf_counter()
+    ttfp, done_t, samples = {}, {}, 0
+    k = 0
+    while len(done_t) < n:
+        now = time.perf_counter() - t0
+        while k < n and arrivals[k] <= now:
+            scene, cfg = sc[picks[k]]
+            w, h = sizes[res[k]]
+            p = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], int(spps[k]), max_passes=7, seed=k)
+            rid = srv.submit(scene, p)
+            meta[rid] = (k, w * h * int(spps[k]))
+            k += 1
+        for e in srv.poll(timeout_ms=1):
+            if e.type == P.EVENT_PASS_COMPLETE and e.pass_number == 1:
+                ttfp[e.request_id] = e.t_ms
+            elif e.type in (P.EVENT_DONE, P.EVENT_ERROR, P.EVENT_CANCELLED):
+                done_t[e.request_id] = e.t_ms
+                samples += meta[e.request_id][1] if e.type == P.EVENT_DONE else 0
+                srv.release(e.request_id)
+    makespan = time.perf_counter() - t0
+    srv.close()
+    t = np.array(list(ttfp.values()))
+    d = np.array(list(done_t.values()))
+    out = {"requests": n, "arrival_rate_per_s": 20, "ttfp_ms_p50": round(float(np.percentile(t, 50)), 2),
+           "ttfp_ms_p99": round(float(np.percentile(t, 99)), 2), "ttfp_ms_max": round(float(t.max()), 2),
+           "done_s_p50": round(float(np.percentile(d, 50)) / 1e3, 3),
+           "done_s_p99": round(float(np.percentile(d, 99)) / 1e3, 3), "makespan_s": round(makespan, 2),
+           "samples_per_s": round(samples / makespan, 1), "integrators": "path tracing only"}
+    log(f"serving: {out}")
+    return out
 
 
 if __name__ == "__main__":

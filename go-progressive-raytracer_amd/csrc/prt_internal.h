@@ -37,10 +37,14 @@ struct prt_scene {
     uint32_t gstack_slots = 0;   // thread slots covered by gstack
     int32_t max_stack = 0;
     int32_t num_cus = 0;
+    int trace_variant[2] = {-1, -1};  // autotuned kernel variant (closest, any)
+    int forced_variant = -1;          // PRT_TRACE_VARIANT override
+    int variant_blocks[8] = {0};
     std::vector<uint32_t> orig_id;  // host copy: leaf order -> original primitive id
     // float64 camera/scene facts needed by the host-side renderer
     int32_t image_width = 0, image_height = 0;
     double aspect = 1.0;
+    prt_camera cam_desc{};
 };
 
 namespace prt {
@@ -48,4 +52,15 @@ namespace prt {
 int scene_alloc(prt_scene* s, size_t bytes, const void* src, void** out);
 // Grid size used by persistent kernels (work-group count).
 int persistent_blocks(const prt_scene* s, int block, int per_cu);
+// NewCamera (camera.go:70-170) in float64, rounded to float32; force_height > 0 overrides
+// int(width / aspect).
+DevCamera make_camera(const prt_camera& c, int& height, int force_height);
+
+// Film internals shared with the serving loop.
+int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t capacity_cap, prt_film** out);
+size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int64_t capacity_cap);
+int film_enqueue(prt_film* f, int32_t spp);  // enqueue spp more samples per pixel, no wait
+bool film_idle(prt_film* f);                 // all enqueued work finished
+int film_wait(prt_film* f, float* ms);       // wait for enqueued work; device time of the last enqueue
+int32_t film_spp_done(const prt_film* f);
 }  // namespace prt

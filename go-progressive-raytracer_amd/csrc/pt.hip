@@ -312,12 +312,14 @@ using namespace prt;
 
 struct prt_film {
     const prt_scene* scene = nullptr;
+    DevScene sc{};  // the scene view with this film's camera
     prt_render_params params{};
     int32_t width = 0, height = 0;
     int64_t npix = 0;
     int64_t capacity = 0;  // path slots
-    int32_t spp_done = 0;
+    int32_t spp_done = 0;  // samples per pixel enqueued so far
     std::vector<void*> bufs;
+    size_t bytes = 0;
     PathBuffers pb{};
     FilmBuffers fb{};
     uint2* gstack = nullptr;
@@ -327,21 +329,47 @@ struct prt_film {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+namespace prt {
+
 namespace {
 
 int film_alloc(prt_film* f, size_t bytes, void** out) {
     void* p = nullptr;
     PRT_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
     f->bufs.push_back(p);
+    f->bytes += std::max<size_t>(bytes, 16);
     *out = p;
     return PRT_OK;
 }
 
-// Renders `spp` samples per pixel for every pixel, starting at sample index spp_done.
-int render_samples(prt_film* f, int32_t spp, float* ms) {
+constexpr size_t kPathBytes = 9 * sizeof(float4) + 5 * sizeof(uint32_t);
+constexpr size_t kPixelBytes = 2 * sizeof(float4) + sizeof(float2) + sizeof(uint32_t);
+
+int64_t film_capacity(int64_t npix, int64_t cap) {
+    return std::max<int64_t>(std::min<int64_t>(npix * 64, cap), std::min<int64_t>(npix, cap));
+}
+
+int film_blocks(const prt_scene* scene) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace<false, true>, kBlock, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    per_cu = std::min(per_cu, 8);
+    return std::max(1, scene->num_cus) * per_cu;
+}
+
+size_t film_gstack_entries(const prt_scene* scene) {
+    return scene->max_stack > kLds ? (size_t)(scene->max_stack - kLds + 1) : 1;
+}
+
+}  // namespace
+
+// Enqueues `spp` samples per pixel for every pixel on the film's stream (no wait).
+int film_enqueue(prt_film* f, int32_t spp) {
     const prt_scene* s = f->scene;
     PRT_HIP(hipSetDevice(s->device));
-    const DevScene& sc = s->dev;
+    const DevScene& sc = f->sc;
     hipStream_t st = f->stream;
     PRT_HIP(hipEventRecord(f->ev0, st));
     int64_t done = 0;
@@ -380,26 +408,40 @@ int render_samples(prt_film* f, int32_t spp, float* ms) {
                     hipLaunchKernelGGL((k_trace<true, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
                                        f->gstack, f->gslots);
             }
-            int film_blocks = (pp.npix + kBlock - 1) / kBlock;
-            hipLaunchKernelGGL(k_film, dim3(film_blocks), dim3(kBlock), 0, st, pp, f->pb, f->fb);
+            int film_blocks_n = (pp.npix + kBlock - 1) / kBlock;
+            hipLaunchKernelGGL(k_film, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb);
             PRT_HIP(hipGetLastError());
         }
         done += chunk_spp;
     }
     PRT_HIP(hipEventRecord(f->ev1, st));
-    PRT_HIP(hipEventSynchronize(f->ev1));
-    float t = 0;
-    PRT_HIP(hipEventElapsedTime(&t, f->ev0, f->ev1));
-    if (ms) *ms = t;
     f->spp_done += spp;
     return PRT_OK;
 }
 
-}  // namespace
+bool film_idle(prt_film* f) { return hipEventQuery(f->ev1) != hipErrorNotReady; }
 
-extern "C" {
+int film_wait(prt_film* f, float* ms) {
+    PRT_HIP(hipSetDevice(f->scene->device));
+    PRT_HIP(hipEventSynchronize(f->ev1));
+    float t = 0;
+    PRT_HIP(hipEventElapsedTime(&t, f->ev0, f->ev1));
+    if (ms) *ms = t;
+    return PRT_OK;
+}
 
-int prt_film_create(const prt_scene* scene, const prt_render_params* p, prt_film** out) {
+int32_t film_spp_done(const prt_film* f) { return f->spp_done; }
+
+size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int64_t cap) {
+    int32_t w = p->width > 0 ? p->width : scene->image_width;
+    int32_t h = p->height > 0 ? p->height : scene->image_height;
+    int64_t npix = (int64_t)w * h;
+    int64_t C = film_capacity(npix, cap);
+    size_t g = film_gstack_entries(scene) * (size_t)film_blocks(scene) * kBlock * sizeof(uint2);
+    return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + 3 * sizeof(uint32_t) * (size_t)C;
+}
+
+int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t cap, prt_film** out) {
     if (!scene || !p || !out) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (p->integrator != PRT_INTEGRATOR_PATH && p->integrator != PRT_INTEGRATOR_BDPT)
@@ -410,17 +452,25 @@ int prt_film_create(const prt_scene* scene, const prt_render_params* p, prt_film
         return fail(PRT_ERR_INVALID_ARGUMENT, "bad pass schedule");
     int32_t w = p->width > 0 ? p->width : scene->image_width;
     int32_t h = p->height > 0 ? p->height : scene->image_height;
-    if (w != scene->image_width || h != scene->image_height)
-        return fail(PRT_ERR_INVALID_ARGUMENT, "film size must match the camera image size");
-    if (w <= 0 || h <= 0) return fail(PRT_ERR_INVALID_ARGUMENT, "empty image");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 28)) return fail(PRT_ERR_INVALID_ARGUMENT, "bad image size");
     PRT_HIP(hipSetDevice(scene->device));
     prt_film* f = new prt_film();
     f->scene = scene;
+    f->sc = scene->dev;
+    if (w != scene->image_width || h != scene->image_height) {
+        // image-size override: camera width and aspect from the requested size (as the web server's
+        // width/height parameters override the scene camera, web/server/render.go:371-404)
+        prt_camera c = scene->cam_desc;
+        c.width = w;
+        c.aspect_ratio = (double)w / (double)h;
+        int H = 0;
+        f->sc.cam = make_camera(c, H, h);
+    }
     f->params = *p;
     f->width = w;
     f->height = h;
     f->npix = (int64_t)w * h;
-    f->capacity = std::max<int64_t>(std::min<int64_t>(f->npix * 64, (int64_t)1 << 23), std::min<int64_t>(f->npix, (int64_t)1 << 23));
+    f->capacity = film_capacity(f->npix, cap);
     int rc;
     auto bail = [&](int code) {
         prt_film_destroy(f);
@@ -428,7 +478,7 @@ int prt_film_create(const prt_scene* scene, const prt_render_params* p, prt_film
     };
     const int64_t C = f->capacity;
     void* ptr;
-#define FALLOC(field, type, count)                                                      \
+#define FALLOC(field, type, count)                                                               \
     if ((rc = film_alloc(f, sizeof(type) * (size_t)(count), &ptr)) != PRT_OK) return bail(rc); \
     field = (type*)ptr;
     FALLOC(f->pb.o, float4, C);
@@ -451,16 +501,9 @@ int prt_film_create(const prt_scene* scene, const prt_render_params* p, prt_film
     FALLOC(f->fb.sq, float2, f->npix);
     FALLOC(f->fb.count, uint32_t, f->npix);
 #undef FALLOC
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace<false, true>, kBlock, 0) !=
-            hipSuccess ||
-        per_cu <= 0)
-        per_cu = 4;
-    per_cu = std::min(per_cu, 8);
-    f->blocks = std::max(1, scene->num_cus) * per_cu;
+    f->blocks = film_blocks(scene);
     f->gslots = (uint32_t)f->blocks * kBlock;
-    size_t depth = (size_t)std::max(1, scene->max_stack - kLds + 1);
-    if ((rc = film_alloc(f, depth * f->gslots * sizeof(uint2), &ptr)) != PRT_OK) return bail(rc);
+    if ((rc = film_alloc(f, film_gstack_entries(scene) * f->gslots * sizeof(uint2), &ptr)) != PRT_OK) return bail(rc);
     f->gstack = (uint2*)ptr;
     hipError_t e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
@@ -470,6 +513,24 @@ int prt_film_create(const prt_scene* scene, const prt_render_params* p, prt_film
     if ((rc = prt_film_reset(f)) != PRT_OK) return bail(rc);
     *out = f;
     return PRT_OK;
+}
+
+}  // namespace prt
+
+namespace {
+constexpr int64_t kDefaultCapacity = (int64_t)1 << 23;
+
+int render_samples(prt_film* f, int32_t spp, float* ms) {
+    int rc = film_enqueue(f, spp);
+    if (rc != PRT_OK) return rc;
+    return film_wait(f, ms);
+}
+}  // namespace
+
+extern "C" {
+
+int prt_film_create(const prt_scene* scene, const prt_render_params* p, prt_film** out) {
+    return film_create_ex(scene, p, kDefaultCapacity, out);
 }
 
 int prt_film_reset(prt_film* f) {
@@ -541,9 +602,11 @@ int prt_film_read(const prt_film* f, float* mean_rgb, float* lum_sum, float* lum
     std::vector<float4> sum(f->npix);
     std::vector<float2> sq(f->npix);
     std::vector<uint32_t> cnt(f->npix);
-    PRT_HIP(hipMemcpy(sum.data(), f->fb.sum, sizeof(float4) * f->npix, hipMemcpyDeviceToHost));
-    PRT_HIP(hipMemcpy(sq.data(), f->fb.sq, sizeof(float2) * f->npix, hipMemcpyDeviceToHost));
-    PRT_HIP(hipMemcpy(cnt.data(), f->fb.count, sizeof(uint32_t) * f->npix, hipMemcpyDeviceToHost));
+    // ordered after the work already enqueued on the film's stream
+    PRT_HIP(hipMemcpyAsync(sum.data(), f->fb.sum, sizeof(float4) * f->npix, hipMemcpyDeviceToHost, f->stream));
+    PRT_HIP(hipMemcpyAsync(sq.data(), f->fb.sq, sizeof(float2) * f->npix, hipMemcpyDeviceToHost, f->stream));
+    PRT_HIP(hipMemcpyAsync(cnt.data(), f->fb.count, sizeof(uint32_t) * f->npix, hipMemcpyDeviceToHost, f->stream));
+    PRT_HIP(hipStreamSynchronize(f->stream));
     for (int64_t i = 0; i < f->npix; i++) {
         if (mean_rgb) {
             float inv = cnt[i] ? 1.0f / (float)cnt[i] : 0.0f;  // GetColor: zero when no samples

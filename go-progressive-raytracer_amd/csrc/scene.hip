@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -95,8 +96,11 @@ float bits(uint32_t u) {
     return f;
 }
 
-// NewCamera (pkg/geometry/camera.go:70-170), float64 then rounded to float32.
-DevCamera make_camera(const prt_camera& c, int& height) {
+}  // namespace
+
+// NewCamera (pkg/geometry/camera.go:70-170), float64 then rounded to float32. force_height > 0
+// replaces int(width / aspect) (used when a film overrides the image size).
+DevCamera make_camera(const prt_camera& c, int& height, int force_height) {
     const double pi = 3.14159265358979323846;
     D3 C = d3(c.center), L = d3(c.look_at), U = d3(c.up);
     D3 w = normalize(C - L);
@@ -108,7 +112,7 @@ DevCamera make_camera(const prt_camera& c, int& height) {
     double la = pi * lr * lr;
     if (c.aperture == 0) la = 1.0;
     int W = c.width;
-    int H = (int)((double)W / c.aspect_ratio);
+    int H = force_height > 0 ? force_height : (int)((double)W / c.aspect_ratio);
     double theta = c.vfov * pi / 180.0;
     double hh = std::tan(theta / 2.0);
     double vh = 2.0 * hh * focus, vw = vh * c.aspect_ratio;
@@ -152,6 +156,7 @@ DevCamera make_camera(const prt_camera& c, int& height) {
     return k;
 }
 
+namespace {
 }  // namespace
 
 int scene_alloc(prt_scene* s, size_t bytes, const void* src, void** out) {
@@ -227,6 +232,104 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect(DevScene sc, const pr
     }
 }
 
+// Persistent closest/any-hit kernel with lane refill: a lane whose ray is finished takes a new
+// ray from the global work counter as soon as REFILL lanes of its wave are idle (REFILL = 64
+// degenerates to fetching whole 64-ray batches), so long rays do not idle the rest of the wave.
+template <bool ANY, bool MIXED, int LDS, int REFILL>
+__global__ __launch_bounds__(kTraceBlock) void k_intersect_refill(DevScene sc, const prt_ray* __restrict__ rays,
+                                                                  int64_t n, prt_hit* __restrict__ hits,
+                                                                  uint8_t* __restrict__ occluded, uint2* gstack,
+                                                                  uint32_t gstride, unsigned long long* work) {
+    __shared__ uint32_t s_node[LDS * kTraceBlock];
+    __shared__ float s_t[LDS * kTraceBlock];
+    TStack<LDS, kTraceBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kTraceBlock + threadIdx.x;
+    st.gstride = gstride;
+    const uint32_t lane = threadIdx.x & 63u;
+    TState s;
+    bool active = false;
+    bool exhausted = false;  // wave-uniform
+    int64_t ridx = 0;
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        int nidle = __popcll(idle);
+        if (!exhausted && (nidle >= REFILL || nidle == 64)) {
+            uint32_t leader = __ffsll((unsigned long long)idle) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(work, (unsigned long long)nidle);
+            base = __shfl(base, leader, 64);
+            if (base + (unsigned long long)nidle >= (unsigned long long)n) exhausted = true;
+            if (!active) {
+                uint32_t rank = __popcll(idle & ((1ull << lane) - 1ull));
+                int64_t i = (int64_t)base + rank;
+                if (i < n) {
+                    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+                    float4 a = rp[0], b = rp[1];
+                    tstate_init(s, TRay{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w});
+                    ridx = i;
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && tstate_step<ANY, MIXED, false, LDS, kTraceBlock>(sc, s, st, nullptr)) {
+            active = false;
+            bool hit = s.best != 0xFFFFFFFFu;
+            if (ANY) {
+                occluded[ridx] = hit ? 1 : 0;
+            } else {
+                int32_t pid = hit ? (int32_t)__float_as_uint(sc.prims[3 * (size_t)s.best + 2].w) : -1;
+                reinterpret_cast<float4*>(hits)[ridx] =
+                    make_float4(hit ? s.tbest : __int_as_float(0x7f800000), __int_as_float(pid),
+                                hit ? s.bu : 0.0f, hit ? s.bv : 0.0f);
+            }
+        }
+    }
+}
+
+// Kernel variants the per-scene autotuner chooses from (identical per-ray traversal).
+struct TraceVariant {
+    int lds;
+    int refill;
+};
+constexpr TraceVariant kVariants[] = {{16, 64}, {8, 64}, {8, 16}, {8, 32}, {8, 8}, {16, 16}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+constexpr int kMinLds = 8;
+constexpr int kDefaultVariant = 2;
+
+template <bool ANY, bool MIXED>
+const void* variant_fn(int v) {
+    switch (v) {
+        case 0: return (const void*)k_intersect_refill<ANY, MIXED, 16, 64>;
+        case 1: return (const void*)k_intersect_refill<ANY, MIXED, 8, 64>;
+        case 2: return (const void*)k_intersect_refill<ANY, MIXED, 8, 16>;
+        case 3: return (const void*)k_intersect_refill<ANY, MIXED, 8, 32>;
+        case 4: return (const void*)k_intersect_refill<ANY, MIXED, 8, 8>;
+        default: return (const void*)k_intersect_refill<ANY, MIXED, 16, 16>;
+    }
+}
+
+template <bool ANY, bool MIXED>
+void launch_variant(int v, int blocks, hipStream_t st, const DevScene& sc, const prt_ray* rays, int64_t n,
+                    prt_hit* hits, uint8_t* occ, uint2* gs, uint32_t gstride, unsigned long long* work) {
+    dim3 g(blocks), b(kTraceBlock);
+    switch (v) {
+        case 0: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 16, 64>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
+        case 1: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 8, 64>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
+        case 2: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 8, 16>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
+        case 3: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 8, 32>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
+        case 4: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 8, 8>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
+        default: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 16, 16>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
+    }
+}
+
 }  // namespace prt
 
 using namespace prt;
@@ -276,33 +379,71 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
     std::lock_guard<std::mutex> lock(g_scratch_mu);
     // scene scratch (work counter + overflow stack) is shared: order launches on it
     unsigned long long* work = (unsigned long long*)scene->bufs[0].p;
-    hipError_t e = hipMemsetAsync(work, 0, sizeof(unsigned long long), st);
-    if (e != hipSuccess) { cleanup(); return hip_fail(e, "hipMemsetAsync(work)"); }
-    int blocks = (int)(scene->gstack_slots / kTraceBlock);
+    hipError_t e;
     const DevScene& sc = scene->dev;
     uint2* gs = scene->gstack;
     uint32_t gstride = scene->gstack_slots;
-    if (mode == 0) {
-        if (sc.mixed)
-            hipLaunchKernelGGL((k_intersect<false, true, false>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
-                               (prt_hit*)d_out0, nullptr, nullptr, nullptr, nullptr, gs, gstride, work);
-        else
-            hipLaunchKernelGGL((k_intersect<false, false, false>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
-                               (prt_hit*)d_out0, nullptr, nullptr, nullptr, nullptr, gs, gstride, work);
-    } else if (mode == 1) {
-        if (sc.mixed)
-            hipLaunchKernelGGL((k_intersect<true, true, false>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
-                               nullptr, (uint8_t*)d_out0, nullptr, nullptr, nullptr, gs, gstride, work);
-        else
-            hipLaunchKernelGGL((k_intersect<true, false, false>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
-                               nullptr, (uint8_t*)d_out0, nullptr, nullptr, nullptr, gs, gstride, work);
-    } else {
+    prt_scene* ms = const_cast<prt_scene*>(scene);
+    auto run = [&](int v, int64_t count) -> hipError_t {
+        hipError_t er = hipMemsetAsync(work, 0, sizeof(unsigned long long), st);
+        if (er != hipSuccess) return er;
+        int blocks = ms->variant_blocks[v];
+        if (mode == 0) {
+            if (sc.mixed) launch_variant<false, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work);
+            else launch_variant<false, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work);
+        } else {
+            if (sc.mixed) launch_variant<true, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work);
+            else launch_variant<true, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work);
+        }
+        return hipGetLastError();
+    };
+    if (mode == 2) {
+        e = hipMemsetAsync(work, 0, sizeof(unsigned long long), st);
+        if (e != hipSuccess) { cleanup(); return hip_fail(e, "hipMemsetAsync(work)"); }
+        int blocks = (int)(scene->gstack_slots / kTraceBlock);
+        blocks = std::min(blocks, std::max(1, scene->num_cus) * 4);
         if (sc.mixed)
             hipLaunchKernelGGL((k_intersect<false, true, true>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
                                nullptr, nullptr, (uint32_t*)d_out0, d_c1, d_c2, gs, gstride, work);
         else
             hipLaunchKernelGGL((k_intersect<false, false, true>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, (const prt_ray*)d_rays, n,
                                nullptr, nullptr, (uint32_t*)d_out0, d_c1, d_c2, gs, gstride, work);
+    } else {
+        int& tuned = ms->trace_variant[mode];
+        if (tuned < 0 && n >= (1 << 20)) {
+            // autotune: time every variant on the first 1M rays of this batch, keep the fastest
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            float best = 1e30f;
+            int bestv = kDefaultVariant;
+            const int64_t m = 1 << 20;
+            for (int v = 0; v < kNumVariants; v++) {
+                float tot = 0;
+                for (int rep = 0; rep < 3; rep++) {
+                    (void)hipEventRecord(a, st);
+                    e = run(v, m);
+                    (void)hipEventRecord(b, st);
+                    (void)hipEventSynchronize(b);
+                    float ms_v = 0;
+                    (void)hipEventElapsedTime(&ms_v, a, b);
+                    if (rep > 0) tot += ms_v;
+                }
+                if (e != hipSuccess) break;
+                if (tot < best) {
+                    best = tot;
+                    bestv = v;
+                }
+            }
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            if (e != hipSuccess) { cleanup(); return hip_fail(e, "intersect autotune"); }
+            tuned = bestv;
+            ms->info.trace_variant = bestv;
+        }
+        int v = ms->forced_variant >= 0 ? ms->forced_variant : (tuned >= 0 ? tuned : kDefaultVariant);
+        e = run(v, n);
+        if (e != hipSuccess) { cleanup(); return hip_fail(e, "intersect kernel launch"); }
     }
     e = hipGetLastError();
     if (e != hipSuccess) { cleanup(); return hip_fail(e, "intersect kernel launch"); }
@@ -652,19 +793,34 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     for (int a = 0; a < 3; a++) s->dev.world_center[a] = (float)s->info.world_center[a];
     s->dev.world_radius = (float)s->info.world_radius;
     int H = 0;
-    s->dev.cam = make_camera(d->camera, H);
+    s->dev.cam = make_camera(d->camera, H, 0);
+    s->cam_desc = d->camera;
     s->image_width = d->camera.width;
     s->image_height = H;
     s->aspect = d->camera.aspect_ratio;
 
-    // overflow stack for the intersection kernels: one slot per resident thread
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_intersect<false, true, false>,
-                                                     kTraceBlock, 0) != hipSuccess || per_cu <= 0)
-        per_cu = 4;
-    per_cu = std::min(per_cu, 8);
-    uint32_t slots = (uint32_t)(s->num_cus * per_cu * kTraceBlock);
-    size_t depth = (size_t)std::max(1, s->max_stack - kTraceLds + 1);
+    // per-variant grid (occupancy-limited, at most 12 work-groups per CU) and one overflow-stack
+    // slot per resident thread of the largest grid
+    int max_blocks = 1;
+    for (int v = 0; v < kNumVariants; v++) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_fn<false, true>(v), kTraceBlock, 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            per_cu = 4;
+        per_cu = std::min(per_cu, 12);
+        s->variant_blocks[v] = s->num_cus * per_cu;
+        max_blocks = std::max(max_blocks, s->variant_blocks[v]);
+    }
+    s->trace_variant[0] = s->trace_variant[1] = -1;
+    s->forced_variant = -1;
+    if (const char* fv = std::getenv("PRT_TRACE_VARIANT")) {
+        int v = std::atoi(fv);
+        if (v >= 0 && v < kNumVariants) s->forced_variant = v;
+    }
+    s->info.trace_variant = s->forced_variant;
+    uint32_t slots = (uint32_t)(max_blocks * kTraceBlock);
+    size_t depth = (size_t)std::max(1, s->max_stack - kMinLds + 1);
     if ((rc = scene_alloc(s, depth * slots * sizeof(uint2), nullptr, &p)) != PRT_OK) return bail(rc);
     s->gstack = (uint2*)p;
     s->gstack_slots = slots;

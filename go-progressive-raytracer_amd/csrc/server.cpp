@@ -1,0 +1,325 @@
+// server.cpp — continuous-batching serving loop for progressive renders (include/prt_server.h).
+//
+// One scheduler thread owns admission and pass sequencing. Each admitted request has a film slot with
+// its own HIP stream; the scheduler enqueues one pass at a time per request and polls the pass's
+// completion event, so passes of different requests overlap on the device while each request's passes
+// stay ordered (RenderProgressive's pass loop, pkg/renderer/progressive.go:286-368).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/prt_server.h"
+#include "prt_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum class St { Queued, Running, Done, Cancelled, Failed };
+
+struct Req {
+    int64_t id = 0;
+    const prt_scene* scene = nullptr;
+    prt_render_params params{};
+    prt_film* film = nullptr;
+    St state = St::Queued;
+    int32_t pass = 0;            // last enqueued pass
+    int32_t completed_pass = 0;  // last pass reported complete
+    int32_t target = 0;          // samples per pixel of the enqueued pass
+    size_t bytes = 0;
+    Clock::time_point t_submit;
+    bool cancel = false;
+    bool release = false;
+    int readers = 0;
+};
+
+// Film slots of serving requests hold at most this many paths, so one request's kernels stay short
+// and passes of concurrent requests interleave on the device.
+constexpr int64_t kServeCapacity = (int64_t)1 << 21;
+
+}  // namespace
+
+struct prt_server {
+    int device = 0;
+    int64_t budget = 0;
+    int max_active = 1;
+    int64_t used = 0;
+    int64_t completed = 0;
+    int64_t next_id = 1;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_events;
+    std::deque<Req*> queue;
+    std::map<int64_t, Req*> reqs;
+    std::vector<Req*> active;
+    std::deque<prt_server_event> events;
+    std::thread worker;
+    bool stop = false;
+
+    double since(const Req* r) const {
+        return std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
+    }
+    void emit(const Req* r, int type, int32_t pass, int32_t target, int err, double dev_ms) {
+        prt_server_event e{};
+        e.request_id = r->id;
+        e.type = type;
+        e.pass_number = pass;
+        e.target_samples = target;
+        e.error_code = err;
+        e.t_ms = since(r);
+        e.device_ms = dev_ms;
+        events.push_back(e);
+        cv_events.notify_all();
+    }
+    void free_film(Req* r) {
+        if (r->film) {
+            prt_film_destroy(r->film);
+            r->film = nullptr;
+            used -= (int64_t)r->bytes;
+        }
+    }
+    // enqueue the next pass of an active request; false on error (event emitted)
+    bool next_pass(Req* r) {
+        int32_t target = 0;
+        int rc = prt_samples_for_pass(&r->params, r->pass + 1, &target);
+        if (rc == PRT_OK) {
+            int32_t need = target - prt::film_spp_done(r->film);
+            r->pass++;
+            r->target = target;
+            if (need > 0) rc = prt::film_enqueue(r->film, need);
+        }
+        if (rc != PRT_OK) {
+            r->state = St::Failed;
+            emit(r, PRT_EVENT_ERROR, r->pass, 0, rc, 0);
+            return false;
+        }
+        return true;
+    }
+    void run() {
+        (void)hipSetDevice(device);
+        std::unique_lock<std::mutex> lk(mu);
+        while (!stop) {
+            bool progressed = false;
+            // ---- admission (FIFO, HBM budget, slot count) ----
+            while (!queue.empty() && (int)active.size() < max_active) {
+                Req* r = queue.front();
+                if (r->cancel) {
+                    queue.pop_front();
+                    r->state = St::Cancelled;
+                    emit(r, PRT_EVENT_CANCELLED, 0, 0, 0, 0);
+                    continue;
+                }
+                if ((int64_t)r->bytes > budget) {
+                    queue.pop_front();
+                    r->state = St::Failed;
+                    emit(r, PRT_EVENT_ERROR, 0, 0, PRT_ERR_OUT_OF_MEMORY, 0);
+                    continue;
+                }
+                if (used + (int64_t)r->bytes > budget) break;  // wait for memory to free up
+                queue.pop_front();
+                int rc = prt::film_create_ex(r->scene, &r->params, kServeCapacity, &r->film);
+                if (rc != PRT_OK) {
+                    r->state = St::Failed;
+                    emit(r, PRT_EVENT_ERROR, 0, 0, rc, 0);
+                    continue;
+                }
+                used += (int64_t)r->bytes;
+                r->state = St::Running;
+                active.push_back(r);
+                if (next_pass(r)) emit(r, PRT_EVENT_ADMITTED, r->pass, r->target, 0, 0);
+                progressed = true;
+            }
+            // ---- pass completion and sequencing ----
+            for (size_t k = 0; k < active.size();) {
+                Req* r = active[k];
+                bool remove = false;
+                if (r->state != St::Running) {
+                    remove = true;
+                } else if (r->pass > r->completed_pass && prt::film_idle(r->film)) {
+                    float ms = 0;
+                    (void)prt::film_wait(r->film, &ms);
+                    r->completed_pass = r->pass;
+                    emit(r, PRT_EVENT_PASS_COMPLETE, r->pass, r->target, 0, ms);
+                    progressed = true;
+                    if (r->cancel) {
+                        r->state = St::Cancelled;
+                        emit(r, PRT_EVENT_CANCELLED, r->pass, r->target, 0, 0);
+                        remove = true;
+                    } else if (r->pass >= r->params.max_passes ||
+                               prt::film_spp_done(r->film) >= r->params.max_samples_per_pixel) {
+                        r->state = St::Done;
+                        completed++;
+                        emit(r, PRT_EVENT_DONE, r->pass, r->target, 0, 0);
+                        remove = true;
+                    } else if (!next_pass(r)) {
+                        remove = true;
+                    }
+                }
+                if (remove) {
+                    active.erase(active.begin() + k);
+                } else {
+                    k++;
+                }
+            }
+            // ---- deferred releases ----
+            for (auto it = reqs.begin(); it != reqs.end();) {
+                Req* r = it->second;
+                bool finished = r->state == St::Done || r->state == St::Cancelled || r->state == St::Failed;
+                if (r->release && finished && r->readers == 0) {
+                    if (r->film) (void)prt::film_wait(r->film, nullptr);
+                    free_film(r);
+                    delete r;
+                    it = reqs.erase(it);
+                    progressed = true;
+                } else {
+                    ++it;
+                }
+            }
+            if (!progressed) cv_work.wait_for(lk, std::chrono::microseconds(100));
+        }
+    }
+};
+
+using namespace prt;
+
+extern "C" {
+
+int prt_server_create(int32_t device, int64_t budget, int32_t max_active, prt_server** out) {
+    if (!out || budget <= 0 || max_active < 1) return fail(PRT_ERR_INVALID_ARGUMENT, "bad server arguments");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+        return fail(PRT_ERR_NO_DEVICE, "no such HIP device");
+    prt_server* s = new prt_server();
+    s->device = device;
+    s->budget = budget;
+    s->max_active = max_active;
+    s->worker = std::thread([s]() { s->run(); });
+    *out = s;
+    return PRT_OK;
+}
+
+int prt_server_submit(prt_server* s, const prt_scene* scene, const prt_render_params* p, int64_t* id) {
+    if (!s || !scene || !p || !id) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (scene->device != s->device) return fail(PRT_ERR_INVALID_ARGUMENT, "scene lives on another device");
+    int32_t t = 0;
+    int rc = prt_samples_for_pass(p, 1, &t);
+    if (rc != PRT_OK) return rc;
+    if (p->integrator != PRT_INTEGRATOR_PATH) return fail(PRT_ERR_UNSUPPORTED, "only path tracing is served");
+    Req* r = new Req();
+    r->scene = scene;
+    r->params = *p;
+    r->bytes = film_device_bytes(scene, p, kServeCapacity);
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        s->reqs[r->id] = r;
+        s->queue.push_back(r);
+    }
+    s->cv_work.notify_all();
+    *id = r->id;
+    return PRT_OK;
+}
+
+int prt_server_poll(prt_server* s, prt_server_event* ev, int32_t max_events, int32_t timeout_ms, int32_t* count) {
+    if (!s || !count || (max_events > 0 && !ev)) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->events.empty() && timeout_ms > 0)
+        s->cv_events.wait_for(lk, std::chrono::milliseconds(timeout_ms), [s]() { return !s->events.empty(); });
+    int32_t n = 0;
+    while (n < max_events && !s->events.empty()) {
+        ev[n++] = s->events.front();
+        s->events.pop_front();
+    }
+    *count = n;
+    return PRT_OK;
+}
+
+int prt_server_cancel(prt_server* s, int64_t id) {
+    if (!s) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(PRT_ERR_INVALID_ARGUMENT, "unknown request");
+    it->second->cancel = true;
+    s->cv_work.notify_all();
+    return PRT_OK;
+}
+
+static int with_film(prt_server* s, int64_t id, const std::function<int(prt_film*)>& fn) {
+    Req* r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) return fail(PRT_ERR_INVALID_ARGUMENT, "unknown request");
+        r = it->second;
+        if (!r->film) return fail(PRT_ERR_INVALID_ARGUMENT, "request has no film (queued, failed or released)");
+        r->readers++;
+    }
+    int rc = fn(r->film);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->readers--;
+    }
+    s->cv_work.notify_all();
+    return rc;
+}
+
+int prt_server_read(prt_server* s, int64_t id, float* mean_rgb, uint32_t* counts) {
+    if (!s) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL server");
+    return with_film(s, id, [&](prt_film* f) { return prt_film_read(f, mean_rgb, nullptr, nullptr, counts); });
+}
+
+int prt_server_read_rgba8(prt_server* s, int64_t id, int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint8_t* rgba) {
+    if (!s) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL server");
+    return with_film(s, id, [&](prt_film* f) { return prt_film_read_rgba8(f, x0, y0, x1, y1, rgba); });
+}
+
+int prt_server_release(prt_server* s, int64_t id) {
+    if (!s) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(PRT_ERR_INVALID_ARGUMENT, "unknown request");
+    it->second->release = true;
+    if (it->second->state == St::Queued || it->second->state == St::Running) it->second->cancel = true;
+    s->cv_work.notify_all();
+    return PRT_OK;
+}
+
+int prt_server_stats_get(prt_server* s, prt_server_stats* st) {
+    if (!s || !st) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    st->queued = (int32_t)s->queue.size();
+    st->active = (int32_t)s->active.size();
+    st->completed = s->completed;
+    st->budget_bytes = s->budget;
+    st->used_bytes = s->used;
+    return PRT_OK;
+}
+
+int prt_server_destroy(prt_server* s) {
+    if (!s) return PRT_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (auto& kv : s->reqs) {
+        Req* r = kv.second;
+        if (r->film) (void)prt::film_wait(r->film, nullptr);
+        s->free_film(r);
+        delete r;
+    }
+    delete s;
+    return PRT_OK;
+}
+
+}  // extern "C"

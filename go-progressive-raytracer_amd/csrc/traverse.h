@@ -170,131 +170,149 @@ struct TStack {
     }
 };
 
-// Closest hit (ANY = false) or any hit (ANY = true). Returns true on a hit.
+// Per-lane traversal state; the traversal advances one node or one leaf per step so that a
+// lane that finishes can take a new ray while the rest of its wave keeps traversing.
+struct TState {
+    TRay r;
+    float idx, idy, idz;
+    WRay w;
+    float tbest, bu, bv;
+    uint32_t best;
+    uint32_t cur;
+    int sp;
+};
+
+__device__ __forceinline__ void tstate_init(TState& s, const TRay& r_in) {
+    s.r = r_in;
+    if (!(s.r.tmin >= 0.0f)) s.r.tmin = 0.0f;
+    // reciprocal direction; zero components become huge so that 0 * inf never occurs
+    s.idx = 1.0f / (fabsf(s.r.dx) > 1e-30f ? s.r.dx : copysignf(1e-30f, s.r.dx));
+    s.idy = 1.0f / (fabsf(s.r.dy) > 1e-30f ? s.r.dy : copysignf(1e-30f, s.r.dy));
+    s.idz = 1.0f / (fabsf(s.r.dz) > 1e-30f ? s.r.dz : copysignf(1e-30f, s.r.dz));
+    s.w = make_wray(s.r.dx, s.r.dy, s.r.dz);
+    s.tbest = s.r.tmax;
+    s.best = 0xFFFFFFFFu;
+    s.bu = s.bv = 0.0f;
+    s.sp = 0;
+    s.cur = 0;  // root node
+}
+
+__device__ __forceinline__ void tstate_result(const TState& s, THit& hit) {
+    hit.t = s.best == 0xFFFFFFFFu ? __int_as_float(0x7f800000) : s.tbest;
+    hit.prim = s.best;
+    hit.u = s.bu;
+    hit.v = s.bv;
+}
+
+// One traversal step. Returns true when the ray is finished (stack empty, or any hit found).
+template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK>
+__device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStack<LDS_DEPTH, BLOCK>& st,
+                                            TCounts* cnt) {
+    const TRay& r = s.r;
+    if (!child_is_leaf(s.cur)) {
+        // ---- inner node: test the 4 child boxes ----
+        const float4* np = reinterpret_cast<const float4*>(sc.nodes + s.cur);
+        float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+        if (COUNT) {
+            cnt->nodes++;
+            cnt->boxes += 4;
+        }
+        const bool px = s.idx >= 0.0f, py = s.idy >= 0.0f, pz = s.idz >= 0.0f;
+        const float idx = s.idx, idy = s.idy, idz = s.idz;
+        float nx0 = ((px ? lx.x : hx.x) - r.ox) * idx, fx0 = ((px ? hx.x : lx.x) - r.ox) * idx;
+        float nx1 = ((px ? lx.y : hx.y) - r.ox) * idx, fx1 = ((px ? hx.y : lx.y) - r.ox) * idx;
+        float nx2 = ((px ? lx.z : hx.z) - r.ox) * idx, fx2 = ((px ? hx.z : lx.z) - r.ox) * idx;
+        float nx3 = ((px ? lx.w : hx.w) - r.ox) * idx, fx3 = ((px ? hx.w : lx.w) - r.ox) * idx;
+        float ny0 = ((py ? ly.x : hy.x) - r.oy) * idy, fy0 = ((py ? hy.x : ly.x) - r.oy) * idy;
+        float ny1 = ((py ? ly.y : hy.y) - r.oy) * idy, fy1 = ((py ? hy.y : ly.y) - r.oy) * idy;
+        float ny2 = ((py ? ly.z : hy.z) - r.oy) * idy, fy2 = ((py ? hy.z : ly.z) - r.oy) * idy;
+        float ny3 = ((py ? ly.w : hy.w) - r.oy) * idy, fy3 = ((py ? hy.w : ly.w) - r.oy) * idy;
+        float nz0 = ((pz ? lz.x : hz.x) - r.oz) * idz, fz0 = ((pz ? hz.x : lz.x) - r.oz) * idz;
+        float nz1 = ((pz ? lz.y : hz.y) - r.oz) * idz, fz1 = ((pz ? hz.y : lz.y) - r.oz) * idz;
+        float nz2 = ((pz ? lz.z : hz.z) - r.oz) * idz, fz2 = ((pz ? hz.z : lz.z) - r.oz) * idz;
+        float nz3 = ((pz ? lz.w : hz.w) - r.oz) * idz, fz3 = ((pz ? hz.w : lz.w) - r.oz) * idz;
+        float tn0 = fmaxf(fmaxf(nx0, ny0), fmaxf(nz0, r.tmin));
+        float tn1 = fmaxf(fmaxf(nx1, ny1), fmaxf(nz1, r.tmin));
+        float tn2 = fmaxf(fmaxf(nx2, ny2), fmaxf(nz2, r.tmin));
+        float tn3 = fmaxf(fmaxf(nx3, ny3), fmaxf(nz3, r.tmin));
+        float tf0 = fminf(fminf(fminf(fx0, fy0), fz0) * kBoxWiden, s.tbest);
+        float tf1 = fminf(fminf(fminf(fx1, fy1), fz1) * kBoxWiden, s.tbest);
+        float tf2 = fminf(fminf(fminf(fx2, fy2), fz2) * kBoxWiden, s.tbest);
+        float tf3 = fminf(fminf(fminf(fx3, fy3), fz3) * kBoxWiden, s.tbest);
+        // sort keys: non-negative float bits with the slot in the low 2 bits (rounds the
+        // entry distance down, which only weakens culling)
+        uint32_t k0 = (tn0 <= tf0 && ch.x != kEmptyChild) ? ((__float_as_uint(tn0) & ~3u) | 0u) : 0xFFFFFFFFu;
+        uint32_t k1 = (tn1 <= tf1 && ch.y != kEmptyChild) ? ((__float_as_uint(tn1) & ~3u) | 1u) : 0xFFFFFFFFu;
+        uint32_t k2 = (tn2 <= tf2 && ch.z != kEmptyChild) ? ((__float_as_uint(tn2) & ~3u) | 2u) : 0xFFFFFFFFu;
+        uint32_t k3 = (tn3 <= tf3 && ch.w != kEmptyChild) ? ((__float_as_uint(tn3) & ~3u) | 3u) : 0xFFFFFFFFu;
+        // 4-element sorting network (ascending)
+        uint32_t t0 = min(k0, k1), t1 = max(k0, k1), t2 = min(k2, k3), t3 = max(k2, k3);
+        uint32_t s0 = min(t0, t2), s2 = max(t0, t2), s1 = min(t1, t3), s3 = max(t1, t3);
+        uint32_t m1 = min(s1, s2), m2 = max(s1, s2);
+        s1 = m1;
+        s2 = m2;
+        if (s0 != 0xFFFFFFFFu) {
+            if (s3 != 0xFFFFFFFFu) st.push(s.sp++, pick4(ch, s3 & 3u), __uint_as_float(s3 & ~3u));
+            if (s2 != 0xFFFFFFFFu) st.push(s.sp++, pick4(ch, s2 & 3u), __uint_as_float(s2 & ~3u));
+            if (s1 != 0xFFFFFFFFu) st.push(s.sp++, pick4(ch, s1 & 3u), __uint_as_float(s1 & ~3u));
+            s.cur = pick4(ch, s0 & 3u);
+            return false;
+        }
+    } else {
+        // ---- leaf: test its primitives ----
+        uint32_t first = leaf_first(s.cur), count = leaf_count(s.cur);
+        for (uint32_t k = 0; k < count; k++) {
+            const float4* pp = sc.prims + 3 * (size_t)(first + k);
+            float4 a = pp[0];
+            float t, u = 0.0f, v = 0.0f;
+            bool h;
+            if (COUNT) cnt->prims++;
+            if (!MIXED) {
+                float4 b = pp[1], c = pp[2];
+                h = tri_test(r, s.w, a, b, c, s.tbest, t, u, v);
+            } else {
+                uint32_t tag = __float_as_uint(a.w);
+                uint32_t kind = prim_kind(tag);
+                float4 b = pp[1];
+                if (kind == kPrimTriangle) {
+                    float4 c = pp[2];
+                    h = tri_test(r, s.w, a, b, c, s.tbest, t, u, v);
+                } else if (kind == kPrimSphere) {
+                    h = sphere_test(r, a, b, s.tbest, t);
+                } else {
+                    float4 c = pp[2];
+                    h = quad_test(r, a, b, c, sc.quads[prim_side(tag)], s.tbest, t, u, v);
+                }
+            }
+            if (h) {
+                s.tbest = t;
+                s.best = first + k;
+                s.bu = u;
+                s.bv = v;
+                if (ANY) return true;
+            }
+        }
+    }
+    // pop the nearest pending entry that can still hold a closer hit
+    for (;;) {
+        if (s.sp == 0) return true;
+        float tn;
+        st.pop(--s.sp, s.cur, tn);
+        if (tn <= s.tbest) return false;
+    }
+}
+
+// Closest hit (ANY = false) or any hit (ANY = true) of one ray. Returns true on a hit.
 template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK>
 __device__ bool trace(const DevScene& sc, const TRay& r_in, THit& hit, TStack<LDS_DEPTH, BLOCK>& st,
                       TCounts* cnt) {
-    TRay r = r_in;
-    if (!(r.tmin >= 0.0f)) r.tmin = 0.0f;
-    // reciprocal direction; zero components become huge so that 0 * inf never occurs
-    float idx = 1.0f / (fabsf(r.dx) > 1e-30f ? r.dx : copysignf(1e-30f, r.dx));
-    float idy = 1.0f / (fabsf(r.dy) > 1e-30f ? r.dy : copysignf(1e-30f, r.dy));
-    float idz = 1.0f / (fabsf(r.dz) > 1e-30f ? r.dz : copysignf(1e-30f, r.dz));
-    const bool px = idx >= 0.0f, py = idy >= 0.0f, pz = idz >= 0.0f;
-    WRay w = make_wray(r.dx, r.dy, r.dz);
-
-    float tbest = r.tmax;
-    uint32_t best = 0xFFFFFFFFu;
-    float bu = 0.0f, bv = 0.0f;
-    int sp = 0;
-    uint32_t cur = 0;  // root node
-    for (;;) {
-        if (!child_is_leaf(cur)) {
-            // ---- inner node: test the 4 child boxes ----
-            const float4* np = reinterpret_cast<const float4*>(sc.nodes + cur);
-            float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-            if (COUNT) {
-                cnt->nodes++;
-                cnt->boxes += 4;
-            }
-            float nx0 = ((px ? lx.x : hx.x) - r.ox) * idx, fx0 = ((px ? hx.x : lx.x) - r.ox) * idx;
-            float nx1 = ((px ? lx.y : hx.y) - r.ox) * idx, fx1 = ((px ? hx.y : lx.y) - r.ox) * idx;
-            float nx2 = ((px ? lx.z : hx.z) - r.ox) * idx, fx2 = ((px ? hx.z : lx.z) - r.ox) * idx;
-            float nx3 = ((px ? lx.w : hx.w) - r.ox) * idx, fx3 = ((px ? hx.w : lx.w) - r.ox) * idx;
-            float ny0 = ((py ? ly.x : hy.x) - r.oy) * idy, fy0 = ((py ? hy.x : ly.x) - r.oy) * idy;
-            float ny1 = ((py ? ly.y : hy.y) - r.oy) * idy, fy1 = ((py ? hy.y : ly.y) - r.oy) * idy;
-            float ny2 = ((py ? ly.z : hy.z) - r.oy) * idy, fy2 = ((py ? hy.z : ly.z) - r.oy) * idy;
-            float ny3 = ((py ? ly.w : hy.w) - r.oy) * idy, fy3 = ((py ? hy.w : ly.w) - r.oy) * idy;
-            float nz0 = ((pz ? lz.x : hz.x) - r.oz) * idz, fz0 = ((pz ? hz.x : lz.x) - r.oz) * idz;
-            float nz1 = ((pz ? lz.y : hz.y) - r.oz) * idz, fz1 = ((pz ? hz.y : lz.y) - r.oz) * idz;
-            float nz2 = ((pz ? lz.z : hz.z) - r.oz) * idz, fz2 = ((pz ? hz.z : lz.z) - r.oz) * idz;
-            float nz3 = ((pz ? lz.w : hz.w) - r.oz) * idz, fz3 = ((pz ? hz.w : lz.w) - r.oz) * idz;
-            float tn0 = fmaxf(fmaxf(nx0, ny0), fmaxf(nz0, r.tmin));
-            float tn1 = fmaxf(fmaxf(nx1, ny1), fmaxf(nz1, r.tmin));
-            float tn2 = fmaxf(fmaxf(nx2, ny2), fmaxf(nz2, r.tmin));
-            float tn3 = fmaxf(fmaxf(nx3, ny3), fmaxf(nz3, r.tmin));
-            float tf0 = fminf(fminf(fx0, fy0), fz0) * kBoxWiden;
-            float tf1 = fminf(fminf(fx1, fy1), fz1) * kBoxWiden;
-            float tf2 = fminf(fminf(fx2, fy2), fz2) * kBoxWiden;
-            float tf3 = fminf(fminf(fx3, fy3), fz3) * kBoxWiden;
-            tf0 = fminf(tf0, tbest);
-            tf1 = fminf(tf1, tbest);
-            tf2 = fminf(tf2, tbest);
-            tf3 = fminf(tf3, tbest);
-            // sort keys: non-negative float bits with the slot in the low 2 bits (rounds the
-            // entry distance down, which only weakens culling)
-            uint32_t k0 = (tn0 <= tf0 && ch.x != kEmptyChild) ? ((__float_as_uint(tn0) & ~3u) | 0u) : 0xFFFFFFFFu;
-            uint32_t k1 = (tn1 <= tf1 && ch.y != kEmptyChild) ? ((__float_as_uint(tn1) & ~3u) | 1u) : 0xFFFFFFFFu;
-            uint32_t k2 = (tn2 <= tf2 && ch.z != kEmptyChild) ? ((__float_as_uint(tn2) & ~3u) | 2u) : 0xFFFFFFFFu;
-            uint32_t k3 = (tn3 <= tf3 && ch.w != kEmptyChild) ? ((__float_as_uint(tn3) & ~3u) | 3u) : 0xFFFFFFFFu;
-            // 4-element sorting network (ascending)
-            uint32_t t0 = min(k0, k1), t1 = max(k0, k1), t2 = min(k2, k3), t3 = max(k2, k3);
-            uint32_t s0 = min(t0, t2), s2 = max(t0, t2), s1 = min(t1, t3), s3 = max(t1, t3);
-            uint32_t m1 = min(s1, s2), m2 = max(s1, s2);
-            s1 = m1;
-            s2 = m2;
-            if (s0 == 0xFFFFFFFFu) goto pop;
-            if (s3 != 0xFFFFFFFFu) st.push(sp++, pick4(ch, s3 & 3u), __uint_as_float(s3 & ~3u));
-            if (s2 != 0xFFFFFFFFu) st.push(sp++, pick4(ch, s2 & 3u), __uint_as_float(s2 & ~3u));
-            if (s1 != 0xFFFFFFFFu) st.push(sp++, pick4(ch, s1 & 3u), __uint_as_float(s1 & ~3u));
-            cur = pick4(ch, s0 & 3u);
-            continue;
-        } else {
-            // ---- leaf: test its primitives ----
-            uint32_t first = leaf_first(cur), count = leaf_count(cur);
-            for (uint32_t k = 0; k < count; k++) {
-                const float4* pp = sc.prims + 3 * (size_t)(first + k);
-                float4 a = pp[0];
-                float t, u = 0.0f, v = 0.0f;
-                bool h;
-                if (COUNT) cnt->prims++;
-                if (!MIXED) {
-                    float4 b = pp[1], c = pp[2];
-                    h = tri_test(r, w, a, b, c, tbest, t, u, v);
-                } else {
-                    uint32_t tag = __float_as_uint(a.w);
-                    uint32_t kind = prim_kind(tag);
-                    float4 b = pp[1];
-                    if (kind == kPrimTriangle) {
-                        float4 c = pp[2];
-                        h = tri_test(r, w, a, b, c, tbest, t, u, v);
-                    } else if (kind == kPrimSphere) {
-                        h = sphere_test(r, a, b, tbest, t);
-                    } else {
-                        float4 c = pp[2];
-                        h = quad_test(r, a, b, c, sc.quads[prim_side(tag)], tbest, t, u, v);
-                    }
-                }
-                if (h) {
-                    tbest = t;
-                    best = first + k;
-                    bu = u;
-                    bv = v;
-                    if (ANY) {
-                        hit.t = t;
-                        hit.prim = best;
-                        hit.u = u;
-                        hit.v = v;
-                        return true;
-                    }
-                }
-            }
-        }
-    pop:
-        for (;;) {
-            if (sp == 0) goto done;
-            float tn;
-            st.pop(--sp, cur, tn);
-            if (tn <= tbest) break;  // entry distance beyond the current closest hit: cull
-        }
+    TState s;
+    tstate_init(s, r_in);
+    while (!tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK>(sc, s, st, cnt)) {
     }
-done:
-    hit.t = best == 0xFFFFFFFFu ? __int_as_float(0x7f800000) : tbest;
-    hit.prim = best;
-    hit.u = bu;
-    hit.v = bv;
-    return best != 0xFFFFFFFFu;
+    tstate_result(s, hit);
+    return s.best != 0xFFFFFFFFu;
 }
 
 }  // namespace prt

@@ -83,7 +83,8 @@ class SceneDescC(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("num_primitives", C.c_int64), ("num_triangles", C.c_int64), ("bvh_nodes", C.c_int64),
                 ("bvh_depth", C.c_int32), ("max_stack", C.c_int32), ("world_center", D3),
-                ("world_radius", C.c_double), ("device_bytes", C.c_int64)]
+                ("world_radius", C.c_double), ("device_bytes", C.c_int64), ("trace_variant", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class RenderParams(C.Structure):
@@ -110,7 +111,22 @@ EXPORTS = [
     "prt_samples_for_pass", "prt_film_create", "prt_render_pass", "prt_render_samples", "prt_film_read",
     "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy",
     "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky",
+    "prt_server_create", "prt_server_submit", "prt_server_poll", "prt_server_cancel", "prt_server_read",
+    "prt_server_read_rgba8", "prt_server_release", "prt_server_stats_get", "prt_server_destroy",
 ]
+
+EVENT_ADMITTED, EVENT_PASS_COMPLETE, EVENT_DONE, EVENT_ERROR, EVENT_CANCELLED = range(5)
+
+
+class ServerEvent(C.Structure):
+    _fields_ = [("request_id", C.c_int64), ("type", C.c_int32), ("pass_number", C.c_int32),
+                ("target_samples", C.c_int32), ("error_code", C.c_int32), ("t_ms", C.c_double),
+                ("device_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int32), ("active", C.c_int32), ("completed", C.c_int64),
+                ("budget_bytes", C.c_int64), ("used_bytes", C.c_int64)]
 
 _lib = None
 
@@ -146,6 +162,15 @@ def lib():
                                              C.POINTER(C.c_int64), vp, C.POINTER(C.c_int64)]),
         "prt_workload_rays": (C.c_int, [vp, vp, C.c_double, C.c_int64, C.c_uint64, vp]),
         "prt_workload_sky": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.c_double, vp]),
+        "prt_server_create": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(vp)]),
+        "prt_server_submit": (C.c_int, [vp, vp, C.POINTER(RenderParams), C.POINTER(C.c_int64)]),
+        "prt_server_poll": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
+        "prt_server_cancel": (C.c_int, [vp, C.c_int64]),
+        "prt_server_read": (C.c_int, [vp, C.c_int64, vp, vp]),
+        "prt_server_read_rgba8": (C.c_int, [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]),
+        "prt_server_release": (C.c_int, [vp, C.c_int64]),
+        "prt_server_stats_get": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "prt_server_destroy": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -457,6 +482,60 @@ class ProgressiveRaytracer:
         out = np.empty((y1 - y0, x1 - x0, 4), np.uint8)
         check(lib().prt_film_read_rgba8(self.handle, x0, y0, x1, y1, _ptr(out)))
         return out
+
+
+class Server:
+    """Device-resident serving loop (include/prt_server.h): continuous batching of progressive
+    renders, the GPU counterpart of the web server's per-request pipeline (web/server/render.go)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=64 << 30, max_active=32):
+        h = C.c_void_p()
+        check(lib().prt_server_create(device, hbm_budget_bytes, max_active, C.byref(h)))
+        self.handle = h
+
+    def submit(self, scene, params):
+        rid = C.c_int64()
+        check(lib().prt_server_submit(self.handle, scene.handle, C.byref(params), C.byref(rid)))
+        return rid.value
+
+    def poll(self, max_events=256, timeout_ms=0):
+        ev = (ServerEvent * max_events)()
+        n = C.c_int32()
+        check(lib().prt_server_poll(self.handle, ev, max_events, timeout_ms, C.byref(n)))
+        return [ev[i] for i in range(n.value)]
+
+    def cancel(self, rid):
+        check(lib().prt_server_cancel(self.handle, rid))
+
+    def read(self, rid, width, height):
+        mean = np.empty((height, width, 3), np.float32)
+        cnt = np.empty((height, width), np.uint32)
+        check(lib().prt_server_read(self.handle, rid, _ptr(mean), _ptr(cnt)))
+        return mean, cnt
+
+    def read_rgba8(self, rid, x0, y0, x1, y1):
+        out = np.empty((y1 - y0, x1 - x0, 4), np.uint8)
+        check(lib().prt_server_read_rgba8(self.handle, rid, x0, y0, x1, y1, _ptr(out)))
+        return out
+
+    def release(self, rid):
+        check(lib().prt_server_release(self.handle, rid))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().prt_server_stats_get(self.handle, C.byref(st)))
+        return st
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().prt_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------------------

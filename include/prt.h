@@ -174,6 +174,8 @@ typedef struct prt_scene_info {
     double world_center[3]; /* reference BVH root box centre (bvh.go:38-48) */
     double world_radius;
     int64_t device_bytes;
+    int32_t trace_variant;  /* traversal kernel variant picked by the autotuner (-1: not tuned yet) */
+    int32_t pad;
 } prt_scene_info;
 
 typedef struct prt_scene prt_scene;

@@ -1102,6 +1102,597 @@ static void build_env(EnvMap& e, const prt_light& pl) {
     e.integral = tot / ((double)e.w * e.h);
 }
 
+// =============================== BDPT (pkg/integrator/bdpt.go, bdpt_mis.go) ======================
+// Draw slots of the counter stream (the reference draws from one per-tile stream in call order):
+// camera path bounce b: slot 1+b; light selection/emission: slot kLightSlot; light path bounce b:
+// slot kLightSlot+1+b; direct lighting at camera vertex t: slot kDirectSlot+t; light tracing from
+// light vertex s: slot kSplatSlot+s.
+static const uint32_t kLightSlot = 70, kDirectSlot = 140, kSplatSlot = 210;
+
+// Camera.CalculateRayPDFs / MapRayToPixel / SampleCameraFromPoint / EvaluateRayImportance,
+// pkg/geometry/camera.go:194-316
+static bool cam_map_ray_to_pixel(const Camera& c, const Ray& r, int& px, int& py) {
+    Vec3 ipc = c.center - c.w * c.focus;
+    Vec3 tp = ipc - r.o;
+    double t = tp.dot(c.w) / r.d.dot(c.w);
+    if (t <= 0) return false;
+    Vec3 hp = r.o + r.d * t;
+    Vec3 rel = hp - ipc;
+    double plx = rel.dot(c.u), ply = rel.dot(c.v);
+    double nx = (plx + c.vw / 2) / c.vw, ny = (ply + c.vh / 2) / c.vh;
+    px = (int)(nx * (double)c.width);
+    py = (int)(ny * (double)c.height);
+    return px >= 0 && px < c.width && py >= 0 && py < c.height;
+}
+static void cam_ray_pdfs(const Camera& c, const Ray& r, double& pos, double& dir) {
+    double ct = r.d.dot(c.forward);
+    pos = dir = 0;
+    if (ct <= c.cos_total_width) return;
+    int x, y;
+    if (!cam_map_ray_to_pixel(c, r, x, y)) return;
+    pos = 1.0 / c.lens_area;
+    dir = 1.0 / (c.plane_area * std::pow(ct, 3));
+}
+static double cam_importance(const Camera& c, const Ray& r) {
+    double ct = r.d.dot(c.forward);
+    if (ct < c.cos_total_width) return 0;
+    return 1.0 / (c.plane_area * c.lens_area * std::pow(ct, 4));
+}
+struct CamSample {
+    Ray ray;
+    double weight = 0, pdf = 0;
+    bool ok = false;
+};
+static CamSample cam_sample_from_point(const Camera& c, Vec3 ref, Vec2 s) {
+    CamSample cs;
+    Vec3 lc = sample_point_in_unit_disk(s) * c.lens_radius;
+    Vec3 lp = c.center + c.u * lc.x + c.v * lc.y;
+    Vec3 dir = (ref - lp).normalize();
+    double dist = (ref - lp).length();
+    cs.ray = {lp, dir};
+    double ct = std::fabs(c.forward.dot(dir));
+    cs.pdf = (dist * dist) / (ct * c.lens_area);
+    cs.weight = cam_importance(c, cs.ray);
+    cs.ok = Vec3(cs.weight, cs.weight, cs.weight).luminance() != 0;
+    return cs;
+}
+
+static bool light_is_area(const Light& L) { return L.type == PRT_LIGHT_QUAD || L.type == PRT_LIGHT_SPHERE; }
+
+struct EmissionSample {
+    Vec3 p, n, dir, emission;
+    double area_pdf = 0, dir_pdf = 0;
+};
+// SampleInfiniteLight (light_sampling.go:89-115) for a given travel direction
+static void infinite_disk(const Light& L, Vec3 direction, Vec2 sp, EmissionSample& es) {
+    Vec3 up = std::fabs(direction.x) > 0.9 ? Vec3(0, 1, 0) : Vec3(1, 0, 0);
+    Vec3 right = direction.cross(up).normalize();
+    up = right.cross(direction).normalize();
+    Vec3 ds = sample_point_in_unit_disk(sp);
+    Vec3 dp = L.world_center + right * (ds.x * L.world_radius) + up * (ds.y * L.world_radius);
+    es.p = dp + direction * -L.world_radius;
+    es.dir = direction;
+    es.n = direction * -1.0;
+    es.area_pdf = 1.0 / (kPi * L.world_radius * L.world_radius);
+}
+// Light.SampleEmission (interfaces.go:31-33)
+static EmissionSample light_sample_emission(const Scene& S, const Light& L, Vec2 sp, Vec2 sd) {
+    EmissionSample es;
+    switch (L.type) {
+        case PRT_LIGHT_QUAD: {  // quad_light.go:107-126
+            const Quad& q = *L.quad;
+            es.p = q.corner + q.u * sp.x + q.v * sp.y;
+            es.n = q.normal;
+            es.dir = sample_cosine_hemisphere(q.normal, sd);
+            es.area_pdf = 1.0 / L.area;
+            es.dir_pdf = es.dir.dot(q.normal) / kPi;
+            es.emission = emissive_emit(S, L.mat, nullptr);
+            return es;
+        }
+        case PRT_LIGHT_SPHERE: {  // sphere_light.go:168-181 + SampleEmissionDirection light_sampling.go:54-80
+            const Sphere& s = *L.sphere;
+            double z = 1.0 - 2.0 * sp.x;
+            double r = std::sqrt(std::max(0.0, 1.0 - z * z));
+            double phi = 2.0 * kPi * sp.y;
+            Vec3 ld(r * std::cos(phi), r * std::sin(phi), z);
+            es.p = s.c + ld * s.r;
+            es.n = ld;
+            es.area_pdf = 1.0 / (4.0 * kPi * s.r * s.r);
+            es.dir = sample_cosine_hemisphere(ld, sd);
+            es.dir_pdf = es.dir.dot(ld) / kPi;
+            es.emission = is_emitter(S, L.mat) ? emissive_emit(S, L.mat, nullptr) : Vec3(0, 0, 0);
+            return es;
+        }
+        case PRT_LIGHT_UNIFORM_INFINITE:
+        case PRT_LIGHT_GRADIENT_INFINITE: {  // uniform_infinite_light.go:101-112, gradient :103-115
+            Vec3 d = sample_on_unit_sphere(sd);
+            infinite_disk(L, d, sp, es);
+            es.dir_pdf = 1.0 / (4.0 * kPi);
+            if (L.type == PRT_LIGHT_UNIFORM_INFINITE) es.emission = L.emission;
+            else {
+                double t = 0.5 * (d.y + 1.0);
+                es.emission = L.bottom * (1.0 - t) + L.top * t;
+            }
+            return es;
+        }
+        case PRT_LIGHT_ENVMAP: {  // importance-sampled direction toward the sky, travelling inward
+            double pdf;
+            Vec3 wi = env_sample(*L.env, sd, pdf);
+            infinite_disk(L, wi * -1.0, sp, es);
+            es.dir_pdf = pdf;
+            es.emission = env_eval(*L.env, wi);
+            return es;
+        }
+    }
+    return es;
+}
+// Light.PDF_Le (interfaces.go:35-39)
+static void light_pdf_le(const Light& L, Vec3 p, Vec3 d, double& pos, double& dir) {
+    pos = dir = 0;
+    switch (L.type) {
+        case PRT_LIGHT_QUAD: {  // quad_light.go:129-176
+            const Quad& q = *L.quad;
+            Vec3 tp = p - q.corner;
+            double uu = q.u.dot(q.u), vv = q.v.dot(q.v), uv = q.u.dot(q.v);
+            if (uu == 0 || vv == 0) return;
+            double det = uu * vv - uv * uv;
+            if (std::fabs(det) < 1e-8) return;
+            double tu = tp.dot(q.u), tv = tp.dot(q.v);
+            double a = (vv * tu - uv * tv) / det, b = (uu * tv - uv * tu) / det;
+            if (a < 0 || a > 1 || b < 0 || b > 1) return;
+            Vec3 rec = q.corner + q.u * a + q.v * b;
+            if ((rec - p).length() > 0.001) return;
+            pos = 1.0 / L.area;
+            double ct = d.dot(q.normal);
+            if (ct <= 0) return;
+            dir = ct / kPi;
+            return;
+        }
+        case PRT_LIGHT_SPHERE: {  // sphere_light.go:184-205
+            const Sphere& s = *L.sphere;
+            if (!(std::fabs((p - s.c).length() - s.r) <= 0.001)) return;
+            Vec3 n = (p - s.c).normalize();
+            if (d.dot(n) <= 0) return;
+            pos = 1.0 / (4.0 * kPi * s.r * s.r);
+            dir = d.dot(n) / kPi;
+            return;
+        }
+        case PRT_LIGHT_UNIFORM_INFINITE:
+        case PRT_LIGHT_GRADIENT_INFINITE:
+            if (L.world_radius <= 0) return;
+            pos = 1.0 / (kPi * L.world_radius * L.world_radius);
+            dir = 1.0 / (4.0 * kPi);
+            return;
+        case PRT_LIGHT_ENVMAP:
+            if (L.world_radius <= 0) return;
+            pos = 1.0 / (kPi * L.world_radius * L.world_radius);
+            dir = env_pdf(*L.env, d * -1.0);
+            return;
+    }
+}
+
+// Vertex, pkg/integrator/bdpt.go:14-44
+struct BVertex {
+    Vec3 p, n, wi, beta, Le;
+    int mat = -1;  // -1: no material (camera, light sample, background)
+    bool front = true;
+    const Light* light = nullptr;
+    int light_index = 0;
+    double pdf_fwd = 0, pdf_rev = 0;
+    bool is_light = false, is_camera = false, is_specular = false, is_inf = false;
+    bool on_surface() const {  // IsOnSurface :33-41
+        if (is_light && light) return light_is_area(*light);
+        return mat >= 0;
+    }
+    Hit hit() const {
+        Hit h;
+        h.p = p;
+        h.n = n;
+        h.front = front;
+        h.mat = mat;
+        return h;
+    }
+};
+struct BPath {
+    std::vector<BVertex> v;
+    int len() const { return (int)v.size(); }
+};
+
+// convertSolidAngleToAreaPdf, bdpt_mis.go:341-362
+static double convert_density(const BVertex& from, const BVertex& next, double pdf) {
+    if (next.is_inf) return pdf;
+    Vec3 d = next.p - from.p;
+    double d2 = d.length2();
+    if (d2 == 0) return 0.0;
+    double inv = 1.0 / d2;
+    if (next.on_surface()) pdf *= std::fabs((d * std::sqrt(inv)).dot(next.n));
+    return pdf * inv;
+}
+static double remap0(double f) { return f != 0 ? f : 1.0; }
+
+struct BDPTCtx {
+    const Scene& S;
+    int max_depth;
+};
+
+static BVertex background_vertex(const Ray& r, Vec3 bg, Vec3 beta, double pdf_fwd) {  // bdpt.go:585-601
+    BVertex v;
+    v.p = r.o + r.d * 1000.0;
+    v.n = r.d * -1.0;
+    v.wi = r.d * -1.0;
+    v.pdf_fwd = pdf_fwd;
+    v.pdf_rev = 0;
+    v.is_light = !bg.is_zero();
+    v.is_inf = true;
+    v.beta = beta;
+    v.Le = bg;
+    return v;
+}
+
+// extendPath, bdpt.go:206-291
+static void extend_path(const BDPTCtx& C, BPath& path, Ray ray, Vec3 beta, double pdf_fwd, Sampler& smp, int max_bounces,
+                        bool camera_path, uint32_t slot0) {
+    const Scene& S = C.S;
+    for (int b = 0; b < max_bounces; b++) {
+        int prev_i = path.len() - 1;
+        Hit h;
+        if (!S.bvh->hit(ray, 0.001, kInf, h, nullptr)) {
+            if (camera_path) {
+                Vec3 e;
+                for (auto& L : S.lights)
+                    if (light_is_infinite(L)) e = e + light_emit(S, L, ray);
+                path.v.push_back(background_vertex(ray, e, beta, pdf_fwd));
+            }
+            break;
+        }
+        BVertex v;
+        v.p = h.p;
+        v.n = h.n;
+        v.front = h.front;
+        v.mat = h.mat;
+        v.wi = ray.d * -1.0;
+        v.beta = beta;
+        v.Le = is_emitter(S, h.mat) ? emissive_emit(S, h.mat, &h) : Vec3(0, 0, 0);
+        v.is_light = !v.Le.is_zero();
+        v.pdf_fwd = convert_density(path.v[prev_i], v, pdf_fwd);
+        smp.set_slot(slot0 + (uint32_t)b);
+        smp.k = 1;
+        Scatter sc;
+        if (!mat_scatter(S, h.mat, ray, h, smp, sc)) {
+            path.v.push_back(v);
+            break;
+        }
+        pdf_fwd = sc.pdf;
+        v.is_specular = sc.specular();
+        double ct = std::fabs(sc.out.d.dot(h.n));
+        if (sc.specular()) beta = beta.mul(sc.att);
+        else beta = beta.mul(sc.att) * (ct / sc.pdf);
+        bool rdelta;
+        double pdf_rev = mat_pdf(S, h.mat, sc.out.d, ray.d * -1.0, h.n, rdelta);
+        if (rdelta) {
+            v.is_specular = true;
+            pdf_rev = 0;
+            pdf_fwd = 0;
+        }
+        path.v[prev_i].pdf_rev = convert_density(v, path.v[prev_i], pdf_rev);
+        path.v.push_back(v);
+        ray = sc.out;
+    }
+}
+
+static double infinite_light_density(const Scene& S, Vec3 p, Vec3 n, Vec3 d) {  // bdpt_mis.go:318-337
+    if (S.lights.empty()) return 0;
+    double tot = 0;
+    for (size_t i = 0; i < S.lights.size(); i++)
+        if (light_is_infinite(S.lights[i])) tot += light_pdf(S, S.lights[i], p, n, d) * S.weights[i];
+    return tot;
+}
+
+static BPath camera_path(const BDPTCtx& C, const Ray& r, Sampler& smp) {  // bdpt.go:106-135
+    BPath path;
+    double pos, dir;
+    cam_ray_pdfs(C.S.cam, r, pos, dir);
+    BVertex cv;
+    cv.p = r.o;
+    cv.n = r.d * -1.0;
+    cv.is_camera = true;
+    cv.beta = Vec3(1, 1, 1);
+    path.v.push_back(cv);
+    extend_path(C, path, r, Vec3(1, 1, 1), dir, smp, C.max_depth, true, 1);
+    return path;
+}
+
+static BPath light_path(const BDPTCtx& C, Sampler& smp) {  // bdpt.go:139-203
+    const Scene& S = C.S;
+    BPath path;
+    if (S.lights.empty()) return path;
+    smp.set_slot(kLightSlot);
+    double sel;
+    int li = select_light(S, smp.get1d(), sel);
+    Vec2 sp = smp.get2d();
+    Vec2 sd = smp.get2d();
+    const Light& L = S.lights[li];
+    EmissionSample es = light_sample_emission(S, L, sp, sd);
+    double ct = std::fabs(es.dir.dot(es.n));
+    BVertex lv;
+    lv.p = es.p;
+    lv.n = es.n;
+    lv.light = &L;
+    lv.light_index = li;
+    lv.pdf_fwd = es.area_pdf * sel;
+    lv.is_light = true;
+    lv.is_inf = light_is_infinite(L);
+    lv.beta = es.emission;
+    lv.Le = es.emission;
+    path.v.push_back(lv);
+    Ray ray{es.p, es.dir};
+    Vec3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
+    extend_path(C, path, ray, beta, es.dir_pdf, smp, C.max_depth - 1, false, kLightSlot + 1);
+    if (path.v[0].is_inf) {
+        if (path.len() > 1) {
+            BVertex& fb = path.v[1];
+            fb.pdf_fwd = es.area_pdf * sel;
+            if (fb.mat >= 0) fb.pdf_fwd *= std::fabs(ray.d.dot(fb.n));
+        }
+        path.v[0].pdf_fwd = infinite_light_density(S, es.p, es.n, es.dir);
+    }
+    return path;
+}
+
+// evaluateBRDF, bdpt.go:564-583
+static Vec3 vertex_brdf(const Scene& S, const BVertex& v, Vec3 out, int mode) {
+    if (v.is_inf) return {0, 0, 0};
+    if (v.is_light && v.mat < 0) return {1, 1, 1};
+    if (v.mat < 0) return {0, 0, 0};
+    return mat_brdf(S, v.mat, v.wi, out, v.hit(), mode);
+}
+
+// calculateLightPdf / calculateLightOriginPdf / calculateVertexPdf, bdpt_mis.go:190-315
+static double vtx_light_pdf(const Scene& S, const BVertex& cur, const BVertex& to) {
+    Vec3 w = to.p - cur.p;
+    double inv = 1.0 / w.length2();
+    w = w * std::sqrt(inv);
+    double pdf = 0;
+    if (cur.is_light) {
+        if (cur.is_inf) {
+            double R = S.bvh->radius;
+            if (R == 0.0) R = 1.0;
+            pdf = 1.0 / (kPi * R * R);
+        } else if (cur.light) {
+            double pos, dir;
+            light_pdf_le(*cur.light, cur.p, w, pos, dir);
+            pdf = dir * inv;
+        }
+    }
+    if (to.on_surface()) pdf *= std::fabs(to.n.dot(w));
+    return pdf;
+}
+static double vtx_light_origin_pdf(const Scene& S, const BVertex& lv, const BVertex& to) {
+    Vec3 w = to.p - lv.p;
+    if (w.length2() == 0) return 0;
+    w = w.normalize();
+    if (lv.is_inf) return infinite_light_density(S, to.p, to.n, w * -1.0);
+    if (!lv.is_light || !lv.light) return 0;
+    if (S.lights.empty()) return 0;
+    double choice = (lv.light_index >= 0 && lv.light_index < (int)S.weights.size()) ? S.weights[lv.light_index] : 0.0;
+    double pos, dir;
+    light_pdf_le(*lv.light, lv.p, w, pos, dir);
+    return pos * choice;
+}
+static double vtx_pdf(const Scene& S, const BVertex& cur, const BVertex* prev, const BVertex& next) {
+    if (cur.is_light) return vtx_light_pdf(S, cur, next);
+    Vec3 wn = next.p - cur.p;
+    if (wn.length2() == 0) return 0;
+    wn = wn.normalize();
+    Vec3 wp;
+    if (prev) {
+        wp = prev->p - cur.p;
+        if (wp.length2() == 0) return 0;
+        wp = wp.normalize();
+    } else if (!cur.is_camera) {
+        return 0;
+    }
+    double pdf;
+    if (cur.is_camera) {
+        double pos;
+        cam_ray_pdfs(S.cam, Ray{cur.p, wn}, pos, pdf);
+        if (pdf == 0) return 0;
+    } else if (cur.mat >= 0) {
+        bool delta;
+        pdf = mat_pdf(S, cur.mat, wp, wn, cur.n, delta);
+        if (delta) return 0;
+    } else {
+        return 0;
+    }
+    return convert_density(cur, next, pdf);
+}
+
+// calculateMISWeight, bdpt_mis.go:13-48 (with the per-vertex rules of :52-187)
+static double mis_weight(const Scene& S, const BPath& cp, const BPath& lp, const BVertex* sv, int s, int t) {
+    if (s + t == 2) return 1.0;
+    double sum = 0, ri = 1.0;
+    for (int i = t - 1; i > 0; i--) {
+        const BVertex& v = cp.v[i];
+        double fwd = v.pdf_fwd, rev = v.pdf_rev;
+        bool conn = !v.is_specular;
+        if (s == 0) {
+            if (i == t - 1 && t > 1) {
+                rev = vtx_light_origin_pdf(S, cp.v[t - 1], cp.v[t - 2]);
+                conn = true;
+            } else if (i == t - 2 && t > 2) {
+                rev = vtx_pdf(S, cp.v[t - 1], nullptr, cp.v[t - 2]);
+            }
+        } else if (t == 1) {
+            if (i == 0) {
+                fwd = sv->pdf_fwd;
+                rev = sv->pdf_rev;
+                conn = true;
+            }
+        } else if (s == 1) {
+            if (i == t - 1 && t > 0) {
+                rev = vtx_pdf(S, *sv, nullptr, cp.v[t - 1]);
+                conn = true;
+            } else if (i == t - 2 && t > 1) {
+                rev = vtx_pdf(S, cp.v[t - 1], sv, cp.v[t - 2]);
+            }
+        } else {
+            if (i == t - 1) {
+                rev = vtx_pdf(S, lp.v[s - 1], &lp.v[s - 2], cp.v[t - 1]);
+                conn = true;
+            } else if (i == t - 2 && t > 1) {
+                rev = vtx_pdf(S, cp.v[t - 1], &lp.v[s - 1], cp.v[t - 2]);
+            }
+        }
+        if (i > 0) conn = conn && !cp.v[i - 1].is_specular;
+        ri *= remap0(rev) / remap0(fwd);
+        if (conn) sum += ri;
+    }
+    ri = 1.0;
+    for (int i = s - 1; i >= 0; i--) {
+        const BVertex& v = lp.v[i];
+        double fwd = v.pdf_fwd, rev = v.pdf_rev;
+        bool conn = !v.is_specular;  // point (delta) lights are out of scope here
+        if (i > 0) conn = conn && !lp.v[i - 1].is_specular;
+        if (s == 0) {
+        } else if (t == 1) {
+            if (i == s - 1 && s > 1) {
+                rev = vtx_pdf(S, *sv, nullptr, lp.v[s - 1]);
+                conn = true;
+            } else if (i == s - 2 && s > 1) {
+                rev = vtx_pdf(S, lp.v[s - 1], sv, lp.v[s - 2]);
+            }
+        } else if (s == 1) {
+            if (i == s - 1 && sv) {
+                fwd = sv->pdf_fwd;
+                rev = vtx_pdf(S, cp.v[t - 1], &cp.v[t - 2], *sv);
+                conn = true;
+            }
+        } else {
+            if (i == s - 1) {
+                rev = vtx_pdf(S, cp.v[t - 1], &cp.v[t - 2], lp.v[s - 1]);
+                conn = true;
+            } else if (i == s - 2 && s > 1) {
+                rev = vtx_pdf(S, lp.v[s - 1], &cp.v[t - 1], lp.v[s - 2]);
+            }
+        }
+        ri *= remap0(rev) / remap0(fwd);
+        if (conn) sum += ri;
+    }
+    return 1.0 / (1.0 + sum);
+}
+
+struct Splat {
+    Ray ray;
+    Vec3 color;
+};
+
+// RayColor, bdpt.go:73-102, with the strategies of :294-561
+static Vec3 bdpt_ray_color(const BDPTCtx& C, const Ray& ray, Sampler& smp, std::vector<Splat>& splats) {
+    const Scene& S = C.S;
+    BPath cp = camera_path(C, ray, smp);
+    BPath lp = light_path(C, smp);
+    Vec3 total;
+    for (int s = 0; s <= lp.len(); s++) {
+        for (int t = 1; t <= cp.len(); t++) {
+            if (s == 1 && t == 1) continue;
+            Vec3 L;
+            bool have_splat = false;
+            Splat sp;
+            BVertex sampled;
+            const BVertex* sv = nullptr;
+            if (s == 0) {  // evaluatePathTracingStrategy :316-330
+                if (t < cp.len()) continue;
+                const BVertex& last = cp.v[t - 1];
+                L = last.Le.mul(last.beta);
+            } else if (t == 1) {  // evaluateLightTracingStrategy :394-471
+                if (s <= 1 || s > lp.len()) continue;
+                const BVertex& lv = lp.v[s - 1];
+                if (lv.is_specular) continue;
+                smp.set_slot(kSplatSlot + (uint32_t)s);
+                CamSample cs = cam_sample_from_point(S.cam, lv.p, smp.get2d());
+                if (!cs.ok) continue;
+                Vec3 f = vertex_brdf(S, lv, cs.ray.d * -1.0, 1);
+                Vec3 cbeta = Vec3(cs.weight, cs.weight, cs.weight) * (1.0 / cs.pdf);
+                double cosine = (cs.ray.d * -1.0).dot(lv.n);
+                if (cosine <= 0) continue;
+                Vec3 contrib = f.mul(cbeta).mul(lv.beta);
+                if (lv.on_surface()) contrib = contrib * cosine;
+                if (contrib.is_zero()) continue;
+                double dist = (lv.p - cs.ray.o).length();
+                Hit bh;
+                if (S.bvh->hit(Ray{lv.p, cs.ray.d * -1.0}, 0.001, dist - 0.001, bh, nullptr)) continue;
+                Vec3 to_light = (lv.p - cs.ray.o).normalize();
+                double lpdf = total_light_pdf(S, cs.ray.o, cs.ray.d * -1.0, to_light);
+                sampled.p = cs.ray.o;
+                sampled.n = cs.ray.d * -1.0;
+                sampled.is_camera = true;
+                sampled.beta = cbeta;
+                sampled.pdf_fwd = cs.pdf;
+                sampled.pdf_rev = lpdf;
+                sv = &sampled;
+                sp.ray = cs.ray;
+                sp.color = contrib;
+                have_splat = true;
+            } else if (s == 1) {  // evaluateDirectLightingStrategy :333-391
+                const BVertex& cv = cp.v[t - 1];
+                if (cv.is_specular || cv.mat < 0) continue;
+                smp.set_slot(kDirectSlot + (uint32_t)t);
+                double sel;
+                int li = select_light(S, smp.get1d(), sel);
+                LightSample ls = light_sample(S, S.lights[li], cv.p, cv.n, smp.get2d());
+                ls.pdf *= sel;
+                if (ls.emission.is_zero() || ls.pdf <= 0) continue;
+                double ct = std::fabs(ls.dir.dot(cv.n));
+                if (ct <= 0) continue;
+                Vec3 f = mat_brdf(S, cv.mat, cv.wi, ls.dir, cv.hit(), 0);
+                Vec3 lbeta = ls.emission * (1.0 / ls.pdf);
+                Vec3 contrib = f.mul(cv.beta).mul(lbeta) * ct;
+                if (contrib.is_zero()) continue;
+                Hit bh;
+                if (S.bvh->hit(Ray{cv.p, ls.dir}, 0.001, ls.dist - 0.001, bh, nullptr)) continue;
+                const Light& L0 = S.lights[li];
+                sampled.p = ls.p;  // LightSample.Point
+                sampled.n = ls.n;
+                sampled.light = &L0;
+                sampled.light_index = li;
+                sampled.pdf_fwd = ls.pdf;
+                sampled.is_light = true;
+                sampled.is_inf = light_is_infinite(L0);
+                sampled.beta = lbeta;
+                sampled.Le = ls.emission;
+                sv = &sampled;
+                L = contrib;
+            } else {  // evaluateConnectionStrategy :489-561
+                const BVertex& lv = lp.v[s - 1];
+                const BVertex& cv = cp.v[t - 1];
+                if (lv.is_specular || cv.is_specular) continue;
+                Vec3 d = lv.p - cv.p;
+                double dist = d.length();
+                if (dist < 0.001) continue;
+                d = d * (1.0 / dist);
+                double cc = d.dot(cv.n), cl = (d * -1.0).dot(lv.n);
+                if (cc <= 0 || cl <= 0) continue;
+                double G = (cc * cl) / (dist * dist);
+                Vec3 fc = vertex_brdf(S, cv, d, 0);
+                Vec3 fl = vertex_brdf(S, lv, d * -1.0, 1);
+                Vec3 contrib = lv.beta.mul(fl).mul(fc).mul(cv.beta) * G;
+                if (contrib.is_zero()) continue;
+                Hit bh;
+                if (S.bvh->hit(Ray{cv.p, d}, 0.001, dist - 0.001, bh, nullptr)) continue;
+                L = contrib;
+            }
+            if (L.is_zero() && !have_splat) continue;
+            double w = mis_weight(S, cp, lp, sv, s, t);
+            total = total + L * w;
+            if (have_splat) {
+                sp.color = sp.color * w;
+                splats.push_back(sp);
+            }
+        }
+    }
+    return total;
+}
+
 }  // namespace orc
 
 // =============================== C interface ===========================================
@@ -1254,12 +1845,14 @@ void orc_intersect(const orc_scene* os, const prt_ray* rays, int64_t n, double* 
 // (y1-y0)*(x1-x0) pixels: sum_rgb (x3), lum_sum, lum_sq_sum (PixelStats.AddSample, stats.go:41-47).
 int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int32_t sample_offset,
                int32_t x0, int32_t y0, int32_t x1, int32_t y1, double* sum_rgb, double* lum_sum,
-               double* lum_sq, int32_t nthreads) {
+               double* lum_sq, int32_t nthreads, double* splat_rgb) {
     const Scene& S = os->s;
-    if (p->integrator != PRT_INTEGRATOR_PATH) return PRT_ERR_UNSUPPORTED;
+    const bool bdpt = p->integrator == PRT_INTEGRATOR_BDPT;
+    if (!bdpt && p->integrator != PRT_INTEGRATOR_PATH) return PRT_ERR_UNSUPPORTED;
     PTConfig cfg;
     cfg.max_depth = p->max_depth;
     cfg.rr_min = p->rr_min_bounces;
+    BDPTCtx bc{S, p->max_depth};
     int W = S.cam.width;
     int tw = x1 - x0, th = y1 - y0;
     // 64x64 tiles over a worker pool, as pkg/renderer/worker_pool.go:49-131
@@ -1267,7 +1860,11 @@ int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int
     int ntx = (tw + ts - 1) / ts, nty = (th + ts - 1) / ts;
     int ntiles = ntx * nty;
     std::atomic<int> next{0};
-    auto worker = [&]() {
+    int nt_ = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    const size_t full = (size_t)S.cam.width * S.cam.height * 3;
+    std::vector<std::vector<double>> tsplat(bdpt && splat_rgb ? nt_ : 0, std::vector<double>(full, 0.0));
+    auto worker = [&](int wid) {
+        std::vector<Splat> splats;
         for (;;) {
             int t = next.fetch_add(1);
             if (t >= ntiles) break;
@@ -1283,7 +1880,23 @@ int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int
                         Vec2 a = smp.get2d();
                         Vec2 b = smp.get2d();
                         Ray r = S.cam.get_ray(i, j, a, b);
-                        Vec3 c = ray_color(S, cfg, r, smp, cfg.max_depth, Vec3(1, 1, 1));
+                        Vec3 c;
+                        if (bdpt) {
+                            splats.clear();
+                            c = bdpt_ray_color(bc, r, smp, splats);
+                            if (!tsplat.empty())
+                                for (auto& sp : splats) {  // SplatQueue.AddSplat via MapRayToPixel
+                                    int sx, sy;
+                                    if (cam_map_ray_to_pixel(S.cam, sp.ray, sx, sy)) {
+                                        double* o = &tsplat[wid][((size_t)sy * S.cam.width + sx) * 3];
+                                        o[0] += sp.color.x;
+                                        o[1] += sp.color.y;
+                                        o[2] += sp.color.z;
+                                    }
+                                }
+                        } else {
+                            c = ray_color(S, cfg, r, smp, cfg.max_depth, Vec3(1, 1, 1));
+                        }
                         sr += c.x;
                         sg += c.y;
                         sb += c.z;
@@ -1299,21 +1912,29 @@ int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int
                 }
         }
     };
-    int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
     std::vector<std::thread> pool;
-    for (int i = 0; i < nt; i++) pool.emplace_back(worker);
+    for (int i = 0; i < nt_; i++) pool.emplace_back(worker, i);
     for (auto& t : pool) t.join();
+    if (!tsplat.empty()) {
+        std::fill(splat_rgb, splat_rgb + full, 0.0);
+        for (auto& b : tsplat)
+            for (size_t k = 0; k < full; k++) splat_rgb[k] += b[k];
+    }
     return PRT_OK;
 }
 
 // Radiance of individual samples: pixels[i] = (x, y), samples[i] = sample index.
 int orc_render_samples(const orc_scene* os, const prt_render_params* p, const int32_t* pixels,
-                       const int32_t* samples, int64_t n, double* out_rgb, int32_t nthreads) {
+                       const int32_t* samples, int64_t n, double* out_rgb, int32_t nthreads, double* splat_rgb) {
     const Scene& S = os->s;
+    const bool bdpt = p->integrator == PRT_INTEGRATOR_BDPT;
     PTConfig cfg;
     cfg.max_depth = p->max_depth;
     cfg.rr_min = p->rr_min_bounces;
+    BDPTCtx bc{S, p->max_depth};
     int W = S.cam.width;
+    const size_t full = (size_t)S.cam.width * S.cam.height * 3;
+    std::vector<std::vector<Splat>> per(bdpt ? n : 0);
     parallel_for(n, nthreads, [&](int64_t b, int64_t e) {
         for (int64_t k = b; k < e; k++) {
             int i = pixels[2 * k], j = pixels[2 * k + 1];
@@ -1323,13 +1944,72 @@ int orc_render_samples(const orc_scene* os, const prt_render_params* p, const in
             Vec2 a = smp.get2d();
             Vec2 bb = smp.get2d();
             Ray r = S.cam.get_ray(i, j, a, bb);
-            Vec3 c = ray_color(S, cfg, r, smp, cfg.max_depth, Vec3(1, 1, 1));
+            Vec3 c = bdpt ? bdpt_ray_color(bc, r, smp, per[k]) : ray_color(S, cfg, r, smp, cfg.max_depth, Vec3(1, 1, 1));
             out_rgb[3 * k] = c.x;
             out_rgb[3 * k + 1] = c.y;
             out_rgb[3 * k + 2] = c.z;
         }
     });
+    if (bdpt && splat_rgb) {
+        std::fill(splat_rgb, splat_rgb + full, 0.0);
+        for (int64_t k = 0; k < n; k++)
+            for (auto& sp : per[k]) {
+                int sx, sy;
+                if (cam_map_ray_to_pixel(S.cam, sp.ray, sx, sy)) {
+                    double* o = &splat_rgb[((size_t)sy * S.cam.width + sx) * 3];
+                    o[0] += sp.color.x;
+                    o[1] += sp.color.y;
+                    o[2] += sp.color.z;
+                }
+            }
+    }
     return PRT_OK;
+}
+
+// MIS weight of strategy (s, t) for hand-built paths (calculateMISWeight, bdpt_mis.go:13-48).
+// Vertex flags: 1 light, 2 camera, 4 specular, 8 infinite light. light_ref indexes the lights of
+// `pool` (the light object a vertex carries, -1 for none); light_index is its index in `os`.
+typedef struct orc_vertex {
+    double p[3], n[3];
+    int32_t mat, light_ref, light_index, flags;
+    double pdf_fwd, pdf_rev;
+} orc_vertex;
+
+static BVertex from_orc(const orc_vertex& o, const orc_scene* pool) {
+    BVertex v;
+    v.p = V(o.p);
+    v.n = V(o.n);
+    v.mat = o.mat;
+    v.light = (pool && o.light_ref >= 0) ? &pool->s.lights[o.light_ref] : nullptr;
+    v.light_index = o.light_index;
+    v.is_light = (o.flags & 1) != 0;
+    v.is_camera = (o.flags & 2) != 0;
+    v.is_specular = (o.flags & 4) != 0;
+    v.is_inf = (o.flags & 8) != 0;
+    v.pdf_fwd = o.pdf_fwd;
+    v.pdf_rev = o.pdf_rev;
+    return v;
+}
+
+double orc_bdpt_mis(const orc_scene* os, const orc_scene* pool, const orc_vertex* cam, int32_t ncam,
+                    const orc_vertex* light, int32_t nlight, const orc_vertex* sampled, int32_t s, int32_t t) {
+    BPath cp, lp;
+    for (int i = 0; i < ncam; i++) cp.v.push_back(from_orc(cam[i], pool));
+    for (int i = 0; i < nlight; i++) lp.v.push_back(from_orc(light[i], pool));
+    BVertex sv;
+    if (sampled) sv = from_orc(*sampled, pool);
+    return mis_weight(os->s, cp, lp, sampled ? &sv : nullptr, s, t);
+}
+
+// Camera importance helpers (camera.go:194-316): pdfs of a ray and its pixel.
+void orc_camera_ray_pdfs(const orc_scene* os, const double* o, const double* d, double* pos, double* dir,
+                         int32_t* px, int32_t* py) {
+    Ray r{V(o), V(d)};
+    cam_ray_pdfs(os->s.cam, r, *pos, *dir);
+    int x = -1, y = -1;
+    if (!cam_map_ray_to_pixel(os->s.cam, r, x, y)) x = y = -1;
+    *px = x;
+    *py = y;
 }
 
 // ---- unit-level entry points used by the golden-vector tests ------------------------

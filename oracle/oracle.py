@@ -33,8 +33,10 @@ def lib():
             "orc_scene_destroy": (None, [vp]),
             "orc_scene_world": (None, [vp, vp, vp, vp]),
             "orc_intersect": (None, [vp, vp, i64, vp, vp, vp, vp, i32, i32, vp]),
-            "orc_render": (C.c_int, [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32]),
-            "orc_render_samples": (C.c_int, [vp, vp, vp, vp, i64, vp, i32]),
+            "orc_render": (C.c_int, [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32, vp]),
+            "orc_render_samples": (C.c_int, [vp, vp, vp, vp, i64, vp, i32, vp]),
+            "orc_bdpt_mis": (d, [vp, vp, vp, i32, vp, i32, vp, i32, i32]),
+            "orc_camera_ray_pdfs": (None, [vp, vp, vp, vp, vp, vp, vp]),
             "orc_draw": (d, [u64, u32, u32, u32]),
             "orc_sample_cosine_hemisphere": (None, [vp, vp, vp]),
             "orc_sample_on_unit_sphere": (None, [vp, vp]),
@@ -61,6 +63,20 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+ORC_VERTEX = np.dtype([("p", "<f8", 3), ("n", "<f8", 3), ("mat", "<i4"), ("light_ref", "<i4"),
+                       ("light_index", "<i4"), ("flags", "<i4"), ("pdf_fwd", "<f8"), ("pdf_rev", "<f8")])
+V_LIGHT, V_CAMERA, V_SPECULAR, V_INFINITE = 1, 2, 4, 8
+
+
+def bdpt_mis(scene, pool, camera_path, light_path, sampled, s, t):
+    """MIS weight of strategy (s, t) for hand-built vertex arrays of ORC_VERTEX records."""
+    cp = np.ascontiguousarray(camera_path, dtype=ORC_VERTEX)
+    lp = np.ascontiguousarray(light_path, dtype=ORC_VERTEX)
+    sv = None if sampled is None else np.ascontiguousarray(sampled, dtype=ORC_VERTEX).reshape(1)
+    return lib().orc_bdpt_mis(scene.h, pool.h if pool is not None else None, _p(cp) if len(cp) else None, len(cp),
+                              _p(lp) if len(lp) else None, len(lp), _p(sv) if sv is not None else None, s, t)
 
 
 def _p(a):
@@ -108,23 +124,30 @@ class OracleScene:
             return t, prim, u, v, cnt
         return t, prim, u, v
 
-    def render(self, params, spp, sample_offset=0, rect=None, nthreads=0):
-        """Per-pixel sums over samples [offset, offset+spp): (sum_rgb (H,W,3), lum_sum, lum_sq)."""
+    def render(self, params, spp, sample_offset=0, rect=None, nthreads=0, splats=False):
+        """Per-pixel sums over samples [offset, offset+spp): (sum_rgb (H,W,3), lum_sum, lum_sq[, splat_rgb])."""
         W, H = self.desc.image_size
         x0, y0, x1, y1 = rect if rect else (0, 0, W, H)
         h, w = y1 - y0, x1 - x0
         s = np.zeros((h, w, 3))
         ls = np.zeros((h, w))
         lq = np.zeros((h, w))
+        sp = np.zeros((H, W, 3)) if splats else None
         rc = lib().orc_render(self.h, C.byref(params), spp, sample_offset, x0, y0, x1, y1, _p(s), _p(ls), _p(lq),
-                              nthreads)
+                              nthreads, _p(sp) if splats else None)
         assert rc == 0, rc
+        if splats:
+            return s, ls, lq, sp
         return s, ls, lq
 
-    def render_samples(self, params, pixels, samples, nthreads=0):
+    def render_samples(self, params, pixels, samples, nthreads=0, splats=False):
         pixels = np.ascontiguousarray(pixels, dtype=np.int32).reshape(-1, 2)
         samples = np.ascontiguousarray(samples, dtype=np.int32).reshape(-1)
         out = np.empty((pixels.shape[0], 3))
+        W, H = self.desc.image_size
+        sp = np.zeros((H, W, 3)) if splats else None
         lib().orc_render_samples(self.h, C.byref(params), _p(pixels), _p(samples), pixels.shape[0], _p(out),
-                                 nthreads)
+                                 nthreads, _p(sp) if splats else None)
+        if splats:
+            return out, sp
         return out

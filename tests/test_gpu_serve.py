@@ -1,0 +1,105 @@
+"""Serving loop (include/prt_server.h): continuous batching of progressive renders.
+
+Isolation bar: each request's final film is bit-identical whether it ran alone or among other
+requests (counter RNG per pixel/sample and per-request chunking make results independent of the mix).
+"""
+import numpy as np
+import pytest
+
+import prtgpu as P
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(w, h, cfg, spp, passes, seed):
+    return P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp, max_passes=passes, seed=seed)
+
+
+def _drain(server, ids, timeout_s=120):
+    import time
+
+    done, events = set(), {i: [] for i in ids}
+    t0 = time.time()
+    while len(done) < len(ids):
+        for e in server.poll(timeout_ms=50):
+            if e.request_id in events:
+                events[e.request_id].append((e.type, e.pass_number, e.target_samples))
+                if e.type in (P.EVENT_DONE, P.EVENT_ERROR, P.EVENT_CANCELLED):
+                    done.add(e.request_id)
+        assert time.time() - t0 < timeout_s, "serving loop did not finish"
+    return events
+
+
+@pytest.fixture(scope="module")
+def sc():
+    d0, c0 = scenes.default_scene(width=96)
+    d2, c2 = scenes.cornell_metre(width=64)
+    return {"default": (P.Scene(d0), c0), "cornell": (P.Scene(d2), c2)}
+
+
+def _requests():
+    # (scene, width, height, spp, passes, seed)
+    return [("default", 96, 54, 24, 4, 1), ("cornell", 64, 64, 40, 5, 2), ("default", 128, 72, 16, 3, 3),
+            ("cornell", 80, 48, 20, 4, 4), ("default", 64, 36, 12, 2, 5), ("cornell", 48, 48, 30, 6, 6)]
+
+
+def test_isolation_alone_vs_mix(sc):
+    reqs = _requests()
+    srv = P.Server(max_active=8)
+    ids = []
+    for name, w, h, spp, passes, seed in reqs:
+        scene, cfg = sc[name]
+        ids.append(srv.submit(scene, _params(w, h, cfg, spp, passes, seed)))
+    ev = _drain(srv, ids)
+    mix = [srv.read(i, w, h) for i, (_, w, h, _, _, _) in zip(ids, reqs)]
+    for rid, (name, w, h, spp, passes, seed) in zip(ids, reqs):
+        kinds = [k for k, _, _ in ev[rid]]
+        assert kinds[0] == P.EVENT_ADMITTED and kinds[-1] == P.EVENT_DONE
+        passes_seen = [p for k, p, _ in ev[rid] if k == P.EVENT_PASS_COMPLETE]
+        assert passes_seen == list(range(1, passes + 1))
+        targets = [t for k, _, t in ev[rid] if k == P.EVENT_PASS_COMPLETE]
+        assert targets[0] == 1 and targets[-1] == spp
+    for rid in ids:
+        srv.release(rid)
+    srv.close()
+    # each request alone on a fresh server
+    for (name, w, h, spp, passes, seed), (m_mix, c_mix) in zip(reqs, mix):
+        solo = P.Server(max_active=1)
+        scene, cfg = sc[name]
+        rid = solo.submit(scene, _params(w, h, cfg, spp, passes, seed))
+        _drain(solo, [rid])
+        m, c = solo.read(rid, w, h)
+        assert np.all(c == spp) and np.all(c_mix == spp)
+        assert np.array_equal(m, m_mix), f"{name} {w}x{h} differs between the mix and the solo render"
+        solo.close()
+
+
+def test_resolution_override_matches_native_camera():
+    d_native, cfg = scenes.default_scene(width=128)  # 128 x 72 natively
+    d_base, _ = scenes.default_scene(width=96)
+    srv = P.Server(max_active=2)
+    a = srv.submit(P.Scene(d_native), _params(0, 0, cfg, 4, 1, 7))
+    b_scene = P.Scene(d_base)
+    b = srv.submit(b_scene, _params(128, 72, cfg, 4, 1, 7))
+    _drain(srv, [a, b])
+    ma, _ = srv.read(a, 128, 72)
+    mb, _ = srv.read(b, 128, 72)
+    assert np.array_equal(ma, mb)
+    srv.close()
+
+
+def test_cancel_and_budget(sc):
+    scene, cfg = sc["cornell"]
+    # a 64 MiB budget holds one 64x64 film slot (~47 MB) at a time: requests are admitted one after another
+    srv = P.Server(hbm_budget_bytes=64 << 20, max_active=8)
+    ids = [srv.submit(scene, _params(64, 64, cfg, 8, 2, s)) for s in range(4)]
+    big = srv.submit(scene, _params(64, 64, cfg, 100000, 1000, 9))
+    srv.cancel(big)
+    ev = _drain(srv, ids + [big])
+    assert ev[big][-1][0] == P.EVENT_CANCELLED
+    for i in ids:
+        assert ev[i][-1][0] == P.EVENT_DONE
+    st = srv.stats()
+    assert st.completed == 4 and st.active == 0
+    srv.close()
