@@ -203,8 +203,8 @@ def pt_section(a, log):
             import oracle as O
 
             os_ = O.OracleScene(desc)
-            # bounded sample: the whole image at a few samples per pixel (~1-5 s of CPU work)
-            c_spp = 4 if w * h <= 400 * 225 else 1
+            # bounded sample: the whole image at a few samples per pixel (~2-4 s of CPU work)
+            c_spp = {0: 256, 1: 64, 2: 8}[len(out)]
             c0 = time.perf_counter()
             os_.render(params, c_spp, 0, nthreads=cores)
             ct = time.perf_counter() - c0
@@ -222,7 +222,8 @@ def pt_section(a, log):
 def serve_section(a, log):
     """Poisson arrivals (rate 20/s, seed 0) of progressive path-tracing requests mixing the three
     render scenes x {256x256, 512x512, 1280x720, 1920x1080} x spp log-uniform in [16, 1024], 7 passes,
-    through the device-resident serving loop (HBM budget 64 GiB, up to 32 active requests).
+    through the device-resident serving loop (HBM budget 192 GiB of the 288 GiB, admission bounded by
+    that budget).
     Time to first pass = submit -> pass 1 (1 spp) readable, per request."""
     import numpy as np
 
@@ -238,7 +239,7 @@ def serve_section(a, log):
     picks = rng.integers(0, 3, n)
     res = rng.integers(0, 4, n)
     spps = np.exp(rng.uniform(np.log(16), np.log(1024), n)).astype(int)
-    srv = P.Server(hbm_budget_bytes=64 << 30, max_active=32)
+    srv = P.Server(hbm_budget_bytes=192 << 30, max_active=1024)
     ids, meta = [], {}
     t0 = time.perf_counter()
     ttfp, done_t, samples = {}, {}, 0

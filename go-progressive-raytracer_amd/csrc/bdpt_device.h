@@ -1,0 +1,363 @@
+// bdpt_device.h — float32 device bidirectional path tracer (pkg/integrator/bdpt.go, bdpt_mis.go).
+//
+// One thread per camera sample: camera subpath, light subpath, then every (s, t) strategy with its
+// MIS weight (calculateMISWeight, bdpt_mis.go:13-48) and an any-hit visibility test. Subpath vertices
+// live in global memory, vertex k of sample i at [k * stride + i] (coalesced across a wave). Light
+// tracing (t = 1) contributions are splatted to the pixel the camera ray maps to (MapRayToPixel) with
+// 64-bit fixed-point atomics, so the splat image is independent of the order of the additions.
+#pragma once
+#include "shading.h"
+#include "traverse.h"
+
+namespace prt {
+
+// Draw slots of the counter stream: camera path bounce b: 1+b; light emission: kLightSlot; light path
+// bounce b: kLightSlot+1+b; direct lighting at camera vertex t: kDirectSlot+t; light tracing from light
+// vertex s: kSplatSlot+s (positions k within a slot).
+constexpr uint32_t kLightSlot = 70, kDirectSlot = 140, kSplatSlot = 210;
+constexpr int kBdptMaxDepth = 64;
+constexpr float kSplatScale = 16777216.0f;  // fixed point: 2^24 units per radiance unit
+
+enum : uint32_t { VF_LIGHT = 1, VF_CAMERA = 2, VF_SPECULAR = 4, VF_INF = 8, VF_FRONT = 16 };
+
+// Vertex, bdpt.go:14-44 (Light is the index of a scene light, -1 when the vertex carries none)
+struct BV {
+    V3 p, n, wi, beta, Le;
+    float pdf_fwd, pdf_rev;
+    int mat, light;
+    uint32_t flags;
+};
+
+struct VertexStore {
+    float4* data;     // 5 float4 per vertex
+    uint32_t stride;  // samples per chunk
+    __device__ __forceinline__ void put(int k, uint32_t i, const BV& v) const {
+        float4* b = data + (size_t)5 * ((size_t)k * stride + i);
+        b[0] = make_float4(v.p.x, v.p.y, v.p.z, v.pdf_fwd);
+        b[1] = make_float4(v.n.x, v.n.y, v.n.z, v.pdf_rev);
+        b[2] = make_float4(v.wi.x, v.wi.y, v.wi.z, __int_as_float(v.mat));
+        b[3] = make_float4(v.beta.x, v.beta.y, v.beta.z, __int_as_float(v.light));
+        b[4] = make_float4(v.Le.x, v.Le.y, v.Le.z, __uint_as_float(v.flags));
+    }
+    __device__ __forceinline__ BV get(int k, uint32_t i) const {
+        const float4* b = data + (size_t)5 * ((size_t)k * stride + i);
+        float4 a = b[0], c = b[1], d = b[2], e = b[3], f = b[4];
+        BV v;
+        v.p = v3(a.x, a.y, a.z);
+        v.pdf_fwd = a.w;
+        v.n = v3(c.x, c.y, c.z);
+        v.pdf_rev = c.w;
+        v.wi = v3(d.x, d.y, d.z);
+        v.mat = __float_as_int(d.w);
+        v.beta = v3(e.x, e.y, e.z);
+        v.light = __float_as_int(e.w);
+        v.Le = v3(f.x, f.y, f.z);
+        v.flags = __float_as_uint(f.w);
+        return v;
+    }
+    __device__ __forceinline__ void set_pdf_rev(int k, uint32_t i, float pdf) const {
+        data[(size_t)5 * ((size_t)k * stride + i) + 1].w = pdf;
+    }
+    __device__ __forceinline__ void set_pdf_fwd(int k, uint32_t i, float pdf) const {
+        data[(size_t)5 * ((size_t)k * stride + i)].w = pdf;
+    }
+};
+
+__device__ __forceinline__ bool light_area(const DevScene& sc, int li) {
+    int t = sc.lights[li].type;
+    return t == PRT_LIGHT_QUAD || t == PRT_LIGHT_SPHERE;
+}
+__device__ __forceinline__ bool on_surface(const DevScene& sc, const BV& v) {  // IsOnSurface :33-41
+    if ((v.flags & VF_LIGHT) && v.light >= 0) return light_area(sc, v.light);
+    return v.mat >= 0;
+}
+__device__ __forceinline__ Surf surf_of(const BV& v) {
+    Surf s;
+    s.p = v.p;
+    s.n = v.n;
+    s.front = (v.flags & VF_FRONT) != 0;
+    s.mat = v.mat;
+    return s;
+}
+
+// convertSolidAngleToAreaPdf, bdpt_mis.go:341-362
+__device__ __forceinline__ float convert_density(const DevScene& sc, const BV& from, const BV& next, float pdf) {
+    if (next.flags & VF_INF) return pdf;
+    V3 d = next.p - from.p;
+    float d2 = dot(d, d);
+    if (d2 == 0.0f) return 0.0f;
+    float inv = 1.0f / d2;
+    if (on_surface(sc, next)) pdf *= fabsf(dot(d * sqrtf(inv), next.n));
+    return pdf * inv;
+}
+__device__ __forceinline__ float remap0(float f) { return f != 0.0f ? f : 1.0f; }
+
+// Camera.CalculateRayPDFs / MapRayToPixel / SampleCameraFromPoint (camera.go:194-316)
+__device__ __forceinline__ bool cam_map(const DevCamera& c, V3 o, V3 d, int& px, int& py) {
+    V3 w = v3(c.w);
+    V3 ipc = v3(c.center) - w * c.focus;
+    float t = dot(ipc - o, w) / dot(d, w);
+    if (!(t > 0.0f)) return false;
+    V3 rel = (o + d * t) - ipc;
+    float nx = (dot(rel, v3(c.u)) + c.vw / 2.0f) / c.vw, ny = (dot(rel, v3(c.v)) + c.vh / 2.0f) / c.vh;
+    float fx = nx * (float)c.width, fy = ny * (float)c.height;
+    if (!(fx > -1.0f && fx < (float)c.width + 1.0f && fy > -1.0f && fy < (float)c.height + 1.0f)) return false;
+    px = (int)fx;  // truncation toward zero as Go's int()
+    py = (int)fy;
+    return px >= 0 && px < c.width && py >= 0 && py < c.height;
+}
+__device__ __forceinline__ float cam_dir_pdf(const DevCamera& c, V3 o, V3 d) {
+    float ct = dot(d, v3(c.forward));
+    if (ct <= c.cos_total_width) return 0.0f;
+    int x, y;
+    if (!cam_map(c, o, d, x, y)) return 0.0f;
+    return 1.0f / (c.plane_area * ct * ct * ct);
+}
+
+// EmissionSample + Light.SampleEmission / PDF_Le (light_sampling.go:54-115, *_light.go)
+struct ESample {
+    V3 p, n, dir, emission;
+    float area_pdf, dir_pdf;
+};
+__device__ __forceinline__ V3 sample_on_unit_sphere(float s0, float s1) {  // sampling.go:97-104
+    float z = 1.0f - 2.0f * s0;
+    float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sp, cp;
+    sincosf(2.0f * kPiF * s1, &sp, &cp);
+    return v3(r * cp, r * sp, z);
+}
+__device__ __forceinline__ void infinite_disk(const DevScene& sc, V3 d, float s0, float s1, ESample& es) {
+    V3 up = fabsf(d.x) > 0.9f ? v3(0, 1, 0) : v3(1, 0, 0);
+    V3 right = normalize(cross(d, up));
+    up = normalize(cross(right, d));
+    V3 ds = sample_point_in_unit_disk(s0, s1);
+    float R = sc.world_radius;
+    V3 dp = v3(sc.world_center) + right * (ds.x * R) + up * (ds.y * R);
+    es.p = dp + d * -R;
+    es.dir = d;
+    es.n = d * -1.0f;
+    es.area_pdf = 1.0f / (kPiF * R * R);
+}
+__device__ __forceinline__ ESample light_sample_emission(const DevScene& sc, const DevLight& L, float p0, float p1,
+                                                         float d0, float d1) {
+    ESample es;
+    if (L.type == PRT_LIGHT_QUAD) {
+        es.p = v3(L.corner) + v3(L.u) * p0 + v3(L.v) * p1;
+        es.n = v3(L.n);
+        es.dir = sample_cosine_hemisphere(es.n, d0, d1);
+        es.area_pdf = 1.0f / L.area;
+        es.dir_pdf = dot(es.dir, es.n) / kPiF;
+        es.emission = emissive(sc, L.mat);
+    } else if (L.type == PRT_LIGHT_SPHERE) {
+        V3 ld = sample_on_unit_sphere(p0, p1);
+        es.p = v3(L.center) + ld * L.radius;
+        es.n = ld;
+        es.area_pdf = 1.0f / (4.0f * kPiF * L.radius * L.radius);
+        es.dir = sample_cosine_hemisphere(ld, d0, d1);
+        es.dir_pdf = dot(es.dir, ld) / kPiF;
+        es.emission = emissive(sc, L.mat);
+    } else if (L.type == PRT_LIGHT_ENVMAP) {
+        float pdf;
+        V3 wi = env_sample(sc.env, d0, d1, pdf);
+        infinite_disk(sc, wi * -1.0f, p0, p1, es);
+        es.dir_pdf = pdf;
+        es.emission = env_eval(sc.env, wi);
+    } else {
+        V3 d = sample_on_unit_sphere(d0, d1);
+        infinite_disk(sc, d, p0, p1, es);
+        es.dir_pdf = 1.0f / (4.0f * kPiF);
+        if (L.type == PRT_LIGHT_UNIFORM_INFINITE) {
+            es.emission = v3(L.emission);
+        } else {
+            float t = 0.5f * (d.y + 1.0f);
+            es.emission = v3(L.bottom) * (1.0f - t) + v3(L.top) * t;
+        }
+    }
+    return es;
+}
+__device__ __forceinline__ void light_pdf_le(const DevScene& sc, const DevLight& L, V3 p, V3 d, float& pos, float& dir) {
+    pos = dir = 0.0f;
+    if (L.type == PRT_LIGHT_QUAD) {  // quad_light.go:129-176
+        V3 tp = p - v3(L.corner), U = v3(L.u), W = v3(L.v);
+        float uu = dot(U, U), vv = dot(W, W), uv = dot(U, W);
+        if (uu == 0.0f || vv == 0.0f) return;
+        float det = uu * vv - uv * uv;
+        if (fabsf(det) < 1e-8f) return;
+        float tu = dot(tp, U), tv = dot(tp, W);
+        float a = (vv * tu - uv * tv) / det, b = (uu * tv - uv * tu) / det;
+        if (a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f) return;
+        V3 rec = v3(L.corner) + U * a + W * b;
+        if (length(rec - p) > 0.001f) return;
+        pos = 1.0f / L.area;
+        float ct = dot(d, v3(L.n));
+        if (ct <= 0.0f) return;
+        dir = ct / kPiF;
+    } else if (L.type == PRT_LIGHT_SPHERE) {  // sphere_light.go:184-205
+        V3 c = v3(L.center);
+        if (!(fabsf(length(p - c) - L.radius) <= 0.001f)) return;
+        V3 n = normalize(p - c);
+        float ct = dot(d, n);
+        if (ct <= 0.0f) return;
+        pos = 1.0f / (4.0f * kPiF * L.radius * L.radius);
+        dir = ct / kPiF;
+    } else {
+        float R = sc.world_radius;
+        if (R <= 0.0f) return;
+        pos = 1.0f / (kPiF * R * R);
+        dir = L.type == PRT_LIGHT_ENVMAP ? env_pdf(sc.env, d * -1.0f) : 1.0f / (4.0f * kPiF);
+    }
+}
+
+// InfiniteLightDensity, bdpt_mis.go:318-337
+__device__ __forceinline__ float infinite_density(const DevScene& sc, V3 p, V3 n, V3 d) {
+    float tot = 0.0f;
+    for (int i = 0; i < sc.num_lights; i++)
+        if (light_is_infinite(sc.lights[i].type)) tot += light_pdf(sc, sc.lights[i], p, n, d) * sc.light_weights[i];
+    return tot;
+}
+
+// evaluateBRDF, bdpt.go:564-583
+__device__ __forceinline__ V3 vertex_brdf(const DevScene& sc, const BV& v, V3 out, int mode) {
+    if (v.flags & VF_INF) return v3(0, 0, 0);
+    if ((v.flags & VF_LIGHT) && v.mat < 0) return v3(1, 1, 1);
+    if (v.mat < 0) return v3(0, 0, 0);
+    return brdf(sc, v.mat, v.wi, out, surf_of(v), mode);
+}
+
+// calculateLightPdf / calculateLightOriginPdf / calculateVertexPdf, bdpt_mis.go:190-315
+__device__ __forceinline__ float vtx_light_pdf(const DevScene& sc, const BV& cur, const BV& to) {
+    V3 w = to.p - cur.p;
+    float inv = 1.0f / dot(w, w);
+    w = w * sqrtf(inv);
+    float pdf = 0.0f;
+    if (cur.flags & VF_LIGHT) {
+        if (cur.flags & VF_INF) {
+            float R = sc.world_radius;
+            if (R == 0.0f) R = 1.0f;
+            pdf = 1.0f / (kPiF * R * R);
+        } else if (cur.light >= 0) {
+            float pos, dir;
+            light_pdf_le(sc, sc.lights[cur.light], cur.p, w, pos, dir);
+            pdf = dir * inv;
+        }
+    }
+    if (on_surface(sc, to)) pdf *= fabsf(dot(to.n, w));
+    return pdf;
+}
+__device__ __forceinline__ float vtx_light_origin_pdf(const DevScene& sc, const BV& lv, const BV& to) {
+    V3 w = to.p - lv.p;
+    if (dot(w, w) == 0.0f) return 0.0f;
+    w = normalize(w);
+    if (lv.flags & VF_INF) return infinite_density(sc, to.p, to.n, w * -1.0f);
+    if (!(lv.flags & VF_LIGHT) || lv.light < 0 || sc.num_lights == 0) return 0.0f;
+    float pos, dir;
+    light_pdf_le(sc, sc.lights[lv.light], lv.p, w, pos, dir);
+    return pos * sc.light_weights[lv.light];
+}
+__device__ __forceinline__ float vtx_pdf(const DevScene& sc, const DevCamera& cam, const BV& cur, const BV* prev,
+                                         const BV& next) {
+    if (cur.flags & VF_LIGHT) return vtx_light_pdf(sc, cur, next);
+    V3 wn = next.p - cur.p;
+    if (dot(wn, wn) == 0.0f) return 0.0f;
+    wn = normalize(wn);
+    V3 wp = v3(0, 0, 0);
+    if (prev) {
+        wp = prev->p - cur.p;
+        if (dot(wp, wp) == 0.0f) return 0.0f;
+        wp = normalize(wp);
+    } else if (!(cur.flags & VF_CAMERA)) {
+        return 0.0f;
+    }
+    float pdf;
+    if (cur.flags & VF_CAMERA) {
+        pdf = cam_dir_pdf(cam, cur.p, wn);
+        if (pdf == 0.0f) return 0.0f;
+    } else if (cur.mat >= 0) {
+        bool delta;
+        pdf = mat_pdf(sc, cur.mat, wp, wn, cur.n, delta);
+        if (delta) return 0.0f;
+    } else {
+        return 0.0f;
+    }
+    return convert_density(sc, cur, next, pdf);
+}
+
+// calculateMISWeight with the per-vertex rules of calculateMIS{Camera,Light}VertexPdfs (:52-187)
+__device__ float mis_weight(const DevScene& sc, const DevCamera& cam, const VertexStore& CP, const VertexStore& LP,
+                            uint32_t i, const BV* sv, int s, int t) {
+    if (s + t == 2) return 1.0f;
+    float sum = 0.0f, ri = 1.0f;
+    for (int k = t - 1; k > 0; k--) {
+        BV v = CP.get(k, i);
+        float fwd = v.pdf_fwd, rev = v.pdf_rev;
+        bool conn = !(v.flags & VF_SPECULAR);
+        if (s == 0) {
+            if (k == t - 1 && t > 1) {
+                rev = vtx_light_origin_pdf(sc, v, CP.get(t - 2, i));
+                conn = true;
+            } else if (k == t - 2 && t > 2) {
+                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), nullptr, v);
+            }
+        } else if (t == 1) {
+            // k > 0 never reaches the camera vertex here
+        } else if (s == 1) {
+            if (k == t - 1) {
+                rev = vtx_pdf(sc, cam, *sv, nullptr, v);
+                conn = true;
+            } else if (k == t - 2 && t > 1) {
+                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), sv, v);
+            }
+        } else {
+            if (k == t - 1) {
+                BV ls2 = LP.get(s - 2, i);
+                rev = vtx_pdf(sc, cam, LP.get(s - 1, i), &ls2, v);
+                conn = true;
+            } else if (k == t - 2 && t > 1) {
+                BV ls1 = LP.get(s - 1, i);
+                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), &ls1, v);
+            }
+        }
+        conn = conn && !(CP.get(k - 1, i).flags & VF_SPECULAR);
+        ri *= remap0(rev) / remap0(fwd);
+        if (conn) sum += ri;
+    }
+    if (t == 1) {
+        // camera vertex 0 takes the sampled camera vertex's densities
+    }
+    ri = 1.0f;
+    for (int k = s - 1; k >= 0; k--) {
+        BV v = LP.get(k, i);
+        float fwd = v.pdf_fwd, rev = v.pdf_rev;
+        bool conn = !(v.flags & VF_SPECULAR);
+        if (k > 0) conn = conn && !(LP.get(k - 1, i).flags & VF_SPECULAR);
+        if (t == 1) {
+            if (k == s - 1 && s > 1) {
+                rev = vtx_pdf(sc, cam, *sv, nullptr, v);
+                conn = true;
+            } else if (k == s - 2 && s > 1) {
+                rev = vtx_pdf(sc, cam, LP.get(s - 1, i), sv, v);
+            }
+        } else if (s == 1) {
+            if (k == s - 1 && sv) {
+                fwd = sv->pdf_fwd;
+                BV c2 = CP.get(t - 2, i);
+                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), &c2, *sv);
+                conn = true;
+            }
+        } else {
+            if (k == s - 1) {
+                BV c2 = CP.get(t - 2, i);
+                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), &c2, v);
+                conn = true;
+            } else if (k == s - 2 && s > 1) {
+                BV c1 = CP.get(t - 1, i);
+                rev = vtx_pdf(sc, cam, LP.get(s - 1, i), &c1, v);
+            }
+        }
+        ri *= remap0(rev) / remap0(fwd);
+        if (conn) sum += ri;
+    }
+    return 1.0f / (1.0f + sum);
+}
+
+}  // namespace prt

@@ -195,72 +195,76 @@ struct Collapser {
         return make_leaf(first, n.count);
     }
 
-    // Emits the 4-wide node for binary node `n2` (inner or, at the root only, a leaf).
-    int32_t emit(int32_t n2, int depth) {
-        out.depth = std::max(out.depth, depth);
-        int32_t id = (int32_t)out.nodes.size();
+    // Emits the 4-wide tree in breadth-first order (the top levels are the first nodes, so a
+    // kernel can keep them in LDS), starting from binary node `root2` (inner or a single leaf).
+    void emit_bfs(int32_t root2) {
+        struct Item {
+            int32_t n2;
+            int depth;
+            int32_t id;
+        };
+        std::vector<Item> queue;
+        queue.reserve(b.nodes.size() / 2 + 4);
         out.nodes.emplace_back();
-        int32_t kids[4];
-        int nk = 0;
-        const Node2& root = b.nodes[n2];
-        if (root.count > 0 || (root.left < 0 && root.right < 0)) {
-            kids[nk++] = n2;
-        } else {
-            kids[nk++] = root.left;
-            kids[nk++] = root.right;
-            while (nk < 4) {
-                int best = -1;
-                double best_area = -1;
-                for (int k = 0; k < nk; k++) {
-                    const Node2& c = b.nodes[kids[k]];
-                    if (c.count > 0) continue;
-                    double a = c.box.area();
-                    if (a > best_area) {
-                        best_area = a;
-                        best = k;
-                    }
-                }
-                if (best < 0) break;
-                int32_t open = kids[best];
-                kids[best] = b.nodes[open].left;
-                kids[nk++] = b.nodes[open].right;
-            }
-        }
-        Node4 node;
-        for (int k = 0; k < 4; k++) {
-            node.lox[k] = node.loy[k] = node.loz[k] = std::numeric_limits<float>::infinity();
-            node.hix[k] = node.hiy[k] = node.hiz[k] = -std::numeric_limits<float>::infinity();
-            node.child[k] = kEmptyChild;
-            node.pad[k] = 0;
-        }
-        uint32_t words[4];
-        for (int k = 0; k < nk; k++) {
-            const Node2& c = b.nodes[kids[k]];
-            if (c.count > 0) {
-                words[k] = leaf_word(c);
-            } else if (c.left < 0) {
-                words[k] = kEmptyChild;  // empty leaf (no primitives)
+        queue.push_back({root2, 1, 0});
+        for (size_t qi = 0; qi < queue.size(); qi++) {
+            const Item it = queue[qi];
+            out.depth = std::max(out.depth, it.depth);
+            int32_t kids[4];
+            int nk = 0;
+            const Node2& root = b.nodes[it.n2];
+            if (root.count > 0 || (root.left < 0 && root.right < 0)) {
+                kids[nk++] = it.n2;
             } else {
-                words[k] = 0;  // filled after recursion
+                kids[nk++] = root.left;
+                kids[nk++] = root.right;
+                while (nk < 4) {
+                    int best = -1;
+                    double best_area = -1;
+                    for (int k = 0; k < nk; k++) {
+                        const Node2& c = b.nodes[kids[k]];
+                        if (c.count > 0) continue;
+                        double a = c.box.area();
+                        if (a > best_area) {
+                            best_area = a;
+                            best = k;
+                        }
+                    }
+                    if (best < 0) break;
+                    int32_t open = kids[best];
+                    kids[best] = b.nodes[open].left;
+                    kids[nk++] = b.nodes[open].right;
+                }
             }
+            Node4 node;
+            for (int k = 0; k < 4; k++) {
+                node.lox[k] = node.loy[k] = node.loz[k] = std::numeric_limits<float>::infinity();
+                node.hix[k] = node.hiy[k] = node.hiz[k] = -std::numeric_limits<float>::infinity();
+                node.child[k] = kEmptyChild;
+                node.pad[k] = 0;
+            }
+            for (int k = 0; k < nk; k++) {
+                const Node2& c = b.nodes[kids[k]];
+                uint32_t word;
+                if (c.count > 0) {
+                    word = leaf_word(c);
+                } else if (c.left < 0) {
+                    continue;  // empty leaf (no primitives)
+                } else {
+                    word = (uint32_t)out.nodes.size();
+                    out.nodes.emplace_back();
+                    queue.push_back({kids[k], it.depth + 1, (int32_t)word});
+                }
+                node.lox[k] = round_down(c.box.lo[0]);
+                node.loy[k] = round_down(c.box.lo[1]);
+                node.loz[k] = round_down(c.box.lo[2]);
+                node.hix[k] = round_up(c.box.hi[0]);
+                node.hiy[k] = round_up(c.box.hi[1]);
+                node.hiz[k] = round_up(c.box.hi[2]);
+                node.child[k] = word;
+            }
+            out.nodes[it.id] = node;
         }
-        for (int k = 0; k < nk; k++) {
-            const Node2& c = b.nodes[kids[k]];
-            if (c.count == 0 && c.left >= 0) words[k] = (uint32_t)emit(kids[k], depth + 1);
-        }
-        for (int k = 0; k < nk; k++) {
-            const Node2& c = b.nodes[kids[k]];
-            if (words[k] == kEmptyChild) continue;
-            node.lox[k] = round_down(c.box.lo[0]);
-            node.loy[k] = round_down(c.box.lo[1]);
-            node.loz[k] = round_down(c.box.lo[2]);
-            node.hix[k] = round_up(c.box.hi[0]);
-            node.hiy[k] = round_up(c.box.hi[1]);
-            node.hiz[k] = round_up(c.box.hi[2]);
-            node.child[k] = words[k];
-        }
-        out.nodes[id] = node;
-        return id;
     }
 };
 
@@ -289,7 +293,7 @@ BVHBuildResult build_bvh4(const std::vector<PrimBounds>& prims, int max_leaf) {
     out.nodes.reserve(b.nodes.size() / 2 + 4);
     out.order.reserve(prims.size());
     Collapser c(b, out);
-    c.emit(root, 1);
+    c.emit_bfs(root);
     out.max_stack = 3 * out.depth + 4;
     return out;
 }

@@ -120,6 +120,7 @@ struct DevScene {
     int32_t num_infinite;
     int32_t mixed;  // 1 if any non-triangle primitive exists
     int32_t max_stack;
+    int32_t num_nodes;
     float world_center[3];
     float world_radius;
 };

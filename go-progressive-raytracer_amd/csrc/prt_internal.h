@@ -60,6 +60,10 @@ DevCamera make_camera(const prt_camera& c, int& height, int force_height);
 int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t capacity_cap, prt_film** out);
 size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int64_t capacity_cap);
 int film_enqueue(prt_film* f, int32_t spp);  // enqueue spp more samples per pixel, no wait
+int film_plan(prt_film* f, int32_t spp);     // split spp more samples into chunks (not enqueued)
+bool film_plan_done(const prt_film* f);      // every planned chunk has been enqueued
+int film_enqueue_chunk(prt_film* f);         // enqueue the next planned chunk, no wait
+double film_chunk_ms(prt_film* f);           // accumulate the finished chunk's device time
 bool film_idle(prt_film* f);                 // all enqueued work finished
 int film_wait(prt_film* f, float* ms);       // wait for enqueued work; device time of the last enqueue
 int32_t film_spp_done(const prt_film* f);

@@ -19,13 +19,15 @@
 #include <vector>
 
 #include "prt_internal.h"
+#include "bdpt_device.h"
 #include "shading.h"
 #include "traverse.h"
 
 namespace prt {
 
 constexpr int kBlock = 256;
-constexpr int kLds = 16;
+constexpr int kLds = 8;
+constexpr int kNodeCache = 64;  // top BVH nodes kept in LDS by the trace kernels
 
 struct PathBuffers {
     float4* o;      // origin
@@ -110,6 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
                                                   uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
+    __shared__ float4 s_cache[kNodeCache * 8];
     TStack<kLds, kBlock> st;
     st.s_node = s_node;
     st.s_t = s_t;
@@ -117,6 +120,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
     st.lane = threadIdx.x;
     st.gslot = blockIdx.x * kBlock + threadIdx.x;
     st.gstride = gstride;
+    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
+    st.ncache = s_cache;
     const uint32_t cur = (uint32_t)bounce & 1u;
     const uint32_t count = ANY ? pb.counters[2] : pb.counters[cur];
     const uint32_t* q = ANY ? pb.squeue : pb.queue[cur];
@@ -148,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
         r.tmin = ANY ? o.w : 0.001f;
         r.tmax = ANY ? d.w : __int_as_float(0x7f800000);
         THit h;
-        bool hit = trace<ANY, MIXED, false, kLds, kBlock>(sc, r, h, st, nullptr);
+        bool hit = trace<ANY, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
         if (ANY) {
             if (!hit) {
                 float4 c = pb.sc[slot];
@@ -262,6 +267,279 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
     }
 }
 
+
+// ---- bidirectional path tracing (BDPTIntegrator.RayColor, pkg/integrator/bdpt.go:73-102) ----------
+
+// extendPath, bdpt.go:206-291. Appends vertices from index n; returns the new length.
+template <bool MIXED>
+__device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i, int n, V3 o, V3 d, V3 beta,
+                           float pdf_fwd, int max_bounces, bool camera_path, uint32_t key, uint32_t slot0,
+                           TStack<kLds, kBlock>& st) {
+    for (int b = 0; b < max_bounces; b++) {
+        BV prev = VS.get(n - 1, i);
+        TRay r{o.x, o.y, o.z, 0.001f, d.x, d.y, d.z, __int_as_float(0x7f800000)};
+        THit h;
+        if (!trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr)) {
+            if (camera_path) {  // createBackgroundVertex, bdpt.go:585-601
+                V3 e = v3(0, 0, 0);
+                for (int li = 0; li < sc.num_lights; li++)
+                    if (light_is_infinite(sc.lights[li].type)) e = e + light_emit(sc, sc.lights[li], d);
+                BV v;
+                v.p = o + d * 1000.0f;
+                v.n = d * -1.0f;
+                v.wi = d * -1.0f;
+                v.beta = beta;
+                v.Le = e;
+                v.pdf_fwd = pdf_fwd;
+                v.pdf_rev = 0.0f;
+                v.mat = -1;
+                v.light = -1;
+                v.flags = VF_INF | ((e.x != 0.0f || e.y != 0.0f || e.z != 0.0f) ? VF_LIGHT : 0u);
+                VS.put(n++, i, v);
+            }
+            break;
+        }
+        Surf s = make_surf(sc, o, d, h);
+        BV v;
+        v.p = s.p;
+        v.n = s.n;
+        v.wi = d * -1.0f;
+        v.beta = beta;
+        v.mat = s.mat;
+        v.light = -1;
+        const DevMaterial& M = sc.mats[s.mat];
+        v.Le = (M.type == 3 && s.front) ? v3(M.emission) : v3(0, 0, 0);
+        v.flags = (s.front ? VF_FRONT : 0u) | ((v.Le.x != 0.0f || v.Le.y != 0.0f || v.Le.z != 0.0f) ? VF_LIGHT : 0u);
+        v.pdf_fwd = convert_density(sc, prev, v, pdf_fwd);
+        v.pdf_rev = 0.0f;
+        ScatterOut so;
+        if (!scatter(sc, s.mat, d, s, key, (slot0 + (uint32_t)b) * 16u + 1u, so)) {
+            VS.put(n++, i, v);
+            break;
+        }
+        pdf_fwd = so.pdf;
+        bool spec = so.pdf <= 0.0f;
+        float ct = fabsf(dot(so.dir, s.n));
+        beta = spec ? mulv(beta, so.att) : mulv(beta, so.att) * (ct / so.pdf);
+        bool rdelta;
+        float pdf_rev = mat_pdf(sc, s.mat, so.dir, d * -1.0f, s.n, rdelta);
+        if (rdelta) {
+            spec = true;
+            pdf_rev = 0.0f;
+            pdf_fwd = 0.0f;
+        }
+        if (spec) v.flags |= VF_SPECULAR;
+        VS.set_pdf_rev(n - 1, i, convert_density(sc, v, prev, pdf_rev));
+        VS.put(n++, i, v);
+        o = s.p;
+        d = so.dir;
+    }
+    return n;
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_bdpt(DevScene sc, PassParams pp, PathBuffers pb, VertexStore CP,
+                                                 VertexStore LP, long long* splat, uint2* gstack, uint32_t gstride) {
+    __shared__ uint32_t s_node[kLds * kBlock];
+    __shared__ float s_t[kLds * kBlock];
+    __shared__ float4 s_cache[kNodeCache * 8];
+    TStack<kLds, kBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kBlock + threadIdx.x;
+    st.gstride = gstride;
+    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
+    st.ncache = s_cache;
+    const DevCamera& cam = sc.cam;
+    const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
+    const uint32_t stride = gridDim.x * kBlock;
+    // subpath vertices are per thread (the store holds one camera and one light path per thread)
+    const uint32_t vi = blockIdx.x * kBlock + threadIdx.x;
+    for (uint32_t i = vi; i < n; i += stride) {
+        uint32_t s_local = i / (uint32_t)pp.npix;
+        uint32_t pix = (uint32_t)pp.px0 + i % (uint32_t)pp.npix;
+        int px = (int)(pix % (uint32_t)pp.width), py = (int)(pix / (uint32_t)pp.width);
+        uint32_t key = path_key(pp.seed, pix, (uint32_t)pp.sample_base + s_local);
+        V3 o, d;
+        camera_ray(cam, px, py, draw(key, 0), draw(key, 1), draw(key, 2), draw(key, 3), o, d);
+        // camera subpath (generateCameraPath, bdpt.go:106-135)
+        BV cv;
+        cv.p = o;
+        cv.n = d * -1.0f;
+        cv.wi = v3(0, 0, 0);
+        cv.beta = v3(1, 1, 1);
+        cv.Le = v3(0, 0, 0);
+        cv.pdf_fwd = cv.pdf_rev = 0.0f;
+        cv.mat = -1;
+        cv.light = -1;
+        cv.flags = VF_CAMERA;
+        CP.put(0, vi, cv);
+        int nc = bdpt_extend<MIXED>(sc, CP, vi, 1, o, d, v3(1, 1, 1), cam_dir_pdf(cam, o, d), pp.max_depth, true, key, 1,
+                                    st);
+        // light subpath (generateLightPath, bdpt.go:139-203)
+        int nl = 0;
+        if (sc.num_lights > 0) {
+            const uint32_t lp0 = kLightSlot * 16u;
+            float sel;
+            int li = select_light(sc, draw(key, lp0), sel);
+            const DevLight& L = sc.lights[li];
+            ESample es = light_sample_emission(sc, L, draw(key, lp0 + 1), draw(key, lp0 + 2), draw(key, lp0 + 3),
+                                               draw(key, lp0 + 4));
+            float ct = fabsf(dot(es.dir, es.n));
+            BV lv;
+            lv.p = es.p;
+            lv.n = es.n;
+            lv.wi = v3(0, 0, 0);
+            lv.beta = es.emission;
+            lv.Le = es.emission;
+            lv.pdf_fwd = es.area_pdf * sel;
+            lv.pdf_rev = 0.0f;
+            lv.mat = -1;
+            lv.light = li;
+            bool inf = light_is_infinite(L.type);
+            lv.flags = VF_LIGHT | (inf ? VF_INF : 0u);
+            LP.put(0, vi, lv);
+            V3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
+            nl = bdpt_extend<MIXED>(sc, LP, vi, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false, key,
+                                    kLightSlot + 1, st);
+            if (inf) {
+                if (nl > 1) {
+                    BV fb = LP.get(1, vi);
+                    float pf = es.area_pdf * sel;
+                    if (fb.mat >= 0) pf *= fabsf(dot(es.dir, fb.n));
+                    LP.set_pdf_fwd(1, vi, pf);
+                }
+                LP.set_pdf_fwd(0, vi, infinite_density(sc, es.p, es.n, es.dir));
+            }
+        }
+        // strategies (RayColor :83-99 with evaluateBDPTStrategy :294-313)
+        V3 total = v3(0, 0, 0);
+        for (int s = 0; s <= nl; s++) {
+            for (int t = 1; t <= nc; t++) {
+                if (s == 1 && t == 1) continue;
+                V3 Lc = v3(0, 0, 0);
+                bool splatted = false;
+                int sx = 0, sy = 0;
+                BV sampled;
+                const BV* sv = nullptr;
+                if (s == 0) {  // evaluatePathTracingStrategy :316-330
+                    if (t < nc) continue;
+                    BV last = CP.get(t - 1, vi);
+                    Lc = mulv(last.Le, last.beta);
+                } else if (t == 1) {  // evaluateLightTracingStrategy :394-471
+                    if (s <= 1) continue;
+                    BV lv = LP.get(s - 1, vi);
+                    if (lv.flags & VF_SPECULAR) continue;
+                    const uint32_t sp0 = (kSplatSlot + (uint32_t)s) * 16u;
+                    V3 lc = sample_point_in_unit_disk(draw(key, sp0), draw(key, sp0 + 1)) * cam.lens_radius;
+                    V3 lens = v3(cam.center) + v3(cam.u) * lc.x + v3(cam.v) * lc.y;
+                    V3 dir = normalize(lv.p - lens);
+                    float dist = length(lv.p - lens);
+                    float cf = fabsf(dot(v3(cam.forward), dir));
+                    float cpdf = (dist * dist) / (cf * cam.lens_area);
+                    float ctw = dot(dir, v3(cam.forward));
+                    float we = ctw < cam.cos_total_width ? 0.0f : 1.0f / (cam.plane_area * cam.lens_area * ctw * ctw * ctw * ctw);
+                    if (we == 0.0f) continue;
+                    V3 back = dir * -1.0f;
+                    V3 f = vertex_brdf(sc, lv, back, 1);
+                    V3 cbeta = v3(we, we, we) * (1.0f / cpdf);
+                    float cosine = dot(back, lv.n);
+                    if (cosine <= 0.0f) continue;
+                    V3 contrib = mulv(mulv(f, cbeta), lv.beta);
+                    if (on_surface(sc, lv)) contrib = contrib * cosine;
+                    if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
+                    float vd = length(lv.p - lens);
+                    TRay vr{lv.p.x, lv.p.y, lv.p.z, 0.001f, back.x, back.y, back.z, vd - 0.001f};
+                    THit vh;
+                    if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr)) continue;
+                    sampled.p = lens;
+                    sampled.n = back;
+                    sampled.wi = v3(0, 0, 0);
+                    sampled.beta = cbeta;
+                    sampled.Le = v3(0, 0, 0);
+                    sampled.pdf_fwd = cpdf;
+                    sampled.pdf_rev = total_light_pdf(sc, lens, back, normalize(lv.p - lens));
+                    sampled.mat = -1;
+                    sampled.light = -1;
+                    sampled.flags = VF_CAMERA;
+                    sv = &sampled;
+                    if (!cam_map(cam, lens, dir, sx, sy)) {
+                        // the splat lands outside the image (SplatQueue drops it)
+                        continue;
+                    }
+                    Lc = contrib;
+                    splatted = true;
+                } else if (s == 1) {  // evaluateDirectLightingStrategy :333-391
+                    BV cvt = CP.get(t - 1, vi);
+                    if ((cvt.flags & VF_SPECULAR) || cvt.mat < 0) continue;
+                    const uint32_t dp0 = (kDirectSlot + (uint32_t)t) * 16u;
+                    float sel;
+                    int li = select_light(sc, draw(key, dp0), sel);
+                    LSample ls = light_sample(sc, sc.lights[li], cvt.p, cvt.n, draw(key, dp0 + 1), draw(key, dp0 + 2));
+                    ls.pdf *= sel;
+                    if ((ls.emission.x == 0.0f && ls.emission.y == 0.0f && ls.emission.z == 0.0f) || ls.pdf <= 0.0f)
+                        continue;
+                    float ct = fabsf(dot(ls.dir, cvt.n));
+                    if (ct <= 0.0f) continue;
+                    V3 f = brdf(sc, cvt.mat, cvt.wi, ls.dir, surf_of(cvt), 0);
+                    V3 lbeta = ls.emission * (1.0f / ls.pdf);
+                    V3 contrib = mulv(mulv(f, cvt.beta), lbeta) * ct;
+                    if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
+                    TRay vr{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - 0.001f};
+                    THit vh;
+                    if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr)) continue;
+                    sampled.p = ls.p;
+                    sampled.n = ls.n;
+                    sampled.wi = v3(0, 0, 0);
+                    sampled.beta = lbeta;
+                    sampled.Le = ls.emission;
+                    sampled.pdf_fwd = ls.pdf;
+                    sampled.pdf_rev = 0.0f;
+                    sampled.mat = -1;
+                    sampled.light = li;
+                    sampled.flags = VF_LIGHT | (light_is_infinite(sc.lights[li].type) ? VF_INF : 0u);
+                    sv = &sampled;
+                    Lc = contrib;
+                } else {  // evaluateConnectionStrategy :489-561
+                    BV lv = LP.get(s - 1, vi), cvt = CP.get(t - 1, vi);
+                    if ((lv.flags & VF_SPECULAR) || (cvt.flags & VF_SPECULAR)) continue;
+                    V3 dd = lv.p - cvt.p;
+                    float dist = length(dd);
+                    if (dist < 0.001f) continue;
+                    dd = dd * (1.0f / dist);
+                    float cc = dot(dd, cvt.n), cl = dot(dd * -1.0f, lv.n);
+                    if (cc <= 0.0f || cl <= 0.0f) continue;
+                    float G = (cc * cl) / (dist * dist);
+                    V3 fc = vertex_brdf(sc, cvt, dd, 0);
+                    V3 fl = vertex_brdf(sc, lv, dd * -1.0f, 1);
+                    V3 contrib = mulv(mulv(mulv(lv.beta, fl), fc), cvt.beta) * G;
+                    if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
+                    TRay vr{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, dd.x, dd.y, dd.z, dist - 0.001f};
+                    THit vh;
+                    if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr)) continue;
+                    Lc = contrib;
+                }
+                if (Lc.x == 0.0f && Lc.y == 0.0f && Lc.z == 0.0f && !splatted) continue;
+                float w = mis_weight(sc, cam, CP, LP, vi, sv, s, t);
+                if (splatted) {
+                    V3 c = Lc * w;
+                    // fixed point holds |c| < 2^36 per splat (non-finite or larger splats are dropped)
+                    if (!(fabsf(c.x) < 6.8e10f && fabsf(c.y) < 6.8e10f && fabsf(c.z) < 6.8e10f)) continue;
+                    long long* o3 = splat + 3 * ((size_t)sy * pp.width + sx);
+                    atomicAdd((unsigned long long*)(o3 + 0), (unsigned long long)llrintf(c.x * kSplatScale));
+                    atomicAdd((unsigned long long*)(o3 + 1), (unsigned long long)llrintf(c.y * kSplatScale));
+                    atomicAdd((unsigned long long*)(o3 + 2), (unsigned long long)llrintf(c.z * kSplatScale));
+                } else {
+                    total = total + Lc * w;
+                }
+            }
+        }
+        pb.L[i] = make_float4(total.x, total.y, total.z, 0.f);
+    }
+}
+
 // Film (PixelStats.AddSample, stats.go:41-47): per pixel, the chunk's samples are summed in
 // sample order, then added to the running sums with Kahan compensation.
 struct FilmBuffers {
@@ -269,6 +547,7 @@ struct FilmBuffers {
     float4* comp;  // Kahan compensation of sum
     float2* sq;    // luminance^2 sum, compensation
     uint32_t* count;
+    long long* splat;  // BDPT light-tracing splats, 3 per pixel, fixed point (kSplatScale units)
 };
 
 __device__ __forceinline__ void kahan(float& s, float& c, float v) {
@@ -325,8 +604,19 @@ struct prt_film {
     uint2* gstack = nullptr;
     uint32_t gslots = 0;
     int blocks = 0;
+    // BDPT: per-thread camera and light subpath stores of the bdpt_blocks-block grid
+    bool bdpt = false;
+    int bdpt_blocks = 0;
+    VertexStore cpath{}, lpath{};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // chunk plan of the samples being rendered (see film_plan)
+    struct Chunk {
+        int32_t px0, npix, spp, sample_base;
+    };
+    std::vector<Chunk> plan;
+    size_t plan_pos = 0;
+    double plan_ms = 0;
 };
 
 namespace prt {
@@ -363,59 +653,133 @@ size_t film_gstack_entries(const prt_scene* scene) {
     return scene->max_stack > kLds ? (size_t)(scene->max_stack - kLds + 1) : 1;
 }
 
+// Grid of the BDPT kernel: resident blocks only (each thread loops over samples), at most the
+// trace grid so the traversal spill stack covers it.
+int bdpt_blocks(const prt_scene* scene, int trace_blocks) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_bdpt<true>, kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 2;
+    return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu));
+}
+
+// Bytes of one subpath store: (max_depth + 1) vertices of 5 float4 per thread.
+size_t bdpt_store_bytes(int max_depth, int threads) {
+    return (size_t)(max_depth + 1) * 5 * sizeof(float4) * (size_t)threads;
+}
+
 }  // namespace
 
-// Enqueues `spp` samples per pixel for every pixel on the film's stream (no wait).
-int film_enqueue(prt_film* f, int32_t spp) {
-    const prt_scene* s = f->scene;
-    PRT_HIP(hipSetDevice(s->device));
-    const DevScene& sc = f->sc;
-    hipStream_t st = f->stream;
-    PRT_HIP(hipEventRecord(f->ev0, st));
+// Splits `spp` more samples per pixel into chunks of at most `capacity` paths. The split depends
+// only on the film's size and sample count, so a render is reproducible however its chunks are
+// scheduled.
+int film_plan(prt_film* f, int32_t spp) {
+    f->plan.clear();
+    f->plan_pos = 0;
+    f->plan_ms = 0;
     int64_t done = 0;
     while (done < spp) {
-        // chunking depends only on the film's own size, so renders are reproducible
         int64_t per = f->capacity / f->npix;
         int32_t chunk_spp = (int32_t)std::max<int64_t>(1, std::min<int64_t>(per, spp - done));
         int64_t pix_step = per >= 1 ? f->npix : f->capacity;
         for (int64_t p0 = 0; p0 < f->npix; p0 += pix_step) {
-            PassParams pp;
-            pp.width = f->width;
-            pp.height = f->height;
-            pp.px0 = (int32_t)p0;
-            pp.npix = (int32_t)std::min<int64_t>(pix_step, f->npix - p0);
-            pp.spp = chunk_spp;
-            pp.sample_base = f->spp_done + (int32_t)done;
-            pp.max_depth = f->params.max_depth;
-            pp.rr_min = f->params.rr_min_bounces;
-            pp.seed = f->params.seed;
-            int64_t n = (int64_t)pp.npix * pp.spp;
-            PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 4 * sizeof(uint32_t), st));
-            int cam_blocks = (int)((n + kBlock - 1) / kBlock);
-            hipLaunchKernelGGL(k_camera, dim3(cam_blocks), dim3(kBlock), 0, st, sc, pp, f->pb);
-            for (int b = 0; b < pp.max_depth; b++) {
-                if (sc.mixed)
-                    hipLaunchKernelGGL((k_trace<false, true>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
-                                       f->gstack, f->gslots);
-                else
-                    hipLaunchKernelGGL((k_trace<false, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
-                                       f->gstack, f->gslots);
-                hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
-                if (sc.mixed)
-                    hipLaunchKernelGGL((k_trace<true, true>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
-                                       f->gstack, f->gslots);
-                else
-                    hipLaunchKernelGGL((k_trace<true, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b,
-                                       f->gstack, f->gslots);
-            }
-            int film_blocks_n = (pp.npix + kBlock - 1) / kBlock;
-            hipLaunchKernelGGL(k_film, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb);
-            PRT_HIP(hipGetLastError());
+            prt_film::Chunk c;
+            c.px0 = (int32_t)p0;
+            c.npix = (int32_t)std::min<int64_t>(pix_step, f->npix - p0);
+            c.spp = chunk_spp;
+            c.sample_base = f->spp_done + (int32_t)done;
+            f->plan.push_back(c);
         }
         done += chunk_spp;
     }
-    PRT_HIP(hipEventRecord(f->ev1, st));
     f->spp_done += spp;
+    return PRT_OK;
+}
+
+bool film_plan_done(const prt_film* f) { return f->plan_pos >= f->plan.size(); }
+
+// Enqueues the next planned chunk on the film's stream (no wait); ev0/ev1 bracket it.
+int film_enqueue_chunk(prt_film* f) {
+    if (film_plan_done(f)) return PRT_OK;
+    const prt_scene* s = f->scene;
+    PRT_HIP(hipSetDevice(s->device));
+    const DevScene& sc = f->sc;
+    hipStream_t st = f->stream;
+    const prt_film::Chunk& c = f->plan[f->plan_pos++];
+    PassParams pp;
+    pp.width = f->width;
+    pp.height = f->height;
+    pp.px0 = c.px0;
+    pp.npix = c.npix;
+    pp.spp = c.spp;
+    pp.sample_base = c.sample_base;
+    pp.max_depth = f->params.max_depth;
+    pp.rr_min = f->params.rr_min_bounces;
+    pp.seed = f->params.seed;
+    int64_t n = (int64_t)pp.npix * pp.spp;
+    PRT_HIP(hipEventRecord(f->ev0, st));
+    if (f->bdpt) {
+        int grid = (int)std::min<int64_t>(f->bdpt_blocks, (n + kBlock - 1) / kBlock);
+        if (sc.mixed)
+            hipLaunchKernelGGL((k_bdpt<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath, f->lpath,
+                               f->fb.splat, f->gstack, f->gslots);
+        else
+            hipLaunchKernelGGL((k_bdpt<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath, f->lpath,
+                               f->fb.splat, f->gstack, f->gslots);
+        hipLaunchKernelGGL(k_film, dim3((pp.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pp, f->pb, f->fb);
+        PRT_HIP(hipGetLastError());
+        PRT_HIP(hipEventRecord(f->ev1, st));
+        return PRT_OK;
+    }
+    PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 4 * sizeof(uint32_t), st));
+    int cam_blocks = (int)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_camera, dim3(cam_blocks), dim3(kBlock), 0, st, sc, pp, f->pb);
+    for (int b = 0; b < pp.max_depth; b++) {
+        if (sc.mixed)
+            hipLaunchKernelGGL((k_trace<false, true>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b, f->gstack,
+                               f->gslots);
+        else
+            hipLaunchKernelGGL((k_trace<false, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b, f->gstack,
+                               f->gslots);
+        hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
+        if (sc.mixed)
+            hipLaunchKernelGGL((k_trace<true, true>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b, f->gstack,
+                               f->gslots);
+        else
+            hipLaunchKernelGGL((k_trace<true, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b, f->gstack,
+                               f->gslots);
+    }
+    int film_blocks_n = (pp.npix + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_film, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb);
+    PRT_HIP(hipGetLastError());
+    PRT_HIP(hipEventRecord(f->ev1, st));
+    return PRT_OK;
+}
+
+// Device time of the last enqueued chunk (call when idle), accumulated over the plan.
+double film_chunk_ms(prt_film* f) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, f->ev0, f->ev1) == hipSuccess) f->plan_ms += t;
+    return f->plan_ms;
+}
+
+// Enqueues `spp` samples per pixel for every pixel on the film's stream (no wait).
+int film_enqueue(prt_film* f, int32_t spp) {
+    int rc = film_plan(f, spp);
+    if (rc != PRT_OK) return rc;
+    hipEvent_t first = nullptr;
+    PRT_HIP(hipEventCreate(&first));
+    (void)hipEventRecord(first, f->stream);
+    while (!film_plan_done(f)) {
+        rc = film_enqueue_chunk(f);
+        if (rc != PRT_OK) {
+            (void)hipEventDestroy(first);
+            return rc;
+        }
+    }
+    // ev0 brackets the whole request: replace the last chunk's start event by the first one
+    std::swap(first, f->ev0);
+    (void)hipEventDestroy(first);
     return PRT_OK;
 }
 
@@ -437,8 +801,12 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     int32_t h = p->height > 0 ? p->height : scene->image_height;
     int64_t npix = (int64_t)w * h;
     int64_t C = film_capacity(npix, cap);
-    size_t g = film_gstack_entries(scene) * (size_t)film_blocks(scene) * kBlock * sizeof(uint2);
-    return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + 3 * sizeof(uint32_t) * (size_t)C;
+    int blocks = film_blocks(scene);
+    size_t g = film_gstack_entries(scene) * (size_t)blocks * kBlock * sizeof(uint2);
+    size_t b = 0;
+    if (p->integrator == PRT_INTEGRATOR_BDPT)
+        b = 2 * bdpt_store_bytes(p->max_depth, bdpt_blocks(scene, blocks) * kBlock) + 3 * sizeof(long long) * npix;
+    return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + b + 3 * sizeof(uint32_t) * (size_t)C;
 }
 
 int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t cap, prt_film** out) {
@@ -446,8 +814,9 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     *out = nullptr;
     if (p->integrator != PRT_INTEGRATOR_PATH && p->integrator != PRT_INTEGRATOR_BDPT)
         return fail(PRT_ERR_INVALID_ARGUMENT, "unknown integrator");
-    if (p->integrator == PRT_INTEGRATOR_BDPT) return fail(PRT_ERR_UNSUPPORTED, "BDPT is not built yet");
     if (p->max_depth < 0 || p->max_depth > 1024) return fail(PRT_ERR_INVALID_ARGUMENT, "max_depth out of range");
+    if (p->integrator == PRT_INTEGRATOR_BDPT && p->max_depth > kBdptMaxDepth)
+        return fail(PRT_ERR_UNSUPPORTED, "BDPT max_depth above 64");
     if (p->max_passes < 1 || p->max_samples_per_pixel < 1 || p->initial_samples < 0)
         return fail(PRT_ERR_INVALID_ARGUMENT, "bad pass schedule");
     int32_t w = p->width > 0 ? p->width : scene->image_width;
@@ -505,6 +874,20 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     f->gslots = (uint32_t)f->blocks * kBlock;
     if ((rc = film_alloc(f, film_gstack_entries(scene) * f->gslots * sizeof(uint2), &ptr)) != PRT_OK) return bail(rc);
     f->gstack = (uint2*)ptr;
+    if (p->integrator == PRT_INTEGRATOR_BDPT) {
+        f->bdpt = true;
+        f->bdpt_blocks = bdpt_blocks(scene, f->blocks);
+        const int threads = f->bdpt_blocks * kBlock;
+        const size_t vb = bdpt_store_bytes(p->max_depth, threads);
+        if ((rc = film_alloc(f, vb, &ptr)) != PRT_OK) return bail(rc);
+        f->cpath.data = (float4*)ptr;
+        f->cpath.stride = (uint32_t)threads;
+        if ((rc = film_alloc(f, vb, &ptr)) != PRT_OK) return bail(rc);
+        f->lpath.data = (float4*)ptr;
+        f->lpath.stride = (uint32_t)threads;
+        if ((rc = film_alloc(f, 3 * sizeof(long long) * (size_t)f->npix, &ptr)) != PRT_OK) return bail(rc);
+        f->fb.splat = (long long*)ptr;
+    }
     hipError_t e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
     e = hipEventCreate(&f->ev0);
@@ -540,6 +923,7 @@ int prt_film_reset(prt_film* f) {
     PRT_HIP(hipMemsetAsync(f->fb.comp, 0, sizeof(float4) * f->npix, f->stream));
     PRT_HIP(hipMemsetAsync(f->fb.sq, 0, sizeof(float2) * f->npix, f->stream));
     PRT_HIP(hipMemsetAsync(f->fb.count, 0, sizeof(uint32_t) * f->npix, f->stream));
+    if (f->fb.splat) PRT_HIP(hipMemsetAsync(f->fb.splat, 0, 3 * sizeof(long long) * f->npix, f->stream));
     PRT_HIP(hipStreamSynchronize(f->stream));
     f->spp_done = 0;
     return PRT_OK;
@@ -602,22 +986,44 @@ int prt_film_read(const prt_film* f, float* mean_rgb, float* lum_sum, float* lum
     std::vector<float4> sum(f->npix);
     std::vector<float2> sq(f->npix);
     std::vector<uint32_t> cnt(f->npix);
+    std::vector<long long> spl(f->fb.splat ? 3 * f->npix : 0);
     // ordered after the work already enqueued on the film's stream
     PRT_HIP(hipMemcpyAsync(sum.data(), f->fb.sum, sizeof(float4) * f->npix, hipMemcpyDeviceToHost, f->stream));
     PRT_HIP(hipMemcpyAsync(sq.data(), f->fb.sq, sizeof(float2) * f->npix, hipMemcpyDeviceToHost, f->stream));
     PRT_HIP(hipMemcpyAsync(cnt.data(), f->fb.count, sizeof(uint32_t) * f->npix, hipMemcpyDeviceToHost, f->stream));
+    if (f->fb.splat)
+        PRT_HIP(hipMemcpyAsync(spl.data(), f->fb.splat, sizeof(long long) * 3 * f->npix, hipMemcpyDeviceToHost,
+                               f->stream));
     PRT_HIP(hipStreamSynchronize(f->stream));
     for (int64_t i = 0; i < f->npix; i++) {
         if (mean_rgb) {
-            float inv = cnt[i] ? 1.0f / (float)cnt[i] : 0.0f;  // GetColor: zero when no samples
-            mean_rgb[3 * i + 0] = cnt[i] ? sum[i].x * inv : 0.f;
-            mean_rgb[3 * i + 1] = cnt[i] ? sum[i].y * inv : 0.f;
-            mean_rgb[3 * i + 2] = cnt[i] ? sum[i].z * inv : 0.f;
+            // GetColor: (samples + splats) / SampleCount, zero when no samples (stats.go:36-55)
+            float inv = cnt[i] ? 1.0f / (float)cnt[i] : 0.0f;
+            for (int c = 0; c < 3; c++) {
+                float v = c == 0 ? sum[i].x : (c == 1 ? sum[i].y : sum[i].z);
+                if (f->fb.splat) v += (float)((double)spl[3 * i + c] * (1.0 / (double)kSplatScale));
+                mean_rgb[3 * i + c] = cnt[i] ? v * inv : 0.f;
+            }
         }
         if (lum_sum) lum_sum[i] = sum[i].w;
         if (lum_sq_sum) lum_sq_sum[i] = sq[i].x;
         if (counts) counts[i] = cnt[i];
     }
+    return PRT_OK;
+}
+
+int prt_film_read_splats(const prt_film* f, float* splat_rgb) {
+    if (!f || !splat_rgb) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!f->fb.splat) {
+        std::memset(splat_rgb, 0, sizeof(float) * 3 * (size_t)f->npix);
+        return PRT_OK;
+    }
+    PRT_HIP(hipSetDevice(f->scene->device));
+    std::vector<long long> spl(3 * f->npix);
+    PRT_HIP(hipMemcpyAsync(spl.data(), f->fb.splat, sizeof(long long) * 3 * f->npix, hipMemcpyDeviceToHost,
+                           f->stream));
+    PRT_HIP(hipStreamSynchronize(f->stream));
+    for (int64_t k = 0; k < 3 * f->npix; k++) splat_rgb[k] = (float)((double)spl[k] * (1.0 / (double)kSplatScale));
     return PRT_OK;
 }
 

@@ -235,20 +235,25 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect(DevScene sc, const pr
 // Persistent closest/any-hit kernel with lane refill: a lane whose ray is finished takes a new
 // ray from the global work counter as soon as REFILL lanes of its wave are idle (REFILL = 64
 // degenerates to fetching whole 64-ray batches), so long rays do not idle the rest of the wave.
-template <bool ANY, bool MIXED, int LDS, int REFILL>
-__global__ __launch_bounds__(kTraceBlock) void k_intersect_refill(DevScene sc, const prt_ray* __restrict__ rays,
-                                                                  int64_t n, prt_hit* __restrict__ hits,
-                                                                  uint8_t* __restrict__ occluded, uint2* gstack,
-                                                                  uint32_t gstride, unsigned long long* work) {
-    __shared__ uint32_t s_node[LDS * kTraceBlock];
-    __shared__ float s_t[LDS * kTraceBlock];
-    TStack<LDS, kTraceBlock> st;
+template <bool ANY, bool MIXED, int LDS, int REFILL, int NCACHE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_intersect_refill(DevScene sc, const prt_ray* __restrict__ rays,
+                                                            int64_t n, prt_hit* __restrict__ hits,
+                                                            uint8_t* __restrict__ occluded, uint2* gstack,
+                                                            uint32_t gstride, unsigned long long* work) {
+    __shared__ uint32_t s_node[LDS * BLOCK];
+    __shared__ float s_t[LDS * BLOCK];
+    __shared__ float4 s_cache[NCACHE > 0 ? NCACHE * 8 : 1];
+    TStack<LDS, BLOCK> st;
     st.s_node = s_node;
     st.s_t = s_t;
     st.g = gstack;
     st.lane = threadIdx.x;
-    st.gslot = blockIdx.x * kTraceBlock + threadIdx.x;
+    st.gslot = blockIdx.x * BLOCK + threadIdx.x;
     st.gstride = gstride;
+    if (NCACHE > 0) {
+        load_node_cache<NCACHE, BLOCK>(sc, s_cache);
+        st.ncache = s_cache;
+    }
     const uint32_t lane = threadIdx.x & 63u;
     TState s;
     bool active = false;
@@ -279,7 +284,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_refill(DevScene sc, c
             if (exhausted) break;
             continue;
         }
-        if (active && tstate_step<ANY, MIXED, false, LDS, kTraceBlock>(sc, s, st, nullptr)) {
+        if (active && tstate_step<ANY, MIXED, false, LDS, BLOCK, NCACHE>(sc, s, st, nullptr)) {
             active = false;
             bool hit = s.best != 0xFFFFFFFFu;
             if (ANY) {
@@ -294,39 +299,43 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_refill(DevScene sc, c
     }
 }
 
-// Kernel variants the per-scene autotuner chooses from (identical per-ray traversal).
-struct TraceVariant {
-    int lds;
-    int refill;
-};
-constexpr TraceVariant kVariants[] = {{16, 64}, {8, 64}, {8, 16}, {8, 32}, {8, 8}, {16, 16}};
-constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
-constexpr int kMinLds = 8;
-constexpr int kDefaultVariant = 2;
+// Kernel variants the per-scene autotuner chooses from (identical per-ray traversal):
+// LDS stack entries per lane, refill threshold (64 = whole-wave batches), LDS node-cache size, block.
+#define PRT_TRACE_VARIANTS(X)   \
+    X(0, 16, 64, 0, 256)        \
+    X(1, 8, 64, 0, 256)         \
+    X(2, 8, 64, 64, 512)        \
+    X(3, 8, 64, 128, 512)       \
+    X(4, 6, 64, 64, 256)        \
+    X(5, 8, 32, 64, 512)
+constexpr int kNumVariants = 6;
+constexpr int kMinLds = 6;
+constexpr int kDefaultVariant = 1;
+constexpr int kVariantBlock[kNumVariants] = {256, 256, 512, 512, 256, 512};
 
 template <bool ANY, bool MIXED>
 const void* variant_fn(int v) {
     switch (v) {
-        case 0: return (const void*)k_intersect_refill<ANY, MIXED, 16, 64>;
-        case 1: return (const void*)k_intersect_refill<ANY, MIXED, 8, 64>;
-        case 2: return (const void*)k_intersect_refill<ANY, MIXED, 8, 16>;
-        case 3: return (const void*)k_intersect_refill<ANY, MIXED, 8, 32>;
-        case 4: return (const void*)k_intersect_refill<ANY, MIXED, 8, 8>;
-        default: return (const void*)k_intersect_refill<ANY, MIXED, 16, 16>;
+#define X(id, lds, refill, nc, blk) \
+    case id:                        \
+        return (const void*)k_intersect_refill<ANY, MIXED, lds, refill, nc, blk>;
+        PRT_TRACE_VARIANTS(X)
+#undef X
     }
+    return nullptr;
 }
 
 template <bool ANY, bool MIXED>
 void launch_variant(int v, int blocks, hipStream_t st, const DevScene& sc, const prt_ray* rays, int64_t n,
                     prt_hit* hits, uint8_t* occ, uint2* gs, uint32_t gstride, unsigned long long* work) {
-    dim3 g(blocks), b(kTraceBlock);
     switch (v) {
-        case 0: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 16, 64>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
-        case 1: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 8, 64>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
-        case 2: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 8, 16>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
-        case 3: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 8, 32>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
-        case 4: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 8, 8>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
-        default: hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, 16, 16>), g, b, 0, st, sc, rays, n, hits, occ, gs, gstride, work); break;
+#define X(id, lds, refill, nc, blk)                                                                            \
+    case id:                                                                                                \
+        hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, lds, refill, nc, blk>), dim3(blocks), dim3(blk), 0, st, \
+                           sc, rays, n, hits, occ, gs, gstride, work);                                      \
+        break;
+        PRT_TRACE_VARIANTS(X)
+#undef X
     }
 }
 
@@ -663,6 +672,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     s->dev.quads = (const QuadExt*)p;
     s->dev.mixed = (nsph + nq) > 0 ? 1 : 0;
     s->dev.max_stack = bvh.max_stack;
+    s->dev.num_nodes = (int32_t)bvh.nodes.size();
 
     // materials (float32 copies)
     std::vector<DevMaterial> mats(std::max(nm, 1));
@@ -804,13 +814,13 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     int max_blocks = 1;
     for (int v = 0; v < kNumVariants; v++) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_fn<false, true>(v), kTraceBlock, 0) !=
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_fn<false, true>(v), kVariantBlock[v], 0) !=
                 hipSuccess ||
             per_cu <= 0)
-            per_cu = 4;
-        per_cu = std::min(per_cu, 12);
+            per_cu = 2;
+        per_cu = std::min(per_cu, 12 * 256 / kVariantBlock[v]);
         s->variant_blocks[v] = s->num_cus * per_cu;
-        max_blocks = std::max(max_blocks, s->variant_blocks[v]);
+        max_blocks = std::max(max_blocks, s->variant_blocks[v] * kVariantBlock[v] / kTraceBlock);
     }
     s->trace_variant[0] = s->trace_variant[1] = -1;
     s->forced_variant = -1;

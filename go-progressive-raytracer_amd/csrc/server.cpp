@@ -39,12 +39,14 @@ struct Req {
     Clock::time_point t_submit;
     bool cancel = false;
     bool release = false;
+    int in_flight = 0;       // chunks of this request on the device (at most 2)
     int readers = 0;
 };
 
-// Film slots of serving requests hold at most this many paths, so one request's kernels stay short
-// and passes of concurrent requests interleave on the device.
-constexpr int64_t kServeCapacity = (int64_t)1 << 21;
+// Film slots of serving requests hold at most this many paths: a pass is enqueued chunk by chunk
+// (one chunk in flight per request), so the scheduler can give first passes the device between
+// chunks of longer renders.
+constexpr int64_t kServeCapacity = (int64_t)1 << 19;
 
 }  // namespace
 
@@ -86,7 +88,8 @@ struct prt_server {
             used -= (int64_t)r->bytes;
         }
     }
-    // enqueue the next pass of an active request; false on error (event emitted)
+    // plan the next pass of an active request; its chunks are enqueued by the scheduler.
+    // false on error (event emitted)
     bool next_pass(Req* r) {
         int32_t target = 0;
         int rc = prt_samples_for_pass(&r->params, r->pass + 1, &target);
@@ -94,7 +97,7 @@ struct prt_server {
             int32_t need = target - prt::film_spp_done(r->film);
             r->pass++;
             r->target = target;
-            if (need > 0) rc = prt::film_enqueue(r->film, need);
+            rc = prt::film_plan(r->film, need > 0 ? need : 0);
         }
         if (rc != PRT_OK) {
             r->state = St::Failed;
@@ -137,36 +140,61 @@ struct prt_server {
                 if (next_pass(r)) emit(r, PRT_EVENT_ADMITTED, r->pass, r->target, 0, 0);
                 progressed = true;
             }
-            // ---- pass completion and sequencing ----
+            // ---- chunk and pass completion ----
             for (size_t k = 0; k < active.size();) {
                 Req* r = active[k];
                 bool remove = false;
                 if (r->state != St::Running) {
                     remove = true;
-                } else if (r->pass > r->completed_pass && prt::film_idle(r->film)) {
-                    float ms = 0;
-                    (void)prt::film_wait(r->film, &ms);
-                    r->completed_pass = r->pass;
-                    emit(r, PRT_EVENT_PASS_COMPLETE, r->pass, r->target, 0, ms);
-                    progressed = true;
-                    if (r->cancel) {
-                        r->state = St::Cancelled;
-                        emit(r, PRT_EVENT_CANCELLED, r->pass, r->target, 0, 0);
-                        remove = true;
-                    } else if (r->pass >= r->params.max_passes ||
-                               prt::film_spp_done(r->film) >= r->params.max_samples_per_pixel) {
-                        r->state = St::Done;
-                        completed++;
-                        emit(r, PRT_EVENT_DONE, r->pass, r->target, 0, 0);
-                        remove = true;
-                    } else if (!next_pass(r)) {
-                        remove = true;
+                } else if (r->in_flight == 0 || prt::film_idle(r->film)) {
+                    double ms = r->in_flight ? prt::film_chunk_ms(r->film) : 0.0;
+                    if (r->in_flight) progressed = true;
+                    r->in_flight = 0;
+                    if (prt::film_plan_done(r->film) && r->pass > r->completed_pass) {
+                        r->completed_pass = r->pass;
+                        emit(r, PRT_EVENT_PASS_COMPLETE, r->pass, r->target, 0, ms);
+                        progressed = true;
+                        if (r->cancel) {
+                            r->state = St::Cancelled;
+                            emit(r, PRT_EVENT_CANCELLED, r->pass, r->target, 0, 0);
+                            remove = true;
+                        } else if (r->pass >= r->params.max_passes ||
+                                   prt::film_spp_done(r->film) >= r->params.max_samples_per_pixel) {
+                            r->state = St::Done;
+                            completed++;
+                            emit(r, PRT_EVENT_DONE, r->pass, r->target, 0, 0);
+                            remove = true;
+                        } else if (!next_pass(r)) {
+                            remove = true;
+                        }
                     }
                 }
                 if (remove) {
                     active.erase(active.begin() + k);
                 } else {
                     k++;
+                }
+            }
+            // ---- chunk dispatch ----
+            // Requests that still owe their first pass get the device first: while any is owed, other
+            // requests receive no new chunks (theirs in flight drain). Otherwise every request keeps
+            // two chunks queued on its stream so the device never waits for the scheduler.
+            bool first_owed = false;
+            for (Req* r : active)
+                if (r->state == St::Running && r->completed_pass == 0) first_owed = true;
+            for (Req* r : active) {
+                if (r->state != St::Running) continue;
+                bool first = r->completed_pass == 0;
+                if (first_owed && !first) continue;
+                int depth = first ? 4 : 2;
+                while (r->in_flight < depth && !prt::film_plan_done(r->film)) {
+                    if (prt::film_enqueue_chunk(r->film) != PRT_OK) {
+                        r->state = St::Failed;
+                        emit(r, PRT_EVENT_ERROR, r->pass, r->target, PRT_ERR_DEVICE, 0);
+                        break;
+                    }
+                    r->in_flight++;
+                    progressed = true;
                 }
             }
             // ---- deferred releases ----
@@ -212,7 +240,8 @@ int prt_server_submit(prt_server* s, const prt_scene* scene, const prt_render_pa
     int32_t t = 0;
     int rc = prt_samples_for_pass(p, 1, &t);
     if (rc != PRT_OK) return rc;
-    if (p->integrator != PRT_INTEGRATOR_PATH) return fail(PRT_ERR_UNSUPPORTED, "only path tracing is served");
+    if (p->integrator != PRT_INTEGRATOR_PATH && p->integrator != PRT_INTEGRATOR_BDPT)
+        return fail(PRT_ERR_INVALID_ARGUMENT, "unknown integrator");
     Req* r = new Req();
     r->scene = scene;
     r->params = *p;

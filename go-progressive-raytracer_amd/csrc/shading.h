@@ -280,6 +280,7 @@ __device__ __forceinline__ float mat_pdf(const DevScene& sc, int m, V3 in, V3 ou
 struct LSample {
     V3 dir, emission;
     float dist, pdf;
+    V3 p, n;  // LightSample.Point / Normal (used by BDPT's sampled light vertex)
 };
 
 // Emit of the light's material with no hit record (emissive.go:30-37); zero if not emissive
@@ -383,6 +384,8 @@ __device__ __forceinline__ LSample sphere_sample_uniform(const DevScene& sc, con
     ls.dir = normalize(dir);
     ls.pdf = 1.0f / (4.0f * kPiF * L.radius * L.radius);
     ls.emission = emissive(sc, L.mat);
+    ls.p = pt;
+    ls.n = ld;
     return ls;
 }
 
@@ -398,6 +401,8 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLig
         float ct = fabsf(dot(qn, dir * -1.0f));
         ls.dir = dir;
         ls.dist = dist;
+        ls.p = spt;
+        ls.n = qn;
         if (ct < 1e-8f) {
             ls.emission = {0, 0, 0};
             ls.pdf = 0.0f;
@@ -429,6 +434,9 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLig
         ls.dist = t;
         ls.pdf = 1.0f / (2.0f * kPiF * (1.0f - ctm));
         ls.emission = emissive(sc, L.mat);
+        ls.p = p + dir * t;
+        V3 outward = (ls.p - c) * (1.0f / L.radius);
+        ls.n = dot(dir, outward) < 0.0f ? outward : outward * -1.0f;  // SetFaceNormal of the re-hit
         return ls;
     }
     if (L.type == PRT_LIGHT_ENVMAP) {
@@ -438,6 +446,8 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLig
         ls.dist = __int_as_float(0x7f800000);
         ls.pdf = pdf;
         ls.emission = env_eval(sc.env, dir);
+        ls.p = p + dir * 1e10f;
+        ls.n = dir * -1.0f;
         return ls;
     }
     // uniform / gradient infinite: cosine hemisphere about the shading normal
@@ -445,6 +455,8 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLig
     ls.dir = dir;
     ls.dist = __int_as_float(0x7f800000);
     ls.pdf = dot(dir, n) / kPiF;
+    ls.p = p + dir * 1e10f;
+    ls.n = dir * -1.0f;
     if (L.type == PRT_LIGHT_UNIFORM_INFINITE) {
         ls.emission = v3(L.emission);
     } else {

@@ -147,6 +147,7 @@ struct TStack {
     uint32_t* s_node;  // [LDS_DEPTH][BLOCK]
     float* s_t;        // [LDS_DEPTH][BLOCK]
     uint2* g;          // [(max_stack - LDS_DEPTH)][gstride]
+    const float4* ncache = nullptr;  // top BVH nodes copied to LDS (breadth-first prefix)
     uint32_t lane;     // threadIdx.x
     uint32_t gslot;    // global thread slot
     uint32_t gstride;
@@ -205,15 +206,34 @@ __device__ __forceinline__ void tstate_result(const TState& s, THit& hit) {
 }
 
 // One traversal step. Returns true when the ray is finished (stack empty, or any hit found).
-template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK>
+template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK, int NCACHE = 0>
 __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStack<LDS_DEPTH, BLOCK>& st,
                                             TCounts* cnt) {
     const TRay& r = s.r;
     if (!child_is_leaf(s.cur)) {
-        // ---- inner node: test the 4 child boxes ----
-        const float4* np = reinterpret_cast<const float4*>(sc.nodes + s.cur);
-        float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-        uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+        // ---- inner node: test the 4 child boxes (top levels from LDS when cached) ----
+        float4 lx, hx, ly, hy, lz, hz;
+        uint4 ch;
+        if (NCACHE > 0 && s.cur < (uint32_t)NCACHE) {
+            const float4* np = st.ncache + 8 * s.cur;
+            lx = np[0];
+            hx = np[1];
+            ly = np[2];
+            hy = np[3];
+            lz = np[4];
+            hz = np[5];
+            float4 c4 = np[6];
+            ch = make_uint4(__float_as_uint(c4.x), __float_as_uint(c4.y), __float_as_uint(c4.z), __float_as_uint(c4.w));
+        } else {
+            const float4* np = reinterpret_cast<const float4*>(sc.nodes + s.cur);
+            lx = np[0];
+            hx = np[1];
+            ly = np[2];
+            hy = np[3];
+            lz = np[4];
+            hz = np[5];
+            ch = reinterpret_cast<const uint4*>(np)[6];
+        }
         if (COUNT) {
             cnt->nodes++;
             cnt->boxes += 4;
@@ -303,13 +323,23 @@ __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStac
     }
 }
 
+// Copies the first NCACHE nodes (the top levels in breadth-first order) into LDS; all threads
+// of the block must call it before traversing.
+template <int NCACHE, int BLOCK>
+__device__ __forceinline__ void load_node_cache(const DevScene& sc, float4* cache) {
+    const int n = (NCACHE < sc.num_nodes ? NCACHE : sc.num_nodes) * 8;
+    const float4* src = reinterpret_cast<const float4*>(sc.nodes);
+    for (int k = threadIdx.x; k < n; k += BLOCK) cache[k] = src[k];
+    __syncthreads();
+}
+
 // Closest hit (ANY = false) or any hit (ANY = true) of one ray. Returns true on a hit.
-template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK>
+template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK, int NCACHE = 0>
 __device__ bool trace(const DevScene& sc, const TRay& r_in, THit& hit, TStack<LDS_DEPTH, BLOCK>& st,
                       TCounts* cnt) {
     TState s;
     tstate_init(s, r_in);
-    while (!tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK>(sc, s, st, cnt)) {
+    while (!tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK, NCACHE>(sc, s, st, cnt)) {
     }
     tstate_result(s, hit);
     return s.best != 0xFFFFFFFFu;

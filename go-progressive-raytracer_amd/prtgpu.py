@@ -109,7 +109,7 @@ EXPORTS = [
     "prt_last_error", "prt_version", "prt_device_count", "prt_scene_create", "prt_scene_info_get",
     "prt_scene_destroy", "prt_intersect_closest", "prt_intersect_any", "prt_traversal_counts",
     "prt_samples_for_pass", "prt_film_create", "prt_render_pass", "prt_render_samples", "prt_film_read",
-    "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy",
+    "prt_film_read_splats", "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy",
     "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky",
     "prt_server_create", "prt_server_submit", "prt_server_poll", "prt_server_cancel", "prt_server_read",
     "prt_server_read_rgba8", "prt_server_release", "prt_server_stats_get", "prt_server_destroy",
@@ -155,6 +155,7 @@ def lib():
         "prt_render_pass": (C.c_int, [vp, C.c_int32, C.POINTER(PassStats)]),
         "prt_render_samples": (C.c_int, [vp, C.c_int32, C.POINTER(PassStats)]),
         "prt_film_read": (C.c_int, [vp, vp, vp, vp, vp]),
+        "prt_film_read_splats": (C.c_int, [vp, vp]),
         "prt_film_read_rgba8": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]),
         "prt_film_reset": (C.c_int, [vp]),
         "prt_film_destroy": (C.c_int, [vp]),
@@ -475,6 +476,12 @@ class ProgressiveRaytracer:
         check(lib().prt_film_read(self.handle, _ptr(mean), _ptr(ls), _ptr(lq), _ptr(cnt)))
         assert mean.size == 3 * n
         return mean, ls, lq, cnt
+
+    def splats(self):
+        """BDPT light-tracing splats accumulated so far (H,W,3); included in read()'s mean."""
+        out = np.empty((self.height, self.width, 3), np.float32)
+        check(lib().prt_film_read_splats(self.handle, _ptr(out)))
+        return out
 
     def image(self, x0=0, y0=0, x1=None, y1=None):
         x1 = self.width if x1 is None else x1

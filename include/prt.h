@@ -245,6 +245,10 @@ int prt_render_samples(prt_film* film, int32_t spp, prt_pass_stats* stats);
  * first/second moment sums and sample counts (PixelStats, stats.go:20-55). */
 int prt_film_read(const prt_film* film, float* mean_rgb, float* lum_sum, float* lum_sq_sum,
                   uint32_t* counts);
+/* BDPT light-tracing (t = 1) splats accumulated so far, width*height*3 floats (the SplatQueue
+ * contents applied by processSplats, progressive.go:370-390; zero for path tracing). The mean
+ * colour of prt_film_read already includes them. */
+int prt_film_read_splats(const prt_film* film, float* splat_rgb);
 /* Gamma-2 RGBA8 image of pixel rectangle [x0,x1)x[y0,y1) (vec3ToColor, progressive.go:482-495). */
 int prt_film_read_rgba8(const prt_film* film, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
                         uint8_t* rgba);

@@ -12,11 +12,14 @@ import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _params(w, h, cfg, spp, passes, seed):
-    return P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp, max_passes=passes, seed=seed)
+def _params(w, h, cfg, spp, passes, seed, integrator=P.INTEGRATOR_PATH):
+    return P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp, max_passes=passes, seed=seed,
+                             integrator=integrator)
 
 
-def _drain(server, ids, timeout_s=120):
+def _drain(server, ids, timeout_s=60, release=False):
+    """Collects events until every request is done; release=True frees each finished request's
+    film slot (a finished request holds its slot and its HBM budget until released)."""
     import time
 
     done, events = set(), {i: [] for i in ids}
@@ -27,6 +30,8 @@ def _drain(server, ids, timeout_s=120):
                 events[e.request_id].append((e.type, e.pass_number, e.target_samples))
                 if e.type in (P.EVENT_DONE, P.EVENT_ERROR, P.EVENT_CANCELLED):
                     done.add(e.request_id)
+                    if release:
+                        server.release(e.request_id)
         assert time.time() - t0 < timeout_s, "serving loop did not finish"
     return events
 
@@ -38,22 +43,27 @@ def sc():
     return {"default": (P.Scene(d0), c0), "cornell": (P.Scene(d2), c2)}
 
 
+BD = P.INTEGRATOR_BDPT
+
+
 def _requests():
-    # (scene, width, height, spp, passes, seed)
-    return [("default", 96, 54, 24, 4, 1), ("cornell", 64, 64, 40, 5, 2), ("default", 128, 72, 16, 3, 3),
-            ("cornell", 80, 48, 20, 4, 4), ("default", 64, 36, 12, 2, 5), ("cornell", 48, 48, 30, 6, 6)]
+    # (scene, width, height, spp, passes, seed, integrator)
+    PT = P.INTEGRATOR_PATH
+    return [("default", 96, 54, 24, 4, 1, PT), ("cornell", 64, 64, 40, 5, 2, PT), ("default", 128, 72, 16, 3, 3, PT),
+            ("cornell", 80, 48, 20, 4, 4, PT), ("default", 64, 36, 12, 2, 5, PT), ("cornell", 48, 48, 30, 6, 6, PT),
+            ("cornell", 48, 48, 12, 3, 7, BD), ("default", 64, 36, 8, 2, 8, BD)]
 
 
 def test_isolation_alone_vs_mix(sc):
     reqs = _requests()
     srv = P.Server(max_active=8)
     ids = []
-    for name, w, h, spp, passes, seed in reqs:
+    for name, w, h, spp, passes, seed, integ in reqs:
         scene, cfg = sc[name]
-        ids.append(srv.submit(scene, _params(w, h, cfg, spp, passes, seed)))
+        ids.append(srv.submit(scene, _params(w, h, cfg, spp, passes, seed, integ)))
     ev = _drain(srv, ids)
-    mix = [srv.read(i, w, h) for i, (_, w, h, _, _, _) in zip(ids, reqs)]
-    for rid, (name, w, h, spp, passes, seed) in zip(ids, reqs):
+    mix = [srv.read(i, w, h) for i, (_, w, h, _, _, _, _) in zip(ids, reqs)]
+    for rid, (name, w, h, spp, passes, seed, integ) in zip(ids, reqs):
         kinds = [k for k, _, _ in ev[rid]]
         assert kinds[0] == P.EVENT_ADMITTED and kinds[-1] == P.EVENT_DONE
         passes_seen = [p for k, p, _ in ev[rid] if k == P.EVENT_PASS_COMPLETE]
@@ -64,10 +74,10 @@ def test_isolation_alone_vs_mix(sc):
         srv.release(rid)
     srv.close()
     # each request alone on a fresh server
-    for (name, w, h, spp, passes, seed), (m_mix, c_mix) in zip(reqs, mix):
+    for (name, w, h, spp, passes, seed, integ), (m_mix, c_mix) in zip(reqs, mix):
         solo = P.Server(max_active=1)
         scene, cfg = sc[name]
-        rid = solo.submit(scene, _params(w, h, cfg, spp, passes, seed))
+        rid = solo.submit(scene, _params(w, h, cfg, spp, passes, seed, integ))
         _drain(solo, [rid])
         m, c = solo.read(rid, w, h)
         assert np.all(c == spp) and np.all(c_mix == spp)
@@ -96,7 +106,7 @@ def test_cancel_and_budget(sc):
     ids = [srv.submit(scene, _params(64, 64, cfg, 8, 2, s)) for s in range(4)]
     big = srv.submit(scene, _params(64, 64, cfg, 100000, 1000, 9))
     srv.cancel(big)
-    ev = _drain(srv, ids + [big])
+    ev = _drain(srv, ids + [big], release=True)
     assert ev[big][-1][0] == P.EVENT_CANCELLED
     for i in ids:
         assert ev[i][-1][0] == P.EVENT_DONE
