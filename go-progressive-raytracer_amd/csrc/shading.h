@@ -87,6 +87,10 @@ struct Surf {
 };
 
 // Hit record of a primitive hit (SetFaceNormal, material/interfaces.go:58-65).
+// The point ray.At(t) is moved back onto the primitive's surface (plane projection for triangles and
+// quads, radial for spheres): in float32, o + t*d far from the origin lies up to |p|*2^-24 off the
+// surface, enough for a grazing continuation ray to re-hit the surface beyond tmin = 0.001, which
+// the float64 reference never does.
 __device__ __forceinline__ Surf make_surf(const DevScene& sc, V3 o, V3 d, const THit& h) {
     const float4* pp = sc.prims + 3 * (size_t)h.prim;
     float4 a = pp[0], b = pp[1];
@@ -100,11 +104,17 @@ __device__ __forceinline__ Surf make_surf(const DevScene& sc, V3 o, V3 d, const 
         float4 c = pp[2];
         V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
         outward = normalize(cross(v1 - v0, v2 - v0));  // triangle.go:92-96
+        s.p = s.p - outward * dot(s.p - v0, outward);
     } else if (kind == kPrimSphere) {
-        outward = (s.p - v3(a.x, a.y, a.z)) * (1.0f / b.x);  // sphere.go:63, sign of r kept
+        V3 c = v3(a.x, a.y, a.z);
+        V3 r = s.p - c;
+        float len = length(r);
+        if (len > 0.0f) s.p = c + r * (fabsf(b.x) / len);
+        outward = (s.p - c) * (1.0f / b.x);  // sphere.go:63, sign of r kept
     } else {
         const QuadExt& q = sc.quads[prim_side(tag)];
         outward = v3(q.nx, q.ny, q.nz);
+        s.p = s.p - outward * (dot(s.p, outward) - q.D);
     }
     s.front = dot(d, outward) < 0.0f;
     s.n = s.front ? outward : outward * -1.0f;
@@ -291,14 +301,11 @@ __device__ __forceinline__ V3 emissive(const DevScene& sc, int m) {
 
 // sphere test for light routines (sphere.go:27-81), returns t or -1
 __device__ __forceinline__ float sphere_hit_t(V3 c, float rad, V3 o, V3 d, float tmin, float tmax) {
-    V3 oc = o - c;
-    float A = dot(d, d), hb = dot(oc, d), C = dot(oc, oc) - rad * rad;
-    float disc = hb * hb - A * C;
-    if (disc < 0.0f) return -1.0f;
-    float sq = sqrtf(disc);
-    float root = (-hb - sq) / A;
+    float t0, t1;
+    if (!sphere_roots(o.x - c.x, o.y - c.y, o.z - c.z, d.x, d.y, d.z, rad, t0, t1)) return -1.0f;
+    float root = t0;
     if (root < tmin || root > tmax) {
-        root = (-hb + sq) / A;
+        root = t1;
         if (root < tmin || root > tmax) return -1.0f;
     }
     return root;

@@ -100,18 +100,36 @@ __device__ __forceinline__ bool tri_test(const TRay& r, const WRay& w, float4 a,
 }
 
 // Sphere test (sphere.go:27-81): nearer root, else farther root, in [tmin, tbest].
-__device__ __forceinline__ bool sphere_test(const TRay& r, float4 a, float4 b, float tbest, float& t_out) {
-    float ocx = r.ox - a.x, ocy = r.oy - a.y, ocz = r.oz - a.z;
-    float rad = b.x;
-    float A = r.dx * r.dx + r.dy * r.dy + r.dz * r.dz;
-    float hb = ocx * r.dx + ocy * r.dy + ocz * r.dz;
-    float C = ocx * ocx + ocy * ocy + ocz * ocz - rad * rad;
-    float disc = hb * hb - A * C;
+// Ray/sphere roots (sphere.go:27-81 solves the same quadratic in float64). In float32 the textbook
+// discriminant hb^2 - A*C cancels catastrophically once |o - c| >> r (rays arriving from an
+// infinite light's sampling disk thousands of units away), so the discriminant is formed from the
+// perpendicular offset l = f - (f.d / d.d) d (disc = A * (r^2 - l.l)) and the roots by the
+// cancellation-free pair q / A, C / q (Haines et al., "Precision Improvements for Ray/Sphere
+// Intersection", Ray Tracing Gems, 2019). Returns false if the ray's line misses the sphere;
+// t0 <= t1 otherwise.
+__device__ __forceinline__ bool sphere_roots(float fx, float fy, float fz, float dx, float dy, float dz, float rad,
+                                             float& t0, float& t1) {
+    float A = dx * dx + dy * dy + dz * dz;
+    float hb = fx * dx + fy * dy + fz * dz;
+    float k = hb / A;
+    float lx = fx - k * dx, ly = fy - k * dy, lz = fz - k * dz;
+    float disc = A * (rad * rad - (lx * lx + ly * ly + lz * lz));
     if (disc < 0.0f) return false;
-    float sq = sqrtf(disc);
-    float root = (-hb - sq) / A;
+    float C = fx * fx + fy * fy + fz * fz - rad * rad;
+    float q = -(hb + copysignf(sqrtf(disc), hb));
+    float a = q / A, b = q != 0.0f ? C / q : a;
+    t0 = fminf(a, b);
+    t1 = fmaxf(a, b);
+    return true;
+}
+
+// Sphere test: the nearer root in [tmin, tbest], else the farther one (sphere.go:46-57).
+__device__ __forceinline__ bool sphere_test(const TRay& r, float4 a, float4 b, float tbest, float& t_out) {
+    float t0, t1;
+    if (!sphere_roots(r.ox - a.x, r.oy - a.y, r.oz - a.z, r.dx, r.dy, r.dz, b.x, t0, t1)) return false;
+    float root = t0;
     if (root < r.tmin || root > tbest) {
-        root = (-hb + sq) / A;
+        root = t1;
         if (root < r.tmin || root > tbest) return false;
     }
     t_out = root;
