@@ -32,6 +32,7 @@ def main():
     ap.add_argument("--no-pt", action="store_true", help="skip the path-tracing throughput section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (time-to-first-pass) section")
     ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--serve-bdpt", type=int, default=1, help="1: half of the served requests use BDPT")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (counter calibration) and exit")
     a = ap.parse_args()
@@ -169,20 +170,25 @@ def pt_section(a, log):
     import prtgpu as P
     import scenes
 
+    PT, BD = P.INTEGRATOR_PATH, P.INTEGRATOR_BDPT
+    # (name, scene, integrator, spp override, oracle spp of the bounded CPU sample)
     configs = [
-        ("C0 default scene 400x225, 64 spp, depth 50 (pinhole)", lambda: scenes.default_scene(), None),
-        ("C2 metre Cornell boxes 512x512, 1024 spp, depth 8", lambda: scenes.cornell_metre(), None),
+        ("C0 default scene 400x225, 64 spp, depth 50 (pinhole)", lambda: scenes.default_scene(), PT, None, 256),
+        ("C2 metre Cornell boxes 512x512, 1024 spp, depth 8", lambda: scenes.cornell_metre(), PT, None, 64),
         ("C3 1.8M-triangle glass icosphere + HDR sky 1024x1024, 256 spp, depth 16",
-         lambda: scenes.mesh_sky_scene(), None),
+         lambda: scenes.mesh_sky_scene(), PT, None, 8),
+        ("C2-BDPT metre Cornell boxes 512x512, bidirectional, 64 spp, depth 8", lambda: scenes.cornell_metre(), BD, 64,
+         16),
     ]
     out = []
     cores = min(os.cpu_count() or 1, 16)
-    for name, make, _ in configs:
+    for name, make, integ, spp_override, c_spp in configs:
         desc, cfg = make()
         w, h = desc.image_size
-        spp = cfg["spp"]
+        spp = spp_override or cfg["spp"]
         scene = P.Scene(desc, device=0)
-        params = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp, max_passes=1, seed=1)
+        params = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp, max_passes=1, seed=1,
+                                   integrator=integ)
         pr = P.ProgressiveRaytracer(scene, params)
         pr.render_samples(1)  # warm-up (kernels, first-touch)
         pr.reset()
@@ -192,7 +198,7 @@ def pt_section(a, log):
         gpu_sps = w * h * spp / wall
         # first-pass latency: one sample per pixel on a fresh film (reference pass 1, progressive.go:104-107)
         pr2 = P.ProgressiveRaytracer(scene, P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp,
-                                                              max_passes=7, seed=2))
+                                                              max_passes=7, seed=2, integrator=integ))
         t0 = time.perf_counter()
         pr2.render_pass(1)
         ttfp_ms = (time.perf_counter() - t0) * 1e3
@@ -203,10 +209,9 @@ def pt_section(a, log):
             import oracle as O
 
             os_ = O.OracleScene(desc)
-            # bounded sample: the whole image at a few samples per pixel (~2-4 s of CPU work)
-            c_spp = {0: 256, 1: 64, 2: 8}[len(out)]
+            # bounded sample: the whole image at a few samples per pixel (~2-6 s of CPU work)
             c0 = time.perf_counter()
-            os_.render(params, c_spp, 0, nthreads=cores)
+            os_.render(params, c_spp, 0, nthreads=cores, splats=integ == BD)
             ct = time.perf_counter() - c0
             cpu_sps = w * h * c_spp / ct
             entry["cpu_samples_per_s"] = round(cpu_sps, 1)
@@ -239,6 +244,7 @@ def serve_section(a, log):
     picks = rng.integers(0, 3, n)
     res = rng.integers(0, 4, n)
     spps = np.exp(rng.uniform(np.log(16), np.log(1024), n)).astype(int)
+    integs = rng.integers(0, 2, n) if a.serve_bdpt else np.zeros(n, int)
     srv = P.Server(hbm_budget_bytes=192 << 30, max_active=1024)
     ids, meta = [], {}
     t0 = time.perf_counter()
@@ -249,7 +255,8 @@ def serve_section(a, log):
         while k < n and arrivals[k] <= now:
             scene, cfg = sc[picks[k]]
             w, h = sizes[res[k]]
-            p = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], int(spps[k]), max_passes=7, seed=k)
+            p = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], int(spps[k]), max_passes=7, seed=k,
+                                  integrator=int(integs[k]))
             rid = srv.submit(scene, p)
             meta[rid] = (k, w * h * int(spps[k]))
             k += 1
@@ -268,7 +275,8 @@ def serve_section(a, log):
            "ttfp_ms_p99": round(float(np.percentile(t, 99)), 2), "ttfp_ms_max": round(float(t.max()), 2),
            "done_s_p50": round(float(np.percentile(d, 50)) / 1e3, 3),
            "done_s_p99": round(float(np.percentile(d, 99)) / 1e3, 3), "makespan_s": round(makespan, 2),
-           "samples_per_s": round(samples / makespan, 1), "integrators": "path tracing only"}
+           "samples_per_s": round(samples / makespan, 1),
+           "integrators": "path tracing and BDPT, 1:1" if a.serve_bdpt else "path tracing only"}
     log(f"serving: {out}")
     return out
 

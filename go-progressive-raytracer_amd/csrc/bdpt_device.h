@@ -28,40 +28,45 @@ struct BV {
     uint32_t flags;
 };
 
+// Subpath vertices in global memory as five planes of float4, plane f of vertex k of sample i at
+// [(f * kmax + k) * stride + i] (each plane coalesced across a wave). Plane 0 is the record the MIS
+// walk reads for every vertex: (pdf_fwd, pdf_rev, flags | (mat + 1) << 8, light).
 struct VertexStore {
-    float4* data;     // 5 float4 per vertex
+    float4* data;
     uint32_t stride;  // samples per chunk
-    __device__ __forceinline__ void put(int k, uint32_t i, const BV& v) const {
-        float4* b = data + (size_t)5 * ((size_t)k * stride + i);
-        b[0] = make_float4(v.p.x, v.p.y, v.p.z, v.pdf_fwd);
-        b[1] = make_float4(v.n.x, v.n.y, v.n.z, v.pdf_rev);
-        b[2] = make_float4(v.wi.x, v.wi.y, v.wi.z, __int_as_float(v.mat));
-        b[3] = make_float4(v.beta.x, v.beta.y, v.beta.z, __int_as_float(v.light));
-        b[4] = make_float4(v.Le.x, v.Le.y, v.Le.z, __uint_as_float(v.flags));
+    uint32_t kmax;    // vertices per subpath
+    __device__ __forceinline__ float4* at(int f, int k, uint32_t i) const {
+        return data + ((size_t)f * kmax + (size_t)k) * stride + i;
     }
+    __device__ __forceinline__ void put(int k, uint32_t i, const BV& v) const {
+        *at(0, k, i) = make_float4(v.pdf_fwd, v.pdf_rev, __uint_as_float(v.flags | ((uint32_t)(v.mat + 1) << 8)),
+                                   __int_as_float(v.light));
+        *at(1, k, i) = make_float4(v.p.x, v.p.y, v.p.z, v.beta.x);
+        *at(2, k, i) = make_float4(v.n.x, v.n.y, v.n.z, v.beta.y);
+        *at(3, k, i) = make_float4(v.wi.x, v.wi.y, v.wi.z, v.beta.z);
+        *at(4, k, i) = make_float4(v.Le.x, v.Le.y, v.Le.z, 0.0f);
+    }
+    __device__ __forceinline__ float4 rec(int k, uint32_t i) const { return *at(0, k, i); }
     __device__ __forceinline__ BV get(int k, uint32_t i) const {
-        const float4* b = data + (size_t)5 * ((size_t)k * stride + i);
-        float4 a = b[0], c = b[1], d = b[2], e = b[3], f = b[4];
+        float4 r = *at(0, k, i), a = *at(1, k, i), b = *at(2, k, i), c = *at(3, k, i), e = *at(4, k, i);
         BV v;
+        v.pdf_fwd = r.x;
+        v.pdf_rev = r.y;
+        uint32_t fm = __float_as_uint(r.z);
+        v.flags = fm & 0xFFu;
+        v.mat = (int)(fm >> 8) - 1;
+        v.light = __float_as_int(r.w);
         v.p = v3(a.x, a.y, a.z);
-        v.pdf_fwd = a.w;
-        v.n = v3(c.x, c.y, c.z);
-        v.pdf_rev = c.w;
-        v.wi = v3(d.x, d.y, d.z);
-        v.mat = __float_as_int(d.w);
-        v.beta = v3(e.x, e.y, e.z);
-        v.light = __float_as_int(e.w);
-        v.Le = v3(f.x, f.y, f.z);
-        v.flags = __float_as_uint(f.w);
+        v.n = v3(b.x, b.y, b.z);
+        v.wi = v3(c.x, c.y, c.z);
+        v.beta = v3(a.w, b.w, c.w);
+        v.Le = v3(e.x, e.y, e.z);
         return v;
     }
-    __device__ __forceinline__ void set_pdf_rev(int k, uint32_t i, float pdf) const {
-        data[(size_t)5 * ((size_t)k * stride + i) + 1].w = pdf;
-    }
-    __device__ __forceinline__ void set_pdf_fwd(int k, uint32_t i, float pdf) const {
-        data[(size_t)5 * ((size_t)k * stride + i)].w = pdf;
-    }
+    __device__ __forceinline__ void set_pdf_rev(int k, uint32_t i, float pdf) const { at(0, k, i)->y = pdf; }
+    __device__ __forceinline__ void set_pdf_fwd(int k, uint32_t i, float pdf) const { at(0, k, i)->x = pdf; }
 };
+__device__ __forceinline__ bool rec_specular(float4 r) { return (__float_as_uint(r.z) & VF_SPECULAR) != 0; }
 
 __device__ __forceinline__ bool light_area(const DevScene& sc, int li) {
     int t = sc.lights[li].type;
@@ -282,80 +287,98 @@ __device__ __forceinline__ float vtx_pdf(const DevScene& sc, const DevCamera& ca
     return convert_density(sc, cur, next, pdf);
 }
 
-// calculateMISWeight with the per-vertex rules of calculateMIS{Camera,Light}VertexPdfs (:52-187)
-__device__ float mis_weight(const DevScene& sc, const DevCamera& cam, const VertexStore& CP, const VertexStore& LP,
-                            uint32_t i, const BV* sv, int s, int t) {
+// calculateMISWeight with the per-vertex rules of calculateMIS{Camera,Light}VertexPdfs (:52-187).
+// ct1 = camera vertex t-1, ls1 = light vertex s-1 (s >= 1), sv = the sampled vertex of the s = 1 and
+// t = 1 strategies. Only the vertices next to the connection are read whole; the walk over the rest
+// reads their records.
+__device__ __forceinline__ float mis_weight(const DevScene& sc, const DevCamera& cam, const VertexStore& CP,
+                                           const VertexStore& LP, uint32_t i, const BV& sv, const BV& ct1,
+                                           const BV& ls1, int s, int t) {
     if (s + t == 2) return 1.0f;
-    float sum = 0.0f, ri = 1.0f;
-    for (int k = t - 1; k > 0; k--) {
-        BV v = CP.get(k, i);
-        float fwd = v.pdf_fwd, rev = v.pdf_rev;
-        bool conn = !(v.flags & VF_SPECULAR);
-        if (s == 0) {
-            if (k == t - 1 && t > 1) {
-                rev = vtx_light_origin_pdf(sc, v, CP.get(t - 2, i));
-                conn = true;
-            } else if (k == t - 2 && t > 2) {
-                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), nullptr, v);
-            }
-        } else if (t == 1) {
-            // k > 0 never reaches the camera vertex here
-        } else if (s == 1) {
-            if (k == t - 1) {
-                rev = vtx_pdf(sc, cam, *sv, nullptr, v);
-                conn = true;
-            } else if (k == t - 2 && t > 1) {
-                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), sv, v);
-            }
-        } else {
-            if (k == t - 1) {
-                BV ls2 = LP.get(s - 2, i);
-                rev = vtx_pdf(sc, cam, LP.get(s - 1, i), &ls2, v);
-                conn = true;
-            } else if (k == t - 2 && t > 1) {
-                BV ls1 = LP.get(s - 1, i);
-                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), &ls1, v);
+    // reverse densities of the vertices next to the connection
+    float c_rev1 = 0.0f, c_rev2 = 0.0f, l_rev1 = 0.0f, l_rev2 = 0.0f;
+    bool has_c2 = false, has_l2 = false;
+    if (s == 0) {
+        if (t > 1) {
+            BV ct2 = CP.get(t - 2, i);
+            c_rev1 = vtx_light_origin_pdf(sc, ct1, ct2);
+            if (t > 2) {
+                c_rev2 = vtx_pdf(sc, cam, ct1, nullptr, ct2);
+                has_c2 = true;
             }
         }
-        conn = conn && !(CP.get(k - 1, i).flags & VF_SPECULAR);
-        ri *= remap0(rev) / remap0(fwd);
-        if (conn) sum += ri;
+    } else if (t == 1) {
+        if (s > 1) {
+            BV ls2 = LP.get(s - 2, i);
+            l_rev1 = vtx_pdf(sc, cam, sv, nullptr, ls1);
+            l_rev2 = vtx_pdf(sc, cam, ls1, &sv, ls2);
+            has_l2 = true;
+        }
+    } else if (s == 1) {
+        BV ct2 = CP.get(t - 2, i);
+        c_rev1 = vtx_pdf(sc, cam, sv, nullptr, ct1);
+        c_rev2 = vtx_pdf(sc, cam, ct1, &sv, ct2);
+        has_c2 = true;
+        l_rev1 = vtx_pdf(sc, cam, ct1, &ct2, sv);
+    } else {
+        BV ct2 = CP.get(t - 2, i), ls2 = LP.get(s - 2, i);
+        c_rev1 = vtx_pdf(sc, cam, ls1, &ls2, ct1);
+        c_rev2 = vtx_pdf(sc, cam, ct1, &ls1, ct2);
+        has_c2 = true;
+        l_rev1 = vtx_pdf(sc, cam, ct1, &ct2, ls1);
+        l_rev2 = vtx_pdf(sc, cam, ls1, &ct1, ls2);
+        has_l2 = true;
     }
-    if (t == 1) {
-        // camera vertex 0 takes the sampled camera vertex's densities
+    float sum = 0.0f, ri = 1.0f;
+    if (t > 1) {
+        float4 cur = CP.rec(t - 1, i);
+        for (int k = t - 1; k > 0; k--) {
+            float4 prev = CP.rec(k - 1, i);
+            float fwd = cur.x, rev = cur.y;
+            bool conn = !rec_specular(cur);
+            if (k == t - 1) {
+                rev = c_rev1;
+                conn = true;
+            } else if (k == t - 2 && has_c2) {
+                rev = c_rev2;
+            }
+            conn = conn && !rec_specular(prev);
+            ri *= remap0(rev) / remap0(fwd);
+            if (conn) sum += ri;
+            cur = prev;
+        }
     }
     ri = 1.0f;
-    for (int k = s - 1; k >= 0; k--) {
-        BV v = LP.get(k, i);
-        float fwd = v.pdf_fwd, rev = v.pdf_rev;
-        bool conn = !(v.flags & VF_SPECULAR);
-        if (k > 0) conn = conn && !(LP.get(k - 1, i).flags & VF_SPECULAR);
-        if (t == 1) {
-            if (k == s - 1 && s > 1) {
-                rev = vtx_pdf(sc, cam, *sv, nullptr, v);
+    if (s > 0) {
+        float4 cur = LP.rec(s - 1, i);
+        for (int k = s - 1; k >= 0; k--) {
+            float4 prev = k > 0 ? LP.rec(k - 1, i) : make_float4(0.f, 0.f, 0.f, 0.f);
+            float fwd = cur.x, rev = cur.y;
+            bool conn = !rec_specular(cur);
+            if (k > 0) conn = conn && !rec_specular(prev);
+            if (t == 1) {
+                if (k == s - 1 && s > 1) {
+                    rev = l_rev1;
+                    conn = true;
+                } else if (k == s - 2 && has_l2) {
+                    rev = l_rev2;
+                }
+            } else if (s == 1) {
+                fwd = sv.pdf_fwd;
+                rev = l_rev1;
                 conn = true;
-            } else if (k == s - 2 && s > 1) {
-                rev = vtx_pdf(sc, cam, LP.get(s - 1, i), sv, v);
+            } else {
+                if (k == s - 1) {
+                    rev = l_rev1;
+                    conn = true;
+                } else if (k == s - 2) {
+                    rev = l_rev2;
+                }
             }
-        } else if (s == 1) {
-            if (k == s - 1 && sv) {
-                fwd = sv->pdf_fwd;
-                BV c2 = CP.get(t - 2, i);
-                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), &c2, *sv);
-                conn = true;
-            }
-        } else {
-            if (k == s - 1) {
-                BV c2 = CP.get(t - 2, i);
-                rev = vtx_pdf(sc, cam, CP.get(t - 1, i), &c2, v);
-                conn = true;
-            } else if (k == s - 2 && s > 1) {
-                BV c1 = CP.get(t - 1, i);
-                rev = vtx_pdf(sc, cam, LP.get(s - 1, i), &c1, v);
-            }
+            ri *= remap0(rev) / remap0(fwd);
+            if (conn) sum += ri;
+            cur = prev;
         }
-        ri *= remap0(rev) / remap0(fwd);
-        if (conn) sum += ri;
     }
     return 1.0f / (1.0f + sum);
 }

@@ -337,9 +337,10 @@ __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i
     return n;
 }
 
+// Subpath generation: one thread per sample writes its camera and light subpaths and their lengths.
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_bdpt(DevScene sc, PassParams pp, PathBuffers pb, VertexStore CP,
-                                                 VertexStore LP, long long* splat, uint2* gstack, uint32_t gstride) {
+__global__ __launch_bounds__(kBlock) void k_bdpt_paths(DevScene sc, PassParams pp, VertexStore CP, VertexStore LP,
+                                                       uint32_t* lens, uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
@@ -355,9 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt(DevScene sc, PassParams pp, Pat
     const DevCamera& cam = sc.cam;
     const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
     const uint32_t stride = gridDim.x * kBlock;
-    // subpath vertices are per thread (the store holds one camera and one light path per thread)
-    const uint32_t vi = blockIdx.x * kBlock + threadIdx.x;
-    for (uint32_t i = vi; i < n; i += stride) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
         uint32_t s_local = i / (uint32_t)pp.npix;
         uint32_t pix = (uint32_t)pp.px0 + i % (uint32_t)pp.npix;
         int px = (int)(pix % (uint32_t)pp.width), py = (int)(pix / (uint32_t)pp.width);
@@ -375,8 +374,8 @@ __global__ __launch_bounds__(kBlock) void k_bdpt(DevScene sc, PassParams pp, Pat
         cv.mat = -1;
         cv.light = -1;
         cv.flags = VF_CAMERA;
-        CP.put(0, vi, cv);
-        int nc = bdpt_extend<MIXED>(sc, CP, vi, 1, o, d, v3(1, 1, 1), cam_dir_pdf(cam, o, d), pp.max_depth, true, key, 1,
+        CP.put(0, i, cv);
+        int nc = bdpt_extend<MIXED>(sc, CP, i, 1, o, d, v3(1, 1, 1), cam_dir_pdf(cam, o, d), pp.max_depth, true, key, 1,
                                     st);
         // light subpath (generateLightPath, bdpt.go:139-203)
         int nl = 0;
@@ -400,140 +399,170 @@ __global__ __launch_bounds__(kBlock) void k_bdpt(DevScene sc, PassParams pp, Pat
             lv.light = li;
             bool inf = light_is_infinite(L.type);
             lv.flags = VF_LIGHT | (inf ? VF_INF : 0u);
-            LP.put(0, vi, lv);
+            LP.put(0, i, lv);
             V3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
-            nl = bdpt_extend<MIXED>(sc, LP, vi, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false, key,
+            nl = bdpt_extend<MIXED>(sc, LP, i, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false, key,
                                     kLightSlot + 1, st);
             if (inf) {
                 if (nl > 1) {
-                    BV fb = LP.get(1, vi);
+                    BV fb = LP.get(1, i);
                     float pf = es.area_pdf * sel;
                     if (fb.mat >= 0) pf *= fabsf(dot(es.dir, fb.n));
-                    LP.set_pdf_fwd(1, vi, pf);
+                    LP.set_pdf_fwd(1, i, pf);
                 }
-                LP.set_pdf_fwd(0, vi, infinite_density(sc, es.p, es.n, es.dir));
+                LP.set_pdf_fwd(0, i, infinite_density(sc, es.p, es.n, es.dir));
             }
         }
-        // strategies (RayColor :83-99 with evaluateBDPTStrategy :294-313)
+        lens[i] = (uint32_t)nc | ((uint32_t)nl << 16);
+    }
+}
+
+// Strategies: one thread per sample evaluates every (s, t) of RayColor (bdpt.go:83-99 with
+// evaluateBDPTStrategy :294-313), camera-side radiance to pb.L, light-tracing splats to `splat`.
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_bdpt_connect(DevScene sc, PassParams pp, PathBuffers pb, VertexStore CP,
+                                                         VertexStore LP, const uint32_t* lens, long long* splat,
+                                                         uint2* gstack, uint32_t gstride) {
+    __shared__ uint32_t s_node[kLds * kBlock];
+    __shared__ float s_t[kLds * kBlock];
+    __shared__ float4 s_cache[kNodeCache * 8];
+    TStack<kLds, kBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kBlock + threadIdx.x;
+    st.gstride = gstride;
+    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
+    st.ncache = s_cache;
+    const DevCamera& cam = sc.cam;
+    const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        uint32_t s_local = i / (uint32_t)pp.npix;
+        uint32_t pix = (uint32_t)pp.px0 + i % (uint32_t)pp.npix;
+        uint32_t key = path_key(pp.seed, pix, (uint32_t)pp.sample_base + s_local);
+        const uint32_t len = lens[i];
+        const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
         V3 total = v3(0, 0, 0);
-        for (int s = 0; s <= nl; s++) {
-            for (int t = 1; t <= nc; t++) {
-                if (s == 1 && t == 1) continue;
-                V3 Lc = v3(0, 0, 0);
-                bool splatted = false;
-                int sx = 0, sy = 0;
-                BV sampled;
-                const BV* sv = nullptr;
-                if (s == 0) {  // evaluatePathTracingStrategy :316-330
-                    if (t < nc) continue;
-                    BV last = CP.get(t - 1, vi);
-                    Lc = mulv(last.Le, last.beta);
-                } else if (t == 1) {  // evaluateLightTracingStrategy :394-471
-                    if (s <= 1) continue;
-                    BV lv = LP.get(s - 1, vi);
-                    if (lv.flags & VF_SPECULAR) continue;
-                    const uint32_t sp0 = (kSplatSlot + (uint32_t)s) * 16u;
-                    V3 lc = sample_point_in_unit_disk(draw(key, sp0), draw(key, sp0 + 1)) * cam.lens_radius;
-                    V3 lens = v3(cam.center) + v3(cam.u) * lc.x + v3(cam.v) * lc.y;
-                    V3 dir = normalize(lv.p - lens);
-                    float dist = length(lv.p - lens);
-                    float cf = fabsf(dot(v3(cam.forward), dir));
-                    float cpdf = (dist * dist) / (cf * cam.lens_area);
-                    float ctw = dot(dir, v3(cam.forward));
-                    float we = ctw < cam.cos_total_width ? 0.0f : 1.0f / (cam.plane_area * cam.lens_area * ctw * ctw * ctw * ctw);
-                    if (we == 0.0f) continue;
-                    V3 back = dir * -1.0f;
-                    V3 f = vertex_brdf(sc, lv, back, 1);
-                    V3 cbeta = v3(we, we, we) * (1.0f / cpdf);
-                    float cosine = dot(back, lv.n);
-                    if (cosine <= 0.0f) continue;
-                    V3 contrib = mulv(mulv(f, cbeta), lv.beta);
-                    if (on_surface(sc, lv)) contrib = contrib * cosine;
-                    if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
-                    float vd = length(lv.p - lens);
-                    TRay vr{lv.p.x, lv.p.y, lv.p.z, 0.001f, back.x, back.y, back.z, vd - 0.001f};
-                    THit vh;
-                    if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr)) continue;
-                    sampled.p = lens;
-                    sampled.n = back;
-                    sampled.wi = v3(0, 0, 0);
-                    sampled.beta = cbeta;
-                    sampled.Le = v3(0, 0, 0);
-                    sampled.pdf_fwd = cpdf;
-                    sampled.pdf_rev = total_light_pdf(sc, lens, back, normalize(lv.p - lens));
-                    sampled.mat = -1;
-                    sampled.light = -1;
-                    sampled.flags = VF_CAMERA;
-                    sv = &sampled;
-                    if (!cam_map(cam, lens, dir, sx, sy)) {
-                        // the splat lands outside the image (SplatQueue drops it)
-                        continue;
-                    }
-                    Lc = contrib;
-                    splatted = true;
-                } else if (s == 1) {  // evaluateDirectLightingStrategy :333-391
-                    BV cvt = CP.get(t - 1, vi);
-                    if ((cvt.flags & VF_SPECULAR) || cvt.mat < 0) continue;
-                    const uint32_t dp0 = (kDirectSlot + (uint32_t)t) * 16u;
-                    float sel;
-                    int li = select_light(sc, draw(key, dp0), sel);
-                    LSample ls = light_sample(sc, sc.lights[li], cvt.p, cvt.n, draw(key, dp0 + 1), draw(key, dp0 + 2));
-                    ls.pdf *= sel;
-                    if ((ls.emission.x == 0.0f && ls.emission.y == 0.0f && ls.emission.z == 0.0f) || ls.pdf <= 0.0f)
-                        continue;
-                    float ct = fabsf(dot(ls.dir, cvt.n));
-                    if (ct <= 0.0f) continue;
-                    V3 f = brdf(sc, cvt.mat, cvt.wi, ls.dir, surf_of(cvt), 0);
-                    V3 lbeta = ls.emission * (1.0f / ls.pdf);
-                    V3 contrib = mulv(mulv(f, cvt.beta), lbeta) * ct;
-                    if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
-                    TRay vr{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - 0.001f};
-                    THit vh;
-                    if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr)) continue;
-                    sampled.p = ls.p;
-                    sampled.n = ls.n;
-                    sampled.wi = v3(0, 0, 0);
-                    sampled.beta = lbeta;
-                    sampled.Le = ls.emission;
-                    sampled.pdf_fwd = ls.pdf;
-                    sampled.pdf_rev = 0.0f;
-                    sampled.mat = -1;
-                    sampled.light = li;
-                    sampled.flags = VF_LIGHT | (light_is_infinite(sc.lights[li].type) ? VF_INF : 0u);
-                    sv = &sampled;
-                    Lc = contrib;
-                } else {  // evaluateConnectionStrategy :489-561
-                    BV lv = LP.get(s - 1, vi), cvt = CP.get(t - 1, vi);
-                    if ((lv.flags & VF_SPECULAR) || (cvt.flags & VF_SPECULAR)) continue;
-                    V3 dd = lv.p - cvt.p;
-                    float dist = length(dd);
-                    if (dist < 0.001f) continue;
-                    dd = dd * (1.0f / dist);
-                    float cc = dot(dd, cvt.n), cl = dot(dd * -1.0f, lv.n);
-                    if (cc <= 0.0f || cl <= 0.0f) continue;
-                    float G = (cc * cl) / (dist * dist);
-                    V3 fc = vertex_brdf(sc, cvt, dd, 0);
-                    V3 fl = vertex_brdf(sc, lv, dd * -1.0f, 1);
-                    V3 contrib = mulv(mulv(mulv(lv.beta, fl), fc), cvt.beta) * G;
-                    if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
-                    TRay vr{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, dd.x, dd.y, dd.z, dist - 0.001f};
-                    THit vh;
-                    if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr)) continue;
-                    Lc = contrib;
-                }
-                if (Lc.x == 0.0f && Lc.y == 0.0f && Lc.z == 0.0f && !splatted) continue;
-                float w = mis_weight(sc, cam, CP, LP, vi, sv, s, t);
-                if (splatted) {
-                    V3 c = Lc * w;
-                    // fixed point holds |c| < 2^36 per splat (non-finite or larger splats are dropped)
-                    if (!(fabsf(c.x) < 6.8e10f && fabsf(c.y) < 6.8e10f && fabsf(c.z) < 6.8e10f)) continue;
-                    long long* o3 = splat + 3 * ((size_t)sy * pp.width + sx);
-                    atomicAdd((unsigned long long*)(o3 + 0), (unsigned long long)llrintf(c.x * kSplatScale));
-                    atomicAdd((unsigned long long*)(o3 + 1), (unsigned long long)llrintf(c.y * kSplatScale));
-                    atomicAdd((unsigned long long*)(o3 + 2), (unsigned long long)llrintf(c.z * kSplatScale));
-                } else {
-                    total = total + Lc * w;
-                }
+        // (s, t) in the reference's order (s outer, t inner), as one flat loop per thread
+        const int nst = (nl + 1) * nc;
+        for (int j = 0; j < nst; j++) {
+            const int s = j / nc, t = j - s * nc + 1;
+            if (s == 1 && t == 1) continue;
+            if (s == 0 && t < nc) continue;
+            V3 Lc = v3(0, 0, 0);
+            bool splatted = false;
+            int sx = 0, sy = 0;
+            BV sampled, ct1, ls1;
+            TRay vr;  // visibility ray of the strategy (one trace call site keeps the kernel call-free)
+            bool vis = false;
+            ct1 = CP.get(t - 1, i);
+            if (s > 0) ls1 = LP.get(s - 1, i);
+            if (s == 0) {  // evaluatePathTracingStrategy :316-330
+                Lc = mulv(ct1.Le, ct1.beta);
+            } else if (t == 1) {  // evaluateLightTracingStrategy :394-471
+                const BV& lv = ls1;
+                if (lv.flags & VF_SPECULAR) continue;
+                const uint32_t sp0 = (kSplatSlot + (uint32_t)s) * 16u;
+                V3 lc = sample_point_in_unit_disk(draw(key, sp0), draw(key, sp0 + 1)) * cam.lens_radius;
+                V3 lens_p = v3(cam.center) + v3(cam.u) * lc.x + v3(cam.v) * lc.y;
+                V3 dir = normalize(lv.p - lens_p);
+                float dist = length(lv.p - lens_p);
+                float cf = fabsf(dot(v3(cam.forward), dir));
+                float cpdf = (dist * dist) / (cf * cam.lens_area);
+                float ctw = dot(dir, v3(cam.forward));
+                float we = ctw < cam.cos_total_width
+                               ? 0.0f
+                               : 1.0f / (cam.plane_area * cam.lens_area * ctw * ctw * ctw * ctw);
+                if (we == 0.0f) continue;
+                V3 back = dir * -1.0f;
+                V3 f = vertex_brdf(sc, lv, back, 1);
+                V3 cbeta = v3(we, we, we) * (1.0f / cpdf);
+                float cosine = dot(back, lv.n);
+                if (cosine <= 0.0f) continue;
+                V3 contrib = mulv(mulv(f, cbeta), lv.beta);
+                if (on_surface(sc, lv)) contrib = contrib * cosine;
+                if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
+                // the splat lands outside the image: SplatQueue drops it, nothing else uses it
+                if (!cam_map(cam, lens_p, dir, sx, sy)) continue;
+                vr = TRay{lv.p.x, lv.p.y, lv.p.z, 0.001f, back.x, back.y, back.z, dist - 0.001f};
+                vis = true;
+                sampled.p = lens_p;
+                sampled.n = back;
+                sampled.wi = v3(0, 0, 0);
+                sampled.beta = cbeta;
+                sampled.Le = v3(0, 0, 0);
+                sampled.pdf_fwd = cpdf;
+                sampled.pdf_rev = total_light_pdf(sc, lens_p, back, dir);
+                sampled.mat = -1;
+                sampled.light = -1;
+                sampled.flags = VF_CAMERA;
+                Lc = contrib;
+                splatted = true;
+            } else if (s == 1) {  // evaluateDirectLightingStrategy :333-391
+                const BV& cvt = ct1;
+                if ((cvt.flags & VF_SPECULAR) || cvt.mat < 0) continue;
+                const uint32_t dp0 = (kDirectSlot + (uint32_t)t) * 16u;
+                float sel;
+                int li = select_light(sc, draw(key, dp0), sel);
+                LSample ls = light_sample(sc, sc.lights[li], cvt.p, cvt.n, draw(key, dp0 + 1), draw(key, dp0 + 2));
+                ls.pdf *= sel;
+                if ((ls.emission.x == 0.0f && ls.emission.y == 0.0f && ls.emission.z == 0.0f) || ls.pdf <= 0.0f)
+                    continue;
+                float ct = fabsf(dot(ls.dir, cvt.n));
+                if (ct <= 0.0f) continue;
+                V3 f = brdf(sc, cvt.mat, cvt.wi, ls.dir, surf_of(cvt), 0);
+                V3 lbeta = ls.emission * (1.0f / ls.pdf);
+                V3 contrib = mulv(mulv(f, cvt.beta), lbeta) * ct;
+                if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
+                vr = TRay{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - 0.001f};
+                vis = true;
+                sampled.p = ls.p;
+                sampled.n = ls.n;
+                sampled.wi = v3(0, 0, 0);
+                sampled.beta = lbeta;
+                sampled.Le = ls.emission;
+                sampled.pdf_fwd = ls.pdf;
+                sampled.pdf_rev = 0.0f;
+                sampled.mat = -1;
+                sampled.light = li;
+                sampled.flags = VF_LIGHT | (light_is_infinite(sc.lights[li].type) ? VF_INF : 0u);
+                Lc = contrib;
+            } else {  // evaluateConnectionStrategy :489-561
+                const BV &lv = ls1, &cvt = ct1;
+                if ((lv.flags & VF_SPECULAR) || (cvt.flags & VF_SPECULAR)) continue;
+                V3 dd = lv.p - cvt.p;
+                float dist = length(dd);
+                if (dist < 0.001f) continue;
+                dd = dd * (1.0f / dist);
+                float cc = dot(dd, cvt.n), cl = dot(dd * -1.0f, lv.n);
+                if (cc <= 0.0f || cl <= 0.0f) continue;
+                float G = (cc * cl) / (dist * dist);
+                V3 fc = vertex_brdf(sc, cvt, dd, 0);
+                V3 fl = vertex_brdf(sc, lv, dd * -1.0f, 1);
+                V3 contrib = mulv(mulv(mulv(lv.beta, fl), fc), cvt.beta) * G;
+                if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
+                vr = TRay{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, dd.x, dd.y, dd.z, dist - 0.001f};
+                vis = true;
+                Lc = contrib;
+            }
+            if (vis) {
+                THit vh;
+                if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr)) continue;
+            }
+            if (Lc.x == 0.0f && Lc.y == 0.0f && Lc.z == 0.0f && !splatted) continue;
+            float w = mis_weight(sc, cam, CP, LP, i, sampled, ct1, ls1, s, t);
+            if (splatted) {
+                V3 c = Lc * w;
+                // fixed point holds |c| < 2^36 per splat (non-finite or larger splats are dropped)
+                if (!(fabsf(c.x) < 6.8e10f && fabsf(c.y) < 6.8e10f && fabsf(c.z) < 6.8e10f)) continue;
+                long long* o3 = splat + 3 * ((size_t)sy * pp.width + sx);
+                atomicAdd((unsigned long long*)(o3 + 0), (unsigned long long)llrintf(c.x * kSplatScale));
+                atomicAdd((unsigned long long*)(o3 + 1), (unsigned long long)llrintf(c.y * kSplatScale));
+                atomicAdd((unsigned long long*)(o3 + 2), (unsigned long long)llrintf(c.z * kSplatScale));
+            } else {
+                total = total + Lc * w;
             }
         }
         pb.L[i] = make_float4(total.x, total.y, total.z, 0.f);
@@ -604,10 +633,11 @@ struct prt_film {
     uint2* gstack = nullptr;
     uint32_t gslots = 0;
     int blocks = 0;
-    // BDPT: per-thread camera and light subpath stores of the bdpt_blocks-block grid
+    // BDPT: camera and light subpath stores and subpath lengths, per path slot
     bool bdpt = false;
     int bdpt_blocks = 0;
     VertexStore cpath{}, lpath{};
+    uint32_t* lens = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // chunk plan of the samples being rendered (see film_plan)
@@ -653,19 +683,27 @@ size_t film_gstack_entries(const prt_scene* scene) {
     return scene->max_stack > kLds ? (size_t)(scene->max_stack - kLds + 1) : 1;
 }
 
-// Grid of the BDPT kernel: resident blocks only (each thread loops over samples), at most the
+// Grid of the BDPT kernels: resident blocks only (each thread loops over samples), at most the
 // trace grid so the traversal spill stack covers it.
 int bdpt_blocks(const prt_scene* scene, int trace_blocks) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_bdpt<true>, kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_bdpt_connect<true>, kBlock, 0) !=
+            hipSuccess ||
         per_cu <= 0)
         per_cu = 2;
-    return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu));
+    return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu * 2));
 }
 
-// Bytes of one subpath store: (max_depth + 1) vertices of 5 float4 per thread.
-size_t bdpt_store_bytes(int max_depth, int threads) {
-    return (size_t)(max_depth + 1) * 5 * sizeof(float4) * (size_t)threads;
+// Bytes of one subpath store: (max_depth + 1) vertices of 5 float4 per path slot.
+size_t bdpt_store_bytes(int max_depth, int64_t slots) {
+    return (size_t)(max_depth + 1) * 5 * sizeof(float4) * (size_t)slots;
+}
+
+// Path slots of a BDPT film: the two subpath stores stay within 2 GiB.
+int64_t bdpt_capacity(int max_depth, int64_t cap) {
+    int64_t c = ((int64_t)1 << 31) / (int64_t)(2 * bdpt_store_bytes(max_depth, 1));
+    c = std::max<int64_t>(c, (int64_t)1 << 16) & ~(int64_t)255;
+    return std::min(cap, c);
 }
 
 }  // namespace
@@ -720,12 +758,17 @@ int film_enqueue_chunk(prt_film* f) {
     PRT_HIP(hipEventRecord(f->ev0, st));
     if (f->bdpt) {
         int grid = (int)std::min<int64_t>(f->bdpt_blocks, (n + kBlock - 1) / kBlock);
-        if (sc.mixed)
-            hipLaunchKernelGGL((k_bdpt<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath, f->lpath,
-                               f->fb.splat, f->gstack, f->gslots);
-        else
-            hipLaunchKernelGGL((k_bdpt<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath, f->lpath,
-                               f->fb.splat, f->gstack, f->gslots);
+        if (sc.mixed) {
+            hipLaunchKernelGGL((k_bdpt_paths<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath,
+                               f->lens, f->gstack, f->gslots);
+            hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
+                               f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
+        } else {
+            hipLaunchKernelGGL((k_bdpt_paths<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath,
+                               f->lens, f->gstack, f->gslots);
+            hipLaunchKernelGGL((k_bdpt_connect<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
+                               f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
+        }
         hipLaunchKernelGGL(k_film, dim3((pp.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pp, f->pb, f->fb);
         PRT_HIP(hipGetLastError());
         PRT_HIP(hipEventRecord(f->ev1, st));
@@ -800,12 +843,12 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     int32_t w = p->width > 0 ? p->width : scene->image_width;
     int32_t h = p->height > 0 ? p->height : scene->image_height;
     int64_t npix = (int64_t)w * h;
-    int64_t C = film_capacity(npix, cap);
+    const bool bd = p->integrator == PRT_INTEGRATOR_BDPT;
+    int64_t C = film_capacity(npix, bd ? bdpt_capacity(p->max_depth, cap) : cap);
     int blocks = film_blocks(scene);
     size_t g = film_gstack_entries(scene) * (size_t)blocks * kBlock * sizeof(uint2);
     size_t b = 0;
-    if (p->integrator == PRT_INTEGRATOR_BDPT)
-        b = 2 * bdpt_store_bytes(p->max_depth, bdpt_blocks(scene, blocks) * kBlock) + 3 * sizeof(long long) * npix;
+    if (bd) b = 2 * bdpt_store_bytes(p->max_depth, C) + sizeof(uint32_t) * C + 3 * sizeof(long long) * npix;
     return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + b + 3 * sizeof(uint32_t) * (size_t)C;
 }
 
@@ -839,7 +882,8 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     f->width = w;
     f->height = h;
     f->npix = (int64_t)w * h;
-    f->capacity = film_capacity(f->npix, cap);
+    f->capacity = film_capacity(
+        f->npix, p->integrator == PRT_INTEGRATOR_BDPT ? bdpt_capacity(p->max_depth, cap) : cap);
     int rc;
     auto bail = [&](int code) {
         prt_film_destroy(f);
@@ -877,14 +921,13 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     if (p->integrator == PRT_INTEGRATOR_BDPT) {
         f->bdpt = true;
         f->bdpt_blocks = bdpt_blocks(scene, f->blocks);
-        const int threads = f->bdpt_blocks * kBlock;
-        const size_t vb = bdpt_store_bytes(p->max_depth, threads);
+        const size_t vb = bdpt_store_bytes(p->max_depth, C);
         if ((rc = film_alloc(f, vb, &ptr)) != PRT_OK) return bail(rc);
-        f->cpath.data = (float4*)ptr;
-        f->cpath.stride = (uint32_t)threads;
+        f->cpath = VertexStore{(float4*)ptr, (uint32_t)C, (uint32_t)(p->max_depth + 1)};
         if ((rc = film_alloc(f, vb, &ptr)) != PRT_OK) return bail(rc);
-        f->lpath.data = (float4*)ptr;
-        f->lpath.stride = (uint32_t)threads;
+        f->lpath = VertexStore{(float4*)ptr, (uint32_t)C, (uint32_t)(p->max_depth + 1)};
+        if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
+        f->lens = (uint32_t*)ptr;
         if ((rc = film_alloc(f, 3 * sizeof(long long) * (size_t)f->npix, &ptr)) != PRT_OK) return bail(rc);
         f->fb.splat = (long long*)ptr;
     }

@@ -43,10 +43,19 @@ struct Req {
     int readers = 0;
 };
 
-// Film slots of serving requests hold at most this many paths: a pass is enqueued chunk by chunk
-// (one chunk in flight per request), so the scheduler can give first passes the device between
-// chunks of longer renders.
+// Film slots of serving requests hold at most this many paths: a pass is enqueued chunk by chunk, so
+// the scheduler can give first passes the device between chunks of longer renders. A BDPT sample
+// costs several path-tracing samples, so its chunks are smaller (still one sample per thread of the
+// resident BDPT grid).
 constexpr int64_t kServeCapacity = (int64_t)1 << 19;
+constexpr int64_t kServeCapacityBdpt = (int64_t)1 << 17;
+// Chunks of later passes in flight on the device at once (over all requests). A new request's first
+// pass waits for at most these, whichever hardware queue its stream lands on.
+constexpr int kBackgroundChunks = 8;
+
+int64_t serve_capacity(const prt_render_params* p) {
+    return p->integrator == PRT_INTEGRATOR_BDPT ? kServeCapacityBdpt : kServeCapacity;
+}
 
 }  // namespace
 
@@ -65,6 +74,7 @@ struct prt_server {
     std::deque<prt_server_event> events;
     std::thread worker;
     bool stop = false;
+    size_t rr_next = 0;  // round-robin position of background dispatch
 
     double since(const Req* r) const {
         return std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
@@ -128,7 +138,7 @@ struct prt_server {
                 }
                 if (used + (int64_t)r->bytes > budget) break;  // wait for memory to free up
                 queue.pop_front();
-                int rc = prt::film_create_ex(r->scene, &r->params, kServeCapacity, &r->film);
+                int rc = prt::film_create_ex(r->scene, &r->params, serve_capacity(&r->params), &r->film);
                 if (rc != PRT_OK) {
                     r->state = St::Failed;
                     emit(r, PRT_EVENT_ERROR, 0, 0, rc, 0);
@@ -176,25 +186,45 @@ struct prt_server {
                 }
             }
             // ---- chunk dispatch ----
-            // Requests that still owe their first pass get the device first: while any is owed, other
-            // requests receive no new chunks (theirs in flight drain). Otherwise every request keeps
-            // two chunks queued on its stream so the device never waits for the scheduler.
+            // Requests that still owe their first pass get the device first: each keeps up to four
+            // chunks queued, and while any is owed the other requests receive no new chunks (theirs in
+            // flight drain). Otherwise requests take turns (round robin) to keep up to two chunks each
+            // queued, within kBackgroundChunks over all requests, so the device never waits for the
+            // scheduler and a newly admitted request never waits behind a long backlog.
             bool first_owed = false;
-            for (Req* r : active)
-                if (r->state == St::Running && r->completed_pass == 0) first_owed = true;
+            int background = 0;
             for (Req* r : active) {
                 if (r->state != St::Running) continue;
-                bool first = r->completed_pass == 0;
-                if (first_owed && !first) continue;
-                int depth = first ? 4 : 2;
-                while (r->in_flight < depth && !prt::film_plan_done(r->film)) {
-                    if (prt::film_enqueue_chunk(r->film) != PRT_OK) {
-                        r->state = St::Failed;
-                        emit(r, PRT_EVENT_ERROR, r->pass, r->target, PRT_ERR_DEVICE, 0);
-                        break;
+                if (r->completed_pass == 0) first_owed = true;
+                else background += r->in_flight;
+            }
+            auto enqueue = [&](Req* r) {
+                if (prt::film_enqueue_chunk(r->film) != PRT_OK) {
+                    r->state = St::Failed;
+                    emit(r, PRT_EVENT_ERROR, r->pass, r->target, PRT_ERR_DEVICE, 0);
+                    return false;
+                }
+                r->in_flight++;
+                progressed = true;
+                return true;
+            };
+            for (Req* r : active) {
+                if (r->state != St::Running || r->completed_pass != 0) continue;
+                while (r->in_flight < 4 && !prt::film_plan_done(r->film) && enqueue(r)) {
+                }
+            }
+            if (!first_owed && !active.empty()) {
+                const size_t n = active.size();
+                const size_t start = rr_next % n;
+                for (int round = 0; round < 2 && background < kBackgroundChunks; round++) {
+                    for (size_t j = 0; j < n && background < kBackgroundChunks; j++) {
+                        const size_t idx = (start + j) % n;
+                        Req* r = active[idx];
+                        if (r->state != St::Running || r->in_flight > round || prt::film_plan_done(r->film)) continue;
+                        if (!enqueue(r)) continue;
+                        background++;
+                        rr_next = idx + 1;
                     }
-                    r->in_flight++;
-                    progressed = true;
                 }
             }
             // ---- deferred releases ----
@@ -245,7 +275,7 @@ int prt_server_submit(prt_server* s, const prt_scene* scene, const prt_render_pa
     Req* r = new Req();
     r->scene = scene;
     r->params = *p;
-    r->bytes = film_device_bytes(scene, p, kServeCapacity);
+    r->bytes = film_device_bytes(scene, p, serve_capacity(p));
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
