@@ -39,7 +39,7 @@ struct prt_scene {
     int32_t num_cus = 0;
     int trace_variant[2] = {-1, -1};  // autotuned kernel variant (closest, any)
     int forced_variant = -1;          // PRT_TRACE_VARIANT override
-    int variant_blocks[8] = {0};
+    int variant_blocks[16] = {0};
     std::vector<uint32_t> orig_id;  // host copy: leaf order -> original primitive id
     // float64 camera/scene facts needed by the host-side renderer
     int32_t image_width = 0, image_height = 0;

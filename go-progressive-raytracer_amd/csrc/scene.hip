@@ -299,26 +299,130 @@ __global__ __launch_bounds__(BLOCK) void k_intersect_refill(DevScene sc, const p
     }
 }
 
-// Kernel variants the per-scene autotuner chooses from (identical per-ray traversal):
-// LDS stack entries per lane, refill threshold (64 = whole-wave batches), LDS node-cache size, block.
-#define PRT_TRACE_VARIANTS(X)   \
-    X(0, 16, 64, 0, 256)        \
-    X(1, 8, 64, 0, 256)         \
-    X(2, 8, 64, 64, 512)        \
-    X(3, 8, 64, 128, 512)       \
-    X(4, 6, 64, 64, 256)        \
-    X(5, 8, 32, 64, 512)
-constexpr int kNumVariants = 6;
+// Persistent closest/any-hit kernel with prefetched lane refill. Ray lengths are heavy-tailed
+// (most rays of a uniform distribution leave the root box at once, a few visit dozens of nodes), so
+// whole-wave batches idle most lanes. Here every lane holds the NEXT ray in registers, loaded while
+// the current one traverses: a lane that finishes starts its next ray in the same step with no wait,
+// and reloads its next-ray registers from the wave's ray chunk. Chunks of kPfChunk rays are reserved
+// one ahead with an atomic whose result is read only when the current chunk runs out, so neither the
+// ray loads nor the work counter add latency to the traversal steps.
+constexpr uint32_t kPfChunk = 256;
+
+template <bool ANY, bool MIXED, int LDS, int NCACHE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_intersect_pf(DevScene sc, const prt_ray* __restrict__ rays, int64_t n,
+                                                        prt_hit* __restrict__ hits, uint8_t* __restrict__ occluded,
+                                                        uint2* gstack, uint32_t gstride, unsigned long long* work) {
+    __shared__ uint32_t s_node[LDS * BLOCK];
+    __shared__ float s_t[LDS * BLOCK];
+    __shared__ float4 s_cache[NCACHE > 0 ? NCACHE * 8 : 1];
+    TStack<LDS, BLOCK> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * BLOCK + threadIdx.x;
+    st.gstride = gstride;
+    if (NCACHE > 0) {
+        load_node_cache<NCACHE, BLOCK>(sc, s_cache);
+        st.ncache = s_cache;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long N = (unsigned long long)n;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    // wave-uniform chunk state: current chunk [cpos, cend); the next chunk starts at nxt when
+    // have_next, else lane 0's pending atomic result holds it
+    unsigned long long pend = 0;
+    if (lane == 0) pend = atomicAdd(work, 2ull * kPfChunk);
+    unsigned long long b0 = __shfl(pend, 0, 64);
+    unsigned long long cpos = b0, cend = min(b0 + kPfChunk, N), nxt = b0 + kPfChunk;
+    bool have_next = true, exhausted = b0 >= N;
+    TState s;
+    bool active = false;
+    int64_t ridx = 0, nidx = -1;
+    float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nb = na;
+    for (;;) {
+        // start the prefetched ray on an idle lane
+        if (!active && nidx >= 0) {
+            tstate_init(s, TRay{na.x, na.y, na.z, na.w, nb.x, nb.y, nb.z, nb.w});
+            ridx = nidx;
+            nidx = -1;
+            active = true;
+        }
+        // hand out the next rays of the chunk to lanes without a prefetched ray
+        uint64_t need = __ballot(nidx < 0);
+        while (need != 0 && !exhausted) {
+            if (cpos >= cend) {
+                if (!have_next) nxt = __shfl(pend, 0, 64);
+                if (nxt >= N) {
+                    exhausted = true;
+                    break;
+                }
+                cpos = nxt;
+                cend = min(nxt + kPfChunk, N);
+                have_next = false;
+                if (lane == 0) pend = atomicAdd(work, (unsigned long long)kPfChunk);
+                continue;
+            }
+            const uint32_t avail = (uint32_t)(cend - cpos);
+            const uint32_t rank = __popcll(need & lt_mask);
+            const bool mine = ((need >> lane) & 1ull) != 0 && rank < avail;
+            if (mine) {
+                nidx = (int64_t)(cpos + rank);
+                const float4* rp = reinterpret_cast<const float4*>(rays + nidx);
+                na = rp[0];
+                nb = rp[1];
+            }
+            const uint32_t took = min((uint32_t)__popcll(need), avail);
+            cpos += took;
+            need = __ballot(nidx < 0);
+        }
+        if (__ballot(active || nidx >= 0) == 0) break;  // exhausted and drained
+        if (active && tstate_step<ANY, MIXED, false, LDS, BLOCK, NCACHE>(sc, s, st, nullptr)) {
+            active = false;
+            bool hit = s.best != 0xFFFFFFFFu;
+            if (ANY) {
+                occluded[ridx] = hit ? 1 : 0;
+            } else {
+                int32_t pid = hit ? (int32_t)__float_as_uint(sc.prims[3 * (size_t)s.best + 2].w) : -1;
+                reinterpret_cast<float4*>(hits)[ridx] =
+                    make_float4(hit ? s.tbest : __int_as_float(0x7f800000), __int_as_float(pid),
+                                hit ? s.bu : 0.0f, hit ? s.bv : 0.0f);
+            }
+        }
+    }
+}
+
+// Kernel variants the per-scene autotuner chooses from (identical per-ray traversal): kernel
+// (0 = batch / threshold refill, 1 = prefetched refill), LDS stack entries per lane, refill
+// threshold (64 = whole-wave batches), LDS node-cache size, block.
+#define PRT_TRACE_VARIANTS(X)      \
+    X(0, 0, 16, 64, 0, 256)        \
+    X(1, 0, 8, 64, 0, 256)         \
+    X(2, 0, 8, 64, 64, 512)        \
+    X(3, 0, 8, 64, 128, 512)       \
+    X(4, 0, 6, 64, 64, 256)        \
+    X(5, 0, 8, 32, 64, 512)        \
+    X(6, 1, 8, 1, 0, 256)          \
+    X(7, 1, 8, 1, 64, 256)         \
+    X(8, 1, 12, 1, 0, 256)         \
+    X(9, 1, 6, 1, 64, 512)
+constexpr int kNumVariants = 10;
 constexpr int kMinLds = 6;
-constexpr int kDefaultVariant = 1;
-constexpr int kVariantBlock[kNumVariants] = {256, 256, 512, 512, 256, 512};
+constexpr int kDefaultVariant = 7;
+constexpr int kVariantBlock[kNumVariants] = {256, 256, 512, 512, 256, 512, 256, 256, 256, 512};
+
+template <bool ANY, bool MIXED, int KIND, int LDS, int REFILL, int NCACHE, int BLOCK>
+const void* kernel_of() {
+    if (KIND == 1) return (const void*)k_intersect_pf<ANY, MIXED, LDS, NCACHE, BLOCK>;
+    return (const void*)k_intersect_refill<ANY, MIXED, LDS, REFILL, NCACHE, BLOCK>;
+}
 
 template <bool ANY, bool MIXED>
 const void* variant_fn(int v) {
     switch (v) {
-#define X(id, lds, refill, nc, blk) \
-    case id:                        \
-        return (const void*)k_intersect_refill<ANY, MIXED, lds, refill, nc, blk>;
+#define X(id, kind, lds, refill, nc, blk) \
+    case id:                              \
+        return kernel_of<ANY, MIXED, kind, lds, refill, nc, blk>();
         PRT_TRACE_VARIANTS(X)
 #undef X
     }
@@ -329,10 +433,14 @@ template <bool ANY, bool MIXED>
 void launch_variant(int v, int blocks, hipStream_t st, const DevScene& sc, const prt_ray* rays, int64_t n,
                     prt_hit* hits, uint8_t* occ, uint2* gs, uint32_t gstride, unsigned long long* work) {
     switch (v) {
-#define X(id, lds, refill, nc, blk)                                                                            \
+#define X(id, kind, lds, refill, nc, blk)                                                                     \
     case id:                                                                                                \
-        hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, lds, refill, nc, blk>), dim3(blocks), dim3(blk), 0, st, \
-                           sc, rays, n, hits, occ, gs, gstride, work);                                      \
+        if (kind == 1)                                                                                      \
+            hipLaunchKernelGGL((k_intersect_pf<ANY, MIXED, lds, nc, blk>), dim3(blocks), dim3(blk), 0, st, sc, \
+                               rays, n, hits, occ, gs, gstride, work);                                      \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, lds, refill, nc, blk>), dim3(blocks),          \
+                               dim3(blk), 0, st, sc, rays, n, hits, occ, gs, gstride, work);                \
         break;
         PRT_TRACE_VARIANTS(X)
 #undef X
