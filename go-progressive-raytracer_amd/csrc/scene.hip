@@ -303,9 +303,9 @@ __global__ __launch_bounds__(BLOCK) void k_intersect_refill(DevScene sc, const p
 // (most rays of a uniform distribution leave the root box at once, a few visit dozens of nodes), so
 // whole-wave batches idle most lanes. Here every lane holds the NEXT ray in registers, loaded while
 // the current one traverses: a lane that finishes starts its next ray in the same step with no wait,
-// and reloads its next-ray registers from the wave's ray chunk. Chunks of kPfChunk rays are reserved
-// one ahead with an atomic whose result is read only when the current chunk runs out, so neither the
-// ray loads nor the work counter add latency to the traversal steps.
+// and reloads its next-ray registers from the wave's ray chunk. Each wave's first chunk of kPfChunk
+// rays is static; later ones are reserved one ahead with an atomic whose result is read only when the
+// current chunk runs out, so neither the ray loads nor the work counter add latency to the steps.
 constexpr uint32_t kPfChunk = 256;
 
 template <bool ANY, bool MIXED, int LDS, int NCACHE, int BLOCK>
@@ -329,13 +329,16 @@ __global__ __launch_bounds__(BLOCK) void k_intersect_pf(DevScene sc, const prt_r
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long N = (unsigned long long)n;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    // wave-uniform chunk state: current chunk [cpos, cend); the next chunk starts at nxt when
-    // have_next, else lane 0's pending atomic result holds it
+    // wave-uniform chunk state: current chunk [cpos, cend); lane 0's pending atomic result locates
+    // the next one
+    // the first chunk of every wave is static (wave index); later chunks come from the work counter,
+    // offset by the static ones, reserved one ahead (no wave waits on an atomic to start)
+    const unsigned long long nwaves = (unsigned long long)gridDim.x * (BLOCK / 64);
+    const unsigned long long wid = (unsigned long long)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    unsigned long long cpos = wid * kPfChunk, cend = min(cpos + kPfChunk, N), nxt = 0;
+    bool exhausted = cpos >= N;
     unsigned long long pend = 0;
-    if (lane == 0) pend = atomicAdd(work, 2ull * kPfChunk);
-    unsigned long long b0 = __shfl(pend, 0, 64);
-    unsigned long long cpos = b0, cend = min(b0 + kPfChunk, N), nxt = b0 + kPfChunk;
-    bool have_next = true, exhausted = b0 >= N;
+    if (!exhausted && lane == 0) pend = atomicAdd(work, (unsigned long long)kPfChunk);
     TState s;
     bool active = false;
     int64_t ridx = 0, nidx = -1;
@@ -352,14 +355,13 @@ __global__ __launch_bounds__(BLOCK) void k_intersect_pf(DevScene sc, const prt_r
         uint64_t need = __ballot(nidx < 0);
         while (need != 0 && !exhausted) {
             if (cpos >= cend) {
-                if (!have_next) nxt = __shfl(pend, 0, 64);
+                nxt = nwaves * kPfChunk + __shfl(pend, 0, 64);
                 if (nxt >= N) {
                     exhausted = true;
                     break;
                 }
                 cpos = nxt;
                 cend = min(nxt + kPfChunk, N);
-                have_next = false;
                 if (lane == 0) pend = atomicAdd(work, (unsigned long long)kPfChunk);
                 continue;
             }
