@@ -109,8 +109,10 @@ def main():
     nodes, boxes, prims = scene.traversal_counts(rays[: 1 << 16])
     bytes_per_ray = 48.0 + 32.0 * boxes.mean() + 36.0 * prims.mean()
     achieved = bytes_per_ray * a.rays / (avg_ms * 1e-3) / 1e9
+    variant = int(scene.info().trace_variant)
+    traffic = profiled_traffic(variant, a.rays)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_ray": round(bytes_per_ray, 1), "box_tests_per_ray": round(float(boxes.mean()), 2),
                 "tri_tests_per_ray": round(float(prims.mean()), 2),
                 "kernel_ms": round(avg_ms, 3)}
@@ -161,6 +163,22 @@ def main():
         "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
+
+
+def profiled_traffic(variant, rays):
+    """HBM bytes per launch of this traversal variant at this ray count, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel); None if none was profiled."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", "closest_hit_c1_icosphere1.31M_16Mrays.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic: variant=(\d+) rays=(\d+) fetch_bytes=(\d+) write_bytes=(\d+)", text)
+    if not m or int(m.group(1)) != variant or int(m.group(2)) != rays:
+        return None
+    return int(m.group(3)) + int(m.group(4))
 
 
 def pt_section(a, log):

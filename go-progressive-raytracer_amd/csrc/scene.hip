@@ -398,20 +398,15 @@ __global__ __launch_bounds__(BLOCK) void k_intersect_pf(DevScene sc, const prt_r
 // (0 = batch / threshold refill, 1 = prefetched refill), LDS stack entries per lane, refill
 // threshold (64 = whole-wave batches), LDS node-cache size, block.
 #define PRT_TRACE_VARIANTS(X)      \
-    X(0, 0, 16, 64, 0, 256)        \
-    X(1, 0, 8, 64, 0, 256)         \
-    X(2, 0, 8, 64, 64, 512)        \
-    X(3, 0, 8, 64, 128, 512)       \
-    X(4, 0, 6, 64, 64, 256)        \
-    X(5, 0, 8, 32, 64, 512)        \
-    X(6, 1, 8, 1, 0, 256)          \
-    X(7, 1, 8, 1, 64, 256)         \
-    X(8, 1, 12, 1, 0, 256)         \
-    X(9, 1, 6, 1, 64, 512)
-constexpr int kNumVariants = 10;
+    X(0, 0, 8, 64, 0, 256)         \
+    X(1, 1, 8, 1, 0, 256)          \
+    X(2, 1, 8, 1, 64, 256)         \
+    X(3, 1, 12, 1, 0, 256)         \
+    X(4, 1, 6, 1, 64, 512)
+constexpr int kNumVariants = 5;
 constexpr int kMinLds = 6;
-constexpr int kDefaultVariant = 7;
-constexpr int kVariantBlock[kNumVariants] = {256, 256, 512, 512, 256, 512, 256, 256, 256, 512};
+constexpr int kDefaultVariant = 2;
+constexpr int kVariantBlock[kNumVariants] = {256, 256, 256, 256, 512};
 
 template <bool ANY, bool MIXED, int KIND, int LDS, int REFILL, int NCACHE, int BLOCK>
 const void* kernel_of() {
@@ -529,7 +524,7 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
                                nullptr, nullptr, (uint32_t*)d_out0, d_c1, d_c2, gs, gstride, work);
     } else {
         int& tuned = ms->trace_variant[mode];
-        if (tuned < 0 && n >= (1 << 20)) {
+        if (ms->forced_variant < 0 && tuned < 0 && n >= (1 << 20)) {
             // autotune: time every variant on the first 1M rays of this batch, keep the fastest
             hipEvent_t a, b;
             (void)hipEventCreate(&a);
@@ -549,8 +544,10 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
                     if (rep > 0) tot += ms_v;
                 }
                 if (e != hipSuccess) break;
-                if (tot < best) {
-                    best = tot;
+                // another variant replaces the default only if it is clearly (3 %) faster
+                float score = v == kDefaultVariant ? tot : tot * 1.03f;
+                if (score < best) {
+                    best = score;
                     bestv = v;
                 }
             }
@@ -558,7 +555,7 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
             (void)hipEventDestroy(b);
             if (e != hipSuccess) { cleanup(); return hip_fail(e, "intersect autotune"); }
             tuned = bestv;
-            ms->info.trace_variant = bestv;
+            if (mode == 0) ms->info.trace_variant = bestv;
         }
         int v = ms->forced_variant >= 0 ? ms->forced_variant : (tuned >= 0 ? tuned : kDefaultVariant);
         e = run(v, n);
