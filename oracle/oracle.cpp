@@ -2123,6 +2123,93 @@ void orc_ray_color(const orc_scene* os, const prt_render_params* p, const double
     out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 // Pass schedule, getSamplesForPass progressive.go:98-123
+int32_t orc_samples_for_pass(int32_t initial, int32_t max_spp, int32_t max_passes, int32_t pass);
+
+// shouldStopSampling, tile_renderer.go:77-101: no stop before max(1, int(target * AdaptiveMinSamples))
+// samples; then stop when the luminance coefficient of variation is below AdaptiveThreshold, or for a
+// (near) black pixel when the variance is below 1e-6.
+static bool should_stop(double ls, double lq, int32_t count, int32_t target, double amin, double thr) {
+    int32_t min_samples = std::max<int32_t>(1, (int32_t)((double)target * amin));
+    if (count < min_samples) return false;
+    double mean = ls / count, mean_sq = lq / count;
+    double var = std::max(0.0, mean_sq - mean * mean);
+    if (mean <= 1e-8) return var < 1e-6;
+    return std::sqrt(var) / mean < thr;
+}
+
+// RenderPass for passes first_pass..last_pass (progressive.go:126-223) with the per-pixel adaptive loop
+// of adaptiveSamplePixelWithSplats (tile_renderer.go:50-73): while count < target and not stopped, take
+// the sample with index = count. Per-pixel state lives in the caller's arrays (full image: sum_rgb
+// W*H*3, lum_sum, lum_sq, counts; splat_rgb W*H*3 for BDPT) and carries over between calls.
+int orc_render_passes(const orc_scene* os, const prt_render_params* p, int32_t first_pass, int32_t last_pass,
+                      int32_t nthreads, double* sum_rgb, double* lum_sum, double* lum_sq, int32_t* counts,
+                      double* splat_rgb) {
+    const Scene& S = os->s;
+    const bool bdpt = p->integrator == PRT_INTEGRATOR_BDPT;
+    if (!bdpt && p->integrator != PRT_INTEGRATOR_PATH) return PRT_ERR_UNSUPPORTED;
+    PTConfig cfg;
+    cfg.max_depth = p->max_depth;
+    cfg.rr_min = p->rr_min_bounces;
+    BDPTCtx bc{S, p->max_depth};
+    const int W = S.cam.width, H = S.cam.height;
+    const size_t full = (size_t)W * H * 3;
+    const int nt_ = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    for (int32_t pass = first_pass; pass <= last_pass; pass++) {
+        const int32_t target = orc_samples_for_pass(p->initial_samples, p->max_samples_per_pixel, p->max_passes, pass);
+        std::atomic<int64_t> next{0};
+        std::vector<std::vector<double>> tsplat(bdpt && splat_rgb ? nt_ : 0, std::vector<double>(full, 0.0));
+        auto worker = [&](int wid) {
+            std::vector<Splat> splats;
+            for (;;) {
+                int64_t k0 = next.fetch_add(256);
+                if (k0 >= (int64_t)W * H) break;
+                for (int64_t k = k0; k < std::min<int64_t>(k0 + 256, (int64_t)W * H); k++) {
+                    int i = (int)(k % W), j = (int)(k / W);
+                    while (counts[k] < target && !should_stop(lum_sum[k], lum_sq[k], counts[k], target,
+                                                              p->adaptive_min_samples, p->adaptive_threshold)) {
+                        Sampler smp;
+                        smp.key = path_key(p->seed, (uint32_t)(j * W + i), (uint32_t)counts[k]);
+                        smp.set_slot(0);
+                        Vec2 a = smp.get2d();
+                        Vec2 b = smp.get2d();
+                        Ray r = S.cam.get_ray(i, j, a, b);
+                        Vec3 c;
+                        if (bdpt) {
+                            splats.clear();
+                            c = bdpt_ray_color(bc, r, smp, splats);
+                            if (!tsplat.empty())
+                                for (auto& sp : splats) {
+                                    int sx, sy;
+                                    if (cam_map_ray_to_pixel(S.cam, sp.ray, sx, sy)) {
+                                        double* o = &tsplat[wid][((size_t)sy * W + sx) * 3];
+                                        o[0] += sp.color.x;
+                                        o[1] += sp.color.y;
+                                        o[2] += sp.color.z;
+                                    }
+                                }
+                        } else {
+                            c = ray_color(S, cfg, r, smp, cfg.max_depth, Vec3(1, 1, 1));
+                        }
+                        sum_rgb[3 * k] += c.x;
+                        sum_rgb[3 * k + 1] += c.y;
+                        sum_rgb[3 * k + 2] += c.z;
+                        double l = c.luminance();
+                        lum_sum[k] += l;
+                        lum_sq[k] += l * l;
+                        counts[k]++;
+                    }
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nt_; t++) pool.emplace_back(worker, t);
+        for (auto& t : pool) t.join();
+        for (auto& b : tsplat)  // processSplats after the pass's tiles (progressive.go:195)
+            for (size_t q = 0; q < full; q++) splat_rgb[q] += b[q];
+    }
+    return PRT_OK;
+}
+
 int32_t orc_samples_for_pass(int32_t initial, int32_t max_spp, int32_t max_passes, int32_t pass) {
     if (max_passes == 1) return max_spp;
     if (pass == 1) return initial;

@@ -35,6 +35,7 @@ def lib():
             "orc_intersect": (None, [vp, vp, i64, vp, vp, vp, vp, i32, i32, vp]),
             "orc_render": (C.c_int, [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32, vp]),
             "orc_render_samples": (C.c_int, [vp, vp, vp, vp, i64, vp, i32, vp]),
+            "orc_render_passes": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp]),
             "orc_bdpt_mis": (d, [vp, vp, vp, i32, vp, i32, vp, i32, i32]),
             "orc_camera_ray_pdfs": (None, [vp, vp, vp, vp, vp, vp, vp]),
             "orc_draw": (d, [u64, u32, u32, u32]),
@@ -139,6 +140,20 @@ class OracleScene:
         if splats:
             return s, ls, lq, sp
         return s, ls, lq
+
+    def new_film(self):
+        """Per-pixel state of orc_render_passes (sums, luminance moments, counts, splats)."""
+        W, H = self.desc.image_size
+        return {"sum": np.zeros((H, W, 3)), "lum": np.zeros((H, W)), "lum2": np.zeros((H, W)),
+                "count": np.zeros((H, W), np.int32), "splat": np.zeros((H, W, 3))}
+
+    def render_passes(self, params, first_pass, last_pass, film=None, nthreads=0):
+        """RenderPass for passes first..last with the adaptive per-pixel stop; updates and returns `film`."""
+        film = film if film is not None else self.new_film()
+        rc = lib().orc_render_passes(self.h, C.byref(params), first_pass, last_pass, nthreads, _p(film["sum"]),
+                                     _p(film["lum"]), _p(film["lum2"]), _p(film["count"]), _p(film["splat"]))
+        assert rc == 0, rc
+        return film
 
     def render_samples(self, params, pixels, samples, nthreads=0, splats=False):
         pixels = np.ascontiguousarray(pixels, dtype=np.int32).reshape(-1, 2)
