@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <atomic>
 #include <vector>
 
 #include "../../include/prt.h"
@@ -30,6 +31,7 @@ struct DevBuf {
 
 struct prt_scene {
     int device = 0;
+    std::atomic<int> users{0};  // films and queued serving requests that read this scene
     prt::DevScene dev{};
     prt_scene_info info{};
     std::vector<prt::DevBuf> bufs;
@@ -61,7 +63,10 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
 size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int64_t capacity_cap);
 int film_enqueue(prt_film* f, int32_t spp);  // enqueue spp more samples per pixel, no wait
 int film_plan(prt_film* f, int32_t spp);     // split spp more samples into chunks (not enqueued)
-bool film_plan_done(const prt_film* f);      // every planned chunk has been enqueued
+int film_plan_pass(prt_film* f, int32_t target);  // plan the pass to `target` spp (uniform or adaptive)
+bool film_round_exhausted(const prt_film* f);     // every chunk planned so far has been enqueued
+bool film_plan_done(const prt_film* f);      // the pass needs no more chunks
+int film_advance(prt_film* f);               // adaptive films, when idle: plan the next round or end the pass
 int film_enqueue_chunk(prt_film* f);         // enqueue the next planned chunk, no wait
 double film_chunk_ms(prt_film* f);           // accumulate the finished chunk's device time
 bool film_idle(prt_film* f);                 // all enqueued work finished

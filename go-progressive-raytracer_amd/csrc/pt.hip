@@ -48,12 +48,27 @@ struct PathBuffers {
 
 struct PassParams {
     int32_t width, height;
-    int32_t px0, npix;      // pixel range of this chunk (linear pixel ids)
+    int32_t px0, npix;      // pixel range of this chunk (linear pixel ids; active-list range if act)
     int32_t spp;            // samples per pixel in this chunk
-    int32_t sample_base;    // first sample index
+    int32_t sample_base;    // first sample index (uniform films)
     int32_t max_depth, rr_min;
     uint64_t seed;
+    // adaptive rounds: pixels come from an active list and each pixel's next sample index is its count
+    const uint32_t* act;
+    const uint32_t* count;
 };
+
+// Pixel and sample index of path slot i of a chunk (slot = sample-major over the chunk's pixels).
+__device__ __forceinline__ void slot_sample(const PassParams& pp, uint32_t i, uint32_t& pix, uint32_t& sample) {
+    const uint32_t s_local = i / (uint32_t)pp.npix, a = i % (uint32_t)pp.npix;
+    if (pp.act) {
+        pix = pp.act[pp.px0 + a];
+        sample = pp.count[pix] + s_local;
+    } else {
+        pix = (uint32_t)pp.px0 + a;
+        sample = (uint32_t)pp.sample_base + s_local;
+    }
+}
 
 // wave-aggregated append to a queue
 __device__ __forceinline__ void queue_push(uint32_t* q, uint32_t* counter, uint32_t value, bool active) {
@@ -86,10 +101,10 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, P
     uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
     bool active = false;
     if (slot < n) {
-        uint32_t s_local = slot / (uint32_t)pp.npix;
-        uint32_t pix = (uint32_t)pp.px0 + slot % (uint32_t)pp.npix;
+        uint32_t pix, sample;
+        slot_sample(pp, slot, pix, sample);
         int i = (int)(pix % (uint32_t)pp.width), j = (int)(pix / (uint32_t)pp.width);
-        uint32_t key = path_key(pp.seed, pix, (uint32_t)pp.sample_base + s_local);
+        uint32_t key = path_key(pp.seed, pix, sample);
         // GetRay(i, j, Get2D(), Get2D()) (tile_renderer.go:56)
         float l0 = draw(key, 0), l1 = draw(key, 1), j0 = draw(key, 2), j1 = draw(key, 3);
         V3 o, d;
@@ -357,10 +372,10 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_paths(DevScene sc, PassParams p
     const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        uint32_t s_local = i / (uint32_t)pp.npix;
-        uint32_t pix = (uint32_t)pp.px0 + i % (uint32_t)pp.npix;
+        uint32_t pix, sample;
+        slot_sample(pp, i, pix, sample);
         int px = (int)(pix % (uint32_t)pp.width), py = (int)(pix / (uint32_t)pp.width);
-        uint32_t key = path_key(pp.seed, pix, (uint32_t)pp.sample_base + s_local);
+        uint32_t key = path_key(pp.seed, pix, sample);
         V3 o, d;
         camera_ray(cam, px, py, draw(key, 0), draw(key, 1), draw(key, 2), draw(key, 3), o, d);
         // camera subpath (generateCameraPath, bdpt.go:106-135)
@@ -439,9 +454,9 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(DevScene sc, PassParams
     const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        uint32_t s_local = i / (uint32_t)pp.npix;
-        uint32_t pix = (uint32_t)pp.px0 + i % (uint32_t)pp.npix;
-        uint32_t key = path_key(pp.seed, pix, (uint32_t)pp.sample_base + s_local);
+        uint32_t pix, sample;
+        slot_sample(pp, i, pix, sample);
+        uint32_t key = path_key(pp.seed, pix, sample);
         const uint32_t len = lens[i];
         const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
         V3 total = v3(0, 0, 0);
@@ -614,6 +629,98 @@ __global__ __launch_bounds__(kBlock) void k_film(PassParams pp, PathBuffers pb, 
     fb.count[px] += (uint32_t)pp.spp;
 }
 
+// ---- adaptive sampling (adaptiveSamplePixelWithSplats / shouldStopSampling, tile_renderer.go:50-101) ----
+// A pass runs in rounds over a list of active pixels. Each round gives every active pixel k samples with
+// sample indices count, count+1, ...; the film kernel replays the reference's per-sample loop: before each
+// sample it applies the stop test and discards the pixel's remaining samples of the round once it stops.
+// A sample's value depends only on (pixel, sample index), so the result equals the sequential loop.
+struct AdaptiveParams {
+    int32_t target;       // samples per pixel of the pass
+    int32_t min_samples;  // max(1, int(target * AdaptiveMinSamples))
+    double threshold;     // AdaptiveThreshold
+    uint32_t* next;       // active list of the next round
+    uint32_t* state;      // [0]: next list length, [1]: min count, [2]: max count of the next list
+};
+
+// shouldStopSampling (tile_renderer.go:77-101) on the Kahan-compensated float32 moments, in float64.
+__device__ __forceinline__ bool adaptive_stop(const AdaptiveParams& ap, uint32_t cnt, double ls, double lq) {
+    if ((int32_t)cnt < ap.min_samples) return false;
+    const double mean = ls / (double)cnt, mean_sq = lq / (double)cnt;
+    const double var = fmax(0.0, mean_sq - mean * mean);
+    if (mean <= 1e-8) return var < 1e-6;
+    return sqrt(var) / mean < ap.threshold;
+}
+
+__device__ __forceinline__ void adaptive_keep(const AdaptiveParams& ap, uint32_t px, uint32_t cnt, bool keep) {
+    uint64_t mask = __ballot(keep);
+    if (mask == 0) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ap.state[0], (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (keep) {
+        ap.next[base + __popcll(mask & ((1ull << lane) - 1ull))] = px;
+        atomicMin(&ap.state[1], cnt);
+        atomicMax(&ap.state[2], cnt);
+    }
+}
+
+// Start of a pass: every pixel that has not reached the target and is not stopped is active.
+__global__ __launch_bounds__(kBlock) void k_adaptive_select(int32_t npix, FilmBuffers fb, AdaptiveParams ap) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    bool keep = false;
+    uint32_t cnt = 0;
+    if (p < npix) {
+        cnt = fb.count[p];
+        const float4 S = fb.sum[p], C = fb.comp[p];
+        const float2 Q = fb.sq[p];
+        keep = (int32_t)cnt < ap.target &&
+               !adaptive_stop(ap, cnt, (double)S.w - (double)C.w, (double)Q.x - (double)Q.y);
+    }
+    adaptive_keep(ap, (uint32_t)p, cnt, keep);
+}
+
+// Per active pixel of the chunk: the reference loop over the round's k samples, each accepted sample
+// added to the film with Kahan compensation; pixels still active join the next round's list.
+__global__ __launch_bounds__(kBlock) void k_film_adaptive(PassParams pp, PathBuffers pb, FilmBuffers fb,
+                                                          AdaptiveParams ap) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    bool keep = false;
+    uint32_t px = 0, cnt = 0;
+    if (p < pp.npix) {
+        px = pp.act[pp.px0 + p];
+        cnt = fb.count[px];
+        float4 S = fb.sum[px], C = fb.comp[px];
+        float2 Q = fb.sq[px];
+        bool stopped = false;
+        for (int s = 0; s < pp.spp; s++) {
+            if ((int32_t)cnt >= ap.target ||
+                adaptive_stop(ap, cnt, (double)S.w - (double)C.w, (double)Q.x - (double)Q.y)) {
+                stopped = true;
+                break;
+            }
+            const float4 v = pb.L[(size_t)s * pp.npix + p];
+            const float lv = 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z;
+            kahan(S.x, C.x, v.x);
+            kahan(S.y, C.y, v.y);
+            kahan(S.z, C.z, v.z);
+            kahan(S.w, C.w, lv);
+            kahan(Q.x, Q.y, lv * lv);
+            cnt++;
+        }
+        if (!stopped)
+            stopped = (int32_t)cnt >= ap.target ||
+                      adaptive_stop(ap, cnt, (double)S.w - (double)C.w, (double)Q.x - (double)Q.y);
+        fb.sum[px] = S;
+        fb.comp[px] = C;
+        fb.sq[px] = Q;
+        fb.count[px] = cnt;
+        keep = !stopped;
+    }
+    adaptive_keep(ap, px, cnt, keep);
+}
+
 }  // namespace prt
 
 using namespace prt;
@@ -647,6 +754,15 @@ struct prt_film {
     std::vector<Chunk> plan;
     size_t plan_pos = 0;
     double plan_ms = 0;
+    // adaptive passes (AdaptiveMinSamples < 1): rounds over an active-pixel list
+    bool adaptive = false;
+    bool ad_running = false;   // a pass is in progress
+    bool ad_selected = false;  // its first active list has been built
+    int32_t ad_target = 0, ad_min = 0;
+    int ad_cur = 0;            // index of the current list
+    uint32_t* act[2] = {nullptr, nullptr};
+    uint32_t* ad_state = nullptr;  // per list: length, min count, max count
+    uint32_t ad_n = 0;             // length of the current list
 };
 
 namespace prt {
@@ -734,11 +850,99 @@ int film_plan(prt_film* f, int32_t spp) {
     return PRT_OK;
 }
 
-bool film_plan_done(const prt_film* f) { return f->plan_pos >= f->plan.size(); }
+bool film_round_exhausted(const prt_film* f) { return f->plan_pos >= f->plan.size(); }
+
+bool film_plan_done(const prt_film* f) { return f->adaptive ? !f->ad_running : film_round_exhausted(f); }
+
+namespace {
+__global__ void k_adaptive_reset(uint32_t* st) {
+    st[0] = 0;
+    st[1] = 0xFFFFFFFFu;
+    st[2] = 0;
+}
+}  // namespace
+
+// Plans the pass that brings every pixel to `target` samples per pixel (RenderPass). Uniform films
+// split the missing samples into chunks; adaptive films start a sequence of rounds (film_advance).
+int film_plan_pass(prt_film* f, int32_t target) {
+    if (!f->adaptive) {
+        int32_t need = target - f->spp_done;
+        return film_plan(f, need > 0 ? need : 0);
+    }
+    f->plan.clear();
+    f->plan_pos = 0;
+    f->plan_ms = 0;
+    f->ad_running = true;
+    f->ad_selected = false;
+    f->ad_target = target;
+    f->ad_min = std::max<int32_t>(1, (int32_t)((double)target * f->params.adaptive_min_samples));
+    f->spp_done = std::max(f->spp_done, target);
+    return PRT_OK;
+}
+
+// Adaptive films: when the enqueued rounds have finished (the film is idle), reads the active list the
+// last round (or the pass start) left and plans the next round's chunks, or ends the pass. Each round
+// gives every active pixel k samples: the mandatory samples up to the minimum count plus, for path
+// tracing, an eighth of the pass's remaining budget that the film kernel may discard; BDPT takes no
+// speculative samples (a discarded sample's splats would already be on the film).
+int film_advance(prt_film* f) {
+    if (!f->adaptive || !f->ad_running || !film_round_exhausted(f)) return PRT_OK;
+    PRT_HIP(hipSetDevice(f->scene->device));
+    hipStream_t st = f->stream;
+    FilmBuffers fb = f->fb;
+    AdaptiveParams ap;
+    ap.target = f->ad_target;
+    ap.min_samples = f->ad_min;
+    ap.threshold = f->params.adaptive_threshold;
+    if (!f->ad_selected) {
+        f->ad_cur = 0;
+        ap.next = f->act[0];
+        ap.state = f->ad_state;
+        hipLaunchKernelGGL(k_adaptive_reset, dim3(1), dim3(1), 0, st, ap.state);
+        hipLaunchKernelGGL(k_adaptive_select, dim3((int)((f->npix + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                           (int32_t)f->npix, fb, ap);
+        PRT_HIP(hipGetLastError());
+        f->ad_selected = true;
+    } else {
+        f->ad_cur ^= 1;  // the list the last round wrote
+    }
+    uint32_t h[3] = {0, 0, 0};
+    PRT_HIP(hipMemcpyAsync(h, f->ad_state + 3 * f->ad_cur, sizeof(h), hipMemcpyDeviceToHost, st));
+    PRT_HIP(hipStreamSynchronize(st));
+    f->ad_n = h[0];
+    f->plan.clear();
+    f->plan_pos = 0;
+    if (f->ad_n == 0) {
+        f->ad_running = false;
+        return PRT_OK;
+    }
+    const int32_t mincnt = (int32_t)h[1], maxcnt = (int32_t)h[2];
+    int64_t k;
+    if (f->bdpt) {
+        k = maxcnt < f->ad_min ? f->ad_min - maxcnt : 1;
+    } else {
+        const int64_t spec = std::max<int64_t>(1, (f->ad_target - f->ad_min + 7) / 8);
+        k = (mincnt < f->ad_min ? f->ad_min - mincnt : 0) + spec;
+    }
+    k = std::max<int64_t>(1, std::min<int64_t>({k, (int64_t)f->ad_target - mincnt, f->capacity}));
+    const int64_t per_chunk = std::max<int64_t>(1, f->capacity / k);
+    for (int64_t a0 = 0; a0 < (int64_t)f->ad_n; a0 += per_chunk) {
+        prt_film::Chunk c;
+        c.px0 = (int32_t)a0;
+        c.npix = (int32_t)std::min<int64_t>(per_chunk, (int64_t)f->ad_n - a0);
+        c.spp = (int32_t)k;
+        c.sample_base = 0;
+        f->plan.push_back(c);
+    }
+    // the next list starts empty
+    hipLaunchKernelGGL(k_adaptive_reset, dim3(1), dim3(1), 0, st, f->ad_state + 3 * (f->ad_cur ^ 1));
+    PRT_HIP(hipGetLastError());
+    return PRT_OK;
+}
 
 // Enqueues the next planned chunk on the film's stream (no wait); ev0/ev1 bracket it.
 int film_enqueue_chunk(prt_film* f) {
-    if (film_plan_done(f)) return PRT_OK;
+    if (film_round_exhausted(f)) return PRT_OK;
     const prt_scene* s = f->scene;
     PRT_HIP(hipSetDevice(s->device));
     const DevScene& sc = f->sc;
@@ -754,6 +958,16 @@ int film_enqueue_chunk(prt_film* f) {
     pp.max_depth = f->params.max_depth;
     pp.rr_min = f->params.rr_min_bounces;
     pp.seed = f->params.seed;
+    pp.act = f->adaptive ? f->act[f->ad_cur] : nullptr;
+    pp.count = f->fb.count;
+    AdaptiveParams ap{};
+    if (f->adaptive) {
+        ap.target = f->ad_target;
+        ap.min_samples = f->ad_min;
+        ap.threshold = f->params.adaptive_threshold;
+        ap.next = f->act[f->ad_cur ^ 1];
+        ap.state = f->ad_state + 3 * (f->ad_cur ^ 1);
+    }
     int64_t n = (int64_t)pp.npix * pp.spp;
     PRT_HIP(hipEventRecord(f->ev0, st));
     if (f->bdpt) {
@@ -769,7 +983,12 @@ int film_enqueue_chunk(prt_film* f) {
             hipLaunchKernelGGL((k_bdpt_connect<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
         }
-        hipLaunchKernelGGL(k_film, dim3((pp.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pp, f->pb, f->fb);
+        if (f->adaptive)
+            hipLaunchKernelGGL(k_film_adaptive, dim3((pp.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pp,
+                               f->pb, f->fb, ap);
+        else
+            hipLaunchKernelGGL(k_film, dim3((pp.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pp, f->pb,
+                               f->fb);
         PRT_HIP(hipGetLastError());
         PRT_HIP(hipEventRecord(f->ev1, st));
         return PRT_OK;
@@ -793,7 +1012,10 @@ int film_enqueue_chunk(prt_film* f) {
                                f->gslots);
     }
     int film_blocks_n = (pp.npix + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_film, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb);
+    if (f->adaptive)
+        hipLaunchKernelGGL(k_film_adaptive, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb, ap);
+    else
+        hipLaunchKernelGGL(k_film, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb);
     PRT_HIP(hipGetLastError());
     PRT_HIP(hipEventRecord(f->ev1, st));
     return PRT_OK;
@@ -813,7 +1035,7 @@ int film_enqueue(prt_film* f, int32_t spp) {
     hipEvent_t first = nullptr;
     PRT_HIP(hipEventCreate(&first));
     (void)hipEventRecord(first, f->stream);
-    while (!film_plan_done(f)) {
+    while (!film_round_exhausted(f)) {
         rc = film_enqueue_chunk(f);
         if (rc != PRT_OK) {
             (void)hipEventDestroy(first);
@@ -849,6 +1071,7 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     size_t g = film_gstack_entries(scene) * (size_t)blocks * kBlock * sizeof(uint2);
     size_t b = 0;
     if (bd) b = 2 * bdpt_store_bytes(p->max_depth, C) + sizeof(uint32_t) * C + 3 * sizeof(long long) * npix;
+    if (p->adaptive_min_samples < 1.0) b += 2 * sizeof(uint32_t) * npix + 6 * sizeof(uint32_t);
     return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + b + 3 * sizeof(uint32_t) * (size_t)C;
 }
 
@@ -862,12 +1085,15 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
         return fail(PRT_ERR_UNSUPPORTED, "BDPT max_depth above 64");
     if (p->max_passes < 1 || p->max_samples_per_pixel < 1 || p->initial_samples < 0)
         return fail(PRT_ERR_INVALID_ARGUMENT, "bad pass schedule");
+    if (!std::isfinite(p->adaptive_min_samples) || !std::isfinite(p->adaptive_threshold))
+        return fail(PRT_ERR_INVALID_ARGUMENT, "adaptive sampling parameters must be finite");
     int32_t w = p->width > 0 ? p->width : scene->image_width;
     int32_t h = p->height > 0 ? p->height : scene->image_height;
     if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 28)) return fail(PRT_ERR_INVALID_ARGUMENT, "bad image size");
     PRT_HIP(hipSetDevice(scene->device));
     prt_film* f = new prt_film();
     f->scene = scene;
+    const_cast<prt_scene*>(scene)->users++;
     f->sc = scene->dev;
     if (w != scene->image_width || h != scene->image_height) {
         // image-size override: camera width and aspect from the requested size (as the web server's
@@ -931,6 +1157,15 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
         if ((rc = film_alloc(f, 3 * sizeof(long long) * (size_t)f->npix, &ptr)) != PRT_OK) return bail(rc);
         f->fb.splat = (long long*)ptr;
     }
+    if (p->adaptive_min_samples < 1.0) {
+        f->adaptive = true;
+        for (int l = 0; l < 2; l++) {
+            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)f->npix, &ptr)) != PRT_OK) return bail(rc);
+            f->act[l] = (uint32_t*)ptr;
+        }
+        if ((rc = film_alloc(f, 6 * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
+        f->ad_state = (uint32_t*)ptr;
+    }
     hipError_t e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
     e = hipEventCreate(&f->ev0);
@@ -951,6 +1186,50 @@ int render_samples(prt_film* f, int32_t spp, float* ms) {
     if (rc != PRT_OK) return rc;
     return film_wait(f, ms);
 }
+
+// One RenderPass to `target` samples per pixel; adaptive films run their rounds to the end.
+int render_to(prt_film* f, int32_t target, float* ms) {
+    if (!f->adaptive) {
+        int32_t need = target - f->spp_done;
+        if (need <= 0) return PRT_OK;
+        return render_samples(f, need, ms);
+    }
+    PRT_HIP(hipSetDevice(f->scene->device));
+    hipEvent_t start = nullptr;
+    PRT_HIP(hipEventCreate(&start));
+    int rc = PRT_OK;
+    if (hipEventRecord(start, f->stream) != hipSuccess) rc = fail(PRT_ERR_DEVICE, "hipEventRecord");
+    if (rc == PRT_OK) rc = film_plan_pass(f, target);
+    while (rc == PRT_OK && !film_plan_done(f)) {
+        rc = film_advance(f);  // waits for the enqueued round
+        while (rc == PRT_OK && !film_round_exhausted(f)) rc = film_enqueue_chunk(f);
+    }
+    if (rc == PRT_OK && hipEventRecord(f->ev1, f->stream) != hipSuccess) rc = fail(PRT_ERR_DEVICE, "hipEventRecord");
+    if (rc == PRT_OK && hipEventSynchronize(f->ev1) != hipSuccess) rc = fail(PRT_ERR_DEVICE, "hipEventSynchronize");
+    if (rc == PRT_OK && ms) {
+        float t = 0;
+        (void)hipEventElapsedTime(&t, start, f->ev1);
+        *ms = t;
+    }
+    (void)hipEventDestroy(start);
+    return rc;
+}
+
+// RenderStats from the per-pixel counts (assembleCurrentImage, progressive.go:395-431).
+int count_stats(const prt_film* f, int64_t& total, int32_t& mn, int32_t& mx) {
+    std::vector<uint32_t> cnt(f->npix);
+    PRT_HIP(hipMemcpyAsync(cnt.data(), f->fb.count, sizeof(uint32_t) * f->npix, hipMemcpyDeviceToHost, f->stream));
+    PRT_HIP(hipStreamSynchronize(f->stream));
+    total = 0;
+    mn = f->params.max_samples_per_pixel;
+    mx = 0;
+    for (uint32_t c : cnt) {
+        total += c;
+        mn = std::min<int32_t>(mn, (int32_t)c);
+        mx = std::max<int32_t>(mx, (int32_t)c);
+    }
+    return PRT_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -969,6 +1248,10 @@ int prt_film_reset(prt_film* f) {
     if (f->fb.splat) PRT_HIP(hipMemsetAsync(f->fb.splat, 0, 3 * sizeof(long long) * f->npix, f->stream));
     PRT_HIP(hipStreamSynchronize(f->stream));
     f->spp_done = 0;
+    f->plan.clear();
+    f->plan_pos = 0;
+    f->ad_running = false;
+    f->ad_selected = false;
     return PRT_OK;
 }
 
@@ -980,6 +1263,7 @@ int prt_film_destroy(prt_film* f) {
     if (f->ev0) (void)hipEventDestroy(f->ev0);
     if (f->ev1) (void)hipEventDestroy(f->ev1);
     if (f->stream) (void)hipStreamDestroy(f->stream);
+    const_cast<prt_scene*>(f->scene)->users--;
     delete f;
     return PRT_OK;
 }
@@ -999,6 +1283,8 @@ static void fill_stats(const prt_film* f, int32_t target, float ms, prt_pass_sta
 int prt_render_samples(prt_film* f, int32_t spp, prt_pass_stats* st) {
     if (!f) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL film");
     if (spp < 0) return fail(PRT_ERR_INVALID_ARGUMENT, "negative spp");
+    if (f->adaptive)
+        return fail(PRT_ERR_UNSUPPORTED, "adaptive films (AdaptiveMinSamples < 1) render by pass (prt_render_pass)");
     float ms = 0;
     if (spp > 0) {
         int rc = render_samples(f, spp, &ms);
@@ -1014,12 +1300,18 @@ int prt_render_pass(prt_film* f, int32_t pass, prt_pass_stats* st) {
     int rc = prt_samples_for_pass(&f->params, pass, &target);
     if (rc != PRT_OK) return rc;
     float ms = 0;
-    int32_t need = target - f->spp_done;
-    if (need > 0) {
-        rc = render_samples(f, need, &ms);
-        if (rc != PRT_OK) return rc;
-    }
+    rc = render_to(f, target, &ms);
+    if (rc != PRT_OK) return rc;
     fill_stats(f, target, ms, st);
+    if (st && f->adaptive) {
+        int64_t total = 0;
+        int32_t mn = 0, mx = 0;
+        if ((rc = count_stats(f, total, mn, mx)) != PRT_OK) return rc;
+        st->total_samples = total;
+        st->average_samples = (double)total / (double)f->npix;
+        st->min_samples = mn;
+        st->max_samples_used = mx;
+    }
     return PRT_OK;
 }
 

@@ -953,6 +953,10 @@ int prt_scene_info_get(const prt_scene* scene, prt_scene_info* info) {
 
 int prt_scene_destroy(prt_scene* scene) {
     if (!scene) return PRT_OK;
+    if (int n = scene->users.load()) {
+        return fail(PRT_ERR_INVALID_ARGUMENT,
+                    "scene is still used by " + std::to_string(n) + " film(s) or serving request(s)");
+    }
     (void)hipSetDevice(scene->device);
     for (auto& b : scene->bufs) (void)hipFree(b.p);
     delete scene;

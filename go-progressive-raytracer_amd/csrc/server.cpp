@@ -104,10 +104,9 @@ struct prt_server {
         int32_t target = 0;
         int rc = prt_samples_for_pass(&r->params, r->pass + 1, &target);
         if (rc == PRT_OK) {
-            int32_t need = target - prt::film_spp_done(r->film);
             r->pass++;
             r->target = target;
-            rc = prt::film_plan(r->film, need > 0 ? need : 0);
+            rc = prt::film_plan_pass(r->film, target);
         }
         if (rc != PRT_OK) {
             r->state = St::Failed;
@@ -160,6 +159,16 @@ struct prt_server {
                     double ms = r->in_flight ? prt::film_chunk_ms(r->film) : 0.0;
                     if (r->in_flight) progressed = true;
                     r->in_flight = 0;
+                    // adaptive passes: the finished round decides the next one (film idle, quick read)
+                    if (prt::film_round_exhausted(r->film) && !prt::film_plan_done(r->film)) {
+                        if (prt::film_advance(r->film) != PRT_OK) {
+                            r->state = St::Failed;
+                            emit(r, PRT_EVENT_ERROR, r->pass, r->target, PRT_ERR_DEVICE, 0);
+                            active.erase(active.begin() + k);
+                            continue;
+                        }
+                        progressed = true;
+                    }
                     if (prt::film_plan_done(r->film) && r->pass > r->completed_pass) {
                         r->completed_pass = r->pass;
                         emit(r, PRT_EVENT_PASS_COMPLETE, r->pass, r->target, 0, ms);
@@ -210,7 +219,7 @@ struct prt_server {
             };
             for (Req* r : active) {
                 if (r->state != St::Running || r->completed_pass != 0) continue;
-                while (r->in_flight < 4 && !prt::film_plan_done(r->film) && enqueue(r)) {
+                while (r->in_flight < 4 && !prt::film_round_exhausted(r->film) && enqueue(r)) {
                 }
             }
             if (!first_owed && !active.empty()) {
@@ -220,7 +229,8 @@ struct prt_server {
                     for (size_t j = 0; j < n && background < kBackgroundChunks; j++) {
                         const size_t idx = (start + j) % n;
                         Req* r = active[idx];
-                        if (r->state != St::Running || r->in_flight > round || prt::film_plan_done(r->film)) continue;
+                        if (r->state != St::Running || r->in_flight > round || prt::film_round_exhausted(r->film))
+                            continue;
                         if (!enqueue(r)) continue;
                         background++;
                         rr_next = idx + 1;
@@ -234,6 +244,7 @@ struct prt_server {
                 if (r->release && finished && r->readers == 0) {
                     if (r->film) (void)prt::film_wait(r->film, nullptr);
                     free_film(r);
+                    const_cast<prt_scene*>(r->scene)->users--;
                     delete r;
                     it = reqs.erase(it);
                     progressed = true;
@@ -274,6 +285,7 @@ int prt_server_submit(prt_server* s, const prt_scene* scene, const prt_render_pa
         return fail(PRT_ERR_INVALID_ARGUMENT, "unknown integrator");
     Req* r = new Req();
     r->scene = scene;
+    const_cast<prt_scene*>(scene)->users++;  // until the request is released
     r->params = *p;
     r->bytes = film_device_bytes(scene, p, serve_capacity(p));
     r->t_submit = Clock::now();
@@ -375,6 +387,7 @@ int prt_server_destroy(prt_server* s) {
         Req* r = kv.second;
         if (r->film) (void)prt::film_wait(r->film, nullptr);
         s->free_film(r);
+        const_cast<prt_scene*>(r->scene)->users--;
         delete r;
     }
     delete s;

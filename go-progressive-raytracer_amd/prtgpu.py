@@ -387,7 +387,7 @@ class Scene:
 
     def close(self):
         if getattr(self, "handle", None):
-            lib().prt_scene_destroy(self.handle)
+            check(lib().prt_scene_destroy(self.handle))  # fails while a film or request still uses it
             self.handle = None
 
     def __del__(self):
@@ -499,10 +499,12 @@ class Server:
         h = C.c_void_p()
         check(lib().prt_server_create(device, hbm_budget_bytes, max_active, C.byref(h)))
         self.handle = h
+        self._scenes = {}  # request id -> Scene, kept alive until the request is released
 
     def submit(self, scene, params):
         rid = C.c_int64()
         check(lib().prt_server_submit(self.handle, scene.handle, C.byref(params), C.byref(rid)))
+        self._scenes[rid.value] = scene
         return rid.value
 
     def poll(self, max_events=256, timeout_ms=0):
@@ -537,6 +539,7 @@ class Server:
         if getattr(self, "handle", None):
             lib().prt_server_destroy(self.handle)
             self.handle = None
+            self._scenes = {}
 
     def __del__(self):
         try:

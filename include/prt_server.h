@@ -53,7 +53,8 @@ typedef struct prt_server prt_server;
 /* hbm_budget_bytes bounds the memory of all active film slots; max_active bounds concurrent requests. */
 int prt_server_create(int32_t device, int64_t hbm_budget_bytes, int32_t max_active, prt_server** out);
 /* Queues a progressive render of `scene` (params: integrator, width/height override, depth, RR, pass
- * schedule, seed). The scene must outlive the request. */
+ * schedule, seed). The scene must outlive the request: prt_scene_destroy fails while a request that
+ * has not been released (or a film) still uses it. */
 int prt_server_submit(prt_server* server, const prt_scene* scene, const prt_render_params* params,
                       int64_t* request_id);
 /* Copies up to max_events pending events; waits up to timeout_ms for the first one (0: no wait). */
