@@ -175,10 +175,10 @@ def profiled_traffic(variant, rays):
         text = open(path).read()
     except OSError:
         return None
-    m = re.search(r"traffic: variant=(\d+) rays=(\d+) fetch_bytes=(\d+) write_bytes=(\d+)", text)
-    if not m or int(m.group(1)) != variant or int(m.group(2)) != rays:
-        return None
-    return int(m.group(3)) + int(m.group(4))
+    for m in re.finditer(r"traffic: variant=(\d+) rays=(\d+) fetch_bytes=(\d+) write_bytes=(\d+)", text):
+        if int(m.group(1)) == variant and int(m.group(2)) == rays:
+            return int(m.group(3)) + int(m.group(4))
+    return None
 
 
 def pt_section(a, log):

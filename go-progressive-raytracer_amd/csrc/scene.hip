@@ -525,16 +525,16 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
     } else {
         int& tuned = ms->trace_variant[mode];
         if (ms->forced_variant < 0 && tuned < 0 && n >= (1 << 20)) {
-            // autotune: time every variant on the first 1M rays of this batch, keep the fastest
+            // autotune: time every variant on the first (up to) 4M rays of this batch, keep the fastest
             hipEvent_t a, b;
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             float best = 1e30f;
             int bestv = kDefaultVariant;
-            const int64_t m = 1 << 20;
+            const int64_t m = std::min<int64_t>(n, (int64_t)1 << 22);
             for (int v = 0; v < kNumVariants; v++) {
                 float tot = 0;
-                for (int rep = 0; rep < 3; rep++) {
+                for (int rep = 0; rep < 4; rep++) {
                     (void)hipEventRecord(a, st);
                     e = run(v, m);
                     (void)hipEventRecord(b, st);
