@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -43,7 +44,7 @@ struct PathBuffers {
     uint32_t* pixel;
     uint32_t* queue[2];
     uint32_t* squeue;
-    uint32_t* counters;  // [0],[1]: path queues, [2]: shadow queue
+    uint32_t* counters;  // queue lengths, one per 256-B line: path queues 0 and 1, shadow queue 2
 };
 
 struct PassParams {
@@ -68,6 +69,38 @@ __device__ __forceinline__ void slot_sample(const PassParams& pp, uint32_t i, ui
         pix = (uint32_t)pp.px0 + a;
         sample = (uint32_t)pp.sample_base + s_local;
     }
+}
+
+// Queue length q lives at counters[q * kCounterStride] (separate L2 lines, so the queues' atomics
+// do not serialize on one line).
+constexpr uint32_t kCounterStride = 64;
+__device__ __forceinline__ uint32_t* qlen(const PathBuffers& pb, uint32_t q) { return pb.counters + q * kCounterStride; }
+
+// Block-aggregated appends to two queues: the block's waves reserve their ranges with LDS atomics and
+// one thread per block reserves the block's range of each queue with one global atomic. Every thread
+// of the block must call it (it synchronizes the block); s_lds holds 4 words.
+__device__ __forceinline__ void block_push2(uint32_t* s_lds, uint32_t* q0, uint32_t* c0, bool a0, uint32_t* q1,
+                                            uint32_t* c1, bool a1, uint32_t value) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    if (threadIdx.x < 2) s_lds[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t m0 = __ballot(a0), m1 = __ballot(a1);
+    uint32_t w0 = 0, w1 = 0;
+    if (lane == 0) {
+        if (m0) w0 = atomicAdd(&s_lds[0], (uint32_t)__popcll(m0));
+        if (m1) w1 = atomicAdd(&s_lds[1], (uint32_t)__popcll(m1));
+    }
+    w0 = __shfl(w0, 0, 64);
+    w1 = __shfl(w1, 0, 64);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_lds[2] = s_lds[0] ? atomicAdd(c0, s_lds[0]) : 0u;
+        s_lds[3] = s_lds[1] ? atomicAdd(c1, s_lds[1]) : 0u;
+    }
+    __syncthreads();
+    if (a0) q0[s_lds[2] + w0 + (uint32_t)__popcll(m0 & lt)] = value;
+    if (a1) q1[s_lds[3] + w1 + (uint32_t)__popcll(m1 & lt)] = value;
 }
 
 // wave-aggregated append to a queue
@@ -119,7 +152,13 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, P
         pb.key[slot] = key;
         pb.pixel[slot] = pix;
     }
-    queue_push(pb.queue[0], &pb.counters[0], slot, active);
+    if (pp.rr_min > 0 && pp.max_depth > 0) {
+        // Russian roulette cannot end a path at level 0: every slot is active, the queue is the identity
+        if (slot < n) pb.queue[0][slot] = slot;
+        if (slot == 0) *qlen(pb, 0) = n;
+    } else {
+        queue_push(pb.queue[0], qlen(pb, 0), slot, active);
+    }
 }
 
 template <bool ANY, bool MIXED>
@@ -138,11 +177,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
     load_node_cache<kNodeCache, kBlock>(sc, s_cache);
     st.ncache = s_cache;
     const uint32_t cur = (uint32_t)bounce & 1u;
-    const uint32_t count = ANY ? pb.counters[2] : pb.counters[cur];
+    const uint32_t count = ANY ? *qlen(pb, 2) : *qlen(pb, cur);
     const uint32_t* q = ANY ? pb.squeue : pb.queue[cur];
     if (!ANY && blockIdx.x == 0 && threadIdx.x == 0) {
-        pb.counters[cur ^ 1u] = 0;  // next bounce's queue (its last reader finished earlier)
-        pb.counters[2] = 0;         // shadow queue of this bounce
+        *qlen(pb, cur ^ 1u) = 0;  // next bounce's queue (its last reader finished earlier)
+        *qlen(pb, 2) = 0;         // shadow queue of this bounce
     }
     const uint32_t stride = gridDim.x * kBlock;
     // uniform trip count per wave so every lane reaches the loop exit together
@@ -182,13 +221,15 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, PathBuffers pb, int b) {
+    __shared__ uint32_t s_push[4];
     const uint32_t cur = (uint32_t)b & 1u;
-    const uint32_t count = pb.counters[cur];
+    const uint32_t count = *qlen(pb, cur);
     const uint32_t* q = pb.queue[cur];
     const uint32_t stride = gridDim.x * kBlock;
     const float kInfF = __int_as_float(0x7f800000);
-    for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < count; base += stride) {
-        uint32_t i = base + (threadIdx.x & 63u);
+    // block-uniform trip count (the queue pushes synchronize the block)
+    for (uint32_t base = blockIdx.x * kBlock; base < count; base += stride) {
+        uint32_t i = base + threadIdx.x;
         bool cont = false, shadow = false;
         uint32_t slot = 0;
         if (i < count) {
@@ -277,8 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
             }
             (void)kInfF;
         }
-        queue_push(pb.queue[cur ^ 1u], &pb.counters[cur ^ 1u], slot, cont);
-        queue_push(pb.squeue, &pb.counters[2], slot, shadow);
+        block_push2(s_push, pb.queue[cur ^ 1u], qlen(pb, cur ^ 1u), cont, pb.squeue, qlen(pb, 2), shadow, slot);
     }
 }
 
@@ -993,23 +1033,22 @@ int film_enqueue_chunk(prt_film* f) {
         PRT_HIP(hipEventRecord(f->ev1, st));
         return PRT_OK;
     }
-    PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 4 * sizeof(uint32_t), st));
+    PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 3 * kCounterStride * sizeof(uint32_t), st));
     int cam_blocks = (int)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_camera, dim3(cam_blocks), dim3(kBlock), 0, st, sc, pp, f->pb);
+    auto trace = [&](bool any, int b) {
+        auto fn = any ? (sc.mixed ? (const void*)k_trace<true, true> : (const void*)k_trace<true, false>)
+                      : (sc.mixed ? (const void*)k_trace<false, true> : (const void*)k_trace<false, false>);
+        PathBuffers pbv = f->pb;
+        uint2* gs = f->gstack;
+        uint32_t gsl = f->gslots;
+        void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b, (void*)&gs, (void*)&gsl};
+        return hipLaunchKernel(fn, dim3(f->blocks), dim3(kBlock), args, 0, st);
+    };
     for (int b = 0; b < pp.max_depth; b++) {
-        if (sc.mixed)
-            hipLaunchKernelGGL((k_trace<false, true>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b, f->gstack,
-                               f->gslots);
-        else
-            hipLaunchKernelGGL((k_trace<false, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b, f->gstack,
-                               f->gslots);
+        PRT_HIP(trace(false, b));
         hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
-        if (sc.mixed)
-            hipLaunchKernelGGL((k_trace<true, true>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b, f->gstack,
-                               f->gslots);
-        else
-            hipLaunchKernelGGL((k_trace<true, false>), dim3(f->blocks), dim3(kBlock), 0, st, sc, f->pb, b, f->gstack,
-                               f->gslots);
+        PRT_HIP(trace(true, b));
     }
     int film_blocks_n = (pp.npix + kBlock - 1) / kBlock;
     if (f->adaptive)
@@ -1134,7 +1173,7 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     FALLOC(f->pb.queue[0], uint32_t, C);
     FALLOC(f->pb.queue[1], uint32_t, C);
     FALLOC(f->pb.squeue, uint32_t, C);
-    FALLOC(f->pb.counters, uint32_t, 4);
+    FALLOC(f->pb.counters, uint32_t, 3 * kCounterStride);
     FALLOC(f->fb.sum, float4, f->npix);
     FALLOC(f->fb.comp, float4, f->npix);
     FALLOC(f->fb.sq, float2, f->npix);
