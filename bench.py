@@ -31,6 +31,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="skip the path-tracing throughput section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (time-to-first-pass) section")
+    ap.add_argument("--pt-config", choices=["c0", "c2", "c3", "c2bdpt"], default=None,
+                    help="run only this render configuration in the path-tracing section")
     ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-bdpt", type=int, default=1, help="1: half of the served requests use BDPT")
     ap.add_argument("--serve-window-s", type=float, default=60.0,
@@ -203,7 +205,10 @@ def pt_section(a, log):
     ]
     out = []
     cores = min(os.cpu_count() or 1, 16)
-    for name, make, integ, spp_override, c_spp in configs:
+    tags = ["c0", "c2", "c3", "c2bdpt"]
+    for tag, (name, make, integ, spp_override, c_spp) in zip(tags, configs):
+        if a.pt_config and tag != a.pt_config:
+            continue
         desc, cfg = make()
         w, h = desc.image_size
         spp = spp_override or cfg["spp"]
