@@ -111,6 +111,7 @@ struct DevScene {
     const Node4* nodes;
     const float4* prims;
     const QuadExt* quads;
+    const float4* tri_normals;  // custom triangle normals; a triangle's side index k > 0 selects entry k
     const DevMaterial* mats;
     const DevLight* lights;
     const float* light_weights;  // normalised selection probabilities

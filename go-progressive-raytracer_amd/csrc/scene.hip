@@ -668,8 +668,41 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     }
     bool world_any = false;
     int64_t gid = 0;
+    std::vector<float4> trin(1, make_float4(0.f, 0.f, 0.f, 0.f));  // custom triangle normals, 1-based
+    std::vector<double> rotated;
     for (int i = 0; i < d->num_meshes; i++) {
         const prt_mesh& m = d->meshes[i];
+        const double* verts = m.vertices;
+        if (m.rotation && m.num_vertices > 0) {  // NewTriangleMesh :57-70 with rotateVertex :170-199
+            rotated.assign(m.vertices, m.vertices + 3 * m.num_vertices);
+            const double* rt = m.rotation;
+            for (int64_t k = 0; k < m.num_vertices; k++) {
+                double* v = &rotated[3 * k];
+                if (m.center)
+                    for (int a = 0; a < 3; a++) v[a] -= m.center[a];
+                if (rt[0] != 0) {
+                    double c = std::cos(rt[0]), sn = std::sin(rt[0]);
+                    double y = v[1] * c - v[2] * sn, z = v[1] * sn + v[2] * c;
+                    v[1] = y;
+                    v[2] = z;
+                }
+                if (rt[1] != 0) {
+                    double c = std::cos(rt[1]), sn = std::sin(rt[1]);
+                    double x = v[0] * c + v[2] * sn, z = -v[0] * sn + v[2] * c;
+                    v[0] = x;
+                    v[2] = z;
+                }
+                if (rt[2] != 0) {
+                    double c = std::cos(rt[2]), sn = std::sin(rt[2]);
+                    double x = v[0] * c - v[1] * sn, y = v[0] * sn + v[1] * c;
+                    v[0] = x;
+                    v[1] = y;
+                }
+                if (m.center)
+                    for (int a = 0; a < 3; a++) v[a] += m.center[a];
+            }
+            verts = rotated.data();
+        }
         PrimBounds mb;
         for (int a = 0; a < 3; a++) {
             mb.lo[a] = INFINITY;
@@ -682,16 +715,25 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
                 return bail(fail(PRT_ERR_INVALID_ARGUMENT, "face index out of bounds"));
             int mat = m.face_materials ? m.face_materials[f] : m.material;
             if (bad_mat(mat)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "face material out of range"));
-            const double* p0 = m.vertices + 3 * fi[0];
-            const double* p1 = m.vertices + 3 * fi[1];
-            const double* p2 = m.vertices + 3 * fi[2];
+            const double* p0 = verts + 3 * fi[0];
+            const double* p1 = verts + 3 * fi[1];
+            const double* p2 = verts + 3 * fi[2];
             PrimBounds& b = bounds[gid];
             for (int a = 0; a < 3; a++) {
                 b.lo[a] = std::min(p0[a], std::min(p1[a], p2[a]));
                 b.hi[a] = std::max(p0[a], std::max(p1[a], p2[a]));
             }
             grow(mb, b);
-            rec[3 * gid + 0] = f4(p0[0], p0[1], p0[2], bits(kPrimTriangle << 30));
+            uint32_t side = 0;
+            if (m.normals) {  // NewTriangleWithNormal (triangle.go:35-49): normalized custom normal
+                const double* cn = m.normals + 3 * f;
+                double l = std::sqrt(cn[0] * cn[0] + cn[1] * cn[1] + cn[2] * cn[2]);
+                double inv = l > 0 ? 1.0 / l : 0.0;
+                side = (uint32_t)trin.size();
+                if (side >= (1u << 30)) return bail(fail(PRT_ERR_UNSUPPORTED, "too many custom normals"));
+                trin.push_back(make_float4((float)(cn[0] * inv), (float)(cn[1] * inv), (float)(cn[2] * inv), 0.f));
+            }
+            rec[3 * gid + 0] = f4(p0[0], p0[1], p0[2], bits((kPrimTriangle << 30) | side));
             rec[3 * gid + 1] = f4(p1[0], p1[1], p1[2], bits((uint32_t)mat));
             rec[3 * gid + 2] = f4(p2[0], p2[1], p2[2], bits((uint32_t)gid));
         }
@@ -777,6 +819,8 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     s->dev.prims = (const float4*)p;
     if ((rc = scene_alloc(s, qext.size() * sizeof(QuadExt), qext.data(), &p)) != PRT_OK) return bail(rc);
     s->dev.quads = (const QuadExt*)p;
+    if ((rc = scene_alloc(s, trin.size() * sizeof(float4), trin.data(), &p)) != PRT_OK) return bail(rc);
+    s->dev.tri_normals = (const float4*)p;
     s->dev.mixed = (nsph + nq) > 0 ? 1 : 0;
     s->dev.max_stack = bvh.max_stack;
     s->dev.num_nodes = (int32_t)bvh.nodes.size();

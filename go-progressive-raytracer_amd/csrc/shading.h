@@ -103,8 +103,10 @@ __device__ __forceinline__ Surf make_surf(const DevScene& sc, V3 o, V3 d, const 
     if (kind == kPrimTriangle) {
         float4 c = pp[2];
         V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
-        outward = normalize(cross(v1 - v0, v2 - v0));  // triangle.go:92-96
-        s.p = s.p - outward * dot(s.p - v0, outward);
+        V3 gn = normalize(cross(v1 - v0, v2 - v0));  // triangle.go:92-96
+        s.p = s.p - gn * dot(s.p - v0, gn);
+        const uint32_t side = prim_side(tag);
+        outward = side ? v3(sc.tri_normals[side].x, sc.tri_normals[side].y, sc.tri_normals[side].z) : gn;
     } else if (kind == kPrimSphere) {
         V3 c = v3(a.x, a.y, a.z);
         V3 r = s.p - c;

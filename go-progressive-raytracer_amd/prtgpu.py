@@ -55,7 +55,9 @@ class Box(C.Structure):
 class Mesh(C.Structure):
     _fields_ = [("vertices", C.POINTER(C.c_double)), ("num_vertices", C.c_int64),
                 ("faces", C.POINTER(C.c_int32)), ("num_faces", C.c_int64),
-                ("face_materials", C.POINTER(C.c_int32)), ("material", C.c_int32), ("pad", C.c_int32)]
+                ("face_materials", C.POINTER(C.c_int32)), ("material", C.c_int32), ("pad", C.c_int32),
+                ("normals", C.POINTER(C.c_double)), ("rotation", C.POINTER(C.c_double)),
+                ("center", C.POINTER(C.c_double))]
 
 
 class Light(C.Structure):
@@ -110,12 +112,18 @@ EXPORTS = [
     "prt_scene_destroy", "prt_intersect_closest", "prt_intersect_any", "prt_traversal_counts",
     "prt_samples_for_pass", "prt_film_create", "prt_render_pass", "prt_render_samples", "prt_film_read",
     "prt_film_read_splats", "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy",
-    "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky",
+    "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky", "prt_ply_load", "prt_ply_free",
     "prt_server_create", "prt_server_submit", "prt_server_poll", "prt_server_cancel", "prt_server_read",
     "prt_server_read_rgba8", "prt_server_release", "prt_server_stats_get", "prt_server_destroy",
 ]
 
 EVENT_ADMITTED, EVENT_PASS_COMPLETE, EVENT_DONE, EVENT_ERROR, EVENT_CANCELLED = range(5)
+
+
+class PlyData(C.Structure):
+    _fields_ = [("num_vertices", C.c_int64), ("num_faces", C.c_int64), ("vertices", C.POINTER(C.c_double)),
+                ("faces", C.POINTER(C.c_int32)), ("normals", C.POINTER(C.c_double)),
+                ("colors", C.POINTER(C.c_double)), ("texcoords", C.POINTER(C.c_double))]
 
 
 class ServerEvent(C.Structure):
@@ -163,6 +171,8 @@ def lib():
                                              C.POINTER(C.c_int64), vp, C.POINTER(C.c_int64)]),
         "prt_workload_rays": (C.c_int, [vp, vp, C.c_double, C.c_int64, C.c_uint64, vp]),
         "prt_workload_sky": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.c_double, vp]),
+        "prt_ply_load": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(PlyData))]),
+        "prt_ply_free": (None, [C.POINTER(PlyData)]),
         "prt_server_create": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(vp)]),
         "prt_server_submit": (C.c_int, [vp, vp, C.POINTER(RenderParams), C.POINTER(C.c_int64)]),
         "prt_server_poll": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
@@ -190,6 +200,30 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
+def load_ply(path):
+    """LoadPLY (pkg/loaders/ply.go:64-110): dict of numpy arrays (vertices (n,3), faces (m,3), and
+    normals / colors (n,3), texcoords (n,2) when the file has them)."""
+    h = C.POINTER(PlyData)()
+    check(lib().prt_ply_load(os.fsencode(path), C.byref(h)))
+    try:
+        d = h.contents
+        nv, nf = d.num_vertices, d.num_faces
+
+        def arr(ptr, n, k, dt):
+            if not ptr:
+                return None
+            return np.ctypeslib.as_array(ptr, shape=(n * k,)).astype(dt).reshape(n, k) if n else np.zeros((0, k), dt)
+
+        out = {"vertices": arr(d.vertices, nv, 3, np.float64), "faces": arr(d.faces, nf, 3, np.int32)}
+        for name, k in (("normals", 3), ("colors", 3), ("texcoords", 2)):
+            a = arr(getattr(d, name), nv, k, np.float64)
+            if a is not None:
+                out[name] = a
+        return out
+    finally:
+        lib().prt_ply_free(h)
+
+
 def device_count():
     n = C.c_int32(0)
     check(lib().prt_device_count(C.byref(n)))
@@ -210,7 +244,7 @@ class SceneDesc:
         self.spheres = []
         self.quads = []
         self.boxes = []
-        self.meshes = []  # (vertices float64 (n,3), faces int32 (m,3), material, face_materials|None)
+        self.meshes = []  # (vertices (n,3), faces (m,3), material, face_materials|None, options dict)
         self.lights = []
         self.light_weights = None
         self.camera = None
@@ -261,11 +295,15 @@ class SceneDesc:
         self.boxes.append(b)
         return len(self.boxes) - 1
 
-    def mesh(self, vertices, faces, material, face_materials=None):
+    def mesh(self, vertices, faces, material, face_materials=None, normals=None, rotation=None, center=None):
+        """NewTriangleMesh (triangle_mesh.go:33-146) with TriangleMeshOptions: per-triangle normals,
+        rotation (radians, X then Y then Z) about `center`."""
         v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
         f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
         fm = None if face_materials is None else np.ascontiguousarray(face_materials, dtype=np.int32)
-        self.meshes.append((v, f, material, fm))
+        opt = {k: (None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(-1))
+               for k, x in (("normals", normals), ("rotation", rotation), ("center", center))}
+        self.meshes.append((v, f, material, fm, opt))
         return len(self.meshes) - 1
 
     # lights
@@ -326,8 +364,11 @@ class SceneDesc:
             return C.cast(a, C.POINTER(ctype))
 
         meshes = []
-        for v, f, m, fm in self.meshes:
+        for v, f, m, fm, opt in self.meshes:
             M = Mesh()
+            for k, x in opt.items():
+                setattr(M, k, x.ctypes.data_as(C.POINTER(C.c_double)) if x is not None else None)
+                keep.append(x)
             M.vertices = v.ctypes.data_as(C.POINTER(C.c_double))
             M.num_vertices = v.shape[0]
             M.faces = f.ctypes.data_as(C.POINTER(C.c_int32))

@@ -117,6 +117,10 @@ typedef struct prt_mesh { /* pkg/geometry/triangle_mesh.go:33-146; primitive id 
     const int32_t* face_materials; /* optional, num_faces entries */
     int32_t material;
     int32_t pad;
+    /* TriangleMeshOptions (triangle_mesh.go:20-26), all optional: */
+    const double* normals;         /* num_faces * 3 custom triangle normals (normalized on use) */
+    const double* rotation;        /* 3 radians, applied X then Y then Z (rotateVertex :170-199) */
+    const double* center;          /* 3, rotation centre (default the origin) */
 } prt_mesh;
 
 /* Light types: pkg/lights/{quad_light,sphere_light,uniform_infinite_light,

@@ -463,15 +463,41 @@ struct Mesh : Shape {
     std::vector<std::unique_ptr<Triangle>> tris;
     std::unique_ptr<BVH> bvh;
     AABB box;
+    // rotateVertex, triangle_mesh.go:170-199: X, then Y, then Z
+    static Vec3 rotate(Vec3 v, Vec3 rot) {
+        if (rot.x != 0) {
+            double c = std::cos(rot.x), s = std::sin(rot.x);
+            v = Vec3(v.x, v.y * c - v.z * s, v.y * s + v.z * c);
+        }
+        if (rot.y != 0) {
+            double c = std::cos(rot.y), s = std::sin(rot.y);
+            v = Vec3(v.x * c + v.z * s, v.y, -v.x * s + v.z * c);
+        }
+        if (rot.z != 0) {
+            double c = std::cos(rot.z), s = std::sin(rot.z);
+            v = Vec3(v.x * c - v.y * s, v.x * s + v.y * c, v.z);
+        }
+        return v;
+    }
     Mesh(const prt_mesh& m, int64_t first_id) {
         tris.reserve(m.num_faces);
         std::vector<const Shape*> sh;
         sh.reserve(m.num_faces);
+        std::vector<Vec3> verts(m.num_vertices);
+        for (int64_t k = 0; k < m.num_vertices; k++) {  // NewTriangleMesh :57-70
+            Vec3 v = V(m.vertices + 3 * k);
+            if (m.rotation) {
+                if (m.center) v = v - V(m.center);
+                v = rotate(v, V(m.rotation));
+                if (m.center) v = v + V(m.center);
+            }
+            verts[k] = v;
+        }
         for (int64_t i = 0; i < m.num_faces; i++) {
             const int32_t* f = m.faces + 3 * i;
             int mat = m.face_materials ? m.face_materials[i] : m.material;
-            tris.emplace_back(new Triangle(V(m.vertices + 3 * f[0]), V(m.vertices + 3 * f[1]),
-                                           V(m.vertices + 3 * f[2]), mat, first_id + i));
+            tris.emplace_back(new Triangle(verts[f[0]], verts[f[1]], verts[f[2]], mat, first_id + i));
+            if (m.normals) tris.back()->normal = V(m.normals + 3 * i).normalize();  // NewTriangleWithNormal :35-49
             sh.push_back(tris.back().get());
         }
         bvh.reset(new BVH(sh));
