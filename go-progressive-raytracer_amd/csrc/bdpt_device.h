@@ -68,9 +68,12 @@ struct VertexStore {
 };
 __device__ __forceinline__ bool rec_specular(float4 r) { return (__float_as_uint(r.z) & VF_SPECULAR) != 0; }
 
-__device__ __forceinline__ bool light_area(const DevScene& sc, int li) {
+__device__ __forceinline__ bool light_area(const DevScene& sc, int li) {  // LightTypeArea
     int t = sc.lights[li].type;
-    return t == PRT_LIGHT_QUAD || t == PRT_LIGHT_SPHERE;
+    return t == PRT_LIGHT_QUAD || t == PRT_LIGHT_SPHERE || t == PRT_LIGHT_DISC || t == PRT_LIGHT_DISC_SPOT;
+}
+__device__ __forceinline__ bool light_delta(const DevScene& sc, int li) {  // LightTypePoint
+    return sc.lights[li].type == PRT_LIGHT_POINT_SPOT;
 }
 __device__ __forceinline__ bool on_surface(const DevScene& sc, const BV& v) {  // IsOnSurface :33-41
     if ((v.flags & VF_LIGHT) && v.light >= 0) return light_area(sc, v.light);
@@ -152,7 +155,26 @@ __device__ __forceinline__ ESample light_sample_emission(const DevScene& sc, con
         es.dir = sample_cosine_hemisphere(es.n, d0, d1);
         es.area_pdf = 1.0f / L.area;
         es.dir_pdf = dot(es.dir, es.n) / kPiF;
-        es.emission = emissive(sc, L.mat);
+        es.emission = emit_dir(sc, L.mat, es.dir);
+    } else if (L.type == PRT_LIGHT_DISC) {  // disc_light.go:104-111 + SampleEmissionDirection
+        disc_sample_uniform(L, p0, p1, es.p, es.n);
+        es.area_pdf = 1.0f / (kPiF * L.radius * L.radius);
+        es.dir = sample_cosine_hemisphere(es.n, d0, d1);
+        es.dir_pdf = dot(es.dir, es.n) / kPiF;
+        es.emission = emit_dir(sc, L.mat, es.dir);
+    } else if (L.type == PRT_LIGHT_DISC_SPOT) {  // disc_spot_light.go:170-192
+        disc_sample_uniform(L, p0, p1, es.p, es.n);
+        es.dir = sample_cone(v3(L.spot_dir), L.cos_total, d0, d1);
+        es.emission = v3(L.emission) * spot_falloff(L, dot(es.dir, v3(L.spot_dir)));
+        es.area_pdf = 1.0f / (kPiF * L.radius * L.radius);
+        es.dir_pdf = uniform_cone_pdf(L.cos_total);
+    } else if (L.type == PRT_LIGHT_POINT_SPOT) {  // point_spot_light.go:153-182
+        es.p = v3(L.from);
+        es.dir = sample_cone(v3(L.spot_dir), L.cos_total, d0, d1);
+        es.emission = v3(L.emission) * spot_falloff(L, dot(es.dir, v3(L.spot_dir)));
+        es.n = es.dir;
+        es.area_pdf = 1.0f;
+        es.dir_pdf = uniform_cone_pdf(L.cos_total);
     } else if (L.type == PRT_LIGHT_SPHERE) {
         V3 ld = sample_on_unit_sphere(p0, p1);
         es.p = v3(L.center) + ld * L.radius;
@@ -160,7 +182,7 @@ __device__ __forceinline__ ESample light_sample_emission(const DevScene& sc, con
         es.area_pdf = 1.0f / (4.0f * kPiF * L.radius * L.radius);
         es.dir = sample_cosine_hemisphere(ld, d0, d1);
         es.dir_pdf = dot(es.dir, ld) / kPiF;
-        es.emission = emissive(sc, L.mat);
+        es.emission = emit_dir(sc, L.mat, es.dir);
     } else if (L.type == PRT_LIGHT_ENVMAP) {
         float pdf;
         V3 wi = env_sample(sc.env, d0, d1, pdf);
@@ -197,6 +219,20 @@ __device__ __forceinline__ void light_pdf_le(const DevScene& sc, const DevLight&
         float ct = dot(d, v3(L.n));
         if (ct <= 0.0f) return;
         dir = ct / kPiF;
+    } else if (L.type == PRT_LIGHT_DISC || L.type == PRT_LIGHT_DISC_SPOT) {  // disc_light.go:114-133
+        V3 tp = p - v3(L.center), n = v3(L.n);  // validatePointOnDisc (:142-155), tolerance 0.001
+        float hn = dot(tp, n);
+        if (fabsf(hn) > 0.001f || length(tp - n * hn) > L.radius) return;
+        float ct = dot(d, n);
+        if (ct <= 0.0f) return;
+        pos = 1.0f / (kPiF * L.radius * L.radius);
+        dir = ct / kPiF;
+        if (L.type == PRT_LIGHT_DISC_SPOT && dot(d, v3(L.spot_dir)) < L.cos_total) dir = 0.0f;  // :195-210
+    } else if (L.type == PRT_LIGHT_POINT_SPOT) {  // point_spot_light.go:185-207
+        if (length(p - v3(L.from)) > 0.001f) return;
+        if (dot(d, v3(L.spot_dir)) < L.cos_total) return;
+        pos = 1.0f;
+        dir = uniform_cone_pdf(L.cos_total);
     } else if (L.type == PRT_LIGHT_SPHERE) {  // sphere_light.go:184-205
         V3 c = v3(L.center);
         if (!(fabsf(length(p - c) - L.radius) <= 0.001f)) return;
@@ -350,12 +386,18 @@ __device__ __forceinline__ float mis_weight(const DevScene& sc, const DevCamera&
     }
     ri = 1.0f;
     if (s > 0) {
+        // light vertices on a delta (point) light are not connectible (bdpt_mis.go:129-139)
+        auto delta = [&](float4 r) {
+            const uint32_t fl = __float_as_uint(r.z);
+            const int li = __float_as_int(r.w);
+            return (fl & VF_LIGHT) != 0 && li >= 0 && light_delta(sc, li);
+        };
         float4 cur = LP.rec(s - 1, i);
         for (int k = s - 1; k >= 0; k--) {
             float4 prev = k > 0 ? LP.rec(k - 1, i) : make_float4(0.f, 0.f, 0.f, 0.f);
             float fwd = cur.x, rev = cur.y;
-            bool conn = !rec_specular(cur);
-            if (k > 0) conn = conn && !rec_specular(prev);
+            bool conn = !rec_specular(cur) && !delta(cur);
+            if (k > 0) conn = conn && !rec_specular(prev) && !delta(prev);
             if (t == 1) {
                 if (k == s - 1 && s > 1) {
                     rev = l_rev1;

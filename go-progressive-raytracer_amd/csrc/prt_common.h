@@ -58,6 +58,7 @@ PRT_HD uint32_t make_leaf(uint32_t first, uint32_t count) { return kLeafBit | (f
 constexpr uint32_t kPrimTriangle = 0u;
 constexpr uint32_t kPrimSphere = 1u;
 constexpr uint32_t kPrimQuad = 2u;
+constexpr uint32_t kPrimDisc = 3u;  // a = (centre, tag), b = (unit normal, material), c = (radius, r^2, 0, id)
 PRT_HD uint32_t prim_kind(uint32_t tag) { return tag >> 30; }
 PRT_HD uint32_t prim_side(uint32_t tag) { return tag & 0x3FFFFFFFu; }
 
@@ -68,12 +69,14 @@ struct QuadExt {
 };
 
 // ---- materials and lights (float32 device copies) ------------------------------------
+constexpr int32_t kMatSpotEmitter = 5;  // the disc of a disc spot light (emits against spot_dir only)
 struct DevMaterial {
     int32_t type, outer, inner, pad;
     float albedo[3];
     float fuzz;
     float ior;
     float emission[3];
+    float spot_dir[3];
 };
 
 struct DevLight {
@@ -85,8 +88,13 @@ struct DevLight {
     float D, area;
     // sphere light
     float center[3], radius;
-    // infinite lights
+    // infinite lights; point spot light emission
     float emission[3], top[3], bottom[3];
+    // disc lights (centre and radius above; n = unit normal) and spot lights
+    float right[3], up[3];
+    float from[3], spot_dir[3];
+    float cos_total, cos_falloff;
+    float from_lo[3];  // from (float64) - from (float32): the position's low part
 };
 
 struct DevEnv {

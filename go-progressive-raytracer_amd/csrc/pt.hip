@@ -253,8 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
                 L = L + mulv(beta, e);
             } else {
                 Surf s = make_surf(sc, o, d, h);
-                const DevMaterial& M = sc.mats[s.mat];
-                if (M.type == 3 && s.front) L = L + mulv(beta, v3(M.emission));  // getEmittedLight :100-105
+                L = L + mulv(beta, emit_hit(sc, s.mat, d, s.front));  // getEmittedLight :100-105
                 ScatterOut so;
                 if (scatter(sc, s.mat, d, s, key, pos + 1, so)) {
                     V3 nd = so.dir;
@@ -325,16 +324,76 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
 
 // ---- bidirectional path tracing (BDPTIntegrator.RayColor, pkg/integrator/bdpt.go:73-102) ----------
 
-// extendPath, bdpt.go:206-291. Appends vertices from index n; returns the new length.
+// Distance along d from the float64 point o_hi + o_lo to the surface of primitive `prim` near the
+// float32 hit distance t32 (plane distance for triangles, quads and discs; the nearer sphere root).
+__device__ double prim_t_f64(const DevScene& sc, uint32_t prim, V3 o_hi, const float* o_lo, V3 d, float t32) {
+    const float4* pp = sc.prims + 3 * (size_t)prim;
+    const float4 a = pp[0], b = pp[1];
+    const uint32_t kind = prim_kind(__float_as_uint(a.w));
+    const double ox = (double)o_hi.x + o_lo[0], oy = (double)o_hi.y + o_lo[1], oz = (double)o_hi.z + o_lo[2];
+    const double dx = d.x, dy = d.y, dz = d.z;
+    double nx, ny, nz, px, py, pz;  // plane normal and a point on it
+    if (kind == kPrimTriangle) {
+        const float4 c = pp[2];
+        double e1x = (double)b.x - a.x, e1y = (double)b.y - a.y, e1z = (double)b.z - a.z;
+        double e2x = (double)c.x - a.x, e2y = (double)c.y - a.y, e2z = (double)c.z - a.z;
+        nx = e1y * e2z - e1z * e2y;
+        ny = e1z * e2x - e1x * e2z;
+        nz = e1x * e2y - e1y * e2x;
+        px = a.x, py = a.y, pz = a.z;
+    } else if (kind == kPrimQuad) {
+        const QuadExt& q = sc.quads[prim_side(__float_as_uint(a.w))];
+        nx = q.nx, ny = q.ny, nz = q.nz;
+        px = a.x, py = a.y, pz = a.z;
+    } else if (kind == kPrimDisc) {
+        nx = b.x, ny = b.y, nz = b.z;
+        px = a.x, py = a.y, pz = a.z;
+    } else {  // sphere: the root of |o + t d - c|^2 = r^2 nearest t32
+        double fx = ox - a.x, fy = oy - a.y, fz = oz - a.z;
+        double A = dx * dx + dy * dy + dz * dz, hb = fx * dx + fy * dy + fz * dz;
+        double C = fx * fx + fy * fy + fz * fz - (double)b.x * b.x;
+        double disc = hb * hb - A * C;
+        if (disc < 0) return (double)t32;
+        double sq = sqrt(disc), t0 = (-hb - sq) / A, t1 = (-hb + sq) / A;
+        return fabs(t0 - t32) <= fabs(t1 - t32) ? t0 : t1;
+    }
+    double den = nx * dx + ny * dy + nz * dz;
+    if (den == 0.0) return (double)t32;
+    return (nx * (px - ox) + ny * (py - oy) + nz * (pz - oz)) / den;
+}
+
+// Closest hit of a ray leaving a point light. The light's float32 position can lie off its float64
+// position by half an ulp, which moves float32 hit distances of surfaces within ~tmin of the light
+// across tmin = 0.001 (the reference's Cornell point light hangs exactly 0.001 below the ceiling).
+// Hits nearer than 0.01 are therefore re-decided in float64 from the light's low part; a rejected
+// hit is skipped and the search resumes beyond it.
+template <bool MIXED>
+__device__ bool trace_from_point(const DevScene& sc, V3 o, const float* o_lo, V3 d, THit& h,
+                                 TStack<kLds, kBlock>& st) {
+    float tmin = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        TRay r{o.x, o.y, o.z, tmin, d.x, d.y, d.z, __int_as_float(0x7f800000)};
+        if (!trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr)) return false;
+        if (h.t >= 0.01f || prim_t_f64(sc, (uint32_t)h.prim, o, o_lo, d, h.t) >= 0.001) return true;
+        tmin = __uint_as_float(__float_as_uint(h.t) + 1u);
+    }
+    TRay r{o.x, o.y, o.z, fmaxf(tmin, 0.01f), d.x, d.y, d.z, __int_as_float(0x7f800000)};
+    return trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
+}
+
+// extendPath, bdpt.go:206-291. Appends vertices from index n; returns the new length. `point_lo`
+// is the low part of the origin when the first segment leaves a point light, else null.
 template <bool MIXED>
 __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i, int n, V3 o, V3 d, V3 beta,
                            float pdf_fwd, int max_bounces, bool camera_path, uint32_t key, uint32_t slot0,
-                           TStack<kLds, kBlock>& st) {
+                           TStack<kLds, kBlock>& st, const float* point_lo = nullptr) {
     for (int b = 0; b < max_bounces; b++) {
         BV prev = VS.get(n - 1, i);
         TRay r{o.x, o.y, o.z, 0.001f, d.x, d.y, d.z, __int_as_float(0x7f800000)};
         THit h;
-        if (!trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr)) {
+        const bool hit = (b == 0 && point_lo) ? trace_from_point<MIXED>(sc, o, point_lo, d, h, st)
+                                              : trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
+        if (!hit) {
             if (camera_path) {  // createBackgroundVertex, bdpt.go:585-601
                 V3 e = v3(0, 0, 0);
                 for (int li = 0; li < sc.num_lights; li++)
@@ -362,8 +421,7 @@ __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i
         v.beta = beta;
         v.mat = s.mat;
         v.light = -1;
-        const DevMaterial& M = sc.mats[s.mat];
-        v.Le = (M.type == 3 && s.front) ? v3(M.emission) : v3(0, 0, 0);
+        v.Le = emit_hit(sc, s.mat, d, s.front);
         v.flags = (s.front ? VF_FRONT : 0u) | ((v.Le.x != 0.0f || v.Le.y != 0.0f || v.Le.z != 0.0f) ? VF_LIGHT : 0u);
         v.pdf_fwd = convert_density(sc, prev, v, pdf_fwd);
         v.pdf_rev = 0.0f;
@@ -457,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_paths(DevScene sc, PassParams p
             LP.put(0, i, lv);
             V3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
             nl = bdpt_extend<MIXED>(sc, LP, i, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false, key,
-                                    kLightSlot + 1, st);
+                                    kLightSlot + 1, st, L.type == PRT_LIGHT_POINT_SPOT ? L.from_lo : nullptr);
             if (inf) {
                 if (nl > 1) {
                     BV fb = LP.get(1, i);

@@ -58,6 +58,34 @@ struct QuadF64 {
 };
 
 // Quad.BoundingBox with the +-0.001 slab for axis-aligned quads (quad.go:44-77,183-210).
+// A disc in float64 with the reference's in-plane frame (NewDisc, disc.go:20-43).
+struct DiscF64 {
+    D3 c, n, right, up;
+    double r;
+    int mat;
+};
+DiscF64 make_disc(D3 c, D3 normal, double r, int mat) {
+    DiscF64 dc{c, normalize(normal), {}, {}, r, mat};
+    D3 a = std::fabs(dc.n.x) > 0.1 ? D3{0, 1, 0} : D3{1, 0, 0};
+    dc.right = normalize(cross(a, dc.n));
+    dc.up = normalize(cross(dc.n, dc.right));
+    return dc;
+}
+// Disc.BoundingBox: the box of the four corners c +- r*right +- r*up (disc.go:91-118).
+PrimBounds disc_bounds(const DiscF64& dc) {
+    D3 re = dc.right * dc.r, ue = dc.up * dc.r;
+    D3 cs[4] = {dc.c + re + ue, dc.c + re - ue, dc.c - re + ue, dc.c - re - ue};
+    PrimBounds b;
+    for (int a = 0; a < 3; a++) {
+        b.lo[a] = b.hi[a] = comp(cs[0], a);
+        for (int k = 1; k < 4; k++) {
+            b.lo[a] = std::min(b.lo[a], comp(cs[k], a));
+            b.hi[a] = std::max(b.hi[a], comp(cs[k], a));
+        }
+    }
+    return b;
+}
+
 PrimBounds quad_bounds(const QuadF64& q) {
     D3 n = normalize(cross(q.u, q.v));
     D3 cs[4] = {q.corner, q.corner + q.u, q.corner + q.v, q.corner + q.u + q.v};
@@ -587,7 +615,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PRT_ERR_NO_DEVICE, "no HIP device");
     if (device < 0 || device >= ndev) return fail(PRT_ERR_INVALID_ARGUMENT, "device index out of range");
     if (d->num_materials < 0 || d->num_spheres < 0 || d->num_quads < 0 || d->num_boxes < 0 ||
-        d->num_meshes < 0 || d->num_lights < 0)
+        d->num_meshes < 0 || d->num_lights < 0 || d->num_discs < 0)
         return fail(PRT_ERR_INVALID_ARGUMENT, "negative count");
     if (d->camera.width <= 0 || !(d->camera.aspect_ratio > 0))
         return fail(PRT_ERR_INVALID_ARGUMENT, "camera width/aspect must be positive");
@@ -645,6 +673,29 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         quads.push_back({c[4], c[0] - c[4], c[5] - c[4], m});
         world_boxes.push_back(wb);
     }
+    // discs: the scene's own, then the disc of each disc spot light with an internal emitting
+    // material appended after the caller's (scene.go:117-122 adds those discs to the shapes)
+    std::vector<DiscF64> discs;
+    for (int i = 0; i < d->num_discs; i++) {
+        const prt_disc& dc = d->discs[i];
+        if (bad_mat(dc.material)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "disc material out of range"));
+        if (!(dc.radius > 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "disc radius must be positive"));
+        D3 n = d3(dc.normal);
+        if (!(len(n) > 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "disc normal must be non-zero"));
+        discs.push_back(make_disc(d3(dc.center), n, dc.radius, dc.material));
+    }
+    std::vector<int> spot_index(std::max(d->num_lights, 1), -1);  // light -> internal material nm + k
+    int nspot = 0;
+    for (int i = 0; i < d->num_lights; i++) {
+        const prt_light& L = d->lights[i];
+        if (L.type != PRT_LIGHT_DISC_SPOT && L.type != PRT_LIGHT_POINT_SPOT) continue;
+        if (!(len(d3(L.to) - d3(L.from)) > 0))
+            return bail(fail(PRT_ERR_INVALID_ARGUMENT, "spot light needs from != to"));
+        if (L.type != PRT_LIGHT_DISC_SPOT) continue;
+        if (!(L.radius > 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "disc spot light radius must be positive"));
+        spot_index[i] = nspot++;
+        discs.push_back(make_disc(d3(L.from), d3(L.to) - d3(L.from), L.radius, nm + spot_index[i]));
+    }
     int64_t ntri = 0;
     for (int i = 0; i < d->num_meshes; i++) {
         const prt_mesh& m = d->meshes[i];
@@ -656,7 +707,8 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     }
     const int64_t nsph = d->num_spheres;
     const int64_t nq = (int64_t)quads.size();
-    const int64_t nprim = ntri + nsph + nq;
+    const int64_t ndisc = (int64_t)discs.size();
+    const int64_t nprim = ntri + nsph + nq + ndisc;
     if (nprim >= (1ll << 28)) return bail(fail(PRT_ERR_UNSUPPORTED, "more than 2^28 primitives"));
 
     std::vector<PrimBounds> bounds(nprim);
@@ -780,6 +832,23 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         grow(world, wb);
         world_any = true;
     }
+    for (int64_t i = 0; i < ndisc; i++, gid++) {
+        const DiscF64& dc = discs[i];
+        PrimBounds ref = disc_bounds(dc);
+        grow(world, ref);
+        world_any = true;
+        // the BVH box is padded: the corner box touches the disc rim, which a float32 plane hit
+        // can overshoot by a few ulps
+        PrimBounds& b = bounds[gid];
+        for (int a = 0; a < 3; a++) {
+            double pad = 1e-5 * dc.r + 1e-6 * std::fabs(comp(dc.c, a));
+            b.lo[a] = ref.lo[a] - pad;
+            b.hi[a] = ref.hi[a] + pad;
+        }
+        rec[3 * gid + 0] = f4(dc.c.x, dc.c.y, dc.c.z, bits(kPrimDisc << 30));
+        rec[3 * gid + 1] = f4(dc.n.x, dc.n.y, dc.n.z, bits((uint32_t)dc.mat));
+        rec[3 * gid + 2] = make_float4((float)dc.r, (float)(dc.r * dc.r), 0.f, bits((uint32_t)gid));
+    }
     // world centre / radius of the reference BVH root box (bvh.go:38-48)
     if (world_any) {
         for (int a = 0; a < 3; a++) s->info.world_center[a] = 0.5 * (world.lo[a] + world.hi[a]);
@@ -821,12 +890,12 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     s->dev.quads = (const QuadExt*)p;
     if ((rc = scene_alloc(s, trin.size() * sizeof(float4), trin.data(), &p)) != PRT_OK) return bail(rc);
     s->dev.tri_normals = (const float4*)p;
-    s->dev.mixed = (nsph + nq) > 0 ? 1 : 0;
+    s->dev.mixed = (nsph + nq + ndisc) > 0 ? 1 : 0;
     s->dev.max_stack = bvh.max_stack;
     s->dev.num_nodes = (int32_t)bvh.nodes.size();
 
     // materials (float32 copies)
-    std::vector<DevMaterial> mats(std::max(nm, 1));
+    std::vector<DevMaterial> mats(std::max(nm + nspot, 1));
     for (int i = 0; i < nm; i++) {
         const prt_material& m = d->materials[i];
         DevMaterial& o = mats[i];
@@ -839,6 +908,18 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         }
         o.fuzz = (float)std::min(1.0, std::max(0.0, m.fuzz));  // NewMetal clamps (metal.go:19-29)
         o.ior = (float)m.ior;
+    }
+    for (int i = 0; i < d->num_lights; i++) {  // discSpotLightMaterial (disc_spot_light.go:12-46)
+        if (spot_index[i] < 0) continue;
+        const prt_light& L = d->lights[i];
+        DevMaterial& o = mats[nm + spot_index[i]];
+        std::memset(&o, 0, sizeof(o));
+        o.type = kMatSpotEmitter;
+        D3 dir = normalize(d3(L.to) - d3(L.from));
+        for (int a = 0; a < 3; a++) {
+            o.emission[a] = (float)L.emission[a];
+            o.spot_dir[a] = (float)comp(dir, a);
+        }
     }
     if ((rc = scene_alloc(s, mats.size() * sizeof(DevMaterial), mats.data(), &p)) != PRT_OK) return bail(rc);
     s->dev.mats = (const DevMaterial*)p;
@@ -882,6 +963,32 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
             for (int a = 0; a < 3; a++) o.center[a] = (float)sp.center[a];
             o.radius = (float)sp.radius;
             o.mat = sp.material;
+        } else if (L.type == PRT_LIGHT_DISC || L.type == PRT_LIGHT_DISC_SPOT || L.type == PRT_LIGHT_POINT_SPOT) {
+            if (L.type == PRT_LIGHT_DISC && (L.shape < 0 || L.shape >= d->num_discs))
+                return bail(fail(PRT_ERR_INVALID_ARGUMENT, "disc light shape out of range"));
+            if (L.type != PRT_LIGHT_DISC) {  // NewDiscSpotLight / NewPointSpotLight
+                const double deg = 3.14159265358979323846 / 180.0;
+                D3 dir = normalize(d3(L.to) - d3(L.from));
+                for (int a = 0; a < 3; a++) {
+                    o.from[a] = (float)L.from[a];
+                    o.from_lo[a] = (float)(L.from[a] - (double)o.from[a]);
+                    o.spot_dir[a] = (float)comp(dir, a);
+                }
+                o.cos_total = (float)std::cos(L.cone_angle * deg);
+                o.cos_falloff = (float)std::cos((L.cone_angle - L.cone_delta) * deg);
+            }
+            if (L.type != PRT_LIGHT_POINT_SPOT) {
+                const DiscF64& dc =
+                    L.type == PRT_LIGHT_DISC ? discs[L.shape] : discs[d->num_discs + spot_index[i]];
+                for (int a = 0; a < 3; a++) {
+                    o.center[a] = (float)comp(dc.c, a);
+                    o.n[a] = (float)comp(dc.n, a);
+                    o.right[a] = (float)comp(dc.right, a);
+                    o.up[a] = (float)comp(dc.up, a);
+                }
+                o.radius = (float)dc.r;
+                o.mat = dc.mat;
+            }
         } else if (L.type == PRT_LIGHT_UNIFORM_INFINITE || L.type == PRT_LIGHT_GRADIENT_INFINITE) {
             ninf++;
         } else if (L.type == PRT_LIGHT_ENVMAP) {

@@ -113,10 +113,13 @@ __device__ __forceinline__ Surf make_surf(const DevScene& sc, V3 o, V3 d, const 
         float len = length(r);
         if (len > 0.0f) s.p = c + r * (fabsf(b.x) / len);
         outward = (s.p - c) * (1.0f / b.x);  // sphere.go:63, sign of r kept
-    } else {
+    } else if (kind == kPrimQuad) {
         const QuadExt& q = sc.quads[prim_side(tag)];
         outward = v3(q.nx, q.ny, q.nz);
         s.p = s.p - outward * (dot(s.p, outward) - q.D);
+    } else {  // disc: SetFaceNormal with the disc normal (disc.go:85)
+        outward = v3(b.x, b.y, b.z);
+        s.p = s.p - outward * dot(s.p - v3(a.x, a.y, a.z), outward);
     }
     s.front = dot(d, outward) < 0.0f;
     s.n = s.front ? outward : outward * -1.0f;
@@ -295,10 +298,18 @@ struct LSample {
     V3 p, n;  // LightSample.Point / Normal (used by BDPT's sampled light vertex)
 };
 
-// Emit of the light's material with no hit record (emissive.go:30-37); zero if not emissive
-__device__ __forceinline__ V3 emissive(const DevScene& sc, int m) {
+// Emitter.Emit for a ray of direction d without a hit record (emissive.go:30-37): the material's
+// emission; the disc spot light's material emits only against its spot direction (d . spot <= -0.3,
+// disc_spot_light.go:26-40). Zero for non-emitters.
+__device__ __forceinline__ V3 emit_dir(const DevScene& sc, int m, V3 d) {
     const DevMaterial& M = sc.mats[m];
-    return M.type == 3 ? v3(M.emission) : v3(0, 0, 0);
+    if (M.type == 3) return v3(M.emission);
+    if (M.type == kMatSpotEmitter && dot(d, v3(M.spot_dir)) <= -0.3f) return v3(M.emission);
+    return v3(0, 0, 0);
+}
+// ... with a hit record: the back face emits nothing
+__device__ __forceinline__ V3 emit_hit(const DevScene& sc, int m, V3 d, bool front) {
+    return front ? emit_dir(sc, m, d) : v3(0, 0, 0);
 }
 
 // sphere test for light routines (sphere.go:27-81), returns t or -1
@@ -366,7 +377,47 @@ __device__ __forceinline__ V3 env_sample(const DevEnv& e, float s0, float s1, fl
     return v3(st * cp, ct, st * sp);
 }
 
-__device__ __forceinline__ bool light_is_infinite(int type) { return type >= PRT_LIGHT_UNIFORM_INFINITE; }
+__device__ __forceinline__ bool light_is_infinite(int type) {
+    return type == PRT_LIGHT_UNIFORM_INFINITE || type == PRT_LIGHT_GRADIENT_INFINITE || type == PRT_LIGHT_ENVMAP;
+}
+
+// falloff, point_spot_light.go:121-139 (the disc spot light uses the same curve)
+__device__ __forceinline__ float spot_falloff(const DevLight& L, float cos_a) {
+    if (cos_a < L.cos_total) return 0.0f;
+    if (cos_a >= L.cos_falloff) return 1.0f;
+    float d = (cos_a - L.cos_total) / (L.cos_falloff - L.cos_total);
+    return d * d * d * d;
+}
+__device__ __forceinline__ V3 sample_cone(V3 w, float cos_total, float s0, float s1) {  // core/sampling.go:70-94
+    V3 u = fabsf(w.x) > 0.1f ? v3(0, 1, 0) : v3(1, 0, 0);
+    u = normalize(cross(u, w));
+    V3 v = cross(w, u);
+    float ct = 1.0f - s0 * (1.0f - cos_total);
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float sp, cp;
+    sincosf(2.0f * kPiF * s1, &sp, &cp);
+    return u * (st * cp) + v * (st * sp) + w * ct;
+}
+__device__ __forceinline__ float uniform_cone_pdf(float cos_total) {  // light_sampling.go:83-85
+    return 1.0f / (2.0f * kPiF * (1.0f - cos_total));
+}
+__device__ __forceinline__ void disc_sample_uniform(const DevLight& L, float s0, float s1, V3& p, V3& n) {
+    float r = sqrtf(s0) * L.radius;  // SampleUniform, disc.go:121-134
+    float sp, cp;
+    sincosf(2.0f * kPiF * s1, &sp, &cp);
+    p = v3(L.center) + v3(L.right) * (r * cp) + v3(L.up) * (r * sp);
+    n = v3(L.n);
+}
+// the disc of a disc light re-hit from p along dir (DiscLight.PDF, disc_light.go:82-101); t or -1
+__device__ __forceinline__ float disc_hit_t(const DevLight& L, V3 p, V3 dir) {
+    V3 n = v3(L.n);
+    float den = dot(n, dir);
+    if (fabsf(den) < 1e-6f) return -1.0f;
+    float t = dot(n, v3(L.center) - p) / den;
+    if (t < 0.001f) return -1.0f;
+    V3 cp = p + dir * t - v3(L.center);
+    return dot(cp, cp) > L.radius * L.radius ? -1.0f : t;
+}
 
 // Light.Emit of an infinite light for a ray leaving the scene
 __device__ __forceinline__ V3 light_emit(const DevScene& sc, const DevLight& L, V3 d) {
@@ -392,7 +443,7 @@ __device__ __forceinline__ LSample sphere_sample_uniform(const DevScene& sc, con
     ls.dist = length(dir);
     ls.dir = normalize(dir);
     ls.pdf = 1.0f / (4.0f * kPiF * L.radius * L.radius);
-    ls.emission = emissive(sc, L.mat);
+    ls.emission = emit_dir(sc, L.mat, ls.dir);
     ls.p = pt;
     ls.n = ld;
     return ls;
@@ -418,7 +469,7 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLig
             return ls;
         }
         ls.pdf = (1.0f / L.area) * dist * dist / ct;
-        ls.emission = dot(dir, qn) < 0.0f ? emissive(sc, L.mat) : v3(0, 0, 0);
+        ls.emission = dot(dir, qn) < 0.0f ? emit_dir(sc, L.mat, dir) : v3(0, 0, 0);
         return ls;
     }
     if (L.type == PRT_LIGHT_SPHERE) {  // sphere_light.go:37-141
@@ -442,10 +493,54 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, const DevLig
         ls.dir = dir;
         ls.dist = t;
         ls.pdf = 1.0f / (2.0f * kPiF * (1.0f - ctm));
-        ls.emission = emissive(sc, L.mat);
+        ls.emission = emit_dir(sc, L.mat, dir);
         ls.p = p + dir * t;
         V3 outward = (ls.p - c) * (1.0f / L.radius);
         ls.n = dot(dir, outward) < 0.0f ? outward : outward * -1.0f;  // SetFaceNormal of the re-hit
+        return ls;
+    }
+    if (L.type == PRT_LIGHT_DISC || L.type == PRT_LIGHT_DISC_SPOT) {
+        // DiscLight.Sample (disc_light.go:31-79: no front-face test), DiscSpotLight.Sample (:105-116)
+        V3 spt, sn;
+        disc_sample_uniform(L, s0, s1, spt, sn);
+        V3 d = spt - p;
+        float dist = length(d);
+        ls.p = spt;
+        ls.n = sn;
+        if (dist == 0.0f) {
+            ls.dir = v3(0, 1, 0);
+            ls.dist = 0.0f;
+            ls.emission = v3(0, 0, 0);
+            ls.pdf = 1.0f;
+            return ls;
+        }
+        V3 dn = normalize(d);
+        float ct = fabsf(dot(sn, dn * -1.0f));
+        ls.pdf = ct < 1e-6f ? 0.0f : (1.0f / (kPiF * L.radius * L.radius)) * dist * dist / ct;
+        ls.dir = dn;
+        ls.dist = dist;
+        ls.emission = emit_dir(sc, L.mat, dn);
+        if (L.type == PRT_LIGHT_DISC_SPOT) ls.emission = ls.emission * spot_falloff(L, dot(v3(L.spot_dir), normalize(p - spt)));
+        return ls;
+    }
+    if (L.type == PRT_LIGHT_POINT_SPOT) {  // PointSpotLight.Sample, point_spot_light.go:47-92 (pdf 1)
+        V3 tl = v3(L.from) - p;
+        float dist = length(tl);
+        ls.p = v3(L.from);
+        ls.pdf = 1.0f;
+        if (dist == 0.0f) {
+            ls.n = v3(0, 1, 0);
+            ls.dir = v3(0, 1, 0);
+            ls.dist = 0.0f;
+            ls.emission = v3(0, 0, 0);
+            return ls;
+        }
+        V3 to_light = normalize(tl);
+        float att = spot_falloff(L, dot(v3(L.spot_dir), to_light * -1.0f));
+        ls.emission = v3(L.emission) * (att / (dist * dist));
+        ls.n = to_light * -1.0f;
+        ls.dir = to_light;
+        ls.dist = dist;
         return ls;
     }
     if (L.type == PRT_LIGHT_ENVMAP) {
@@ -499,6 +594,18 @@ __device__ __forceinline__ float light_pdf(const DevScene& sc, const DevLight& L
         float stm = L.radius / dc;
         float ctm = sqrtf(fmaxf(0.0f, 1.0f - stm * stm));
         return 1.0f / (2.0f * kPiF * (1.0f - ctm));
+    }
+    if (L.type == PRT_LIGHT_DISC || L.type == PRT_LIGHT_DISC_SPOT) {  // disc_light.go:82-101
+        float t = disc_hit_t(L, p, dir);
+        if (t < 0.0f) return 0.0f;
+        float ct = fabsf(dot(v3(L.n), dir * -1.0f));
+        if (ct < 1e-6f) return 0.0f;
+        return (1.0f / (kPiF * L.radius * L.radius)) * t * t / ct;
+    }
+    if (L.type == PRT_LIGHT_POINT_SPOT) {  // point_spot_light.go:96-118: 1 within 0.999 of the light
+        V3 tl = v3(L.from) - p;
+        if (dot(tl, tl) == 0.0f) return 0.0f;
+        return dot(dir, normalize(tl)) > 0.999f ? 1.0f : 0.0f;
     }
     if (L.type == PRT_LIGHT_ENVMAP) return env_pdf(sc.env, dir);
     float ct = dot(dir, n);  // uniform/gradient: cosine hemisphere

@@ -156,6 +156,18 @@ __device__ __forceinline__ bool quad_test(const TRay& r, float4 a, float4 b, flo
     return true;
 }
 
+// Disc test (disc.go:46-88): plane hit with |n.d| >= 1e-6, t in [tmin, tbest], |p - c|^2 <= r^2.
+__device__ __forceinline__ bool disc_test(const TRay& r, float4 a, float4 b, float4 c, float tbest, float& t_out) {
+    float den = b.x * r.dx + b.y * r.dy + b.z * r.dz;
+    if (fabsf(den) < 1e-6f) return false;
+    float t = (b.x * (a.x - r.ox) + b.y * (a.y - r.oy) + b.z * (a.z - r.oz)) / den;
+    if (t < r.tmin || t > tbest) return false;
+    float px = r.ox + t * r.dx - a.x, py = r.oy + t * r.dy - a.y, pz = r.oz + t * r.dz - a.z;
+    if (px * px + py * py + pz * pz > c.y) return false;
+    t_out = t;
+    return true;
+}
+
 constexpr float kBoxWiden = 1.0f + 2.0f * (3.0f * 5.9604645e-08f) / (1.0f - 3.0f * 5.9604645e-08f);
 
 // Traversal stack: the first LDS_DEPTH entries per lane live in LDS (lane-major so each lane
@@ -318,9 +330,11 @@ __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStac
                     h = tri_test(r, s.w, a, b, c, s.tbest, t, u, v);
                 } else if (kind == kPrimSphere) {
                     h = sphere_test(r, a, b, s.tbest, t);
-                } else {
+                } else if (kind == kPrimQuad) {
                     float4 c = pp[2];
                     h = quad_test(r, a, b, c, sc.quads[prim_side(tag)], s.tbest, t, u, v);
+                } else {
+                    h = disc_test(r, a, b, pp[2], s.tbest, t);
                 }
             }
             if (h) {

@@ -22,6 +22,7 @@ ERRORS = {1: "invalid argument", 2: "device error", 3: "out of memory", 4: "no d
 
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_EMISSIVE, MAT_LAYERED = range(5)
 LIGHT_QUAD, LIGHT_SPHERE, LIGHT_UNIFORM_INFINITE, LIGHT_GRADIENT_INFINITE, LIGHT_ENVMAP = range(5)
+LIGHT_DISC, LIGHT_DISC_SPOT, LIGHT_POINT_SPOT = 5, 6, 7
 INTEGRATOR_PATH, INTEGRATOR_BDPT = 0, 1
 BUFFERS_HOST, BUFFERS_DEVICE = 0, 1
 
@@ -60,9 +61,15 @@ class Mesh(C.Structure):
                 ("center", C.POINTER(C.c_double))]
 
 
+class Disc(C.Structure):
+    _fields_ = [("center", D3), ("normal", D3), ("radius", C.c_double), ("material", C.c_int32), ("pad", C.c_int32)]
+
+
 class Light(C.Structure):
     _fields_ = [("type", C.c_int32), ("shape", C.c_int32), ("emission", D3), ("top", D3), ("bottom", D3),
-                ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_int32), ("env_height", C.c_int32)]
+                ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_int32), ("env_height", C.c_int32),
+                ("from_", D3), ("to", D3), ("cone_angle", C.c_double), ("cone_delta", C.c_double),
+                ("radius", C.c_double)]
 
 
 class Camera(C.Structure):
@@ -79,7 +86,8 @@ class SceneDescC(C.Structure):
                 ("meshes", C.POINTER(Mesh)), ("num_meshes", C.c_int32),
                 ("lights", C.POINTER(Light)), ("num_lights", C.c_int32),
                 ("light_weights", C.POINTER(C.c_double)),
-                ("camera", Camera)]
+                ("camera", Camera),
+                ("discs", C.POINTER(Disc)), ("num_discs", C.c_int32)]
 
 
 class SceneInfo(C.Structure):
@@ -244,6 +252,7 @@ class SceneDesc:
         self.spheres = []
         self.quads = []
         self.boxes = []
+        self.discs = []
         self.meshes = []  # (vertices (n,3), faces (m,3), material, face_materials|None, options dict)
         self.lights = []
         self.light_weights = None
@@ -289,6 +298,13 @@ class SceneDesc:
         self.quads.append(q)
         return len(self.quads) - 1
 
+    def disc(self, center, normal, radius, material):
+        """NewDisc (pkg/geometry/disc.go:20-43)."""
+        dsc = Disc()
+        dsc.center, dsc.normal, dsc.radius, dsc.material = D3(*center), D3(*normal), radius, material
+        self.discs.append(dsc)
+        return len(self.discs) - 1
+
     def box(self, center, size, rotation, material):
         b = Box()
         b.center, b.size, b.rotation, b.material = D3(*center), D3(*size), D3(*rotation), material
@@ -319,6 +335,28 @@ class SceneDesc:
         q = self.quad(corner, u, v, m)
         L = Light()
         L.type, L.shape = LIGHT_QUAD, q
+        self.lights.append(L)
+
+    def add_disc_light(self, center, normal, radius, emission):
+        """A disc area light (NewDiscLight, pkg/lights/disc_light.go:17-22) whose disc is a scene shape."""
+        m = self.emissive(emission)
+        dsc = self.disc(center, normal, radius, m)
+        L = Light()
+        L.type, L.shape = LIGHT_DISC, dsc
+        self.lights.append(L)
+
+    def add_spot_light(self, frm, to, emission, cone_angle, cone_delta, radius):
+        """AddSpotLight (pkg/scene/scene.go:117-122): a disc spot light; its disc joins the shapes."""
+        L = Light()
+        L.type, L.shape, L.emission = LIGHT_DISC_SPOT, -1, D3(*emission)
+        L.from_, L.to, L.cone_angle, L.cone_delta, L.radius = D3(*frm), D3(*to), cone_angle, cone_delta, radius
+        self.lights.append(L)
+
+    def add_point_spot_light(self, frm, to, emission, cone_angle, cone_delta):
+        """AddPointSpotLight (pkg/scene/scene.go:125-129): a point spot light (a delta light)."""
+        L = Light()
+        L.type, L.shape, L.emission = LIGHT_POINT_SPOT, -1, D3(*emission)
+        L.from_, L.to, L.cone_angle, L.cone_delta = D3(*frm), D3(*to), cone_angle, cone_delta
         self.lights.append(L)
 
     def add_uniform_infinite_light(self, emission):
@@ -386,6 +424,7 @@ class SceneDesc:
         d.spheres, d.num_spheres = arr(Sphere, self.spheres), len(self.spheres)
         d.quads, d.num_quads = arr(Quad, self.quads), len(self.quads)
         d.boxes, d.num_boxes = arr(Box, self.boxes), len(self.boxes)
+        d.discs, d.num_discs = arr(Disc, self.discs), len(self.discs)
         d.meshes, d.num_meshes = arr(Mesh, meshes), len(meshes)
         d.lights, d.num_lights = arr(Light, self.lights), len(self.lights)
         if self.light_weights is not None:

@@ -50,7 +50,9 @@ def ground_quad(s, center, size, material):
 def cornell_scene(scale=1.0, width=400, light="reference", tall_angle_deg=-20.0, short_angle_deg=18.0):
     """NewCornellScene(CornellBoxes, CornellQuadLight) (pkg/scene/cornell.go:32-257), uniformly
     scaled by `scale`. light="reference": 130x105 quad, emission (18,15,8)*2.5;
-    light="metre": radiance 15 on the same (scaled) quad."""
+    light="metre": radiance 15 on the same (scaled) quad; light="point": CornellPointLight, a
+    180-degree point spot light of intensity (18,15,8)*20000 at the quad's centre; light="sphere":
+    CornellSphereLight, a radius-52.5 sphere light of emission (18,15,8)*2.05 (cornell.go:150-190)."""
     k = scale
     s = P.SceneDesc()
     s.set_camera(center=(278 * k, 278 * k, -800 * k), look_at=(278 * k, 278 * k, 0), up=(0, 1, 0), width=width,
@@ -63,8 +65,16 @@ def cornell_scene(scale=1.0, width=400, light="reference", tall_angle_deg=-20.0,
     s.quad((0, 0, 556 * k), (556 * k, 0, 0), (0, 556 * k, 0), white)    # back wall
     s.quad((0, 0, 0), (0, 0, 556 * k), (0, 556 * k, 0), red)            # left wall
     s.quad((556 * k, 0, 0), (0, 0, 556 * k), (0, 556 * k, 0), green)    # right wall
-    emission = (18.0 * 2.5, 15.0 * 2.5, 8.0 * 2.5) if light == "reference" else (15.0, 15.0, 15.0)
-    s.add_quad_light((213 * k, (556 - 0.001) * k, 227 * k), (130 * k, 0, 0), (0, 0, 105 * k), emission)
+    centre = (278.0 * k, (556 - 0.001) * k, 279.5 * k)
+    if light == "point":
+        # the reference passes (0,-1,0) as the spot's target point; a 180-degree cone lights every
+        # direction, so only the sampled cone frame depends on it
+        s.add_point_spot_light(centre, (0, -1, 0), (18.0 * 20000, 15.0 * 20000, 8.0 * 20000), 180.0, 0.0)
+    elif light == "sphere":
+        s.add_sphere_light(centre, 52.5 * k, (18.0 * 2.05, 15.0 * 2.05, 8.0 * 2.05))
+    else:
+        emission = (18.0 * 2.5, 15.0 * 2.5, 8.0 * 2.5) if light == "reference" else (15.0, 15.0, 15.0)
+        s.add_quad_light((213 * k, (556 - 0.001) * k, 227 * k), (130 * k, 0, 0), (0, 0, 105 * k), emission)
     white_box = s.lambertian((0.73, 0.73, 0.73))
     mirror = s.metal((0.9, 0.9, 0.9), 0.0)
     s.box((370 * k, 82.5 * k, 169 * k), (82.5 * k, 82.5 * k, 82.5 * k), (0, short_angle_deg * math.pi / 180, 0),
@@ -159,3 +169,56 @@ def dragon_scene(ply_path=None, finish="gold", width=900):
             raise ValueError("PLY normals are per vertex; NewTriangleMesh needs one normal per triangle")
         s.mesh(d["vertices"], d["faces"], mat, normals=n, rotation=rot, center=(0.0, 0.0, 0.0))
     return s, dict(max_depth=50, rr_min_bounces=15, spp=200, adaptive_min_samples=0.15, adaptive_threshold=0.01)
+
+
+def caustic_glass_scene(mesh_dir=None, light="spot", width=525):
+    """NewCausticGlassScene (pkg/scene/caustic_glass.go:17-188): the glass.pbrt camera, a spot light
+    from (0,5,9) to (-5,2.75,0) of intensity (139.81,118.64,105.39) with a 30-degree cone and 5-degree
+    falloff (light="spot": disc spot light of radius 0.7; light="point": point spot light), a
+    uniform 0.1 sky, and, when `mesh_dir` holds them, mesh_00001.ply (glass, ior 1.25) and
+    mesh_00002.ply (Lambertian 0.64)."""
+    import os
+    s = P.SceneDesc()
+    s.set_camera(center=(-5.5, 7, -5.5), look_at=(-4.75, 2.25, 0), up=(0, 1, 0), width=width,
+                 aspect_ratio=525.0 / 750.0, vfov=30.0 * 1.5, aperture=0.0, focus_distance=0.0)
+    frm, to = (0, 5, 9), (-5, 2.75, 0)
+    intensity = (139.8113403320, 118.6366500854, 105.3887557983)
+    if light == "point":
+        s.add_point_spot_light(frm, to, intensity, 30.0, 5.0)
+    else:
+        s.add_spot_light(frm, to, intensity, 30.0, 5.0, 0.7)
+    s.add_uniform_infinite_light((0.1, 0.1, 0.1))
+    if mesh_dir:
+        for name, mat in (("mesh_00001.ply", s.dielectric(1.25)), ("mesh_00002.ply", s.lambertian((0.6399999857,) * 3))):
+            path = os.path.join(mesh_dir, name)
+            if os.path.exists(path):
+                ply = P.load_ply(path)
+                s.mesh(ply.vertices, ply.faces, mat)
+    cfg = dict(max_depth=20, rr_min_bounces=16, spp=8192, adaptive_min_samples=0.2, adaptive_threshold=0.005)
+    return s, cfg
+
+
+def lights_scene(width=96):
+    """A metre-scale Cornell box lit by every finite light type: a disc light under the ceiling, a
+    disc spot light and a point spot light aimed at the floor from the upper corners, and a
+    Lambertian disc and a glass sphere on the floor."""
+    s = P.SceneDesc()
+    s.set_camera(center=(0.278, 0.278, -0.8), look_at=(0.278, 0.278, 0), up=(0, 1, 0), width=width,
+                 aspect_ratio=1.0, vfov=40.0, aperture=0.0, focus_distance=0.0)
+    white = s.lambertian((0.73, 0.73, 0.73))
+    red = s.lambertian((0.65, 0.05, 0.05))
+    green = s.lambertian((0.12, 0.45, 0.15))
+    k = 0.001
+    s.quad((0, 0, 0), (556 * k, 0, 0), (0, 0, 556 * k), white)
+    s.quad((0, 556 * k, 0), (556 * k, 0, 0), (0, 0, 556 * k), white)
+    s.quad((0, 0, 556 * k), (556 * k, 0, 0), (0, 556 * k, 0), white)
+    s.quad((0, 0, 0), (0, 0, 556 * k), (0, 556 * k, 0), red)
+    s.quad((556 * k, 0, 0), (0, 0, 556 * k), (0, 556 * k, 0), green)
+    s.add_disc_light((0.278, 0.5559, 0.2795), (0, -1, 0), 0.06, (12.0, 11.0, 9.0))
+    s.add_spot_light((0.05, 0.5, 0.05), (0.3, 0.0, 0.35), (4.0, 6.0, 9.0), 35.0, 8.0, 0.02)
+    s.add_point_spot_light((0.5, 0.5, 0.1), (0.2, 0.0, 0.4), (0.08, 0.06, 0.03), 40.0, 10.0)
+    blue = s.lambertian((0.2, 0.3, 0.7))
+    s.disc((0.15, 0.004, 0.3), (0, 1, 0), 0.09, blue)
+    s.sphere((0.38, 0.1, 0.3), 0.1, s.dielectric(1.5))
+    cfg = dict(max_depth=6, rr_min_bounces=3, spp=16)
+    return s, cfg

@@ -74,6 +74,7 @@ typedef struct prt_hit {
 #define PRT_MAT_DIELECTRIC 2
 #define PRT_MAT_EMISSIVE 3
 #define PRT_MAT_LAYERED 4
+/* (type 5 is internal: the disc of a disc spot light, discSpotLightMaterial, disc_spot_light.go:12-46) */
 
 typedef struct prt_material {
     int32_t type;
@@ -100,6 +101,14 @@ typedef struct prt_quad { /* pkg/geometry/quad.go:98-121 */
     int32_t material;
     int32_t pad;
 } prt_quad;
+
+typedef struct prt_disc { /* pkg/geometry/disc.go:11-43 (normal is normalized on use) */
+    double center[3];
+    double normal[3];
+    double radius;
+    int32_t material;
+    int32_t pad;
+} prt_disc;
 
 typedef struct prt_box { /* pkg/geometry/box.go:9-145: size = half extents, rotation X->Y->Z radians */
     double center[3];
@@ -131,6 +140,10 @@ typedef struct prt_mesh { /* pkg/geometry/triangle_mesh.go:33-146; primitive id 
 #define PRT_LIGHT_UNIFORM_INFINITE 2
 #define PRT_LIGHT_GRADIENT_INFINITE 3
 #define PRT_LIGHT_ENVMAP 4
+#define PRT_LIGHT_DISC 5       /* pkg/lights/disc_light.go: shape = index into discs[] */
+#define PRT_LIGHT_DISC_SPOT 6  /* pkg/lights/disc_spot_light.go (AddSpotLight, scene.go:117-122): from, to,
+                                  emission, cone_angle, cone_delta, radius; its disc joins the shapes */
+#define PRT_LIGHT_POINT_SPOT 7 /* pkg/lights/point_spot_light.go: from, to, emission, cone_angle, cone_delta */
 
 typedef struct prt_light {
     int32_t type;
@@ -141,6 +154,11 @@ typedef struct prt_light {
     const float* env_rgb;/* envmap: env_height rows * env_width * 3 linear RGB, row 0 = +Y pole */
     int32_t env_width;
     int32_t env_height;
+    double from[3];      /* spot lights: position */
+    double to[3];        /* spot lights: aim point */
+    double cone_angle;   /* spot lights: total cone angle, degrees */
+    double cone_delta;   /* spot lights: falloff transition angle, degrees */
+    double radius;       /* disc spot light: disc radius */
 } prt_light;
 
 typedef struct prt_camera { /* pkg/geometry/camera.go:18-32 */
@@ -164,10 +182,12 @@ typedef struct prt_scene_desc {
     const prt_light* lights;       int32_t num_lights;
     const double* light_weights;   /* NULL: uniform light sampler (scene.go:62-64) */
     prt_camera camera;
+    const prt_disc* discs;         int32_t num_discs;
 } prt_scene_desc;
 
 /* Global primitive numbering used by prt_hit.prim_id: all mesh triangles (mesh by
- * mesh, face order), then spheres, then quads, then 6 quads per box (box.go:34-89 face order). */
+ * mesh, face order), then spheres, then quads, then 6 quads per box (box.go:34-89 face order),
+ * then discs, then the discs of disc spot lights (in light order). */
 
 typedef struct prt_scene_info {
     int64_t num_primitives;

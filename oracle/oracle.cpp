@@ -324,6 +324,48 @@ struct Quad : Shape {
     }
 };
 
+// ---- Disc, pkg/geometry/disc.go:11-134 ---------------------------------------------
+struct Disc : Shape {
+    Vec3 c, n, right, up;
+    double r;
+    int mat;
+    int64_t id;
+    Disc(Vec3 center, Vec3 normal, double radius, int m, int64_t i) : c(center), r(radius), mat(m), id(i) {
+        n = normal.normalize();  // NewDisc :20-43
+        right = std::fabs(n.x) > 0.1 ? Vec3(0, 1, 0) : Vec3(1, 0, 0);
+        right = right.cross(n).normalize();
+        up = n.cross(right).normalize();
+    }
+    bool hit(const Ray& ray, double tmin, double tmax, Hit& h, Counters* cnt) const override {  // :46-88
+        if (cnt) cnt->prim++;
+        double den = n.dot(ray.d);
+        if (std::fabs(den) < 1e-6) return false;
+        double t = n.dot(c - ray.o) / den;
+        if (t < tmin || t > tmax) return false;
+        Vec3 p = ray.at(t);
+        Vec3 cp = p - c;
+        if (cp.length2() > r * r) return false;
+        h.t = t;
+        h.p = p;
+        h.uv = {(cp.dot(right) / r + 1.0) / 2.0, (cp.dot(up) / r + 1.0) / 2.0};
+        h.mat = mat;
+        h.prim = id;
+        h.set_face_normal(ray, n);
+        return true;
+    }
+    AABB bbox() const override {  // BoundingBox :91-118
+        Vec3 re = right * r, ue = up * r;
+        Vec3 cs[4] = {c + re + ue, c + re - ue, c - re + ue, c - re - ue};
+        return aabb_from_points(cs, 4);
+    }
+    void sample_uniform(Vec2 s, Vec3& p, Vec3& nn) const {  // SampleUniform :121-134
+        double rr = std::sqrt(s.x) * r;
+        double th = 2.0 * kPi * s.y;
+        p = c + right * (rr * std::cos(th)) + up * (rr * std::sin(th));
+        nn = n;
+    }
+};
+
 // ---- Box, pkg/geometry/box.go:34-140 ----------------------------------------------
 struct Box : Shape {
     std::unique_ptr<Quad> faces[6];
@@ -561,10 +603,12 @@ struct Camera {
 };
 
 // ---- materials, pkg/material ----------------------------------------------------------
+static const int kMatSpotEmitter = 5;  // discSpotLightMaterial, disc_spot_light.go:12-46
 struct Material {
     int type = 0, outer = -1, inner = -1;
     Vec3 albedo, emission;
     double fuzz = 0, ior = 1;
+    Vec3 spot_dir;  // kMatSpotEmitter: the spot direction
 };
 struct Scatter {
     Ray in, out;
@@ -620,6 +664,10 @@ struct Light {
     Vec3 world_center;
     double world_radius = 0;
     std::shared_ptr<EnvMap> env;
+    // disc lights (disc_light.go), spot lights (disc_spot_light.go, point_spot_light.go)
+    std::unique_ptr<Disc> disc;
+    Vec3 from, spot_dir;
+    double cos_total = 0, cos_falloff = 0;
 };
 
 struct Scene {
@@ -633,11 +681,17 @@ struct Scene {
     int64_t num_prims = 0;
 };
 
-static Vec3 emissive_emit(const Scene& S, int m, const Hit* h) {  // emissive.go:30-37
+// Emitter.Emit for the ray direction d: emissive.go:30-37; the disc spot light's material additionally
+// emits only against its spot direction (rayIn.Direction . spot <= -0.3, disc_spot_light.go:26-40)
+static Vec3 emissive_emit(const Scene& S, int m, const Hit* h, Vec3 d) {
     if (h && !h->front) return {0, 0, 0};
-    return S.mats[m].emission;
+    const Material& M = S.mats[m];
+    if (M.type == kMatSpotEmitter && d.dot(M.spot_dir) > -0.3) return {0, 0, 0};
+    return M.emission;
 }
-static bool is_emitter(const Scene& S, int m) { return m >= 0 && S.mats[m].type == PRT_MAT_EMISSIVE; }
+static bool is_emitter(const Scene& S, int m) {
+    return m >= 0 && (S.mats[m].type == PRT_MAT_EMISSIVE || S.mats[m].type == kMatSpotEmitter);
+}
 
 static bool mat_scatter(const Scene& S, int m, const Ray& in, const Hit& h, Sampler& smp, Scatter& out) {
     const Material& M = S.mats[m];
@@ -677,7 +731,8 @@ static bool mat_scatter(const Scene& S, int m, const Ray& in, const Hit& h, Samp
             return true;
         }
         case PRT_MAT_EMISSIVE:
-            return false;  // emissive.go:24-27
+        case kMatSpotEmitter:
+            return false;  // emissive.go:24-27, disc_spot_light.go:21-23
         case PRT_MAT_LAYERED: {  // layered.go:24-69
             Hit oh = h;
             oh.mat = M.outer;
@@ -746,6 +801,7 @@ static Vec3 mat_brdf(const Scene& S, int m, Vec3 in, Vec3 outd, const Hit& h, in
             return {0, 0, 0};
         }
         case PRT_MAT_EMISSIVE:
+        case kMatSpotEmitter:
             return {0, 0, 0};
         case PRT_MAT_LAYERED:  // layered.go:73-87
             if (is_reflection_path(in, outd, h.n)) return mat_brdf(S, M.outer, in, outd, h, mode);
@@ -768,6 +824,7 @@ static double mat_pdf(const Scene& S, int m, Vec3 in, Vec3 outd, Vec3 n, bool& d
             delta = true;
             return 0;
         case PRT_MAT_EMISSIVE:
+        case kMatSpotEmitter:
             delta = false;
             return 0;
         case PRT_MAT_LAYERED: {  // layered.go:90-104
@@ -868,13 +925,84 @@ static LightSample sphere_sample_uniform(const Scene& S, const Light& L, Vec3 p,
     ls.dist = dir.length();
     ls.dir = dir.normalize();
     ls.pdf = 1.0 / (4.0 * kPi * sp.r * sp.r);
-    ls.emission = emissive_emit(S, L.mat, nullptr);
+    ls.emission = emissive_emit(S, L.mat, nullptr, ls.dir);
+    return ls;
+}
+
+// falloff, point_spot_light.go:121-139 and disc_spot_light.go (same curve)
+static double spot_falloff(const Light& L, double cos_a) {
+    if (cos_a < L.cos_total) return 0;
+    if (cos_a >= L.cos_falloff) return 1;
+    double d = (cos_a - L.cos_total) / (L.cos_falloff - L.cos_total);
+    return d * d * d * d;
+}
+static Vec3 sample_cone(Vec3 w, double cos_total, Vec2 s) {  // SampleCone, core/sampling.go:70-94
+    Vec3 u = std::fabs(w.x) > 0.1 ? Vec3(0, 1, 0) : Vec3(1, 0, 0);
+    u = u.cross(w).normalize();
+    Vec3 v = w.cross(u);
+    double ct = 1.0 - s.x * (1.0 - cos_total);
+    double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+    double phi = 2.0 * kPi * s.y;
+    return u * (st * std::cos(phi)) + v * (st * std::sin(phi)) + w * ct;
+}
+static double uniform_cone_pdf(double cos_total) { return 1.0 / (2.0 * kPi * (1.0 - cos_total)); }  // :83-85
+
+// DiscLight.Sample, disc_light.go:31-79 (no front-face test: Emit is called without a hit record)
+static LightSample disc_light_sample(const Scene& S, const Light& L, Vec3 p, Vec2 s) {
+    LightSample ls;
+    Vec3 sp, nn;
+    L.disc->sample_uniform(s, sp, nn);
+    Vec3 d = sp - p;
+    double dist = d.length();
+    Vec3 dn = d.normalize();
+    ls.p = sp;
+    ls.n = nn;
+    if (dist == 0) {
+        ls.dir = {0, 1, 0};
+        ls.dist = 0;
+        ls.pdf = 1.0;
+        return ls;
+    }
+    double pdf = 1.0 / (kPi * L.disc->r * L.disc->r);
+    double ct = std::fabs(nn.dot(dn * -1.0));
+    ls.pdf = ct < 1e-6 ? 0.0 : pdf * dist * dist / ct;
+    ls.dir = dn;
+    ls.dist = dist;
+    ls.emission = emissive_emit(S, L.mat, nullptr, dn);
     return ls;
 }
 
 static LightSample light_sample(const Scene& S, const Light& L, Vec3 p, Vec3 n, Vec2 s) {
     LightSample ls;
     switch (L.type) {
+        case PRT_LIGHT_DISC:
+            return disc_light_sample(S, L, p, s);
+        case PRT_LIGHT_DISC_SPOT: {  // DiscSpotLight.Sample, disc_spot_light.go:105-116
+            ls = disc_light_sample(S, L, p, s);
+            Vec3 lp = (p - ls.p).normalize();
+            ls.emission = ls.emission * spot_falloff(L, L.spot_dir.dot(lp));
+            return ls;
+        }
+        case PRT_LIGHT_POINT_SPOT: {  // PointSpotLight.Sample, point_spot_light.go:47-92
+            Vec3 tl = L.from - p;
+            double dist = tl.length();
+            ls.p = L.from;
+            if (dist == 0) {
+                ls.n = {0, 1, 0};
+                ls.dir = {0, 1, 0};
+                ls.dist = 0;
+                ls.pdf = 1.0;
+                return ls;
+            }
+            Vec3 to_light = tl.normalize();
+            double att = spot_falloff(L, L.spot_dir.dot(to_light * -1.0));
+            ls.emission = L.emission * (att / (dist * dist));
+            ls.pdf = 1.0;
+            ls.n = to_light * -1.0;
+            ls.dir = to_light;
+            ls.dist = dist;
+            return ls;
+        }
         case PRT_LIGHT_QUAD: {  // quad_light.go:36-84
             const Quad& q = *L.quad;
             Vec3 sp = q.corner + q.u * s.x + q.v * s.y;
@@ -894,7 +1022,7 @@ static LightSample light_sample(const Scene& S, const Light& L, Vec3 p, Vec3 n, 
             }
             ls.pdf = pdf * dist * dist / ct;
             bool front = dir.dot(q.normal) < 0;
-            ls.emission = front ? emissive_emit(S, L.mat, nullptr) : Vec3(0, 0, 0);
+            ls.emission = front ? emissive_emit(S, L.mat, nullptr, dir) : Vec3(0, 0, 0);
             return ls;
         }
         case PRT_LIGHT_SPHERE: {  // sphere_light.go:37-141
@@ -920,7 +1048,7 @@ static LightSample light_sample(const Scene& S, const Light& L, Vec3 p, Vec3 n, 
             ls.dir = dir;
             ls.dist = h.t;
             ls.pdf = 1.0 / (2.0 * kPi * (1.0 - ctm));
-            ls.emission = emissive_emit(S, L.mat, nullptr);
+            ls.emission = emissive_emit(S, L.mat, nullptr, dir);
             return ls;
         }
         case PRT_LIGHT_UNIFORM_INFINITE:
@@ -956,6 +1084,21 @@ static LightSample light_sample(const Scene& S, const Light& L, Vec3 p, Vec3 n, 
 
 static double light_pdf(const Scene& S, const Light& L, Vec3 p, Vec3 n, Vec3 dir) {
     switch (L.type) {
+        case PRT_LIGHT_DISC:
+        case PRT_LIGHT_DISC_SPOT: {  // DiscLight.PDF, disc_light.go:82-101
+            Ray r{p, dir};
+            Hit h;
+            if (!L.disc->hit(r, 0.001, kInf, h, nullptr)) return 0;
+            double area = 1.0 / (kPi * L.disc->r * L.disc->r);
+            double ct = std::fabs(L.disc->n.dot(dir * -1.0));
+            if (ct < 1e-6) return 0;
+            return area * h.t * h.t / ct;
+        }
+        case PRT_LIGHT_POINT_SPOT: {  // point_spot_light.go:96-118: 1 within 0.999 of the light, else 0
+            Vec3 tl = L.from - p;
+            if (tl.length() == 0) return 0;
+            return dir.dot(tl.normalize()) > 0.999 ? 1.0 : 0.0;
+        }
         case PRT_LIGHT_QUAD: {  // quad_light.go:87-104
             Ray r{p, dir};
             Hit h;
@@ -1060,7 +1203,7 @@ static Vec3 ray_color(const Scene& S, const PTConfig& cfg, const Ray& ray, Sampl
             if (light_is_infinite(L)) e = e + light_emit(S, L, ray);
         return e * rrc;
     }
-    Vec3 le = is_emitter(S, h.mat) ? emissive_emit(S, h.mat, &h) : Vec3(0, 0, 0);  // getEmittedLight :100-105
+    Vec3 le = is_emitter(S, h.mat) ? emissive_emit(S, h.mat, &h, ray.d) : Vec3(0, 0, 0);  // getEmittedLight :100-105
     Scatter sc;
     if (!mat_scatter(S, h.mat, ray, h, smp, sc)) return le * rrc;
     Vec3 cs;
@@ -1183,7 +1326,11 @@ static CamSample cam_sample_from_point(const Camera& c, Vec3 ref, Vec2 s) {
     return cs;
 }
 
-static bool light_is_area(const Light& L) { return L.type == PRT_LIGHT_QUAD || L.type == PRT_LIGHT_SPHERE; }
+static bool light_is_area(const Light& L) {  // LightTypeArea lights
+    return L.type == PRT_LIGHT_QUAD || L.type == PRT_LIGHT_SPHERE || L.type == PRT_LIGHT_DISC ||
+           L.type == PRT_LIGHT_DISC_SPOT;
+}
+static bool light_is_delta(const Light& L) { return L.type == PRT_LIGHT_POINT_SPOT; }  // LightTypePoint
 
 struct EmissionSample {
     Vec3 p, n, dir, emission;
@@ -1205,6 +1352,31 @@ static void infinite_disk(const Light& L, Vec3 direction, Vec2 sp, EmissionSampl
 static EmissionSample light_sample_emission(const Scene& S, const Light& L, Vec2 sp, Vec2 sd) {
     EmissionSample es;
     switch (L.type) {
+        case PRT_LIGHT_DISC: {  // disc_light.go:104-111 + SampleEmissionDirection light_sampling.go:57-80
+            L.disc->sample_uniform(sp, es.p, es.n);
+            es.area_pdf = 1.0 / (kPi * L.disc->r * L.disc->r);
+            es.dir = sample_cosine_hemisphere(es.n, sd);
+            es.dir_pdf = es.dir.dot(es.n) / kPi;
+            es.emission = emissive_emit(S, L.mat, nullptr, es.dir);
+            return es;
+        }
+        case PRT_LIGHT_DISC_SPOT: {  // disc_spot_light.go:170-192
+            L.disc->sample_uniform(sp, es.p, es.n);
+            es.dir = sample_cone(L.spot_dir, L.cos_total, sd);
+            es.emission = L.emission * spot_falloff(L, es.dir.dot(L.spot_dir));
+            es.area_pdf = 1.0 / (kPi * L.disc->r * L.disc->r);
+            es.dir_pdf = uniform_cone_pdf(L.cos_total);
+            return es;
+        }
+        case PRT_LIGHT_POINT_SPOT: {  // point_spot_light.go:153-182
+            es.p = L.from;
+            es.dir = sample_cone(L.spot_dir, L.cos_total, sd);
+            es.emission = L.emission * spot_falloff(L, es.dir.dot(L.spot_dir));
+            es.n = es.dir;
+            es.area_pdf = 1.0;
+            es.dir_pdf = uniform_cone_pdf(L.cos_total);
+            return es;
+        }
         case PRT_LIGHT_QUAD: {  // quad_light.go:107-126
             const Quad& q = *L.quad;
             es.p = q.corner + q.u * sp.x + q.v * sp.y;
@@ -1212,7 +1384,7 @@ static EmissionSample light_sample_emission(const Scene& S, const Light& L, Vec2
             es.dir = sample_cosine_hemisphere(q.normal, sd);
             es.area_pdf = 1.0 / L.area;
             es.dir_pdf = es.dir.dot(q.normal) / kPi;
-            es.emission = emissive_emit(S, L.mat, nullptr);
+            es.emission = emissive_emit(S, L.mat, nullptr, es.dir);
             return es;
         }
         case PRT_LIGHT_SPHERE: {  // sphere_light.go:168-181 + SampleEmissionDirection light_sampling.go:54-80
@@ -1226,7 +1398,7 @@ static EmissionSample light_sample_emission(const Scene& S, const Light& L, Vec2
             es.area_pdf = 1.0 / (4.0 * kPi * s.r * s.r);
             es.dir = sample_cosine_hemisphere(ld, sd);
             es.dir_pdf = es.dir.dot(ld) / kPi;
-            es.emission = is_emitter(S, L.mat) ? emissive_emit(S, L.mat, nullptr) : Vec3(0, 0, 0);
+            es.emission = is_emitter(S, L.mat) ? emissive_emit(S, L.mat, nullptr, es.dir) : Vec3(0, 0, 0);
             return es;
         }
         case PRT_LIGHT_UNIFORM_INFINITE:
@@ -1253,9 +1425,30 @@ static EmissionSample light_sample_emission(const Scene& S, const Light& L, Vec2
     return es;
 }
 // Light.PDF_Le (interfaces.go:35-39)
+static bool on_disc(const Disc& dc, Vec3 p, double tol) {  // validatePointOnDisc, disc_light.go:142-155
+    Vec3 tp = p - dc.c;
+    if (std::fabs(tp.dot(dc.n)) > tol) return false;
+    return (tp - dc.n * tp.dot(dc.n)).length() <= dc.r;
+}
 static void light_pdf_le(const Light& L, Vec3 p, Vec3 d, double& pos, double& dir) {
     pos = dir = 0;
     switch (L.type) {
+        case PRT_LIGHT_DISC:
+        case PRT_LIGHT_DISC_SPOT: {  // disc_light.go:114-133, disc_spot_light.go:195-210
+            if (!on_disc(*L.disc, p, 0.001)) return;
+            double ct = d.dot(L.disc->n);
+            if (ct <= 0) return;
+            pos = 1.0 / (kPi * L.disc->r * L.disc->r);
+            dir = ct / kPi;
+            if (L.type == PRT_LIGHT_DISC_SPOT && d.dot(L.spot_dir) < L.cos_total) dir = 0;
+            return;
+        }
+        case PRT_LIGHT_POINT_SPOT:  // point_spot_light.go:185-207
+            if ((p - L.from).length() > 0.001) return;
+            if (d.dot(L.spot_dir) < L.cos_total) return;
+            pos = 1.0;
+            dir = uniform_cone_pdf(L.cos_total);
+            return;
         case PRT_LIGHT_QUAD: {  // quad_light.go:129-176
             const Quad& q = *L.quad;
             Vec3 tp = p - q.corner;
@@ -1378,7 +1571,7 @@ static void extend_path(const BDPTCtx& C, BPath& path, Ray ray, Vec3 beta, doubl
         v.mat = h.mat;
         v.wi = ray.d * -1.0;
         v.beta = beta;
-        v.Le = is_emitter(S, h.mat) ? emissive_emit(S, h.mat, &h) : Vec3(0, 0, 0);
+        v.Le = is_emitter(S, h.mat) ? emissive_emit(S, h.mat, &h, ray.d) : Vec3(0, 0, 0);
         v.is_light = !v.Le.is_zero();
         v.pdf_fwd = convert_density(path.v[prev_i], v, pdf_fwd);
         smp.set_slot(slot0 + (uint32_t)b);
@@ -1577,8 +1770,10 @@ static double mis_weight(const Scene& S, const BPath& cp, const BPath& lp, const
     for (int i = s - 1; i >= 0; i--) {
         const BVertex& v = lp.v[i];
         double fwd = v.pdf_fwd, rev = v.pdf_rev;
-        bool conn = !v.is_specular;  // point (delta) lights are out of scope here
-        if (i > 0) conn = conn && !lp.v[i - 1].is_specular;
+        // light vertices: connectible if not specular and not on a delta (point) light (bdpt_mis.go:129-139)
+        auto delta = [](const BVertex& x) { return x.is_light && x.light && light_is_delta(*x.light); };
+        bool conn = !v.is_specular && !delta(v);
+        if (i > 0) conn = conn && !lp.v[i - 1].is_specular && !delta(lp.v[i - 1]);
         if (s == 0) {
         } else if (t == 1) {
             if (i == s - 1 && s > 1) {
@@ -1763,6 +1958,24 @@ orc_scene* orc_scene_create(const prt_scene_desc* d) {
         S.owned.emplace_back(new Box(V(b.center), V(b.size), V(b.rotation), b.material, id));
         id += 6;
     }
+    for (int i = 0; i < d->num_discs; i++) {
+        const prt_disc& dc = d->discs[i];
+        S.owned.emplace_back(new Disc(V(dc.center), V(dc.normal), dc.radius, dc.material, id++));
+    }
+    // disc spot lights: an internal emitting material and their disc join the shapes (scene.go:117-122)
+    std::vector<int> spot_mat(d->num_lights, -1);
+    for (int i = 0; i < d->num_lights; i++) {
+        const prt_light& pl = d->lights[i];
+        if (pl.type != PRT_LIGHT_DISC_SPOT) continue;
+        Vec3 dir = (V(pl.to) - V(pl.from)).normalize();
+        Material m;
+        m.type = kMatSpotEmitter;
+        m.emission = V(pl.emission);
+        m.spot_dir = dir;
+        spot_mat[i] = (int)S.mats.size();
+        S.mats.push_back(m);
+        S.owned.emplace_back(new Disc(V(pl.from), dir, pl.radius, spot_mat[i], id++));
+    }
     S.num_prims = id;
     for (auto& o : S.owned) S.shapes.push_back(o.get());
     S.bvh.reset(new BVH(S.shapes));  // Scene.Preprocess scene.go:45-48
@@ -1788,6 +2001,20 @@ orc_scene* orc_scene_create(const prt_scene_desc* d) {
         } else if (pl.type == PRT_LIGHT_ENVMAP) {
             L.env = std::make_shared<EnvMap>();
             build_env(*L.env, pl);
+        } else if (pl.type == PRT_LIGHT_DISC) {
+            const prt_disc& dc = d->discs[pl.shape];
+            L.disc.reset(new Disc(V(dc.center), V(dc.normal), dc.radius, dc.material, -1));
+            L.mat = dc.material;
+        } else if (pl.type == PRT_LIGHT_DISC_SPOT || pl.type == PRT_LIGHT_POINT_SPOT) {
+            // NewDiscSpotLight / NewPointSpotLight: direction from -> to, cosines of the cone angles
+            L.from = V(pl.from);
+            L.spot_dir = (V(pl.to) - V(pl.from)).normalize();
+            L.cos_total = std::cos(pl.cone_angle * kPi / 180.0);
+            L.cos_falloff = std::cos((pl.cone_angle - pl.cone_delta) * kPi / 180.0);
+            if (pl.type == PRT_LIGHT_DISC_SPOT) {
+                L.disc.reset(new Disc(L.from, L.spot_dir, pl.radius, spot_mat[i], -1));
+                L.mat = spot_mat[i];
+            }
         }
         S.lights.push_back(std::move(L));
     }
@@ -2060,7 +2287,8 @@ void orc_sample_point_in_unit_sphere(const double* s, double* out) {
 }
 double orc_power_heuristic(int32_t nf, double f, int32_t ng, double g) { return power_heuristic(nf, f, ng, g); }
 double orc_reflectance(double c, double r) { return reflectance(c, r); }
-// Single-shape intersection: kind 0 triangle (9 doubles), 1 sphere (4), 2 quad (9).
+// Single-shape intersection: kind 0 triangle (9 doubles), 1 sphere (4), 2 quad (9),
+// 3 disc (centre 3, normal 3, radius).
 // out: t, p(3), n(3), front, u, v.  Returns 1 on hit.
 int32_t orc_shape_hit(int32_t kind, const double* g, const double* o, const double* d, double tmin,
                       double tmax, double* out) {
@@ -2070,6 +2298,7 @@ int32_t orc_shape_hit(int32_t kind, const double* g, const double* o, const doub
     if (kind == 0) hit = Triangle(V(g), V(g + 3), V(g + 6), 0, 0).hit(r, tmin, tmax, h, nullptr);
     else if (kind == 1) hit = Sphere(V(g), g[3], 0, 0).hit(r, tmin, tmax, h, nullptr);
     else if (kind == 2) hit = Quad(V(g), V(g + 3), V(g + 6), 0, 0).hit(r, tmin, tmax, h, nullptr);
+    else if (kind == 3) hit = Disc(V(g), V(g + 3), g[6], 0, 0).hit(r, tmin, tmax, h, nullptr);
     if (!hit) return 0;
     out[0] = h.t;
     out[1] = h.p.x; out[2] = h.p.y; out[3] = h.p.z;
@@ -2077,6 +2306,14 @@ int32_t orc_shape_hit(int32_t kind, const double* g, const double* o, const doub
     out[7] = h.front ? 1 : 0;
     out[8] = h.uv.x; out[9] = h.uv.y;
     return 1;
+}
+// Disc frame and box: out = right(3), up(3), box min(3), box max(3).
+void orc_disc_frame(const double* g, double* out) {
+    Disc dc(V(g), V(g + 3), g[6], 0, 0);
+    AABB b = dc.bbox();
+    out[0] = dc.right.x; out[1] = dc.right.y; out[2] = dc.right.z;
+    out[3] = dc.up.x; out[4] = dc.up.y; out[5] = dc.up.z;
+    out[6] = b.mn.x; out[7] = b.mn.y; out[8] = b.mn.z; out[9] = b.mx.x; out[10] = b.mx.y; out[11] = b.mx.z;
 }
 void orc_quad_bbox(const double* g, double* out) {
     AABB b = Quad(V(g), V(g + 3), V(g + 6), 0, 0).bbox();
@@ -2103,6 +2340,22 @@ void orc_light_sample(const orc_scene* os, int32_t li, const double* p, const do
 }
 double orc_light_pdf(const orc_scene* os, int32_t li, const double* p, const double* n, const double* d) {
     return light_pdf(os->s, os->s.lights[li], V(p), V(n), V(d));
+}
+// SampleEmission (out: p3, n3, dir3, emission3, area_pdf, dir_pdf) and PDF_Le (out: pos, dir).
+void orc_light_sample_emission(const orc_scene* os, int32_t li, const double* sp, const double* sd, double* out) {
+    EmissionSample es = light_sample_emission(os->s, os->s.lights[li], Vec2{sp[0], sp[1]}, Vec2{sd[0], sd[1]});
+    const Vec3* v[4] = {&es.p, &es.n, &es.dir, &es.emission};
+    for (int k = 0; k < 4; k++) {
+        out[3 * k] = v[k]->x; out[3 * k + 1] = v[k]->y; out[3 * k + 2] = v[k]->z;
+    }
+    out[12] = es.area_pdf;
+    out[13] = es.dir_pdf;
+}
+void orc_light_pdf_le(const orc_scene* os, int32_t li, const double* p, const double* d, double* out) {
+    double pos = 0, dir = 0;
+    light_pdf_le(os->s.lights[li], V(p), V(d), pos, dir);
+    out[0] = pos;
+    out[1] = dir;
 }
 // Material queries: scatter with explicit stream key (returns 1 if scattered;
 // out: dir3, att3, pdf), BRDF (mode 0 radiance / 1 importance), PDF (out delta).

@@ -47,6 +47,9 @@ def lib():
             "orc_reflectance": (d, [d, d]),
             "orc_shape_hit": (i32, [i32, vp, vp, vp, d, d, vp]),
             "orc_quad_bbox": (None, [vp, vp]),
+            "orc_disc_frame": (None, [vp, vp]),
+            "orc_light_sample_emission": (None, [vp, i32, vp, vp, vp]),
+            "orc_light_pdf_le": (None, [vp, i32, vp, vp, vp]),
             "orc_aabb_hit": (i32, [vp, vp, vp, d, d]),
             "orc_camera_get_ray": (None, [vp, i32, i32, vp, vp, vp, vp]),
             "orc_camera_height": (i32, [vp]),
