@@ -59,8 +59,15 @@ constexpr uint32_t kPrimTriangle = 0u;
 constexpr uint32_t kPrimSphere = 1u;
 constexpr uint32_t kPrimQuad = 2u;
 constexpr uint32_t kPrimDisc = 3u;  // a = (centre, tag), b = (unit normal, material), c = (radius, r^2, 0, id)
-PRT_HD uint32_t prim_kind(uint32_t tag) { return tag >> 30; }
-PRT_HD uint32_t prim_side(uint32_t tag) { return tag & 0x3FFFFFFFu; }
+// a = (base centre, tag with side = capped), b = (unit axis, material), c = (radius, height, 0, id)
+constexpr uint32_t kPrimCylinder = 4u;
+// a = (base centre, tag with side = capped), b = (unit axis, material), c = (base r, top r, height, id)
+constexpr uint32_t kPrimCone = 5u;
+constexpr uint32_t kKindShift = 29u;
+constexpr uint32_t kSideMask = (1u << kKindShift) - 1u;
+PRT_HD uint32_t prim_tag(uint32_t kind, uint32_t side) { return (kind << kKindShift) | side; }
+PRT_HD uint32_t prim_kind(uint32_t tag) { return tag >> kKindShift; }
+PRT_HD uint32_t prim_side(uint32_t tag) { return tag & kSideMask; }
 
 // Quad side data (float32 copies of float64 precomputations, NewQuad quad.go:98-121).
 struct QuadExt {

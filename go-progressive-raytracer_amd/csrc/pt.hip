@@ -348,6 +348,8 @@ __device__ double prim_t_f64(const DevScene& sc, uint32_t prim, V3 o_hi, const f
     } else if (kind == kPrimDisc) {
         nx = b.x, ny = b.y, nz = b.z;
         px = a.x, py = a.y, pz = a.z;
+    } else if (kind != kPrimSphere) {  // cylinders and cones keep their float32 distance
+        return (double)t32;
     } else {  // sphere: the root of |o + t d - c|^2 = r^2 nearest t32
         double fx = ox - a.x, fy = oy - a.y, fz = oz - a.z;
         double A = dx * dx + dy * dy + dz * dz, hb = fx * dx + fy * dy + fz * dz;

@@ -684,6 +684,35 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         if (!(len(n) > 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "disc normal must be non-zero"));
         discs.push_back(make_disc(d3(dc.center), n, dc.radius, dc.material));
     }
+    // cylinders and cones (NewCylinder cylinder.go:24-39, NewCone cone.go:27-80 with its checks)
+    struct AxialF64 {
+        D3 base, top, axis;
+        double rb, rt, height;
+        bool capped, cone;
+        int mat;
+    };
+    std::vector<AxialF64> axials;
+    for (int i = 0; i < d->num_cylinders; i++) {
+        const prt_cylinder& c = d->cylinders[i];
+        if (bad_mat(c.material)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "cylinder material out of range"));
+        D3 b = d3(c.base_center), t = d3(c.top_center);
+        double h = len(t - b);
+        if (!(c.radius > 0) || !(h > 0))
+            return bail(fail(PRT_ERR_INVALID_ARGUMENT, "cylinder needs a positive radius and height"));
+        axials.push_back({b, t, (t - b) * (1.0 / h), c.radius, c.radius, h, c.capped != 0, false, c.material});
+    }
+    for (int i = 0; i < d->num_cones; i++) {
+        const prt_cone& c = d->cones[i];
+        if (bad_mat(c.material)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "cone material out of range"));
+        D3 b = d3(c.base_center), t = d3(c.top_center);
+        double h = len(t - b);
+        if (!(c.base_radius > 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "cone base radius must be positive"));
+        if (!(c.top_radius >= 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "cone top radius must be non-negative"));
+        if (!(c.base_radius > c.top_radius))
+            return bail(fail(PRT_ERR_INVALID_ARGUMENT, "cone base radius must exceed its top radius"));
+        if (!(h > 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "cone height must be positive"));
+        axials.push_back({b, t, (t - b) * (1.0 / h), c.base_radius, c.top_radius, h, c.capped != 0, true, c.material});
+    }
     std::vector<int> spot_index(std::max(d->num_lights, 1), -1);  // light -> internal material nm + k
     int nspot = 0;
     for (int i = 0; i < d->num_lights; i++) {
@@ -708,7 +737,9 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     const int64_t nsph = d->num_spheres;
     const int64_t nq = (int64_t)quads.size();
     const int64_t ndisc = (int64_t)discs.size();
-    const int64_t nprim = ntri + nsph + nq + ndisc;
+    const int64_t nuser_disc = d->num_discs;
+    const int64_t naxial = (int64_t)axials.size();
+    const int64_t nprim = ntri + nsph + nq + ndisc + naxial;
     if (nprim >= (1ll << 28)) return bail(fail(PRT_ERR_UNSUPPORTED, "more than 2^28 primitives"));
 
     std::vector<PrimBounds> bounds(nprim);
@@ -782,10 +813,10 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
                 double l = std::sqrt(cn[0] * cn[0] + cn[1] * cn[1] + cn[2] * cn[2]);
                 double inv = l > 0 ? 1.0 / l : 0.0;
                 side = (uint32_t)trin.size();
-                if (side >= (1u << 30)) return bail(fail(PRT_ERR_UNSUPPORTED, "too many custom normals"));
+                if (side > kSideMask) return bail(fail(PRT_ERR_UNSUPPORTED, "too many custom normals"));
                 trin.push_back(make_float4((float)(cn[0] * inv), (float)(cn[1] * inv), (float)(cn[2] * inv), 0.f));
             }
-            rec[3 * gid + 0] = f4(p0[0], p0[1], p0[2], bits((kPrimTriangle << 30) | side));
+            rec[3 * gid + 0] = f4(p0[0], p0[1], p0[2], bits(prim_tag(kPrimTriangle, side)));
             rec[3 * gid + 1] = f4(p1[0], p1[1], p1[2], bits((uint32_t)mat));
             rec[3 * gid + 2] = f4(p2[0], p2[1], p2[2], bits((uint32_t)gid));
         }
@@ -808,7 +839,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         }
         grow(world, ref);
         world_any = true;
-        rec[3 * gid + 0] = f4(sp.center[0], sp.center[1], sp.center[2], bits((kPrimSphere << 30) | (uint32_t)i));
+        rec[3 * gid + 0] = f4(sp.center[0], sp.center[1], sp.center[2], bits(prim_tag(kPrimSphere, (uint32_t)i)));
         rec[3 * gid + 1] = make_float4((float)r, 0.f, 0.f, bits((uint32_t)sp.material));
         rec[3 * gid + 2] = make_float4(0.f, 0.f, 0.f, bits((uint32_t)gid));
     }
@@ -824,7 +855,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         double D = dot(n, q.corner);
         D3 w = n * (1.0 / dot(n, cross(q.u, q.v)));
         qext[i] = QuadExt{(float)n.x, (float)n.y, (float)n.z, (float)D, (float)w.x, (float)w.y, (float)w.z, 0.f};
-        rec[3 * gid + 0] = f4(q.corner.x, q.corner.y, q.corner.z, bits((kPrimQuad << 30) | (uint32_t)i));
+        rec[3 * gid + 0] = f4(q.corner.x, q.corner.y, q.corner.z, bits(prim_tag(kPrimQuad, (uint32_t)i)));
         rec[3 * gid + 1] = f4(q.u.x, q.u.y, q.u.z, bits((uint32_t)q.mat));
         rec[3 * gid + 2] = f4(q.v.x, q.v.y, q.v.z, bits((uint32_t)gid));
     }
@@ -832,8 +863,8 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         grow(world, wb);
         world_any = true;
     }
-    for (int64_t i = 0; i < ndisc; i++, gid++) {
-        const DiscF64& dc = discs[i];
+    // user discs, cylinders, cones, then the discs of disc spot lights
+    auto put_disc = [&](const DiscF64& dc) {
         PrimBounds ref = disc_bounds(dc);
         grow(world, ref);
         world_any = true;
@@ -845,10 +876,39 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
             b.lo[a] = ref.lo[a] - pad;
             b.hi[a] = ref.hi[a] + pad;
         }
-        rec[3 * gid + 0] = f4(dc.c.x, dc.c.y, dc.c.z, bits(kPrimDisc << 30));
+        rec[3 * gid + 0] = f4(dc.c.x, dc.c.y, dc.c.z, bits(prim_tag(kPrimDisc, 0)));
         rec[3 * gid + 1] = f4(dc.n.x, dc.n.y, dc.n.z, bits((uint32_t)dc.mat));
         rec[3 * gid + 2] = make_float4((float)dc.r, (float)(dc.r * dc.r), 0.f, bits((uint32_t)gid));
+        gid++;
+    };
+    for (int64_t i = 0; i < nuser_disc; i++) put_disc(discs[i]);
+    for (const AxialF64& c : axials) {
+        // world box: Cylinder/Cone.BoundingBox, the base-top segment box widened by the (base)
+        // radius on every axis the axis is not parallel to (cylinder.go:41-88, cone.go:83-137)
+        PrimBounds ref, &b = bounds[gid];
+        for (int a = 0; a < 3; a++) {
+            double lo = std::min(comp(c.base, a), comp(c.top, a)), hi = std::max(comp(c.base, a), comp(c.top, a));
+            double ext = std::fabs(comp(c.axis, a)) > 0.9999 ? 0.0 : c.rb;
+            ref.lo[a] = lo - ext;
+            ref.hi[a] = hi + ext;
+            // traversal box: the exact box of the two end discs (a disc of radius r about unit
+            // axis v spans r * sqrt(1 - v_a^2) along axis a), padded
+            double s = std::sqrt(std::max(0.0, 1.0 - comp(c.axis, a) * comp(c.axis, a)));
+            double eb = c.rb * s, et = c.rt * s;
+            double pad = 1e-5 * c.rb + 1e-6 * std::max(std::fabs(comp(c.base, a)), std::fabs(comp(c.top, a)));
+            b.lo[a] = std::min(comp(c.base, a) - eb, comp(c.top, a) - et) - pad;
+            b.hi[a] = std::max(comp(c.base, a) + eb, comp(c.top, a) + et) + pad;
+        }
+        grow(world, ref);
+        world_any = true;
+        const uint32_t kind = c.cone ? kPrimCone : kPrimCylinder;
+        rec[3 * gid + 0] = f4(c.base.x, c.base.y, c.base.z, bits(prim_tag(kind, c.capped ? 1u : 0u)));
+        rec[3 * gid + 1] = f4(c.axis.x, c.axis.y, c.axis.z, bits((uint32_t)c.mat));
+        rec[3 * gid + 2] = c.cone ? make_float4((float)c.rb, (float)c.rt, (float)c.height, bits((uint32_t)gid))
+                                  : make_float4((float)c.rb, (float)c.height, 0.f, bits((uint32_t)gid));
+        gid++;
     }
+    for (int64_t i = nuser_disc; i < ndisc; i++) put_disc(discs[i]);
     // world centre / radius of the reference BVH root box (bvh.go:38-48)
     if (world_any) {
         for (int a = 0; a < 3; a++) s->info.world_center[a] = 0.5 * (world.lo[a] + world.hi[a]);
@@ -890,7 +950,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     s->dev.quads = (const QuadExt*)p;
     if ((rc = scene_alloc(s, trin.size() * sizeof(float4), trin.data(), &p)) != PRT_OK) return bail(rc);
     s->dev.tri_normals = (const float4*)p;
-    s->dev.mixed = (nsph + nq + ndisc) > 0 ? 1 : 0;
+    s->dev.mixed = (nsph + nq + ndisc + naxial) > 0 ? 1 : 0;
     s->dev.max_stack = bvh.max_stack;
     s->dev.num_nodes = (int32_t)bvh.nodes.size();
 

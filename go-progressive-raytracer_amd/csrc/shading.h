@@ -117,9 +117,34 @@ __device__ __forceinline__ Surf make_surf(const DevScene& sc, V3 o, V3 d, const 
         const QuadExt& q = sc.quads[prim_side(tag)];
         outward = v3(q.nx, q.ny, q.nz);
         s.p = s.p - outward * (dot(s.p, outward) - q.D);
-    } else {  // disc: SetFaceNormal with the disc normal (disc.go:85)
+    } else if (kind == kPrimDisc) {  // SetFaceNormal with the disc normal (disc.go:85)
         outward = v3(b.x, b.y, b.z);
         s.p = s.p - outward * dot(s.p - v3(a.x, a.y, a.z), outward);
+    } else {  // cylinder or cone: h.u holds the part hit (0 body, 1 base cap, 2 top cap)
+        const float4 c = pp[2];
+        const V3 base = v3(a.x, a.y, a.z), axis = v3(b.x, b.y, b.z);
+        const bool cone = kind == kPrimCone;
+        const float H = cone ? c.z : c.y;
+        const int part = (int)h.u;
+        if (part == 0) {
+            float hh = dot(s.p - base, axis);
+            V3 centre = base + axis * hh;
+            V3 radial = s.p - centre;
+            float rl = length(radial);
+            if (!cone) {  // radial normal (cylinder.go:181-184)
+                outward = rl > 0.0f ? radial * (1.0f / rl) : radial;
+                s.p = centre + outward * c.x;
+            } else {  // normalize(radial + axis (rb - rt) / H) with the unnormalized radial (cone.go:221-229)
+                float rs = c.x - hh * ((c.x - c.y) / H);  // the surface's radius at this height
+                if (rl > 0.0f) radial = radial * (rs / rl);
+                s.p = centre + radial;
+                outward = normalize(radial + axis * ((c.x - c.y) / H));
+            }
+        } else {
+            outward = part == 1 ? axis * -1.0f : axis;
+            V3 cc = part == 1 ? base : base + axis * H;
+            s.p = s.p - axis * dot(s.p - cc, axis);
+        }
     }
     s.front = dot(d, outward) < 0.0f;
     s.n = s.front ? outward : outward * -1.0f;

@@ -168,6 +168,138 @@ __device__ __forceinline__ bool disc_test(const TRay& r, float4 a, float4 b, flo
     return true;
 }
 
+// End-cap disc of a cylinder or cone (hitCap, cylinder.go:213-262, cone.go:299-349): plane through
+// (cx,cy,cz) with normal +-(vx,vy,vz), |d.n| >= 1e-8, t in [tmin, tmax], |p - c| <= rad.
+__device__ __forceinline__ bool cap_test(const TRay& r, float cx, float cy, float cz, float vx, float vy, float vz,
+                                         float rad, float tmax, float& t_out) {
+    float den = vx * r.dx + vy * r.dy + vz * r.dz;
+    if (fabsf(den) < 1e-8f) return false;
+    float t = (vx * (cx - r.ox) + vy * (cy - r.oy) + vz * (cz - r.oz)) / den;
+    if (t < r.tmin || t > tmax) return false;
+    float px = r.ox + t * r.dx - cx, py = r.oy + t * r.dy - cy, pz = r.oz + t * r.dz - cz;
+    if (px * px + py * py + pz * pz > rad * rad) return false;
+    t_out = t;
+    return true;
+}
+
+// Cylinder (Cylinder.Hit, cylinder.go:91-210): the lateral surface's first root (near, then far)
+// with t in [tmin, tmax] and height in [0, H], then the base and top caps when capped, each
+// accepted at t <= the closest so far. The quadratic is solved with the perpendicular offset l of
+// the ray from the axis (discriminant a (r^2 - |l|^2)) and the cancellation-free root pair, so
+// distant origins keep their accuracy. part: 0 body, 1 base cap, 2 top cap.
+__device__ __forceinline__ bool cylinder_test(const TRay& r, float4 a, float4 b, float4 c, float tbest, float& t_out,
+                                              float& part) {
+    const float vx = b.x, vy = b.y, vz = b.z, rad = c.x, H = c.y;
+    float ex = r.ox - a.x, ey = r.oy - a.y, ez = r.oz - a.z;
+    float dv = r.dx * vx + r.dy * vy + r.dz * vz, ev = ex * vx + ey * vy + ez * vz;
+    float px = r.dx - dv * vx, py = r.dy - dv * vy, pz = r.dz - dv * vz;  // d perpendicular
+    float qx = ex - ev * vx, qy = ey - ev * vy, qz = ez - ev * vz;        // (o - base) perpendicular
+    float A = px * px + py * py + pz * pz;
+    bool hit = false;
+    float best = tbest;
+    if (fabsf(A) >= 1e-8f) {
+        float hb = qx * px + qy * py + qz * pz;
+        float s = hb / A;
+        float lx = qx - s * px, ly = qy - s * py, lz = qz - s * pz;
+        float disc = A * (rad * rad - (lx * lx + ly * ly + lz * lz));
+        if (disc >= 0.0f) {
+            float qq = -(hb + copysignf(sqrtf(disc), hb));
+            float C = qx * qx + qy * qy + qz * qz - rad * rad;
+            float r0 = qq / A, r1 = qq != 0.0f ? C / qq : r0;
+            float tn = fminf(r0, r1), tf = fmaxf(r0, r1);
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                float t = k == 0 ? tn : tf;
+                if (t < r.tmin || t > best) continue;
+                float h = ev + t * dv;
+                if (h < 0.0f || h > H) continue;
+                best = t;
+                part = 0.0f;
+                hit = true;
+                break;
+            }
+        }
+    }
+    if (prim_side(__float_as_uint(a.w))) {
+        float t;
+        if (cap_test(r, a.x, a.y, a.z, -vx, -vy, -vz, rad, best, t)) {
+            best = t;
+            part = 1.0f;
+            hit = true;
+        }
+        if (cap_test(r, a.x + vx * H, a.y + vy * H, a.z + vz * H, vx, vy, vz, rad, best, t)) {
+            best = t;
+            part = 2.0f;
+            hit = true;
+        }
+    }
+    t_out = best;
+    return hit;
+}
+
+// Cone or frustum (Cone.Hit, cone.go:140-296). The lateral quadratic about the apex is solved from
+// an origin moved along the ray to the point nearest the axis midpoint (coefficients of the
+// shape's size, not the origin's distance); like the reference, the root (-b - sqrt D) / 2a is
+// tried first, then (-b + sqrt D) / 2a, each valid for t in [tmin, tmax], height within
+// [-1e-8, H + 1e-8] and on the apex's lower nappe. Caps: the base, and the top of a frustum.
+__device__ __forceinline__ bool cone_test(const TRay& r, float4 a, float4 b, float4 c, float tbest, float& t_out,
+                                          float& part) {
+    const float vx = b.x, vy = b.y, vz = b.z, rb = c.x, rt = c.y, H = c.z;
+    const float tn = (rb - rt) / H, k1 = 1.0f + tn * tn;
+    const float ha = rb / tn;  // apex height above the base
+    float dd = r.dx * r.dx + r.dy * r.dy + r.dz * r.dz;
+    // shift: o' = o + ts d, the ray point nearest the axis midpoint
+    float mx = a.x + vx * (0.5f * H) - r.ox, my = a.y + vy * (0.5f * H) - r.oy, mz = a.z + vz * (0.5f * H) - r.oz;
+    float ts = (mx * r.dx + my * r.dy + mz * r.dz) / dd;
+    float ox = r.ox + ts * r.dx, oy = r.oy + ts * r.dy, oz = r.oz + ts * r.dz;
+    float cx = ox - (a.x + vx * ha), cy = oy - (a.y + vy * ha), cz = oz - (a.z + vz * ha);  // o' - apex
+    float dv = r.dx * vx + r.dy * vy + r.dz * vz, cv = cx * vx + cy * vy + cz * vz;
+    float A = dd - k1 * dv * dv;
+    bool hit = false;
+    float best = tbest;
+    if (fabsf(A) >= 1e-8f) {
+        float hb = (r.dx * cx + r.dy * cy + r.dz * cz) - k1 * dv * cv;
+        float C = (cx * cx + cy * cy + cz * cz) - k1 * cv * cv;
+        float disc = hb * hb - A * C;
+        if (disc >= 0.0f) {
+            float sq = sqrtf(disc);
+            float qq = -(hb + copysignf(sq, hb));
+            float ra = qq / A, rc = qq != 0.0f ? C / qq : ra;
+            // (-b - sqrt D) / 2a is qq / A when b >= 0, else C / qq
+            float first = hb >= 0.0f ? ra : rc, second = hb >= 0.0f ? rc : ra;
+            float hbase = cv + ha;  // height of o' above the base
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                float tp = k == 0 ? first : second;
+                float t = ts + tp;
+                if (t < r.tmin || t > best) continue;
+                float h = hbase + tp * dv;
+                if (h < -1e-8f || h > H + 1e-8f) continue;
+                if (cv + tp * dv > 1e-8f) continue;  // upper nappe
+                best = t;
+                part = 0.0f;
+                hit = true;
+                break;
+            }
+        }
+    }
+    if (prim_side(__float_as_uint(a.w))) {
+        float t;
+        if (cap_test(r, a.x, a.y, a.z, -vx, -vy, -vz, rb, best, t)) {
+            best = t;
+            part = 1.0f;
+            hit = true;
+        }
+        if (rt > 0.0f && cap_test(r, a.x + vx * H, a.y + vy * H, a.z + vz * H, vx, vy, vz, rt, best, t)) {
+            best = t;
+            part = 2.0f;
+            hit = true;
+        }
+    }
+    t_out = best;
+    return hit;
+}
+
 constexpr float kBoxWiden = 1.0f + 2.0f * (3.0f * 5.9604645e-08f) / (1.0f - 3.0f * 5.9604645e-08f);
 
 // Traversal stack: the first LDS_DEPTH entries per lane live in LDS (lane-major so each lane
@@ -333,8 +465,12 @@ __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStac
                 } else if (kind == kPrimQuad) {
                     float4 c = pp[2];
                     h = quad_test(r, a, b, c, sc.quads[prim_side(tag)], s.tbest, t, u, v);
-                } else {
+                } else if (kind == kPrimDisc) {
                     h = disc_test(r, a, b, pp[2], s.tbest, t);
+                } else if (kind == kPrimCylinder) {
+                    h = cylinder_test(r, a, b, pp[2], s.tbest, t, u);
+                } else {
+                    h = cone_test(r, a, b, pp[2], s.tbest, t, u);
                 }
             }
             if (h) {

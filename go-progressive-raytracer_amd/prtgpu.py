@@ -65,6 +65,16 @@ class Disc(C.Structure):
     _fields_ = [("center", D3), ("normal", D3), ("radius", C.c_double), ("material", C.c_int32), ("pad", C.c_int32)]
 
 
+class Cylinder(C.Structure):
+    _fields_ = [("base_center", D3), ("top_center", D3), ("radius", C.c_double), ("capped", C.c_int32),
+                ("material", C.c_int32)]
+
+
+class Cone(C.Structure):
+    _fields_ = [("base_center", D3), ("base_radius", C.c_double), ("top_center", D3), ("top_radius", C.c_double),
+                ("capped", C.c_int32), ("material", C.c_int32)]
+
+
 class Light(C.Structure):
     _fields_ = [("type", C.c_int32), ("shape", C.c_int32), ("emission", D3), ("top", D3), ("bottom", D3),
                 ("env_rgb", C.POINTER(C.c_float)), ("env_width", C.c_int32), ("env_height", C.c_int32),
@@ -87,7 +97,9 @@ class SceneDescC(C.Structure):
                 ("lights", C.POINTER(Light)), ("num_lights", C.c_int32),
                 ("light_weights", C.POINTER(C.c_double)),
                 ("camera", Camera),
-                ("discs", C.POINTER(Disc)), ("num_discs", C.c_int32)]
+                ("discs", C.POINTER(Disc)), ("num_discs", C.c_int32),
+                ("cylinders", C.POINTER(Cylinder)), ("num_cylinders", C.c_int32),
+                ("cones", C.POINTER(Cone)), ("num_cones", C.c_int32)]
 
 
 class SceneInfo(C.Structure):
@@ -253,6 +265,8 @@ class SceneDesc:
         self.quads = []
         self.boxes = []
         self.discs = []
+        self.cylinders = []
+        self.cones = []
         self.meshes = []  # (vertices (n,3), faces (m,3), material, face_materials|None, options dict)
         self.lights = []
         self.light_weights = None
@@ -304,6 +318,23 @@ class SceneDesc:
         dsc.center, dsc.normal, dsc.radius, dsc.material = D3(*center), D3(*normal), radius, material
         self.discs.append(dsc)
         return len(self.discs) - 1
+
+    def cylinder(self, base_center, top_center, radius, capped, material):
+        """NewCylinder (pkg/geometry/cylinder.go:24-39)."""
+        c = Cylinder()
+        c.base_center, c.top_center, c.radius = D3(*base_center), D3(*top_center), radius
+        c.capped, c.material = 1 if capped else 0, material
+        self.cylinders.append(c)
+        return len(self.cylinders) - 1
+
+    def cone(self, base_center, base_radius, top_center, top_radius, capped, material):
+        """NewCone (pkg/geometry/cone.go:27-80); its argument checks are made by prt_scene_create."""
+        c = Cone()
+        c.base_center, c.base_radius = D3(*base_center), base_radius
+        c.top_center, c.top_radius = D3(*top_center), top_radius
+        c.capped, c.material = 1 if capped else 0, material
+        self.cones.append(c)
+        return len(self.cones) - 1
 
     def box(self, center, size, rotation, material):
         b = Box()
@@ -425,6 +456,8 @@ class SceneDesc:
         d.quads, d.num_quads = arr(Quad, self.quads), len(self.quads)
         d.boxes, d.num_boxes = arr(Box, self.boxes), len(self.boxes)
         d.discs, d.num_discs = arr(Disc, self.discs), len(self.discs)
+        d.cylinders, d.num_cylinders = arr(Cylinder, self.cylinders), len(self.cylinders)
+        d.cones, d.num_cones = arr(Cone, self.cones), len(self.cones)
         d.meshes, d.num_meshes = arr(Mesh, meshes), len(meshes)
         d.lights, d.num_lights = arr(Light, self.lights), len(self.lights)
         if self.light_weights is not None:

@@ -222,3 +222,49 @@ def lights_scene(width=96):
     s.sphere((0.38, 0.1, 0.3), 0.1, s.dielectric(1.5))
     cfg = dict(max_depth=6, rr_min_bounces=3, spp=16)
     return s, cfg
+
+
+def _primitive_test_camera(s, width):
+    s.set_camera(center=(0, 1.5, 4), look_at=(0, 1, 0), up=(0, 1, 0), width=width, aspect_ratio=16.0 / 9.0,
+                 vfov=50.0, aperture=0.0, focus_distance=0.0)
+
+
+def cylinder_scene(width=400):
+    """NewCylinderTestScene (pkg/scene/cylinder_scene.go:9-122): an open gold tube, two capped
+    Lambertian cylinders and a capped glass cylinder on a ground quad, a sphere light and a blue
+    gradient sky; 200 spp, depth 50, Russian roulette from bounce 20."""
+    s = P.SceneDesc()
+    _primitive_test_camera(s, width)
+    gray, red, blue = s.lambertian((0.5, 0.5, 0.5)), s.lambertian((0.8, 0.2, 0.2)), s.lambertian((0.2, 0.2, 0.8))
+    gold, glass = s.metal((0.8, 0.6, 0.2), 0.1), s.dielectric(1.5)
+    ground_quad(s, (0, 0, 0), 10000.0, gray)
+    s.cylinder((-0.3, 1.0, -1.5), (0, 1.2, 2.0), 0.35, False, gold)
+    s.cylinder((-2.5, 0.3, 0), (-1.5, 0.3, 0), 0.3, True, blue)
+    s.cylinder((1.8, 0, 0), (1.8, 2, 0), 0.5, True, red)
+    s.cylinder((0.5, 0, 1), (0.5, 0.6, 1), 0.2, True, glass)
+    s.add_sphere_light((3, 5, 3), 1.5, (10.0, 10.0, 10.0))
+    s.add_gradient_infinite_light((0.5, 0.7, 1.0), (1.0, 1.0, 1.0))
+    cfg = dict(max_depth=50, rr_min_bounces=20, spp=200, adaptive_min_samples=0.15, adaptive_threshold=0.01)
+    return s, cfg
+
+
+def cone_scene(width=400):
+    """NewConeTestScene (pkg/scene/cone_scene.go:9-160): pointed, frustum, tilted and glass cones
+    (capped and open) on a ground quad, a sphere light and a blue gradient sky; 200 spp, depth 50."""
+    s = P.SceneDesc()
+    _primitive_test_camera(s, width)
+    gray, red = s.lambertian((0.5, 0.5, 0.5)), s.lambertian((0.8, 0.2, 0.2))
+    blue, green = s.lambertian((0.2, 0.2, 0.8)), s.lambertian((0.2, 0.8, 0.2))
+    gold, glass = s.metal((0.8, 0.6, 0.2), 0.1), s.dielectric(1.5)
+    ground_quad(s, (0, 0, 0), 10000.0, gray)
+    s.cone((0, 0, 0), 0.5, (0, 2, 0), 0.0, True, red)
+    s.cone((-2, 0.8, -0.8), 0.5, (-2, 0.2, 0.5), 0.2, True, gold)
+    s.cone((2, 0, 0), 0.8, (2, 0.6, 0), 0.5, True, blue)
+    s.cone((2, 0.6, 0), 0.5, (2, 1.8, 0), 0.0, True, glass)
+    s.cone((-1.5, 0, -0.5), 0.4, (-1.2, 1.2, -0.3), 0.15, False, green)
+    s.cone((-0.8, 0, 1.2), 0.3, (-0.8, 0.8, 1.2), 0.0, True, glass)
+    s.cone((0.8, 0.2, 1.2), 0.3, (0.8, 1.0, 1.2), 0.0, True, glass)
+    s.add_sphere_light((3, 5, 3), 1.5, (10.0, 10.0, 10.0))
+    s.add_gradient_infinite_light((0.5, 0.7, 1.0), (1.0, 1.0, 1.0))
+    cfg = dict(max_depth=50, rr_min_bounces=20, spp=200, adaptive_min_samples=0.15, adaptive_threshold=0.01)
+    return s, cfg

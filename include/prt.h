@@ -55,7 +55,8 @@ typedef struct prt_ray {
 /* 16 bytes. prim_id = -1 on a miss (t = +inf). u, v are the barycentric weights of
  * vertex 1 and vertex 2 for triangles (the Moller-Trumbore u, v of
  * pkg/geometry/triangle.go:107-175), the quad's (alpha, beta) for quads
- * (pkg/geometry/quad.go:133-180) and 0 for spheres. */
+ * (pkg/geometry/quad.go:133-180), 0 for spheres and discs, and for cylinders and cones the
+ * part hit in u (0 body, 1 base cap, 2 top cap) with v = 0. */
 typedef struct prt_hit {
     float t;
     int32_t prim_id;
@@ -109,6 +110,23 @@ typedef struct prt_disc { /* pkg/geometry/disc.go:11-43 (normal is normalized on
     int32_t material;
     int32_t pad;
 } prt_disc;
+
+typedef struct prt_cylinder { /* NewCylinder, pkg/geometry/cylinder.go:11-39 */
+    double base_center[3];
+    double top_center[3];
+    double radius;
+    int32_t capped;    /* non-zero: closed by two discs */
+    int32_t material;
+} prt_cylinder;
+
+typedef struct prt_cone { /* NewCone, pkg/geometry/cone.go:11-80: base_radius > top_radius >= 0, height > 0 */
+    double base_center[3];
+    double base_radius;
+    double top_center[3];
+    double top_radius; /* 0: pointed cone, > 0: frustum */
+    int32_t capped;    /* non-zero: base disc, and a top disc for a frustum */
+    int32_t material;
+} prt_cone;
 
 typedef struct prt_box { /* pkg/geometry/box.go:9-145: size = half extents, rotation X->Y->Z radians */
     double center[3];
@@ -183,11 +201,13 @@ typedef struct prt_scene_desc {
     const double* light_weights;   /* NULL: uniform light sampler (scene.go:62-64) */
     prt_camera camera;
     const prt_disc* discs;         int32_t num_discs;
+    const prt_cylinder* cylinders; int32_t num_cylinders;
+    const prt_cone* cones;         int32_t num_cones;
 } prt_scene_desc;
 
 /* Global primitive numbering used by prt_hit.prim_id: all mesh triangles (mesh by
  * mesh, face order), then spheres, then quads, then 6 quads per box (box.go:34-89 face order),
- * then discs, then the discs of disc spot lights (in light order). */
+ * then discs, cylinders, cones, then the discs of disc spot lights (in light order). */
 
 typedef struct prt_scene_info {
     int64_t num_primitives;

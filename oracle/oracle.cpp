@@ -366,6 +366,186 @@ struct Disc : Shape {
     }
 };
 
+// Cap disc of a cylinder or cone (hitCap, cylinder.go:213-262 / cone.go:299-349): plane hit with
+// |d.n| >= 1e-8, t in [tmin, tmax], |p - c| <= radius; uv from the (axis x ref) frame.
+static bool hit_cap(const Ray& ray, Vec3 c, Vec3 n, double radius, double tmin, double tmax, int mat, int64_t id,
+                    Hit& h) {
+    double den = ray.d.dot(n);
+    if (std::fabs(den) < 1e-8) return false;
+    double t = (c - ray.o).dot(n) / den;
+    if (t < tmin || t > tmax) return false;
+    Vec3 p = ray.at(t);
+    Vec3 lp = p - c;
+    if (lp.length() > radius) return false;
+    Vec3 ref = std::fabs(n.y) < 0.9 ? Vec3(0, 1, 0) : Vec3(1, 0, 0);
+    Vec3 tg = n.cross(ref).normalize();
+    Vec3 bt = n.cross(tg);
+    h.t = t;
+    h.p = p;
+    h.uv = {(lp.dot(tg) / radius + 1.0) / 2.0, (lp.dot(bt) / radius + 1.0) / 2.0};
+    h.mat = mat;
+    h.prim = id;
+    h.set_face_normal(ray, n);
+    return true;
+}
+// u of a body hit: the angle of the radial vector in the (axis x ref, axis x tangent) frame
+static double body_u(Vec3 axis, Vec3 radial) {
+    Vec3 ref = std::fabs(axis.y) < 0.9 ? Vec3(0, 1, 0) : Vec3(1, 0, 0);
+    Vec3 tg = axis.cross(ref).normalize();
+    Vec3 bt = axis.cross(tg);
+    return (std::atan2(radial.dot(bt), radial.dot(tg)) + kPi) / (2.0 * kPi);
+}
+// Bounding box of the base-top segment widened by `ext` on every axis the unit axis is not
+// parallel to (|axis_i| > 0.9999): Cylinder/Cone.BoundingBox (cylinder.go:41-88, cone.go:83-137).
+static AABB axis_box(Vec3 b, Vec3 t, Vec3 axis, double ext) {
+    Vec3 mn(std::min(b.x, t.x), std::min(b.y, t.y), std::min(b.z, t.z));
+    Vec3 mx(std::max(b.x, t.x), std::max(b.y, t.y), std::max(b.z, t.z));
+    double ex = std::fabs(axis.x) > 0.9999 ? 0 : ext, ey = std::fabs(axis.y) > 0.9999 ? 0 : ext,
+           ez = std::fabs(axis.z) > 0.9999 ? 0 : ext;
+    return AABB{{mn.x - ex, mn.y - ey, mn.z - ez}, {mx.x + ex, mx.y + ey, mx.z + ez}};
+}
+
+// ---- Cylinder, pkg/geometry/cylinder.go:11-262 --------------------------------------
+struct Cylinder : Shape {
+    Vec3 base, top, axis;
+    double r, height;
+    bool capped;
+    int mat;
+    int64_t id;
+    Cylinder(Vec3 b, Vec3 t, double radius, bool cap, int m, int64_t i)
+        : base(b), top(t), r(radius), capped(cap), mat(m), id(i) {
+        height = (t - b).length();
+        axis = (t - b).normalize();
+    }
+    // hitBody :122-210: the first root (near, then far) with t in range and 0 <= h <= height
+    bool body(const Ray& ray, double tmin, double tmax, Hit& h) const {
+        Vec3 dl = ray.o - base;
+        double dv = ray.d.dot(axis), lv = dl.dot(axis);
+        double a = ray.d.length2() - dv * dv;
+        double b = 2.0 * (dl.dot(ray.d) - lv * dv);
+        double c = dl.length2() - lv * lv - r * r;
+        if (std::fabs(a) < 1e-8) return false;
+        double disc = b * b - 4 * a * c;
+        if (disc < 0) return false;
+        double sq = std::sqrt(disc);
+        double roots[2] = {(-b - sq) / (2 * a), (-b + sq) / (2 * a)};
+        for (double t : roots) {
+            if (t < tmin || t > tmax) continue;
+            Vec3 p = ray.at(t);
+            double hh = (p - base).dot(axis);
+            if (hh < 0 || hh > height) continue;
+            Vec3 radial = p - (base + axis * hh);
+            h.t = t;
+            h.p = p;
+            h.uv = {body_u(axis, radial), hh / height};
+            h.mat = mat;
+            h.prim = id;
+            h.set_face_normal(ray, radial.normalize());
+            return true;
+        }
+        return false;
+    }
+    bool hit(const Ray& ray, double tmin, double tmax, Hit& h, Counters* cnt) const override {  // :91-119
+        if (cnt) cnt->prim++;
+        bool any = false;
+        double closest = tmax;
+        Hit tmp;
+        if (body(ray, tmin, closest, tmp)) {
+            h = tmp;
+            closest = tmp.t;
+            any = true;
+        }
+        if (capped) {
+            if (hit_cap(ray, base, axis * -1.0, r, tmin, closest, mat, id, tmp)) {
+                h = tmp;
+                closest = tmp.t;
+                any = true;
+            }
+            if (hit_cap(ray, top, axis, r, tmin, closest, mat, id, tmp)) {
+                h = tmp;
+                any = true;
+            }
+        }
+        return any;
+    }
+    AABB bbox() const override { return axis_box(base, top, axis, r); }
+};
+
+// ---- Cone, pkg/geometry/cone.go:11-349 ----------------------------------------------
+struct Cone : Shape {
+    Vec3 base, top, axis, apex;
+    double rb, rt, height, tan_a;
+    bool capped;
+    int mat;
+    int64_t id;
+    Cone(Vec3 b, double base_r, Vec3 t, double top_r, bool cap, int m, int64_t i)
+        : base(b), top(t), rb(base_r), rt(top_r), capped(cap), mat(m), id(i) {  // NewCone :27-80
+        height = (t - b).length();
+        axis = (t - b).normalize();
+        tan_a = (rb - rt) / height;
+        apex = rt == 0 ? t : t + axis * (rt * height / (rb - rt));
+    }
+    bool valid(const Ray& ray, double t, double tmin, double tmax) const {  // validateIntersection :265-296
+        if (t < tmin || t > tmax) return false;
+        Vec3 p = ray.at(t);
+        double hh = (p - base).dot(axis);
+        if (hh < -1e-8 || hh > height + 1e-8) return false;
+        return (p - apex).dot(axis) <= 1e-8;
+    }
+    // hitBody :167-262: tries (-b - sqrt D) / 2a first, then (-b + sqrt D) / 2a, whatever the sign of a
+    bool body(const Ray& ray, double tmin, double tmax, Hit& h) const {
+        Vec3 co = ray.o - apex;
+        double dv = ray.d.dot(axis), cv = co.dot(axis), k = tan_a * tan_a;
+        double a = ray.d.length2() - (1 + k) * dv * dv;
+        double b = 2.0 * (ray.d.dot(co) - (1 + k) * dv * cv);
+        double c = co.length2() - (1 + k) * cv * cv;
+        if (std::fabs(a) < 1e-8) return false;
+        double disc = b * b - 4 * a * c;
+        if (disc < 0) return false;
+        double sq = std::sqrt(disc);
+        double t = (-b - sq) / (2 * a);
+        if (!valid(ray, t, tmin, tmax)) {
+            t = (-b + sq) / (2 * a);
+            if (!valid(ray, t, tmin, tmax)) return false;
+        }
+        Vec3 p = ray.at(t);
+        double hh = (p - base).dot(axis);
+        Vec3 radial = p - (base + axis * hh);
+        h.t = t;
+        h.p = p;
+        h.uv = {body_u(axis, radial), hh / height};
+        h.mat = mat;
+        h.prim = id;
+        // normalize(radial + axis (rb - rt) / height), with the unnormalized radial (:221-229)
+        h.set_face_normal(ray, (radial + axis * ((rb - rt) / height)).normalize());
+        return true;
+    }
+    bool hit(const Ray& ray, double tmin, double tmax, Hit& h, Counters* cnt) const override {  // :140-164
+        if (cnt) cnt->prim++;
+        bool any = false;
+        double closest = tmax;
+        Hit tmp;
+        if (body(ray, tmin, closest, tmp)) {
+            h = tmp;
+            closest = tmp.t;
+            any = true;
+        }
+        if (capped) {
+            if (hit_cap(ray, base, axis * -1.0, rb, tmin, closest, mat, id, tmp)) {
+                h = tmp;
+                closest = tmp.t;
+                any = true;
+            }
+            if (rt > 0 && hit_cap(ray, top, axis, rt, tmin, closest, mat, id, tmp)) {
+                h = tmp;
+                any = true;
+            }
+        }
+        return any;
+    }
+    AABB bbox() const override { return axis_box(base, top, axis, rb); }
+};
+
 // ---- Box, pkg/geometry/box.go:34-140 ----------------------------------------------
 struct Box : Shape {
     std::unique_ptr<Quad> faces[6];
@@ -1962,6 +2142,15 @@ orc_scene* orc_scene_create(const prt_scene_desc* d) {
         const prt_disc& dc = d->discs[i];
         S.owned.emplace_back(new Disc(V(dc.center), V(dc.normal), dc.radius, dc.material, id++));
     }
+    for (int i = 0; i < d->num_cylinders; i++) {
+        const prt_cylinder& c = d->cylinders[i];
+        S.owned.emplace_back(new Cylinder(V(c.base_center), V(c.top_center), c.radius, c.capped != 0, c.material, id++));
+    }
+    for (int i = 0; i < d->num_cones; i++) {
+        const prt_cone& c = d->cones[i];
+        S.owned.emplace_back(
+            new Cone(V(c.base_center), c.base_radius, V(c.top_center), c.top_radius, c.capped != 0, c.material, id++));
+    }
     // disc spot lights: an internal emitting material and their disc join the shapes (scene.go:117-122)
     std::vector<int> spot_mat(d->num_lights, -1);
     for (int i = 0; i < d->num_lights; i++) {
@@ -2288,7 +2477,8 @@ void orc_sample_point_in_unit_sphere(const double* s, double* out) {
 double orc_power_heuristic(int32_t nf, double f, int32_t ng, double g) { return power_heuristic(nf, f, ng, g); }
 double orc_reflectance(double c, double r) { return reflectance(c, r); }
 // Single-shape intersection: kind 0 triangle (9 doubles), 1 sphere (4), 2 quad (9),
-// 3 disc (centre 3, normal 3, radius).
+// 3 disc (centre 3, normal 3, radius), 4 cylinder (base 3, top 3, radius, capped),
+// 5 cone (base 3, base radius, top 3, top radius, capped).
 // out: t, p(3), n(3), front, u, v.  Returns 1 on hit.
 int32_t orc_shape_hit(int32_t kind, const double* g, const double* o, const double* d, double tmin,
                       double tmax, double* out) {
@@ -2299,6 +2489,8 @@ int32_t orc_shape_hit(int32_t kind, const double* g, const double* o, const doub
     else if (kind == 1) hit = Sphere(V(g), g[3], 0, 0).hit(r, tmin, tmax, h, nullptr);
     else if (kind == 2) hit = Quad(V(g), V(g + 3), V(g + 6), 0, 0).hit(r, tmin, tmax, h, nullptr);
     else if (kind == 3) hit = Disc(V(g), V(g + 3), g[6], 0, 0).hit(r, tmin, tmax, h, nullptr);
+    else if (kind == 4) hit = Cylinder(V(g), V(g + 3), g[6], g[7] != 0, 0, 0).hit(r, tmin, tmax, h, nullptr);
+    else if (kind == 5) hit = Cone(V(g), g[3], V(g + 4), g[7], g[8] != 0, 0, 0).hit(r, tmin, tmax, h, nullptr);
     if (!hit) return 0;
     out[0] = h.t;
     out[1] = h.p.x; out[2] = h.p.y; out[3] = h.p.z;
@@ -2314,6 +2506,12 @@ void orc_disc_frame(const double* g, double* out) {
     out[0] = dc.right.x; out[1] = dc.right.y; out[2] = dc.right.z;
     out[3] = dc.up.x; out[4] = dc.up.y; out[5] = dc.up.z;
     out[6] = b.mn.x; out[7] = b.mn.y; out[8] = b.mn.z; out[9] = b.mx.x; out[10] = b.mx.y; out[11] = b.mx.z;
+}
+// Cylinder (kind 4) or cone (kind 5) box in the layout of orc_shape_hit: out = min(3), max(3).
+void orc_axial_bbox(int32_t kind, const double* g, double* out) {
+    AABB b = kind == 4 ? Cylinder(V(g), V(g + 3), g[6], g[7] != 0, 0, 0).bbox()
+                       : Cone(V(g), g[3], V(g + 4), g[7], g[8] != 0, 0, 0).bbox();
+    out[0] = b.mn.x; out[1] = b.mn.y; out[2] = b.mn.z; out[3] = b.mx.x; out[4] = b.mx.y; out[5] = b.mx.z;
 }
 void orc_quad_bbox(const double* g, double* out) {
     AABB b = Quad(V(g), V(g + 3), V(g + 6), 0, 0).bbox();

@@ -48,6 +48,7 @@ def lib():
             "orc_shape_hit": (i32, [i32, vp, vp, vp, d, d, vp]),
             "orc_quad_bbox": (None, [vp, vp]),
             "orc_disc_frame": (None, [vp, vp]),
+            "orc_axial_bbox": (None, [i32, vp, vp]),
             "orc_light_sample_emission": (None, [vp, i32, vp, vp, vp]),
             "orc_light_pdf_le": (None, [vp, i32, vp, vp, vp]),
             "orc_aabb_hit": (i32, [vp, vp, vp, d, d]),
