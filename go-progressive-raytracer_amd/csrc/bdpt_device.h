@@ -26,15 +26,18 @@ struct BV {
     float pdf_fwd, pdf_rev;
     int mat, light;
     uint32_t flags;
+    float u = 0.0f, v = 0.0f;  // surface UV (textured scenes)
 };
 
-// Subpath vertices in global memory as five planes of float4, plane f of vertex k of sample i at
-// [(f * kmax + k) * stride + i] (each plane coalesced across a wave). Plane 0 is the record the MIS
-// walk reads for every vertex: (pdf_fwd, pdf_rev, flags | (mat + 1) << 8, light).
+// Subpath vertices in global memory as five planes of float4 (six with surface UVs in textured
+// scenes), plane f of vertex k of sample i at [(f * kmax + k) * stride + i] (each plane coalesced
+// across a wave). Plane 0 is the record the MIS walk reads for every vertex:
+// (pdf_fwd, pdf_rev, flags | (mat + 1) << 8, light).
 struct VertexStore {
     float4* data;
     uint32_t stride;  // samples per chunk
     uint32_t kmax;    // vertices per subpath
+    uint32_t uv;      // 1: plane 5 holds (u, v)
     __device__ __forceinline__ float4* at(int f, int k, uint32_t i) const {
         return data + ((size_t)f * kmax + (size_t)k) * stride + i;
     }
@@ -45,6 +48,7 @@ struct VertexStore {
         *at(2, k, i) = make_float4(v.n.x, v.n.y, v.n.z, v.beta.y);
         *at(3, k, i) = make_float4(v.wi.x, v.wi.y, v.wi.z, v.beta.z);
         *at(4, k, i) = make_float4(v.Le.x, v.Le.y, v.Le.z, 0.0f);
+        if (uv) *at(5, k, i) = make_float4(v.u, v.v, 0.0f, 0.0f);
     }
     __device__ __forceinline__ float4 rec(int k, uint32_t i) const { return *at(0, k, i); }
     __device__ __forceinline__ BV get(int k, uint32_t i) const {
@@ -61,6 +65,11 @@ struct VertexStore {
         v.wi = v3(c.x, c.y, c.z);
         v.beta = v3(a.w, b.w, c.w);
         v.Le = v3(e.x, e.y, e.z);
+        if (uv) {
+            float4 t = *at(5, k, i);
+            v.u = t.x;
+            v.v = t.y;
+        }
         return v;
     }
     __device__ __forceinline__ void set_pdf_rev(int k, uint32_t i, float pdf) const { at(0, k, i)->y = pdf; }
@@ -85,6 +94,8 @@ __device__ __forceinline__ Surf surf_of(const BV& v) {
     s.n = v.n;
     s.front = (v.flags & VF_FRONT) != 0;
     s.mat = v.mat;
+    s.u = v.u;
+    s.v = v.v;
     return s;
 }
 

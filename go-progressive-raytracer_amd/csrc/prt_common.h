@@ -77,13 +77,21 @@ struct QuadExt {
 
 // ---- materials and lights (float32 device copies) ------------------------------------
 constexpr int32_t kMatSpotEmitter = 5;  // the disc of a disc spot light (emits against spot_dir only)
+constexpr int32_t kMatMix = 6;
 struct DevMaterial {
-    int32_t type, outer, inner, pad;
+    int32_t type, outer, inner;
+    int32_t tex;  // lambertian / metal albedo: 0 solid, k > 0 texture k - 1
     float albedo[3];
     float fuzz;
     float ior;
     float emission[3];
     float spot_dir[3];
+    float ratio;  // mix: probability of material2 (inner)
+};
+
+struct DevTexture {
+    int32_t width, height;
+    const float* rgb;  // height*width*3, row 0 at the top
 };
 
 struct DevLight {
@@ -130,6 +138,8 @@ struct DevScene {
     const DevMaterial* mats;
     const DevLight* lights;
     const float* light_weights;  // normalised selection probabilities
+    const DevTexture* textures;
+    const float2* tri_uv;  // per leaf-ordered primitive 3 vertex UVs (x = NaN: none), or null
     DevEnv env;
     DevCamera cam;
     int32_t num_lights;
@@ -137,6 +147,7 @@ struct DevScene {
     int32_t mixed;  // 1 if any non-triangle primitive exists
     int32_t max_stack;
     int32_t num_nodes;
+    int32_t textured;  // 1 if any material reads a texture: hits carry UVs
     float world_center[3];
     float world_radius;
 };

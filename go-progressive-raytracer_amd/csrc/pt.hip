@@ -425,6 +425,8 @@ __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i
         v.light = -1;
         v.Le = emit_hit(sc, s.mat, d, s.front);
         v.flags = (s.front ? VF_FRONT : 0u) | ((v.Le.x != 0.0f || v.Le.y != 0.0f || v.Le.z != 0.0f) ? VF_LIGHT : 0u);
+        v.u = s.u;
+        v.v = s.v;
         v.pdf_fwd = convert_density(sc, prev, v, pdf_fwd);
         v.pdf_rev = 0.0f;
         ScatterOut so;
@@ -910,14 +912,14 @@ int bdpt_blocks(const prt_scene* scene, int trace_blocks) {
     return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu * 2));
 }
 
-// Bytes of one subpath store: (max_depth + 1) vertices of 5 float4 per path slot.
-size_t bdpt_store_bytes(int max_depth, int64_t slots) {
-    return (size_t)(max_depth + 1) * 5 * sizeof(float4) * (size_t)slots;
+// Bytes of one subpath store: (max_depth + 1) vertices of 5 float4 (6 with UVs) per path slot.
+size_t bdpt_store_bytes(int max_depth, int64_t slots, bool uv) {
+    return (size_t)(max_depth + 1) * (uv ? 6 : 5) * sizeof(float4) * (size_t)slots;
 }
 
 // Path slots of a BDPT film: the two subpath stores stay within 2 GiB.
-int64_t bdpt_capacity(int max_depth, int64_t cap) {
-    int64_t c = ((int64_t)1 << 31) / (int64_t)(2 * bdpt_store_bytes(max_depth, 1));
+int64_t bdpt_capacity(int max_depth, int64_t cap, bool uv) {
+    int64_t c = ((int64_t)1 << 31) / (int64_t)(2 * bdpt_store_bytes(max_depth, 1, uv));
     c = std::max<int64_t>(c, (int64_t)1 << 16) & ~(int64_t)255;
     return std::min(cap, c);
 }
@@ -1165,11 +1167,12 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     int32_t h = p->height > 0 ? p->height : scene->image_height;
     int64_t npix = (int64_t)w * h;
     const bool bd = p->integrator == PRT_INTEGRATOR_BDPT;
-    int64_t C = film_capacity(npix, bd ? bdpt_capacity(p->max_depth, cap) : cap);
+    const bool uv = scene->dev.textured != 0;
+    int64_t C = film_capacity(npix, bd ? bdpt_capacity(p->max_depth, cap, uv) : cap);
     int blocks = film_blocks(scene);
     size_t g = film_gstack_entries(scene) * (size_t)blocks * kBlock * sizeof(uint2);
     size_t b = 0;
-    if (bd) b = 2 * bdpt_store_bytes(p->max_depth, C) + sizeof(uint32_t) * C + 3 * sizeof(long long) * npix;
+    if (bd) b = 2 * bdpt_store_bytes(p->max_depth, C, uv) + sizeof(uint32_t) * C + 3 * sizeof(long long) * npix;
     if (p->adaptive_min_samples < 1.0) b += 2 * sizeof(uint32_t) * npix + 6 * sizeof(uint32_t);
     return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + b + 3 * sizeof(uint32_t) * (size_t)C;
 }
@@ -1208,7 +1211,7 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     f->height = h;
     f->npix = (int64_t)w * h;
     f->capacity = film_capacity(
-        f->npix, p->integrator == PRT_INTEGRATOR_BDPT ? bdpt_capacity(p->max_depth, cap) : cap);
+        f->npix, p->integrator == PRT_INTEGRATOR_BDPT ? bdpt_capacity(p->max_depth, cap, scene->dev.textured != 0) : cap);
     int rc;
     auto bail = [&](int code) {
         prt_film_destroy(f);
@@ -1246,11 +1249,12 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     if (p->integrator == PRT_INTEGRATOR_BDPT) {
         f->bdpt = true;
         f->bdpt_blocks = bdpt_blocks(scene, f->blocks);
-        const size_t vb = bdpt_store_bytes(p->max_depth, C);
+        const uint32_t uv = scene->dev.textured ? 1u : 0u;
+        const size_t vb = bdpt_store_bytes(p->max_depth, C, uv != 0);
         if ((rc = film_alloc(f, vb, &ptr)) != PRT_OK) return bail(rc);
-        f->cpath = VertexStore{(float4*)ptr, (uint32_t)C, (uint32_t)(p->max_depth + 1)};
+        f->cpath = VertexStore{(float4*)ptr, (uint32_t)C, (uint32_t)(p->max_depth + 1), uv};
         if ((rc = film_alloc(f, vb, &ptr)) != PRT_OK) return bail(rc);
-        f->lpath = VertexStore{(float4*)ptr, (uint32_t)C, (uint32_t)(p->max_depth + 1)};
+        f->lpath = VertexStore{(float4*)ptr, (uint32_t)C, (uint32_t)(p->max_depth + 1), uv};
         if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
         f->lens = (uint32_t*)ptr;
         if ((rc = film_alloc(f, 3 * sizeof(long long) * (size_t)f->npix, &ptr)) != PRT_OK) return bail(rc);

@@ -615,20 +615,32 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PRT_ERR_NO_DEVICE, "no HIP device");
     if (device < 0 || device >= ndev) return fail(PRT_ERR_INVALID_ARGUMENT, "device index out of range");
     if (d->num_materials < 0 || d->num_spheres < 0 || d->num_quads < 0 || d->num_boxes < 0 ||
-        d->num_meshes < 0 || d->num_lights < 0 || d->num_discs < 0)
+        d->num_meshes < 0 || d->num_lights < 0 || d->num_discs < 0 || d->num_cylinders < 0 || d->num_cones < 0 ||
+        d->num_textures < 0)
         return fail(PRT_ERR_INVALID_ARGUMENT, "negative count");
+    for (int i = 0; i < d->num_textures; i++)
+        if (d->textures[i].width <= 0 || d->textures[i].height <= 0 || !d->textures[i].rgb)
+            return fail(PRT_ERR_INVALID_ARGUMENT, "texture needs pixels");
     if (d->camera.width <= 0 || !(d->camera.aspect_ratio > 0))
         return fail(PRT_ERR_INVALID_ARGUMENT, "camera width/aspect must be positive");
     const int nm = d->num_materials;
     auto bad_mat = [&](int m) { return m < 0 || m >= nm; };
+    // composite materials (Layered, Mix) take two basic components (Lambertian, Metal, Dielectric,
+    // Emissive): the device evaluates one level of composition
+    auto basic = [&](int m) { return !bad_mat(m) && d->materials[m].type <= PRT_MAT_EMISSIVE; };
+    bool textured = false;
     for (int i = 0; i < nm; i++) {
         const prt_material& m = d->materials[i];
-        if (m.type < PRT_MAT_LAMBERTIAN || m.type > PRT_MAT_LAYERED)
+        if (m.type < PRT_MAT_LAMBERTIAN || (m.type > PRT_MAT_LAYERED && m.type != PRT_MAT_MIX))
             return fail(PRT_ERR_INVALID_ARGUMENT, "unknown material type");
-        if (m.type == PRT_MAT_LAYERED) {
-            if (bad_mat(m.outer) || bad_mat(m.inner) || m.outer == i || m.inner == i ||
-                d->materials[m.outer].type == PRT_MAT_LAYERED || d->materials[m.inner].type == PRT_MAT_LAYERED)
-                return fail(PRT_ERR_INVALID_ARGUMENT, "layered material needs two non-layered components");
+        if (m.type == PRT_MAT_LAYERED && !(basic(m.outer) && basic(m.inner)))
+            return fail(PRT_ERR_INVALID_ARGUMENT, "layered material needs two basic components");
+        if (m.type == PRT_MAT_MIX && !(basic(m.outer) && basic(m.inner)))
+            return fail(PRT_ERR_INVALID_ARGUMENT, "mix material needs two basic components");
+        if (m.type == PRT_MAT_LAMBERTIAN || m.type == PRT_MAT_METAL) {
+            if (m.texture < 0 || m.texture > d->num_textures)
+                return fail(PRT_ERR_INVALID_ARGUMENT, "material texture out of range");
+            textured |= m.texture > 0;
         }
     }
     PRT_HIP(hipSetDevice(device));
@@ -752,6 +764,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     bool world_any = false;
     int64_t gid = 0;
     std::vector<float4> trin(1, make_float4(0.f, 0.f, 0.f, 0.f));  // custom triangle normals, 1-based
+    std::vector<float2> uvg;  // vertex UVs of mesh triangles by global id (x = NaN: none), if any mesh has them
     std::vector<double> rotated;
     for (int i = 0; i < d->num_meshes; i++) {
         const prt_mesh& m = d->meshes[i];
@@ -815,6 +828,11 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
                 side = (uint32_t)trin.size();
                 if (side > kSideMask) return bail(fail(PRT_ERR_UNSUPPORTED, "too many custom normals"));
                 trin.push_back(make_float4((float)(cn[0] * inv), (float)(cn[1] * inv), (float)(cn[2] * inv), 0.f));
+            }
+            if (m.uvs) {  // TriangleMeshOptions.VertexUVs (triangle_mesh.go:97-112)
+                if (uvg.empty()) uvg.assign(3 * (size_t)nprim, make_float2(NAN, NAN));
+                for (int k = 0; k < 3; k++)
+                    uvg[3 * gid + k] = make_float2((float)m.uvs[2 * fi[k]], (float)m.uvs[2 * fi[k] + 1]);
             }
             rec[3 * gid + 0] = f4(p0[0], p0[1], p0[2], bits(prim_tag(kPrimTriangle, side)));
             rec[3 * gid + 1] = f4(p1[0], p1[1], p1[2], bits((uint32_t)mat));
@@ -950,6 +968,25 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     s->dev.quads = (const QuadExt*)p;
     if ((rc = scene_alloc(s, trin.size() * sizeof(float4), trin.data(), &p)) != PRT_OK) return bail(rc);
     s->dev.tri_normals = (const float4*)p;
+    if (!uvg.empty()) {  // vertex UVs in leaf order
+        std::vector<float2> uvl(3 * (size_t)nprim);
+        for (int64_t k = 0; k < nprim; k++)
+            for (int j = 0; j < 3; j++) uvl[3 * k + j] = uvg[3 * (size_t)bvh.order[k] + j];
+        if ((rc = scene_alloc(s, uvl.size() * sizeof(float2), uvl.data(), &p)) != PRT_OK) return bail(rc);
+        s->dev.tri_uv = (const float2*)p;
+    }
+    s->dev.textured = textured ? 1 : 0;
+    if (d->num_textures > 0) {
+        std::vector<DevTexture> tex(d->num_textures);
+        for (int i = 0; i < d->num_textures; i++) {
+            const prt_texture& t = d->textures[i];
+            if ((rc = scene_alloc(s, (size_t)t.width * t.height * 3 * sizeof(float), t.rgb, &p)) != PRT_OK)
+                return bail(rc);
+            tex[i] = DevTexture{t.width, t.height, (const float*)p};
+        }
+        if ((rc = scene_alloc(s, tex.size() * sizeof(DevTexture), tex.data(), &p)) != PRT_OK) return bail(rc);
+        s->dev.textures = (const DevTexture*)p;
+    }
     s->dev.mixed = (nsph + nq + ndisc + naxial) > 0 ? 1 : 0;
     s->dev.max_stack = bvh.max_stack;
     s->dev.num_nodes = (int32_t)bvh.nodes.size();
@@ -968,6 +1005,8 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         }
         o.fuzz = (float)std::min(1.0, std::max(0.0, m.fuzz));  // NewMetal clamps (metal.go:19-29)
         o.ior = (float)m.ior;
+        o.tex = (m.type == PRT_MAT_LAMBERTIAN || m.type == PRT_MAT_METAL) ? m.texture : 0;
+        o.ratio = (float)std::min(1.0, std::max(0.0, m.ratio));  // NewMix clamps (mix.go:17-27)
     }
     for (int i = 0; i < d->num_lights; i++) {  // discSpotLightMaterial (disc_spot_light.go:12-46)
         if (spot_index[i] < 0) continue;

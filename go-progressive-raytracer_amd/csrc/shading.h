@@ -84,7 +84,16 @@ struct Surf {
     V3 p, n;
     bool front;
     int mat;
+    float u = 0.0f, v = 0.0f;  // SurfaceInteraction.UV, set when the scene has textures
 };
+
+// In-plane frame of the reference's cap and body UVs: t = normalize(n x ref), b = n x t with
+// ref = (0,1,0) unless |n.y| >= 0.9 (cylinder.go:190-200, 243-252; cone.go:236-246, 329-338).
+__device__ __forceinline__ void uv_frame(V3 n, V3& t, V3& b) {
+    V3 ref = fabsf(n.y) < 0.9f ? v3(0, 1, 0) : v3(1, 0, 0);
+    t = normalize(cross(n, ref));
+    b = cross(n, t);
+}
 
 // Hit record of a primitive hit (SetFaceNormal, material/interfaces.go:58-65).
 // The point ray.At(t) is moved back onto the primitive's surface (plane projection for triangles and
@@ -148,7 +157,74 @@ __device__ __forceinline__ Surf make_surf(const DevScene& sc, V3 o, V3 d, const 
     }
     s.front = dot(d, outward) < 0.0f;
     s.n = s.front ? outward : outward * -1.0f;
+    if (sc.textured) {  // hit UVs of each shape (the reference's Hit functions)
+        float u = h.u, v = h.v;  // triangle barycentrics (triangle.go:151-161), quad (alpha, beta) (quad.go:165)
+        if (kind == kPrimTriangle) {
+            if (sc.tri_uv) {
+                const float2* tu = sc.tri_uv + 3 * (size_t)h.prim;
+                const float2 t0 = tu[0];
+                if (t0.x == t0.x) {  // not NaN: the mesh has vertex UVs
+                    const float2 t1 = tu[1], t2 = tu[2];
+                    float w = 1.0f - h.u - h.v;
+                    u = t0.x * w + t1.x * h.u + t2.x * h.v;
+                    v = t0.y * w + t1.y * h.u + t2.y * h.v;
+                }
+            }
+        } else if (kind == kPrimSphere) {  // sphere.go:64-68
+            float ny = fmaxf(-1.0f, fminf(1.0f, outward.y));
+            u = (atan2f(-outward.z, outward.x) + kPiF) / (2.0f * kPiF);
+            v = acosf(-ny) / kPiF;
+        } else if (kind == kPrimDisc) {  // disc.go:76-80 with NewDisc's frame
+            V3 n = v3(b.x, b.y, b.z);
+            V3 right = normalize(cross(fabsf(n.x) > 0.1f ? v3(0, 1, 0) : v3(1, 0, 0), n));
+            V3 up = normalize(cross(n, right));
+            V3 cp = s.p - v3(a.x, a.y, a.z);
+            float r = pp[2].x;
+            u = (dot(cp, right) / r + 1.0f) * 0.5f;
+            v = (dot(cp, up) / r + 1.0f) * 0.5f;
+        } else if (kind == kPrimCylinder || kind == kPrimCone) {
+            const float4 c = pp[2];
+            const V3 base = v3(a.x, a.y, a.z), axis = v3(b.x, b.y, b.z);
+            const bool cone = kind == kPrimCone;
+            const float H = cone ? c.z : c.y;
+            const int part = (int)h.u;
+            V3 t, bt;
+            if (part == 0) {
+                float hh = dot(s.p - base, axis);
+                V3 radial = s.p - (base + axis * hh);
+                uv_frame(axis, t, bt);
+                u = (atan2f(dot(radial, bt), dot(radial, t)) + kPiF) / (2.0f * kPiF);
+                v = hh / H;
+            } else {
+                V3 n = part == 1 ? axis * -1.0f : axis;
+                V3 cc = part == 1 ? base : base + axis * H;
+                float rad = cone ? (part == 1 ? c.x : c.y) : c.x;
+                uv_frame(n, t, bt);
+                V3 lp = s.p - cc;
+                u = (dot(lp, t) / rad + 1.0f) * 0.5f;
+                v = (dot(lp, bt) / rad + 1.0f) * 0.5f;
+            }
+        }
+        s.u = u;
+        s.v = v;
+    }
     return s;
+}
+
+// ImageTexture.Evaluate (image_texture.go:24-55): wrap by truncation, flip v, nearest texel, clamp.
+__device__ __forceinline__ V3 texture_eval(const DevTexture& t, float u, float v) {
+    u = u - truncf(u);
+    v = v - truncf(v);
+    if (u < 0.0f) u += 1.0f;
+    if (v < 0.0f) v += 1.0f;
+    int x = (int)fminf(fmaxf(u * (float)t.width, 0.0f), (float)(t.width - 1));
+    int y = (int)fminf(fmaxf((1.0f - v) * (float)t.height, 0.0f), (float)(t.height - 1));
+    const float* p = t.rgb + 3 * ((size_t)y * t.width + x);
+    return v3(p[0], p[1], p[2]);
+}
+// Albedo.Evaluate(hit.UV, hit.Point): the solid colour or the material's texture
+__device__ __forceinline__ V3 albedo_of(const DevScene& sc, const DevMaterial& M, const Surf& s) {
+    return M.tex > 0 ? texture_eval(sc.textures[M.tex - 1], s.u, s.v) : v3(M.albedo);
 }
 
 // ---- materials, pkg/material ------------------------------------------------------------
@@ -178,8 +254,8 @@ __device__ __forceinline__ bool is_reflection_path(V3 in, V3 out, V3 n) {  // la
 }
 
 // Scatter of a non-layered material; draws from stream positions base+k.
-__device__ __forceinline__ bool scatter_basic(const DevMaterial& M, V3 din, const Surf& s, uint32_t key, uint32_t pos,
-                                              uint32_t& used, ScatterOut& out) {
+__device__ __forceinline__ bool scatter_basic(const DevScene& sc, const DevMaterial& M, V3 din, const Surf& s,
+                                              uint32_t key, uint32_t pos, uint32_t& used, ScatterOut& out) {
     switch (M.type) {
         case 0: {  // Lambertian, lambertian.go:25-50
             float u0 = draw(key, pos), u1 = draw(key, pos + 1);
@@ -189,7 +265,7 @@ __device__ __forceinline__ bool scatter_basic(const DevMaterial& M, V3 din, cons
             if (ct < 0.0f) ct = 0.0f;
             out.dir = d;
             out.pdf = ct / kPiF;
-            out.att = v3(M.albedo) * (1.0f / kPiF);
+            out.att = albedo_of(sc, M, s) * (1.0f / kPiF);
             return true;
         }
         case 1: {  // Metal, metal.go:38-59
@@ -201,7 +277,7 @@ __device__ __forceinline__ bool scatter_basic(const DevMaterial& M, V3 din, cons
                 r = r + sample_point_in_unit_sphere(u0, u1, u2) * M.fuzz;
             }
             out.dir = r;
-            out.att = v3(M.albedo);
+            out.att = albedo_of(sc, M, s);
             out.pdf = 0.0f;
             return dot(r, s.n) > 0.0f;
         }
@@ -228,14 +304,19 @@ __device__ __forceinline__ bool scatter_basic(const DevMaterial& M, V3 din, cons
     }
 }
 
-// Material.Scatter including Layered (layered.go:24-69). Draws start at pos.
+// Material.Scatter including Layered (layered.go:24-69) and Mix (mix.go:30-37, one draw picks
+// material2 with probability ratio). Draws start at pos.
 __device__ __forceinline__ bool scatter(const DevScene& sc, int m, V3 din, const Surf& s, uint32_t key, uint32_t pos,
                                         ScatterOut& out) {
     const DevMaterial& M = sc.mats[m];
     uint32_t used = 0;
-    if (M.type != 4) return scatter_basic(M, din, s, key, pos, used, out);
+    if (M.type == kMatMix) {
+        const int pick = draw(key, pos) < M.ratio ? M.inner : M.outer;
+        return scatter_basic(sc, sc.mats[pick], din, s, key, pos + 1, used, out);
+    }
+    if (M.type != 4) return scatter_basic(sc, M, din, s, key, pos, used, out);
     ScatterOut o;
-    if (!scatter_basic(sc.mats[M.outer], din, s, key, pos, used, o)) return false;
+    if (!scatter_basic(sc, sc.mats[M.outer], din, s, key, pos, used, o)) return false;
     V3 sd = normalize(o.dir);
     if (!(dot(sd, s.n) < 0.0f)) {
         out = o;
@@ -243,7 +324,7 @@ __device__ __forceinline__ bool scatter(const DevScene& sc, int m, V3 din, const
     }
     ScatterOut i;
     uint32_t used2 = 0;
-    if (!scatter_basic(sc.mats[M.inner], sd, s, key, pos + used, used2, i)) {
+    if (!scatter_basic(sc, sc.mats[M.inner], sd, s, key, pos + used, used2, i)) {
         out = o;
         return true;
     }
@@ -253,14 +334,15 @@ __device__ __forceinline__ bool scatter(const DevScene& sc, int m, V3 din, const
     return true;
 }
 
-__device__ __forceinline__ V3 brdf_basic(const DevMaterial& M, V3 in, V3 outd, const Surf& s, int mode) {
+__device__ __forceinline__ V3 brdf_basic(const DevScene& sc, const DevMaterial& M, V3 in, V3 outd, const Surf& s,
+                                        int mode) {
     switch (M.type) {
         case 0:  // lambertian.go:53-63
             if (dot(outd, s.n) <= 0.0f) return {0, 0, 0};
-            return v3(M.albedo) * (1.0f / kPiF);
+            return albedo_of(sc, M, s) * (1.0f / kPiF);
         case 1: {  // metal.go:62-74
             V3 r = reflect(in * -1.0f, s.n);
-            if (length(outd - r) < 0.001f) return v3(M.albedo);
+            if (length(outd - r) < 0.001f) return albedo_of(sc, M, s);
             return {0, 0, 0};
         }
         case 2: {  // dielectric.go:63-143
@@ -293,9 +375,13 @@ __device__ __forceinline__ V3 brdf_basic(const DevMaterial& M, V3 in, V3 outd, c
 }
 __device__ __forceinline__ V3 brdf(const DevScene& sc, int m, V3 in, V3 outd, const Surf& s, int mode) {
     const DevMaterial& M = sc.mats[m];
-    if (M.type != 4) return brdf_basic(M, in, outd, s, mode);
-    if (is_reflection_path(in, outd, s.n)) return brdf_basic(sc.mats[M.outer], in, outd, s, mode);  // layered.go:73-87
-    return brdf_basic(sc.mats[M.inner], in, outd, s, mode);
+    if (M.type == kMatMix)  // mix.go:40-46
+        return brdf_basic(sc, sc.mats[M.outer], in, outd, s, mode) * (1.0f - M.ratio) +
+               brdf_basic(sc, sc.mats[M.inner], in, outd, s, mode) * M.ratio;
+    if (M.type != 4) return brdf_basic(sc, M, in, outd, s, mode);
+    if (is_reflection_path(in, outd, s.n))
+        return brdf_basic(sc, sc.mats[M.outer], in, outd, s, mode);  // layered.go:73-87
+    return brdf_basic(sc, sc.mats[M.inner], in, outd, s, mode);
 }
 __device__ __forceinline__ float pdf_basic(const DevMaterial& M, V3 outd, V3 n, bool& delta) {
     if (M.type == 0) {  // lambertian.go:66-72
@@ -308,6 +394,13 @@ __device__ __forceinline__ float pdf_basic(const DevMaterial& M, V3 outd, V3 n, 
 }
 __device__ __forceinline__ float mat_pdf(const DevScene& sc, int m, V3 in, V3 outd, V3 n, bool& delta) {
     const DevMaterial& M = sc.mats[m];
+    if (M.type == kMatMix) {  // mix.go:49-56: never delta
+        bool d1, d2;
+        float p = pdf_basic(sc.mats[M.outer], outd, n, d1) * (1.0f - M.ratio) +
+                  pdf_basic(sc.mats[M.inner], outd, n, d2) * M.ratio;
+        delta = false;
+        return p;
+    }
     if (M.type != 4) return pdf_basic(M, outd, n, delta);
     bool d2;
     float p = is_reflection_path(in, outd, n) ? pdf_basic(sc.mats[M.outer], outd, n, d2)

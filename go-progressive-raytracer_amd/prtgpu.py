@@ -21,6 +21,7 @@ ERRORS = {1: "invalid argument", 2: "device error", 3: "out of memory", 4: "no d
           5: "unsupported", 6: "cancelled"}
 
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_EMISSIVE, MAT_LAYERED = range(5)
+MAT_MIX = 6
 LIGHT_QUAD, LIGHT_SPHERE, LIGHT_UNIFORM_INFINITE, LIGHT_GRADIENT_INFINITE, LIGHT_ENVMAP = range(5)
 LIGHT_DISC, LIGHT_DISC_SPOT, LIGHT_POINT_SPOT = 5, 6, 7
 INTEGRATOR_PATH, INTEGRATOR_BDPT = 0, 1
@@ -37,8 +38,13 @@ D3 = C.c_double * 3
 
 
 class Material(C.Structure):
-    _fields_ = [("type", C.c_int32), ("outer", C.c_int32), ("inner", C.c_int32), ("pad", C.c_int32),
-                ("albedo", D3), ("fuzz", C.c_double), ("ior", C.c_double), ("emission", D3)]
+    _fields_ = [("type", C.c_int32), ("outer", C.c_int32), ("inner", C.c_int32), ("texture", C.c_int32),
+                ("albedo", D3), ("fuzz", C.c_double), ("ior", C.c_double), ("emission", D3),
+                ("ratio", C.c_double)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", C.POINTER(C.c_float))]
 
 
 class Sphere(C.Structure):
@@ -58,7 +64,7 @@ class Mesh(C.Structure):
                 ("faces", C.POINTER(C.c_int32)), ("num_faces", C.c_int64),
                 ("face_materials", C.POINTER(C.c_int32)), ("material", C.c_int32), ("pad", C.c_int32),
                 ("normals", C.POINTER(C.c_double)), ("rotation", C.POINTER(C.c_double)),
-                ("center", C.POINTER(C.c_double))]
+                ("center", C.POINTER(C.c_double)), ("uvs", C.POINTER(C.c_double))]
 
 
 class Disc(C.Structure):
@@ -99,7 +105,8 @@ class SceneDescC(C.Structure):
                 ("camera", Camera),
                 ("discs", C.POINTER(Disc)), ("num_discs", C.c_int32),
                 ("cylinders", C.POINTER(Cylinder)), ("num_cylinders", C.c_int32),
-                ("cones", C.POINTER(Cone)), ("num_cones", C.c_int32)]
+                ("cones", C.POINTER(Cone)), ("num_cones", C.c_int32),
+                ("textures", C.POINTER(Texture)), ("num_textures", C.c_int32)]
 
 
 class SceneInfo(C.Structure):
@@ -267,6 +274,7 @@ class SceneDesc:
         self.discs = []
         self.cylinders = []
         self.cones = []
+        self.textures = []
         self.meshes = []  # (vertices (n,3), faces (m,3), material, face_materials|None, options dict)
         self.lights = []
         self.light_weights = None
@@ -281,14 +289,30 @@ class SceneDesc:
         m.fuzz = kw.get("fuzz", 0.0)
         m.ior = kw.get("ior", 1.0)
         m.emission = D3(*kw.get("emission", (0, 0, 0)))
+        m.texture = kw.get("texture", 0)
+        m.ratio = kw.get("ratio", 0.0)
         self.materials.append(m)
         return len(self.materials) - 1
 
-    def lambertian(self, albedo):
-        return self._mat(type=MAT_LAMBERTIAN, albedo=albedo)
+    def texture(self, rgb):
+        """An ImageTexture (image_texture.go:15-22) from an (height, width, 3) array, row 0 at the top;
+        returns the value to pass as `texture=` (1-based)."""
+        a = np.ascontiguousarray(rgb, dtype=np.float32)
+        assert a.ndim == 3 and a.shape[2] == 3
+        self.textures.append(a)
+        return len(self.textures)
 
-    def metal(self, albedo, fuzz):
-        return self._mat(type=MAT_METAL, albedo=albedo, fuzz=min(1.0, max(0.0, fuzz)))
+    def lambertian(self, albedo, texture=0):
+        """NewLambertian, or NewTexturedLambertian (lambertian.go:15-22) with texture > 0."""
+        return self._mat(type=MAT_LAMBERTIAN, albedo=albedo, texture=texture)
+
+    def metal(self, albedo, fuzz, texture=0):
+        """NewMetal, or NewTexturedMetal (metal.go:14-35) with texture > 0."""
+        return self._mat(type=MAT_METAL, albedo=albedo, fuzz=min(1.0, max(0.0, fuzz)), texture=texture)
+
+    def mix(self, material1, material2, ratio):
+        """NewMix (mix.go:17-27): material2 with probability ratio (clamped to [0, 1])."""
+        return self._mat(type=MAT_MIX, outer=material1, inner=material2, ratio=min(1.0, max(0.0, ratio)))
 
     def dielectric(self, ior):
         return self._mat(type=MAT_DIELECTRIC, ior=ior)
@@ -342,14 +366,15 @@ class SceneDesc:
         self.boxes.append(b)
         return len(self.boxes) - 1
 
-    def mesh(self, vertices, faces, material, face_materials=None, normals=None, rotation=None, center=None):
+    def mesh(self, vertices, faces, material, face_materials=None, normals=None, rotation=None, center=None,
+             uvs=None):
         """NewTriangleMesh (triangle_mesh.go:33-146) with TriangleMeshOptions: per-triangle normals,
-        rotation (radians, X then Y then Z) about `center`."""
+        rotation (radians, X then Y then Z) about `center`, per-vertex texture coordinates."""
         v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
         f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
         fm = None if face_materials is None else np.ascontiguousarray(face_materials, dtype=np.int32)
         opt = {k: (None if x is None else np.ascontiguousarray(x, dtype=np.float64).reshape(-1))
-               for k, x in (("normals", normals), ("rotation", rotation), ("center", center))}
+               for k, x in (("normals", normals), ("rotation", rotation), ("center", center), ("uvs", uvs))}
         self.meshes.append((v, f, material, fm, opt))
         return len(self.meshes) - 1
 
@@ -458,6 +483,14 @@ class SceneDesc:
         d.discs, d.num_discs = arr(Disc, self.discs), len(self.discs)
         d.cylinders, d.num_cylinders = arr(Cylinder, self.cylinders), len(self.cylinders)
         d.cones, d.num_cones = arr(Cone, self.cones), len(self.cones)
+        texs = []
+        for a in self.textures:
+            T = Texture()
+            T.height, T.width = a.shape[0], a.shape[1]
+            T.rgb = a.ctypes.data_as(C.POINTER(C.c_float))
+            keep.append(a)
+            texs.append(T)
+        d.textures, d.num_textures = arr(Texture, texs), len(texs)
         d.meshes, d.num_meshes = arr(Mesh, meshes), len(meshes)
         d.lights, d.num_lights = arr(Light, self.lights), len(self.lights)
         if self.light_weights is not None:
@@ -687,3 +720,24 @@ def hdr_sky(width=1024, height=512, seed=0, sun_peak=1.0e4):
     rgb = np.empty((height, width, 3), np.float32)
     check(lib().prt_workload_sky(width, height, seed, sun_peak, _ptr(rgb)))
     return rgb
+
+
+# ---- procedural textures (pkg/material/procedural_textures.go) ----------------------------------
+def checkerboard_texture(width, height, check_size, color1, color2):
+    """NewCheckerboardTexture (:8-31): color1 where (x / check + y / check) is even."""
+    y, x = np.mgrid[0:height, 0:width]
+    even = ((x // check_size + y // check_size) % 2) == 0
+    return np.where(even[..., None], np.array(color1, float), np.array(color2, float))
+
+
+def uv_debug_texture(width, height):
+    """NewUVDebugTexture (:34-46): (x / (width - 1), y / (height - 1), 0)."""
+    y, x = np.mgrid[0:height, 0:width]
+    return np.stack([x / (width - 1), y / (height - 1), np.zeros((height, width))], -1)
+
+
+def gradient_texture(width, height, color1, color2):
+    """NewGradientTexture (:49-64): color1 at the top row to color2 at the bottom row."""
+    t = (np.arange(height) / (height - 1))[:, None, None]
+    row = np.array(color1, float) * (1.0 - t) + np.array(color2, float) * t
+    return np.broadcast_to(row, (height, width, 3)).copy()

@@ -268,3 +268,83 @@ def cone_scene(width=400):
     s.add_gradient_infinite_light((0.5, 0.7, 1.0), (1.0, 1.0, 1.0))
     cfg = dict(max_depth=50, rr_min_bounces=20, spp=200, adaptive_min_samples=0.15, adaptive_threshold=0.01)
     return s, cfg
+
+
+def texture_scene(width=800):
+    """NewTextureTestScene (pkg/scene/texture_test_scene.go:11-161): a checkerboard sphere, a gradient
+    cylinder, a UV-debug frustum, a brick-checker box, a checkerboard disc, a gradient quad and a
+    UV-debug triangle on a brick-checker ground; a sphere light and a dim gradient sky; 100 spp,
+    depth 10, Russian roulette from bounce 5."""
+    s = P.SceneDesc()
+    s.set_camera(center=(0, 2, 10), look_at=(0, 1, 0), up=(0, 1, 0), width=width, aspect_ratio=16.0 / 9.0,
+                 vfov=50.0, aperture=0.0, focus_distance=0.0)
+    checker = s.texture(P.checkerboard_texture(256, 256, 32, (0.9, 0.9, 0.9), (0.2, 0.2, 0.8)))
+    gradient = s.texture(P.gradient_texture(256, 256, (1.0, 0.2, 0.2), (0.2, 1.0, 0.2)))
+    uvdebug = s.texture(P.uv_debug_texture(256, 256))
+    brick = s.texture(P.checkerboard_texture(512, 512, 16, (0.7, 0.3, 0.1), (0.5, 0.2, 0.05)))
+    checker_m = s.lambertian((0, 0, 0), texture=checker)
+    gradient_m = s.lambertian((0, 0, 0), texture=gradient)
+    uvdebug_m = s.lambertian((0, 0, 0), texture=uvdebug)
+    brick_m = s.lambertian((0, 0, 0), texture=brick)
+    s.sphere((-6, 1, 0), 1.0, checker_m)
+    s.cylinder((-4, 0, 0), (-4, 2, 0), 0.6, True, gradient_m)
+    s.cone((-2, 0, 0), 0.8, (-2, 2, 0), 0.2, True, uvdebug_m)
+    s.box((0, 1, 0), (0.8, 0.8, 0.8), (0, 0, 0), brick_m)
+    s.disc((2, 1.2, 0), (0, 0, 1), 0.9, checker_m)
+    s.quad((3.5, 0, 0.2), (1.5, 0, -0.3), (0, 2, 0), gradient_m)
+    s.mesh(np.array([[5.5, 0, 0], [7, 0, 0], [6.25, 2, 0]], float), np.array([[0, 1, 2]]), uvdebug_m)
+    s.quad((-10, 0, -5), (20, 0, 0), (0, 0, 15), brick_m)
+    s.add_sphere_light((0, 8, 5), 2.0, (20, 20, 20))
+    s.add_gradient_infinite_light((0.3, 0.4, 0.6), (0.2, 0.2, 0.2))
+    cfg = dict(max_depth=10, rr_min_bounces=5, spp=100, adaptive_min_samples=0.15, adaptive_threshold=0.01)
+    return s, cfg
+
+
+def oklch_to_rgb(l, c, h):
+    """OKLCh -> linear sRGB through OKLab (Ottosson's published matrices), clamped to [0, 1]; the
+    sphere grid's palette (spheregrid.go:11-38)."""
+    a, b = c * math.cos(math.radians(h)), c * math.sin(math.radians(h))
+    lms = np.array([l + 0.3963377774 * a + 0.2158037573 * b,
+                    l - 0.1055613458 * a - 0.0638541728 * b,
+                    l - 0.0894841775 * a - 1.2914855480 * b]) ** 3
+    m = np.array([[4.0767416621, -3.3077115913, 0.2309699292],
+                  [-1.2684380046, 2.6097574011, -0.3413193965],
+                  [-0.0041960863, -0.7034186147, 1.7076147010]])
+    return tuple(float(x) for x in np.clip(m @ lms, 0.0, 1.0))
+
+
+def sphere_grid_scene(grid_size=10, finish="metallic", width=800):
+    """NewSphereGridScene (pkg/scene/spheregrid.go:41-160): grid_size^2 spheres over a 9x9 area,
+    OKLCh hues; finish = matte | glossy (glass over Lambertian) | mirror | glass | mixed (50/50
+    Lambertian/metal Mix) | metallic; a sphere light, a gradient sky, f/0.02 lens; 100 spp, depth 40."""
+    s = P.SceneDesc()
+    s.set_camera(center=(4.5, 6, 18), look_at=(4.5, 0.8, 4.5), up=(0, 1, 0), width=width, aspect_ratio=16.0 / 9.0,
+                 vfov=40.0, aperture=0.02, focus_distance=0.0)
+    s.add_sphere_light((20, 25, 20), 8, (8.0, 7.5, 6.5))
+    ground_quad(s, (0, 0, 0), 10000.0, s.lambertian((0.5, 0.5, 0.5)))
+    area = 9.0
+    spacing = area / (grid_size - 1)
+    radius = max(0.02, min(0.35, spacing * 0.35))
+    for i in range(grid_size):
+        for j in range(grid_size):
+            x = i * spacing - area / 2.0 + 4.5
+            z = j * spacing - area / 2.0 + 4.5
+            hue = math.fmod(i / (grid_size - 1) * 330.0 + j / (grid_size - 1) * 120.0, 360.0)
+            chroma = 0.05 + 0.8 * 0.2 + 0.2 * 0.2 * (j / (grid_size - 1))
+            color = oklch_to_rgb(0.65, chroma, hue)
+            if finish == "matte":
+                m = s.lambertian(color)
+            elif finish == "glossy":
+                m = s.layered(s.dielectric(1.5), s.lambertian(color))
+            elif finish == "mirror":
+                m = s.metal((0.9, 0.9, 0.9), 0.0)
+            elif finish == "glass":
+                m = s.dielectric(1.5)
+            elif finish == "mixed":
+                m = s.mix(s.lambertian(color), s.metal(color, 0.1), 0.5)
+            else:
+                m = s.metal(color, 0.05)
+            s.sphere((x, radius, z), radius, m)
+    s.add_gradient_infinite_light((0.5, 0.7, 1.0), (1.0, 1.0, 1.0))
+    cfg = dict(max_depth=40, rr_min_bounces=12, spp=100, adaptive_min_samples=0.12, adaptive_threshold=0.015)
+    return s, cfg

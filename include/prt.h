@@ -76,17 +76,29 @@ typedef struct prt_hit {
 #define PRT_MAT_EMISSIVE 3
 #define PRT_MAT_LAYERED 4
 /* (type 5 is internal: the disc of a disc spot light, discSpotLightMaterial, disc_spot_light.go:12-46) */
+#define PRT_MAT_MIX 6 /* Mix (mix.go:9-56): material2 with probability ratio, else material1 */
 
 typedef struct prt_material {
     int32_t type;
-    int32_t outer;      /* layered: index of the outer material */
-    int32_t inner;      /* layered: index of the inner material */
-    int32_t pad;
+    int32_t outer;      /* layered: index of the outer material; mix: material1 */
+    int32_t inner;      /* layered: index of the inner material; mix: material2 */
+    int32_t texture;    /* lambertian / metal: 0 = the solid albedo, k > 0 = textures[k - 1]
+                           (NewTexturedLambertian lambertian.go:20-22, NewTexturedMetal metal.go:26-35) */
     double albedo[3];   /* lambertian / metal */
     double fuzz;        /* metal: clamped to [0,1] as NewMetal does (metal.go:19-29) */
     double ior;         /* dielectric */
     double emission[3]; /* emissive */
+    double ratio;       /* mix: clamped to [0,1] as NewMix does (mix.go:17-27) */
 } prt_material;
+
+/* ImageTexture (pkg/material/image_texture.go:8-55): row-major RGB, row 0 at the top (v = 1),
+ * sampled nearest-neighbour with wrapped UVs. NewCheckerboardTexture / NewUVDebugTexture /
+ * NewGradientTexture (procedural_textures.go) build such images on the host. */
+typedef struct prt_texture {
+    int32_t width;
+    int32_t height;
+    const float* rgb; /* width * height * 3 */
+} prt_texture;
 
 typedef struct prt_sphere { /* pkg/geometry/sphere.go:11-16; a negative radius flips the normal */
     double center[3];
@@ -148,6 +160,7 @@ typedef struct prt_mesh { /* pkg/geometry/triangle_mesh.go:33-146; primitive id 
     const double* normals;         /* num_faces * 3 custom triangle normals (normalized on use) */
     const double* rotation;        /* 3 radians, applied X then Y then Z (rotateVertex :170-199) */
     const double* center;          /* 3, rotation centre (default the origin) */
+    const double* uvs;             /* num_vertices * 2 per-vertex texture coordinates (VertexUVs) */
 } prt_mesh;
 
 /* Light types: pkg/lights/{quad_light,sphere_light,uniform_infinite_light,
@@ -203,6 +216,7 @@ typedef struct prt_scene_desc {
     const prt_disc* discs;         int32_t num_discs;
     const prt_cylinder* cylinders; int32_t num_cylinders;
     const prt_cone* cones;         int32_t num_cones;
+    const prt_texture* textures;   int32_t num_textures;
 } prt_scene_desc;
 
 /* Global primitive numbering used by prt_hit.prim_id: all mesh triangles (mesh by

@@ -202,6 +202,8 @@ struct Triangle : Shape {
     AABB box;
     int mat;
     int64_t id;
+    bool has_uv = false;  // NewTriangleWithUVs (triangle.go:51-69)
+    Vec2 uv0, uv1, uv2;
     Triangle(Vec3 a, Vec3 b, Vec3 c, int m, int64_t i) : v0(a), v1(b), v2(c), mat(m), id(i) {
         normal = (v1 - v0).cross(v2 - v0).normalize();  // computeNormal :92-96
         Vec3 p[3] = {v0, v1, v2};
@@ -226,7 +228,12 @@ struct Triangle : Shape {
         h.t = t;
         h.p = r.at(t);
         h.mat = mat;
-        h.uv = {u, v};
+        if (has_uv) {  // barycentric interpolation of the vertex UVs, :151-161
+            double w = 1.0 - u - v;
+            h.uv = {uv0.x * w + uv1.x * u + uv2.x * v, uv0.y * w + uv1.y * u + uv2.y * v};
+        } else {
+            h.uv = {u, v};
+        }
         h.prim = id;
         h.set_face_normal(r, normal);
         return true;
@@ -720,6 +727,13 @@ struct Mesh : Shape {
             int mat = m.face_materials ? m.face_materials[i] : m.material;
             tris.emplace_back(new Triangle(verts[f[0]], verts[f[1]], verts[f[2]], mat, first_id + i));
             if (m.normals) tris.back()->normal = V(m.normals + 3 * i).normalize();  // NewTriangleWithNormal :35-49
+            if (m.uvs) {  // TriangleMeshOptions.VertexUVs, triangle_mesh.go:97-112
+                Triangle& t = *tris.back();
+                t.has_uv = true;
+                t.uv0 = {m.uvs[2 * f[0]], m.uvs[2 * f[0] + 1]};
+                t.uv1 = {m.uvs[2 * f[1]], m.uvs[2 * f[1] + 1]};
+                t.uv2 = {m.uvs[2 * f[2]], m.uvs[2 * f[2] + 1]};
+            }
             sh.push_back(tris.back().get());
         }
         bvh.reset(new BVH(sh));
@@ -786,10 +800,27 @@ struct Camera {
 static const int kMatSpotEmitter = 5;  // discSpotLightMaterial, disc_spot_light.go:12-46
 struct Material {
     int type = 0, outer = -1, inner = -1;
+    int tex = 0;  // 0: solid albedo, k > 0: texture k - 1
     Vec3 albedo, emission;
-    double fuzz = 0, ior = 1;
+    double fuzz = 0, ior = 1, ratio = 0;
     Vec3 spot_dir;  // kMatSpotEmitter: the spot direction
 };
+struct Texture {
+    int w = 0, h = 0;
+    std::vector<double> rgb;  // h*w*3, row 0 at the top
+};
+// ImageTexture.Evaluate (image_texture.go:24-55): wrap to [0, 1) by truncation, flip v, nearest
+// texel, clamp.
+static Vec3 texture_eval(const Texture& t, Vec2 uv) {
+    double u = uv.x - (double)(int64_t)uv.x, v = uv.y - (double)(int64_t)uv.y;
+    if (u < 0) u += 1.0;
+    if (v < 0) v += 1.0;
+    int64_t x = (int64_t)(u * t.w), y = (int64_t)((1.0 - v) * t.h);
+    x = std::max<int64_t>(0, std::min<int64_t>(x, t.w - 1));
+    y = std::max<int64_t>(0, std::min<int64_t>(y, t.h - 1));
+    const double* p = &t.rgb[3 * (size_t)(y * t.w + x)];
+    return {p[0], p[1], p[2]};
+}
 struct Scatter {
     Ray in, out;
     Vec3 att;
@@ -857,9 +888,15 @@ struct Scene {
     std::unique_ptr<BVH> bvh;
     std::vector<Light> lights;
     std::vector<double> weights;
+    std::vector<Texture> textures;
     Camera cam;
     int64_t num_prims = 0;
 };
+
+// Albedo.Evaluate(hit.UV, hit.Point): SolidColor or ImageTexture (color_source.go, image_texture.go)
+static Vec3 albedo_at(const Scene& S, const Material& M, const Hit& h) {
+    return M.tex > 0 ? texture_eval(S.textures[M.tex - 1], h.uv) : M.albedo;
+}
 
 // Emitter.Emit for the ray direction d: emissive.go:30-37; the disc spot light's material additionally
 // emits only against its spot direction (rayIn.Direction . spot <= -0.3, disc_spot_light.go:26-40)
@@ -883,7 +920,7 @@ static bool mat_scatter(const Scene& S, int m, const Ray& in, const Hit& h, Samp
             out.in = in;
             out.out = {h.p, d};
             out.pdf = ct / kPi;
-            out.att = M.albedo * (1.0 / kPi);
+            out.att = albedo_at(S, M, h) * (1.0 / kPi);
             return true;
         }
         case PRT_MAT_METAL: {  // metal.go:38-59
@@ -891,7 +928,7 @@ static bool mat_scatter(const Scene& S, int m, const Ray& in, const Hit& h, Samp
             if (M.fuzz > 0) r = r + sample_point_in_unit_sphere(smp.get3d()) * M.fuzz;
             out.in = in;
             out.out = {h.p, r};
-            out.att = M.albedo;
+            out.att = albedo_at(S, M, h);
             out.pdf = 0;
             return r.dot(h.n) > 0;
         }
@@ -937,6 +974,11 @@ static bool mat_scatter(const Scene& S, int m, const Ray& in, const Hit& h, Samp
             out.pdf = i.pdf;
             return true;
         }
+        case PRT_MAT_MIX: {  // mix.go:30-37: one draw picks the material
+            Hit mh = h;
+            mh.mat = smp.get1d() < M.ratio ? M.inner : M.outer;
+            return mat_scatter(S, mh.mat, in, mh, smp, out);
+        }
     }
     return false;
 }
@@ -946,11 +988,11 @@ static Vec3 mat_brdf(const Scene& S, int m, Vec3 in, Vec3 outd, const Hit& h, in
     switch (M.type) {
         case PRT_MAT_LAMBERTIAN: {  // lambertian.go:53-63
             if (outd.dot(h.n) <= 0) return {0, 0, 0};
-            return M.albedo * (1.0 / kPi);
+            return albedo_at(S, M, h) * (1.0 / kPi);
         }
         case PRT_MAT_METAL: {  // metal.go:62-74
             Vec3 r = reflect(in.neg(), h.n);
-            if ((outd - r).length() < 0.001) return M.albedo;
+            if ((outd - r).length() < 0.001) return albedo_at(S, M, h);
             return {0, 0, 0};
         }
         case PRT_MAT_DIELECTRIC: {  // dielectric.go:63-143
@@ -986,6 +1028,9 @@ static Vec3 mat_brdf(const Scene& S, int m, Vec3 in, Vec3 outd, const Hit& h, in
         case PRT_MAT_LAYERED:  // layered.go:73-87
             if (is_reflection_path(in, outd, h.n)) return mat_brdf(S, M.outer, in, outd, h, mode);
             return mat_brdf(S, M.inner, in, outd, h, mode);
+        case PRT_MAT_MIX:  // mix.go:40-46
+            return mat_brdf(S, M.outer, in, outd, h, mode) * (1.0 - M.ratio) +
+                   mat_brdf(S, M.inner, in, outd, h, mode) * M.ratio;
     }
     return {0, 0, 0};
 }
@@ -1012,6 +1057,12 @@ static double mat_pdf(const Scene& S, int m, Vec3 in, Vec3 outd, Vec3 n, bool& d
             double p;
             if (is_reflection_path(in, outd, n)) p = mat_pdf(S, M.outer, in, outd, n, d2);
             else p = mat_pdf(S, M.inner, in, outd, n, d2);
+            delta = false;
+            return p;
+        }
+        case PRT_MAT_MIX: {  // mix.go:49-56: never delta
+            bool d1, d2;
+            double p = mat_pdf(S, M.outer, in, outd, n, d1) * (1.0 - M.ratio) + mat_pdf(S, M.inner, in, outd, n, d2) * M.ratio;
             delta = false;
             return p;
         }
@@ -1674,6 +1725,7 @@ static void light_pdf_le(const Light& L, Vec3 p, Vec3 d, double& pos, double& di
 struct BVertex {
     Vec3 p, n, wi, beta, Le;
     int mat = -1;  // -1: no material (camera, light sample, background)
+    Vec2 uv;
     bool front = true;
     const Light* light = nullptr;
     int light_index = 0;
@@ -1689,6 +1741,7 @@ struct BVertex {
         h.n = n;
         h.front = front;
         h.mat = mat;
+        h.uv = uv;
         return h;
     }
 };
@@ -1749,6 +1802,7 @@ static void extend_path(const BDPTCtx& C, BPath& path, Ray ray, Vec3 beta, doubl
         v.n = h.n;
         v.front = h.front;
         v.mat = h.mat;
+        v.uv = h.uv;
         v.wi = ray.d * -1.0;
         v.beta = beta;
         v.Le = is_emitter(S, h.mat) ? emissive_emit(S, h.mat, &h, ray.d) : Vec3(0, 0, 0);
@@ -2118,7 +2172,17 @@ orc_scene* orc_scene_create(const prt_scene_desc* d) {
         m.emission = V(pm.emission);
         m.fuzz = std::min(1.0, std::max(0.0, pm.fuzz));
         m.ior = pm.ior;
+        m.tex = pm.texture;
+        m.ratio = std::min(1.0, std::max(0.0, pm.ratio));  // NewMix clamps (mix.go:17-27)
         S.mats.push_back(m);
+    }
+    for (int i = 0; i < d->num_textures; i++) {
+        const prt_texture& pt = d->textures[i];
+        Texture t;
+        t.w = pt.width;
+        t.h = pt.height;
+        t.rgb.assign(pt.rgb, pt.rgb + (size_t)pt.width * pt.height * 3);
+        S.textures.push_back(std::move(t));
     }
     int64_t id = 0;
     for (int i = 0; i < d->num_meshes; i++) {
@@ -2506,6 +2570,15 @@ void orc_disc_frame(const double* g, double* out) {
     out[0] = dc.right.x; out[1] = dc.right.y; out[2] = dc.right.z;
     out[3] = dc.up.x; out[4] = dc.up.y; out[5] = dc.up.z;
     out[6] = b.mn.x; out[7] = b.mn.y; out[8] = b.mn.z; out[9] = b.mx.x; out[10] = b.mx.y; out[11] = b.mx.z;
+}
+// ImageTexture.Evaluate of a width x height float RGB image at (u, v): out = rgb.
+void orc_texture_eval(const float* rgb, int32_t width, int32_t height, double u, double v, double* out) {
+    Texture t;
+    t.w = width;
+    t.h = height;
+    t.rgb.assign(rgb, rgb + (size_t)width * height * 3);
+    Vec3 c = texture_eval(t, Vec2{u, v});
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 // Cylinder (kind 4) or cone (kind 5) box in the layout of orc_shape_hit: out = min(3), max(3).
 void orc_axial_bbox(int32_t kind, const double* g, double* out) {

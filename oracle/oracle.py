@@ -49,6 +49,7 @@ def lib():
             "orc_quad_bbox": (None, [vp, vp]),
             "orc_disc_frame": (None, [vp, vp]),
             "orc_axial_bbox": (None, [i32, vp, vp]),
+            "orc_texture_eval": (None, [vp, i32, i32, d, d, vp]),
             "orc_light_sample_emission": (None, [vp, i32, vp, vp, vp]),
             "orc_light_pdf_le": (None, [vp, i32, vp, vp, vp]),
             "orc_aabb_hit": (i32, [vp, vp, vp, d, d]),
