@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libprt.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["api.cpp", "bvh_build.cpp", "workload.cpp", "ply.cpp", "server.cpp", "scene.hip", "pt.hip"]
+SOURCES = ["api.cpp", "bvh_build.cpp", "workload.cpp", "ply.cpp", "server.cpp", "sse.cpp", "scene.hip", "pt.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PRT_OFFLOAD_ARCH", "gfx950")
@@ -48,7 +48,7 @@ def build(jobs=None, verbose=False):
                 print(errs, file=sys.stderr)
             objs.append(obj)
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread"]
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

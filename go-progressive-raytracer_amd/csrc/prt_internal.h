@@ -72,4 +72,8 @@ double film_chunk_ms(prt_film* f);           // accumulate the finished chunk's 
 bool film_idle(prt_film* f);                 // all enqueued work finished
 int film_wait(prt_film* f, float* ms);       // wait for enqueued work; device time of the last enqueue
 int32_t film_spp_done(const prt_film* f);
+size_t film_snapshot_bytes(const prt_film* f);
+int film_snapshot(prt_film* f, void* dst, hipEvent_t done);  // enqueue copies of the accumulators, record done
+int film_read_snapshot(const prt_film* f, const void* src, hipEvent_t done, hipStream_t stream, float* mean_rgb,
+                       uint32_t* counts);
 }  // namespace prt

@@ -869,6 +869,57 @@ struct prt_film {
 
 namespace prt {
 
+// GetColor: (samples + splats) / SampleCount, zero when no samples (stats.go:36-55)
+void film_mean(int64_t npix, const float4* sum, const uint32_t* cnt, const long long* spl, float* mean_rgb) {
+    for (int64_t i = 0; i < npix; i++) {
+        float inv = cnt[i] ? 1.0f / (float)cnt[i] : 0.0f;
+        for (int c = 0; c < 3; c++) {
+            float v = c == 0 ? sum[i].x : (c == 1 ? sum[i].y : sum[i].z);
+            if (spl) v += (float)((double)spl[3 * i + c] * (1.0 / (double)kSplatScale));
+            mean_rgb[3 * i + c] = cnt[i] ? v * inv : 0.f;
+        }
+    }
+}
+
+// Film snapshots: device copies of the accumulators (sums, counts, splats) taken on the film's stream,
+// so that they hold the film as it was when the work enqueued before them finished.
+size_t film_snapshot_bytes(const prt_film* f) {
+    return (size_t)f->npix * (sizeof(float4) + sizeof(uint32_t) + (f->fb.splat ? 3 * sizeof(long long) : 0));
+}
+int film_snapshot(prt_film* f, void* dst, hipEvent_t done) {
+    char* p = (char*)dst;
+    PRT_HIP(hipSetDevice(f->scene->device));
+    PRT_HIP(hipMemcpyAsync(p, f->fb.sum, sizeof(float4) * f->npix, hipMemcpyDeviceToDevice, f->stream));
+    p += sizeof(float4) * f->npix;
+    PRT_HIP(hipMemcpyAsync(p, f->fb.count, sizeof(uint32_t) * f->npix, hipMemcpyDeviceToDevice, f->stream));
+    p += sizeof(uint32_t) * f->npix;
+    if (f->fb.splat)
+        PRT_HIP(hipMemcpyAsync(p, f->fb.splat, 3 * sizeof(long long) * f->npix, hipMemcpyDeviceToDevice, f->stream));
+    PRT_HIP(hipEventRecord(done, f->stream));
+    return PRT_OK;
+}
+// Reads a snapshot (after `done`) through `stream`: mean RGB as prt_film_read gives it, and counts.
+int film_read_snapshot(const prt_film* f, const void* src, hipEvent_t done, hipStream_t stream, float* mean_rgb,
+                       uint32_t* counts) {
+    PRT_HIP(hipSetDevice(f->scene->device));
+    std::vector<float4> sum(f->npix);
+    std::vector<uint32_t> cnt(f->npix);
+    std::vector<long long> spl(f->fb.splat ? 3 * f->npix : 0);
+    const char* p = (const char*)src;
+    PRT_HIP(hipStreamWaitEvent(stream, done, 0));
+    PRT_HIP(hipMemcpyAsync(sum.data(), p, sizeof(float4) * f->npix, hipMemcpyDeviceToHost, stream));
+    p += sizeof(float4) * f->npix;
+    PRT_HIP(hipMemcpyAsync(cnt.data(), p, sizeof(uint32_t) * f->npix, hipMemcpyDeviceToHost, stream));
+    p += sizeof(uint32_t) * f->npix;
+    if (f->fb.splat)
+        PRT_HIP(hipMemcpyAsync(spl.data(), p, 3 * sizeof(long long) * f->npix, hipMemcpyDeviceToHost, stream));
+    PRT_HIP(hipStreamSynchronize(stream));
+    if (mean_rgb) film_mean(f->npix, sum.data(), cnt.data(), f->fb.splat ? spl.data() : nullptr, mean_rgb);
+    if (counts) std::memcpy(counts, cnt.data(), sizeof(uint32_t) * f->npix);
+    return PRT_OK;
+}
+
+
 namespace {
 
 int film_alloc(prt_film* f, size_t bytes, void** out) {
@@ -1433,16 +1484,8 @@ int prt_film_read(const prt_film* f, float* mean_rgb, float* lum_sum, float* lum
         PRT_HIP(hipMemcpyAsync(spl.data(), f->fb.splat, sizeof(long long) * 3 * f->npix, hipMemcpyDeviceToHost,
                                f->stream));
     PRT_HIP(hipStreamSynchronize(f->stream));
+    if (mean_rgb) prt::film_mean(f->npix, sum.data(), cnt.data(), f->fb.splat ? spl.data() : nullptr, mean_rgb);
     for (int64_t i = 0; i < f->npix; i++) {
-        if (mean_rgb) {
-            // GetColor: (samples + splats) / SampleCount, zero when no samples (stats.go:36-55)
-            float inv = cnt[i] ? 1.0f / (float)cnt[i] : 0.0f;
-            for (int c = 0; c < 3; c++) {
-                float v = c == 0 ? sum[i].x : (c == 1 ? sum[i].y : sum[i].z);
-                if (f->fb.splat) v += (float)((double)spl[3 * i + c] * (1.0 / (double)kSplatScale));
-                mean_rgb[3 * i + c] = cnt[i] ? v * inv : 0.f;
-            }
-        }
         if (lum_sum) lum_sum[i] = sum[i].w;
         if (lum_sq_sum) lum_sq_sum[i] = sq[i].x;
         if (counts) counts[i] = cnt[i];
@@ -1478,9 +1521,9 @@ int prt_film_read_rgba8(const prt_film* f, int32_t x0, int32_t y0, int32_t x1, i
         for (int x = x0; x < x1; x++) {
             int64_t i = (int64_t)y * f->width + x;
             uint8_t* o = rgba + 4 * ((int64_t)(y - y0) * tw + (x - x0));
-            for (int c = 0; c < 3; c++) {  // vec3ToColor: gamma 2, clamp, uint8 truncation
+            for (int c = 0; c < 3; c++) {  // vec3ToColor: gamma 2, clamp, uint8 truncation (NaN -> 0)
                 double v = std::sqrt((double)mean[3 * i + c]);
-                v = std::max(0.0, std::min(1.0, v));
+                v = v > 0.0 ? std::min(1.0, v) : 0.0;
                 o[c] = (uint8_t)(255.0 * v);
             }
             o[3] = 255;

@@ -947,6 +947,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         s->orig_id[k] = i;
     }
     s->info.num_primitives = nprim;
+    s->info.reference_primitives = ntri + nsph + d->num_quads + d->num_boxes + ndisc + naxial;
     s->info.num_triangles = ntri;
     s->info.bvh_nodes = (int64_t)bvh.nodes.size();
     s->info.bvh_depth = bvh.depth;

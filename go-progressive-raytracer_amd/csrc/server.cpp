@@ -41,6 +41,16 @@ struct Req {
     bool release = false;
     int in_flight = 0;       // chunks of this request on the device (at most 2)
     int readers = 0;
+    // PRT_SUBMIT_SNAPSHOTS: copies of the film taken when passes completed, the newest kSnapSlots kept
+    bool snapshots = false;
+    struct Snap {
+        int32_t pass = 0;  // 0: empty
+        int readers = 0;
+        void* buf = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    static constexpr int kSnapSlots = 3;
+    Snap snap[kSnapSlots];
 };
 
 // Film slots of serving requests hold at most this many paths: a pass is enqueued chunk by chunk, so
@@ -75,6 +85,7 @@ struct prt_server {
     std::thread worker;
     bool stop = false;
     size_t rr_next = 0;  // round-robin position of background dispatch
+    hipStream_t read_stream = nullptr;  // snapshot reads (non-blocking, default priority)
 
     double since(const Req* r) const {
         return std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
@@ -91,7 +102,35 @@ struct prt_server {
         events.push_back(e);
         cv_events.notify_all();
     }
+    void free_snapshots(Req* r) {
+        for (auto& sn : r->snap) {
+            if (sn.buf) (void)hipFree(sn.buf);
+            if (sn.done) (void)hipEventDestroy(sn.done);
+            sn = Req::Snap{};
+        }
+    }
+    int alloc_snapshots(Req* r) {
+        const size_t b = prt::film_snapshot_bytes(r->film);
+        for (auto& sn : r->snap) {
+            if (hipMalloc(&sn.buf, std::max<size_t>(b, 16)) != hipSuccess) return PRT_ERR_OUT_OF_MEMORY;
+            if (hipEventCreateWithFlags(&sn.done, hipEventDisableTiming) != hipSuccess) return PRT_ERR_DEVICE;
+        }
+        return PRT_OK;
+    }
+    // Copy of the film as the finished pass left it, enqueued before the next pass's chunks. The slot
+    // is the oldest one nobody reads; if all are being read the pass is not kept.
+    int take_snapshot(Req* r) {
+        Req::Snap* best = nullptr;
+        for (auto& sn : r->snap)
+            if (sn.readers == 0 && (!best || sn.pass < best->pass)) best = &sn;
+        if (!best) return PRT_OK;
+        best->pass = 0;
+        int rc = prt::film_snapshot(r->film, best->buf, best->done);
+        if (rc == PRT_OK) best->pass = r->pass;
+        return rc;
+    }
     void free_film(Req* r) {
+        free_snapshots(r);
         if (r->film) {
             prt_film_destroy(r->film);
             r->film = nullptr;
@@ -143,6 +182,12 @@ struct prt_server {
                     emit(r, PRT_EVENT_ERROR, 0, 0, rc, 0);
                     continue;
                 }
+                if (r->snapshots && (rc = alloc_snapshots(r)) != PRT_OK) {
+                    free_film(r);
+                    r->state = St::Failed;
+                    emit(r, PRT_EVENT_ERROR, 0, 0, rc, 0);
+                    continue;
+                }
                 used += (int64_t)r->bytes;
                 r->state = St::Running;
                 active.push_back(r);
@@ -171,6 +216,12 @@ struct prt_server {
                     }
                     if (prt::film_plan_done(r->film) && r->pass > r->completed_pass) {
                         r->completed_pass = r->pass;
+                        if (r->snapshots && take_snapshot(r) != PRT_OK) {
+                            r->state = St::Failed;
+                            emit(r, PRT_EVENT_ERROR, r->pass, r->target, PRT_ERR_DEVICE, 0);
+                            active.erase(active.begin() + k);
+                            continue;
+                        }
                         emit(r, PRT_EVENT_PASS_COMPLETE, r->pass, r->target, 0, ms);
                         progressed = true;
                         if (r->cancel) {
@@ -270,13 +321,24 @@ int prt_server_create(int32_t device, int64_t budget, int32_t max_active, prt_se
     s->device = device;
     s->budget = budget;
     s->max_active = max_active;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->read_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail(PRT_ERR_DEVICE, "cannot create the server's read stream");
+    }
     s->worker = std::thread([s]() { s->run(); });
     *out = s;
     return PRT_OK;
 }
 
 int prt_server_submit(prt_server* s, const prt_scene* scene, const prt_render_params* p, int64_t* id) {
+    return prt_server_submit_ex(s, scene, p, 0u, id);
+}
+
+int prt_server_submit_ex(prt_server* s, const prt_scene* scene, const prt_render_params* p, uint32_t flags,
+                         int64_t* id) {
     if (!s || !scene || !p || !id) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (flags & ~PRT_SUBMIT_SNAPSHOTS) return fail(PRT_ERR_INVALID_ARGUMENT, "unknown submit flags");
     if (scene->device != s->device) return fail(PRT_ERR_INVALID_ARGUMENT, "scene lives on another device");
     int32_t t = 0;
     int rc = prt_samples_for_pass(p, 1, &t);
@@ -288,6 +350,13 @@ int prt_server_submit(prt_server* s, const prt_scene* scene, const prt_render_pa
     const_cast<prt_scene*>(scene)->users++;  // until the request is released
     r->params = *p;
     r->bytes = film_device_bytes(scene, p, serve_capacity(p));
+    if (flags & PRT_SUBMIT_SNAPSHOTS) {  // sums, counts and (BDPT) splats per pixel, per slot
+        const int64_t w = p->width > 0 ? p->width : scene->image_width;
+        const int64_t h = p->height > 0 ? p->height : scene->image_height;
+        const size_t px = 16 + 4 + (p->integrator == PRT_INTEGRATOR_BDPT ? 24 : 0);
+        r->snapshots = true;
+        r->bytes += (size_t)Req::kSnapSlots * px * (size_t)(w * h);
+    }
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -348,6 +417,33 @@ int prt_server_read(prt_server* s, int64_t id, float* mean_rgb, uint32_t* counts
     return with_film(s, id, [&](prt_film* f) { return prt_film_read(f, mean_rgb, nullptr, nullptr, counts); });
 }
 
+int prt_server_read_pass(prt_server* s, int64_t id, int32_t pass, float* mean_rgb, uint32_t* counts) {
+    if (!s) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL server");
+    Req* r;
+    Req::Snap* sn = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) return fail(PRT_ERR_INVALID_ARGUMENT, "unknown request");
+        r = it->second;
+        if (!r->snapshots) return fail(PRT_ERR_INVALID_ARGUMENT, "request was not submitted with PRT_SUBMIT_SNAPSHOTS");
+        if (!r->film) return fail(PRT_ERR_INVALID_ARGUMENT, "request has no film (queued, failed or released)");
+        for (auto& x : r->snap)
+            if (pass > 0 && x.pass == pass) sn = &x;
+        if (!sn) return fail(PRT_ERR_INVALID_ARGUMENT, "no snapshot of that pass (not complete or no longer kept)");
+        sn->readers++;
+        r->readers++;
+    }
+    int rc = prt::film_read_snapshot(r->film, sn->buf, sn->done, s->read_stream, mean_rgb, counts);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        sn->readers--;
+        r->readers--;
+    }
+    s->cv_work.notify_all();
+    return rc;
+}
+
 int prt_server_read_rgba8(prt_server* s, int64_t id, int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint8_t* rgba) {
     if (!s) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL server");
     return with_film(s, id, [&](prt_film* f) { return prt_film_read_rgba8(f, x0, y0, x1, y1, rgba); });
@@ -390,6 +486,7 @@ int prt_server_destroy(prt_server* s) {
         const_cast<prt_scene*>(r->scene)->users--;
         delete r;
     }
+    if (s->read_stream) (void)hipStreamDestroy(s->read_stream);
     delete s;
     return PRT_OK;
 }

@@ -113,7 +113,7 @@ class SceneInfo(C.Structure):
     _fields_ = [("num_primitives", C.c_int64), ("num_triangles", C.c_int64), ("bvh_nodes", C.c_int64),
                 ("bvh_depth", C.c_int32), ("max_stack", C.c_int32), ("world_center", D3),
                 ("world_radius", C.c_double), ("device_bytes", C.c_int64), ("trace_variant", C.c_int32),
-                ("pad", C.c_int32)]
+                ("pad", C.c_int32), ("reference_primitives", C.c_int64)]
 
 
 class RenderParams(C.Structure):
@@ -142,7 +142,10 @@ EXPORTS = [
     "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky", "prt_ply_load", "prt_ply_free",
     "prt_server_create", "prt_server_submit", "prt_server_poll", "prt_server_cancel", "prt_server_read",
     "prt_server_read_rgba8", "prt_server_release", "prt_server_stats_get", "prt_server_destroy",
+    "prt_server_submit_ex", "prt_server_read_pass", "prt_png_encode_rgba8", "prt_sse_format_pass",
+    "prt_sse_format_complete", "prt_sse_format_error", "prt_sse_server_event", "prt_free",
 ]
+SUBMIT_SNAPSHOTS = 1
 
 EVENT_ADMITTED, EVENT_PASS_COMPLETE, EVENT_DONE, EVENT_ERROR, EVENT_CANCELLED = range(5)
 
@@ -157,6 +160,12 @@ class ServerEvent(C.Structure):
     _fields_ = [("request_id", C.c_int64), ("type", C.c_int32), ("pass_number", C.c_int32),
                 ("target_samples", C.c_int32), ("error_code", C.c_int32), ("t_ms", C.c_double),
                 ("device_ms", C.c_double)]
+
+
+class SsePassInfo(C.Structure):
+    _fields_ = [("pass_number", C.c_int32), ("total_passes", C.c_int32), ("target_samples", C.c_int32),
+                ("max_samples_per_pixel", C.c_int32), ("tile_size", C.c_int32), ("pad", C.c_int32),
+                ("primitive_count", C.c_int64), ("elapsed_ms", C.c_int64)]
 
 
 class ServerStats(C.Structure):
@@ -209,6 +218,16 @@ def lib():
         "prt_server_release": (C.c_int, [vp, C.c_int64]),
         "prt_server_stats_get": (C.c_int, [vp, C.POINTER(ServerStats)]),
         "prt_server_destroy": (C.c_int, [vp]),
+        "prt_server_submit_ex": (C.c_int, [vp, vp, C.POINTER(RenderParams), C.c_uint32, C.POINTER(C.c_int64)]),
+        "prt_server_read_pass": (C.c_int, [vp, C.c_int64, C.c_int32, vp, vp]),
+        "prt_png_encode_rgba8": (C.c_int, [vp, C.c_int32, C.c_int32, C.POINTER(vp), C.POINTER(C.c_int64)]),
+        "prt_sse_format_pass": (C.c_int, [C.c_int32, C.c_int32, vp, vp, C.POINTER(SsePassInfo), C.POINTER(vp),
+                                          C.POINTER(C.c_int64)]),
+        "prt_sse_format_complete": (C.c_int, [C.POINTER(vp), C.POINTER(C.c_int64)]),
+        "prt_sse_format_error": (C.c_int, [C.c_char_p, C.POINTER(vp), C.POINTER(C.c_int64)]),
+        "prt_sse_server_event": (C.c_int, [vp, vp, C.POINTER(RenderParams), C.c_int32, C.POINTER(ServerEvent),
+                                           C.POINTER(vp), C.POINTER(C.c_int64)]),
+        "prt_free": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -647,11 +666,25 @@ class Server:
         self.handle = h
         self._scenes = {}  # request id -> Scene, kept alive until the request is released
 
-    def submit(self, scene, params):
+    def submit(self, scene, params, snapshots=False):
+        """Queues a render; snapshots=True keeps the film of each finished pass (read_pass, sse)."""
         rid = C.c_int64()
-        check(lib().prt_server_submit(self.handle, scene.handle, C.byref(params), C.byref(rid)))
+        check(lib().prt_server_submit_ex(self.handle, scene.handle, C.byref(params),
+                                         SUBMIT_SNAPSHOTS if snapshots else 0, C.byref(rid)))
         self._scenes[rid.value] = scene
         return rid.value
+
+    def read_pass(self, rid, pass_number, width, height):
+        mean = np.empty((height, width, 3), np.float32)
+        cnt = np.empty((height, width), np.uint32)
+        check(lib().prt_server_read_pass(self.handle, rid, pass_number, _ptr(mean), _ptr(cnt)))
+        return mean, cnt
+
+    def sse(self, event, params, tile_size=64):
+        """The SSE text (bytes) of one event of a snapshots request (include/prt_sse.h)."""
+        scene = self._scenes[event.request_id]
+        return _take_text(lambda t, n: lib().prt_sse_server_event(self.handle, scene.handle, C.byref(params),
+                                                                    tile_size, C.byref(event), t, n))
 
     def poll(self, max_events=256, timeout_ms=0):
         ev = (ServerEvent * max_events)()
@@ -741,3 +774,40 @@ def gradient_texture(width, height, color1, color2):
     t = (np.arange(height) / (height - 1))[:, None, None]
     row = np.array(color1, float) * (1.0 - t) + np.array(color2, float) * t
     return np.broadcast_to(row, (height, width, 3)).copy()
+
+
+# ---- SSE stream text and PNG (include/prt_sse.h) ----------------------------------------------
+def _take_text(call):
+    t, n = C.c_void_p(), C.c_int64()
+    check(call(C.byref(t), C.byref(n)))
+    if not t.value:
+        return b""
+    try:
+        return C.string_at(t.value, n.value)
+    finally:
+        lib().prt_free(t)
+
+
+def png_encode_rgba8(rgba):
+    """PNG bytes of an (h, w, 4) uint8 image (opaque: written as 8-bit RGB)."""
+    a = np.ascontiguousarray(rgba, dtype=np.uint8)
+    return _take_text(lambda t, n: lib().prt_png_encode_rgba8(_ptr(a), a.shape[1], a.shape[0], t, n))
+
+
+def sse_format_pass(mean_rgb, counts, pass_number, total_passes, target_samples, max_samples_per_pixel,
+                    primitive_count, elapsed_ms, tile_size=64):
+    """Tile events (twice per tile) and the passComplete event of one pass, as SSE text (bytes)."""
+    m = np.ascontiguousarray(mean_rgb, dtype=np.float32)
+    c = np.ascontiguousarray(counts, dtype=np.uint32)
+    h, w = c.shape
+    info = SsePassInfo(pass_number, total_passes, target_samples, max_samples_per_pixel, tile_size, 0,
+                       primitive_count, elapsed_ms)
+    return _take_text(lambda t, n: lib().prt_sse_format_pass(w, h, _ptr(m), _ptr(c), C.byref(info), t, n))
+
+
+def sse_format_complete():
+    return _take_text(lambda t, n: lib().prt_sse_format_complete(t, n))
+
+
+def sse_format_error(message):
+    return _take_text(lambda t, n: lib().prt_sse_format_error(message.encode(), t, n))

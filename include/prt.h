@@ -234,6 +234,8 @@ typedef struct prt_scene_info {
     int64_t device_bytes;
     int32_t trace_variant;  /* traversal kernel variant picked by the autotuner (-1: not tuned yet) */
     int32_t pad;
+    int64_t reference_primitives; /* Scene.GetPrimitiveCount (scene.go:81-99): mesh triangles + 1 per
+                                     other shape (a box counts once, a disc spot light's disc too) */
 } prt_scene_info;
 
 typedef struct prt_scene prt_scene;

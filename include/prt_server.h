@@ -57,6 +57,13 @@ int prt_server_create(int32_t device, int64_t hbm_budget_bytes, int32_t max_acti
  * has not been released (or a film) still uses it. */
 int prt_server_submit(prt_server* server, const prt_scene* scene, const prt_render_params* params,
                       int64_t* request_id);
+/* Flags of prt_server_submit_ex. */
+#define PRT_SUBMIT_SNAPSHOTS 1u /* keep a copy of the film at the end of each pass (the newest three) */
+/* prt_server_submit with flags. A PRT_SUBMIT_SNAPSHOTS request also holds three film copies in the HBM
+ * budget, so that its passes can be read exactly as they ended while later passes render
+ * (prt_server_read_pass; RenderPass hands each pass's image to its caller, progressive.go:126-223). */
+int prt_server_submit_ex(prt_server* server, const prt_scene* scene, const prt_render_params* params,
+                         uint32_t flags, int64_t* request_id);
 /* Copies up to max_events pending events; waits up to timeout_ms for the first one (0: no wait). */
 int prt_server_poll(prt_server* server, prt_server_event* events, int32_t max_events, int32_t timeout_ms,
                     int32_t* count);
@@ -65,6 +72,10 @@ int prt_server_cancel(prt_server* server, int64_t request_id);
 int prt_server_read(prt_server* server, int64_t request_id, float* mean_rgb, uint32_t* counts);
 int prt_server_read_rgba8(prt_server* server, int64_t request_id, int32_t x0, int32_t y0, int32_t x1,
                           int32_t y1, uint8_t* rgba);
+/* The film as pass `pass_number` left it (mean RGB, counts), for PRT_SUBMIT_SNAPSHOTS requests; fails
+ * for a pass that has not completed or is older than the three newest. */
+int prt_server_read_pass(prt_server* server, int64_t request_id, int32_t pass_number, float* mean_rgb,
+                         uint32_t* counts);
 /* Frees the film slot of a finished or cancelled request. */
 int prt_server_release(prt_server* server, int64_t request_id);
 int prt_server_stats_get(prt_server* server, prt_server_stats* stats);
