@@ -6,11 +6,13 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <atomic>
 #include <charconv>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/prt_sse.h"
@@ -183,20 +185,31 @@ int prt_sse_format_pass(int32_t w, int32_t h, const float* mean_rgb, const uint3
     to_rgba8(mean_rgb, npix, img);
     const int ts = info->tile_size;
     const int tx = (w + ts - 1) / ts, ty = (h + ts - 1) / ts, ntiles = tx * ty;
-    // one PNG per tile (NewTileGrid order, progressive.go:455-479), sent twice as RenderPass does
+    // one PNG per tile (NewTileGrid order, progressive.go:455-479), sent twice as RenderPass does;
+    // tiles are encoded on up to 16 threads
     std::vector<std::string> b64(ntiles);
-    std::vector<uint8_t> tile;
-    std::string png;
-    for (int j = 0; j < ty; j++)
-        for (int i = 0; i < tx; i++) {
+    std::atomic<int> next{0};
+    std::atomic<bool> ok{true};
+    auto work = [&]() {
+        std::vector<uint8_t> tile;
+        std::string png;
+        for (int k; (k = next.fetch_add(1)) < ntiles;) {
+            const int i = k % tx, j = k / tx;
             const int x0 = i * ts, y0 = j * ts, x1 = std::min(x0 + ts, w), y1 = std::min(y0 + ts, h);
             const int tw = x1 - x0, th = y1 - y0;
             tile.resize(4 * (size_t)tw * th);
             for (int y = 0; y < th; y++)
                 std::memcpy(&tile[4 * (size_t)y * tw], &img[4 * ((size_t)(y0 + y) * w + x0)], 4 * (size_t)tw);
-            if (!png_rgb(tile.data(), tw, th, png)) return fail(PRT_ERR_DEVICE, "zlib compression failed");
-            base64(png, b64[j * tx + i]);
+            if (!png_rgb(tile.data(), tw, th, png)) ok = false;
+            else base64(png, b64[k]);
         }
+    };
+    const int nthreads = std::max(1, std::min({16, ntiles / 4, (int)std::thread::hardware_concurrency()}));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (!ok) return fail(PRT_ERR_DEVICE, "zlib compression failed");
     std::string out;
     for (int round = 0; round < 2; round++)
         for (int k = 0; k < ntiles; k++) {
