@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libprt.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["api.cpp", "bvh_build.cpp", "workload.cpp", "ply.cpp", "server.cpp", "sse.cpp", "scene.hip", "pt.hip"]
+SOURCES = ["api.cpp", "bvh_build.cpp", "workload.cpp", "ply.cpp", "pbrt.cpp", "server.cpp", "sse.cpp", "scene.hip", "pt.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PRT_OFFLOAD_ARCH", "gfx950")

@@ -725,10 +725,18 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         if (!(h > 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "cone height must be positive"));
         axials.push_back({b, t, (t - b) * (1.0 / h), c.base_radius, c.top_radius, h, c.capped != 0, true, c.material});
     }
+    // lights that bring their own material, appended after the caller's in light order: the disc
+    // spot light's emitter (its disc joins the shapes) and the emissive material of a sphere light
+    // whose sphere is not a shape (NewSphereLight without AddSphereLight)
     std::vector<int> spot_index(std::max(d->num_lights, 1), -1);  // light -> internal material nm + k
     int nspot = 0;
     for (int i = 0; i < d->num_lights; i++) {
         const prt_light& L = d->lights[i];
+        if (L.type == PRT_LIGHT_SPHERE && L.shape == -1) {
+            if (!(L.radius != 0)) return bail(fail(PRT_ERR_INVALID_ARGUMENT, "detached sphere light needs a radius"));
+            spot_index[i] = nspot++;
+            continue;
+        }
         if (L.type != PRT_LIGHT_DISC_SPOT && L.type != PRT_LIGHT_POINT_SPOT) continue;
         if (!(len(d3(L.to) - d3(L.from)) > 0))
             return bail(fail(PRT_ERR_INVALID_ARGUMENT, "spot light needs from != to"));
@@ -1014,6 +1022,11 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         const prt_light& L = d->lights[i];
         DevMaterial& o = mats[nm + spot_index[i]];
         std::memset(&o, 0, sizeof(o));
+        if (L.type == PRT_LIGHT_SPHERE) {  // NewEmissive(emission)
+            o.type = PRT_MAT_EMISSIVE;
+            for (int a = 0; a < 3; a++) o.emission[a] = (float)L.emission[a];
+            continue;
+        }
         o.type = kMatSpotEmitter;
         D3 dir = normalize(d3(L.to) - d3(L.from));
         for (int a = 0; a < 3; a++) {
@@ -1056,6 +1069,10 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
             o.D = (float)dot(n, c);
             o.area = (float)len(cross(u, v));  // NewQuadLight quad_light.go:19-29
             o.mat = q.material;
+        } else if (L.type == PRT_LIGHT_SPHERE && L.shape == -1) {  // sphere not among the shapes
+            for (int a = 0; a < 3; a++) o.center[a] = (float)L.from[a];
+            o.radius = (float)L.radius;
+            o.mat = nm + spot_index[i];
         } else if (L.type == PRT_LIGHT_SPHERE) {
             if (L.shape < 0 || L.shape >= d->num_spheres)
                 return bail(fail(PRT_ERR_INVALID_ARGUMENT, "sphere light shape out of range"));
@@ -1138,7 +1155,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         } else {
             return bail(fail(PRT_ERR_UNSUPPORTED, "unknown light type"));
         }
-        if ((L.type == PRT_LIGHT_QUAD || L.type == PRT_LIGHT_SPHERE) && bad_mat(o.mat))
+        if ((L.type == PRT_LIGHT_QUAD || (L.type == PRT_LIGHT_SPHERE && L.shape != -1)) && bad_mat(o.mat))
             return bail(fail(PRT_ERR_INVALID_ARGUMENT, "light material out of range"));
     }
     // light selection probabilities (NewUniformLightSampler / NewWeightedLightSampler)

@@ -109,6 +109,13 @@ class SceneDescC(C.Structure):
                 ("textures", C.POINTER(Texture)), ("num_textures", C.c_int32)]
 
 
+class PbrtSceneC(C.Structure):
+    _fields_ = [("desc", SceneDescC), ("width", C.c_int32), ("height", C.c_int32),
+                ("samples_per_pixel", C.c_int32), ("max_depth", C.c_int32), ("rr_min_bounces", C.c_int32),
+                ("pad", C.c_int32), ("adaptive_min_samples", C.c_double), ("adaptive_threshold", C.c_double),
+                ("reference_shapes", C.c_int64), ("impl", C.c_void_p)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("num_primitives", C.c_int64), ("num_triangles", C.c_int64), ("bvh_nodes", C.c_int64),
                 ("bvh_depth", C.c_int32), ("max_stack", C.c_int32), ("world_center", D3),
@@ -144,6 +151,7 @@ EXPORTS = [
     "prt_server_read_rgba8", "prt_server_release", "prt_server_stats_get", "prt_server_destroy",
     "prt_server_submit_ex", "prt_server_read_pass", "prt_png_encode_rgba8", "prt_sse_format_pass",
     "prt_sse_format_complete", "prt_sse_format_error", "prt_sse_server_event", "prt_free",
+    "prt_pbrt_load", "prt_pbrt_parse", "prt_pbrt_free",
 ]
 SUBMIT_SNAPSHOTS = 1
 
@@ -228,6 +236,9 @@ def lib():
         "prt_sse_server_event": (C.c_int, [vp, vp, C.POINTER(RenderParams), C.c_int32, C.POINTER(ServerEvent),
                                            C.POINTER(vp), C.POINTER(C.c_int64)]),
         "prt_free": (None, [vp]),
+        "prt_pbrt_load": (C.c_int, [C.c_char_p, C.POINTER(C.POINTER(PbrtSceneC))]),
+        "prt_pbrt_parse": (C.c_int, [C.c_char_p, C.c_int64, C.POINTER(C.POINTER(PbrtSceneC))]),
+        "prt_pbrt_free": (None, [C.POINTER(PbrtSceneC)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -244,6 +255,56 @@ def check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class PbrtScene:
+    """A scene read by the PBRT loader (prt_pbrt_load / prt_pbrt_parse; scene.NewPBRTScene). Usable
+    wherever a SceneDesc is (Scene, the oracle); `sampling` holds the PBRT sampling defaults."""
+
+    def __init__(self, handle):
+        self._h = handle
+        o = handle.contents
+        self.camera = o.desc.camera
+        self.sampling = dict(width=o.width, height=o.height, spp=o.samples_per_pixel, max_depth=o.max_depth,
+                             rr_min_bounces=o.rr_min_bounces, adaptive_min_samples=o.adaptive_min_samples,
+                             adaptive_threshold=o.adaptive_threshold)
+        self.reference_shapes = o.reference_shapes
+
+    @property
+    def desc(self):
+        return self._h.contents.desc
+
+    @property
+    def image_size(self):
+        w = self.camera.width
+        return w, int(w / self.camera.aspect_ratio)
+
+    def to_c(self):
+        return self._h.contents.desc, self
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().prt_pbrt_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def load_pbrt(path):
+    """LoadPBRT + NewPBRTScene: the path must be a .pbrt file under a scenes/ directory (or the
+    temporary directory), as the reference's validateFilePath requires."""
+    h = C.POINTER(PbrtSceneC)()
+    check(lib().prt_pbrt_load(os.fsencode(path), C.byref(h)))
+    return PbrtScene(h)
+
+
+def parse_pbrt(text):
+    """ParsePBRT + NewPBRTScene on a string."""
+    b = text.encode() if isinstance(text, str) else bytes(text)
+    h = C.POINTER(PbrtSceneC)()
+    check(lib().prt_pbrt_parse(b, len(b), C.byref(h)))
+    return PbrtScene(h)
 
 
 def load_ply(path):

@@ -178,7 +178,10 @@ typedef struct prt_mesh { /* pkg/geometry/triangle_mesh.go:33-146; primitive id 
 
 typedef struct prt_light {
     int32_t type;
-    int32_t shape;       /* quad/sphere lights: index into quads[] / spheres[] (the emitting shape) */
+    int32_t shape;       /* quad/sphere lights: index into quads[] / spheres[] (the emitting shape);
+                            a sphere light with shape -1 has no shape (NewSphereLight without
+                            AddSphereLight, as the PBRT "point" light, pbrt_scene.go:348-361): centre
+                            `from`, `radius`, `emission` */
     double emission[3];  /* uniform infinite light */
     double top[3];       /* gradient light top colour */
     double bottom[3];    /* gradient light bottom colour */
@@ -189,7 +192,7 @@ typedef struct prt_light {
     double to[3];        /* spot lights: aim point */
     double cone_angle;   /* spot lights: total cone angle, degrees */
     double cone_delta;   /* spot lights: falloff transition angle, degrees */
-    double radius;       /* disc spot light: disc radius */
+    double radius;       /* disc spot light: disc radius; shape-less sphere light: sphere radius */
 } prt_light;
 
 typedef struct prt_camera { /* pkg/geometry/camera.go:18-32 */

@@ -12,6 +12,8 @@
 
 #include <stdint.h>
 
+#include "prt.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -29,6 +31,35 @@ typedef struct prt_ply_data { /* PLYData, pkg/loaders/ply.go:54-72 */
 /* Loads a PLY file (LoadPLY, pkg/loaders/ply.go:64-110). */
 int prt_ply_load(const char* path, prt_ply_data** out);
 void prt_ply_free(prt_ply_data* data);
+
+/* A PBRT-subset scene converted as scene.NewPBRTScene does (pkg/scene/pbrt_scene.go:15-74): `desc`
+ * is ready for prt_scene_create (its arrays are owned by this object), the sampling settings are the
+ * reference's PBRT defaults with the Film resolution. Parsing follows loaders.ParsePBRT
+ * (pkg/loaders/pbrt.go): statements and continuation lines, AttributeBegin/End blocks with their
+ * material and area-light state, LookAt/Camera/Film; materials diffuse, conductor, dielectric;
+ * shapes sphere, bilinearPatch, trianglemesh, box; lights point (a 0.1-radius sphere light that is
+ * not a shape), distant and infinite (uniform), infinite-gradient, and area lights on bilinearPatch
+ * and sphere shapes. Translate/Rotate/Scale/Transform are parsed and ignored, as in the reference.
+ * Errors (unsupported types, invalid values, a shape without a material) fail with a message. */
+typedef struct prt_pbrt_scene {
+    prt_scene_desc desc;
+    int32_t width, height;              /* SamplingConfig.Width / Height */
+    int32_t samples_per_pixel;          /* 150 (the Sampler statement is not read) */
+    int32_t max_depth;                  /* 5 */
+    int32_t rr_min_bounces;             /* 16 */
+    int32_t pad;
+    double adaptive_min_samples;        /* 0.20 */
+    double adaptive_threshold;          /* 0.005 */
+    int64_t reference_shapes;           /* len(Scene.Shapes) */
+    void* impl;
+} prt_pbrt_scene;
+
+/* LoadPBRT (pbrt.go:97-110): the path must name a .pbrt file under a scenes/ directory or the
+ * temporary directory (validateFilePath :440-480). */
+int prt_pbrt_load(const char* path, prt_pbrt_scene** out);
+/* ParsePBRT (pbrt.go:72-95) on text in memory. */
+int prt_pbrt_parse(const char* text, int64_t length, prt_pbrt_scene** out);
+void prt_pbrt_free(prt_pbrt_scene* scene);
 
 #ifdef __cplusplus
 }

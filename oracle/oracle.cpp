@@ -2216,9 +2216,19 @@ orc_scene* orc_scene_create(const prt_scene_desc* d) {
             new Cone(V(c.base_center), c.base_radius, V(c.top_center), c.top_radius, c.capped != 0, c.material, id++));
     }
     // disc spot lights: an internal emitting material and their disc join the shapes (scene.go:117-122)
+    // ... and a sphere light without a shape (shape -1: NewSphereLight without AddSphereLight) gets
+    // its emissive material here too, in light order
     std::vector<int> spot_mat(d->num_lights, -1);
     for (int i = 0; i < d->num_lights; i++) {
         const prt_light& pl = d->lights[i];
+        if (pl.type == PRT_LIGHT_SPHERE && pl.shape == -1) {
+            Material m;
+            m.type = PRT_MAT_EMISSIVE;
+            m.emission = V(pl.emission);
+            spot_mat[i] = (int)S.mats.size();
+            S.mats.push_back(m);
+            continue;
+        }
         if (pl.type != PRT_LIGHT_DISC_SPOT) continue;
         Vec3 dir = (V(pl.to) - V(pl.from)).normalize();
         Material m;
@@ -2247,6 +2257,9 @@ orc_scene* orc_scene_create(const prt_scene_desc* d) {
             L.quad.reset(new Quad(V(q.corner), V(q.u), V(q.v), q.material, -1));
             L.area = V(q.u).cross(V(q.v)).length();  // NewQuadLight quad_light.go:19-29
             L.mat = q.material;
+        } else if (pl.type == PRT_LIGHT_SPHERE && pl.shape == -1) {
+            L.sphere.reset(new Sphere(V(pl.from), pl.radius, spot_mat[i], -1));
+            L.mat = spot_mat[i];
         } else if (pl.type == PRT_LIGHT_SPHERE) {
             const prt_sphere& sp = d->spheres[pl.shape];
             L.sphere.reset(new Sphere(V(sp.center), sp.radius, sp.material, -1));
