@@ -364,37 +364,31 @@ __device__ double prim_t_f64(const DevScene& sc, uint32_t prim, V3 o_hi, const f
     return (nx * (px - ox) + ny * (py - oy) + nz * (pz - oz)) / den;
 }
 
-// Closest hit of a ray leaving a point light. The light's float32 position can lie off its float64
-// position by half an ulp, which moves float32 hit distances of surfaces within ~tmin of the light
-// across tmin = 0.001 (the reference's Cornell point light hangs exactly 0.001 below the ceiling).
-// Hits nearer than 0.01 are therefore re-decided in float64 from the light's low part; a rejected
-// hit is skipped and the search resumes beyond it.
-template <bool MIXED>
-__device__ bool trace_from_point(const DevScene& sc, V3 o, const float* o_lo, V3 d, THit& h,
-                                 TStack<kLds, kBlock>& st) {
-    float tmin = 0.0f;
-    for (int k = 0; k < 4; k++) {
-        TRay r{o.x, o.y, o.z, tmin, d.x, d.y, d.z, __int_as_float(0x7f800000)};
-        if (!trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr)) return false;
-        if (h.t >= 0.01f || prim_t_f64(sc, (uint32_t)h.prim, o, o_lo, d, h.t) >= 0.001) return true;
-        tmin = __uint_as_float(__float_as_uint(h.t) + 1u);
-    }
-    TRay r{o.x, o.y, o.z, fmaxf(tmin, 0.01f), d.x, d.y, d.z, __int_as_float(0x7f800000)};
-    return trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
-}
-
 // extendPath, bdpt.go:206-291. Appends vertices from index n; returns the new length. `point_lo`
-// is the low part of the origin when the first segment leaves a point light, else null.
+// is the low part of the origin when the first segment leaves a point light, else null: the light's
+// float32 position can lie off its float64 position by half an ulp, which moves float32 hit
+// distances of surfaces within ~tmin of the light across tmin = 0.001 (the reference's Cornell
+// point light hangs exactly 0.001 below the ceiling). Hits of that segment nearer than 0.01 are
+// re-decided in float64 from the light's low part; a rejected hit is skipped and the search resumes
+// beyond it (after four rejections, beyond 0.01). One traversal call site keeps the kernel small.
 template <bool MIXED>
 __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i, int n, V3 o, V3 d, V3 beta,
                            float pdf_fwd, int max_bounces, bool camera_path, uint32_t key, uint32_t slot0,
-                           TStack<kLds, kBlock>& st, const float* point_lo = nullptr) {
+                           TStack<kLds, kBlock>& st, const float* point_lo) {
     for (int b = 0; b < max_bounces; b++) {
         BV prev = VS.get(n - 1, i);
-        TRay r{o.x, o.y, o.z, 0.001f, d.x, d.y, d.z, __int_as_float(0x7f800000)};
         THit h;
-        const bool hit = (b == 0 && point_lo) ? trace_from_point<MIXED>(sc, o, point_lo, d, h, st)
-                                              : trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
+        bool hit;
+        const bool redecide = b == 0 && point_lo;
+        float tmin = redecide ? 0.0f : 0.001f;
+        for (int k = 0;; k++) {
+            TRay r{o.x, o.y, o.z, tmin, d.x, d.y, d.z, __int_as_float(0x7f800000)};
+            hit = trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
+            if (!redecide || !hit || h.t >= 0.01f || k == 4) break;
+            if (prim_t_f64(sc, h.prim, o, point_lo, d, h.t) >= 0.001) break;
+            const float next = __uint_as_float(__float_as_uint(h.t) + 1u);
+            tmin = k == 3 ? fmaxf(next, 0.01f) : next;
+        }
         if (!hit) {
             if (camera_path) {  // createBackgroundVertex, bdpt.go:585-601
                 V3 e = v3(0, 0, 0);
@@ -456,7 +450,7 @@ __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i
 
 // Subpath generation: one thread per sample writes its camera and light subpaths and their lengths.
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_bdpt_paths(DevScene sc, PassParams pp, VertexStore CP, VertexStore LP,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bdpt_paths(DevScene sc, PassParams pp, VertexStore CP, VertexStore LP,
                                                        uint32_t* lens, uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
@@ -492,10 +486,10 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_paths(DevScene sc, PassParams p
         cv.light = -1;
         cv.flags = VF_CAMERA;
         CP.put(0, i, cv);
-        int nc = bdpt_extend<MIXED>(sc, CP, i, 1, o, d, v3(1, 1, 1), cam_dir_pdf(cam, o, d), pp.max_depth, true, key, 1,
-                                    st);
+        const uint32_t nc = (uint32_t)bdpt_extend<MIXED>(sc, CP, i, 1, o, d, v3(1, 1, 1), cam_dir_pdf(cam, o, d),
+                                                         pp.max_depth, true, key, 1, st, nullptr);
         // light subpath (generateLightPath, bdpt.go:139-203)
-        int nl = 0;
+        uint32_t nl = 0;
         if (sc.num_lights > 0) {
             const uint32_t lp0 = kLightSlot * 16u;
             float sel;
@@ -518,9 +512,10 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_paths(DevScene sc, PassParams p
             lv.flags = VF_LIGHT | (inf ? VF_INF : 0u);
             LP.put(0, i, lv);
             V3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
-            nl = bdpt_extend<MIXED>(sc, LP, i, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false, key,
-                                    kLightSlot + 1, st, L.type == PRT_LIGHT_POINT_SPOT ? L.from_lo : nullptr);
-            if (inf) {
+            nl = (uint32_t)bdpt_extend<MIXED>(sc, LP, i, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false,
+                                              key, kLightSlot + 1, st,
+                                              L.type == PRT_LIGHT_POINT_SPOT ? L.from_lo : nullptr);
+            if (inf) {  // infinite-light density fix-ups (bdpt.go:182-199)
                 if (nl > 1) {
                     BV fb = LP.get(1, i);
                     float pf = es.area_pdf * sel;
@@ -530,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_paths(DevScene sc, PassParams p
                 LP.set_pdf_fwd(0, i, infinite_density(sc, es.p, es.n, es.dir));
             }
         }
-        lens[i] = (uint32_t)nc | ((uint32_t)nl << 16);
+        lens[i] = nc | (nl << 16);
     }
 }
 

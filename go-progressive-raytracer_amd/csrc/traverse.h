@@ -502,9 +502,11 @@ __device__ __forceinline__ void load_node_cache(const DevScene& sc, float4* cach
 }
 
 // Closest hit (ANY = false) or any hit (ANY = true) of one ray. Returns true on a hit.
+// Always inlined: its callers have one call site each, and a call would pass the ray, hit and stack
+// through scratch memory (AMDGPU's inliner gives up on large kernels by basic-block count).
 template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK, int NCACHE = 0>
-__device__ bool trace(const DevScene& sc, const TRay& r_in, THit& hit, TStack<LDS_DEPTH, BLOCK>& st,
-                      TCounts* cnt) {
+__device__ __forceinline__ bool trace(const DevScene& sc, const TRay& r_in, THit& hit, TStack<LDS_DEPTH, BLOCK>& st,
+                                      TCounts* cnt) {
     TState s;
     tstate_init(s, r_in);
     while (!tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK, NCACHE>(sc, s, st, cnt)) {
