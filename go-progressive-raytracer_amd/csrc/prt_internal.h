@@ -70,6 +70,12 @@ int film_advance(prt_film* f);               // adaptive films, when idle: plan 
 int film_enqueue_chunk(prt_film* f);         // enqueue the next planned chunk, no wait
 double film_chunk_ms(prt_film* f);           // accumulate the finished chunk's device time
 bool film_idle(prt_film* f);                 // all enqueued work finished
+int film_track_chunks(prt_film* f);          // bracket every later chunk with ring events
+int film_chunks_pending(prt_film* f);        // tracked chunks not finished yet (adds finished ones' time)
+double film_plan_ms(const prt_film* f);      // device time of the plan's finished tracked chunks
+int64_t film_capacity_paths(const prt_film* f);
+void film_set_chunk_paths(prt_film* f, int64_t n);  // paths per chunk of later plans (at most capacity)
+int64_t bdpt_grid_paths(const prt_scene* scene);    // BDPT paths of one full grid (resident blocks twice)
 int film_wait(prt_film* f, float* ms);       // wait for enqueued work; device time of the last enqueue
 int32_t film_spp_done(const prt_film* f);
 size_t film_snapshot_bytes(const prt_film* f);

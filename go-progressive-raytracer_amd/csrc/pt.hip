@@ -844,6 +844,13 @@ struct prt_film {
     uint32_t* lens = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // serving films (film_track_chunks): each chunk bracketed by a pair of ring events, so that the
+    // scheduler sees chunks finish one by one; at most kRing chunks may be in flight
+    static constexpr int kRing = 8;
+    bool track = false;
+    hipEvent_t rev0[kRing] = {}, rev1[kRing] = {};
+    int64_t enq = 0, seen = 0;
+    int64_t chunk_paths = 0;  // paths per planned chunk (at most capacity)
     // chunk plan of the samples being rendered (see film_plan)
     struct Chunk {
         int32_t px0, npix, spp, sample_base;
@@ -981,9 +988,9 @@ int film_plan(prt_film* f, int32_t spp) {
     f->plan_ms = 0;
     int64_t done = 0;
     while (done < spp) {
-        int64_t per = f->capacity / f->npix;
+        int64_t per = f->chunk_paths / f->npix;
         int32_t chunk_spp = (int32_t)std::max<int64_t>(1, std::min<int64_t>(per, spp - done));
-        int64_t pix_step = per >= 1 ? f->npix : f->capacity;
+        int64_t pix_step = per >= 1 ? f->npix : f->chunk_paths;
         for (int64_t p0 = 0; p0 < f->npix; p0 += pix_step) {
             prt_film::Chunk c;
             c.px0 = (int32_t)p0;
@@ -1072,8 +1079,8 @@ int film_advance(prt_film* f) {
         const int64_t spec = std::max<int64_t>(1, (f->ad_target - f->ad_min + 7) / 8);
         k = (mincnt < f->ad_min ? f->ad_min - mincnt : 0) + spec;
     }
-    k = std::max<int64_t>(1, std::min<int64_t>({k, (int64_t)f->ad_target - mincnt, f->capacity}));
-    const int64_t per_chunk = std::max<int64_t>(1, f->capacity / k);
+    k = std::max<int64_t>(1, std::min<int64_t>({k, (int64_t)f->ad_target - mincnt, f->chunk_paths}));
+    const int64_t per_chunk = std::max<int64_t>(1, f->chunk_paths / k);
     for (int64_t a0 = 0; a0 < (int64_t)f->ad_n; a0 += per_chunk) {
         prt_film::Chunk c;
         c.px0 = (int32_t)a0;
@@ -1091,6 +1098,8 @@ int film_advance(prt_film* f) {
 // Enqueues the next planned chunk on the film's stream (no wait); ev0/ev1 bracket it.
 int film_enqueue_chunk(prt_film* f) {
     if (film_round_exhausted(f)) return PRT_OK;
+    if (f->track && f->enq - f->seen >= prt_film::kRing)
+        return fail(PRT_ERR_INVALID_ARGUMENT, "too many tracked chunks in flight");
     const prt_scene* s = f->scene;
     PRT_HIP(hipSetDevice(s->device));
     const DevScene& sc = f->sc;
@@ -1117,6 +1126,8 @@ int film_enqueue_chunk(prt_film* f) {
         ap.state = f->ad_state + 3 * (f->ad_cur ^ 1);
     }
     int64_t n = (int64_t)pp.npix * pp.spp;
+    const int ring = (int)(f->enq % prt_film::kRing);
+    if (f->track) PRT_HIP(hipEventRecord(f->rev0[ring], st));
     PRT_HIP(hipEventRecord(f->ev0, st));
     if (f->bdpt) {
         int grid = (int)std::min<int64_t>(f->bdpt_blocks, (n + kBlock - 1) / kBlock);
@@ -1139,6 +1150,8 @@ int film_enqueue_chunk(prt_film* f) {
                                f->fb);
         PRT_HIP(hipGetLastError());
         PRT_HIP(hipEventRecord(f->ev1, st));
+        if (f->track) PRT_HIP(hipEventRecord(f->rev1[ring], st));
+        f->enq++;
         return PRT_OK;
     }
     PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 3 * kCounterStride * sizeof(uint32_t), st));
@@ -1165,6 +1178,8 @@ int film_enqueue_chunk(prt_film* f) {
         hipLaunchKernelGGL(k_film, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb);
     PRT_HIP(hipGetLastError());
     PRT_HIP(hipEventRecord(f->ev1, st));
+    if (f->track) PRT_HIP(hipEventRecord(f->rev1[ring], st));
+    f->enq++;
     return PRT_OK;
 }
 
@@ -1174,6 +1189,38 @@ double film_chunk_ms(prt_film* f) {
     if (hipEventElapsedTime(&t, f->ev0, f->ev1) == hipSuccess) f->plan_ms += t;
     return f->plan_ms;
 }
+
+int film_track_chunks(prt_film* f) {
+    if (f->track) return PRT_OK;
+    PRT_HIP(hipSetDevice(f->scene->device));
+    for (int i = 0; i < prt_film::kRing; i++) {
+        PRT_HIP(hipEventCreate(&f->rev0[i]));
+        PRT_HIP(hipEventCreate(&f->rev1[i]));
+    }
+    f->track = true;
+    f->seen = f->enq;
+    return PRT_OK;
+}
+
+// Chunks still on the device; each chunk seen finished adds its device time to the plan's.
+int film_chunks_pending(prt_film* f) {
+    while (f->seen < f->enq) {
+        const int i = (int)(f->seen % prt_film::kRing);
+        if (hipEventQuery(f->rev1[i]) == hipErrorNotReady) break;
+        float t = 0;
+        if (hipEventElapsedTime(&t, f->rev0[i], f->rev1[i]) == hipSuccess) f->plan_ms += t;
+        f->seen++;
+    }
+    return (int)(f->enq - f->seen);
+}
+
+double film_plan_ms(const prt_film* f) { return f->plan_ms; }
+
+int64_t film_capacity_paths(const prt_film* f) { return f->capacity; }
+
+void film_set_chunk_paths(prt_film* f, int64_t n) { f->chunk_paths = std::max<int64_t>(1, std::min(n, f->capacity)); }
+
+int64_t bdpt_grid_paths(const prt_scene* scene) { return (int64_t)bdpt_blocks(scene, film_blocks(scene)) * kBlock; }
 
 // Enqueues `spp` samples per pixel for every pixel on the film's stream (no wait).
 int film_enqueue(prt_film* f, int32_t spp) {
@@ -1258,6 +1305,7 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     f->npix = (int64_t)w * h;
     f->capacity = film_capacity(
         f->npix, p->integrator == PRT_INTEGRATOR_BDPT ? bdpt_capacity(p->max_depth, cap, scene->dev.textured != 0) : cap);
+    f->chunk_paths = f->capacity;
     int rc;
     auto bail = [&](int code) {
         prt_film_destroy(f);
@@ -1411,6 +1459,10 @@ int prt_film_destroy(prt_film* f) {
     for (void* p : f->bufs) (void)hipFree(p);
     if (f->ev0) (void)hipEventDestroy(f->ev0);
     if (f->ev1) (void)hipEventDestroy(f->ev1);
+    for (int i = 0; i < prt_film::kRing; i++) {
+        if (f->rev0[i]) (void)hipEventDestroy(f->rev0[i]);
+        if (f->rev1[i]) (void)hipEventDestroy(f->rev1[i]);
+    }
     if (f->stream) (void)hipStreamDestroy(f->stream);
     const_cast<prt_scene*>(f->scene)->users--;
     delete f;

@@ -39,7 +39,7 @@ struct Req {
     Clock::time_point t_submit;
     bool cancel = false;
     bool release = false;
-    int in_flight = 0;       // chunks of this request on the device (at most 2)
+    int in_flight = 0;       // chunks of this request on the device
     int readers = 0;
     // PRT_SUBMIT_SNAPSHOTS: copies of the film taken when passes completed, the newest kSnapSlots kept
     bool snapshots = false;
@@ -53,18 +53,27 @@ struct Req {
     Snap snap[kSnapSlots];
 };
 
-// Film slots of serving requests hold at most this many paths: a pass is enqueued chunk by chunk, so
-// the scheduler can give first passes the device between chunks of longer renders. A BDPT sample
-// costs several path-tracing samples, so its chunks are smaller (still one sample per thread of the
-// resident BDPT grid).
-constexpr int64_t kServeCapacity = (int64_t)1 << 19;
-constexpr int64_t kServeCapacityBdpt = (int64_t)1 << 17;
+// A pass is enqueued chunk by chunk, so the scheduler can give first passes the device between
+// chunks of longer renders. A first pass (one sample per pixel) runs as one chunk over the whole
+// image where the film's path slots allow it (path tracing: up to 2^21 paths; BDPT: the subpath
+// stores' 2 GiB bound, at least one full grid of the BDPT kernels): each chunk ends in a latency-bound
+// tail of long paths, paid once instead of per chunk. Later passes run in chunks small enough that a
+// new request's first pass waits little behind them; a BDPT sample costs several path-tracing
+// samples, so its later chunks are smaller.
+constexpr int64_t kFirstChunkMax = (int64_t)1 << 21;
+constexpr int64_t kLaterChunk = (int64_t)1 << 19;
+constexpr int64_t kLaterChunkBdpt = (int64_t)1 << 17;
 // Chunks of later passes in flight on the device at once (over all requests). A new request's first
 // pass waits for at most these, whichever hardware queue its stream lands on.
 constexpr int kBackgroundChunks = 8;
+// Chunks of a first pass in flight: one running, one queued behind it.
+constexpr int kFirstPassChunks = 2;
 
-int64_t serve_capacity(const prt_render_params* p) {
-    return p->integrator == PRT_INTEGRATOR_BDPT ? kServeCapacityBdpt : kServeCapacity;
+int64_t serve_capacity(const prt_scene* s, const prt_render_params* p) {
+    const int64_t w = p->width > 0 ? p->width : s->image_width;
+    const int64_t h = p->height > 0 ? p->height : s->image_height;
+    if (p->integrator == PRT_INTEGRATOR_BDPT) return std::max(w * h, prt::bdpt_grid_paths(s));
+    return std::max(kLaterChunk, std::min(w * h, kFirstChunkMax));
 }
 
 }  // namespace
@@ -145,6 +154,9 @@ struct prt_server {
         if (rc == PRT_OK) {
             r->pass++;
             r->target = target;
+            const bool bdpt = r->params.integrator == PRT_INTEGRATOR_BDPT;
+            prt::film_set_chunk_paths(r->film, r->pass == 1 ? prt::film_capacity_paths(r->film)
+                                                            : (bdpt ? kLaterChunkBdpt : kLaterChunk));
             rc = prt::film_plan_pass(r->film, target);
         }
         if (rc != PRT_OK) {
@@ -176,19 +188,16 @@ struct prt_server {
                 }
                 if (used + (int64_t)r->bytes > budget) break;  // wait for memory to free up
                 queue.pop_front();
-                int rc = prt::film_create_ex(r->scene, &r->params, serve_capacity(&r->params), &r->film);
+                int rc = prt::film_create_ex(r->scene, &r->params, serve_capacity(r->scene, &r->params), &r->film);
+                if (rc == PRT_OK) rc = prt::film_track_chunks(r->film);
+                if (rc == PRT_OK && r->snapshots) rc = alloc_snapshots(r);
+                used += (int64_t)r->bytes;  // free_film returns it
                 if (rc != PRT_OK) {
-                    r->state = St::Failed;
-                    emit(r, PRT_EVENT_ERROR, 0, 0, rc, 0);
-                    continue;
-                }
-                if (r->snapshots && (rc = alloc_snapshots(r)) != PRT_OK) {
                     free_film(r);
                     r->state = St::Failed;
                     emit(r, PRT_EVENT_ERROR, 0, 0, rc, 0);
                     continue;
                 }
-                used += (int64_t)r->bytes;
                 r->state = St::Running;
                 active.push_back(r);
                 if (next_pass(r)) emit(r, PRT_EVENT_ADMITTED, r->pass, r->target, 0, 0);
@@ -198,12 +207,11 @@ struct prt_server {
             for (size_t k = 0; k < active.size();) {
                 Req* r = active[k];
                 bool remove = false;
+                const int was_in_flight = r->in_flight;
                 if (r->state != St::Running) {
                     remove = true;
-                } else if (r->in_flight == 0 || prt::film_idle(r->film)) {
-                    double ms = r->in_flight ? prt::film_chunk_ms(r->film) : 0.0;
-                    if (r->in_flight) progressed = true;
-                    r->in_flight = 0;
+                } else if (r->in_flight == 0 || (r->in_flight = prt::film_chunks_pending(r->film)) == 0) {
+                    const double ms = prt::film_plan_ms(r->film);
                     // adaptive passes: the finished round decides the next one (film idle, quick read)
                     if (prt::film_round_exhausted(r->film) && !prt::film_plan_done(r->film)) {
                         if (prt::film_advance(r->film) != PRT_OK) {
@@ -239,6 +247,7 @@ struct prt_server {
                         }
                     }
                 }
+                if (r->in_flight != was_in_flight) progressed = true;
                 if (remove) {
                     active.erase(active.begin() + k);
                 } else {
@@ -246,8 +255,8 @@ struct prt_server {
                 }
             }
             // ---- chunk dispatch ----
-            // Requests that still owe their first pass get the device first: each keeps up to four
-            // chunks queued, and while any is owed the other requests receive no new chunks (theirs in
+            // Requests that still owe their first pass get the device first: each keeps
+            // kFirstPassChunks chunks queued, and while any is owed the other requests receive no new chunks (theirs in
             // flight drain). Otherwise requests take turns (round robin) to keep up to two chunks each
             // queued, within kBackgroundChunks over all requests, so the device never waits for the
             // scheduler and a newly admitted request never waits behind a long backlog.
@@ -270,7 +279,7 @@ struct prt_server {
             };
             for (Req* r : active) {
                 if (r->state != St::Running || r->completed_pass != 0) continue;
-                while (r->in_flight < 4 && !prt::film_round_exhausted(r->film) && enqueue(r)) {
+                while (r->in_flight < kFirstPassChunks && !prt::film_round_exhausted(r->film) && enqueue(r)) {
                 }
             }
             if (!first_owed && !active.empty()) {
@@ -349,7 +358,7 @@ int prt_server_submit_ex(prt_server* s, const prt_scene* scene, const prt_render
     r->scene = scene;
     const_cast<prt_scene*>(scene)->users++;  // until the request is released
     r->params = *p;
-    r->bytes = film_device_bytes(scene, p, serve_capacity(p));
+    r->bytes = film_device_bytes(scene, p, serve_capacity(scene, p));
     if (flags & PRT_SUBMIT_SNAPSHOTS) {  // sums, counts and (BDPT) splats per pixel, per slot
         const int64_t w = p->width > 0 ? p->width : scene->image_width;
         const int64_t h = p->height > 0 ? p->height : scene->image_height;
