@@ -18,7 +18,12 @@ constexpr uint32_t kLightSlot = 70, kDirectSlot = 140, kSplatSlot = 210;
 constexpr int kBdptMaxDepth = 64;
 constexpr float kSplatScale = 16777216.0f;  // fixed point: 2^24 units per radiance unit
 
-enum : uint32_t { VF_LIGHT = 1, VF_CAMERA = 2, VF_SPECULAR = 4, VF_INF = 8, VF_FRONT = 16 };
+// VF_AREA / VF_DELTA: a vertex on a light (light >= 0) whose type is an area light (LightTypeArea) /
+// a point light (LightTypePoint), set where the vertex is made so that the MIS weights read no light.
+// VF_CONN: the vertex counts in its subpath's MIS walk (store_walk_sums).
+enum : uint32_t {
+    VF_LIGHT = 1, VF_CAMERA = 2, VF_SPECULAR = 4, VF_INF = 8, VF_FRONT = 16, VF_AREA = 32, VF_DELTA = 64, VF_CONN = 128
+};
 
 // Vertex, bdpt.go:14-44 (Light is the index of a scene light, -1 when the vertex carries none)
 struct BV {
@@ -27,12 +32,13 @@ struct BV {
     int mat, light;
     uint32_t flags;
     float u = 0.0f, v = 0.0f;  // surface UV (textured scenes)
+    float q = 0.0f;            // MIS walk sum below this vertex (store_walk_sums)
 };
 
 // Subpath vertices in global memory as five planes of float4 (six with surface UVs in textured
 // scenes), plane f of vertex k of sample i at [(f * kmax + k) * stride + i] (each plane coalesced
-// across a wave). Plane 0 is the record the MIS walk reads for every vertex:
-// (pdf_fwd, pdf_rev, flags | (mat + 1) << 8, light).
+// across a wave). Plane 0 is the vertex record (pdf_fwd, pdf_rev, flags | (mat + 1) << 8, light);
+// plane 4 holds (Le, q).
 struct VertexStore {
     float4* data;
     uint32_t stride;  // samples per chunk
@@ -47,7 +53,7 @@ struct VertexStore {
         *at(1, k, i) = make_float4(v.p.x, v.p.y, v.p.z, v.beta.x);
         *at(2, k, i) = make_float4(v.n.x, v.n.y, v.n.z, v.beta.y);
         *at(3, k, i) = make_float4(v.wi.x, v.wi.y, v.wi.z, v.beta.z);
-        *at(4, k, i) = make_float4(v.Le.x, v.Le.y, v.Le.z, 0.0f);
+        *at(4, k, i) = make_float4(v.Le.x, v.Le.y, v.Le.z, v.q);
         if (uv) *at(5, k, i) = make_float4(v.u, v.v, 0.0f, 0.0f);
     }
     __device__ __forceinline__ float4 rec(int k, uint32_t i) const { return *at(0, k, i); }
@@ -65,6 +71,7 @@ struct VertexStore {
         v.wi = v3(c.x, c.y, c.z);
         v.beta = v3(a.w, b.w, c.w);
         v.Le = v3(e.x, e.y, e.z);
+        v.q = e.w;
         if (uv) {
             float4 t = *at(5, k, i);
             v.u = t.x;
@@ -84,8 +91,14 @@ __device__ __forceinline__ bool light_area(const DevScene& sc, int li) {  // Lig
 __device__ __forceinline__ bool light_delta(const DevScene& sc, int li) {  // LightTypePoint
     return sc.lights[li].type == PRT_LIGHT_POINT_SPOT;
 }
+// flags of a vertex made on light li
+__device__ __forceinline__ uint32_t light_vertex_flags(const DevScene& sc, int li) {
+    const int t = sc.lights[li].type;
+    return VF_LIGHT | (light_is_infinite(t) ? VF_INF : 0u) | (light_area(sc, li) ? VF_AREA : 0u) |
+           (light_delta(sc, li) ? VF_DELTA : 0u);
+}
 __device__ __forceinline__ bool on_surface(const DevScene& sc, const BV& v) {  // IsOnSurface :33-41
-    if ((v.flags & VF_LIGHT) && v.light >= 0) return light_area(sc, v.light);
+    if ((v.flags & VF_LIGHT) && v.light >= 0) return (v.flags & VF_AREA) != 0;
     return v.mat >= 0;
 }
 __device__ __forceinline__ Surf surf_of(const BV& v) {
@@ -334,104 +347,88 @@ __device__ __forceinline__ float vtx_pdf(const DevScene& sc, const DevCamera& ca
     return convert_density(sc, cur, next, pdf);
 }
 
+// The part of a subpath's MIS walk that no connection changes, once per subpath (after its densities
+// are final). With r_m = remap0(rev_m) / remap0(fwd_m) and conn_m the walk's rule for vertex m (neither m
+// nor the vertex before it specular or on a point light; the camera walk ends at vertex 1, the light walk
+// at vertex 0), P(m) = sum over k <= m of conn_k * r_k * ... * r_m = r_m * (conn_m + P(m - 1)). Vertex m
+// keeps P(m - 1) as q and conn_m as VF_CONN.
+__device__ __forceinline__ void store_walk_sums(const VertexStore& VS, uint32_t i, int n, bool light) {
+    float P = 0.0f;
+    bool prev_ok = true;
+    for (int m = 0; m < n; m++) {
+        float4* rp = VS.at(0, m, i);
+        const float4 r = *rp;
+        uint32_t fm = __float_as_uint(r.z);
+        const bool ok = !(fm & (VF_SPECULAR | VF_DELTA));
+        VS.at(4, m, i)->w = P;
+        if (m > 0 || light) {
+            const bool conn = ok && prev_ok;
+            if (conn) {
+                fm |= VF_CONN;
+                rp->z = __uint_as_float(fm);
+            }
+            P = (remap0(r.y) / remap0(r.x)) * ((conn ? 1.0f : 0.0f) + P);
+        }
+        prev_ok = ok;
+    }
+}
+
 // calculateMISWeight with the per-vertex rules of calculateMIS{Camera,Light}VertexPdfs (:52-187).
 // ct1 = camera vertex t-1, ls1 = light vertex s-1 (s >= 1), sv = the sampled vertex of the s = 1 and
-// t = 1 strategies. Only the vertices next to the connection are read whole; the walk over the rest
-// reads their records.
+// t = 1 strategies. Only the two vertices next to the connection on each side are read.
 __device__ __forceinline__ float mis_weight(const DevScene& sc, const DevCamera& cam, const VertexStore& CP,
                                            const VertexStore& LP, uint32_t i, const BV& sv, const BV& ct1,
                                            const BV& ls1, int s, int t) {
     if (s + t == 2) return 1.0f;
+    BV ct2, ls2;
+    // each side's walk past its second vertex: (conn + q) and that vertex's forward density
+    float c_tail = 0.0f, c_fwd2 = 1.0f, l_tail = 0.0f, l_fwd2 = 1.0f;
+    bool c_first = false;
+    if (t > 1) {
+        ct2 = CP.get(t - 2, i);
+        c_first = !(ct2.flags & VF_SPECULAR);
+        c_tail = ((ct2.flags & VF_CONN) ? 1.0f : 0.0f) + ct2.q;
+        c_fwd2 = remap0(ct2.pdf_fwd);
+    }
+    if (s > 1) {
+        ls2 = LP.get(s - 2, i);
+        l_tail = ((ls2.flags & VF_CONN) ? 1.0f : 0.0f) + ls2.q;
+        l_fwd2 = remap0(ls2.pdf_fwd);
+    }
     // reverse densities of the vertices next to the connection
     float c_rev1 = 0.0f, c_rev2 = 0.0f, l_rev1 = 0.0f, l_rev2 = 0.0f;
-    bool has_c2 = false, has_l2 = false;
     if (s == 0) {
-        if (t > 1) {
-            BV ct2 = CP.get(t - 2, i);
-            c_rev1 = vtx_light_origin_pdf(sc, ct1, ct2);
-            if (t > 2) {
-                c_rev2 = vtx_pdf(sc, cam, ct1, nullptr, ct2);
-                has_c2 = true;
-            }
-        }
+        if (t > 1) c_rev1 = vtx_light_origin_pdf(sc, ct1, ct2);
+        if (t > 2) c_rev2 = vtx_pdf(sc, cam, ct1, nullptr, ct2);
     } else if (t == 1) {
         if (s > 1) {
-            BV ls2 = LP.get(s - 2, i);
             l_rev1 = vtx_pdf(sc, cam, sv, nullptr, ls1);
             l_rev2 = vtx_pdf(sc, cam, ls1, &sv, ls2);
-            has_l2 = true;
         }
     } else if (s == 1) {
-        BV ct2 = CP.get(t - 2, i);
         c_rev1 = vtx_pdf(sc, cam, sv, nullptr, ct1);
         c_rev2 = vtx_pdf(sc, cam, ct1, &sv, ct2);
-        has_c2 = true;
         l_rev1 = vtx_pdf(sc, cam, ct1, &ct2, sv);
     } else {
-        BV ct2 = CP.get(t - 2, i), ls2 = LP.get(s - 2, i);
         c_rev1 = vtx_pdf(sc, cam, ls1, &ls2, ct1);
         c_rev2 = vtx_pdf(sc, cam, ct1, &ls1, ct2);
-        has_c2 = true;
         l_rev1 = vtx_pdf(sc, cam, ct1, &ct2, ls1);
         l_rev2 = vtx_pdf(sc, cam, ls1, &ct1, ls2);
-        has_l2 = true;
     }
-    float sum = 0.0f, ri = 1.0f;
-    if (t > 1) {
-        float4 cur = CP.rec(t - 1, i);
-        for (int k = t - 1; k > 0; k--) {
-            float4 prev = CP.rec(k - 1, i);
-            float fwd = cur.x, rev = cur.y;
-            bool conn = !rec_specular(cur);
-            if (k == t - 1) {
-                rev = c_rev1;
-                conn = true;
-            } else if (k == t - 2 && has_c2) {
-                rev = c_rev2;
-            }
-            conn = conn && !rec_specular(prev);
-            ri *= remap0(rev) / remap0(fwd);
-            if (conn) sum += ri;
-            cur = prev;
-        }
+    // The walks of calculateMISWeight (bdpt_mis.go:30-50): from the connection outwards, ri is the running
+    // product of remap0(rev) / remap0(fwd) and connectible vertices add ri. Only the two vertices next to
+    // the connection take the densities above; the rest of each walk is the stored sum q (store_walk_sums).
+    float sum = 0.0f;
+    if (t > 1) {  // camera walk, vertices t-1 .. 1
+        float ri = remap0(c_rev1) / remap0(ct1.pdf_fwd);
+        if (c_first) sum += ri;
+        if (t > 2) sum += ri * (remap0(c_rev2) / c_fwd2) * c_tail;
     }
-    ri = 1.0f;
-    if (s > 0) {
-        // light vertices on a delta (point) light are not connectible (bdpt_mis.go:129-139)
-        auto delta = [&](float4 r) {
-            const uint32_t fl = __float_as_uint(r.z);
-            const int li = __float_as_int(r.w);
-            return (fl & VF_LIGHT) != 0 && li >= 0 && light_delta(sc, li);
-        };
-        float4 cur = LP.rec(s - 1, i);
-        for (int k = s - 1; k >= 0; k--) {
-            float4 prev = k > 0 ? LP.rec(k - 1, i) : make_float4(0.f, 0.f, 0.f, 0.f);
-            float fwd = cur.x, rev = cur.y;
-            bool conn = !rec_specular(cur) && !delta(cur);
-            if (k > 0) conn = conn && !rec_specular(prev) && !delta(prev);
-            if (t == 1) {
-                if (k == s - 1 && s > 1) {
-                    rev = l_rev1;
-                    conn = true;
-                } else if (k == s - 2 && has_l2) {
-                    rev = l_rev2;
-                }
-            } else if (s == 1) {
-                fwd = sv.pdf_fwd;
-                rev = l_rev1;
-                conn = true;
-            } else {
-                if (k == s - 1) {
-                    rev = l_rev1;
-                    conn = true;
-                } else if (k == s - 2) {
-                    rev = l_rev2;
-                }
-            }
-            ri *= remap0(rev) / remap0(fwd);
-            if (conn) sum += ri;
-            cur = prev;
-        }
+    if (s == 1) {  // light walk, the sampled light vertex only
+        sum += remap0(l_rev1) / remap0(sv.pdf_fwd);
+    } else if (s > 1) {  // light walk, vertices s-1 .. 0
+        const float ri = remap0(l_rev1) / remap0(ls1.pdf_fwd);
+        sum += ri + ri * (remap0(l_rev2) / l_fwd2) * l_tail;
     }
     return 1.0f / (1.0f + sum);
 }

@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             lv.mat = -1;
             lv.light = li;
             bool inf = light_is_infinite(L.type);
-            lv.flags = VF_LIGHT | (inf ? VF_INF : 0u);
+            lv.flags = light_vertex_flags(sc, li);
             LP.put(0, i, lv);
             V3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
             nl = (uint32_t)bdpt_extend<MIXED>(sc, LP, i, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false,
@@ -529,10 +529,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     }
 }
 
+// The MIS walk sums of both subpaths of every sample (store_walk_sums), between the subpath and the
+// strategy kernels.
+__global__ __launch_bounds__(kBlock) void k_bdpt_walks(PassParams pp, VertexStore CP, VertexStore LP,
+                                                       const uint32_t* lens) {
+    const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t len = lens[i];
+    store_walk_sums(CP, i, (int)(len & 0xFFFFu), false);
+    store_walk_sums(LP, i, (int)(len >> 16), true);
+}
+
 // Strategies: one thread per sample evaluates every (s, t) of RayColor (bdpt.go:83-99 with
 // evaluateBDPTStrategy :294-313), camera-side radiance to pb.L, light-tracing splats to `splat`.
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_bdpt_connect(DevScene sc, PassParams pp, PathBuffers pb, VertexStore CP,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bdpt_connect(DevScene sc, PassParams pp, PathBuffers pb, VertexStore CP,
                                                          VertexStore LP, const uint32_t* lens, long long* splat,
                                                          uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
@@ -639,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_connect(DevScene sc, PassParams
                 sampled.pdf_rev = 0.0f;
                 sampled.mat = -1;
                 sampled.light = li;
-                sampled.flags = VF_LIGHT | (light_is_infinite(sc.lights[li].type) ? VF_INF : 0u);
+                sampled.flags = light_vertex_flags(sc, li);
                 Lc = contrib;
             } else {  // evaluateConnectionStrategy :489-561
                 const BV &lv = ls1, &cvt = ct1;
@@ -1131,14 +1143,17 @@ int film_enqueue_chunk(prt_film* f) {
     PRT_HIP(hipEventRecord(f->ev0, st));
     if (f->bdpt) {
         int grid = (int)std::min<int64_t>(f->bdpt_blocks, (n + kBlock - 1) / kBlock);
+        const int walk_blocks = (int)((n + kBlock - 1) / kBlock);
         if (sc.mixed) {
             hipLaunchKernelGGL((k_bdpt_paths<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath,
                                f->lens, f->gstack, f->gslots);
+            hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
             hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
         } else {
             hipLaunchKernelGGL((k_bdpt_paths<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath,
                                f->lens, f->gstack, f->gslots);
+            hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
             hipLaunchKernelGGL((k_bdpt_connect<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
         }
