@@ -571,8 +571,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         V3 total = v3(0, 0, 0);
         // (s, t) in the reference's order (s outer, t inner), as one flat loop per thread
         const int nst = (nl + 1) * nc;
+        int s = 0, t = 0;  // pair j = s * nc + (t - 1), stepped without a division
         for (int j = 0; j < nst; j++) {
-            const int s = j / nc, t = j - s * nc + 1;
+            if (++t > nc) {
+                t = 1;
+                s++;
+            }
             if (s == 1 && t == 1) continue;
             if (s == 0 && t < nc) continue;
             V3 Lc = v3(0, 0, 0);

@@ -336,8 +336,10 @@ __global__ __launch_bounds__(BLOCK) void k_intersect_refill(DevScene sc, const p
 // current chunk runs out, so neither the ray loads nor the work counter add latency to the steps.
 constexpr uint32_t kPfChunk = 256;
 
+// Triangle-only scenes: 6 waves per SIMD (80 VGPRs, the LDS bound of the 8-entry stack and 64-node cache)
+// hide more node-fetch latency than the register count the compiler picks on its own (84, 5 waves).
 template <bool ANY, bool MIXED, int LDS, int NCACHE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_intersect_pf(DevScene sc, const prt_ray* __restrict__ rays, int64_t n,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 6))) void k_intersect_pf(DevScene sc, const prt_ray* __restrict__ rays, int64_t n,
                                                         prt_hit* __restrict__ hits, uint8_t* __restrict__ occluded,
                                                         uint2* gstack, uint32_t gstride, unsigned long long* work) {
     __shared__ uint32_t s_node[LDS * BLOCK];
@@ -1189,8 +1191,8 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     int max_blocks = 1;
     for (int v = 0; v < kNumVariants; v++) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, variant_fn<false, true>(v), kVariantBlock[v], 0) !=
-                hipSuccess ||
+        const void* fn = s->dev.mixed ? variant_fn<false, true>(v) : variant_fn<false, false>(v);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kVariantBlock[v], 0) != hipSuccess ||
             per_cu <= 0)
             per_cu = 2;
         per_cu = std::min(per_cu, 12 * 256 / kVariantBlock[v]);
