@@ -27,7 +27,7 @@
 namespace prt {
 
 constexpr int kBlock = 256;
-constexpr int kLds = 8;
+constexpr int kLds = 6;
 constexpr int kNodeCache = 64;  // top BVH nodes kept in LDS by the trace kernels
 
 struct PathBuffers {
@@ -714,6 +714,30 @@ __device__ __forceinline__ void kahan(float& s, float& c, float v) {
     s = t;
 }
 
+// vec3ToColor of GetColor (progressive.go:482-495, stats.go:36-55) for the pixels of a rectangle, in the
+// operations and order of the host mean (film_mean): (sum + splats) * (1 / count), then uint8(255 *
+// min(1, sqrt(mean))) in double with sqrt correctly rounded; NaN and negative means give 0.
+__global__ __launch_bounds__(kBlock) void k_film_rgba8(int32_t width, int32_t x0, int32_t y0, int32_t tw, int64_t n,
+                                                       const float4* sum, const uint32_t* count,
+                                                       const long long* splat, uchar4* out) {
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = (int64_t)(y0 + (int32_t)(k / tw)) * width + (x0 + (int32_t)(k % tw));
+    const uint32_t c = count[i];
+    const float inv = c ? 1.0f / (float)c : 0.0f;
+    const float4 S = sum[i];
+    uint8_t o[3];
+    for (int ch = 0; ch < 3; ch++) {
+        float v = ch == 0 ? S.x : (ch == 1 ? S.y : S.z);
+        if (splat) v += (float)((double)splat[3 * i + ch] * (1.0 / (double)kSplatScale));
+        const float mean = c ? v * inv : 0.0f;
+        double g = sqrt((double)mean);
+        g = g > 0.0 ? fmin(1.0, g) : 0.0;
+        o[ch] = (uint8_t)(255.0 * g);
+    }
+    out[k] = make_uchar4(o[0], o[1], o[2], 255);
+}
+
 __global__ __launch_bounds__(kBlock) void k_film(PassParams pp, PathBuffers pb, FilmBuffers fb) {
     int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= pp.npix) return;
@@ -867,6 +891,9 @@ struct prt_film {
     hipEvent_t rev0[kRing] = {}, rev1[kRing] = {};
     int64_t enq = 0, seen = 0;
     int64_t chunk_paths = 0;  // paths per planned chunk (at most capacity)
+    // RGBA8 reads (prt_film_read_rgba8): device image and pinned host staging, allocated at first use
+    uchar4* rgba_dev = nullptr;
+    uchar4* rgba_host = nullptr;
     // chunk plan of the samples being rendered (see film_plan)
     struct Chunk {
         int32_t px0, npix, spp, sample_base;
@@ -1476,6 +1503,7 @@ int prt_film_destroy(prt_film* f) {
     (void)hipSetDevice(f->scene->device);
     if (f->stream) (void)hipStreamSynchronize(f->stream);
     for (void* p : f->bufs) (void)hipFree(p);
+    if (f->rgba_host) (void)hipHostFree(f->rgba_host);
     if (f->ev0) (void)hipEventDestroy(f->ev0);
     if (f->ev1) (void)hipEventDestroy(f->ev1);
     for (int i = 0; i < prt_film::kRing; i++) {
@@ -1578,22 +1606,30 @@ int prt_film_read_rgba8(const prt_film* f, int32_t x0, int32_t y0, int32_t x1, i
     if (!f || !rgba) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
     if (x0 < 0 || y0 < 0 || x1 > f->width || y1 > f->height || x0 > x1 || y0 > y1)
         return fail(PRT_ERR_INVALID_ARGUMENT, "rectangle outside the image");
-    std::vector<float> mean(3 * f->npix);
-    std::vector<uint32_t> cnt(f->npix);
-    int rc = prt_film_read(f, mean.data(), nullptr, nullptr, cnt.data());
-    if (rc != PRT_OK) return rc;
-    const int tw = x1 - x0;
-    for (int y = y0; y < y1; y++)
-        for (int x = x0; x < x1; x++) {
-            int64_t i = (int64_t)y * f->width + x;
-            uint8_t* o = rgba + 4 * ((int64_t)(y - y0) * tw + (x - x0));
-            for (int c = 0; c < 3; c++) {  // vec3ToColor: gamma 2, clamp, uint8 truncation (NaN -> 0)
-                double v = std::sqrt((double)mean[3 * i + c]);
-                v = v > 0.0 ? std::min(1.0, v) : 0.0;
-                o[c] = (uint8_t)(255.0 * v);
-            }
-            o[3] = 255;
-        }
+    const int32_t tw = x1 - x0, th = y1 - y0;
+    const int64_t n = (int64_t)tw * th;
+    if (n == 0) return PRT_OK;
+    prt_film* m = const_cast<prt_film*>(f);
+    PRT_HIP(hipSetDevice(f->scene->device));
+    if (!m->rgba_dev) {
+        void* p = nullptr;
+        int rc = film_alloc(m, sizeof(uchar4) * (size_t)f->npix, &p);
+        if (rc != PRT_OK) return rc;
+        m->rgba_dev = (uchar4*)p;
+    }
+    if (!m->rgba_host) {
+        void* p = nullptr;
+        hipError_t e = hipHostMalloc(&p, sizeof(uchar4) * (size_t)f->npix, hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(rgba)");
+        m->rgba_host = (uchar4*)p;
+    }
+    // ordered after the work already enqueued on the film's stream
+    hipLaunchKernelGGL(k_film_rgba8, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, f->stream,
+                       f->width, x0, y0, tw, n, f->fb.sum, f->fb.count, f->fb.splat, m->rgba_dev);
+    PRT_HIP(hipGetLastError());
+    PRT_HIP(hipMemcpyAsync(m->rgba_host, m->rgba_dev, sizeof(uchar4) * (size_t)n, hipMemcpyDeviceToHost, f->stream));
+    PRT_HIP(hipStreamSynchronize(f->stream));
+    std::memcpy(rgba, m->rgba_host, sizeof(uchar4) * (size_t)n);
     return PRT_OK;
 }
 

@@ -164,3 +164,21 @@ def test_mesh_options_rotation_and_custom_normals():
         e.set_camera(center=(0, 0, 5), look_at=(0, 0, 0), up=(0, 1, 0), width=8, aspect_ratio=1.0, vfov=20.0)
         m = gpu_first_sample(e, dict(max_depth=4, rr_min_bounces=4))
         assert np.allclose(m[4, 4], expect)
+
+
+@pytest.mark.parametrize("integrator", [P.INTEGRATOR_PATH, P.INTEGRATOR_BDPT])
+def test_rgba8_image_is_the_gamma_of_the_mean(integrator):
+    """prt_film_read_rgba8 (converted on the device) equals uint8(255 * min(1, sqrt(mean))) computed in
+    float64 from the film's mean (BDPT: with its splats), for the whole image and a rectangle."""
+    s, cfg = scenes.cornell_scene(width=72)
+    w, h = s.image_size
+    p = P.sampling_params(w, h, 5, 16, 8, seed=4, max_passes=3, integrator=integrator)
+    pr = P.ProgressiveRaytracer(P.Scene(s), p)
+    pr.render_pass(2)
+    mean = pr.read()[0].astype(np.float64)
+    exp = (255 * np.where(mean > 0, np.minimum(1.0, np.sqrt(np.maximum(mean, 0))), 0)).astype(np.uint8)
+    img = pr.image()
+    assert np.array_equal(img[..., :3], exp) and np.all(img[..., 3] == 255)
+    sub = pr.image(5, 7, 41, 30)
+    assert sub.shape == (23, 36, 4) and np.array_equal(sub[..., :3], exp[7:30, 5:41])
+    assert pr.image(3, 3, 3, 9).shape == (6, 0, 4)
