@@ -529,6 +529,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     }
 }
 
+// Maximum of a small non-negative value (below 128) over the active lanes of the wave.
+__device__ __forceinline__ int wave_max_small(int v) {
+    int m = 0;
+    for (int bit = 6; bit >= 0; bit--) {
+        const int c = m | (1 << bit);
+        if (__ballot(v >= c) != 0) m = c;
+    }
+    return m;
+}
+
 // The MIS walk sums of both subpaths of every sample (store_walk_sums), between the subpath and the
 // strategy kernels.
 __global__ __launch_bounds__(kBlock) void k_bdpt_walks(PassParams pp, VertexStore CP, VertexStore LP,
@@ -569,16 +579,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         const uint32_t len = lens[i];
         const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
         V3 total = v3(0, 0, 0);
-        // (s, t) in the reference's order (s outer, t inner), as one flat loop per thread
-        const int nst = (nl + 1) * nc;
-        int s = 0, t = 0;  // pair j = s * nc + (t - 1), stepped without a division
-        for (int j = 0; j < nst; j++) {
-            if (++t > nc) {
+        // The reference's pairs (s = 0 with t = nc, every other s <= nl and t <= nc but (1, 1)) in four
+        // phases: (0, nc); (1, t) for t = 2..nc; (s, 1) for s = 2..nl; (s, t >= 2) for s = 2..nl in s-major
+        // order. The first three run in steps of the same length across the wave, so that a wave evaluates
+        // one kind of strategy at a time; the camera-side sum keeps the reference's (s outer, t inner)
+        // order (phase 3 only splats, with integer atomics).
+        const int lb = nl >= 1 ? nc - 1 : 0, lc = nl >= 2 ? nl - 1 : 0;
+        const int wb = wave_max_small(lb), wc = wave_max_small(lc);
+        const int npairs = 1 + wb + wc + lc * (nc - 1);
+        int ds = 2, dt = 1;  // phase 4 position
+        for (int j = 0; j < npairs; j++) {
+            int s, t;
+            if (j == 0) {
+                s = 0;
+                t = nc;
+            } else if (j <= wb) {
+                if (j > lb) continue;
+                s = 1;
+                t = j + 1;
+            } else if (j <= wb + wc) {
+                const int k = j - wb;
+                if (k > lc) continue;
+                s = k + 1;
                 t = 1;
-                s++;
+            } else {
+                if (++dt > nc) {
+                    dt = 2;
+                    ds++;
+                }
+                s = ds;
+                t = dt;
             }
-            if (s == 1 && t == 1) continue;
-            if (s == 0 && t < nc) continue;
             V3 Lc = v3(0, 0, 0);
             bool splatted = false;
             int sx = 0, sy = 0;
