@@ -30,15 +30,15 @@ constexpr int kBlock = 256;
 constexpr int kLds = 6;
 constexpr int kNodeCache = 64;  // top BVH nodes kept in LDS by the trace kernels
 
+// The Russian-roulette throughput rides in the free fourth lanes of o, d and beta; a shadow ray starts
+// at the path's current origin o (written for paths that continue or cast a shadow ray).
 struct PathBuffers {
-    float4* o;      // origin
-    float4* d;      // direction (unnormalised as the reference passes scatter.Scattered)
-    float4* beta;   // path weight
-    float4* thr;    // Russian-roulette throughput
+    float4* o;      // origin, throughput x
+    float4* d;      // direction (unnormalised as the reference passes scatter.Scattered), throughput y
+    float4* beta;   // path weight, throughput z
     float4* L;      // radiance gathered so far
     float4* hit;    // t, prim (bits), u, v
-    float4* so;     // shadow ray origin, tmin
-    float4* sd;     // shadow ray direction, tmax
+    float4* sd;     // shadow ray direction, tmax (tmin 0.001)
     float4* sc;     // shadow contribution
     uint32_t* key;
     uint32_t* pixel;
@@ -144,10 +144,9 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, P
         camera_ray(sc.cam, i, j, l0, l1, j0, j1, o, d);
         V3 beta = v3(1, 1, 1);
         if (pp.max_depth > 0) active = russian_roulette(pp, key, 0, v3(1, 1, 1), beta);
-        pb.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
-        pb.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
-        pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
-        pb.thr[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        pb.o[slot] = make_float4(o.x, o.y, o.z, 1.f);
+        pb.d[slot] = make_float4(d.x, d.y, d.z, 1.f);
+        pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 1.f);
         pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         pb.key[slot] = key;
         pb.pixel[slot] = pix;
@@ -190,13 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
         if (i >= count) continue;
         uint32_t slot = q[i];
         float4 o, d;
-        if (ANY) {
-            o = pb.so[slot];
-            d = pb.sd[slot];
-        } else {
-            o = pb.o[slot];
-            d = pb.d[slot];
-        }
+        o = pb.o[slot];
+        d = ANY ? pb.sd[slot] : pb.d[slot];
         TRay r;
         r.ox = o.x;
         r.oy = o.y;
@@ -204,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
         r.dx = d.x;
         r.dy = d.y;
         r.dz = d.z;
-        r.tmin = ANY ? o.w : 0.001f;
+        r.tmin = 0.001f;
         r.tmax = ANY ? d.w : __int_as_float(0x7f800000);
         THit h;
         bool hit = trace<ANY, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
@@ -235,10 +229,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
         if (i < count) {
             slot = q[i];
             float4 o4 = pb.o[slot], d4 = pb.d[slot], h4 = pb.hit[slot];
-            float4 b4 = pb.beta[slot], t4 = pb.thr[slot], l4 = pb.L[slot];
+            float4 b4 = pb.beta[slot];
             uint32_t key = pb.key[slot];
             V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
-            V3 beta = v3(b4.x, b4.y, b4.z), thr = v3(t4.x, t4.y, t4.z), L = v3(l4.x, l4.y, l4.z);
+            V3 beta = v3(b4.x, b4.y, b4.z), thr = v3(o4.w, d4.w, b4.w);
+            // L is read and written only when this bounce adds to it (a zero term leaves it unchanged;
+            // a non-finite one is added as the reference adds it)
+            V3 add = v3(0, 0, 0);
             THit h;
             h.t = h4.x;
             h.prim = __float_as_uint(h4.y);
@@ -250,10 +247,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
                 V3 e = v3(0, 0, 0);
                 for (int li = 0; li < sc.num_lights; li++)
                     if (light_is_infinite(sc.lights[li].type)) e = e + light_emit(sc, sc.lights[li], d);
-                L = L + mulv(beta, e);
+                add = mulv(beta, e);
             } else {
                 Surf s = make_surf(sc, o, d, h);
-                L = L + mulv(beta, emit_hit(sc, s.mat, d, s.front));  // getEmittedLight :100-105
+                add = mulv(beta, emit_hit(sc, s.mat, d, s.front));  // getEmittedLight :100-105
                 ScatterOut so;
                 if (scatter(sc, s.mat, d, s, key, pos + 1, so)) {
                     V3 nd = so.dir;
@@ -278,7 +275,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
                                     V3 c = mulv(beta, mulv(f, ls.emission) * (cosine * w / ls.pdf));
                                     if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
                                         float tmax = ls.dist - 0.001f;
-                                        pb.so[slot] = make_float4(s.p.x, s.p.y, s.p.z, 0.001f);
                                         pb.sd[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, tmax);
                                         pb.sc[slot] = make_float4(c.x, c.y, c.z, 0.f);
                                         shadow = true;
@@ -297,10 +293,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
                             cont = true;
                         }
                     }
-                    if (cont) {
-                        o = s.p;
-                        d = nd;
-                    }
+                    if (cont) d = nd;
+                    if (cont || shadow) o = s.p;
                 }
             }
             // next level: depth limit (:37-39) and Russian roulette (:42-45)
@@ -308,12 +302,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
                 if (b + 1 >= pp.max_depth) cont = false;
                 else cont = russian_roulette(pp, key, b + 1, thr, beta);
             }
-            pb.L[slot] = make_float4(L.x, L.y, L.z, 0.f);
+            if (add.x != 0.0f || add.y != 0.0f || add.z != 0.0f) {
+                const float4 l4 = pb.L[slot];
+                pb.L[slot] = make_float4(l4.x + add.x, l4.y + add.y, l4.z + add.z, 0.f);
+            }
+            if (cont || shadow) pb.o[slot] = make_float4(o.x, o.y, o.z, thr.x);
             if (cont) {
-                pb.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
-                pb.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
-                pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 0.f);
-                pb.thr[slot] = make_float4(thr.x, thr.y, thr.z, 0.f);
+                pb.d[slot] = make_float4(d.x, d.y, d.z, thr.y);
+                pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, thr.z);
             }
             (void)kInfF;
         }
@@ -1007,7 +1003,7 @@ int film_alloc(prt_film* f, size_t bytes, void** out) {
     return PRT_OK;
 }
 
-constexpr size_t kPathBytes = 9 * sizeof(float4) + 5 * sizeof(uint32_t);
+constexpr size_t kPathBytes = 7 * sizeof(float4) + 5 * sizeof(uint32_t);
 constexpr size_t kPixelBytes = 2 * sizeof(float4) + sizeof(float2) + sizeof(uint32_t);
 
 int64_t film_capacity(int64_t npix, int64_t cap) {
@@ -1396,10 +1392,8 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     FALLOC(f->pb.o, float4, C);
     FALLOC(f->pb.d, float4, C);
     FALLOC(f->pb.beta, float4, C);
-    FALLOC(f->pb.thr, float4, C);
     FALLOC(f->pb.L, float4, C);
     FALLOC(f->pb.hit, float4, C);
-    FALLOC(f->pb.so, float4, C);
     FALLOC(f->pb.sd, float4, C);
     FALLOC(f->pb.sc, float4, C);
     FALLOC(f->pb.key, uint32_t, C);
