@@ -41,7 +41,6 @@ struct PathBuffers {
     float4* sd;     // shadow ray direction, tmax (tmin 0.001)
     float4* sc;     // shadow contribution
     uint32_t* key;
-    uint32_t* pixel;
     uint32_t* queue[2];
     uint32_t* squeue;
     uint32_t* counters;  // queue lengths, one per 256-B line: path queues 0 and 1, shadow queue 2
@@ -149,7 +148,6 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, P
         pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, 1.f);
         pb.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         pb.key[slot] = key;
-        pb.pixel[slot] = pix;
     }
     if (pp.rr_min > 0 && pp.max_depth > 0) {
         // Russian roulette cannot end a path at level 0: every slot is active, the queue is the identity
@@ -1003,7 +1001,7 @@ int film_alloc(prt_film* f, size_t bytes, void** out) {
     return PRT_OK;
 }
 
-constexpr size_t kPathBytes = 7 * sizeof(float4) + 5 * sizeof(uint32_t);
+constexpr size_t kPathBytes = 7 * sizeof(float4) + 4 * sizeof(uint32_t);
 constexpr size_t kPixelBytes = 2 * sizeof(float4) + sizeof(float2) + sizeof(uint32_t);
 
 int64_t film_capacity(int64_t npix, int64_t cap) {
@@ -1397,7 +1395,6 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     FALLOC(f->pb.sd, float4, C);
     FALLOC(f->pb.sc, float4, C);
     FALLOC(f->pb.key, uint32_t, C);
-    FALLOC(f->pb.pixel, uint32_t, C);
     FALLOC(f->pb.queue[0], uint32_t, C);
     FALLOC(f->pb.queue[1], uint32_t, C);
     FALLOC(f->pb.squeue, uint32_t, C);
