@@ -44,8 +44,8 @@ def main():
                     help="only run a device-to-device copy of this size (counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (this build runs on a single "
-              "MI355X; sharding across GPUs is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (the renderer runs on a single "
+              "MI355X here)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
