@@ -42,8 +42,8 @@ struct PathBuffers {
     float4* sc;     // shadow contribution
     uint32_t* key;
     uint32_t* queue[2];
-    uint32_t* squeue;
-    uint32_t* counters;  // queue lengths, one per 256-B line: path queues 0 and 1, shadow queue 2
+    uint32_t* squeue[2];  // shadow rays of even / odd bounces
+    uint32_t* counters;   // queue lengths, one per 256-B line: path queues 0 and 1, shadow queues 2 and 3
 };
 
 struct PassParams {
@@ -158,9 +158,14 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, P
     }
 }
 
-template <bool ANY, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, int bounce, uint2* gstack,
-                                                  uint32_t gstride) {
+// One trace launch per bounce: the shadow rays the previous bounce cast (any hit, shadow queue sq, or
+// none when sq < 0) and, when `closest`, this bounce's extension rays (closest hit, path queue bounce & 1).
+// Neither depends on the other; both read the path origin. A wave takes 64-ray batches of one kind
+// (the extension rays' range is padded to whole batches), so its traversal mode is uniform.
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(DevScene sc, PathBuffers pb,
+                                                                                         int bounce, int closest, int sq,
+                                                  uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
@@ -174,21 +179,24 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
     load_node_cache<kNodeCache, kBlock>(sc, s_cache);
     st.ncache = s_cache;
     const uint32_t cur = (uint32_t)bounce & 1u;
-    const uint32_t count = ANY ? *qlen(pb, 2) : *qlen(pb, cur);
-    const uint32_t* q = ANY ? pb.squeue : pb.queue[cur];
-    if (!ANY && blockIdx.x == 0 && threadIdx.x == 0) {
+    const uint32_t nc = closest ? *qlen(pb, cur) : 0u;
+    const uint32_t ns = sq >= 0 ? *qlen(pb, 2u + (uint32_t)sq) : 0u;
+    const uint32_t* qc = pb.queue[cur];
+    const uint32_t* qs = pb.squeue[sq > 0 ? 1 : 0];
+    if (closest && blockIdx.x == 0 && threadIdx.x == 0) {
         *qlen(pb, cur ^ 1u) = 0;  // next bounce's queue (its last reader finished earlier)
-        *qlen(pb, 2) = 0;         // shadow queue of this bounce
+        *qlen(pb, 2u + cur) = 0;  // the shadow queue this bounce's shading fills (the last launch read it)
     }
+    const uint32_t nc64 = (nc + 63u) & ~63u;
     const uint32_t stride = gridDim.x * kBlock;
     // uniform trip count per wave so every lane reaches the loop exit together
-    for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < count; base += stride) {
-        uint32_t i = base + (threadIdx.x & 63u);
-        if (i >= count) continue;
-        uint32_t slot = q[i];
-        float4 o, d;
-        o = pb.o[slot];
-        d = ANY ? pb.sd[slot] : pb.d[slot];
+    for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nc64 + ns; base += stride) {
+        const bool any = base >= nc64;  // wave-uniform
+        const uint32_t i = (any ? base - nc64 : base) + (threadIdx.x & 63u);
+        if (i >= (any ? ns : nc)) continue;
+        const uint32_t slot = any ? qs[i] : qc[i];
+        const float4 o = pb.o[slot];
+        const float4 d = any ? pb.sd[slot] : pb.d[slot];
         TRay r;
         r.ox = o.x;
         r.oy = o.y;
@@ -197,16 +205,17 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, PathBuffers pb, i
         r.dy = d.y;
         r.dz = d.z;
         r.tmin = 0.001f;
-        r.tmax = ANY ? d.w : __int_as_float(0x7f800000);
         THit h;
-        bool hit = trace<ANY, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
-        if (ANY) {
-            if (!hit) {
+        if (any) {
+            r.tmax = d.w;
+            if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr)) {
                 float4 c = pb.sc[slot];
                 float4 l = pb.L[slot];
                 pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
             }
         } else {
+            r.tmax = __int_as_float(0x7f800000);
+            trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
             pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
         }
     }
@@ -311,7 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
             }
             (void)kInfF;
         }
-        block_push2(s_push, pb.queue[cur ^ 1u], qlen(pb, cur ^ 1u), cont, pb.squeue, qlen(pb, 2), shadow, slot);
+        block_push2(s_push, pb.queue[cur ^ 1u], qlen(pb, cur ^ 1u), cont, pb.squeue[cur], qlen(pb, 2u + cur), shadow,
+                    slot);
     }
 }
 
@@ -1001,7 +1011,7 @@ int film_alloc(prt_film* f, size_t bytes, void** out) {
     return PRT_OK;
 }
 
-constexpr size_t kPathBytes = 7 * sizeof(float4) + 4 * sizeof(uint32_t);
+constexpr size_t kPathBytes = 7 * sizeof(float4) + 5 * sizeof(uint32_t);
 constexpr size_t kPixelBytes = 2 * sizeof(float4) + sizeof(float2) + sizeof(uint32_t);
 
 int64_t film_capacity(int64_t npix, int64_t cap) {
@@ -1010,7 +1020,7 @@ int64_t film_capacity(int64_t npix, int64_t cap) {
 
 int film_blocks(const prt_scene* scene) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace<false, true>, kBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace<true>, kBlock, 0) !=
             hipSuccess ||
         per_cu <= 0)
         per_cu = 4;
@@ -1225,23 +1235,24 @@ int film_enqueue_chunk(prt_film* f) {
         f->enq++;
         return PRT_OK;
     }
-    PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 3 * kCounterStride * sizeof(uint32_t), st));
+    PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 4 * kCounterStride * sizeof(uint32_t), st));
     int cam_blocks = (int)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_camera, dim3(cam_blocks), dim3(kBlock), 0, st, sc, pp, f->pb);
-    auto trace = [&](bool any, int b) {
-        auto fn = any ? (sc.mixed ? (const void*)k_trace<true, true> : (const void*)k_trace<true, false>)
-                      : (sc.mixed ? (const void*)k_trace<false, true> : (const void*)k_trace<false, false>);
+    // bounce b: one trace launch (the shadow rays of bounce b - 1 and the extension rays of bounce b),
+    // then shading; a last launch traces the shadow rays of the last bounce
+    auto trace = [&](int b, int closest, int sq) {
+        auto fn = sc.mixed ? (const void*)k_trace<true> : (const void*)k_trace<false>;
         PathBuffers pbv = f->pb;
         uint2* gs = f->gstack;
         uint32_t gsl = f->gslots;
-        void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b, (void*)&gs, (void*)&gsl};
+        void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b, (void*)&closest, (void*)&sq, (void*)&gs, (void*)&gsl};
         return hipLaunchKernel(fn, dim3(f->blocks), dim3(kBlock), args, 0, st);
     };
     for (int b = 0; b < pp.max_depth; b++) {
-        PRT_HIP(trace(false, b));
+        PRT_HIP(trace(b, 1, b > 0 ? ((b - 1) & 1) : -1));
         hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
-        PRT_HIP(trace(true, b));
     }
+    if (pp.max_depth > 0) PRT_HIP(trace(pp.max_depth, 0, (pp.max_depth - 1) & 1));
     int film_blocks_n = (pp.npix + kBlock - 1) / kBlock;
     if (f->adaptive)
         hipLaunchKernelGGL(k_film_adaptive, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb, ap);
@@ -1397,8 +1408,9 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     FALLOC(f->pb.key, uint32_t, C);
     FALLOC(f->pb.queue[0], uint32_t, C);
     FALLOC(f->pb.queue[1], uint32_t, C);
-    FALLOC(f->pb.squeue, uint32_t, C);
-    FALLOC(f->pb.counters, uint32_t, 3 * kCounterStride);
+    FALLOC(f->pb.squeue[0], uint32_t, C);
+    FALLOC(f->pb.squeue[1], uint32_t, C);
+    FALLOC(f->pb.counters, uint32_t, 4 * kCounterStride);
     FALLOC(f->fb.sum, float4, f->npix);
     FALLOC(f->fb.comp, float4, f->npix);
     FALLOC(f->fb.sq, float2, f->npix);
