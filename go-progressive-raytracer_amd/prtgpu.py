@@ -662,7 +662,10 @@ class ProgressiveRaytracer:
         h = C.c_void_p()
         check(lib().prt_film_create(scene.handle, C.byref(params), C.byref(h)))
         self.handle = h
-        self.width, self.height = scene.desc.image_size
+        # the film's size: the params' width/height where set (the web server's override), else the camera's
+        w, hh = scene.desc.image_size
+        self.width = params.width if params.width > 0 else w
+        self.height = params.height if params.height > 0 else hh
 
     def close(self):
         if getattr(self, "handle", None):
