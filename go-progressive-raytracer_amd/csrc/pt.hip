@@ -1532,6 +1532,13 @@ int prt_film_reset(prt_film* f) {
     return PRT_OK;
 }
 
+int prt_film_size(const prt_film* f, int32_t* width, int32_t* height) {
+    if (!f || !width || !height) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
+    *width = f->width;
+    *height = f->height;
+    return PRT_OK;
+}
+
 int prt_film_destroy(prt_film* f) {
     if (!f) return PRT_OK;
     (void)hipSetDevice(f->scene->device);

@@ -145,7 +145,7 @@ EXPORTS = [
     "prt_last_error", "prt_version", "prt_device_count", "prt_scene_create", "prt_scene_info_get",
     "prt_scene_destroy", "prt_intersect_closest", "prt_intersect_any", "prt_traversal_counts",
     "prt_samples_for_pass", "prt_film_create", "prt_render_pass", "prt_render_samples", "prt_film_read",
-    "prt_film_read_splats", "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy",
+    "prt_film_read_splats", "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy", "prt_film_size",
     "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky", "prt_ply_load", "prt_ply_free",
     "prt_server_create", "prt_server_submit", "prt_server_poll", "prt_server_cancel", "prt_server_read",
     "prt_server_read_rgba8", "prt_server_release", "prt_server_stats_get", "prt_server_destroy",
@@ -209,6 +209,7 @@ def lib():
         "prt_film_read": (C.c_int, [vp, vp, vp, vp, vp]),
         "prt_film_read_splats": (C.c_int, [vp, vp]),
         "prt_film_read_rgba8": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp]),
+        "prt_film_size": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "prt_film_reset": (C.c_int, [vp]),
         "prt_film_destroy": (C.c_int, [vp]),
         "prt_workload_icosphere": (C.c_int, [C.c_int32, C.c_double, vp, C.c_double, C.c_uint64, vp,
@@ -662,10 +663,10 @@ class ProgressiveRaytracer:
         h = C.c_void_p()
         check(lib().prt_film_create(scene.handle, C.byref(params), C.byref(h)))
         self.handle = h
-        # the film's size: the params' width/height where set (the web server's override), else the camera's
-        w, hh = scene.desc.image_size
-        self.width = params.width if params.width > 0 else w
-        self.height = params.height if params.height > 0 else hh
+        # the film's own size (the params' width/height where set, else the camera's) sizes every read
+        fw, fh = C.c_int32(), C.c_int32()
+        check(lib().prt_film_size(h, C.byref(fw), C.byref(fh)))
+        self.width, self.height = fw.value, fh.value
 
     def close(self):
         if getattr(self, "handle", None):

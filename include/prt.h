@@ -273,8 +273,9 @@ int prt_traversal_counts(const prt_scene* scene, const prt_ray* rays, int64_t n,
 
 typedef struct prt_render_params {
     int32_t integrator;
-    int32_t width;                 /* SamplingConfig.Width (scene.go:22-30) */
-    int32_t height;                /* SamplingConfig.Height; <= 0: int(width / aspect) */
+    int32_t width;                 /* SamplingConfig.Width (scene.go:22-30); > 0 overrides the camera's */
+    int32_t height;                /* SamplingConfig.Height; > 0 overrides the camera's (the web server sets
+                                      both: camera width and aspect width/height, server.go:237-241) */
     int32_t max_depth;             /* SamplingConfig.MaxDepth */
     int32_t rr_min_bounces;        /* SamplingConfig.RussianRouletteMinBounces */
     int32_t tile_size;             /* ProgressiveConfig.TileSize (64) */
@@ -317,6 +318,10 @@ int prt_film_read_rgba8(const prt_film* film, int32_t x0, int32_t y0, int32_t x1
                         uint8_t* rgba);
 int prt_film_reset(prt_film* film);
 int prt_film_destroy(prt_film* film);
+/* Image size of a film (the params' width/height where set, else the camera's). No counterpart in the
+ * reference, whose callers hold the size in SamplingConfig: lets FFI callers size the buffers that
+ * prt_film_read and prt_film_read_rgba8 fill. */
+int prt_film_size(const prt_film* film, int32_t* width, int32_t* height);
 
 #ifdef __cplusplus
 }
