@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "prt_internal.h"
@@ -926,7 +927,9 @@ struct prt_film {
     hipEvent_t rev0[kRing] = {}, rev1[kRing] = {};
     int64_t enq = 0, seen = 0;
     int64_t chunk_paths = 0;  // paths per planned chunk (at most capacity)
-    // RGBA8 reads (prt_film_read_rgba8): device image and pinned host staging, allocated at first use
+    // RGBA8 reads (prt_film_read_rgba8): device image and pinned host staging, allocated at first use and
+    // shared by the film's readers under rgba_mu (a served film may be read from several threads)
+    std::mutex rgba_mu;
     uchar4* rgba_dev = nullptr;
     uchar4* rgba_host = nullptr;
     // chunk plan of the samples being rendered (see film_plan)
@@ -1651,6 +1654,7 @@ int prt_film_read_rgba8(const prt_film* f, int32_t x0, int32_t y0, int32_t x1, i
     const int64_t n = (int64_t)tw * th;
     if (n == 0) return PRT_OK;
     prt_film* m = const_cast<prt_film*>(f);
+    std::lock_guard<std::mutex> lock(m->rgba_mu);
     PRT_HIP(hipSetDevice(f->scene->device));
     if (!m->rgba_dev) {
         void* p = nullptr;

@@ -165,3 +165,30 @@ def test_multi_chunk_first_pass_matches_solo(sc, integrator, size):
     print(f"{w}x{h} integrator {integrator}: max relative difference {err.max():.3g}, "
           f"identical pixels {np.all(m == m2, axis=2).mean():.5f}, mean {m2.mean():.6f}")
     assert m2.mean() > 0 and err.max() <= 1e-5
+
+
+def test_concurrent_rgba8_reads_of_a_served_film(sc):
+    """prt_server_read_rgba8 from several threads at once (the device conversion's image and staging
+    buffers are shared by a film's readers): every read equals a single-threaded one."""
+    import threading
+
+    scene, cfg = sc["default"]
+    srv = P.Server(max_active=2)
+    rid = srv.submit(scene, _params(320, 180, cfg, 4, 2, 12))
+    _drain(srv, [rid])
+    ref = srv.read_rgba8(rid, 0, 0, 320, 180)
+    outs = [None] * 8
+
+    def work(k):
+        outs[k] = srv.read_rgba8(rid, 0, 0, 320, 180) if k % 2 == 0 else srv.read_rgba8(rid, 10, 20, 200, 150)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert ref[..., 3].min() == 255 and ref[..., :3].max() > 0
+    for k, o in enumerate(outs):
+        assert np.array_equal(o, ref if k % 2 == 0 else ref[20:150, 10:200]), k
+    srv.release(rid)
+    srv.close()
