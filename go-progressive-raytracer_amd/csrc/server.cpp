@@ -189,9 +189,11 @@ struct prt_server {
                 if (used + (int64_t)r->bytes > budget) break;  // wait for memory to free up
                 queue.pop_front();
                 int rc = prt::film_create_ex(r->scene, &r->params, serve_capacity(r->scene, &r->params), &r->film);
-                if (rc == PRT_OK) rc = prt::film_track_chunks(r->film);
-                if (rc == PRT_OK && r->snapshots) rc = alloc_snapshots(r);
-                used += (int64_t)r->bytes;  // free_film returns it
+                if (rc == PRT_OK) {
+                    used += (int64_t)r->bytes;  // free_film returns it with the film
+                    rc = prt::film_track_chunks(r->film);
+                    if (rc == PRT_OK && r->snapshots) rc = alloc_snapshots(r);
+                }
                 if (rc != PRT_OK) {
                     free_film(r);
                     r->state = St::Failed;
