@@ -347,6 +347,64 @@ __device__ __forceinline__ float vtx_pdf(const DevScene& sc, const DevCamera& ca
     return convert_density(sc, cur, next, pdf);
 }
 
+// A segment between two vertices as vtx_pdf uses it, computed once where several densities share it:
+// n = normalize(v) (the direction), u = v * sqrt(1 / |v|^2) and inv = 1 / |v|^2 (the area conversion).
+// The reversed segment negates n and u exactly.
+struct Seg {
+    V3 n, u;
+    float inv;
+    bool zero;
+};
+__device__ __forceinline__ Seg seg_of(V3 v) {
+    Seg g;
+    const float d2 = dot(v, v);
+    g.zero = d2 == 0.0f;
+    g.n = normalize(v);
+    g.inv = 1.0f / d2;
+    g.u = v * sqrtf(g.inv);
+    return g;
+}
+__device__ __forceinline__ Seg seg_rev(const Seg& g) {
+    Seg r = g;
+    r.n = g.n * -1.0f;
+    r.u = g.u * -1.0f;
+    return r;
+}
+// vtx_pdf(cur, prev, next) for a previous vertex, with the segments cur->prev (sp) and cur->next (sn).
+__device__ __forceinline__ float vtx_pdf_seg(const DevScene& sc, const DevCamera& cam, const BV& cur, const Seg& sp,
+                                             const BV& next, const Seg& sn) {
+    if (cur.flags & VF_LIGHT) {  // vtx_light_pdf
+        float pdf = 0.0f;
+        if (cur.flags & VF_INF) {
+            float R = sc.world_radius;
+            if (R == 0.0f) R = 1.0f;
+            pdf = 1.0f / (kPiF * R * R);
+        } else if (cur.light >= 0) {
+            float pos, dir;
+            light_pdf_le(sc, sc.lights[cur.light], cur.p, sn.u, pos, dir);
+            pdf = dir * sn.inv;
+        }
+        if (on_surface(sc, next)) pdf *= fabsf(dot(next.n, sn.u));
+        return pdf;
+    }
+    if (sn.zero || sp.zero) return 0.0f;
+    float pdf;
+    if (cur.flags & VF_CAMERA) {
+        pdf = cam_dir_pdf(cam, cur.p, sn.n);
+        if (pdf == 0.0f) return 0.0f;
+    } else if (cur.mat >= 0) {
+        bool delta;
+        pdf = mat_pdf(sc, cur.mat, sp.n, sn.n, cur.n, delta);
+        if (delta) return 0.0f;
+    } else {
+        return 0.0f;
+    }
+    if (next.flags & VF_INF) return pdf;  // convert_density
+    if (sn.zero) return 0.0f;
+    if (on_surface(sc, next)) pdf *= fabsf(dot(sn.u, next.n));
+    return pdf * sn.inv;
+}
+
 // The part of a subpath's MIS walk that no connection changes, once per subpath (after its densities
 // are final). With r_m = remap0(rev_m) / remap0(fwd_m) and conn_m the walk's rule for vertex m (neither m
 // nor the vertex before it specular or on a point light; the camera walk ends at vertex 1, the light walk
@@ -409,11 +467,13 @@ __device__ __forceinline__ float mis_weight(const DevScene& sc, const DevCamera&
         c_rev1 = vtx_pdf(sc, cam, sv, nullptr, ct1);
         c_rev2 = vtx_pdf(sc, cam, ct1, &sv, ct2);
         l_rev1 = vtx_pdf(sc, cam, ct1, &ct2, sv);
-    } else {
-        c_rev1 = vtx_pdf(sc, cam, ls1, &ls2, ct1);
-        c_rev2 = vtx_pdf(sc, cam, ct1, &ls1, ct2);
-        l_rev1 = vtx_pdf(sc, cam, ct1, &ct2, ls1);
-        l_rev2 = vtx_pdf(sc, cam, ls1, &ct1, ls2);
+    } else {  // the three segments the four densities share: ct1->ls1, ct1->ct2, ls1->ls2
+        const Seg a = seg_of(ls1.p - ct1.p), b = seg_of(ct2.p - ct1.p), c = seg_of(ls2.p - ls1.p);
+        const Seg ar = seg_rev(a);
+        c_rev1 = vtx_pdf_seg(sc, cam, ls1, c, ct1, ar);
+        c_rev2 = vtx_pdf_seg(sc, cam, ct1, a, ct2, b);
+        l_rev1 = vtx_pdf_seg(sc, cam, ct1, b, ls1, a);
+        l_rev2 = vtx_pdf_seg(sc, cam, ls1, ar, ls2, c);
     }
     // The walks of calculateMISWeight (bdpt_mis.go:30-50): from the connection outwards, ri is the running
     // product of remap0(rev) / remap0(fwd) and connectible vertices add ri. Only the two vertices next to
