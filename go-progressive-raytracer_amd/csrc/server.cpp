@@ -57,12 +57,14 @@ struct Req {
 // chunks of longer renders. A first pass (one sample per pixel) runs as one chunk over the whole
 // image where the film's path slots allow it (path tracing: up to 2^21 paths; BDPT: the subpath
 // stores' 2 GiB bound, at least one full grid of the BDPT kernels): each chunk ends in a latency-bound
-// tail of long paths, paid once instead of per chunk. Later passes run in chunks small enough that a
-// new request's first pass waits little behind them; a BDPT sample costs several path-tracing
-// samples, so its later chunks are smaller.
+// tail of long paths, paid once instead of per chunk. Later passes run in chunks large enough to
+// amortise that tail and small enough that a new request's first pass waits little behind them
+// (measured at 20 requests/s: 2^19 paths gave 284 M samples/s, 2^20 gave 381 M at the same
+// time-to-first-pass p99, 2^21 gave 405 M with the p99 up by a third); a BDPT sample costs several
+// path-tracing samples, so its later chunks are smaller.
 constexpr int64_t kFirstChunkMax = (int64_t)1 << 21;
-constexpr int64_t kLaterChunk = (int64_t)1 << 19;
-constexpr int64_t kLaterChunkBdpt = (int64_t)1 << 17;
+constexpr int64_t kLaterChunk = (int64_t)1 << 20;
+constexpr int64_t kLaterChunkBdpt = (int64_t)1 << 18;
 // Chunks of later passes in flight on the device at once (over all requests). A new request's first
 // pass waits for at most these, whichever hardware queue its stream lands on.
 constexpr int kBackgroundChunks = 8;
