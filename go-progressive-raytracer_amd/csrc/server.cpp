@@ -58,17 +58,18 @@ struct Req {
 // image where the film's path slots allow it (path tracing: up to 2^21 paths; BDPT: the subpath
 // stores' 2 GiB bound, at least one full grid of the BDPT kernels): each chunk ends in a latency-bound
 // tail of long paths, paid once instead of per chunk. Later passes run in chunks large enough to
-// amortise that tail and small enough that a new request's first pass waits little behind them
-// (measured at 20 requests/s: 2^19 paths gave 284 M samples/s, 2^20 gave 381 M at the same
-// time-to-first-pass p99, 2^21 gave 405 M with the p99 up by a third); a BDPT sample costs several
-// path-tracing samples, so its later chunks are smaller.
+// amortise that tail; a BDPT sample costs several path-tracing samples, so its later chunks hold
+// fewer paths. Few such chunks are in flight at once (kBackgroundChunks), so a new request's first
+// pass still waits little behind them. Measured at 20 requests/s (Poisson, mixed scenes, sizes and
+// integrators): later chunks of 2^19 / 2^17 paths with 8 in flight gave 284 M samples/s at
+// time-to-first-pass p50 8.4 ms, p99 26.8 ms; 2^20 / 2^18 with 6 in flight 380 M at 8.1 ms / 23-26 ms;
+// 2^21 / 2^18 with 4 in flight 396 M at 7.3 ms / 20.5 ms (with 8 in flight: 406 M at 11.8 / 35 ms).
 constexpr int64_t kFirstChunkMax = (int64_t)1 << 21;
-constexpr int64_t kLaterChunk = (int64_t)1 << 20;
+constexpr int64_t kLaterChunk = (int64_t)1 << 21;
 constexpr int64_t kLaterChunkBdpt = (int64_t)1 << 18;
 // Chunks of later passes in flight on the device at once (over all requests). A new request's first
-// pass waits for at most these, whichever hardware queue its stream lands on (measured at 20
-// requests/s: 6 gave the throughput of 8 and 12 with a lower time-to-first-pass, 4 cost 5 % of it).
-constexpr int kBackgroundChunks = 6;
+// pass waits for at most these, whichever hardware queue its stream lands on.
+constexpr int kBackgroundChunks = 4;
 // Chunks of a first pass in flight: one running, one queued behind it.
 constexpr int kFirstPassChunks = 2;
 
