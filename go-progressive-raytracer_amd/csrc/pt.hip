@@ -1352,6 +1352,7 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     size_t b = 0;
     if (bd) b = 2 * bdpt_store_bytes(p->max_depth, C, uv) + sizeof(uint32_t) * C + 3 * sizeof(long long) * npix;
     if (p->adaptive_min_samples < 1.0) b += 2 * sizeof(uint32_t) * npix + 6 * sizeof(uint32_t);
+    b += sizeof(uchar4) * (size_t)npix;  // the device image of RGBA8 reads, allocated at the first read
     return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + b + 3 * sizeof(uint32_t) * (size_t)C;
 }
 

@@ -175,15 +175,21 @@ struct prt_server {
         std::unique_lock<std::mutex> lk(mu);
         while (!stop) {
             bool progressed = false;
+            // ---- cancelled while queued: leave the queue now, whatever the slots and budget ----
+            for (auto it = queue.begin(); it != queue.end();) {
+                Req* r = *it;
+                if (!r->cancel) {
+                    ++it;
+                    continue;
+                }
+                it = queue.erase(it);
+                r->state = St::Cancelled;
+                emit(r, PRT_EVENT_CANCELLED, 0, 0, 0, 0);
+                progressed = true;
+            }
             // ---- admission (FIFO, HBM budget, slot count) ----
             while (!queue.empty() && (int)active.size() < max_active) {
                 Req* r = queue.front();
-                if (r->cancel) {
-                    queue.pop_front();
-                    r->state = St::Cancelled;
-                    emit(r, PRT_EVENT_CANCELLED, 0, 0, 0, 0);
-                    continue;
-                }
                 if ((int64_t)r->bytes > budget) {
                     queue.pop_front();
                     r->state = St::Failed;
@@ -216,6 +222,14 @@ struct prt_server {
                 const int was_in_flight = r->in_flight;
                 if (r->state != St::Running) {
                     remove = true;
+                } else if (r->cancel) {
+                    // no more chunks are dispatched; the request ends once its chunks on the device finish
+                    if (r->in_flight > 0) r->in_flight = prt::film_chunks_pending(r->film);
+                    if (r->in_flight == 0) {
+                        r->state = St::Cancelled;
+                        emit(r, PRT_EVENT_CANCELLED, r->completed_pass, r->target, 0, 0);
+                        remove = true;
+                    }
                 } else if (r->in_flight == 0 || (r->in_flight = prt::film_chunks_pending(r->film)) == 0) {
                     const double ms = prt::film_plan_ms(r->film);
                     // adaptive passes: the finished round decides the next one (film idle, quick read)
@@ -238,11 +252,7 @@ struct prt_server {
                         }
                         emit(r, PRT_EVENT_PASS_COMPLETE, r->pass, r->target, 0, ms);
                         progressed = true;
-                        if (r->cancel) {
-                            r->state = St::Cancelled;
-                            emit(r, PRT_EVENT_CANCELLED, r->pass, r->target, 0, 0);
-                            remove = true;
-                        } else if (r->pass >= r->params.max_passes ||
+                        if (r->pass >= r->params.max_passes ||
                                    prt::film_spp_done(r->film) >= r->params.max_samples_per_pixel) {
                             r->state = St::Done;
                             completed++;
@@ -270,7 +280,7 @@ struct prt_server {
             int background = 0;
             for (Req* r : active) {
                 if (r->state != St::Running) continue;
-                if (r->completed_pass == 0) first_owed = true;
+                if (r->completed_pass == 0 && !r->cancel) first_owed = true;
                 else background += r->in_flight;
             }
             auto enqueue = [&](Req* r) {
@@ -284,7 +294,7 @@ struct prt_server {
                 return true;
             };
             for (Req* r : active) {
-                if (r->state != St::Running || r->completed_pass != 0) continue;
+                if (r->state != St::Running || r->cancel || r->completed_pass != 0) continue;
                 while (r->in_flight < kFirstPassChunks && !prt::film_round_exhausted(r->film) && enqueue(r)) {
                 }
             }
@@ -295,7 +305,8 @@ struct prt_server {
                     for (size_t j = 0; j < n && background < kBackgroundChunks; j++) {
                         const size_t idx = (start + j) % n;
                         Req* r = active[idx];
-                        if (r->state != St::Running || r->in_flight > round || prt::film_round_exhausted(r->film))
+                        if (r->state != St::Running || r->cancel || r->in_flight > round ||
+                            prt::film_round_exhausted(r->film))
                             continue;
                         if (!enqueue(r)) continue;
                         background++;

@@ -2362,9 +2362,10 @@ void orc_intersect(const orc_scene* os, const prt_ray* rays, int64_t n, double* 
 // Path-traced radiance sums per pixel for samples [sample_offset, sample_offset + spp)
 // over the pixel rectangle [x0,x1)x[y0,y1) of the camera image. Output arrays are
 // (y1-y0)*(x1-x0) pixels: sum_rgb (x3), lum_sum, lum_sq_sum (PixelStats.AddSample, stats.go:41-47).
+// sq_rgb (optional, x3): per-channel sums of squares of the camera-side samples, for per-pixel variances.
 int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int32_t sample_offset,
                int32_t x0, int32_t y0, int32_t x1, int32_t y1, double* sum_rgb, double* lum_sum,
-               double* lum_sq, int32_t nthreads, double* splat_rgb) {
+               double* lum_sq, int32_t nthreads, double* splat_rgb, double* sq_rgb) {
     const Scene& S = os->s;
     const bool bdpt = p->integrator == PRT_INTEGRATOR_BDPT;
     if (!bdpt && p->integrator != PRT_INTEGRATOR_PATH) return PRT_ERR_UNSUPPORTED;
@@ -2391,7 +2392,7 @@ int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int
             for (int j = y0 + ty * ts; j < std::min(y1, y0 + (ty + 1) * ts); j++)
                 for (int i = x0 + tx * ts; i < std::min(x1, x0 + (tx + 1) * ts); i++) {
                     size_t o = (size_t)(j - y0) * tw + (i - x0);
-                    double sr = 0, sg = 0, sb = 0, ls = 0, lq = 0;
+                    double sr = 0, sg = 0, sb = 0, ls = 0, lq = 0, qr = 0, qg = 0, qb = 0;
                     for (int s = sample_offset; s < sample_offset + spp; s++) {
                         Sampler smp;
                         smp.key = path_key(p->seed, (uint32_t)(j * W + i), (uint32_t)s);
@@ -2419,6 +2420,9 @@ int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int
                         sr += c.x;
                         sg += c.y;
                         sb += c.z;
+                        qr += c.x * c.x;
+                        qg += c.y * c.y;
+                        qb += c.z * c.z;
                         double l = c.luminance();
                         ls += l;
                         lq += l * l;
@@ -2428,6 +2432,11 @@ int orc_render(const orc_scene* os, const prt_render_params* p, int32_t spp, int
                     sum_rgb[o * 3 + 2] = sb;
                     if (lum_sum) lum_sum[o] = ls;
                     if (lum_sq) lum_sq[o] = lq;
+                    if (sq_rgb) {
+                        sq_rgb[o * 3 + 0] = qr;
+                        sq_rgb[o * 3 + 1] = qg;
+                        sq_rgb[o * 3 + 2] = qb;
+                    }
                 }
         }
     };

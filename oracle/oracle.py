@@ -33,7 +33,7 @@ def lib():
             "orc_scene_destroy": (None, [vp]),
             "orc_scene_world": (None, [vp, vp, vp, vp]),
             "orc_intersect": (None, [vp, vp, i64, vp, vp, vp, vp, i32, i32, vp]),
-            "orc_render": (C.c_int, [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32, vp]),
+            "orc_render": (C.c_int, [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp]),
             "orc_render_samples": (C.c_int, [vp, vp, vp, vp, i64, vp, i32, vp]),
             "orc_render_passes": (C.c_int, [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp]),
             "orc_bdpt_mis": (d, [vp, vp, vp, i32, vp, i32, vp, i32, i32]),
@@ -130,8 +130,9 @@ class OracleScene:
             return t, prim, u, v, cnt
         return t, prim, u, v
 
-    def render(self, params, spp, sample_offset=0, rect=None, nthreads=0, splats=False):
-        """Per-pixel sums over samples [offset, offset+spp): (sum_rgb (H,W,3), lum_sum, lum_sq[, splat_rgb])."""
+    def render(self, params, spp, sample_offset=0, rect=None, nthreads=0, splats=False, squares=False):
+        """Per-pixel sums over samples [offset, offset+spp): (sum_rgb (H,W,3), lum_sum, lum_sq[, splat_rgb]
+        [, sq_rgb]); sq_rgb (squares=True) holds the per-channel sums of squared camera-side samples."""
         W, H = self.desc.image_size
         x0, y0, x1, y1 = rect if rect else (0, 0, W, H)
         h, w = y1 - y0, x1 - x0
@@ -139,12 +140,12 @@ class OracleScene:
         ls = np.zeros((h, w))
         lq = np.zeros((h, w))
         sp = np.zeros((H, W, 3)) if splats else None
+        sq = np.zeros((h, w, 3)) if squares else None
         rc = lib().orc_render(self.h, C.byref(params), spp, sample_offset, x0, y0, x1, y1, _p(s), _p(ls), _p(lq),
-                              nthreads, _p(sp) if splats else None)
+                              nthreads, _p(sp) if splats else None, _p(sq) if squares else None)
         assert rc == 0, rc
-        if splats:
-            return s, ls, lq, sp
-        return s, ls, lq
+        out = (s, ls, lq) + ((sp,) if splats else ()) + ((sq,) if squares else ())
+        return out
 
     def new_film(self):
         """Per-pixel state of orc_render_passes (sums, luminance moments, counts, splats)."""

@@ -1,0 +1,191 @@
+"""Converged reference images: 65,536-spp float64 oracle renders of the benchmark configurations.
+
+For each configuration this writes tests/golden/converged/<name>.npz holding the per-pixel mean
+(float32, H x W x 3) and the per-pixel variance of ONE sample (float32, H x W x 3), and records in
+tests/golden/converged.json the scene, integrator, sample count, seed, pixel rectangle, the file's
+SHA-256 and a calibration of the per-pixel 3-sigma test (below).
+
+The oracle (oracle/liboracle.so) is the float64 restatement of the reference path
+(pkg/integrator/path_tracing.go, bdpt.go, bdpt_mis.go); the fixtures follow the reference's
+converged-image method (pkg/renderer/progressive_integration_test.go:29-552: high-spp reference,
+per-pixel statistics) with PixelStats' moments (pkg/renderer/stats.go:20-55).
+
+Variances:
+* path tracing: from the per-channel sums of squares of all 65,536 camera samples;
+* BDPT: a pixel's estimate also holds light-tracing splats from every other pixel's samples, so the
+  variance of one sample's worth is measured from batch means: nb batches of B samples per pixel, each
+  batch image = (camera sums + splats) / B, variance of one sample = B * var(batch images).
+
+Calibration: the 3-sigma test the GPU tests apply (z = (image - fixture mean) / sqrt(var * (1/n +
+1/N))) is applied to independent oracle renders at the configuration's sample count (4 seeds). A
+correct renderer's outlier fraction is therefore known per configuration: heavy-tailed pixels
+(caustics) exceed the normal tail's 0.27 % at low sample counts, and the tests allow the larger of the
+two.
+
+Run (takes hours on 8 cores; resumable, one checkpoint per batch):
+    python tests/golden/make_converged.py [--only NAME] [--threads N] [--work DIR]
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "go-progressive-raytracer_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+OUT_DIR = os.path.join(HERE, "converged")
+MANIFEST = os.path.join(HERE, "converged.json")
+FIXTURE_SPP = 65536
+FIXTURE_SEED = 0x5EED0000
+CALIB_SEEDS = (0x5EED1001, 0x5EED1002, 0x5EED1003, 0x5EED1004)
+
+# name: scene constructor (in scenes.py) and its arguments, integrator, pixel rectangle (x0, y0, x1, y1)
+# of the camera image or None for all of it, samples per batch, the configuration's sample count
+CONFIGS = {
+    "c0_default_400x225": dict(scene="default_scene", args={}, integrator="path", rect=None, batch=1024,
+                               config_spp=64),
+    "c2_cornell_metre_128": dict(scene="cornell_metre", args={"width": 128}, integrator="path", rect=None,
+                                 batch=1024, config_spp=1024),
+    "c3_mesh_sky_crop128": dict(scene="mesh_sky_scene", args={}, integrator="path", rect=(448, 704, 576, 832),
+                                batch=1024, config_spp=256),
+    "c2_bdpt_cornell_metre_128": dict(scene="cornell_metre", args={"width": 128}, integrator="bdpt", rect=None,
+                                      batch=256, config_spp=64),
+}
+
+
+def make_scene(cfg):
+    import scenes
+
+    return getattr(scenes, cfg["scene"])(**cfg["args"])
+
+
+def params(P, desc, scfg, cfg, spp, seed):
+    w, h = desc.image_size
+    integ = P.INTEGRATOR_BDPT if cfg["integrator"] == "bdpt" else P.INTEGRATOR_PATH
+    return P.sampling_params(w, h, scfg["max_depth"], scfg["rr_min_bounces"], spp, max_passes=1, seed=seed,
+                             integrator=integ)
+
+
+def render_batch(O, os_, p, cfg, spp, offset, threads, rect):
+    """Sums over samples [offset, offset + spp) of the rectangle: (camera sums, per-channel squares,
+    splats cropped to the rectangle or None)."""
+    bdpt = cfg["integrator"] == "bdpt"
+    out = os_.render(p, spp, offset, rect=rect, nthreads=threads, splats=bdpt, squares=True)
+    s, sq = out[0], out[-1]
+    sp = None
+    if bdpt:
+        x0, y0, x1, y1 = rect
+        sp = out[3][y0:y1, x0:x1]
+    return s, sq, sp
+
+
+def converge(name, cfg, threads, work):
+    import oracle as O
+    import prtgpu as P
+
+    desc, scfg = make_scene(cfg)
+    W, H = desc.image_size
+    rect = cfg["rect"] or (0, 0, W, H)
+    h, w = rect[3] - rect[1], rect[2] - rect[0]
+    p = params(P, desc, scfg, cfg, FIXTURE_SPP, FIXTURE_SEED)
+    os_ = O.OracleScene(desc)
+    ck = os.path.join(work, name + ".ckpt.npz")
+    B = cfg["batch"]
+    nb = FIXTURE_SPP // B
+    st = {"done": 0, "sum": np.zeros((h, w, 3)), "sq": np.zeros((h, w, 3)), "bsum": np.zeros((h, w, 3)),
+          "bsq": np.zeros((h, w, 3)), "seconds": 0.0}
+    if os.path.exists(ck):
+        z = np.load(ck)
+        st = {k: (z[k] if z[k].ndim else z[k].item()) for k in z.files}
+        st["done"] = int(st["done"])
+    while st["done"] < nb:
+        t0 = time.time()
+        s, sq, sp = render_batch(O, os_, p, cfg, B, st["done"] * B, threads, rect)
+        st["sum"] += s + (sp if sp is not None else 0)
+        st["sq"] += sq
+        img = (s + (sp if sp is not None else 0)) / B
+        st["bsum"] += img
+        st["bsq"] += img * img
+        st["done"] += 1
+        st["seconds"] += time.time() - t0
+        tmp = ck + ".tmp.npz"
+        np.savez(tmp, **st)
+        os.replace(tmp, ck)
+        print(f"[{name}] batch {st['done']}/{nb} ({time.time() - t0:.1f}s)", flush=True)
+    n = FIXTURE_SPP
+    mean = st["sum"] / n
+    if cfg["integrator"] == "bdpt":
+        bm = st["bsum"] / nb
+        var = B * np.maximum(st["bsq"] / nb - bm * bm, 0.0) * nb / (nb - 1)
+    else:
+        var = np.maximum(st["sq"] - st["sum"] * st["sum"] / n, 0.0) / (n - 1)
+    return desc, scfg, rect, mean, var, float(st["seconds"])
+
+
+def zscores(image, mean, var, n_image, n_fixture):
+    """Per-pixel, per-channel z of an n_image-sample estimate against the fixture; a relative floor of
+    1e-4 |mean| (plus 1e-12) keeps noise-free pixels (constant radiance) from dividing by zero."""
+    sd = np.sqrt(var * (1.0 / n_image + 1.0 / n_fixture) + (1e-4 * np.abs(mean)) ** 2 + 1e-12)
+    return (image - mean) / sd
+
+
+def calibrate(cfg, desc, scfg, rect, mean, var, threads):
+    import oracle as O
+    import prtgpu as P
+
+    os_ = O.OracleScene(desc)
+    n = cfg["config_spp"]
+    out = []
+    for seed in CALIB_SEEDS:
+        p = params(P, desc, scfg, cfg, n, seed)
+        s, _, sp = render_batch(O, os_, p, cfg, n, 0, threads, rect)
+        img = (s + (sp if sp is not None else 0)) / n
+        z = zscores(img, mean, var, n, FIXTURE_SPP)
+        out.append({"seed": seed, "frac_abs_z_gt_3": float((np.abs(z) > 3).mean()), "mean_z": float(z.mean()),
+                    "std_z": float(z.std())})
+    return out
+
+
+def sha256(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        h.update(f.read())
+    return h.hexdigest()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", default=None)
+    ap.add_argument("--threads", type=int, default=len(os.sched_getaffinity(0)))
+    ap.add_argument("--work", default="/tmp/prt_converged")
+    a = ap.parse_args()
+    os.makedirs(OUT_DIR, exist_ok=True)
+    os.makedirs(a.work, exist_ok=True)
+    man = json.load(open(MANIFEST)) if os.path.exists(MANIFEST) else {}
+    for name, cfg in CONFIGS.items():
+        if a.only and name != a.only:
+            continue
+        desc, scfg, rect, mean, var, secs = converge(name, cfg, a.threads, a.work)
+        path = os.path.join(OUT_DIR, name + ".npz")
+        np.savez_compressed(path, mean=mean.astype(np.float32), var=var.astype(np.float32))
+        calib = calibrate(cfg, desc, scfg, rect, mean, var, a.threads)
+        W, H = desc.image_size
+        man[name] = {"scene": cfg["scene"], "scene_args": cfg["args"], "integrator": cfg["integrator"],
+                     "image_size": [W, H], "rect": list(rect), "fixture_spp": FIXTURE_SPP, "fixture_seed": FIXTURE_SEED,
+                     "max_depth": scfg["max_depth"], "rr_min_bounces": scfg["rr_min_bounces"],
+                     "config_spp": cfg["config_spp"], "variance": "per-sample moments" if cfg["integrator"] == "path"
+                     else f"batch means, {FIXTURE_SPP // cfg['batch']} batches of {cfg['batch']}",
+                     "oracle_seconds": round(secs, 1), "threads": a.threads, "sha256": sha256(path),
+                     "calibration": calib}
+        json.dump(man, open(MANIFEST, "w"), indent=1, sort_keys=True)
+        print(f"[{name}] written: {man[name]}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

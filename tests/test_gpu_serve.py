@@ -120,6 +120,64 @@ def test_cancel_and_budget(sc):
     srv.close()
 
 
+@pytest.mark.parametrize("limit", ["slots", "budget"])
+def test_cancel_while_queued_behind_active_render(limit):
+    """Requests cancelled (or released) while they wait behind a full slot limit or a full HBM budget
+    leave the queue at once: CANCELLED arrives while the active render is still running, and the scene
+    they held can be destroyed once everything is released. Cancelling the running render itself stops
+    its chunk dispatch: it ends long before its pass would."""
+    import time
+
+    d, cfg = scenes.cornell_metre(width=64)
+    scene = P.Scene(d)
+    long_p = _params(256, 256, cfg, 1 << 20, 2, 21)  # pass 2 = 2^20 spp: minutes of device time
+    if limit == "slots":
+        srv = P.Server(max_active=1)
+    else:  # a budget of 1.5 film slots of this size: the first request fits, the next ones wait
+        probe = P.Server(max_active=1)
+        r = probe.submit(scene, _params(256, 256, cfg, 1, 1, 20))
+        _drain(probe, [r])
+        one = probe.stats().used_bytes
+        probe.release(r)
+        probe.close()
+        assert one > 0
+        srv = P.Server(hbm_budget_bytes=int(1.5 * one), max_active=8)
+    a = srv.submit(scene, long_p)
+    while True:  # wait until the long render is admitted and past its first pass
+        evs = [e for e in srv.poll(timeout_ms=50) if e.request_id == a]
+        if any(e.type == P.EVENT_PASS_COMPLETE for e in evs):
+            break
+    b = srv.submit(scene, _params(256, 256, cfg, 4, 1, 22))
+    c = srv.submit(scene, _params(256, 256, cfg, 4, 1, 23))
+    time.sleep(0.05)
+    assert srv.stats().queued == 2
+    srv.cancel(b)
+    srv.release(c)
+    got, t0 = {}, time.time()
+    while len(got) < 2:
+        for e in srv.poll(timeout_ms=50):
+            assert e.request_id != a or e.type == P.EVENT_ADMITTED, "the long render ended first"
+            if e.request_id in (b, c):
+                got[e.request_id] = e.type
+        assert time.time() - t0 < 10, "queued requests were not cancelled while the long render ran"
+    assert got == {b: P.EVENT_CANCELLED, c: P.EVENT_CANCELLED}
+    srv.release(b)
+    t1 = time.time()
+    srv.cancel(a)
+    ev = _drain(srv, [a], timeout_s=20)
+    assert ev[a][-1][0] == P.EVENT_CANCELLED and time.time() - t1 < 5
+    srv.release(a)
+    deadline = time.time() + 5
+    while True:  # releases are deferred to the scheduler thread
+        try:
+            scene.close()
+            break
+        except P.PrtError:
+            assert time.time() < deadline, "scene still held after every request was released"
+            time.sleep(0.01)
+    srv.close()
+
+
 def test_scene_in_use_cannot_be_destroyed():
     """A scene referenced by a film or an unreleased request refuses prt_scene_destroy (no dangling
     device pointers); the Python wrappers keep their scenes alive for as long as the C side needs them."""
