@@ -434,6 +434,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
     X(3, 1, 12, 1, 0, 256)         \
     X(4, 1, 6, 1, 64, 512)
 constexpr int kNumVariants = 5;
+constexpr int kVariantLds[kNumVariants] = {
+#define X(id, kind, lds, refill, nc, blk) lds,
+    PRT_TRACE_VARIANTS(X)
+#undef X
+};
 constexpr int kMinLds = 6;
 constexpr int kDefaultVariant = 2;
 constexpr int kVariantBlock[kNumVariants] = {256, 256, 256, 256, 512};
@@ -607,6 +612,36 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
     }
     return PRT_OK;
 }
+
+// ---- allocation-free closest hit (prt_intersect_closest_static) ------------------------------------
+// The work counter and the traversal overflow stack of the intersection kernels live in one static
+// device array instead of scene memory: 256 B for the counter, then the stack, entry e of thread slot
+// g at [e * slots + g] as in scene memory. Each device holds its own copy (resolved once per device
+// with hipGetSymbolAddress); calls on one device share it, so calls that overlap on one device race.
+constexpr size_t kStaticScratchBytes = (size_t)PRT_STATIC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mu;
+void* g_static_ptr[kMaxDevices] = {};
+
+__global__ void k_clear_counter(unsigned long long* c) { *c = 0ull; }
+
+// Device address of this device's `scratch` (the device must be current).
+int static_scratch(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(PRT_ERR_INVALID_ARGUMENT, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (!g_static_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress");
+        g_static_ptr[device] = p;
+    }
+    *out = g_static_ptr[device];
+    return PRT_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -1241,6 +1276,42 @@ int prt_intersect_closest(const prt_scene* scene, const prt_ray* rays, int64_t n
 int prt_intersect_any(const prt_scene* scene, const prt_ray* rays, int64_t n, uint8_t* occluded, uint32_t flags,
                       void* stream) {
     return launch_intersect(scene, 1, rays, n, occluded, nullptr, nullptr, flags, stream);
+}
+
+int prt_intersect_closest_static(const prt_scene* scene, const prt_ray* rays, int64_t n, prt_hit* hits,
+                                 void* stream) {
+    if (!scene) return fail(PRT_ERR_INVALID_ARGUMENT, "scene is NULL");
+    if (n < 0) return fail(PRT_ERR_INVALID_ARGUMENT, "negative ray count");
+    if (n == 0) return PRT_OK;
+    if (!rays || !hits) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL buffer");
+    PRT_HIP(hipSetDevice(scene->device));
+    void* base = nullptr;
+    int rc = static_scratch(scene->device, &base);
+    if (rc != PRT_OK) return rc;
+    // the variant the autotuner picked for this scene (no tuning here: it times launches), else the default
+    const int v = scene->forced_variant >= 0 ? scene->forced_variant
+                                             : (scene->trace_variant[0] >= 0 ? scene->trace_variant[0] : kDefaultVariant);
+    const int block = kVariantBlock[v];
+    const size_t entries = (size_t)std::max(1, scene->max_stack - kVariantLds[v] + 1);
+    const size_t per_block = entries * (size_t)block * sizeof(uint2);
+    const int64_t fit = (int64_t)((kStaticScratchBytes - 256) / per_block);
+    if (fit < 1)
+        return fail(PRT_ERR_UNSUPPORTED, "the scene's traversal stack bound (" + std::to_string(scene->max_stack) +
+                                             " entries) needs more than the static scratch of " +
+                                             std::to_string(kStaticScratchBytes) + " bytes");
+    // the full occupancy grid where the stack fits, else as many work-groups as it holds
+    const int blocks = (int)std::min<int64_t>(scene->variant_blocks[v], fit);
+    unsigned long long* work = (unsigned long long*)base;
+    uint2* gs = (uint2*)((char*)base + 256);
+    const uint32_t slots = (uint32_t)blocks * (uint32_t)block;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(1), 0, st, work);
+    if (scene->dev.mixed)
+        launch_variant<false, true>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work);
+    else
+        launch_variant<false, false>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work);
+    PRT_HIP(hipGetLastError());
+    return PRT_OK;
 }
 
 int prt_traversal_counts(const prt_scene* scene, const prt_ray* rays, int64_t n, uint32_t* node_visits,

@@ -144,6 +144,7 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<i4"), ("u", "<f4"), ("v", "<f4
 EXPORTS = [
     "prt_last_error", "prt_version", "prt_device_count", "prt_scene_create", "prt_scene_info_get",
     "prt_scene_destroy", "prt_intersect_closest", "prt_intersect_any", "prt_traversal_counts",
+    "prt_intersect_closest_static",
     "prt_samples_for_pass", "prt_film_create", "prt_render_pass", "prt_render_samples", "prt_film_read",
     "prt_film_read_splats", "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy", "prt_film_size",
     "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky", "prt_ply_load", "prt_ply_free",
@@ -202,6 +203,7 @@ def lib():
         "prt_intersect_closest": (C.c_int, [vp, vp, C.c_int64, vp, C.c_uint32, vp]),
         "prt_intersect_any": (C.c_int, [vp, vp, C.c_int64, vp, C.c_uint32, vp]),
         "prt_traversal_counts": (C.c_int, [vp, vp, C.c_int64, vp, vp, vp, C.c_uint32, vp]),
+        "prt_intersect_closest_static": (C.c_int, [vp, vp, C.c_int64, vp, vp]),
         "prt_samples_for_pass": (C.c_int, [C.POINTER(RenderParams), C.c_int32, C.POINTER(C.c_int32)]),
         "prt_film_create": (C.c_int, [vp, C.POINTER(RenderParams), C.POINTER(vp)]),
         "prt_render_pass": (C.c_int, [vp, C.c_int32, C.POINTER(PassStats)]),
@@ -638,6 +640,11 @@ class Scene:
     def intersect_closest_device(self, rays_ptr, n, hits_ptr, stream=None):
         check(lib().prt_intersect_closest(self.handle, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr),
                                           BUFFERS_DEVICE, C.c_void_p(stream) if stream else None))
+
+    def intersect_closest_static(self, rays_ptr, n, hits_ptr, stream=None):
+        """Allocation-free closest hit on device buffers (enqueue only; the caller synchronizes)."""
+        check(lib().prt_intersect_closest_static(self.handle, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr),
+                                                 C.c_void_p(stream) if stream else None))
 
     def intersect_any(self, rays):
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)

@@ -267,6 +267,20 @@ int prt_traversal_counts(const prt_scene* scene, const prt_ray* rays, int64_t n,
                          uint32_t* node_visits, uint32_t* box_tests, uint32_t* prim_tests,
                          uint32_t flags, void* stream);
 
+/* Allocation-free closest hit. No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same hits as prt_intersect_closest with PRT_BUFFERS_DEVICE; rays and hits are device pointers on
+ * the scene's device. The work counter and traversal overflow stack live in a static per-device
+ * array of PRT_STATIC_SCRATCH_BYTES instead of scene memory (the kernel grid shrinks if the scene's
+ * stack bound needs more than that; PRT_ERR_UNSUPPORTED if one work-group's stack does not fit).
+ * The call only enqueues work on `stream` (NULL: default stream) and returns: it calls no hipMalloc,
+ * hipFree, memset or synchronization and uses the variant the autotuner picked for the scene, if any.
+ * Every call on one device shares that device's array: calls that overlap on one device race (order
+ * them on one stream). */
+#define PRT_STATIC_SCRATCH_BYTES (64u << 20)
+int prt_intersect_closest_static(const prt_scene* scene, const prt_ray* rays, int64_t n, prt_hit* hits,
+                                 void* stream);
+
 /* ---- progressive rendering (render level) -------------------------------- */
 #define PRT_INTEGRATOR_PATH 0 /* pkg/integrator/path_tracing.go */
 #define PRT_INTEGRATOR_BDPT 1 /* pkg/integrator/bdpt.go */
