@@ -2624,6 +2624,27 @@ void orc_camera_get_ray(const orc_scene* os, int32_t i, int32_t j, const double*
     d[0] = r.d.x; d[1] = r.d.y; d[2] = r.d.z;
 }
 int32_t orc_camera_height(const orc_scene* os) { return os->s.cam.height; }
+// NewCamera's derived quantities (camera.go:70-170): forward3, lens area, image-plane area at z = 1,
+// cosTotalWidth, viewport width and height, focus distance, width, height.
+void orc_camera_info(const orc_scene* os, double* out) {
+    const Camera& c = os->s.cam;
+    out[0] = c.forward.x; out[1] = c.forward.y; out[2] = c.forward.z;
+    out[3] = c.lens_area; out[4] = c.plane_area; out[5] = c.cos_total_width;
+    out[6] = c.vw; out[7] = c.vh; out[8] = c.focus; out[9] = c.width; out[10] = c.height;
+}
+// SampleCameraFromPoint (camera.go:226-262): out = ray origin3, direction3, weight (We), pdf; returns 1
+// when the sample contributes (the reference returns nil otherwise).
+int32_t orc_camera_sample_from_point(const orc_scene* os, const double* ref, const double* s, double* out) {
+    CamSample cs = cam_sample_from_point(os->s.cam, V(ref), Vec2{s[0], s[1]});
+    out[0] = cs.ray.o.x; out[1] = cs.ray.o.y; out[2] = cs.ray.o.z;
+    out[3] = cs.ray.d.x; out[4] = cs.ray.d.y; out[5] = cs.ray.d.z;
+    out[6] = cs.weight; out[7] = cs.pdf;
+    return cs.ok ? 1 : 0;
+}
+// EvaluateRayImportance (camera.go:264-279): We of a ray (one channel; the reference's three are equal).
+double orc_camera_importance(const orc_scene* os, const double* o, const double* d) {
+    return cam_importance(os->s.cam, Ray{V(o), V(d)});
+}
 // Light queries on light `li` of the scene: Sample (out: dir3, dist, emission3, pdf), PDF.
 void orc_light_sample(const orc_scene* os, int32_t li, const double* p, const double* n, const double* s,
                       double* out) {

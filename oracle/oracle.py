@@ -55,6 +55,9 @@ def lib():
             "orc_aabb_hit": (i32, [vp, vp, vp, d, d]),
             "orc_camera_get_ray": (None, [vp, i32, i32, vp, vp, vp, vp]),
             "orc_camera_height": (i32, [vp]),
+            "orc_camera_info": (None, [vp, vp]),
+            "orc_camera_sample_from_point": (i32, [vp, vp, vp, vp]),
+            "orc_camera_importance": (d, [vp, vp, vp]),
             "orc_light_sample": (None, [vp, i32, vp, vp, vp, vp]),
             "orc_light_pdf": (d, [vp, i32, vp, vp, vp]),
             "orc_material_scatter": (i32, [vp, i32, vp, vp, vp, vp, i32, u32, vp]),
