@@ -40,7 +40,8 @@ def _drain(server, ids, timeout_s=60, release=False):
 def sc():
     d0, c0 = scenes.default_scene(width=96)
     d2, c2 = scenes.cornell_metre(width=64)
-    return {"default": (P.Scene(d0), c0), "cornell": (P.Scene(d2), c2)}
+    d3, c3 = scenes.mesh_sky_scene(width=96)  # the benchmark's 1.8M-triangle glass icosphere + HDR sky
+    return {"default": (P.Scene(d0), c0), "cornell": (P.Scene(d2), c2), "mesh_sky": (P.Scene(d3), c3)}
 
 
 BD = P.INTEGRATOR_BDPT
@@ -53,7 +54,8 @@ def _requests():
             ("default", 128, 72, 16, 3, 3, PT, 1.0), ("cornell", 80, 48, 20, 4, 4, PT, 1.0),
             ("default", 64, 36, 12, 2, 5, PT, 1.0), ("cornell", 48, 48, 30, 6, 6, PT, 1.0),
             ("cornell", 48, 48, 12, 3, 7, BD, 1.0), ("default", 64, 36, 8, 2, 8, BD, 1.0),
-            ("default", 64, 36, 32, 3, 9, PT, 0.15), ("cornell", 40, 40, 16, 2, 10, BD, 0.25)]
+            ("default", 64, 36, 32, 3, 9, PT, 0.15), ("cornell", 40, 40, 16, 2, 10, BD, 0.25),
+            ("mesh_sky", 96, 96, 12, 3, 11, PT, 1.0), ("mesh_sky", 64, 64, 6, 2, 12, BD, 1.0)]
 
 
 def test_isolation_alone_vs_mix(sc):
