@@ -27,7 +27,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--frequency", type=int, default=256)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-repeats", type=int, default=5,
+                    help="repeats of each CPU baseline sample (trimmed mean: min and max dropped)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="skip the path-tracing throughput section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (time-to-first-pass) section")
@@ -117,26 +118,36 @@ def main():
     traffic = profiled_traffic(variant, a.rays)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "achieved_basis": "algorithmic bytes (32 B ray + 16 B hit + 32 B per box test + 36 B per "
+                                  "triangle test) per launch / kernel time; most node reads hit in L2",
                 "bytes_per_ray": round(bytes_per_ray, 1), "box_tests_per_ray": round(float(boxes.mean()), 2),
                 "tri_tests_per_ray": round(float(prims.mean()), 2),
                 "kernel_ms": round(avg_ms, 3)}
+    if traffic:
+        hbm = traffic / (avg_ms * 1e-3) / 1e9
+        roofline.update({"traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel variant at this "
+                                           "ray count, calibrated against a copy kernel (profiles/)",
+                         "hbm_measured_GBps": round(hbm, 1), "hbm_measured_frac": round(hbm / HBM_PEAK_GBS, 4),
+                         "limiter": "latency: dependent node fetches per ray (6 waves/SIMD at 80 VGPRs); "
+                                    f"measured HBM traffic runs at {hbm / 1e3:.2f} TB/s"})
 
     cpu = None
     if not a.no_cpu:
         import oracle as O
 
-        cores = min(os.cpu_count() or 1, 16)
+        host = host_cpu()
+        cores = host["threads"]
         os_ = O.OracleScene(desc)
-        done, t_c = 0, 0.0
-        chunk = 1 << 20
-        while done < a.rays and t_c < a.cpu_seconds:
-            c0 = time.perf_counter()
-            os_.intersect(rays[done:done + chunk], nthreads=cores)
-            t_c += time.perf_counter() - c0
-            done += min(chunk, a.rays - done)
-        cpu = {"value": round(done / t_c / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-               "sample": f"first {done} of the {a.rays} rays through the float64 oracle (reference midpoint BVH, "
-                         f"leaf 8, Moller-Trumbore), {cores} threads, {t_c:.1f}s"}
+        reps = [timed(lambda: os_.intersect(rays, nthreads=cores)) for _ in range(a.cpu_repeats)]
+        n1 = min(a.rays, 1 << 20)
+        t1 = timed(lambda: os_.intersect(rays[:n1], nthreads=1))
+        tm = trimmed_mean(reps)
+        cpu = {"value": round(a.rays / tm / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+               "sample": f"all {a.rays} rays through the float64 oracle (reference midpoint BVH, leaf 8, "
+                         f"Moller-Trumbore) on {cores} threads, trimmed mean of {len(reps)} runs "
+                         f"({', '.join(f'{t:.2f}' for t in reps)} s)",
+               "one_thread_value": round(n1 / t1 / 1e6, 3), "one_thread_sample": f"first {n1} rays, {t1:.2f}s",
+               **{k: v for k, v in host.items() if k != "threads"}}
 
     secondary = {}
     if not a.no_pt:
@@ -169,6 +180,41 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def host_cpu():
+    """The host the CPU baselines run on: all threads this process may use (affinity mask, bounded by a
+    cgroup CPU quota if one is set), the CPU model and the machine's logical CPU count."""
+    threads = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(per)
+            threads = max(1, min(threads, int(quota)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"threads": threads, "cpu_model": model, "nproc": os.cpu_count(), "cgroup_cpu_quota": quota}
+
+
+def timed(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
+def trimmed_mean(xs):
+    """Mean without the fastest and slowest run (benchmark.sh:158-175 drops min and max)."""
+    xs = sorted(xs)
+    return sum(xs[1:-1]) / (len(xs) - 2) if len(xs) > 2 else sum(xs) / len(xs)
+
+
 def profiled_traffic(variant, rays):
     """HBM bytes per launch of this traversal variant at this ray count, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy kernel); None if none was profiled."""
@@ -185,30 +231,62 @@ def profiled_traffic(variant, rays):
     return None
 
 
+def load_fixture(name):
+    """Converged 65,536-spp oracle image of a configuration (tests/golden/make_converged.py): per-pixel mean
+    and one-sample variance over its pixel rectangle, or None if it has not been generated."""
+    import hashlib
+
+    import numpy as np
+
+    gold = os.path.join(ROOT, "tests", "golden")
+    try:
+        man = json.load(open(os.path.join(gold, "converged.json")))[name]
+        path = os.path.join(gold, "converged", name + ".npz")
+        if hashlib.sha256(open(path, "rb").read()).hexdigest() != man["sha256"]:
+            return None
+        z = np.load(path)
+        return {"mean": z["mean"].astype(np.float64), "var": z["var"].astype(np.float64), **man}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def rmse(img, fx):
+    import numpy as np
+
+    return float(np.sqrt(np.mean((np.asarray(img, np.float64) - fx["mean"]) ** 2)))
+
+
 def pt_section(a, log):
-    """Path-tracing samples/s on the three render configurations, each set next to the oracle
-    (the float64 restatement of the reference estimator) timed on a bounded crop with all host
-    threads. Both draw the same counter-RNG stream per (pixel, sample), so at equal samples per
-    pixel they have the same expected RMSE: samples/s is the equal-RMSE rate."""
+    """Path tracing and BDPT on the render configurations: GPU samples/s at full size; per-kernel device
+    time and (path tracing) traversal counts of the trace kernel from prt_film_probe, giving rays/s and
+    the algorithmic-byte fraction of the HBM roofline; time to the first pass; and, against the
+    configuration's converged 65,536-spp oracle image, the measured RMSE of the GPU image and of the
+    oracle's CPU image (the float64 restatement of the reference estimator, timed on all host threads),
+    from which the speedup at equal RMSE follows:
+        CPU spp needed for the GPU's RMSE  n_c' = n_c (RMSE_c^2 - e^2) / (RMSE_g^2 - e^2),
+        e^2 = the fixture's own error (mean variance / 65,536),
+        speedup = (n_c' x pixels / CPU samples/s) / (n_g x pixels / GPU samples/s)."""
     import numpy as np
 
     import prtgpu as P
     import scenes
 
     PT, BD = P.INTEGRATOR_PATH, P.INTEGRATOR_BDPT
-    # (name, scene, integrator, spp override, oracle spp of the bounded CPU sample)
+    # (tag, name, scene, integrator, spp override, fixture, CPU samples per pixel of one timed repeat)
     configs = [
-        ("C0 default scene 400x225, 64 spp, depth 50 (pinhole)", lambda: scenes.default_scene(), PT, None, 256),
-        ("C2 metre Cornell boxes 512x512, 1024 spp, depth 8", lambda: scenes.cornell_metre(), PT, None, 64),
-        ("C3 1.8M-triangle glass icosphere + HDR sky 1024x1024, 256 spp, depth 16",
-         lambda: scenes.mesh_sky_scene(), PT, None, 8),
-        ("C2-BDPT metre Cornell boxes 512x512, bidirectional, 64 spp, depth 8", lambda: scenes.cornell_metre(), BD, 64,
-         16),
+        ("c0", "C0 default scene 400x225, 64 spp, depth 50 (pinhole)", lambda **k: scenes.default_scene(**k), PT,
+         None, "c0_default_400x225", 64),
+        ("c2", "C2 metre Cornell boxes 512x512, 1024 spp, depth 8", lambda **k: scenes.cornell_metre(**k), PT, None,
+         "c2_cornell_metre_128", 256),
+        ("c3", "C3 1.8M-triangle glass icosphere + HDR sky 1024x1024, 256 spp, depth 16",
+         lambda **k: scenes.mesh_sky_scene(**k), PT, None, "c3_mesh_sky_crop128", 128),
+        ("c2bdpt", "C2-BDPT metre Cornell boxes 512x512, bidirectional, 64 spp, depth 8",
+         lambda **k: scenes.cornell_metre(**k), BD, 64, "c2_bdpt_cornell_metre_128", 64),
     ]
     out = []
-    cores = min(os.cpu_count() or 1, 16)
-    tags = ["c0", "c2", "c3", "c2bdpt"]
-    for tag, (name, make, integ, spp_override, c_spp) in zip(tags, configs):
+    host = host_cpu()
+    cores = host["threads"]
+    for tag, name, make, integ, spp_override, fx_name, c_spp in configs:
         if a.pt_config and tag != a.pt_config:
             continue
         desc, cfg = make()
@@ -224,32 +302,111 @@ def pt_section(a, log):
         st = pr.render_samples(spp)
         wall = time.perf_counter() - t0
         gpu_sps = w * h * spp / wall
+        img = pr.read()[0]
+        # per-kernel device time, then traversal counts, of the same render (renders are deterministic)
+        pr.reset()
+        pr.probe(P.PROBE_TIME)
+        pr.render_samples(spp)
+        kt = pr.probe(P.PROBE_COUNT if integ == PT else 0)
+        kernels = {k: round(v, 2) for k, v in kt.items() if k.endswith("_ms") and v > 0}
+        entry = {"config": name, "samples_per_s": round(gpu_sps, 1), "render_s": round(wall, 3),
+                 "device_ms": round(st.device_ms, 2), "kernel_ms": kernels}
+        if integ == PT:
+            pr.reset()
+            pr.render_samples(spp)
+            c = pr.probe(0)
+            rays = c["extension_rays"] + c["shadow_rays"]
+            boxes = c["extension_box_tests"] + c["shadow_box_tests"]
+            prims = c["extension_prim_tests"] + c["shadow_prim_tests"]
+            tb = 48.0 * rays + 32.0 * boxes + 36.0 * prims
+            tr_s = kt["trace_ms"] * 1e-3
+            entry["trace"] = {"rays": int(rays), "extension_rays": int(c["extension_rays"]),
+                              "shadow_rays": int(c["shadow_rays"]), "launches": int(c["trace_launches"]),
+                              "rays_per_launch": round(rays / max(1, c["trace_launches"]), 1),
+                              "grays_per_s": round(rays / tr_s / 1e9, 3),
+                              "box_tests_per_ray": round(boxes / rays, 2), "prim_tests_per_ray": round(prims / rays, 2)}
+            entry["roofline"] = {"kernel": "k_trace", "bound": "hbm", "achieved": round(tb / tr_s / 1e9, 1),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(tb / tr_s / 1e9 / HBM_PEAK_GBS, 4),
+                                 "achieved_basis": "algorithmic bytes of all trace launches of the render (48 B per "
+                                                   "ray + 32 B per box test + 36 B per primitive test) / their summed "
+                                                   "device time"}
         # first-pass latency: one sample per pixel on a fresh film (reference pass 1, progressive.go:104-107)
         pr2 = P.ProgressiveRaytracer(scene, P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp,
                                                               max_passes=7, seed=2, integrator=integ))
         t0 = time.perf_counter()
         pr2.render_pass(1)
-        ttfp_ms = (time.perf_counter() - t0) * 1e3
+        entry["first_pass_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
         pr2.close()
-        entry = {"config": name, "samples_per_s": round(gpu_sps, 1), "render_s": round(wall, 3),
-                 "device_ms": round(st.device_ms, 2), "first_pass_ms": round(ttfp_ms, 2)}
+        fx = load_fixture(fx_name)
+        if fx is not None:
+            x0, y0, x1, y1 = fx["rect"]
+            if [w, h] == fx["image_size"]:
+                g_img = img[y0:y1, x0:x1]
+            else:  # the fixture is of a smaller image of the same scene: render it at the configuration's spp
+                d2, _ = make(width=fx["image_size"][0])
+                sc2 = P.Scene(d2, device=0)
+                p2 = P.sampling_params(fx["image_size"][0], fx["image_size"][1], cfg["max_depth"],
+                                       cfg["rr_min_bounces"], spp, max_passes=1, seed=1, integrator=integ)
+                f2 = P.ProgressiveRaytracer(sc2, p2)
+                f2.render_samples(spp)
+                g_img = f2.read()[0][y0:y1, x0:x1]
+                f2.close()
+                sc2.close()
+            n_fix = fx["fixture_spp"]
+            e2 = float(fx["var"].mean()) / n_fix
+            rg = rmse(g_img, fx)
+            entry["rmse"] = {"fixture": fx_name, "fixture_spp": n_fix, "pixels": int((x1 - x0) * (y1 - y0)),
+                             "gpu": round(rg, 6), "gpu_spp": spp,
+                             "gpu_expected_unbiased": round(float(np.sqrt(fx["var"].mean() * (1 / spp + 1 / n_fix))), 6)}
         if not a.no_cpu:
             import oracle as O
 
-            os_ = O.OracleScene(desc)
-            # bounded sample: the whole image at a few samples per pixel (~2-6 s of CPU work)
-            c0 = time.perf_counter()
-            os_.render(params, c_spp, 0, nthreads=cores, splats=integ == BD)
-            ct = time.perf_counter() - c0
-            cpu_sps = w * h * c_spp / ct
+            if fx is not None:
+                dc = desc if [w, h] == fx["image_size"] else make(width=fx["image_size"][0])[0]
+                rect = tuple(fx["rect"])
+            else:  # no converged image: a bounded sample, the centre 128x128 pixels of the image
+                cw, ch = min(w, 128), min(h, 128)
+                dc, rect = desc, ((w - cw) // 2, (h - ch) // 2, (w + cw) // 2, (h + ch) // 2)
+            os_ = O.OracleScene(dc)
+            pc = P.sampling_params(dc.image_size[0], dc.image_size[1], cfg["max_depth"], cfg["rr_min_bounces"],
+                                   c_spp, max_passes=1, seed=7, integrator=integ)
+            npx = (rect[2] - rect[0]) * (rect[3] - rect[1])
+            acc, reps = 0.0, []
+            for k in range(a.cpu_repeats):
+                t0 = time.perf_counter()
+                r = os_.render(pc, c_spp, k * c_spp, rect=rect, nthreads=cores, splats=integ == BD)
+                reps.append(time.perf_counter() - t0)
+                part = r[0]
+                if integ == BD:
+                    x0, y0, x1, y1 = rect
+                    part = part + r[3][y0:y1, x0:x1]
+                acc = acc + part
+            spp1 = max(1, c_spp // cores)
+            t1 = timed(lambda: os_.render(pc, spp1, 0, rect=rect, nthreads=1, splats=integ == BD))
+            cpu_sps = npx * c_spp / trimmed_mean(reps)
             entry["cpu_samples_per_s"] = round(cpu_sps, 1)
-            entry["cpu_sample"] = f"full {w}x{h} image, {c_spp} spp, {cores} threads, {ct:.2f}s"
-            entry["speedup_equal_rmse"] = round(gpu_sps / cpu_sps, 1)
+            entry["cpu_one_thread_samples_per_s"] = round(npx * spp1 / t1, 1)
+            entry["cpu_sample"] = (f"{rect[2] - rect[0]}x{rect[3] - rect[1]} pixels of the {dc.image_size[0]}x"
+                                   f"{dc.image_size[1]} image, {c_spp} spp per run on {cores} threads, trimmed mean "
+                                   f"of {len(reps)} runs ({', '.join(f'{t:.2f}' for t in reps)} s); one thread: "
+                                   f"{spp1} spp, {t1:.2f} s")
+            entry["speedup_equal_spp"] = round(gpu_sps / cpu_sps, 1)
+            if fx is not None:
+                n_c = c_spp * len(reps)
+                rc = rmse(acc / n_c, fx)
+                rg = entry["rmse"]["gpu"]
+                entry["rmse"].update({"cpu": round(rc, 6), "cpu_spp": n_c})
+                if rg * rg > e2 and rc * rc > e2:
+                    need = n_c * (rc * rc - e2) / (rg * rg - e2)
+                    entry["rmse"]["cpu_spp_for_gpu_rmse"] = round(need, 1)
+                    t_cpu = need * npx / cpu_sps
+                    t_gpu = spp * npx / gpu_sps
+                    entry["speedup_equal_rmse"] = round(t_cpu / t_gpu, 1)
         log(f"{name}: {entry}")
         out.append(entry)
         pr.close()
         scene.close()
-    return {"pt": out, "cores": cores}
+    return {"pt": out, "cpu_host": host}
 
 
 def serve_section(a, log):
@@ -331,7 +488,7 @@ def serve_section(a, log):
     if kcpu > 0:
         import oracle as O
 
-        cores = min(os.cpu_count() or 1, 16)
+        cores = host_cpu()["threads"]
         factories = [scenes.default_scene, scenes.cornell_metre, scenes.mesh_sky_scene]
         by_k = {meta[r][0]: r for r in meta}
         cpu_ms, gpu_ms = [], []

@@ -163,10 +163,15 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, P
 // none when sq < 0) and, when `closest`, this bounce's extension rays (closest hit, path queue bounce & 1).
 // Neither depends on the other; both read the path origin. A wave takes 64-ray batches of one kind
 // (the extension rays' range is padded to whole batches), so its traversal mode is uniform.
-template <bool MIXED>
+// Traversal statistics of the counting trace kernel (prt_film_probe): extension rays, their node
+// visits, box tests and primitive tests, then the same four for shadow rays.
+constexpr int kProbeCounts = 8;
+
+template <bool MIXED, bool COUNT = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(DevScene sc, PathBuffers pb,
                                                                                          int bounce, int closest, int sq,
-                                                  uint2* gstack, uint32_t gstride) {
+                                                  uint2* gstack, uint32_t gstride,
+                                                  unsigned long long* probe = nullptr) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
@@ -190,11 +195,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
     }
     const uint32_t nc64 = (nc + 63u) & ~63u;
     const uint32_t stride = gridDim.x * kBlock;
+    TCounts ce{0, 0, 0}, cs{0, 0, 0};
+    uint32_t re = 0, rs = 0;
     // uniform trip count per wave so every lane reaches the loop exit together
     for (uint32_t base = blockIdx.x * kBlock + (threadIdx.x & ~63u); base < nc64 + ns; base += stride) {
         const bool any = base >= nc64;  // wave-uniform
         const uint32_t i = (any ? base - nc64 : base) + (threadIdx.x & 63u);
         if (i >= (any ? ns : nc)) continue;
+        if (COUNT) {
+            if (any) rs++;
+            else re++;
+        }
         const uint32_t slot = any ? qs[i] : qc[i];
         const float4 o = pb.o[slot];
         const float4 d = any ? pb.sd[slot] : pb.d[slot];
@@ -209,15 +220,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
         THit h;
         if (any) {
             r.tmax = d.w;
-            if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr)) {
+            if (!trace<true, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &cs)) {
                 float4 c = pb.sc[slot];
                 float4 l = pb.L[slot];
                 pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
             }
         } else {
             r.tmax = __int_as_float(0x7f800000);
-            trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
+            trace<false, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &ce);
             pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+        }
+    }
+    if (COUNT) {  // wave sums, one 64-bit atomic per count and wave
+        uint32_t v[kProbeCounts] = {re, ce.nodes, ce.boxes, ce.prims, rs, cs.nodes, cs.boxes, cs.prims};
+        for (int k = 0; k < kProbeCounts; k++) {
+            unsigned long long x = v[k];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+            if ((threadIdx.x & 63u) == 0 && x) atomicAdd(&probe[k], x);
         }
     }
 }
@@ -948,6 +967,15 @@ struct prt_film {
     uint32_t* act[2] = {nullptr, nullptr};
     uint32_t* ad_state = nullptr;  // per list: length, min count, max count
     uint32_t ad_n = 0;             // length of the current list
+    // measurement probe (prt_film_probe): counting trace kernels and per-kernel-kind event timing
+    uint32_t probe_flags = 0;
+    unsigned long long* probe_dev = nullptr;  // traversal counts of the counting trace kernel
+    uint64_t probe_launches = 0;              // trace launches enqueued while probing
+    struct Mark {
+        int kind;  // PRT_PROBE_KIND_*
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
 };
 
 namespace prt {
@@ -1176,6 +1204,20 @@ int film_advance(prt_film* f) {
     return PRT_OK;
 }
 
+// Per-kernel timing of prt_film_probe: an event pair around each launch of a kernel kind.
+hipError_t probe_mark(prt_film* f, int kind, bool begin) {
+    if (!(f->probe_flags & PRT_PROBE_TIME)) return hipSuccess;
+    hipEvent_t e = nullptr;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    if (begin) {
+        f->marks.push_back({kind, e, nullptr});
+    } else {
+        f->marks.back().b = e;
+    }
+    return hipEventRecord(e, f->stream);
+}
+
 // Enqueues the next planned chunk on the film's stream (no wait); ev0/ev1 bracket it.
 int film_enqueue_chunk(prt_film* f) {
     if (film_round_exhausted(f)) return PRT_OK;
@@ -1213,25 +1255,33 @@ int film_enqueue_chunk(prt_film* f) {
     if (f->bdpt) {
         int grid = (int)std::min<int64_t>(f->bdpt_blocks, (n + kBlock - 1) / kBlock);
         const int walk_blocks = (int)((n + kBlock - 1) / kBlock);
-        if (sc.mixed) {
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_PATHS, true));
+        if (sc.mixed)
             hipLaunchKernelGGL((k_bdpt_paths<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath,
                                f->lens, f->gstack, f->gslots);
-            hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
-            hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
-                               f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
-        } else {
+        else
             hipLaunchKernelGGL((k_bdpt_paths<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath,
                                f->lens, f->gstack, f->gslots);
-            hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_PATHS, false));
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, true));
+        hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, false));
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, true));
+        if (sc.mixed)
+            hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
+                               f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
+        else
             hipLaunchKernelGGL((k_bdpt_connect<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
-        }
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, false));
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_FILM, true));
         if (f->adaptive)
             hipLaunchKernelGGL(k_film_adaptive, dim3((pp.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pp,
                                f->pb, f->fb, ap);
         else
             hipLaunchKernelGGL(k_film, dim3((pp.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, st, pp, f->pb,
                                f->fb);
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_FILM, false));
         PRT_HIP(hipGetLastError());
         PRT_HIP(hipEventRecord(f->ev1, st));
         if (f->track) PRT_HIP(hipEventRecord(f->rev1[ring], st));
@@ -1240,27 +1290,41 @@ int film_enqueue_chunk(prt_film* f) {
     }
     PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 4 * kCounterStride * sizeof(uint32_t), st));
     int cam_blocks = (int)((n + kBlock - 1) / kBlock);
+    PRT_HIP(probe_mark(f, PRT_PROBE_KIND_CAMERA, true));
     hipLaunchKernelGGL(k_camera, dim3(cam_blocks), dim3(kBlock), 0, st, sc, pp, f->pb);
+    PRT_HIP(probe_mark(f, PRT_PROBE_KIND_CAMERA, false));
     // bounce b: one trace launch (the shadow rays of bounce b - 1 and the extension rays of bounce b),
     // then shading; a last launch traces the shadow rays of the last bounce
+    const bool counting = (f->probe_flags & PRT_PROBE_COUNT) != 0;
     auto trace = [&](int b, int closest, int sq) {
-        auto fn = sc.mixed ? (const void*)k_trace<true> : (const void*)k_trace<false>;
+        const void* fn = counting ? (sc.mixed ? (const void*)k_trace<true, true> : (const void*)k_trace<false, true>)
+                                  : (sc.mixed ? (const void*)k_trace<true> : (const void*)k_trace<false>);
         PathBuffers pbv = f->pb;
         uint2* gs = f->gstack;
         uint32_t gsl = f->gslots;
-        void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b, (void*)&closest, (void*)&sq, (void*)&gs, (void*)&gsl};
-        return hipLaunchKernel(fn, dim3(f->blocks), dim3(kBlock), args, 0, st);
+        unsigned long long* pr = f->probe_dev;
+        void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b,  (void*)&closest,
+                        (void*)&sq, (void*)&gs,  (void*)&gsl, (void*)&pr};
+        hipError_t e = probe_mark(f, PRT_PROBE_KIND_TRACE, true);
+        if (e == hipSuccess) e = hipLaunchKernel(fn, dim3(f->blocks), dim3(kBlock), args, 0, st);
+        if (e == hipSuccess) e = probe_mark(f, PRT_PROBE_KIND_TRACE, false);
+        if (f->probe_flags) f->probe_launches++;
+        return e;
     };
     for (int b = 0; b < pp.max_depth; b++) {
         PRT_HIP(trace(b, 1, b > 0 ? ((b - 1) & 1) : -1));
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_SHADE, true));
         hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
+        PRT_HIP(probe_mark(f, PRT_PROBE_KIND_SHADE, false));
     }
     if (pp.max_depth > 0) PRT_HIP(trace(pp.max_depth, 0, (pp.max_depth - 1) & 1));
     int film_blocks_n = (pp.npix + kBlock - 1) / kBlock;
+    PRT_HIP(probe_mark(f, PRT_PROBE_KIND_FILM, true));
     if (f->adaptive)
         hipLaunchKernelGGL(k_film_adaptive, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb, ap);
     else
         hipLaunchKernelGGL(k_film, dim3(film_blocks_n), dim3(kBlock), 0, st, pp, f->pb, f->fb);
+    PRT_HIP(probe_mark(f, PRT_PROBE_KIND_FILM, false));
     PRT_HIP(hipGetLastError());
     PRT_HIP(hipEventRecord(f->ev1, st));
     if (f->track) PRT_HIP(hipEventRecord(f->rev1[ring], st));
@@ -1547,6 +1611,10 @@ int prt_film_destroy(prt_film* f) {
     if (!f) return PRT_OK;
     (void)hipSetDevice(f->scene->device);
     if (f->stream) (void)hipStreamSynchronize(f->stream);
+    for (auto& m : f->marks) {
+        if (m.a) (void)hipEventDestroy(m.a);
+        if (m.b) (void)hipEventDestroy(m.b);
+    }
     for (void* p : f->bufs) (void)hipFree(p);
     if (f->rgba_host) (void)hipHostFree(f->rgba_host);
     if (f->ev0) (void)hipEventDestroy(f->ev0);
@@ -1644,6 +1712,42 @@ int prt_film_read_splats(const prt_film* f, float* splat_rgb) {
                            f->stream));
     PRT_HIP(hipStreamSynchronize(f->stream));
     for (int64_t k = 0; k < 3 * f->npix; k++) splat_rgb[k] = (float)((double)spl[k] * (1.0 / (double)kSplatScale));
+    return PRT_OK;
+}
+
+int prt_film_probe(prt_film* f, uint32_t flags, prt_film_probe_stats* out) {
+    if (!f) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL film");
+    if (flags & ~(PRT_PROBE_COUNT | PRT_PROBE_TIME)) return fail(PRT_ERR_INVALID_ARGUMENT, "unknown probe flags");
+    PRT_HIP(hipSetDevice(f->scene->device));
+    PRT_HIP(hipStreamSynchronize(f->stream));
+    prt_film_probe_stats st{};
+    if (f->probe_dev) {
+        unsigned long long c[kProbeCounts] = {};
+        PRT_HIP(hipMemcpy(c, f->probe_dev, sizeof(c), hipMemcpyDeviceToHost));
+        uint64_t* o = &st.extension_rays;
+        for (int k = 0; k < kProbeCounts; k++) o[k] = c[k];
+    }
+    st.trace_launches = f->probe_launches;
+    double* ms[] = {&st.camera_ms, &st.trace_ms, &st.shade_ms, &st.film_ms, &st.bdpt_paths_ms, &st.bdpt_walks_ms,
+                    &st.bdpt_connect_ms};
+    for (auto& m : f->marks) {
+        float t = 0;
+        if (m.a && m.b && hipEventElapsedTime(&t, m.a, m.b) == hipSuccess && m.kind >= 0 && m.kind < 7)
+            *ms[m.kind] += t;
+        if (m.a) (void)hipEventDestroy(m.a);
+        if (m.b) (void)hipEventDestroy(m.b);
+    }
+    f->marks.clear();
+    f->probe_launches = 0;
+    if ((flags & PRT_PROBE_COUNT) && !f->probe_dev) {
+        void* p = nullptr;
+        int rc = film_alloc(f, sizeof(unsigned long long) * kProbeCounts, &p);
+        if (rc != PRT_OK) return rc;
+        f->probe_dev = (unsigned long long*)p;
+    }
+    if (f->probe_dev) PRT_HIP(hipMemset(f->probe_dev, 0, sizeof(unsigned long long) * kProbeCounts));
+    f->probe_flags = flags;
+    if (out) *out = st;
     return PRT_OK;
 }
 

@@ -146,6 +146,7 @@ EXPORTS = [
     "prt_scene_destroy", "prt_intersect_closest", "prt_intersect_any", "prt_traversal_counts",
     "prt_intersect_closest_static",
     "prt_samples_for_pass", "prt_film_create", "prt_render_pass", "prt_render_samples", "prt_film_read",
+    "prt_film_probe",
     "prt_film_read_splats", "prt_film_read_rgba8", "prt_film_reset", "prt_film_destroy", "prt_film_size",
     "prt_workload_icosphere", "prt_workload_rays", "prt_workload_sky", "prt_ply_load", "prt_ply_free",
     "prt_server_create", "prt_server_submit", "prt_server_poll", "prt_server_cancel", "prt_server_read",
@@ -157,6 +158,20 @@ EXPORTS = [
 SUBMIT_SNAPSHOTS = 1
 
 EVENT_ADMITTED, EVENT_PASS_COMPLETE, EVENT_DONE, EVENT_ERROR, EVENT_CANCELLED = range(5)
+
+
+class ProbeStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("extension_rays", "extension_node_visits", "extension_box_tests",
+                                          "extension_prim_tests", "shadow_rays", "shadow_node_visits",
+                                          "shadow_box_tests", "shadow_prim_tests", "trace_launches")] + \
+               [(n, C.c_double) for n in ("camera_ms", "trace_ms", "shade_ms", "film_ms", "bdpt_paths_ms",
+                                          "bdpt_walks_ms", "bdpt_connect_ms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+PROBE_COUNT, PROBE_TIME = 1, 2
 
 
 class PlyData(C.Structure):
@@ -214,6 +229,7 @@ def lib():
         "prt_film_size": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
         "prt_film_reset": (C.c_int, [vp]),
         "prt_film_destroy": (C.c_int, [vp]),
+        "prt_film_probe": (C.c_int, [vp, C.c_uint32, C.POINTER(ProbeStats)]),
         "prt_workload_icosphere": (C.c_int, [C.c_int32, C.c_double, vp, C.c_double, C.c_uint64, vp,
                                              C.POINTER(C.c_int64), vp, C.POINTER(C.c_int64)]),
         "prt_workload_rays": (C.c_int, [vp, vp, C.c_double, C.c_int64, C.c_uint64, vp]),
@@ -713,6 +729,13 @@ class ProgressiveRaytracer:
         check(lib().prt_film_read(self.handle, _ptr(mean), _ptr(ls), _ptr(lq), _ptr(cnt)))
         assert mean.size == 3 * n
         return mean, ls, lq, cnt
+
+    def probe(self, flags=0):
+        """prt_film_probe: statistics gathered since the last call (dict); `flags` sets what later renders
+        gather (PROBE_COUNT traversal counts, PROBE_TIME per-kernel device time)."""
+        st = ProbeStats()
+        check(lib().prt_film_probe(self.handle, flags, C.byref(st)))
+        return st.as_dict()
 
     def splats(self):
         """BDPT light-tracing splats accumulated so far (H,W,3); included in read()'s mean."""

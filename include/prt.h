@@ -337,6 +337,33 @@ int prt_film_destroy(prt_film* film);
  * prt_film_read and prt_film_read_rgba8 fill. */
 int prt_film_size(const prt_film* film, int32_t* width, int32_t* height);
 
+/* Measurement helper (no reference counterpart): a film's kernel-level statistics for profiling and
+ * benchmarks. Waits for the film's enqueued work, returns in *stats (may be NULL) what was gathered
+ * since the previous call, clears it, and sets what later renders gather:
+ *   PRT_PROBE_COUNT  path-tracing traces run a counting variant of the trace kernel (same results,
+ *                    slower) that counts rays, 4-wide node visits, ray/box slab tests and ray/primitive
+ *                    tests, separately for extension (closest-hit) and shadow (any-hit) rays;
+ *   PRT_PROBE_TIME   every kernel launch is bracketed by an event pair and the device time is summed
+ *                    per kernel kind.
+ * flags = 0 turns both off. */
+#define PRT_PROBE_COUNT 1u
+#define PRT_PROBE_TIME 2u
+#define PRT_PROBE_KIND_CAMERA 0
+#define PRT_PROBE_KIND_TRACE 1
+#define PRT_PROBE_KIND_SHADE 2
+#define PRT_PROBE_KIND_FILM 3
+#define PRT_PROBE_KIND_BDPT_PATHS 4
+#define PRT_PROBE_KIND_BDPT_WALKS 5
+#define PRT_PROBE_KIND_BDPT_CONNECT 6
+typedef struct prt_film_probe_stats {
+    uint64_t extension_rays, extension_node_visits, extension_box_tests, extension_prim_tests;
+    uint64_t shadow_rays, shadow_node_visits, shadow_box_tests, shadow_prim_tests;
+    uint64_t trace_launches;
+    double camera_ms, trace_ms, shade_ms, film_ms; /* device time per kernel kind (PRT_PROBE_TIME) */
+    double bdpt_paths_ms, bdpt_walks_ms, bdpt_connect_ms;
+} prt_film_probe_stats;
+int prt_film_probe(prt_film* film, uint32_t flags, prt_film_probe_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,3 +28,28 @@ def test_profiled_traffic_of_every_variant():
     vals = [b.profiled_traffic(v, rays) for v in range(5)]
     assert all(isinstance(v, int) and 1e9 < v < 1e10 for v in vals), vals
     assert b.profiled_traffic(2, rays + 1) is None and b.profiled_traffic(99, rays) is None
+
+
+def test_host_cpu_and_trimmed_mean():
+    b = _bench()
+    h = b.host_cpu()
+    assert 1 <= h["threads"] <= len(os.sched_getaffinity(0)) and h["nproc"] >= 1 and h["cpu_model"]
+    assert b.trimmed_mean([5.0, 1.0, 2.0, 3.0, 100.0]) == 10.0 / 3.0  # min and max dropped
+    assert b.trimmed_mean([2.0, 4.0]) == 3.0
+
+
+def test_fixture_loader_checks_the_pin():
+    """load_fixture returns the converged image only when its SHA-256 matches the manifest; RMSE against
+    the fixture of the fixture's own mean is zero."""
+    import json
+
+    b = _bench()
+    man = os.path.join(ROOT, "tests", "golden", "converged.json")
+    if not os.path.exists(man):
+        return
+    for name in json.load(open(man)):
+        fx = b.load_fixture(name)
+        assert fx is not None, name
+        assert fx["mean"].shape == fx["var"].shape and fx["mean"].shape[2] == 3
+        assert b.rmse(fx["mean"], fx) == 0.0
+    assert b.load_fixture("no_such_fixture") is None

@@ -182,3 +182,29 @@ def test_rgba8_image_is_the_gamma_of_the_mean(integrator):
     sub = pr.image(5, 7, 41, 30)
     assert sub.shape == (23, 36, 4) and np.array_equal(sub[..., :3], exp[7:30, 5:41])
     assert pr.image(3, 3, 3, 9).shape == (6, 0, 4)
+
+
+def test_probe_counts_and_times():
+    """prt_film_probe: the counting trace kernel leaves films bit-identical; at depth 1 every camera sample
+    traces exactly one extension ray; each node visit is four box tests; per-kernel times are positive;
+    a probe call clears what it returns."""
+    s, cfg = scenes.default_scene(width=96)
+    scene = P.Scene(s)
+    w, h = s.image_size
+    plain = P.ProgressiveRaytracer(scene, params_for(s, cfg, 4, seed=2))
+    plain.render_samples(4)
+    probed = P.ProgressiveRaytracer(scene, params_for(s, cfg, 4, seed=2))
+    probed.probe(P.PROBE_COUNT | P.PROBE_TIME)
+    probed.render_samples(4)
+    st = probed.probe(0)
+    assert np.array_equal(plain.read()[0], probed.read()[0])
+    assert st["extension_rays"] >= w * h * 4 and st["shadow_rays"] > 0
+    assert st["extension_box_tests"] == 4 * st["extension_node_visits"] > 0
+    assert st["shadow_box_tests"] == 4 * st["shadow_node_visits"] > 0
+    assert st["trace_launches"] == cfg["max_depth"] + 1
+    assert st["trace_ms"] > 0 and st["shade_ms"] > 0 and st["camera_ms"] > 0 and st["film_ms"] > 0
+    assert probed.probe(0)["extension_rays"] == 0
+    one = P.ProgressiveRaytracer(scene, P.sampling_params(w, h, 1, cfg["rr_min_bounces"], 3, seed=2))
+    one.probe(P.PROBE_COUNT)
+    one.render_samples(3)
+    assert one.probe(0)["extension_rays"] == w * h * 3
