@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -175,15 +176,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
-    TStack<kLds, kBlock> st;
-    st.s_node = s_node;
-    st.s_t = s_t;
-    st.g = gstack;
-    st.lane = threadIdx.x;
-    st.gslot = blockIdx.x * kBlock + threadIdx.x;
-    st.gstride = gstride;
-    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
-    st.ncache = s_cache;
     const uint32_t cur = (uint32_t)bounce & 1u;
     const uint32_t nc = closest ? *qlen(pb, cur) : 0u;
     const uint32_t ns = sq >= 0 ? *qlen(pb, 2u + (uint32_t)sq) : 0u;
@@ -195,6 +187,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
     }
     const uint32_t nc64 = (nc + 63u) & ~63u;
     const uint32_t stride = gridDim.x * kBlock;
+    // work-groups without rays leave before loading the node cache (the deep bounces of a pass trace few
+    // rays on the full persistent grid)
+    if (blockIdx.x * kBlock >= nc64 + ns) return;
+    TStack<kLds, kBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kBlock + threadIdx.x;
+    st.gstride = gstride;
+    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
+    st.ncache = s_cache;
     TCounts ce{0, 0, 0}, cs{0, 0, 0};
     uint32_t re = 0, rs = 0;
     // uniform trip count per wave so every lane reaches the loop exit together
@@ -946,11 +950,10 @@ struct prt_film {
     hipEvent_t rev0[kRing] = {}, rev1[kRing] = {};
     int64_t enq = 0, seen = 0;
     int64_t chunk_paths = 0;  // paths per planned chunk (at most capacity)
-    // RGBA8 reads (prt_film_read_rgba8): device image and pinned host staging, allocated at first use and
-    // shared by the film's readers under rgba_mu (a served film may be read from several threads)
+    // RGBA8 reads (prt_film_read_rgba8): device image allocated with the film, shared by the film's readers
+    // under rgba_mu (a served film may be read from several threads); host staging from a process-wide pool
     std::mutex rgba_mu;
     uchar4* rgba_dev = nullptr;
-    uchar4* rgba_host = nullptr;
     // chunk plan of the samples being rendered (see film_plan)
     struct Chunk {
         int32_t px0, npix, spp, sample_base;
@@ -1032,6 +1035,40 @@ int film_read_snapshot(const prt_film* f, const void* src, hipEvent_t done, hipS
 
 
 namespace {
+
+// Pinned host staging of RGBA8 reads, shared by all films of the process: pinning pages costs
+// milliseconds, so buffers are kept for reuse; each read holds one buffer while it copies.
+std::mutex g_pinned_mu;
+std::vector<std::pair<size_t, void*>> g_pinned_free;  // (capacity, buffer) not in use
+std::map<void*, size_t> g_pinned_cap;                  // capacity of every buffer
+
+int pinned_take(size_t bytes, void** out) {
+    {
+        std::lock_guard<std::mutex> lock(g_pinned_mu);
+        size_t best = g_pinned_free.size();
+        for (size_t i = 0; i < g_pinned_free.size(); i++)
+            if (g_pinned_free[i].first >= bytes && (best == g_pinned_free.size() || g_pinned_free[i].first <
+                                                                                          g_pinned_free[best].first))
+                best = i;
+        if (best < g_pinned_free.size()) {
+            *out = g_pinned_free[best].second;
+            g_pinned_free.erase(g_pinned_free.begin() + (ptrdiff_t)best);
+            return PRT_OK;
+        }
+    }
+    // round up to 1 MiB so that reads of nearby sizes share buffers
+    const size_t cap = (bytes + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+    hipError_t e = hipHostMalloc(out, cap, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(RGBA8 staging)");
+    std::lock_guard<std::mutex> lock(g_pinned_mu);
+    g_pinned_cap[*out] = cap;
+    return PRT_OK;
+}
+
+void pinned_give(void* p, size_t) {
+    std::lock_guard<std::mutex> lock(g_pinned_mu);
+    g_pinned_free.push_back({g_pinned_cap[p], p});
+}
 
 int film_alloc(prt_film* f, size_t bytes, void** out) {
     void* p = nullptr;
@@ -1511,6 +1548,9 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
         if ((rc = film_alloc(f, 6 * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
         f->ad_state = (uint32_t*)ptr;
     }
+    // the device image of RGBA8 reads (allocated here, so a pass's first read does not allocate)
+    if ((rc = film_alloc(f, sizeof(uchar4) * (size_t)f->npix, &ptr)) != PRT_OK) return bail(rc);
+    f->rgba_dev = (uchar4*)ptr;
     hipError_t e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
     e = hipEventCreate(&f->ev0);
@@ -1616,7 +1656,6 @@ int prt_film_destroy(prt_film* f) {
         if (m.b) (void)hipEventDestroy(m.b);
     }
     for (void* p : f->bufs) (void)hipFree(p);
-    if (f->rgba_host) (void)hipHostFree(f->rgba_host);
     if (f->ev0) (void)hipEventDestroy(f->ev0);
     if (f->ev1) (void)hipEventDestroy(f->ev1);
     for (int i = 0; i < prt_film::kRing; i++) {
@@ -1761,25 +1800,19 @@ int prt_film_read_rgba8(const prt_film* f, int32_t x0, int32_t y0, int32_t x1, i
     prt_film* m = const_cast<prt_film*>(f);
     std::lock_guard<std::mutex> lock(m->rgba_mu);
     PRT_HIP(hipSetDevice(f->scene->device));
-    if (!m->rgba_dev) {
-        void* p = nullptr;
-        int rc = film_alloc(m, sizeof(uchar4) * (size_t)f->npix, &p);
-        if (rc != PRT_OK) return rc;
-        m->rgba_dev = (uchar4*)p;
-    }
-    if (!m->rgba_host) {
-        void* p = nullptr;
-        hipError_t e = hipHostMalloc(&p, sizeof(uchar4) * (size_t)f->npix, hipHostMallocDefault);
-        if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(rgba)");
-        m->rgba_host = (uchar4*)p;
-    }
+    const size_t bytes = sizeof(uchar4) * (size_t)n;
+    void* stage = nullptr;
+    int rc = pinned_take(bytes, &stage);
+    if (rc != PRT_OK) return rc;
     // ordered after the work already enqueued on the film's stream
     hipLaunchKernelGGL(k_film_rgba8, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, f->stream,
                        f->width, x0, y0, tw, n, f->fb.sum, f->fb.count, f->fb.splat, m->rgba_dev);
-    PRT_HIP(hipGetLastError());
-    PRT_HIP(hipMemcpyAsync(m->rgba_host, m->rgba_dev, sizeof(uchar4) * (size_t)n, hipMemcpyDeviceToHost, f->stream));
-    PRT_HIP(hipStreamSynchronize(f->stream));
-    std::memcpy(rgba, m->rgba_host, sizeof(uchar4) * (size_t)n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(stage, m->rgba_dev, bytes, hipMemcpyDeviceToHost, f->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+    if (e == hipSuccess) std::memcpy(rgba, stage, bytes);
+    pinned_give(stage, bytes);
+    if (e != hipSuccess) return hip_fail(e, "RGBA8 read");
     return PRT_OK;
 }
 
