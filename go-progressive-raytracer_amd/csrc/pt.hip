@@ -579,8 +579,9 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_walks(PassParams pp, VertexStor
     store_walk_sums(LP, i, (int)(len >> 16), true);
 }
 
-// Strategies: one thread per sample evaluates every (s, t) of RayColor (bdpt.go:83-99 with
-// evaluateBDPTStrategy :294-313), camera-side radiance to pb.L, light-tracing splats to `splat`.
+// Strategies with a sampled or single vertex: one thread per sample evaluates (0, nc), (1, t) and (s, 1) of
+// RayColor (bdpt.go:83-99 with evaluateBDPTStrategy :294-313), camera-side radiance to pb.L, light-tracing
+// splats to `splat`. k_bdpt_connect_general adds the general connections to pb.L afterwards.
 template <bool MIXED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bdpt_connect(DevScene sc, PassParams pp, PathBuffers pb, VertexStore CP,
                                                          VertexStore LP, const uint32_t* lens, long long* splat,
@@ -614,7 +615,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         // order (phase 3 only splats, with integer atomics).
         const int lb = nl >= 1 ? nc - 1 : 0, lc = nl >= 2 ? nl - 1 : 0;
         const int wb = wave_max_small(lb), wc = wave_max_small(lc);
-        const int npairs = 1 + wb + wc + lc * (nc - 1);
+        // the general connections (s >= 2, t >= 2) run in k_bdpt_connect_general after this kernel
+        const int npairs = 1 + wb + wc;
         int ds = 2, dt = 1;  // phase 4 position
         for (int j = 0; j < npairs; j++) {
             int s, t;
@@ -751,6 +753,104 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             } else {
                 total = total + Lc * w;
             }
+        }
+        pb.L[i] = make_float4(total.x, total.y, total.z, 0.f);
+    }
+}
+
+// calculateMISWeight for a general connection (s >= 2, t >= 2): mis_weight's last case, with the light side's
+// walk past ls2 (l_tail, l_fwd2) and the segment ls1 -> ls2 (c) computed once per light vertex.
+__device__ __forceinline__ float mis_general(const DevScene& sc, const DevCamera& cam, const BV& ct1, const BV& ct2,
+                                            const BV& ls1, const BV& ls2, const Seg& c, float l_tail, float l_fwd2,
+                                            int t) {
+    const bool c_first = !(ct2.flags & VF_SPECULAR);
+    const float c_tail = ((ct2.flags & VF_CONN) ? 1.0f : 0.0f) + ct2.q;
+    const float c_fwd2 = remap0(ct2.pdf_fwd);
+    const Seg a = seg_of(ls1.p - ct1.p), b = seg_of(ct2.p - ct1.p);
+    const Seg ar = seg_rev(a);
+    const float c_rev1 = vtx_pdf_seg(sc, cam, ls1, c, ct1, ar);
+    const float c_rev2 = vtx_pdf_seg(sc, cam, ct1, a, ct2, b);
+    const float l_rev1 = vtx_pdf_seg(sc, cam, ct1, b, ls1, a);
+    const float l_rev2 = vtx_pdf_seg(sc, cam, ls1, ar, ls2, c);
+    float sum = 0.0f;
+    float ri = remap0(c_rev1) / remap0(ct1.pdf_fwd);
+    if (c_first) sum += ri;
+    if (t > 2) sum += ri * (remap0(c_rev2) / c_fwd2) * c_tail;
+    ri = remap0(l_rev1) / remap0(ls1.pdf_fwd);
+    sum += ri + ri * (remap0(l_rev2) / l_fwd2) * l_tail;
+    return 1.0f / (1.0f + sum);
+}
+
+// General connections (evaluateConnectionStrategy, bdpt.go:489-561): every (s, t) with s, t >= 2 in the
+// reference's order (s outer, t inner), added to the camera-side radiance pb.L. Each light vertex is read
+// once per sample (vertex s - 2 of a row is vertex s - 1 of the previous row) and each camera vertex once
+// per row (vertex t - 2 is the previous step's t - 1), instead of four vertex reads per strategy.
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bdpt_connect_general(
+    DevScene sc, PassParams pp, PathBuffers pb, VertexStore CP, VertexStore LP, const uint32_t* lens, uint2* gstack,
+    uint32_t gstride) {
+    __shared__ uint32_t s_node[kLds * kBlock];
+    __shared__ float s_t[kLds * kBlock];
+    __shared__ float4 s_cache[kNodeCache * 8];
+    TStack<kLds, kBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kBlock + threadIdx.x;
+    st.gstride = gstride;
+    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
+    st.ncache = s_cache;
+    const DevCamera& cam = sc.cam;
+    const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const uint32_t len = lens[i];
+        const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
+        if (nl < 2 || nc < 2) continue;
+        const float4 l4 = pb.L[i];
+        V3 total = v3(l4.x, l4.y, l4.z);
+        BV ls2 = LP.get(0, i);
+        for (int s = 2; s <= nl; s++) {
+            const BV ls1 = LP.get(s - 1, i);
+            if (!(ls1.flags & VF_SPECULAR)) {
+                const float l_tail = ((ls2.flags & VF_CONN) ? 1.0f : 0.0f) + ls2.q, l_fwd2 = remap0(ls2.pdf_fwd);
+                const Seg c = seg_of(ls2.p - ls1.p);
+                BV ct2 = CP.get(0, i);
+                for (int t = 2; t <= nc; t++) {
+                    const BV ct1 = CP.get(t - 1, i);
+                    if (!(ct1.flags & VF_SPECULAR)) {
+                        V3 dd = ls1.p - ct1.p;
+                        const float dist = length(dd);
+                        bool ok = dist >= 0.001f;
+                        V3 contrib = v3(0, 0, 0);
+                        TRay vr;
+                        if (ok) {
+                            dd = dd * (1.0f / dist);
+                            const float cc = dot(dd, ct1.n), cl = dot(dd * -1.0f, ls1.n);
+                            ok = cc > 0.0f && cl > 0.0f;
+                            if (ok) {
+                                const float G = (cc * cl) / (dist * dist);
+                                const V3 fc = vertex_brdf(sc, ct1, dd, 0);
+                                const V3 fl = vertex_brdf(sc, ls1, dd * -1.0f, 1);
+                                contrib = mulv(mulv(mulv(ls1.beta, fl), fc), ct1.beta) * G;
+                                ok = contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f;
+                                vr = TRay{ct1.p.x, ct1.p.y, ct1.p.z, 0.001f, dd.x, dd.y, dd.z, dist - 0.001f};
+                            }
+                        }
+                        if (ok) {
+                            // the weight before the visibility ray: only the weighted contribution stays live
+                            // across the traversal
+                            const V3 lw = contrib * mis_general(sc, cam, ct1, ct2, ls1, ls2, c, l_tail, l_fwd2, t);
+                            THit vh;
+                            if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr))
+                                total = total + lw;
+                        }
+                    }
+                    ct2 = ct1;
+                }
+            }
+            ls2 = ls1;
         }
         pb.L[i] = make_float4(total.x, total.y, total.z, 0.f);
     }
@@ -939,6 +1039,7 @@ struct prt_film {
     // BDPT: camera and light subpath stores and subpath lengths, per path slot
     bool bdpt = false;
     int bdpt_blocks = 0;
+    int bdpt_general_blocks = 0;  // grid of k_bdpt_connect_general (its own occupancy)
     VertexStore cpath{}, lpath{};
     uint32_t* lens = nullptr;
     hipStream_t stream = nullptr;
@@ -1106,6 +1207,16 @@ int bdpt_blocks(const prt_scene* scene, int trace_blocks) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_bdpt_connect<true>, kBlock, 0) !=
             hipSuccess ||
+        per_cu <= 0)
+        per_cu = 2;
+    return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu * 2));
+}
+
+// Grid of the general-connection kernel: its resident blocks twice, at most the trace grid (stack slots).
+int bdpt_general_blocks(const prt_scene* scene, int trace_blocks) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_bdpt_connect_general<true>, kBlock,
+                                                     0) != hipSuccess ||
         per_cu <= 0)
         per_cu = 2;
     return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu * 2));
@@ -1291,6 +1402,7 @@ int film_enqueue_chunk(prt_film* f) {
     PRT_HIP(hipEventRecord(f->ev0, st));
     if (f->bdpt) {
         int grid = (int)std::min<int64_t>(f->bdpt_blocks, (n + kBlock - 1) / kBlock);
+        int grid_g = (int)std::min<int64_t>(f->bdpt_general_blocks, (n + kBlock - 1) / kBlock);
         const int walk_blocks = (int)((n + kBlock - 1) / kBlock);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_PATHS, true));
         if (sc.mixed)
@@ -1304,12 +1416,17 @@ int film_enqueue_chunk(prt_film* f) {
         hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, true));
-        if (sc.mixed)
+        if (sc.mixed) {
             hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
-        else
+            hipLaunchKernelGGL((k_bdpt_connect_general<true>), dim3(grid_g), dim3(kBlock), 0, st, sc,
+                               pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
+        } else {
             hipLaunchKernelGGL((k_bdpt_connect<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
+            hipLaunchKernelGGL((k_bdpt_connect_general<false>), dim3(grid_g), dim3(kBlock), 0, st,
+                               sc, pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
+        }
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_FILM, true));
         if (f->adaptive)
@@ -1528,6 +1645,7 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     if (p->integrator == PRT_INTEGRATOR_BDPT) {
         f->bdpt = true;
         f->bdpt_blocks = bdpt_blocks(scene, f->blocks);
+        f->bdpt_general_blocks = bdpt_general_blocks(scene, f->blocks);
         const uint32_t uv = scene->dev.textured ? 1u : 0u;
         const size_t vb = bdpt_store_bytes(p->max_depth, C, uv != 0);
         if ((rc = film_alloc(f, vb, &ptr)) != PRT_OK) return bail(rc);
