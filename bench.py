@@ -272,14 +272,16 @@ def pt_section(a, log):
     import scenes
 
     PT, BD = P.INTEGRATOR_PATH, P.INTEGRATOR_BDPT
-    # (tag, name, scene, integrator, spp override, fixture, CPU samples per pixel of one timed repeat)
+    # (tag, name, scene, integrator, spp override, fixture, CPU samples per pixel of one timed repeat: the
+    # configuration's own, so that GPU and CPU errors are compared at equal samples per pixel and the
+    # rare high-variance paths of heavy-tailed scenes enter both)
     configs = [
         ("c0", "C0 default scene 400x225, 64 spp, depth 50 (pinhole)", lambda **k: scenes.default_scene(**k), PT,
          None, "c0_default_400x225", 64),
         ("c2", "C2 metre Cornell boxes 512x512, 1024 spp, depth 8", lambda **k: scenes.cornell_metre(**k), PT, None,
-         "c2_cornell_metre_128", 256),
+         "c2_cornell_metre_128", 1024),
         ("c3", "C3 1.8M-triangle glass icosphere + HDR sky 1024x1024, 256 spp, depth 16",
-         lambda **k: scenes.mesh_sky_scene(**k), PT, None, "c3_mesh_sky_crop128", 128),
+         lambda **k: scenes.mesh_sky_scene(**k), PT, None, "c3_mesh_sky_crop128", 256),
         ("c2bdpt", "C2-BDPT metre Cornell boxes 512x512, bidirectional, 64 spp, depth 8",
          lambda **k: scenes.cornell_metre(**k), BD, 64, "c2_bdpt_cornell_metre_128", 64),
     ]
@@ -296,7 +298,7 @@ def pt_section(a, log):
         params = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp, max_passes=1, seed=1,
                                    integrator=integ)
         pr = P.ProgressiveRaytracer(scene, params)
-        pr.render_samples(1)  # warm-up (kernels, first-touch)
+        pr.render_samples(spp)  # warm-up (kernels, first-touch, the scene's trace-kernel autotuning)
         pr.reset()
         t0 = time.perf_counter()
         st = pr.render_samples(spp)
@@ -311,6 +313,8 @@ def pt_section(a, log):
         kernels = {k: round(v, 2) for k, v in kt.items() if k.endswith("_ms") and v > 0}
         entry = {"config": name, "samples_per_s": round(gpu_sps, 1), "render_s": round(wall, 3),
                  "device_ms": round(st.device_ms, 2), "kernel_ms": kernels}
+        if integ == PT:  # the trace-kernel occupancy the per-scene autotuner picked on the first large chunk
+            entry["autotune"] = {"trace_waves_per_simd": int(scene.info().pt_trace_waves)}
         if integ == PT:
             pr.reset()
             pr.render_samples(spp)
@@ -339,24 +343,36 @@ def pt_section(a, log):
         pr2.close()
         fx = load_fixture(fx_name)
         if fx is not None:
+            # mean squared error against the converged image over independent GPU renders at the
+            # configuration's spp (seeds 1..k): single images of heavy-tailed scenes scatter widely
             x0, y0, x1, y1 = fx["rect"]
-            if [w, h] == fx["image_size"]:
-                g_img = img[y0:y1, x0:x1]
-            else:  # the fixture is of a smaller image of the same scene: render it at the configuration's spp
-                d2, _ = make(width=fx["image_size"][0])
+            same = [w, h] == fx["image_size"]
+            if same:
+                sc2, d2 = scene, desc
+            else:  # the fixture is of a smaller image of the same scene
+                d2 = make(width=fx["image_size"][0])[0]
                 sc2 = P.Scene(d2, device=0)
-                p2 = P.sampling_params(fx["image_size"][0], fx["image_size"][1], cfg["max_depth"],
-                                       cfg["rr_min_bounces"], spp, max_passes=1, seed=1, integrator=integ)
-                f2 = P.ProgressiveRaytracer(sc2, p2)
-                f2.render_samples(spp)
-                g_img = f2.read()[0][y0:y1, x0:x1]
-                f2.close()
+            fw, fh = d2.image_size
+            k_seeds = 4 if fw * fh * spp > (1 << 26) else 8
+            mses = []
+            for seed in range(1, k_seeds + 1):
+                if same and seed == 1:
+                    g_img = img[y0:y1, x0:x1]
+                else:
+                    f2 = P.ProgressiveRaytracer(sc2, P.sampling_params(fw, fh, cfg["max_depth"], cfg["rr_min_bounces"],
+                                                                       spp, max_passes=1, seed=seed, integrator=integ))
+                    f2.render_samples(spp)
+                    g_img = f2.read()[0][y0:y1, x0:x1]
+                    f2.close()
+                mses.append(rmse(g_img, fx) ** 2)
+            if not same:
                 sc2.close()
             n_fix = fx["fixture_spp"]
             e2 = float(fx["var"].mean()) / n_fix
-            rg = rmse(g_img, fx)
+            mse_g = float(np.mean(mses))
             entry["rmse"] = {"fixture": fx_name, "fixture_spp": n_fix, "pixels": int((x1 - x0) * (y1 - y0)),
-                             "gpu": round(rg, 6), "gpu_spp": spp,
+                             "gpu": round(mse_g ** 0.5, 6), "gpu_spp": spp, "gpu_renders": k_seeds,
+                             "gpu_per_render": [round(m ** 0.5, 6) for m in mses],
                              "gpu_expected_unbiased": round(float(np.sqrt(fx["var"].mean() * (1 / spp + 1 / n_fix))), 6)}
         if not a.no_cpu:
             import oracle as O
@@ -368,10 +384,14 @@ def pt_section(a, log):
                 cw, ch = min(w, 128), min(h, 128)
                 dc, rect = desc, ((w - cw) // 2, (h - ch) // 2, (w + cw) // 2, (h + ch) // 2)
             os_ = O.OracleScene(dc)
+            rw, rh = rect[2] - rect[0], rect[3] - rect[1]
+            tile = 64  # the reference's tile size, halved until every host thread has tiles to render
+            while tile > 8 and ((rw + tile - 1) // tile) * ((rh + tile - 1) // tile) < 4 * cores:
+                tile //= 2
             pc = P.sampling_params(dc.image_size[0], dc.image_size[1], cfg["max_depth"], cfg["rr_min_bounces"],
-                                   c_spp, max_passes=1, seed=7, integrator=integ)
-            npx = (rect[2] - rect[0]) * (rect[3] - rect[1])
-            acc, reps = 0.0, []
+                                   c_spp, max_passes=1, seed=7, integrator=integ, tile_size=tile)
+            npx = rw * rh
+            reps, cmses = [], []
             for k in range(a.cpu_repeats):
                 t0 = time.perf_counter()
                 r = os_.render(pc, c_spp, k * c_spp, rect=rect, nthreads=cores, splats=integ == BD)
@@ -380,24 +400,24 @@ def pt_section(a, log):
                 if integ == BD:
                     x0, y0, x1, y1 = rect
                     part = part + r[3][y0:y1, x0:x1]
-                acc = acc + part
+                if fx is not None:
+                    cmses.append(rmse(part / c_spp, fx) ** 2)
             spp1 = max(1, c_spp // cores)
             t1 = timed(lambda: os_.render(pc, spp1, 0, rect=rect, nthreads=1, splats=integ == BD))
             cpu_sps = npx * c_spp / trimmed_mean(reps)
             entry["cpu_samples_per_s"] = round(cpu_sps, 1)
             entry["cpu_one_thread_samples_per_s"] = round(npx * spp1 / t1, 1)
-            entry["cpu_sample"] = (f"{rect[2] - rect[0]}x{rect[3] - rect[1]} pixels of the {dc.image_size[0]}x"
-                                   f"{dc.image_size[1]} image, {c_spp} spp per run on {cores} threads, trimmed mean "
-                                   f"of {len(reps)} runs ({', '.join(f'{t:.2f}' for t in reps)} s); one thread: "
+            entry["cpu_sample"] = (f"{rw}x{rh} pixels of the {dc.image_size[0]}x{dc.image_size[1]} image in "
+                                   f"{tile}x{tile} tiles, {c_spp} spp per run on {cores} threads, trimmed mean of "
+                                   f"{len(reps)} runs ({', '.join(f'{t:.2f}' for t in reps)} s); one thread: "
                                    f"{spp1} spp, {t1:.2f} s")
             entry["speedup_equal_spp"] = round(gpu_sps / cpu_sps, 1)
             if fx is not None:
-                n_c = c_spp * len(reps)
-                rc = rmse(acc / n_c, fx)
-                rg = entry["rmse"]["gpu"]
-                entry["rmse"].update({"cpu": round(rc, 6), "cpu_spp": n_c})
-                if rg * rg > e2 and rc * rc > e2:
-                    need = n_c * (rc * rc - e2) / (rg * rg - e2)
+                mse_c = float(np.mean(cmses))
+                entry["rmse"].update({"cpu": round(mse_c ** 0.5, 6), "cpu_spp": c_spp, "cpu_renders": len(cmses),
+                                      "cpu_per_render": [round(m ** 0.5, 6) for m in cmses]})
+                if mse_g > e2 and mse_c > e2:
+                    need = c_spp * (mse_c - e2) / (mse_g - e2)
                     entry["rmse"]["cpu_spp_for_gpu_rmse"] = round(need, 1)
                     t_cpu = need * npx / cpu_sps
                     t_gpu = spp * npx / gpu_sps

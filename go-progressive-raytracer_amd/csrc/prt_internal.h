@@ -40,6 +40,8 @@ struct prt_scene {
     int32_t max_stack = 0;
     int32_t num_cus = 0;
     int trace_variant[2] = {-1, -1};  // autotuned kernel variant (closest, any)
+    std::atomic<int> pt_trace_variant{-1};  // autotuned occupancy variant of the path tracer's trace kernel
+    float pt_trace_ms[4] = {0, 0, 0, 0};    // its timing of each variant (first bounce of the tuning chunk)
     int forced_variant = -1;          // PRT_TRACE_VARIANT override
     int variant_blocks[16] = {0};
     std::vector<uint32_t> orig_id;  // host copy: leaf order -> original primitive id

@@ -168,8 +168,15 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, P
 // visits, box tests and primitive tests, then the same four for shadow rays.
 constexpr int kProbeCounts = 8;
 
-template <bool MIXED, bool COUNT = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(DevScene sc, PathBuffers pb,
+// Occupancy variants of the trace kernel the per-scene autotuner chooses from: the kernel compiled for at
+// least 7 (the default) or 8 waves per SIMD, and the default with its resident work-groups per CU capped at
+// 6 or 5 by unused dynamic LDS. The same traversal, so every variant gives the same hits.
+constexpr int kTraceVariants = 4;
+constexpr int kTraceKernelWaves[kTraceVariants] = {7, 8, 7, 7};
+constexpr int kTraceGroupCap[kTraceVariants] = {0, 0, 6, 5};  // 0: as resources allow
+
+template <bool MIXED, bool COUNT = false, int WAVES = 7>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(DevScene sc, PathBuffers pb,
                                                                                          int bounce, int closest, int sq,
                                                   uint2* gstack, uint32_t gstride,
                                                   unsigned long long* probe = nullptr) {
@@ -856,8 +863,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     }
 }
 
-// Film (PixelStats.AddSample, stats.go:41-47): per pixel, the chunk's samples are summed in
-// sample order, then added to the running sums with Kahan compensation.
+// Film (PixelStats.AddSample, stats.go:41-47): per pixel, each sample is added to the running sums with
+// Kahan compensation, in sample order.
 struct FilmBuffers {
     float4* sum;   // rgb sum, luminance sum
     float4* comp;  // Kahan compensation of sum
@@ -900,25 +907,20 @@ __global__ __launch_bounds__(kBlock) void k_film_rgba8(int32_t width, int32_t x0
 __global__ __launch_bounds__(kBlock) void k_film(PassParams pp, PathBuffers pb, FilmBuffers fb) {
     int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= pp.npix) return;
-    float r = 0, g = 0, b = 0, l = 0, l2 = 0;
-    float cr = 0, cg = 0, cb = 0, cl = 0, cl2 = 0;
-    for (int s = 0; s < pp.spp; s++) {
-        float4 v = pb.L[(size_t)s * pp.npix + p];
-        float lv = 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z;
-        kahan(r, cr, v.x);
-        kahan(g, cg, v.y);
-        kahan(b, cb, v.z);
-        kahan(l, cl, lv);
-        kahan(l2, cl2, lv * lv);
-    }
-    int px = pp.px0 + p;
+    // every sample is added to the running sums with its own Kahan step, in sample order: the film depends
+    // only on the pixel's sequence of samples, not on how a pass's samples are grouped into chunks
+    const int px = pp.px0 + p;
     float4 S = fb.sum[px], C = fb.comp[px];
     float2 Q = fb.sq[px];
-    kahan(S.x, C.x, r);
-    kahan(S.y, C.y, g);
-    kahan(S.z, C.z, b);
-    kahan(S.w, C.w, l);
-    kahan(Q.x, Q.y, l2);
+    for (int s = 0; s < pp.spp; s++) {
+        const float4 v = pb.L[(size_t)s * pp.npix + p];
+        const float lv = 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z;
+        kahan(S.x, C.x, v.x);
+        kahan(S.y, C.y, v.y);
+        kahan(S.z, C.z, v.z);
+        kahan(S.w, C.w, lv);
+        kahan(Q.x, Q.y, lv * lv);
+    }
     fb.sum[px] = S;
     fb.comp[px] = C;
     fb.sq[px] = Q;
@@ -1036,6 +1038,8 @@ struct prt_film {
     uint2* gstack = nullptr;
     uint32_t gslots = 0;
     int blocks = 0;
+    int trace_grid[4] = {0, 0, 0, 0};  // persistent grid of each trace-kernel variant
+    size_t trace_pad[4] = {0, 0, 0, 0};  // its dynamic LDS
     // BDPT: camera and light subpath stores and subpath lengths, per path slot
     bool bdpt = false;
     int bdpt_blocks = 0;
@@ -1185,6 +1189,39 @@ constexpr size_t kPixelBytes = 2 * sizeof(float4) + sizeof(float2) + sizeof(uint
 
 int64_t film_capacity(int64_t npix, int64_t cap) {
     return std::max<int64_t>(std::min<int64_t>(npix * 64, cap), std::min<int64_t>(npix, cap));
+}
+
+template <bool MIXED, bool COUNT>
+const void* trace_fn_of(int v) {
+    if (kTraceKernelWaves[v] == 8) return (const void*)k_trace<MIXED, COUNT, 8>;
+    return (const void*)k_trace<MIXED, COUNT, 7>;
+}
+const void* trace_fn(bool mixed, bool count, int v) {
+    if (mixed) return count ? trace_fn_of<true, true>(v) : trace_fn_of<true, false>(v);
+    return count ? trace_fn_of<false, true>(v) : trace_fn_of<false, false>(v);
+}
+
+// Dynamic LDS (unused by the kernel) that caps a variant at kTraceGroupCap work-groups per CU.
+size_t trace_lds_pad(int device, int v) {
+    if (!kTraceGroupCap[v]) return 0;
+    int cu_lds = 0;
+    if (hipDeviceGetAttribute(&cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||
+        cu_lds <= 0)
+        cu_lds = 160 * 1024;
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, trace_fn(false, false, v)) != hipSuccess) return 0;
+    const size_t per_group = ((size_t)cu_lds / (size_t)kTraceGroupCap[v]) & ~(size_t)511;
+    return per_group > fa.sharedSizeBytes ? per_group - fa.sharedSizeBytes : 0;
+}
+
+// Persistent grid of a trace variant: the work-groups it keeps resident (at most 8 per CU).
+int trace_grid(const prt_scene* scene, int v) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_fn(scene->dev.mixed != 0, false, v), kBlock,
+                                                     trace_lds_pad(scene->device, v)) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    return std::max(1, scene->num_cus) * std::min(per_cu, 8);
 }
 
 int film_blocks(const prt_scene* scene) {
@@ -1450,23 +1487,66 @@ int film_enqueue_chunk(prt_film* f) {
     // bounce b: one trace launch (the shadow rays of bounce b - 1 and the extension rays of bounce b),
     // then shading; a last launch traces the shadow rays of the last bounce
     const bool counting = (f->probe_flags & PRT_PROBE_COUNT) != 0;
-    auto trace = [&](int b, int closest, int sq) {
-        const void* fn = counting ? (sc.mixed ? (const void*)k_trace<true, true> : (const void*)k_trace<false, true>)
-                                  : (sc.mixed ? (const void*)k_trace<true> : (const void*)k_trace<false>);
+    auto launch_trace = [&](int v, int b, int closest, int sq) {
         PathBuffers pbv = f->pb;
         uint2* gs = f->gstack;
         uint32_t gsl = f->gslots;
         unsigned long long* pr = f->probe_dev;
         void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b,  (void*)&closest,
                         (void*)&sq, (void*)&gs,  (void*)&gsl, (void*)&pr};
+        return hipLaunchKernel(trace_fn(sc.mixed != 0, counting, v), dim3(f->trace_grid[v]), dim3(kBlock), args,
+                               f->trace_pad[v], st);
+    };
+    // per-scene autotuning of the trace kernel's occupancy: on the first chunk of at least 2^20 paths, the
+    // first bounce's trace (which only writes the hit records of the identity queue, so it may run several
+    // times) is timed with every variant, twice; the fastest stays the scene's choice (the default unless
+    // another is more than 3 % faster)
+    prt_scene* ms = const_cast<prt_scene*>(s);
+    int tv = ms->pt_trace_variant.load();
+    bool traced0 = false;
+    if (tv < 0 && n >= ((int64_t)1 << 20) && !f->probe_flags && pp.max_depth > 0) {
+        hipEvent_t ev[2 * kTraceVariants + 1] = {};
+        hipError_t e = hipSuccess;
+        for (auto& x : ev)
+            if (e == hipSuccess) e = hipEventCreate(&x);
+        if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+        for (int r = 0; r < 2 && e == hipSuccess; r++)
+            for (int v = 0; v < kTraceVariants && e == hipSuccess; v++) {
+                e = launch_trace(v, 0, 1, -1);
+                if (e == hipSuccess) e = hipEventRecord(ev[1 + r * kTraceVariants + v], st);
+            }
+        if (e == hipSuccess) e = hipEventSynchronize(ev[2 * kTraceVariants]);
+        float best = 1e30f;
+        int bestv = 0;
+        for (int v = 0; v < kTraceVariants && e == hipSuccess; v++) {
+            float t = 0;
+            e = hipEventElapsedTime(&t, ev[kTraceVariants + v], ev[1 + kTraceVariants + v]);
+            ms->pt_trace_ms[v] = t;
+            const float score = v == 0 ? t : t * 1.03f;
+            if (score < best) {
+                best = score;
+                bestv = v;
+            }
+        }
+        for (auto& x : ev)
+            if (x) (void)hipEventDestroy(x);
+        PRT_HIP(e);
+        int expect = -1;
+        ms->pt_trace_variant.compare_exchange_strong(expect, bestv);
+        tv = ms->pt_trace_variant.load();
+        ms->info.pt_trace_waves = f->trace_grid[tv] / std::max(1, s->num_cus);  // work-groups of 4 waves per CU
+        traced0 = true;
+    }
+    if (tv < 0) tv = 0;
+    auto trace = [&](int b, int closest, int sq) {
         hipError_t e = probe_mark(f, PRT_PROBE_KIND_TRACE, true);
-        if (e == hipSuccess) e = hipLaunchKernel(fn, dim3(f->blocks), dim3(kBlock), args, 0, st);
+        if (e == hipSuccess) e = launch_trace(tv, b, closest, sq);
         if (e == hipSuccess) e = probe_mark(f, PRT_PROBE_KIND_TRACE, false);
         if (f->probe_flags) f->probe_launches++;
         return e;
     };
     for (int b = 0; b < pp.max_depth; b++) {
-        PRT_HIP(trace(b, 1, b > 0 ? ((b - 1) & 1) : -1));
+        if (!(b == 0 && traced0)) PRT_HIP(trace(b, 1, b > 0 ? ((b - 1) & 1) : -1));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_SHADE, true));
         hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_SHADE, false));
@@ -1566,6 +1646,7 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     const bool uv = scene->dev.textured != 0;
     int64_t C = film_capacity(npix, bd ? bdpt_capacity(p->max_depth, cap, uv) : cap);
     int blocks = film_blocks(scene);
+    for (int v = 0; v < kTraceVariants; v++) blocks = std::max(blocks, trace_grid(scene, v));
     size_t g = film_gstack_entries(scene) * (size_t)blocks * kBlock * sizeof(uint2);
     size_t b = 0;
     if (bd) b = 2 * bdpt_store_bytes(p->max_depth, C, uv) + sizeof(uint32_t) * C + 3 * sizeof(long long) * npix;
@@ -1639,7 +1720,13 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     FALLOC(f->fb.count, uint32_t, f->npix);
 #undef FALLOC
     f->blocks = film_blocks(scene);
-    f->gslots = (uint32_t)f->blocks * kBlock;
+    int slot_blocks = f->blocks;
+    for (int v = 0; v < kTraceVariants; v++) {
+        f->trace_grid[v] = trace_grid(scene, v);
+        f->trace_pad[v] = trace_lds_pad(scene->device, v);
+        slot_blocks = std::max(slot_blocks, f->trace_grid[v]);
+    }
+    f->gslots = (uint32_t)slot_blocks * kBlock;
     if ((rc = film_alloc(f, film_gstack_entries(scene) * f->gslots * sizeof(uint2), &ptr)) != PRT_OK) return bail(rc);
     f->gstack = (uint2*)ptr;
     if (p->integrator == PRT_INTEGRATOR_BDPT) {

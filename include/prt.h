@@ -236,7 +236,9 @@ typedef struct prt_scene_info {
     double world_radius;
     int64_t device_bytes;
     int32_t trace_variant;  /* traversal kernel variant picked by the autotuner (-1: not tuned yet) */
-    int32_t pad;
+    int32_t pt_trace_waves; /* resident waves per SIMD of the path tracer's trace kernel variant picked
+                               for this scene by timing the variants on the first render chunk of >= 2^20
+                               paths (0: not tuned yet) */
     int64_t reference_primitives; /* Scene.GetPrimitiveCount (scene.go:81-99): mesh triangles + 1 per
                                      other shape (a box counts once, a disc spot light's disc too) */
 } prt_scene_info;

@@ -103,10 +103,10 @@ def test_deterministic_and_reset():
     pr.render_samples(2)
     b = pr.read()[0]
     sb = pr.splats()
-    # sample sums are per-chunk Kahan sums (order may differ between 4 and 2+2); splats are exact
-    # fixed-point sums of the same contributions
+    # each sample is added with its own compensated step in sample order and splats are exact fixed-point
+    # sums: 4 samples at once and 2 + 2 give the same film
     assert np.array_equal(sa, sb)
-    assert np.allclose(a, b, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(a, b)
     pr2 = P.ProgressiveRaytracer(scene, params_for(s, cfg, 4))
     pr2.render_samples(4)
     assert np.array_equal(pr2.read()[0], a) and np.array_equal(pr2.splats(), sa)

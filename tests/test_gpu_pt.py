@@ -208,3 +208,20 @@ def test_probe_counts_and_times():
     one.probe(P.PROBE_COUNT)
     one.render_samples(3)
     assert one.probe(0)["extension_rays"] == w * h * 3
+
+
+def test_film_independent_of_chunking():
+    """8 spp rendered in one call, as 4 + 4, and as eight single samples give bit-identical films (every
+    sample enters the compensated sums on its own, in sample order)."""
+    s, cfg = scenes.cornell_metre(width=64)
+    scene = P.Scene(s)
+    films = []
+    for split in ([8], [4, 4], [1] * 8):
+        pr = P.ProgressiveRaytracer(scene, params_for(s, cfg, 8, seed=4))
+        for k in split:
+            pr.render_samples(k)
+        films.append(pr.read())
+        pr.close()
+    for other in films[1:]:
+        for x, y in zip(films[0], other):
+            assert np.array_equal(x, y)
