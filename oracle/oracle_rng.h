@@ -1,6 +1,6 @@
 // Counter-based RNG shared in definition (not in code) with the device path.
 // TEST INFRASTRUCTURE ONLY: part of the oracle; the product has its own copy of the
-// definition in go-progressive-raytracer_amd/csrc/prt_rng.h.
+// definition in go-progressive-raytracer_amd/csrc/prt_common.h (path_key, draw).
 //
 // Keyed by (seed, pixel, sample); a draw is addressed by a stream position
 // pos = slot * 16 + k where slot 0 is the camera sample and slot b+1 is bounce b.
