@@ -128,8 +128,9 @@ def main():
         roofline.update({"traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel variant at this "
                                            "ray count, calibrated against a copy kernel (profiles/)",
                          "hbm_measured_GBps": round(hbm, 1), "hbm_measured_frac": round(hbm / HBM_PEAK_GBS, 4),
-                         "limiter": "dependent node fetches at 6 waves/SIMD (80 VGPRs) with the VALU ~60 % busy "
-                                    "(a 4-wide node step is ~136 VALU instructions, a leaf step ~100); "
+                         "limiter": "dependent node fetches at 6 waves/SIMD (80 VGPRs) with the VALU ~67 % busy "
+                                    "(a 4-wide node step is ~136 VALU instructions, a leaf step ~100; leaves "
+                                    "gated to 2/8 of a wave); "
                                     f"measured HBM traffic runs at {hbm / 1e3:.2f} TB/s"})
 
     cpu = None
@@ -299,8 +300,14 @@ def pt_section(a, log):
         params = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp, max_passes=1, seed=1,
                                    integrator=integ)
         pr = P.ProgressiveRaytracer(scene, params)
-        pr.render_samples(spp)  # warm-up (kernels, first-touch, the scene's trace-kernel autotuning)
-        pr.reset()
+        # warm-up (kernels, first touch) until the scene's autotuner has settled: it times launches inside a
+        # chunk other than the film's first and a later chunk reads them
+        tuned = lambda: (scene.info().pt_trace_leaf_gate if integ == PT else scene.info().bdpt_trace_leaf_gate) >= 0
+        for _ in range(4):
+            pr.render_samples(spp)
+            pr.reset()
+            if tuned():
+                break
         t0 = time.perf_counter()
         st = pr.render_samples(spp)
         wall = time.perf_counter() - t0
@@ -314,8 +321,12 @@ def pt_section(a, log):
         kernels = {k: round(v, 2) for k, v in kt.items() if k.endswith("_ms") and v > 0}
         entry = {"config": name, "samples_per_s": round(gpu_sps, 1), "render_s": round(wall, 3),
                  "device_ms": round(st.device_ms, 2), "kernel_ms": kernels}
-        if integ == PT:  # the trace-kernel occupancy the per-scene autotuner picked on the first large chunk
-            entry["autotune"] = {"trace_waves_per_simd": int(scene.info().pt_trace_waves)}
+        info = scene.info()  # the choices the per-scene autotuners made on the first large chunk
+        if integ == PT:
+            entry["autotune"] = {"trace_waves_per_simd": int(info.pt_trace_waves),
+                                 "leaf_gate_eighths": int(info.pt_trace_leaf_gate)}
+        else:
+            entry["autotune"] = {"leaf_gate_eighths": int(info.bdpt_trace_leaf_gate)}
         if integ == PT:
             pr.reset()
             pr.render_samples(spp)

@@ -4,6 +4,7 @@
 
 #include <string>
 #include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "../../include/prt.h"
@@ -27,6 +28,16 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// A per-scene autotuning job: timing launches enqueued with events on the stream of the film that
+// started it. No host thread waits for them; a later chunk of any film of the scene reads the events once
+// the last has completed and records the winner.
+struct TuneJob {
+    std::mutex mu;
+    bool pending = false;
+    int runs = 0;              // timed launches per round; events ev[0 .. 2 * runs]
+    hipEvent_t ev[24] = {};
+};
+
 }  // namespace prt
 
 struct prt_scene {
@@ -40,8 +51,10 @@ struct prt_scene {
     int32_t max_stack = 0;
     int32_t num_cus = 0;
     int trace_variant[2] = {-1, -1};  // autotuned kernel variant (closest, any)
-    std::atomic<int> pt_trace_variant{-1};  // autotuned occupancy variant of the path tracer's trace kernel
-    float pt_trace_ms[4] = {0, 0, 0, 0};    // its timing of each variant (first bounce of the tuning chunk)
+    int pt_trace_variant = 0;               // occupancy variant of the path tracer's trace kernel (PRT_PT_TRACE_VARIANT)
+    std::atomic<int> pt_trace_gate{-1};     // autotuned leaf gate of the path tracer's trace kernel (eighths)
+    std::atomic<int> bdpt_trace_gate{-1};   // autotuned leaf gate of the BDPT kernels' traversals (eighths)
+    prt::TuneJob pt_tune, bdpt_tune;
     int forced_variant = -1;          // PRT_TRACE_VARIANT override
     int variant_blocks[16] = {0};
     std::vector<uint32_t> orig_id;  // host copy: leaf order -> original primitive id

@@ -55,6 +55,8 @@ struct PassParams {
     int32_t sample_base;    // first sample index (uniform films)
     int32_t max_depth, rr_min;
     uint64_t seed;
+    int32_t trace_gate;     // leaf gate of the traversals inside the BDPT kernels (eighths of a wave, 0 = none)
+    int32_t tune_stride;    // the subpath kernel generates every tune_stride-th slot only (autotuning runs), else 1
     // adaptive rounds: pixels come from an active list and each pixel's next sample index is its count
     const uint32_t* act;
     const uint32_t* count;
@@ -168,10 +170,15 @@ __global__ __launch_bounds__(kBlock) void k_camera(DevScene sc, PassParams pp, P
 // visits, box tests and primitive tests, then the same four for shadow rays.
 constexpr int kProbeCounts = 8;
 
-// Occupancy variants of the trace kernel the per-scene autotuner chooses from: the kernel compiled for at
-// least 7 (the default) or 8 waves per SIMD, and the default with its resident work-groups per CU capped at
-// 6 or 5 by unused dynamic LDS. The same traversal, so every variant gives the same hits.
+// Occupancy variants of the trace kernel: the kernel compiled for at least 7 (the default) or 8 waves per
+// SIMD, and the default with its resident work-groups per CU capped at 6 or 5 by unused dynamic LDS. The
+// same traversal, so every variant gives the same hits. A scene uses the default unless
+// PRT_PT_TRACE_VARIANT pins another (timing them on a sample of one bounce's rays picked variants that were
+// slower over whole renders).
 constexpr int kTraceVariants = 4;
+// Leaf gates (eighths of a wave's running lanes, see leaf_gate) the autotuners choose from; 0 first.
+constexpr int kGateChoices = 3;
+constexpr int kGateChoice[kGateChoices] = {0, 4, 8};
 constexpr int kTraceKernelWaves[kTraceVariants] = {7, 8, 7, 7};
 constexpr int kTraceGroupCap[kTraceVariants] = {0, 0, 6, 5};  // 0: as resources allow
 
@@ -179,12 +186,14 @@ template <bool MIXED, bool COUNT = false, int WAVES = 7>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(DevScene sc, PathBuffers pb,
                                                                                          int bounce, int closest, int sq,
                                                   uint2* gstack, uint32_t gstride,
-                                                  unsigned long long* probe = nullptr) {
+                                                  unsigned long long* probe = nullptr, int gate = 0,
+                                                  uint32_t tune_stride = 1) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
     const uint32_t cur = (uint32_t)bounce & 1u;
-    const uint32_t nc = closest ? *qlen(pb, cur) : 0u;
+    // autotuning runs (tune_stride > 1) trace every tune_stride-th extension ray of the queue only
+    const uint32_t nc = closest ? (*qlen(pb, cur) + tune_stride - 1u) / tune_stride : 0u;
     const uint32_t ns = sq >= 0 ? *qlen(pb, 2u + (uint32_t)sq) : 0u;
     const uint32_t* qc = pb.queue[cur];
     const uint32_t* qs = pb.squeue[sq > 0 ? 1 : 0];
@@ -217,7 +226,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
             if (any) rs++;
             else re++;
         }
-        const uint32_t slot = any ? qs[i] : qc[i];
+        const uint32_t slot = any ? qs[i] : qc[i * tune_stride];
         const float4 o = pb.o[slot];
         const float4 d = any ? pb.sd[slot] : pb.d[slot];
         TRay r;
@@ -231,14 +240,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
         THit h;
         if (any) {
             r.tmax = d.w;
-            if (!trace<true, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &cs)) {
+            if (!trace<true, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &cs, gate)) {
                 float4 c = pb.sc[slot];
                 float4 l = pb.L[slot];
                 pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
             }
         } else {
             r.tmax = __int_as_float(0x7f800000);
-            trace<false, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &ce);
+            trace<false, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &ce, gate);
             pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
         }
     }
@@ -409,7 +418,7 @@ __device__ double prim_t_f64(const DevScene& sc, uint32_t prim, V3 o_hi, const f
 template <bool MIXED>
 __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i, int n, V3 o, V3 d, V3 beta,
                            float pdf_fwd, int max_bounces, bool camera_path, uint32_t key, uint32_t slot0,
-                           TStack<kLds, kBlock>& st, const float* point_lo) {
+                           TStack<kLds, kBlock>& st, const float* point_lo, int gate) {
     for (int b = 0; b < max_bounces; b++) {
         BV prev = VS.get(n - 1, i);
         THit h;
@@ -418,7 +427,7 @@ __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i
         float tmin = redecide ? 0.0f : 0.001f;
         for (int k = 0;; k++) {
             TRay r{o.x, o.y, o.z, tmin, d.x, d.y, d.z, __int_as_float(0x7f800000)};
-            hit = trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
+            hit = trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr, gate);
             if (!redecide || !hit || h.t >= 0.01f || k == 4) break;
             if (prim_t_f64(sc, h.prim, o, point_lo, d, h.t) >= 0.001) break;
             const float next = __uint_as_float(__float_as_uint(h.t) + 1u);
@@ -500,9 +509,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     load_node_cache<kNodeCache, kBlock>(sc, s_cache);
     st.ncache = s_cache;
     const DevCamera& cam = sc.cam;
-    const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
+    const uint32_t ts = (uint32_t)pp.tune_stride;
+    const uint32_t n = ((uint32_t)pp.npix * (uint32_t)pp.spp + ts - 1u) / ts;
     const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
+        const uint32_t i = j * ts;
         uint32_t pix, sample;
         slot_sample(pp, i, pix, sample);
         int px = (int)(pix % (uint32_t)pp.width), py = (int)(pix / (uint32_t)pp.width);
@@ -522,7 +533,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         cv.flags = VF_CAMERA;
         CP.put(0, i, cv);
         const uint32_t nc = (uint32_t)bdpt_extend<MIXED>(sc, CP, i, 1, o, d, v3(1, 1, 1), cam_dir_pdf(cam, o, d),
-                                                         pp.max_depth, true, key, 1, st, nullptr);
+                                                         pp.max_depth, true, key, 1, st, nullptr, pp.trace_gate);
         // light subpath (generateLightPath, bdpt.go:139-203)
         uint32_t nl = 0;
         if (sc.num_lights > 0) {
@@ -549,7 +560,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             V3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
             nl = (uint32_t)bdpt_extend<MIXED>(sc, LP, i, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false,
                                               key, kLightSlot + 1, st,
-                                              L.type == PRT_LIGHT_POINT_SPOT ? L.from_lo : nullptr);
+                                              L.type == PRT_LIGHT_POINT_SPOT ? L.from_lo : nullptr, pp.trace_gate);
             if (inf) {  // infinite-light density fix-ups (bdpt.go:182-199)
                 if (nl > 1) {
                     BV fb = LP.get(1, i);
@@ -745,7 +756,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             }
             if (vis) {
                 THit vh;
-                if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr)) continue;
+                if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr, pp.trace_gate)) continue;
             }
             if (Lc.x == 0.0f && Lc.y == 0.0f && Lc.z == 0.0f && !splatted) continue;
             float w = mis_weight(sc, cam, CP, LP, i, sampled, ct1, ls1, s, t);
@@ -850,7 +861,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                             // across the traversal
                             const V3 lw = contrib * mis_general(sc, cam, ct1, ct2, ls1, ls2, c, l_tail, l_fwd2, t);
                             THit vh;
-                            if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr))
+                            if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr,
+                                                                                     pp.trace_gate))
                                 total = total + lw;
                         }
                     }
@@ -1403,6 +1415,34 @@ hipError_t probe_mark(prt_film* f, int kind, bool begin) {
     return hipEventRecord(e, f->stream);
 }
 
+// Starts a tuning job of `runs` timed launches on stream st: ev[0] is recorded here; the caller then
+// enqueues one warm-up launch, records ev[1], and records ev[2 + k] after timed launch k.
+static hipError_t tune_begin(TuneJob& j, int runs, hipStream_t st) {
+    if (runs + 2 > (int)(sizeof(j.ev) / sizeof(j.ev[0]))) return hipErrorInvalidValue;
+    for (int k = 0; k < runs + 2; k++)
+        if (!j.ev[k]) {
+            hipError_t e = hipEventCreate(&j.ev[k]);
+            if (e != hipSuccess) return e;
+        }
+    j.runs = runs;
+    return hipEventRecord(j.ev[0], st);
+}
+
+// Device time of each timed launch of a pending job, once its last event has completed; false while it
+// runs or when no job is pending (a job whose events report an error is dropped).
+static bool tune_collect(TuneJob& j, float* t) {
+    if (!j.pending) return false;
+    hipError_t q = hipEventQuery(j.ev[j.runs + 1]);
+    if (q == hipErrorNotReady) return false;
+    j.pending = false;
+    if (q != hipSuccess) return false;
+    for (int k = 0; k < j.runs; k++) {
+        t[k] = 0.0f;
+        if (hipEventElapsedTime(&t[k], j.ev[1 + k], j.ev[2 + k]) != hipSuccess) return false;
+    }
+    return true;
+}
+
 // Enqueues the next planned chunk on the film's stream (no wait); ev0/ev1 bracket it.
 int film_enqueue_chunk(prt_film* f) {
     if (film_round_exhausted(f)) return PRT_OK;
@@ -1423,6 +1463,8 @@ int film_enqueue_chunk(prt_film* f) {
     pp.max_depth = f->params.max_depth;
     pp.rr_min = f->params.rr_min_bounces;
     pp.seed = f->params.seed;
+    pp.trace_gate = 0;
+    pp.tune_stride = 1;
     pp.act = f->adaptive ? f->act[f->ad_cur] : nullptr;
     pp.count = f->fb.count;
     AdaptiveParams ap{};
@@ -1441,13 +1483,55 @@ int film_enqueue_chunk(prt_film* f) {
         int grid = (int)std::min<int64_t>(f->bdpt_blocks, (n + kBlock - 1) / kBlock);
         int grid_g = (int)std::min<int64_t>(f->bdpt_general_blocks, (n + kBlock - 1) / kBlock);
         const int walk_blocks = (int)((n + kBlock - 1) / kBlock);
+        auto launch_paths = [&](const PassParams& q) {
+            if (sc.mixed)
+                hipLaunchKernelGGL((k_bdpt_paths<true>), dim3(grid), dim3(kBlock), 0, st, sc, q, f->cpath, f->lpath,
+                                   f->lens, f->gstack, f->gslots);
+            else
+                hipLaunchKernelGGL((k_bdpt_paths<false>), dim3(grid), dim3(kBlock), 0, st, sc, q, f->cpath,
+                                   f->lpath, f->lens, f->gstack, f->gslots);
+            return hipGetLastError();
+        };
+        // per-scene leaf gate of the BDPT traversals, on a chunk of at least 2^18 paths other than the film's
+        // first (which may be a latency-bound first pass): after a warm-up launch, the subpath kernel is
+        // timed with each gate on about 2^18 of the chunk's samples (the regular launch writes their
+        // subpaths again). Nothing waits for the timings: a later chunk of a film of the scene reads them
+        // and keeps the fastest gate (gate 0 unless another is more than 3 % faster) for every BDPT kernel.
+        prt_scene* ms = const_cast<prt_scene*>(s);
+        {
+            std::lock_guard<std::mutex> lk(ms->bdpt_tune.mu);
+            float t[kGateChoices];
+            if (tune_collect(ms->bdpt_tune, t)) {
+                float best = t[0];
+                int bestg = 0;
+                for (int g = 1; g < kGateChoices; g++)
+                    if (t[g] * 1.03f < best) {
+                        best = t[g] * 1.03f;
+                        bestg = kGateChoice[g];
+                    }
+                int expect = -1;
+                ms->bdpt_trace_gate.compare_exchange_strong(expect, bestg);
+                ms->info.bdpt_trace_leaf_gate = ms->bdpt_trace_gate.load();
+            }
+            if (ms->bdpt_trace_gate.load() < 0 && !ms->bdpt_tune.pending && n >= ((int64_t)1 << 18) &&
+                !f->probe_flags && !f->adaptive && f->enq > 0) {
+                PassParams q = pp;
+                q.tune_stride = (int32_t)std::max<int64_t>(1, n >> 18);
+                hipError_t e = tune_begin(ms->bdpt_tune, kGateChoices, st);
+                if (e == hipSuccess) e = launch_paths(q);
+                if (e == hipSuccess) e = hipEventRecord(ms->bdpt_tune.ev[1], st);
+                for (int g = 0; g < kGateChoices && e == hipSuccess; g++) {
+                    q.trace_gate = kGateChoice[g];
+                    e = launch_paths(q);
+                    if (e == hipSuccess) e = hipEventRecord(ms->bdpt_tune.ev[2 + g], st);
+                }
+                PRT_HIP(e);
+                ms->bdpt_tune.pending = true;
+            }
+        }
+        pp.trace_gate = std::max(0, ms->bdpt_trace_gate.load());
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_PATHS, true));
-        if (sc.mixed)
-            hipLaunchKernelGGL((k_bdpt_paths<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath,
-                               f->lens, f->gstack, f->gslots);
-        else
-            hipLaunchKernelGGL((k_bdpt_paths<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath,
-                               f->lens, f->gstack, f->gslots);
+        PRT_HIP(launch_paths(pp));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_PATHS, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, true));
         hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
@@ -1487,66 +1571,69 @@ int film_enqueue_chunk(prt_film* f) {
     // bounce b: one trace launch (the shadow rays of bounce b - 1 and the extension rays of bounce b),
     // then shading; a last launch traces the shadow rays of the last bounce
     const bool counting = (f->probe_flags & PRT_PROBE_COUNT) != 0;
-    auto launch_trace = [&](int v, int b, int closest, int sq) {
+    auto launch_trace = [&](int v, int gate, int b, int closest, int sq, uint32_t tstride = 1) {
         PathBuffers pbv = f->pb;
         uint2* gs = f->gstack;
         uint32_t gsl = f->gslots;
         unsigned long long* pr = f->probe_dev;
-        void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b,  (void*)&closest,
-                        (void*)&sq, (void*)&gs,  (void*)&gsl, (void*)&pr};
+        int g = gate;
+        void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b,  (void*)&closest, (void*)&sq,
+                        (void*)&gs, (void*)&gsl, (void*)&pr, (void*)&g,       (void*)&tstride};
         return hipLaunchKernel(trace_fn(sc.mixed != 0, counting, v), dim3(f->trace_grid[v]), dim3(kBlock), args,
                                f->trace_pad[v], st);
     };
-    // per-scene autotuning of the trace kernel's occupancy: on the first chunk of at least 2^20 paths, the
-    // first bounce's trace (which only writes the hit records of the identity queue, so it may run several
-    // times) is timed with every variant, twice; the fastest stays the scene's choice (the default unless
-    // another is more than 3 % faster)
+    // per-scene leaf gate of the trace kernel, on a chunk of at least 2^20 paths other than the film's first
+    // (which may be a latency-bound first pass): between the first bounce's shading and the second bounce's
+    // trace, after a warm-up launch, the second bounce's closest-hit trace of about 2^20 of its rays (it
+    // only writes their hit records, which the regular trace writes again) is timed with each gate.
+    // Nothing waits for the timings: a later chunk of a film of the scene reads them and keeps the
+    // fastest gate (gate 0 unless another is more than 3 % faster).
     prt_scene* ms = const_cast<prt_scene*>(s);
-    int tv = ms->pt_trace_variant.load();
-    bool traced0 = false;
-    if (tv < 0 && n >= ((int64_t)1 << 20) && !f->probe_flags && pp.max_depth > 0) {
-        hipEvent_t ev[2 * kTraceVariants + 1] = {};
-        hipError_t e = hipSuccess;
-        for (auto& x : ev)
-            if (e == hipSuccess) e = hipEventCreate(&x);
-        if (e == hipSuccess) e = hipEventRecord(ev[0], st);
-        for (int r = 0; r < 2 && e == hipSuccess; r++)
-            for (int v = 0; v < kTraceVariants && e == hipSuccess; v++) {
-                e = launch_trace(v, 0, 1, -1);
-                if (e == hipSuccess) e = hipEventRecord(ev[1 + r * kTraceVariants + v], st);
-            }
-        if (e == hipSuccess) e = hipEventSynchronize(ev[2 * kTraceVariants]);
-        float best = 1e30f;
-        int bestv = 0;
-        for (int v = 0; v < kTraceVariants && e == hipSuccess; v++) {
-            float t = 0;
-            e = hipEventElapsedTime(&t, ev[kTraceVariants + v], ev[1 + kTraceVariants + v]);
-            ms->pt_trace_ms[v] = t;
-            const float score = v == 0 ? t : t * 1.03f;
-            if (score < best) {
-                best = score;
-                bestv = v;
-            }
+    const int tv = ms->pt_trace_variant;
+    ms->info.pt_trace_waves = f->trace_grid[tv] / std::max(1, s->num_cus);  // work-groups of 4 waves per CU
+    {
+        std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
+        float t[kGateChoices];
+        if (tune_collect(ms->pt_tune, t)) {
+            float best = t[0];
+            int bestg = 0;
+            for (int g = 1; g < kGateChoices; g++)
+                if (t[g] * 1.03f < best) {
+                    best = t[g] * 1.03f;
+                    bestg = kGateChoice[g];
+                }
+            int expect = -1;
+            ms->pt_trace_gate.compare_exchange_strong(expect, bestg);
+            ms->info.pt_trace_leaf_gate = ms->pt_trace_gate.load();
         }
-        for (auto& x : ev)
-            if (x) (void)hipEventDestroy(x);
-        PRT_HIP(e);
-        int expect = -1;
-        ms->pt_trace_variant.compare_exchange_strong(expect, bestv);
-        tv = ms->pt_trace_variant.load();
-        ms->info.pt_trace_waves = f->trace_grid[tv] / std::max(1, s->num_cus);  // work-groups of 4 waves per CU
-        traced0 = true;
     }
-    if (tv < 0) tv = 0;
+    const bool tune_here = n >= ((int64_t)1 << 20) && !f->probe_flags && pp.max_depth > 1 && !f->adaptive &&
+                           f->enq > 0 && ms->pt_trace_gate.load() < 0;
+    auto tune_bounce1 = [&]() -> hipError_t {
+        std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
+        if (ms->pt_trace_gate.load() >= 0 || ms->pt_tune.pending) return hipSuccess;
+        const uint32_t tstride = (uint32_t)std::max<int64_t>(1, n >> 20);
+        hipError_t e = tune_begin(ms->pt_tune, kGateChoices, st);
+        if (e == hipSuccess) e = launch_trace(tv, 0, 1, 1, -1, tstride);
+        if (e == hipSuccess) e = hipEventRecord(ms->pt_tune.ev[1], st);
+        for (int g = 0; g < kGateChoices && e == hipSuccess; g++) {
+            e = launch_trace(tv, kGateChoice[g], 1, 1, -1, tstride);
+            if (e == hipSuccess) e = hipEventRecord(ms->pt_tune.ev[2 + g], st);
+        }
+        if (e == hipSuccess) ms->pt_tune.pending = true;
+        return e;
+    };
+    const int tgate = std::max(0, ms->pt_trace_gate.load());
     auto trace = [&](int b, int closest, int sq) {
         hipError_t e = probe_mark(f, PRT_PROBE_KIND_TRACE, true);
-        if (e == hipSuccess) e = launch_trace(tv, b, closest, sq);
+        if (e == hipSuccess) e = launch_trace(tv, tgate, b, closest, sq);
         if (e == hipSuccess) e = probe_mark(f, PRT_PROBE_KIND_TRACE, false);
         if (f->probe_flags) f->probe_launches++;
         return e;
     };
     for (int b = 0; b < pp.max_depth; b++) {
-        if (!(b == 0 && traced0)) PRT_HIP(trace(b, 1, b > 0 ? ((b - 1) & 1) : -1));
+        if (b == 1 && tune_here) PRT_HIP(tune_bounce1());
+        PRT_HIP(trace(b, 1, b > 0 ? ((b - 1) & 1) : -1));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_SHADE, true));
         hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_SHADE, false));

@@ -338,7 +338,7 @@ constexpr uint32_t kPfChunk = 256;
 
 // Triangle-only scenes: 6 waves per SIMD (80 VGPRs, the LDS bound of the 8-entry stack and 64-node cache)
 // hide more node-fetch latency than the register count the compiler picks on its own (84, 5 waves).
-template <bool ANY, bool MIXED, int LDS, int NCACHE, int BLOCK>
+template <bool ANY, bool MIXED, int LDS, int NCACHE, int BLOCK, int GATE = 0>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 6))) void k_intersect_pf(DevScene sc, const prt_ray* __restrict__ rays, int64_t n,
                                                         prt_hit* __restrict__ hits, uint8_t* __restrict__ occluded,
                                                         uint2* gstack, uint32_t gstride, unsigned long long* work) {
@@ -409,7 +409,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
             need = __ballot(nidx < 0);
         }
         if (__ballot(active || nidx >= 0) == 0) break;  // exhausted and drained
-        if (active && tstate_step<ANY, MIXED, false, LDS, BLOCK, NCACHE>(sc, s, st, nullptr)) {
+        const bool leaf_ok = GATE > 0 ? leaf_gate(active, s.cur, GATE) : true;
+        if (active && tstate_step<ANY, MIXED, false, LDS, BLOCK, NCACHE>(sc, s, st, nullptr, leaf_ok)) {
             active = false;
             bool hit = s.best != 0xFFFFFFFFu;
             if (ANY) {
@@ -426,26 +427,34 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
 
 // Kernel variants the per-scene autotuner chooses from (identical per-ray traversal): kernel
 // (0 = batch / threshold refill, 1 = prefetched refill), LDS stack entries per lane, refill
-// threshold (64 = whole-wave batches), LDS node-cache size, block.
+// threshold (kernel 0; 64 = whole-wave batches) or leaf gate (kernel 1; a leaf step waits for gate/8 of
+// the running lanes, 0 = no gate), LDS node-cache size, block.
 #define PRT_TRACE_VARIANTS(X)      \
     X(0, 0, 8, 64, 0, 256)         \
-    X(1, 1, 8, 1, 0, 256)          \
-    X(2, 1, 8, 1, 64, 256)         \
-    X(3, 1, 12, 1, 0, 256)         \
-    X(4, 1, 6, 1, 64, 512)
-constexpr int kNumVariants = 5;
+    X(1, 1, 8, 0, 0, 256)          \
+    X(2, 1, 8, 0, 64, 256)         \
+    X(3, 1, 12, 0, 0, 256)         \
+    X(4, 1, 6, 0, 64, 512)         \
+    X(5, 1, 8, 2, 64, 256)         \
+    X(6, 1, 8, 4, 64, 256)         \
+    X(7, 1, 8, 8, 64, 256)
+constexpr int kNumVariants = 8;
 constexpr int kVariantLds[kNumVariants] = {
 #define X(id, kind, lds, refill, nc, blk) lds,
     PRT_TRACE_VARIANTS(X)
 #undef X
 };
 constexpr int kMinLds = 6;
-constexpr int kDefaultVariant = 2;
-constexpr int kVariantBlock[kNumVariants] = {256, 256, 256, 256, 512};
+constexpr int kDefaultVariant = 5;
+constexpr int kVariantBlock[kNumVariants] = {
+#define X(id, kind, lds, refill, nc, blk) blk,
+    PRT_TRACE_VARIANTS(X)
+#undef X
+};
 
 template <bool ANY, bool MIXED, int KIND, int LDS, int REFILL, int NCACHE, int BLOCK>
 const void* kernel_of() {
-    if (KIND == 1) return (const void*)k_intersect_pf<ANY, MIXED, LDS, NCACHE, BLOCK>;
+    if (KIND == 1) return (const void*)k_intersect_pf<ANY, MIXED, LDS, NCACHE, BLOCK, REFILL>;
     return (const void*)k_intersect_refill<ANY, MIXED, LDS, REFILL, NCACHE, BLOCK>;
 }
 
@@ -468,7 +477,7 @@ void launch_variant(int v, int blocks, hipStream_t st, const DevScene& sc, const
 #define X(id, kind, lds, refill, nc, blk)                                                                     \
     case id:                                                                                                \
         if (kind == 1)                                                                                      \
-            hipLaunchKernelGGL((k_intersect_pf<ANY, MIXED, lds, nc, blk>), dim3(blocks), dim3(blk), 0, st, sc, \
+            hipLaunchKernelGGL((k_intersect_pf<ANY, MIXED, lds, nc, blk, refill>), dim3(blocks), dim3(blk), 0, st, sc, \
                                rays, n, hits, occ, gs, gstride, work);                                      \
         else                                                                                                \
             hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, lds, refill, nc, blk>), dim3(blocks),          \
@@ -560,13 +569,13 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
     } else {
         int& tuned = ms->trace_variant[mode];
         if (ms->forced_variant < 0 && tuned < 0 && n >= (1 << 20)) {
-            // autotune: time every variant on the first (up to) 4M rays of this batch, keep the fastest
+            // autotune: time every variant on the first (up to) 8M rays of this batch, keep the fastest
             hipEvent_t a, b;
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             float best = 1e30f;
             int bestv = kDefaultVariant;
-            const int64_t m = std::min<int64_t>(n, (int64_t)1 << 22);
+            const int64_t m = std::min<int64_t>(n, (int64_t)1 << 23);
             for (int v = 0; v < kNumVariants; v++) {
                 float tot = 0;
                 for (int rep = 0; rep < 4; rep++) {
@@ -1241,6 +1250,19 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         if (v >= 0 && v < kNumVariants) s->forced_variant = v;
     }
     s->info.trace_variant = s->forced_variant;
+    s->info.pt_trace_leaf_gate = -1;
+    s->info.bdpt_trace_leaf_gate = -1;
+    if (const char* fp = std::getenv("PRT_PT_TRACE_VARIANT")) {  // pins the path tracer's trace occupancy variant
+        int v = std::atoi(fp);
+        if (v >= 0 && v < 4) s->pt_trace_variant = v;
+    }
+    if (const char* fg = std::getenv("PRT_TRACE_LEAF_GATE")) {  // pins the render kernels' leaf gate (0-8)
+        int g = std::atoi(fg);
+        if (g >= 0 && g <= 8) {
+            s->pt_trace_gate = s->bdpt_trace_gate = g;
+            s->info.pt_trace_leaf_gate = s->info.bdpt_trace_leaf_gate = g;
+        }
+    }
     uint32_t slots = (uint32_t)(max_blocks * kTraceBlock);
     size_t depth = (size_t)std::max(1, s->max_stack - kMinLds + 1);
     if ((rc = scene_alloc(s, depth * slots * sizeof(uint2), nullptr, &p)) != PRT_OK) return bail(rc);
@@ -1263,6 +1285,12 @@ int prt_scene_destroy(prt_scene* scene) {
                     "scene is still used by " + std::to_string(n) + " film(s) or serving request(s)");
     }
     (void)hipSetDevice(scene->device);
+    for (prt::TuneJob* j : {&scene->pt_tune, &scene->bdpt_tune})
+        for (hipEvent_t& e : j->ev)
+            if (e) {
+                (void)hipEventSynchronize(e);
+                (void)hipEventDestroy(e);
+            }
     for (auto& b : scene->bufs) (void)hipFree(b.p);
     delete scene;
     return PRT_OK;

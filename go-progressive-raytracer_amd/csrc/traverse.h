@@ -367,10 +367,24 @@ __device__ __forceinline__ void tstate_result(const TState& s, THit& hit) {
     hit.v = s.bv;
 }
 
-// One traversal step. Returns true when the ray is finished (stack empty, or any hit found).
+// Wave-uniform leaf gate. A wave executes the leaf branch of a step whenever any of its lanes stands at a
+// leaf, and a leaf step costs about as much as a node step; when only a few lanes stand at leaves the rest
+// of the wave pays for them in every step. With the gate, lanes at a leaf wait (keep their state) until at
+// least `eighths`/8 of the wave's running lanes stand at one or no lane has an inner node to test, so the
+// leaf branch runs for more lanes at a time. Each ray still visits the same nodes and leaves in the same
+// order.
+__device__ __forceinline__ bool leaf_gate(bool running, uint32_t cur, int eighths) {
+    const bool at_leaf = running && child_is_leaf(cur);
+    const uint64_t inner = __ballot(running && !at_leaf);
+    const uint64_t leaf = __ballot(at_leaf);
+    return inner == 0 || __popcll(leaf) * 8 >= __popcll(leaf | inner) * eighths;
+}
+
+// One traversal step. Returns true when the ray is finished (stack empty, or any hit found). A lane at
+// a leaf leaves the step without progress when `leaf_ok` is false (see leaf_gate).
 template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK, int NCACHE = 0>
 __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStack<LDS_DEPTH, BLOCK>& st,
-                                            TCounts* cnt) {
+                                            TCounts* cnt, bool leaf_ok = true) {
     const TRay& r = s.r;
     if (!child_is_leaf(s.cur)) {
         // ---- inner node: test the 4 child boxes (top levels from LDS when cached) ----
@@ -443,6 +457,7 @@ __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStac
         }
     } else {
         // ---- leaf: test its primitives ----
+        if (!leaf_ok) return false;
         uint32_t first = leaf_first(s.cur), count = leaf_count(s.cur);
         for (uint32_t k = 0; k < count; k++) {
             const float4* pp = sc.prims + 3 * (size_t)(first + k);
@@ -504,12 +519,15 @@ __device__ __forceinline__ void load_node_cache(const DevScene& sc, float4* cach
 // Closest hit (ANY = false) or any hit (ANY = true) of one ray. Returns true on a hit.
 // Always inlined: its callers have one call site each, and a call would pass the ray, hit and stack
 // through scratch memory (AMDGPU's inliner gives up on large kernels by basic-block count).
+// gate > 0 (wave-uniform) gates leaves (leaf_gate) at gate/8 of the running lanes.
 template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK, int NCACHE = 0>
 __device__ __forceinline__ bool trace(const DevScene& sc, const TRay& r_in, THit& hit, TStack<LDS_DEPTH, BLOCK>& st,
-                                      TCounts* cnt) {
+                                      TCounts* cnt, int gate = 0) {
     TState s;
     tstate_init(s, r_in);
-    while (!tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK, NCACHE>(sc, s, st, cnt)) {
+    for (;;) {
+        const bool ok = gate > 0 ? leaf_gate(true, s.cur, gate) : true;
+        if (tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK, NCACHE>(sc, s, st, cnt, ok)) break;
     }
     tstate_result(s, hit);
     return s.best != 0xFFFFFFFFu;

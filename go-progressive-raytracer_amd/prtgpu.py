@@ -120,7 +120,8 @@ class SceneInfo(C.Structure):
     _fields_ = [("num_primitives", C.c_int64), ("num_triangles", C.c_int64), ("bvh_nodes", C.c_int64),
                 ("bvh_depth", C.c_int32), ("max_stack", C.c_int32), ("world_center", D3),
                 ("world_radius", C.c_double), ("device_bytes", C.c_int64), ("trace_variant", C.c_int32),
-                ("pt_trace_waves", C.c_int32), ("reference_primitives", C.c_int64)]
+                ("pt_trace_waves", C.c_int32), ("reference_primitives", C.c_int64),
+                ("pt_trace_leaf_gate", C.c_int32), ("bdpt_trace_leaf_gate", C.c_int32)]
 
 
 class RenderParams(C.Structure):

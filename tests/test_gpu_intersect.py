@@ -169,3 +169,29 @@ def test_empty_scene_and_edge_cases():
     assert list(h["prim_id"]) == [0, 0, -1, -1]
     assert np.allclose(h["t"][:2], 1.0)
     assert np.allclose(h["u"][0], 0.25) and np.allclose(h["v"][0], 0.25)
+
+
+@pytest.mark.parametrize("name", ["mesh", "default"])
+def test_every_kernel_variant_gives_the_same_hits(name, monkeypatch):
+    """The autotuner's kernel variants (wave batches or prefetched refill, stack depth, LDS node cache,
+    leaf gate) differ only in how lanes are scheduled: closest hits and occlusion are identical."""
+    if name == "mesh":
+        s, info = scenes.mesh_scene(frequency=64)
+        v = info["vertices"]
+        rays = P.random_rays(v.min(0), v.max(0), 1 << 17, scale=2.0, seed=5)
+    else:
+        s, _ = scenes.default_scene()
+        rays = _random_rays_in_box([-3, 0.01, -4], [3, 3, 3], 1 << 17, 13)
+    ref_h = ref_o = None
+    for v in range(8):
+        monkeypatch.setenv("PRT_TRACE_VARIANT", str(v))
+        scene = P.Scene(s)
+        assert scene.info().trace_variant == v
+        h = scene.intersect_closest(rays)
+        o = scene.intersect_any(rays)
+        if ref_h is None:
+            ref_h, ref_o = h, o
+            assert (h["prim_id"] >= 0).mean() > 0.05
+        else:
+            assert np.array_equal(h.view(np.uint32), ref_h.view(np.uint32)), v
+            assert np.array_equal(o, ref_o), v

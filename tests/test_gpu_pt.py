@@ -225,3 +225,48 @@ def test_film_independent_of_chunking():
     for other in films[1:]:
         for x, y in zip(films[0], other):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("integrator", ["pt", "bdpt"])
+def test_leaf_gate_leaves_films_unchanged(integrator, monkeypatch):
+    """The leaf gate of the render kernels' traversals (pinned per scene with PRT_TRACE_LEAF_GATE, else
+    picked by the autotuner) only reschedules lanes: films, splats included, are bit-identical for every
+    gate. The autotuner times launches inside a chunk of >= 2^20 (path tracing) or 2^18 (BDPT) paths other
+    than the film's first, and a later chunk reads the timings: rendering in three calls tunes the scene
+    during the film, and a second film then renders with the tuned gate."""
+    s, cfg = scenes.cornell_metre(width=128)
+    w, h = s.image_size
+    integ = P.INTEGRATOR_BDPT if integrator == "bdpt" else P.INTEGRATOR_PATH
+    per_call = 16 if integrator == "bdpt" else 64  # 128^2 x per_call = 2^18 / 2^20 paths per chunk
+
+    def render(scene):
+        pr = P.ProgressiveRaytracer(scene, P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"],
+                                                             3 * per_call, seed=3, integrator=integ))
+        for _ in range(3):
+            pr.render_samples(per_call)
+        out = pr.read() + (pr.splats(),)
+        pr.close()
+        return out
+
+    def tuned_gate(scene):
+        i = scene.info()
+        return i.bdpt_trace_leaf_gate if integrator == "bdpt" else i.pt_trace_leaf_gate
+
+    films = []
+    for g in ("0", "4", "8"):
+        monkeypatch.setenv("PRT_TRACE_LEAF_GATE", g)
+        scene = P.Scene(s)
+        assert scene.info().pt_trace_leaf_gate == int(g) and scene.info().bdpt_trace_leaf_gate == int(g)
+        films.append(render(scene))
+        assert tuned_gate(scene) == int(g)  # a pinned gate is kept
+    monkeypatch.delenv("PRT_TRACE_LEAF_GATE")
+    scene = P.Scene(s)
+    assert tuned_gate(scene) == -1
+    films.append(render(scene))  # the second call starts the tuning, the third reads it
+    assert tuned_gate(scene) in (0, 4, 8)
+    if integrator == "pt":
+        assert scene.info().pt_trace_waves > 0
+    films.append(render(scene))  # renders with the tuned gate from its first chunk
+    for other in films[1:]:
+        for x, y in zip(films[0], other):
+            assert np.array_equal(x, y)
