@@ -1443,6 +1443,22 @@ static bool tune_collect(TuneJob& j, float* t) {
     return true;
 }
 
+// The leaf gate of a finished tuning job whose timed launches ran every gate twice, interleaved
+// (t[k] = gate kGateChoice[k % kGateChoices]): the fastest by the shorter of its two times; gate 0 unless
+// another is more than 3 % faster.
+static int pick_gate(const float* t) {
+    float best = std::min(t[0], t[kGateChoices]);
+    int bestg = 0;
+    for (int g = 1; g < kGateChoices; g++) {
+        const float tg = std::min(t[g], t[kGateChoices + g]) * 1.03f;
+        if (tg < best) {
+            best = tg;
+            bestg = kGateChoice[g];
+        }
+    }
+    return bestg;
+}
+
 // Enqueues the next planned chunk on the film's stream (no wait); ev0/ev1 bracket it.
 int film_enqueue_chunk(prt_film* f) {
     if (film_round_exhausted(f)) return PRT_OK;
@@ -1494,21 +1510,15 @@ int film_enqueue_chunk(prt_film* f) {
         };
         // per-scene leaf gate of the BDPT traversals, on a chunk of at least 2^18 paths other than the film's
         // first (which may be a latency-bound first pass): after a warm-up launch, the subpath kernel is
-        // timed with each gate on about 2^18 of the chunk's samples (the regular launch writes their
+        // timed twice with each gate on about 2^18 of the chunk's samples (the regular launch writes their
         // subpaths again). Nothing waits for the timings: a later chunk of a film of the scene reads them
         // and keeps the fastest gate (gate 0 unless another is more than 3 % faster) for every BDPT kernel.
         prt_scene* ms = const_cast<prt_scene*>(s);
         {
             std::lock_guard<std::mutex> lk(ms->bdpt_tune.mu);
-            float t[kGateChoices];
+            float t[2 * kGateChoices];
             if (tune_collect(ms->bdpt_tune, t)) {
-                float best = t[0];
-                int bestg = 0;
-                for (int g = 1; g < kGateChoices; g++)
-                    if (t[g] * 1.03f < best) {
-                        best = t[g] * 1.03f;
-                        bestg = kGateChoice[g];
-                    }
+                const int bestg = pick_gate(t);
                 int expect = -1;
                 ms->bdpt_trace_gate.compare_exchange_strong(expect, bestg);
                 ms->info.bdpt_trace_leaf_gate = ms->bdpt_trace_gate.load();
@@ -1517,13 +1527,13 @@ int film_enqueue_chunk(prt_film* f) {
                 !f->probe_flags && !f->adaptive && f->enq > 0) {
                 PassParams q = pp;
                 q.tune_stride = (int32_t)std::max<int64_t>(1, n >> 18);
-                hipError_t e = tune_begin(ms->bdpt_tune, kGateChoices, st);
+                hipError_t e = tune_begin(ms->bdpt_tune, 2 * kGateChoices, st);
                 if (e == hipSuccess) e = launch_paths(q);
                 if (e == hipSuccess) e = hipEventRecord(ms->bdpt_tune.ev[1], st);
-                for (int g = 0; g < kGateChoices && e == hipSuccess; g++) {
-                    q.trace_gate = kGateChoice[g];
+                for (int k = 0; k < 2 * kGateChoices && e == hipSuccess; k++) {
+                    q.trace_gate = kGateChoice[k % kGateChoices];
                     e = launch_paths(q);
-                    if (e == hipSuccess) e = hipEventRecord(ms->bdpt_tune.ev[2 + g], st);
+                    if (e == hipSuccess) e = hipEventRecord(ms->bdpt_tune.ev[2 + k], st);
                 }
                 PRT_HIP(e);
                 ms->bdpt_tune.pending = true;
@@ -1585,7 +1595,7 @@ int film_enqueue_chunk(prt_film* f) {
     // per-scene leaf gate of the trace kernel, on a chunk of at least 2^20 paths other than the film's first
     // (which may be a latency-bound first pass): between the first bounce's shading and the second bounce's
     // trace, after a warm-up launch, the second bounce's closest-hit trace of about 2^20 of its rays (it
-    // only writes their hit records, which the regular trace writes again) is timed with each gate.
+    // only writes their hit records, which the regular trace writes again) is timed twice with each gate.
     // Nothing waits for the timings: a later chunk of a film of the scene reads them and keeps the
     // fastest gate (gate 0 unless another is more than 3 % faster).
     prt_scene* ms = const_cast<prt_scene*>(s);
@@ -1593,15 +1603,9 @@ int film_enqueue_chunk(prt_film* f) {
     ms->info.pt_trace_waves = f->trace_grid[tv] / std::max(1, s->num_cus);  // work-groups of 4 waves per CU
     {
         std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
-        float t[kGateChoices];
+        float t[2 * kGateChoices];
         if (tune_collect(ms->pt_tune, t)) {
-            float best = t[0];
-            int bestg = 0;
-            for (int g = 1; g < kGateChoices; g++)
-                if (t[g] * 1.03f < best) {
-                    best = t[g] * 1.03f;
-                    bestg = kGateChoice[g];
-                }
+            const int bestg = pick_gate(t);
             int expect = -1;
             ms->pt_trace_gate.compare_exchange_strong(expect, bestg);
             ms->info.pt_trace_leaf_gate = ms->pt_trace_gate.load();
@@ -1613,12 +1617,12 @@ int film_enqueue_chunk(prt_film* f) {
         std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
         if (ms->pt_trace_gate.load() >= 0 || ms->pt_tune.pending) return hipSuccess;
         const uint32_t tstride = (uint32_t)std::max<int64_t>(1, n >> 20);
-        hipError_t e = tune_begin(ms->pt_tune, kGateChoices, st);
+        hipError_t e = tune_begin(ms->pt_tune, 2 * kGateChoices, st);
         if (e == hipSuccess) e = launch_trace(tv, 0, 1, 1, -1, tstride);
         if (e == hipSuccess) e = hipEventRecord(ms->pt_tune.ev[1], st);
-        for (int g = 0; g < kGateChoices && e == hipSuccess; g++) {
-            e = launch_trace(tv, kGateChoice[g], 1, 1, -1, tstride);
-            if (e == hipSuccess) e = hipEventRecord(ms->pt_tune.ev[2 + g], st);
+        for (int k = 0; k < 2 * kGateChoices && e == hipSuccess; k++) {
+            e = launch_trace(tv, kGateChoice[k % kGateChoices], 1, 1, -1, tstride);
+            if (e == hipSuccess) e = hipEventRecord(ms->pt_tune.ev[2 + k], st);
         }
         if (e == hipSuccess) ms->pt_tune.pending = true;
         return e;
