@@ -45,8 +45,18 @@ struct PathBuffers {
     uint32_t* key;
     uint32_t* queue[2];
     uint32_t* squeue[2];  // shadow rays of even / odd bounces
-    uint32_t* counters;   // queue lengths, one per 256-B line: path queues 0 and 1, shadow queues 2 and 3
+    uint32_t* counters;   // queue lengths, one per 256-B line: path queues 0 and 1, shadow queues 2 and 3,
+                          // long-ray queue 4
+    uint32_t* lq;         // long rays parked by a trace launch (path slot; kLongAny set for shadow rays)
+    uint32_t lcap;        // capacity of lq
 };
+
+// Long rays: a trace launch parks a ray that is still traversing after its step budget in the long-ray
+// queue (up to its capacity; a ray the full queue refuses keeps traversing), and k_trace_long then
+// finishes each parked ray with a whole wave (coop_trace).
+constexpr uint32_t kLongAny = 0x80000000u;
+constexpr uint32_t kLongQueue = 4;
+constexpr uint32_t kNumCounters = 5;
 
 struct PassParams {
     int32_t width, height;
@@ -78,6 +88,13 @@ __device__ __forceinline__ void slot_sample(const PassParams& pp, uint32_t i, ui
 // do not serialize on one line).
 constexpr uint32_t kCounterStride = 64;
 __device__ __forceinline__ uint32_t* qlen(const PathBuffers& pb, uint32_t q) { return pb.counters + q * kCounterStride; }
+
+__device__ __forceinline__ bool park_ray(const PathBuffers& pb, uint32_t v) {
+    const uint32_t k = atomicAdd(qlen(pb, kLongQueue), 1u);
+    if (k >= pb.lcap) return false;
+    pb.lq[k] = v;
+    return true;
+}
 
 // Block-aggregated appends to two queues: the block's waves reserve their ranges with LDS atomics and
 // one thread per block reserves the block's range of each queue with one global atomic. Every thread
@@ -187,7 +204,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
                                                                                          int bounce, int closest, int sq,
                                                   uint2* gstack, uint32_t gstride,
                                                   unsigned long long* probe = nullptr, int gate = 0,
-                                                  uint32_t tune_stride = 1) {
+                                                  uint32_t tune_stride = 1, uint32_t budget = 0,
+                                                  uint32_t budget_small = 0, uint32_t small_rays_per_lane = 1) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
@@ -203,6 +221,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
     }
     const uint32_t nc64 = (nc + 63u) & ~63u;
     const uint32_t stride = gridDim.x * kBlock;
+    // a launch with few rays per lane lasts as long as its longest rays (each wave's batches are fixed by
+    // the grid stride): park early
+    if (nc + ns <= small_rays_per_lane * stride) budget = budget_small;
     // work-groups without rays leave before loading the node cache (the deep bounces of a pass trace few
     // rays on the full persistent grid)
     if (blockIdx.x * kBlock >= nc64 + ns) return;
@@ -238,16 +259,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
         r.dz = d.z;
         r.tmin = 0.001f;
         THit h;
+        // rays still traversing after `budget` steps are parked for k_trace_long (closest hit: with the
+        // best hit so far in the hit record)
         if (any) {
             r.tmax = d.w;
-            if (!trace<true, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &cs, gate)) {
+            auto park = [&]() { return park_ray(pb, slot | kLongAny); };
+            if (trace_budget<true, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &cs, gate, budget, park) ==
+                kTraceMiss) {
                 float4 c = pb.sc[slot];
                 float4 l = pb.L[slot];
                 pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
             }
         } else {
             r.tmax = __int_as_float(0x7f800000);
-            trace<false, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &ce, gate);
+            auto park = [&]() { return park_ray(pb, slot); };
+            trace_budget<false, MIXED, COUNT, kLds, kBlock, kNodeCache>(sc, r, h, st, &ce, gate, budget, park);
             pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
         }
     }
@@ -261,8 +287,58 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
     }
 }
 
+// The rays the last trace launch parked, one wave per ray (coop_trace): closest hits complete their hit
+// records, unoccluded shadow rays add their contribution. Their results equal the one-lane traversal's.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_trace_long(DevScene sc, PathBuffers pb, uint2* gstack, uint32_t gstride) {
+    __shared__ uint32_t s_node[kLds * 64];
+    __shared__ float s_t[kLds * 64];
+    __shared__ uint32_t f_node[2 * kCoopFrontier];
+    __shared__ float f_t[2 * kCoopFrontier];
+    const uint32_t n = min(*qlen(pb, kLongQueue), pb.lcap);
+    if (blockIdx.x >= n) return;
+    TStack<kLds, 64> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * 64 + threadIdx.x;
+    st.gstride = gstride;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t v = pb.lq[i];
+        const bool any = (v & kLongAny) != 0;
+        const uint32_t slot = v & ~kLongAny;
+        const float4 o = pb.o[slot];
+        const float4 d = any ? pb.sd[slot] : pb.d[slot];
+        const TRay r{o.x, o.y, o.z, 0.001f, d.x, d.y, d.z, any ? d.w : __int_as_float(0x7f800000)};
+        THit h;
+        if (any) {
+            h.t = __int_as_float(0x7f800000);
+            h.prim = 0xFFFFFFFFu;
+            h.u = h.v = 0.0f;
+            const bool occluded = coop_trace<true, MIXED, kLds>(sc, r, h, st, f_node, f_t);
+            if (!occluded && threadIdx.x == 0) {
+                float4 c = pb.sc[slot];
+                float4 l = pb.L[slot];
+                pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+            }
+        } else {
+            const float4 hp = pb.hit[slot];
+            h.t = hp.x;
+            h.prim = __float_as_uint(hp.y);
+            h.u = hp.z;
+            h.v = hp.w;
+            coop_trace<false, MIXED, kLds>(sc, r, h, st, f_node, f_t);
+            if (threadIdx.x == 0) pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+        }
+        __syncthreads();  // the frontier buffers are reused by the next ray
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, PathBuffers pb, int b) {
     __shared__ uint32_t s_push[4];
+    // the long-ray queue of the next trace launch starts empty (this bounce's k_trace_long has finished)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *qlen(pb, kLongQueue) = 0;
     const uint32_t cur = (uint32_t)b & 1u;
     const uint32_t count = *qlen(pb, cur);
     const uint32_t* q = pb.queue[cur];
@@ -1052,6 +1128,10 @@ struct prt_film {
     int blocks = 0;
     int trace_grid[4] = {0, 0, 0, 0};  // persistent grid of each trace-kernel variant
     size_t trace_pad[4] = {0, 0, 0, 0};  // its dynamic LDS
+    uint32_t park_steps = 0;  // step budget of the path tracer's traversals before a ray is parked (0: none)
+    uint32_t park_steps_small = 0;  // the budget of launches with few rays per lane of the trace grid
+    uint32_t park_small_rays_per_lane = 1;  // launches of at most this many rays per lane are "few"
+    int long_grid = 0;        // work-groups (one wave each) of k_trace_long
     // BDPT: camera and light subpath stores and subpath lengths, per path slot
     bool bdpt = false;
     int bdpt_blocks = 0;
@@ -1244,6 +1324,36 @@ int film_blocks(const prt_scene* scene) {
         per_cu = 4;
     per_cu = std::min(per_cu, 8);
     return std::max(1, scene->num_cus) * per_cu;
+}
+
+// Long-ray parking of the path tracer: scenes of at least 2^16 primitives (whose deep bounces end in
+// grazing rays along a dense mesh) park rays after 128 steps in launches that fill the grid, after 64 in
+// smaller ones; PRT_PT_PARK_STEPS and PRT_PT_PARK_STEPS_SMALL override the budgets (0: no parking).
+constexpr int64_t kLongCapacity = (int64_t)1 << 20;
+uint32_t park_steps(const prt_scene* scene, bool small) {
+    if (const char* e = std::getenv(small ? "PRT_PT_PARK_STEPS_SMALL" : "PRT_PT_PARK_STEPS")) {
+        const long v = std::atol(e);
+        if (v >= 0 && v < (1l << 30)) return (uint32_t)v;
+    }
+    if (scene->info.num_primitives < (1 << 16)) return 0u;
+    return small ? 64u : 128u;
+}
+
+uint32_t park_small_rays_per_lane() {
+    if (const char* e = std::getenv("PRT_PT_PARK_SMALL_RAYS_PER_LANE")) {
+        const long v = std::atol(e);
+        if (v >= 1 && v < (1l << 20)) return (uint32_t)v;
+    }
+    return 4u;
+}
+
+// Resident one-wave work-groups of k_trace_long.
+int long_grid(const prt_scene* scene) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace_long<true>, 64, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 16;
+    return std::max(1, scene->num_cus) * std::min(per_cu, 32);
 }
 
 size_t film_gstack_entries(const prt_scene* scene) {
@@ -1573,7 +1683,7 @@ int film_enqueue_chunk(prt_film* f) {
         f->enq++;
         return PRT_OK;
     }
-    PRT_HIP(hipMemsetAsync(f->pb.counters, 0, 4 * kCounterStride * sizeof(uint32_t), st));
+    PRT_HIP(hipMemsetAsync(f->pb.counters, 0, kNumCounters * kCounterStride * sizeof(uint32_t), st));
     int cam_blocks = (int)((n + kBlock - 1) / kBlock);
     PRT_HIP(probe_mark(f, PRT_PROBE_KIND_CAMERA, true));
     hipLaunchKernelGGL(k_camera, dim3(cam_blocks), dim3(kBlock), 0, st, sc, pp, f->pb);
@@ -1581,16 +1691,28 @@ int film_enqueue_chunk(prt_film* f) {
     // bounce b: one trace launch (the shadow rays of bounce b - 1 and the extension rays of bounce b),
     // then shading; a last launch traces the shadow rays of the last bounce
     const bool counting = (f->probe_flags & PRT_PROBE_COUNT) != 0;
-    auto launch_trace = [&](int v, int gate, int b, int closest, int sq, uint32_t tstride = 1) {
+    // the regular trace launches park long rays (not the counting or tuning launches)
+    const uint32_t park = counting ? 0u : f->park_steps, park_small = counting ? 0u : f->park_steps_small;
+    auto launch_trace = [&](int v, int gate, int b, int closest, int sq, uint32_t tstride = 1, uint32_t budget = 0,
+                            uint32_t budget_small = 0) {
         PathBuffers pbv = f->pb;
         uint2* gs = f->gstack;
         uint32_t gsl = f->gslots;
         unsigned long long* pr = f->probe_dev;
         int g = gate;
+        uint32_t small_per_lane = f->park_small_rays_per_lane;
         void* args[] = {(void*)&sc, (void*)&pbv, (void*)&b,  (void*)&closest, (void*)&sq,
-                        (void*)&gs, (void*)&gsl, (void*)&pr, (void*)&g,       (void*)&tstride};
-        return hipLaunchKernel(trace_fn(sc.mixed != 0, counting, v), dim3(f->trace_grid[v]), dim3(kBlock), args,
-                               f->trace_pad[v], st);
+                        (void*)&gs, (void*)&gsl, (void*)&pr, (void*)&g,       (void*)&tstride,
+                        (void*)&budget, (void*)&budget_small, (void*)&small_per_lane};
+        hipError_t e = hipLaunchKernel(trace_fn(sc.mixed != 0, counting, v), dim3(f->trace_grid[v]), dim3(kBlock),
+                                       args, f->trace_pad[v], st);
+        if (e != hipSuccess || (budget == 0 && budget_small == 0)) return e;
+        if (sc.mixed)
+            hipLaunchKernelGGL(k_trace_long<true>, dim3(f->long_grid), dim3(64), 0, st, sc, f->pb, f->gstack, f->gslots);
+        else
+            hipLaunchKernelGGL(k_trace_long<false>, dim3(f->long_grid), dim3(64), 0, st, sc, f->pb, f->gstack,
+                               f->gslots);
+        return hipGetLastError();
     };
     // per-scene leaf gate of the trace kernel, on a chunk of at least 2^20 paths other than the film's first
     // (which may be a latency-bound first pass): between the first bounce's shading and the second bounce's
@@ -1630,7 +1752,7 @@ int film_enqueue_chunk(prt_film* f) {
     const int tgate = std::max(0, ms->pt_trace_gate.load());
     auto trace = [&](int b, int closest, int sq) {
         hipError_t e = probe_mark(f, PRT_PROBE_KIND_TRACE, true);
-        if (e == hipSuccess) e = launch_trace(tv, tgate, b, closest, sq);
+        if (e == hipSuccess) e = launch_trace(tv, tgate, b, closest, sq, 1, park, park_small);
         if (e == hipSuccess) e = probe_mark(f, PRT_PROBE_KIND_TRACE, false);
         if (f->probe_flags) f->probe_launches++;
         return e;
@@ -1743,6 +1865,7 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     if (bd) b = 2 * bdpt_store_bytes(p->max_depth, C, uv) + sizeof(uint32_t) * C + 3 * sizeof(long long) * npix;
     if (p->adaptive_min_samples < 1.0) b += 2 * sizeof(uint32_t) * npix + 6 * sizeof(uint32_t);
     b += sizeof(uchar4) * (size_t)npix;  // the device image of RGBA8 reads, allocated at the first read
+    b += sizeof(uint32_t) * (size_t)std::min<int64_t>(2 * C, kLongCapacity);  // long-ray queue
     return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + b + 3 * sizeof(uint32_t) * (size_t)C;
 }
 
@@ -1804,7 +1927,9 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     FALLOC(f->pb.queue[1], uint32_t, C);
     FALLOC(f->pb.squeue[0], uint32_t, C);
     FALLOC(f->pb.squeue[1], uint32_t, C);
-    FALLOC(f->pb.counters, uint32_t, 4 * kCounterStride);
+    FALLOC(f->pb.counters, uint32_t, kNumCounters * kCounterStride);
+    f->pb.lcap = (uint32_t)std::min<int64_t>(2 * C, kLongCapacity);
+    FALLOC(f->pb.lq, uint32_t, f->pb.lcap);
     FALLOC(f->fb.sum, float4, f->npix);
     FALLOC(f->fb.comp, float4, f->npix);
     FALLOC(f->fb.sq, float2, f->npix);
@@ -1818,6 +1943,10 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
         slot_blocks = std::max(slot_blocks, f->trace_grid[v]);
     }
     f->gslots = (uint32_t)slot_blocks * kBlock;
+    f->park_steps = park_steps(scene, false);
+    f->park_steps_small = park_steps(scene, true);
+    f->park_small_rays_per_lane = park_small_rays_per_lane();
+    f->long_grid = (int)std::min<uint32_t>(f->gslots / 64u, (uint32_t)long_grid(scene));
     if ((rc = film_alloc(f, film_gstack_entries(scene) * f->gslots * sizeof(uint2), &ptr)) != PRT_OK) return bail(rc);
     f->gstack = (uint2*)ptr;
     if (p->integrator == PRT_INTEGRATOR_BDPT) {

@@ -488,7 +488,9 @@ __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStac
                     h = cone_test(r, a, b, pp[2], s.tbest, t, u);
                 }
             }
-            if (h) {
+            // closest hit: the least (t, primitive index) pair, so that the result does not depend on the
+            // order in which leaves are visited (an equal t is kept only from a lower index)
+            if (h && (ANY || t < s.tbest || first + k < s.best)) {
                 s.tbest = t;
                 s.best = first + k;
                 s.bu = u;
@@ -531,6 +533,211 @@ __device__ __forceinline__ bool trace(const DevScene& sc, const TRay& r_in, THit
     }
     tstate_result(s, hit);
     return s.best != 0xFFFFFFFFu;
+}
+
+// trace() with a step budget. After `budget` steps (0: no budget) an unfinished ray offers itself to
+// `park` (a callable returning true when it took the ray) and, when taken, stops with its best hit so
+// far in `hit`; a ray the callable refuses keeps traversing. Returns kTraceMiss, kTraceHit or
+// kTraceParked. Whether a ray is parked does not change its result: the closest hit is the least
+// (t, primitive index) pair whichever traversal finds it.
+constexpr int kTraceMiss = 0, kTraceHit = 1, kTraceParked = 2;
+template <bool ANY, bool MIXED, bool COUNT, int LDS_DEPTH, int BLOCK, int NCACHE, typename Park>
+__device__ __forceinline__ int trace_budget(const DevScene& sc, const TRay& r_in, THit& hit,
+                                            TStack<LDS_DEPTH, BLOCK>& st, TCounts* cnt, int gate, uint32_t budget,
+                                            Park park) {
+    TState s;
+    tstate_init(s, r_in);
+    uint32_t steps = 0;
+    bool parked = false;
+    for (;;) {
+        const bool ok = gate > 0 ? leaf_gate(true, s.cur, gate) : true;
+        if (tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK, NCACHE>(sc, s, st, cnt, ok)) break;
+        if (++steps == budget && park()) {
+            parked = true;
+            break;
+        }
+    }
+    tstate_result(s, hit);
+    return parked ? kTraceParked : (s.best != 0xFFFFFFFFu ? kTraceHit : kTraceMiss);
+}
+
+// Minimum of an unsigned value over the 64 lanes of a wave; every lane must be active. DPP steps reduce
+// each row of 16 lanes (swap neighbours, swap pairs, mirror halves of 8, mirror the row), then the four
+// row minima are read as scalars.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return min(min(a, b), min(c, d));
+}
+
+// Minimum of a non-negative float (or +inf) over the wave's lanes: such floats order like their bits.
+__device__ __forceinline__ float wave_min_nonneg(float x) { return __uint_as_float(wave_min_u32(__float_as_uint(x))); }
+
+// Exclusive prefix sum over the wave's lanes of a count in [0, 7], and the wave's total.
+__device__ __forceinline__ uint32_t wave_scan7(uint32_t c, uint32_t& total) {
+    const uint64_t lt = (1ull << (threadIdx.x & 63u)) - 1ull;
+    const uint64_t b0 = __ballot(c & 1u), b1 = __ballot(c & 2u), b2 = __ballot(c & 4u);
+    total = (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+    return (uint32_t)(__popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt));
+}
+
+// Wave-cooperative traversal of ONE ray by the 64 lanes of a one-wave work-group (every lane passes
+// the same ray). For rays that visit hundreds of nodes (grazing rays along a dense mesh), whose
+// one-lane traversal is a long chain of dependent node fetches:
+//  1. the wave expands a frontier breadth-first, a node per lane per round, until it holds at least 64
+//     entries or only leaves (at most 256 entries: fewer than 64 nodes expand into at most 4 each);
+//  2. each lane traverses frontier entries depth-first with the one-lane step (its own stack), taking
+//     the next untaken entry when it finishes one; after every step the lanes share the wave's least
+//     hit distance to cull with.
+// `hit` holds the ray's best hit so far on entry (prim 0xFFFFFFFF: none) and the least (t, primitive
+// index) pair over all primitive hits with t in [tmin, tbest] on exit, the result of trace(). Any hit
+// (ANY): returns as soon as a lane finds one. f_node / f_t: 2 x 256 LDS entries. Returns true on a hit.
+constexpr uint32_t kCoopFrontier = 256;
+template <bool ANY, bool MIXED, int LDS_DEPTH>
+__device__ __forceinline__ bool coop_trace(const DevScene& sc, const TRay& r_in, THit& hit, TStack<LDS_DEPTH, 64>& st,
+                                           uint32_t* f_node, float* f_t) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const float kInfF = __int_as_float(0x7f800000);
+    TState s;
+    tstate_init(s, r_in);
+    const TRay& r = s.r;
+    // wave-shared bound: the best hit distance found so far by any lane (or the entry best)
+    float bound = hit.prim != 0xFFFFFFFFu ? hit.t : s.tbest;
+    uint32_t nf = 1, cur = 0;  // frontier size, current buffer (entries at f_*[cur * 256 + i])
+    if (lane == 0) {
+        f_node[0] = 0;  // the root
+        f_t[0] = r.tmin;
+    }
+    __syncthreads();
+    // ---- 1. breadth-first frontier ----
+    while (nf < 64) {
+        const bool have = lane < nf;
+        const uint32_t node = have ? f_node[cur * kCoopFrontier + lane] : 0u;
+        const float tn = have ? f_t[cur * kCoopFrontier + lane] : kInfF;
+        const bool live = have && tn <= bound;
+        const bool inner = live && !child_is_leaf(node);
+        if (__ballot(inner) == 0) break;  // only leaves left: phase 2 takes them as they are
+        // children the ray enters before the bound (entry distances rounded down like the one-lane step)
+        bool in[4] = {false, false, false, false};
+        float ent[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        uint4 ch = make_uint4(0, 0, 0, 0);
+        uint32_t cnt = live ? 1u : 0u;
+        if (inner) {
+            const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            ch = reinterpret_cast<const uint4*>(np)[6];
+            const float lo[3][4] = {{lx.x, lx.y, lx.z, lx.w}, {ly.x, ly.y, ly.z, ly.w}, {lz.x, lz.y, lz.z, lz.w}};
+            const float hi[3][4] = {{hx.x, hx.y, hx.z, hx.w}, {hy.x, hy.y, hy.z, hy.w}, {hz.x, hz.y, hz.z, hz.w}};
+            cnt = 0;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float nx = ((s.idx >= 0.0f ? lo[0][c] : hi[0][c]) - r.ox) * s.idx;
+                const float fx = ((s.idx >= 0.0f ? hi[0][c] : lo[0][c]) - r.ox) * s.idx;
+                const float ny = ((s.idy >= 0.0f ? lo[1][c] : hi[1][c]) - r.oy) * s.idy;
+                const float fy = ((s.idy >= 0.0f ? hi[1][c] : lo[1][c]) - r.oy) * s.idy;
+                const float nz = ((s.idz >= 0.0f ? lo[2][c] : hi[2][c]) - r.oz) * s.idz;
+                const float fz = ((s.idz >= 0.0f ? hi[2][c] : lo[2][c]) - r.oz) * s.idz;
+                const float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
+                const float tfar = fminf(fminf(fminf(fx, fy), fz) * kBoxWiden, bound);
+                in[c] = tnear <= tfar && pick4(ch, (uint32_t)c) != kEmptyChild;
+                ent[c] = __uint_as_float(__float_as_uint(tnear) & ~3u);
+                cnt += in[c] ? 1u : 0u;
+            }
+        }
+        uint32_t total;
+        const uint32_t off = wave_scan7(cnt, total);
+        const uint32_t nxt = cur ^ 1u;
+        if (inner) {
+            uint32_t k = off;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (in[c]) {
+                    f_node[nxt * kCoopFrontier + k] = pick4(ch, (uint32_t)c);
+                    f_t[nxt * kCoopFrontier + k] = ent[c];
+                    k++;
+                }
+            }
+        } else if (live) {
+            f_node[nxt * kCoopFrontier + off] = node;
+            f_t[nxt * kCoopFrontier + off] = tn;
+        }
+        __syncthreads();
+        cur = nxt;
+        nf = total;
+    }
+    // ---- 2. depth-first traversal of the frontier entries, lanes taking entries in turn ----
+    bool active = false, owns = false;
+    uint32_t taken = 0;  // wave-uniform: entries handed out so far
+    uint32_t obest = 0xFFFFFFFFu;
+    float ot = kInfF, ou = 0.0f, ov = 0.0f;
+    bool any_hit = false;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        if (taken < nf && idle != 0) {
+            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+            if (!active && taken + rank < nf) {
+                const uint32_t e = taken + rank;
+                const float tn = f_t[cur * kCoopFrontier + e];
+                if (tn <= bound) {
+                    s.cur = f_node[cur * kCoopFrontier + e];
+                    s.sp = 0;
+                    s.tbest = bound;
+                    s.best = 0xFFFFFFFFu;
+                    active = true;
+                }
+            }
+            taken = min(nf, taken + (uint32_t)__popcll(idle));
+        }
+        if (__ballot(active) == 0) {
+            if (taken >= nf) break;
+            continue;
+        }
+        if (active) {
+            // adopt a closer bound found by another lane (the lane's own candidate no longer wins)
+            if (bound < s.tbest) {
+                s.tbest = bound;
+                s.best = 0xFFFFFFFFu;
+            }
+            const bool done = tstate_step<ANY, MIXED, false, LDS_DEPTH, 64, 0>(sc, s, st, nullptr, true);
+            if (s.best != 0xFFFFFFFFu) {  // the lane's least (t, index) pair so far
+                if (!owns || s.tbest < ot || (s.tbest == ot && s.best < obest)) {
+                    ot = s.tbest;
+                    obest = s.best;
+                    ou = s.bu;
+                    ov = s.bv;
+                }
+                owns = true;
+                if (ANY) any_hit = true;
+            }
+            if (done) active = false;
+        }
+        if (ANY) {
+            if (__ballot(any_hit) != 0) break;
+        }
+        bound = fminf(bound, wave_min_nonneg(owns ? ot : kInfF));
+    }
+    if (ANY) {
+        const bool found = __ballot(any_hit) != 0;
+        if (found) hit.prim = 0u;
+        return found;
+    }
+    // the least (t, index) pair over the lanes and the entry best
+    const float wt = wave_min_nonneg(owns ? ot : kInfF);
+    const uint32_t wp = wave_min_u32(owns && ot == wt ? obest : 0xFFFFFFFFu);
+    if (wp == 0xFFFFFFFFu) return hit.prim != 0xFFFFFFFFu;  // nothing closer than the entry best
+    const bool better = hit.prim == 0xFFFFFFFFu || wt < hit.t || (wt == hit.t && wp < hit.prim);
+    if (!better) return true;
+    const uint64_t holder = __ballot(owns && ot == wt && obest == wp);
+    const int hl = __ffsll((unsigned long long)holder) - 1;
+    hit.t = wt;
+    hit.prim = wp;
+    hit.u = __shfl(ou, hl, 64);
+    hit.v = __shfl(ov, hl, 64);
+    return true;
 }
 
 }  // namespace prt

@@ -106,3 +106,48 @@ def test_nonuniform_light_weights(integrator):
     desc_u, _ = scenes.default_scene(width=128)
     gu, _, _, _ = _first_sample(desc_u, cfg, integrator, 8)
     assert _agree(g, gu) < 0.9
+
+
+def _film(scene, desc, cfg, spp, seed):
+    w, h = desc.image_size
+    pr = P.ProgressiveRaytracer(scene, P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], spp,
+                                                         seed=seed))
+    pr.render_samples(spp)
+    out = pr.read()
+    pr.close()
+    return out
+
+
+def _mesh_under_sky():
+    """Triangle-only scene: the 1.31M-triangle displaced icosphere under a uniform infinite light (shadow
+    rays towards the light are any-hit traversals of the mesh)."""
+    desc, _ = scenes.mesh_scene(frequency=256, width=96)
+    desc.add_uniform_infinite_light((0.8, 0.9, 1.0))
+    return desc, dict(max_depth=8, rr_min_bounces=3)
+
+
+@pytest.mark.parametrize("case", ["mesh_sky_mixed", "mesh_triangles_only"])
+def test_long_ray_parking_leaves_films_unchanged(case, mesh_sky, monkeypatch):
+    """Rays still traversing after the step budget (PRT_PT_PARK_STEPS, PRT_PT_PARK_STEPS_SMALL for launches
+    with fewer rays than the grid's lanes; scenes of >= 2^16 primitives park after 128 / 64 steps by
+    default) are finished by a whole wave each (coop_trace); the closest hit is the
+    least (t, primitive index) pair whichever traversal finds it, so films are bit-identical with parking
+    off (0), at the default, and at budgets of 24 and 4 steps that park most rays (closest and any hit)."""
+    if case == "mesh_sky_mixed":
+        desc, cfg, scene, _ = mesh_sky
+    else:
+        desc, cfg = _mesh_under_sky()
+        scene = P.Scene(desc)
+    films = {}
+    for budget in ("0", None, "24", "4"):
+        for var in ("PRT_PT_PARK_STEPS", "PRT_PT_PARK_STEPS_SMALL"):
+            if budget is None:
+                monkeypatch.delenv(var, raising=False)
+            else:
+                monkeypatch.setenv(var, budget)
+        films[budget] = _film(scene, desc, cfg, 4, seed=11)
+    base = films["0"]
+    assert base[0].mean() > 0
+    for b, f in films.items():
+        for x, y in zip(base, f):
+            assert np.array_equal(x, y), f"film with park budget {b} differs from the film without parking"
