@@ -48,6 +48,8 @@ struct prt_scene {
     std::vector<prt::DevBuf> bufs;
     uint2* gstack = nullptr;     // traversal stack overflow for the intersection kernels
     uint32_t gstack_slots = 0;   // thread slots covered by gstack
+    uint32_t* park = nullptr;    // park queue of the intersection kernels (count, then ray indices)
+    int32_t park_steps = -1;     // their step budget before a ray is parked: -1 per batch size, 0 none
     int32_t max_stack = 0;
     int32_t num_cus = 0;
     int trace_variant[2] = {-1, -1};  // autotuned kernel variant (closest, any)

@@ -334,14 +334,43 @@ __global__ __launch_bounds__(BLOCK) void k_intersect_refill(DevScene sc, const p
 // and reloads its next-ray registers from the wave's ray chunk. Each wave's first chunk of kPfChunk
 // rays is static; later ones are reserved one ahead with an atomic whose result is read only when the
 // current chunk runs out, so neither the ray loads nor the work counter add latency to the steps.
-constexpr uint32_t kPfChunk = 256;
+#ifndef PRT_PF_CHUNK
+#define PRT_PF_CHUNK 256
+#endif
+constexpr uint32_t kPfChunk = PRT_PF_CHUNK;
 
 // Triangle-only scenes: 6 waves per SIMD (80 VGPRs, the LDS bound of the 8-entry stack and 64-node cache)
 // hide more node-fetch latency than the register count the compiler picks on its own (84, 5 waves).
+// Long rays: a lane whose ray is still traversing after `budget` steps (0: never) parks it in the park
+// queue (up to its capacity; a refused ray keeps traversing) with its best hit so far in the hit record
+// (leaf-ordered primitive index), and takes its next ray; k_intersect_long then finishes every parked ray
+// with a whole wave. A ray's result does not depend on which kernel finishes it.
+struct ParkQueue {
+    uint32_t* count;  // on its own 256-B line
+    uint32_t* q;      // ray indices
+    uint32_t cap;
+    uint32_t budget;
+};
+constexpr uint32_t kParkCapacity = 1u << 18;
+
+// Step budget of a batch of n rays on a grid of `lanes` threads: a wave per parked ray pays off where
+// lanes would otherwise idle, so batches of a few rays per lane (latency-bound, their time set by the
+// longest rays) park early and large batches park only their longest rays. PRT_TRACE_PARK_STEPS pins
+// one budget for every batch (0: no parking). Rays are indexed with 32 bits in the park queue: batches
+// beyond 2^32 rays do not park.
+uint32_t park_budget(const prt_scene* s, int64_t n, int64_t lanes) {
+    if (n > (int64_t)0xFFFFFFFFll) return 0u;
+    if (s->park_steps >= 0) return (uint32_t)s->park_steps;
+    const int64_t per_lane = n / std::max<int64_t>(1, lanes);
+    return per_lane <= 1 ? 32u : (per_lane <= 16 ? 48u : 96u);
+}
+constexpr int kLongLds = 8;  // LDS stack entries per lane of k_intersect_long (deeper entries spill)
+
 template <bool ANY, bool MIXED, int LDS, int NCACHE, int BLOCK, int GATE = 0>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 6))) void k_intersect_pf(DevScene sc, const prt_ray* __restrict__ rays, int64_t n,
                                                         prt_hit* __restrict__ hits, uint8_t* __restrict__ occluded,
-                                                        uint2* gstack, uint32_t gstride, unsigned long long* work) {
+                                                        uint2* gstack, uint32_t gstride, unsigned long long* work,
+                                                        ParkQueue park) {
     __shared__ uint32_t s_node[LDS * BLOCK];
     __shared__ float s_t[LDS * BLOCK];
     __shared__ float4 s_cache[NCACHE > 0 ? NCACHE * 8 : 1];
@@ -372,6 +401,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
     TState s;
     bool active = false;
     int64_t ridx = 0, nidx = -1;
+    uint32_t steps = 0;
     float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nb = na;
     for (;;) {
         // start the prefetched ray on an idle lane
@@ -380,6 +410,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
             ridx = nidx;
             nidx = -1;
             active = true;
+            steps = 0;
         }
         // hand out the next rays of the chunk to lanes without a prefetched ray
         uint64_t need = __ballot(nidx < 0);
@@ -410,19 +441,85 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
         }
         if (__ballot(active || nidx >= 0) == 0) break;  // exhausted and drained
         const bool leaf_ok = GATE > 0 ? leaf_gate(active, s.cur, GATE) : true;
-        if (active && tstate_step<ANY, MIXED, false, LDS, BLOCK, NCACHE>(sc, s, st, nullptr, leaf_ok)) {
-            active = false;
-            bool hit = s.best != 0xFFFFFFFFu;
-            if (ANY) {
-                occluded[ridx] = hit ? 1 : 0;
-            } else {
-                int32_t pid = hit ? (int32_t)__float_as_uint(sc.prims[3 * (size_t)s.best + 2].w) : -1;
-                reinterpret_cast<float4*>(hits)[ridx] =
-                    make_float4(hit ? s.tbest : __int_as_float(0x7f800000), __int_as_float(pid),
-                                hit ? s.bu : 0.0f, hit ? s.bv : 0.0f);
+        if (active) {
+            if (tstate_step<ANY, MIXED, false, LDS, BLOCK, NCACHE>(sc, s, st, nullptr, leaf_ok)) {
+                active = false;
+                bool hit = s.best != 0xFFFFFFFFu;
+                if (ANY) {
+                    occluded[ridx] = hit ? 1 : 0;
+                } else {
+                    int32_t pid = hit ? (int32_t)__float_as_uint(sc.prims[3 * (size_t)s.best + 2].w) : -1;
+                    reinterpret_cast<float4*>(hits)[ridx] =
+                        make_float4(hit ? s.tbest : __int_as_float(0x7f800000), __int_as_float(pid),
+                                    hit ? s.bu : 0.0f, hit ? s.bv : 0.0f);
+                }
+            } else if (++steps == park.budget) {
+                const uint32_t k = atomicAdd(park.count, 1u);
+                if (k < park.cap) {
+                    park.q[k] = (uint32_t)ridx;
+                    if (!ANY)
+                        reinterpret_cast<float4*>(hits)[ridx] =
+                            make_float4(s.tbest, __uint_as_float(s.best), s.bu, s.bv);
+                    active = false;
+                }
             }
         }
     }
+}
+
+// Finishes the rays k_intersect_pf parked, one wave per ray (coop_trace), and writes their results as
+// k_intersect_pf would have.
+template <bool ANY, bool MIXED>
+__global__ __launch_bounds__(64) void k_intersect_long(DevScene sc, const prt_ray* __restrict__ rays,
+                                                       prt_hit* __restrict__ hits, uint8_t* __restrict__ occluded,
+                                                       uint2* gstack, uint32_t gstride, ParkQueue park) {
+    __shared__ uint32_t s_node[kLongLds * 64];
+    __shared__ float s_t[kLongLds * 64];
+    __shared__ uint32_t f_node[2 * kCoopFrontier];
+    __shared__ float f_t[2 * kCoopFrontier];
+    const uint32_t n = min(*park.count, park.cap);
+    if (blockIdx.x >= n) return;
+    TStack<kLongLds, 64> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * 64 + threadIdx.x;
+    st.gstride = gstride;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t ridx = park.q[i];
+        const float4* rp = reinterpret_cast<const float4*>(rays + ridx);
+        const float4 a = rp[0], b = rp[1];
+        const TRay r{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        THit h;
+        if (ANY) {
+            h.t = __int_as_float(0x7f800000);
+            h.prim = 0xFFFFFFFFu;
+            h.u = h.v = 0.0f;
+            const bool occ = coop_trace<true, MIXED, kLongLds>(sc, r, h, st, f_node, f_t);
+            if (threadIdx.x == 0) occluded[ridx] = occ ? 1 : 0;
+        } else {
+            const float4 hp = reinterpret_cast<const float4*>(hits)[ridx];
+            h.t = hp.x;
+            h.prim = __float_as_uint(hp.y);
+            h.u = hp.z;
+            h.v = hp.w;
+            const bool hit = coop_trace<false, MIXED, kLongLds>(sc, r, h, st, f_node, f_t);
+            if (threadIdx.x == 0) {
+                const int32_t pid = hit ? (int32_t)__float_as_uint(sc.prims[3 * (size_t)h.prim + 2].w) : -1;
+                reinterpret_cast<float4*>(hits)[ridx] = make_float4(hit ? h.t : __int_as_float(0x7f800000),
+                                                                    __int_as_float(pid), hit ? h.u : 0.0f,
+                                                                    hit ? h.v : 0.0f);
+            }
+        }
+        __syncthreads();  // the frontier buffers are reused by the next ray
+    }
+}
+
+// Zeroes the work counter and the park count before an intersection launch.
+__global__ void k_clear_counters(unsigned long long* work, uint32_t* park_count) {
+    *work = 0ull;
+    *park_count = 0u;
 }
 
 // Kernel variants the per-scene autotuner chooses from (identical per-ray traversal): kernel
@@ -445,6 +542,7 @@ constexpr int kVariantLds[kNumVariants] = {
 #undef X
 };
 constexpr int kMinLds = 6;
+static_assert(kMinLds <= kLongLds, "the scene's spill stack must cover k_intersect_long");
 constexpr int kDefaultVariant = 5;
 constexpr int kVariantBlock[kNumVariants] = {
 #define X(id, kind, lds, refill, nc, blk) blk,
@@ -470,22 +568,42 @@ const void* variant_fn(int v) {
     return nullptr;
 }
 
+// Launches variant v; the prefetched-refill variants park long rays (park.budget > 0) and are followed
+// by k_intersect_long on `long_blocks` one-wave work-groups (at most gstride / 64).
 template <bool ANY, bool MIXED>
 void launch_variant(int v, int blocks, hipStream_t st, const DevScene& sc, const prt_ray* rays, int64_t n,
-                    prt_hit* hits, uint8_t* occ, uint2* gs, uint32_t gstride, unsigned long long* work) {
+                    prt_hit* hits, uint8_t* occ, uint2* gs, uint32_t gstride, unsigned long long* work,
+                    ParkQueue park, int long_blocks) {
+    bool pf = false;
     switch (v) {
 #define X(id, kind, lds, refill, nc, blk)                                                                     \
     case id:                                                                                                \
-        if (kind == 1)                                                                                      \
+        if (kind == 1) {                                                                                    \
             hipLaunchKernelGGL((k_intersect_pf<ANY, MIXED, lds, nc, blk, refill>), dim3(blocks), dim3(blk), 0, st, sc, \
-                               rays, n, hits, occ, gs, gstride, work);                                      \
-        else                                                                                                \
+                               rays, n, hits, occ, gs, gstride, work, park);                                \
+            pf = true;                                                                                      \
+        } else {                                                                                            \
             hipLaunchKernelGGL((k_intersect_refill<ANY, MIXED, lds, refill, nc, blk>), dim3(blocks),          \
                                dim3(blk), 0, st, sc, rays, n, hits, occ, gs, gstride, work);                \
+        }                                                                                                   \
         break;
         PRT_TRACE_VARIANTS(X)
 #undef X
     }
+    if (pf && park.budget > 0)
+        hipLaunchKernelGGL((k_intersect_long<ANY, MIXED>), dim3(long_blocks), dim3(64), 0, st, sc, rays, hits, occ, gs,
+                           gstride, park);
+}
+
+// Resident one-wave work-groups of k_intersect_long, at most `slots` / 64 (its stack slots).
+int long_blocks(const prt_scene* s, uint32_t slots) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_intersect_long<false, true>, 64, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 16;
+    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(1, s->num_cus) * std::min(per_cu, 32),
+                                                       slots / 64));
 }
 
 }  // namespace prt
@@ -542,16 +660,19 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
     uint2* gs = scene->gstack;
     uint32_t gstride = scene->gstack_slots;
     prt_scene* ms = const_cast<prt_scene*>(scene);
+    // park queue: count on the first 256-B line, then the ray indices
+    ParkQueue park{scene->park, scene->park + 64, kParkCapacity, 0u};
+    const int lb = long_blocks(scene, gstride);
     auto run = [&](int v, int64_t count) -> hipError_t {
-        hipError_t er = hipMemsetAsync(work, 0, sizeof(unsigned long long), st);
-        if (er != hipSuccess) return er;
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, st, work, park.count);
         int blocks = ms->variant_blocks[v];
+        park.budget = park_budget(scene, count, (int64_t)blocks * kVariantBlock[v]);
         if (mode == 0) {
-            if (sc.mixed) launch_variant<false, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work);
-            else launch_variant<false, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work);
+            if (sc.mixed) launch_variant<false, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work, park, lb);
+            else launch_variant<false, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work, park, lb);
         } else {
-            if (sc.mixed) launch_variant<true, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work);
-            else launch_variant<true, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work);
+            if (sc.mixed) launch_variant<true, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work, park, lb);
+            else launch_variant<true, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work, park, lb);
         }
         return hipGetLastError();
     };
@@ -623,11 +744,14 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
 }
 
 // ---- allocation-free closest hit (prt_intersect_closest_static) ------------------------------------
-// The work counter and the traversal overflow stack of the intersection kernels live in one static
-// device array instead of scene memory: 256 B for the counter, then the stack, entry e of thread slot
-// g at [e * slots + g] as in scene memory. Each device holds its own copy (resolved once per device
+// The work counter, the park queue and the traversal overflow stack of the intersection kernels live in
+// one static device array instead of scene memory: 256 B for the work counter, the park count (256 B)
+// and kStaticParkCapacity ray indices, then the stack, entry e of thread slot g at [e * slots + g] as in
+// scene memory. Each device holds its own copy (resolved once per device
 // with hipGetSymbolAddress); calls on one device share it, so calls that overlap on one device race.
 constexpr size_t kStaticScratchBytes = (size_t)PRT_STATIC_SCRATCH_BYTES;
+constexpr uint32_t kStaticParkCapacity = 1u << 16;
+constexpr size_t kStaticStackOffset = 512 + sizeof(uint32_t) * (size_t)kStaticParkCapacity;
 __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
 
 namespace {
@@ -635,7 +759,6 @@ constexpr int kMaxDevices = 64;
 std::mutex g_static_mu;
 void* g_static_ptr[kMaxDevices] = {};
 
-__global__ void k_clear_counter(unsigned long long* c) { *c = 0ull; }
 
 // Device address of this device's `scratch` (the device must be current).
 int static_scratch(int device, void** out) {
@@ -1268,6 +1391,14 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     if ((rc = scene_alloc(s, depth * slots * sizeof(uint2), nullptr, &p)) != PRT_OK) return bail(rc);
     s->gstack = (uint2*)p;
     s->gstack_slots = slots;
+    // park queue of the intersection kernels: count (256 B), then kParkCapacity ray indices
+    if ((rc = scene_alloc(s, 256 + sizeof(uint32_t) * (size_t)kParkCapacity, nullptr, &p)) != PRT_OK) return bail(rc);
+    s->park = (uint32_t*)p;
+    s->park_steps = -1;  // chosen per batch (park_budget)
+    if (const char* fp = std::getenv("PRT_TRACE_PARK_STEPS")) {  // pins the batch kernels' park budget (0: none)
+        long b = std::atol(fp);
+        if (b >= 0 && b < (1l << 30)) s->park_steps = (int32_t)b;
+    }
     *out = s;
     return PRT_OK;
 }
@@ -1320,9 +1451,10 @@ int prt_intersect_closest_static(const prt_scene* scene, const prt_ray* rays, in
     const int v = scene->forced_variant >= 0 ? scene->forced_variant
                                              : (scene->trace_variant[0] >= 0 ? scene->trace_variant[0] : kDefaultVariant);
     const int block = kVariantBlock[v];
-    const size_t entries = (size_t)std::max(1, scene->max_stack - kVariantLds[v] + 1);
+    // stack entries per thread slot: the variant's spill, or k_intersect_long's if that spills sooner
+    const size_t entries = (size_t)std::max(1, scene->max_stack - std::min(kVariantLds[v], kLongLds) + 1);
     const size_t per_block = entries * (size_t)block * sizeof(uint2);
-    const int64_t fit = (int64_t)((kStaticScratchBytes - 256) / per_block);
+    const int64_t fit = (int64_t)((kStaticScratchBytes - kStaticStackOffset) / per_block);
     if (fit < 1)
         return fail(PRT_ERR_UNSUPPORTED, "the scene's traversal stack bound (" + std::to_string(scene->max_stack) +
                                              " entries) needs more than the static scratch of " +
@@ -1330,14 +1462,17 @@ int prt_intersect_closest_static(const prt_scene* scene, const prt_ray* rays, in
     // the full occupancy grid where the stack fits, else as many work-groups as it holds
     const int blocks = (int)std::min<int64_t>(scene->variant_blocks[v], fit);
     unsigned long long* work = (unsigned long long*)base;
-    uint2* gs = (uint2*)((char*)base + 256);
+    uint32_t* pk = (uint32_t*)((char*)base + 256);
+    const ParkQueue park{pk, pk + 64, kStaticParkCapacity, park_budget(scene, n, (int64_t)blocks * block)};
+    uint2* gs = (uint2*)((char*)base + kStaticStackOffset);
     const uint32_t slots = (uint32_t)blocks * (uint32_t)block;
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(1), 0, st, work);
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, st, work, park.count);
+    const int lb = long_blocks(scene, slots);
     if (scene->dev.mixed)
-        launch_variant<false, true>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work);
+        launch_variant<false, true>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work, park, lb);
     else
-        launch_variant<false, false>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work);
+        launch_variant<false, false>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work, park, lb);
     PRT_HIP(hipGetLastError());
     return PRT_OK;
 }

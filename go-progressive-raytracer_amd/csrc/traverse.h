@@ -11,6 +11,12 @@
 
 namespace prt {
 
+// Primitive tests round every product and sum on its own (no fused multiply-add): the compiler fuses
+// differently in each kernel it inlines a test into, and a hit must come out bit-identical whichever
+// traversal kernel finds it. The watertight triangle test also needs it: an edge shared by two
+// triangles must give exactly negated edge functions in both.
+#define PRT_EXACT_ROUNDING _Pragma("clang fp contract(off)")
+
 struct TRay {
     float ox, oy, oz, tmin;
     float dx, dy, dz, tmax;
@@ -65,6 +71,7 @@ __device__ __forceinline__ WRay make_wray(float dx, float dy, float dz) {
 // (u, v = barycentric weights of v1, v2) and returns true.
 __device__ __forceinline__ bool tri_test(const TRay& r, const WRay& w, float4 a, float4 b, float4 c,
                                          float tbest, float& t_out, float& u_out, float& v_out) {
+    PRT_EXACT_ROUNDING
     float Ax = a.x - r.ox, Ay = a.y - r.oy, Az = a.z - r.oz;
     float Bx = b.x - r.ox, By = b.y - r.oy, Bz = b.z - r.oz;
     float Cx = c.x - r.ox, Cy = c.y - r.oy, Cz = c.z - r.oz;
@@ -109,6 +116,7 @@ __device__ __forceinline__ bool tri_test(const TRay& r, const WRay& w, float4 a,
 // t0 <= t1 otherwise.
 __device__ __forceinline__ bool sphere_roots(float fx, float fy, float fz, float dx, float dy, float dz, float rad,
                                              float& t0, float& t1) {
+    PRT_EXACT_ROUNDING
     float A = dx * dx + dy * dy + dz * dz;
     float hb = fx * dx + fy * dy + fz * dz;
     float k = hb / A;
@@ -139,6 +147,7 @@ __device__ __forceinline__ bool sphere_test(const TRay& r, float4 a, float4 b, f
 // Quad test (quad.go:133-180).
 __device__ __forceinline__ bool quad_test(const TRay& r, float4 a, float4 b, float4 c, const QuadExt& q,
                                           float tbest, float& t_out, float& u_out, float& v_out) {
+    PRT_EXACT_ROUNDING
     float den = r.dx * q.nx + r.dy * q.ny + r.dz * q.nz;
     if (fabsf(den) < 1e-8f) return false;
     float t = (q.D - (r.ox * q.nx + r.oy * q.ny + r.oz * q.nz)) / den;
@@ -158,6 +167,7 @@ __device__ __forceinline__ bool quad_test(const TRay& r, float4 a, float4 b, flo
 
 // Disc test (disc.go:46-88): plane hit with |n.d| >= 1e-6, t in [tmin, tbest], |p - c|^2 <= r^2.
 __device__ __forceinline__ bool disc_test(const TRay& r, float4 a, float4 b, float4 c, float tbest, float& t_out) {
+    PRT_EXACT_ROUNDING
     float den = b.x * r.dx + b.y * r.dy + b.z * r.dz;
     if (fabsf(den) < 1e-6f) return false;
     float t = (b.x * (a.x - r.ox) + b.y * (a.y - r.oy) + b.z * (a.z - r.oz)) / den;
@@ -172,6 +182,7 @@ __device__ __forceinline__ bool disc_test(const TRay& r, float4 a, float4 b, flo
 // (cx,cy,cz) with normal +-(vx,vy,vz), |d.n| >= 1e-8, t in [tmin, tmax], |p - c| <= rad.
 __device__ __forceinline__ bool cap_test(const TRay& r, float cx, float cy, float cz, float vx, float vy, float vz,
                                          float rad, float tmax, float& t_out) {
+    PRT_EXACT_ROUNDING
     float den = vx * r.dx + vy * r.dy + vz * r.dz;
     if (fabsf(den) < 1e-8f) return false;
     float t = (vx * (cx - r.ox) + vy * (cy - r.oy) + vz * (cz - r.oz)) / den;
@@ -189,6 +200,7 @@ __device__ __forceinline__ bool cap_test(const TRay& r, float cx, float cy, floa
 // distant origins keep their accuracy. part: 0 body, 1 base cap, 2 top cap.
 __device__ __forceinline__ bool cylinder_test(const TRay& r, float4 a, float4 b, float4 c, float tbest, float& t_out,
                                               float& part) {
+    PRT_EXACT_ROUNDING
     const float vx = b.x, vy = b.y, vz = b.z, rad = c.x, H = c.y;
     float ex = r.ox - a.x, ey = r.oy - a.y, ez = r.oz - a.z;
     float dv = r.dx * vx + r.dy * vy + r.dz * vz, ev = ex * vx + ey * vy + ez * vz;
@@ -244,6 +256,7 @@ __device__ __forceinline__ bool cylinder_test(const TRay& r, float4 a, float4 b,
 // [-1e-8, H + 1e-8] and on the apex's lower nappe. Caps: the base, and the top of a frustum.
 __device__ __forceinline__ bool cone_test(const TRay& r, float4 a, float4 b, float4 c, float tbest, float& t_out,
                                           float& part) {
+    PRT_EXACT_ROUNDING
     const float vx = b.x, vy = b.y, vz = b.z, rb = c.x, rt = c.y, H = c.z;
     const float tn = (rb - rt) / H, k1 = 1.0f + tn * tn;
     const float ha = rb / tn;  // apex height above the base

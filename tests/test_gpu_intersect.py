@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import devmem as D
 import prtgpu as P
 import scenes
 
@@ -195,3 +196,47 @@ def test_every_kernel_variant_gives_the_same_hits(name, monkeypatch):
         else:
             assert np.array_equal(h.view(np.uint32), ref_h.view(np.uint32)), v
             assert np.array_equal(o, ref_o), v
+
+
+@pytest.mark.parametrize("name", ["mesh", "default"])
+def test_parked_rays_give_the_same_hits(name, monkeypatch):
+    """Rays still traversing after the park budget (PRT_TRACE_PARK_STEPS, default 48) are finished by a
+    whole wave each (k_intersect_long); the closest hit is the least (t, primitive index) pair whichever
+    kernel finds it, so hits and occlusion are bit-identical without parking (0), at the default, and at
+    budgets of 8 and 2 steps (at 2, more rays park than the 2^18-entry queue takes: the refused ones keep
+    traversing). Every kernel variant is checked at budget 2, and so is the allocation-free entry point
+    (its park queue of 2^16 entries lives in its static scratch)."""
+    if name == "mesh":
+        s, info = scenes.mesh_scene(frequency=128)
+        v = info["vertices"]
+        rays = P.random_rays(v.min(0), v.max(0), 1 << 20, scale=2.0, seed=9)
+    else:
+        s, _ = scenes.default_scene()
+        rays = _random_rays_in_box([-3, 0.01, -4], [3, 3, 3], 1 << 20, 17)
+    monkeypatch.setenv("PRT_TRACE_VARIANT", "5")
+    monkeypatch.setenv("PRT_TRACE_PARK_STEPS", "0")
+    base = P.Scene(s)
+    ref_h, ref_o = base.intersect_closest(rays), base.intersect_any(rays)
+    assert (ref_h["prim_id"] >= 0).mean() > 0.05
+    cases = [(b, "5") for b in (None, "8", "2")] + [("2", str(v)) for v in range(8) if v != 5]
+    for budget, var in cases:
+        if budget is None:
+            monkeypatch.delenv("PRT_TRACE_PARK_STEPS", raising=False)
+        else:
+            monkeypatch.setenv("PRT_TRACE_PARK_STEPS", budget)
+        monkeypatch.setenv("PRT_TRACE_VARIANT", var)
+        scene = P.Scene(s)
+        h, o = scene.intersect_closest(rays), scene.intersect_any(rays)
+        assert np.array_equal(h.view(np.uint32), ref_h.view(np.uint32)), (budget, var)
+        assert np.array_equal(o, ref_o), (budget, var)
+        if budget == "2" and var == "5":
+            d_rays = D.DeviceBuffer.from_array(rays)
+            out = D.DeviceBuffer(rays.shape[0] * 16)
+            stream = D.Stream()
+            scene.intersect_closest_static(d_rays.ptr, rays.shape[0], out.ptr, stream.handle)
+            stream.synchronize()
+            got = out.to_array(P.HIT_DTYPE, rays.shape[0])
+            assert np.array_equal(got.view(np.uint32), ref_h.view(np.uint32)), "allocation-free entry point"
+            out.free()
+            d_rays.free()
+        scene.close()
