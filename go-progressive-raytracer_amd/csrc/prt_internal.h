@@ -38,6 +38,20 @@ struct TuneJob {
     hipEvent_t ev[24] = {};
 };
 
+// Survival profile of a scene's path-tracing chunks: the fraction of a chunk's paths still traced at each
+// bounce, copied back from the device without a host wait (the copy is enqueued after a chunk and read by
+// a later chunk of any film of the scene once its event has completed). Chunks pick the bounce from which
+// one launch finishes the remaining paths by it.
+struct PathProfile {
+    std::mutex mu;
+    std::vector<double> frac;    // frac[b]: paths at bounce b / paths at bounce 0; negative: unknown
+    uint32_t* host = nullptr;    // pinned copy target
+    int host_len = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    int pending_upto = -1;       // bounces 0 .. pending_upto of the pending copy were recorded
+};
+
 }  // namespace prt
 
 struct prt_scene {
@@ -57,6 +71,7 @@ struct prt_scene {
     std::atomic<int> pt_trace_gate{-1};     // autotuned leaf gate of the path tracer's trace kernel (eighths)
     std::atomic<int> bdpt_trace_gate{-1};   // autotuned leaf gate of the BDPT kernels' traversals (eighths)
     prt::TuneJob pt_tune, bdpt_tune;
+    prt::PathProfile pt_profile;
     int forced_variant = -1;          // PRT_TRACE_VARIANT override
     int variant_blocks[16] = {0};
     std::vector<uint32_t> orig_id;  // host copy: leaf order -> original primitive id

@@ -49,6 +49,7 @@ struct PathBuffers {
                           // long-ray queue 4
     uint32_t* lq;         // long rays parked by a trace launch (path slot; kLongAny set for shadow rays)
     uint32_t lcap;        // capacity of lq
+    uint32_t* bounce_paths;  // paths traced at each bounce of the last chunk (written by k_trace)
 };
 
 // Long rays: a trace launch parks a ray that is still traversing after its step budget in the long-ray
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
     if (closest && blockIdx.x == 0 && threadIdx.x == 0) {
         *qlen(pb, cur ^ 1u) = 0;  // next bounce's queue (its last reader finished earlier)
         *qlen(pb, 2u + cur) = 0;  // the shadow queue this bounce's shading fills (the last launch read it)
+        if (tune_stride == 1) pb.bounce_paths[bounce] = *qlen(pb, cur);
     }
     const uint32_t nc64 = (nc + 63u) & ~63u;
     const uint32_t stride = gridDim.x * kBlock;
@@ -335,6 +337,99 @@ __global__ __launch_bounds__(64) void k_trace_long(DevScene sc, PathBuffers pb, 
     }
 }
 
+// One path's bounce b (PathTracingIntegrator.rayColorRecursive, path_tracing.go:35-189) from its hit
+// record: adds the emission (or the infinite lights on a miss) to L, samples the BSDF and a light, and
+// writes the continuation (cont: origin, direction, weights) and the shadow ray (shadow: direction, tmax,
+// contribution) of path slot `slot`.
+__device__ __forceinline__ void shade_path(const DevScene& sc, const PassParams& pp, const PathBuffers& pb,
+                                           uint32_t slot, int b, bool& cont, bool& shadow) {
+    cont = false;
+    shadow = false;
+    float4 o4 = pb.o[slot], d4 = pb.d[slot], h4 = pb.hit[slot];
+    float4 b4 = pb.beta[slot];
+    uint32_t key = pb.key[slot];
+    V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+    V3 beta = v3(b4.x, b4.y, b4.z), thr = v3(o4.w, d4.w, b4.w);
+    // L is read and written only when this bounce adds to it (a zero term leaves it unchanged;
+    // a non-finite one is added as the reference adds it)
+    V3 add = v3(0, 0, 0);
+    THit h;
+    h.t = h4.x;
+    h.prim = __float_as_uint(h4.y);
+    h.u = h4.z;
+    h.v = h4.w;
+    const uint32_t pos = (uint32_t)(b + 1) * 16u;
+    if (h.prim == 0xFFFFFFFFu) {
+        // miss: sum of infinite-light emission (:49-60)
+        V3 e = v3(0, 0, 0);
+        for (int li = 0; li < sc.num_lights; li++)
+            if (light_is_infinite(sc.lights[li].type)) e = e + light_emit(sc, sc.lights[li], d);
+        add = mulv(beta, e);
+    } else {
+        Surf s = make_surf(sc, o, d, h);
+        add = mulv(beta, emit_hit(sc, s.mat, d, s.front));  // getEmittedLight :100-105
+        ScatterOut so;
+        if (scatter(sc, s.mat, d, s, key, pos + 1, so)) {
+            V3 nd = so.dir;
+            if (so.pdf <= 0.0f) {  // specular (:88-97)
+                beta = mulv(beta, so.att);
+                thr = mulv(thr, so.att);
+                cont = true;
+            } else {
+                // direct lighting (:116-157), light draws at positions 8..10
+                if (sc.num_lights > 0) {
+                    float prob;
+                    int li = select_light(sc, draw(key, pos + 8), prob);
+                    LSample ls = light_sample(sc, sc.lights[li], s.p, s.n, draw(key, pos + 9), draw(key, pos + 10));
+                    ls.pdf *= prob;
+                    float cosine = dot(ls.dir, s.n);
+                    if (lum(ls.emission) > 0.0f && ls.pdf > 0.0f && cosine > 0.0f) {
+                        bool delta;
+                        float mp = mat_pdf(sc, s.mat, d, ls.dir, s.n, delta);
+                        if (!delta) {
+                            float w = power_heuristic(ls.pdf, mp);
+                            V3 f = brdf(sc, s.mat, d, ls.dir, s, 0);
+                            V3 c = mulv(beta, mulv(f, ls.emission) * (cosine * w / ls.pdf));
+                            if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+                                float tmax = ls.dist - 0.001f;
+                                pb.sd[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, tmax);
+                                pb.sc[slot] = make_float4(c.x, c.y, c.z, 0.f);
+                                shadow = true;
+                            }
+                        }
+                    }
+                }
+                // indirect lighting (:160-189)
+                V3 sd = normalize(nd);
+                float cosine = dot(sd, s.n);
+                if (cosine > 0.0f) {
+                    float lp = total_light_pdf(sc, s.p, s.n, sd);
+                    float w = power_heuristic(so.pdf, lp);
+                    thr = mulv(thr, so.att) * (cosine / so.pdf);
+                    beta = mulv(beta, so.att) * (cosine * w / so.pdf);
+                    cont = true;
+                }
+            }
+            if (cont) d = nd;
+            if (cont || shadow) o = s.p;
+        }
+    }
+    // next level: depth limit (:37-39) and Russian roulette (:42-45)
+    if (cont) {
+        if (b + 1 >= pp.max_depth) cont = false;
+        else cont = russian_roulette(pp, key, b + 1, thr, beta);
+    }
+    if (add.x != 0.0f || add.y != 0.0f || add.z != 0.0f) {
+        const float4 l4 = pb.L[slot];
+        pb.L[slot] = make_float4(l4.x + add.x, l4.y + add.y, l4.z + add.z, 0.f);
+    }
+    if (cont || shadow) pb.o[slot] = make_float4(o.x, o.y, o.z, thr.x);
+    if (cont) {
+        pb.d[slot] = make_float4(d.x, d.y, d.z, thr.y);
+        pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, thr.z);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, PathBuffers pb, int b) {
     __shared__ uint32_t s_push[4];
     // the long-ray queue of the next trace launch starts empty (this bounce's k_trace_long has finished)
@@ -343,7 +438,6 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
     const uint32_t count = *qlen(pb, cur);
     const uint32_t* q = pb.queue[cur];
     const uint32_t stride = gridDim.x * kBlock;
-    const float kInfF = __int_as_float(0x7f800000);
     // block-uniform trip count (the queue pushes synchronize the block)
     for (uint32_t base = blockIdx.x * kBlock; base < count; base += stride) {
         uint32_t i = base + threadIdx.x;
@@ -351,96 +445,61 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
         uint32_t slot = 0;
         if (i < count) {
             slot = q[i];
-            float4 o4 = pb.o[slot], d4 = pb.d[slot], h4 = pb.hit[slot];
-            float4 b4 = pb.beta[slot];
-            uint32_t key = pb.key[slot];
-            V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
-            V3 beta = v3(b4.x, b4.y, b4.z), thr = v3(o4.w, d4.w, b4.w);
-            // L is read and written only when this bounce adds to it (a zero term leaves it unchanged;
-            // a non-finite one is added as the reference adds it)
-            V3 add = v3(0, 0, 0);
-            THit h;
-            h.t = h4.x;
-            h.prim = __float_as_uint(h4.y);
-            h.u = h4.z;
-            h.v = h4.w;
-            const uint32_t pos = (uint32_t)(b + 1) * 16u;
-            if (h.prim == 0xFFFFFFFFu) {
-                // miss: sum of infinite-light emission (:49-60)
-                V3 e = v3(0, 0, 0);
-                for (int li = 0; li < sc.num_lights; li++)
-                    if (light_is_infinite(sc.lights[li].type)) e = e + light_emit(sc, sc.lights[li], d);
-                add = mulv(beta, e);
-            } else {
-                Surf s = make_surf(sc, o, d, h);
-                add = mulv(beta, emit_hit(sc, s.mat, d, s.front));  // getEmittedLight :100-105
-                ScatterOut so;
-                if (scatter(sc, s.mat, d, s, key, pos + 1, so)) {
-                    V3 nd = so.dir;
-                    if (so.pdf <= 0.0f) {  // specular (:88-97)
-                        beta = mulv(beta, so.att);
-                        thr = mulv(thr, so.att);
-                        cont = true;
-                    } else {
-                        // direct lighting (:116-157), light draws at positions 8..10
-                        if (sc.num_lights > 0) {
-                            float prob;
-                            int li = select_light(sc, draw(key, pos + 8), prob);
-                            LSample ls = light_sample(sc, sc.lights[li], s.p, s.n, draw(key, pos + 9), draw(key, pos + 10));
-                            ls.pdf *= prob;
-                            float cosine = dot(ls.dir, s.n);
-                            if (lum(ls.emission) > 0.0f && ls.pdf > 0.0f && cosine > 0.0f) {
-                                bool delta;
-                                float mp = mat_pdf(sc, s.mat, d, ls.dir, s.n, delta);
-                                if (!delta) {
-                                    float w = power_heuristic(ls.pdf, mp);
-                                    V3 f = brdf(sc, s.mat, d, ls.dir, s, 0);
-                                    V3 c = mulv(beta, mulv(f, ls.emission) * (cosine * w / ls.pdf));
-                                    if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
-                                        float tmax = ls.dist - 0.001f;
-                                        pb.sd[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, tmax);
-                                        pb.sc[slot] = make_float4(c.x, c.y, c.z, 0.f);
-                                        shadow = true;
-                                    }
-                                }
-                            }
-                        }
-                        // indirect lighting (:160-189)
-                        V3 sd = normalize(nd);
-                        float cosine = dot(sd, s.n);
-                        if (cosine > 0.0f) {
-                            float lp = total_light_pdf(sc, s.p, s.n, sd);
-                            float w = power_heuristic(so.pdf, lp);
-                            thr = mulv(thr, so.att) * (cosine / so.pdf);
-                            beta = mulv(beta, so.att) * (cosine * w / so.pdf);
-                            cont = true;
-                        }
-                    }
-                    if (cont) d = nd;
-                    if (cont || shadow) o = s.p;
-                }
-            }
-            // next level: depth limit (:37-39) and Russian roulette (:42-45)
-            if (cont) {
-                if (b + 1 >= pp.max_depth) cont = false;
-                else cont = russian_roulette(pp, key, b + 1, thr, beta);
-            }
-            if (add.x != 0.0f || add.y != 0.0f || add.z != 0.0f) {
-                const float4 l4 = pb.L[slot];
-                pb.L[slot] = make_float4(l4.x + add.x, l4.y + add.y, l4.z + add.z, 0.f);
-            }
-            if (cont || shadow) pb.o[slot] = make_float4(o.x, o.y, o.z, thr.x);
-            if (cont) {
-                pb.d[slot] = make_float4(d.x, d.y, d.z, thr.y);
-                pb.beta[slot] = make_float4(beta.x, beta.y, beta.z, thr.z);
-            }
-            (void)kInfF;
+            shade_path(sc, pp, pb, slot, b, cont, shadow);
         }
         block_push2(s_push, pb.queue[cur ^ 1u], qlen(pb, cur ^ 1u), cont, pb.squeue[cur], qlen(pb, 2u + cur), shadow,
                     slot);
     }
 }
 
+// The last bounces of a chunk once few paths are left: a lane takes a path of bounce b0 (its hit traced)
+// and runs it to the end, shading, tracing its shadow ray and its next extension ray bounce after bounce,
+// in one launch instead of two per bounce. A path's radiance terms are added in the wavefront order
+// (emission of bounce b, shadow ray of bounce b, emission of bounce b + 1, ...), so films are identical.
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, PassParams pp, PathBuffers pb, int b0, uint2* gstack,
+                                                 uint32_t gstride) {
+    __shared__ uint32_t s_node[kLds * kBlock];
+    __shared__ float s_t[kLds * kBlock];
+    __shared__ float4 s_cache[kNodeCache * 8];
+    const uint32_t cur = (uint32_t)b0 & 1u;
+    const uint32_t count = *qlen(pb, cur);
+    if (blockIdx.x * kBlock >= count) return;
+    TStack<kLds, kBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kBlock + threadIdx.x;
+    st.gstride = gstride;
+    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
+    st.ncache = s_cache;
+    const uint32_t* q = pb.queue[cur];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += gridDim.x * kBlock) {
+        const uint32_t slot = q[i];
+        for (int b = b0;; b++) {
+            bool cont, shadow;
+            shade_path(sc, pp, pb, slot, b, cont, shadow);
+            if (!cont && !shadow) break;
+            const float4 o = pb.o[slot];
+            THit h;
+            if (shadow) {
+                const float4 sd = pb.sd[slot];
+                const TRay r{o.x, o.y, o.z, 0.001f, sd.x, sd.y, sd.z, sd.w};
+                if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr)) {
+                    const float4 c = pb.sc[slot];
+                    const float4 l = pb.L[slot];
+                    pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+                }
+            }
+            if (!cont) break;
+            const float4 d = pb.d[slot];
+            const TRay r{o.x, o.y, o.z, 0.001f, d.x, d.y, d.z, __int_as_float(0x7f800000)};
+            trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
+            pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+        }
+    }
+}
 
 // ---- bidirectional path tracing (BDPTIntegrator.RayColor, pkg/integrator/bdpt.go:73-102) ----------
 
@@ -1131,6 +1190,8 @@ struct prt_film {
     uint32_t park_steps = 0;  // step budget of the path tracer's traversals before a ray is parked (0: none)
     uint32_t park_steps_small = 0;  // the budget of launches with few rays per lane of the trace grid
     uint32_t park_small_rays_per_lane = 1;  // launches of at most this many rays per lane are "few"
+    int64_t tail_paths = 0;  // the tail kernel finishes a chunk from the first bounce expected to hold this few paths
+    int tail_blocks = 0;     // its grid
     int long_grid = 0;        // work-groups (one wave each) of k_trace_long
     // BDPT: camera and light subpath stores and subpath lengths, per path slot
     bool bdpt = false;
@@ -1345,6 +1406,27 @@ uint32_t park_small_rays_per_lane() {
         if (v >= 1 && v < (1l << 20)) return (uint32_t)v;
     }
     return 4u;
+}
+
+// Resident work-groups of k_tail (at most the trace grid, whose spill-stack slots it uses).
+int tail_grid(const prt_scene* scene, int trace_blocks) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_tail<true>, kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu));
+}
+
+// Paths below which the last bounces of a chunk run as one launch (k_tail): the threads k_tail keeps
+// resident (C0's first pass 1.4 -> 0.9 ms and 64-spp render 1.52 -> 1.79 G samples/s at 2^18 on 256 CUs;
+// 2^16 gave 0.9 ms and 1.75 G, 2^20 0.85 ms and 1.32 G); none in scenes that park long rays (the tail
+// kernel traverses with one lane per ray). PRT_PT_TAIL_PATHS overrides (0: never).
+int64_t tail_paths(int tail_blocks, bool parks) {
+    if (const char* e = std::getenv("PRT_PT_TAIL_PATHS")) {
+        const long long v = std::atoll(e);
+        if (v >= 0) return (int64_t)v;
+    }
+    return parks ? 0 : (int64_t)tail_blocks * kBlock;
 }
 
 // Resident one-wave work-groups of k_trace_long.
@@ -1569,6 +1651,61 @@ static int pick_gate(const float* t) {
     return bestg;
 }
 
+// Reads a completed profile copy into the scene's survival fractions (caller holds the profile lock).
+static void profile_collect(PathProfile& pf) {
+    if (!pf.pending || hipEventQuery(pf.ev) != hipSuccess) return;
+    pf.pending = false;
+    const double n0 = (double)pf.host[0];
+    if (n0 <= 0.0) return;
+    if ((int)pf.frac.size() < pf.pending_upto + 1) pf.frac.resize(pf.pending_upto + 1, -1.0);
+    for (int b = 0; b <= pf.pending_upto; b++) pf.frac[b] = (double)pf.host[b] / n0;
+}
+
+// First bounce b >= 1 at which a chunk of n paths is expected to trace at most `limit` paths (by the
+// scene's survival profile), else max_depth (no tail).
+static int tail_bounce(prt_scene* s, int64_t n, int64_t limit, int max_depth) {
+    if (limit <= 0 || max_depth < 2) return max_depth;
+    std::lock_guard<std::mutex> lk(s->pt_profile.mu);
+    profile_collect(s->pt_profile);
+    const std::vector<double>& fr = s->pt_profile.frac;
+    for (int b = 1; b < max_depth && b < (int)fr.size(); b++)
+        if (fr[b] >= 0.0 && (double)n * fr[b] <= (double)limit) return b;
+    return max_depth;
+}
+
+// After a chunk: copies its paths per bounce (bounces 0 .. upto) to the scene's pinned buffer, unless a
+// copy is pending.
+static hipError_t profile_record(prt_scene* s, prt_film* f, int upto, hipStream_t st) {
+    PathProfile& pf = s->pt_profile;
+    std::lock_guard<std::mutex> lk(pf.mu);
+    profile_collect(pf);
+    if (pf.pending) return hipSuccess;
+    if (pf.host_len < upto + 1) {
+        if (pf.host) {
+            hipError_t e = hipHostFree(pf.host);
+            pf.host = nullptr;
+            pf.host_len = 0;
+            if (e != hipSuccess) return e;
+        }
+        const int len = std::max(upto + 1, 64);
+        hipError_t e = hipHostMalloc((void**)&pf.host, sizeof(uint32_t) * (size_t)len, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        pf.host_len = len;
+    }
+    if (!pf.ev) {
+        hipError_t e = hipEventCreateWithFlags(&pf.ev, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipMemcpyAsync(pf.host, f->pb.bounce_paths, sizeof(uint32_t) * (size_t)(upto + 1),
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(pf.ev, st);
+    if (e == hipSuccess) {
+        pf.pending = true;
+        pf.pending_upto = upto;
+    }
+    return e;
+}
+
 // Enqueues the next planned chunk on the film's stream (no wait); ev0/ev1 bracket it.
 int film_enqueue_chunk(prt_film* f) {
     if (film_round_exhausted(f)) return PRT_OK;
@@ -1757,14 +1894,30 @@ int film_enqueue_chunk(prt_film* f) {
         if (f->probe_flags) f->probe_launches++;
         return e;
     };
-    for (int b = 0; b < pp.max_depth; b++) {
+    // the bounce from which k_tail finishes the chunk: the first the scene's survival profile expects to
+    // hold at most tail_paths paths (probed chunks run every bounce as wavefront launches)
+    const int b0 = f->probe_flags ? pp.max_depth : tail_bounce(ms, n, f->tail_paths, pp.max_depth);
+    for (int b = 0; b < b0; b++) {
         if (b == 1 && tune_here) PRT_HIP(tune_bounce1());
         PRT_HIP(trace(b, 1, b > 0 ? ((b - 1) & 1) : -1));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_SHADE, true));
         hipLaunchKernelGGL(k_shade, dim3(f->blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_SHADE, false));
     }
-    if (pp.max_depth > 0) PRT_HIP(trace(pp.max_depth, 0, (pp.max_depth - 1) & 1));
+    if (b0 < pp.max_depth) {
+        // bounce b0's trace (with the shadow rays of b0 - 1), then every later bounce in one launch
+        PRT_HIP(trace(b0, 1, b0 > 0 ? ((b0 - 1) & 1) : -1));
+        if (sc.mixed)
+            hipLaunchKernelGGL(k_tail<true>, dim3(f->tail_blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b0, f->gstack,
+                               f->gslots);
+        else
+            hipLaunchKernelGGL(k_tail<false>, dim3(f->tail_blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b0, f->gstack,
+                               f->gslots);
+    } else if (pp.max_depth > 0) {
+        PRT_HIP(trace(pp.max_depth, 0, (pp.max_depth - 1) & 1));
+    }
+    // paths per bounce of this chunk (bounces 0 .. b0 ran as wavefront trace launches) for the profile
+    if (!f->probe_flags && pp.max_depth > 0 && !f->adaptive) PRT_HIP(profile_record(ms, f, std::min(b0, pp.max_depth - 1), st));
     int film_blocks_n = (pp.npix + kBlock - 1) / kBlock;
     PRT_HIP(probe_mark(f, PRT_PROBE_KIND_FILM, true));
     if (f->adaptive)
@@ -1866,6 +2019,7 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     if (p->adaptive_min_samples < 1.0) b += 2 * sizeof(uint32_t) * npix + 6 * sizeof(uint32_t);
     b += sizeof(uchar4) * (size_t)npix;  // the device image of RGBA8 reads, allocated at the first read
     b += sizeof(uint32_t) * (size_t)std::min<int64_t>(2 * C, kLongCapacity);  // long-ray queue
+    b += sizeof(uint32_t) * (size_t)(p->max_depth + 1);                          // paths per bounce
     return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + b + 3 * sizeof(uint32_t) * (size_t)C;
 }
 
@@ -1930,6 +2084,7 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     FALLOC(f->pb.counters, uint32_t, kNumCounters * kCounterStride);
     f->pb.lcap = (uint32_t)std::min<int64_t>(2 * C, kLongCapacity);
     FALLOC(f->pb.lq, uint32_t, f->pb.lcap);
+    FALLOC(f->pb.bounce_paths, uint32_t, p->max_depth + 1);
     FALLOC(f->fb.sum, float4, f->npix);
     FALLOC(f->fb.comp, float4, f->npix);
     FALLOC(f->fb.sq, float2, f->npix);
@@ -1946,6 +2101,8 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     f->park_steps = park_steps(scene, false);
     f->park_steps_small = park_steps(scene, true);
     f->park_small_rays_per_lane = park_small_rays_per_lane();
+    f->tail_blocks = tail_grid(scene, f->trace_grid[scene->pt_trace_variant]);
+    f->tail_paths = tail_paths(f->tail_blocks, f->park_steps != 0 || f->park_steps_small != 0);
     f->long_grid = (int)std::min<uint32_t>(f->gslots / 64u, (uint32_t)long_grid(scene));
     if ((rc = film_alloc(f, film_gstack_entries(scene) * f->gslots * sizeof(uint2), &ptr)) != PRT_OK) return bail(rc);
     f->gstack = (uint2*)ptr;

@@ -1422,6 +1422,11 @@ int prt_scene_destroy(prt_scene* scene) {
                 (void)hipEventSynchronize(e);
                 (void)hipEventDestroy(e);
             }
+    if (scene->pt_profile.ev) {
+        (void)hipEventSynchronize(scene->pt_profile.ev);
+        (void)hipEventDestroy(scene->pt_profile.ev);
+    }
+    if (scene->pt_profile.host) (void)hipHostFree(scene->pt_profile.host);
     for (auto& b : scene->bufs) (void)hipFree(b.p);
     delete scene;
     return PRT_OK;

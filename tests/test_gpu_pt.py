@@ -270,3 +270,34 @@ def test_leaf_gate_leaves_films_unchanged(integrator, monkeypatch):
     for other in films[1:]:
         for x, y in zip(films[0], other):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("name", ["default", "cornell_metre", "mesh_under_sky"])
+def test_tail_kernel_leaves_films_unchanged(name, monkeypatch):
+    """From the first bounce at which a chunk is expected to trace few paths (PRT_PT_TAIL_PATHS, default
+    2^16, read from the survival profile that earlier chunks of the scene record on the device), one launch
+    (k_tail) runs every remaining path to its end: shading, its shadow ray and its next ray bounce after
+    bounce. A path's radiance terms keep the wavefront order, so films are bit-identical with the tail
+    switched off, forced from bounce 1, and from an intermediate bounce. The first film of each scene
+    records the profile; the second uses it."""
+    if name == "default":
+        s, cfg = scenes.default_scene(width=96)
+    elif name == "cornell_metre":
+        s, cfg = scenes.cornell_metre(width=64)
+    else:  # triangles only, under a uniform sky
+        s, _ = scenes.mesh_scene(frequency=24, width=64)
+        s.add_uniform_infinite_light((0.8, 0.9, 1.0))
+        cfg = dict(max_depth=12, rr_min_bounces=3)
+    films = []
+    for limit in ("0", str(1 << 40), "2000"):
+        monkeypatch.setenv("PRT_PT_TAIL_PATHS", limit)
+        scene = P.Scene(s)
+        for _ in range(2):
+            pr = P.ProgressiveRaytracer(scene, params_for(s, cfg, 8, seed=6))
+            pr.render_samples(8)
+            films.append(pr.read())
+            pr.close()
+    assert films[0][0].mean() > 0
+    for other in films[1:]:
+        for x, y in zip(films[0], other):
+            assert np.array_equal(x, y)
