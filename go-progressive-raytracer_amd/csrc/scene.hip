@@ -442,6 +442,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
         if (__ballot(active || nidx >= 0) == 0) break;  // exhausted and drained
         const bool leaf_ok = GATE > 0 ? leaf_gate(active, s.cur, GATE) : true;
         if (active) {
+            // a lane held at a leaf by the gate makes no progress this step and does not count it
+            const bool progress = leaf_ok || !child_is_leaf(s.cur);
             if (tstate_step<ANY, MIXED, false, LDS, BLOCK, NCACHE>(sc, s, st, nullptr, leaf_ok)) {
                 active = false;
                 bool hit = s.best != 0xFFFFFFFFu;
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
                         make_float4(hit ? s.tbest : __int_as_float(0x7f800000), __int_as_float(pid),
                                     hit ? s.bu : 0.0f, hit ? s.bv : 0.0f);
                 }
-            } else if (++steps == park.budget) {
+            } else if (progress && ++steps == park.budget) {
                 const uint32_t k = atomicAdd(park.count, 1u);
                 if (k < park.cap) {
                     park.q[k] = (uint32_t)ridx;

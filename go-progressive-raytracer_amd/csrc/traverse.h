@@ -548,9 +548,9 @@ __device__ __forceinline__ bool trace(const DevScene& sc, const TRay& r_in, THit
     return s.best != 0xFFFFFFFFu;
 }
 
-// trace() with a step budget. After `budget` steps (0: no budget) an unfinished ray offers itself to
-// `park` (a callable returning true when it took the ray) and, when taken, stops with its best hit so
-// far in `hit`; a ray the callable refuses keeps traversing. Returns kTraceMiss, kTraceHit or
+// trace() with a step budget. After `budget` steps that made progress (0: no budget) an unfinished ray
+// offers itself to `park` (a callable returning true when it took the ray) and, when taken, stops with
+// its best hit so far in `hit`; a ray the callable refuses keeps traversing. Returns kTraceMiss, kTraceHit or
 // kTraceParked. Whether a ray is parked does not change its result: the closest hit is the least
 // (t, primitive index) pair whichever traversal finds it.
 constexpr int kTraceMiss = 0, kTraceHit = 1, kTraceParked = 2;
@@ -564,8 +564,9 @@ __device__ __forceinline__ int trace_budget(const DevScene& sc, const TRay& r_in
     bool parked = false;
     for (;;) {
         const bool ok = gate > 0 ? leaf_gate(true, s.cur, gate) : true;
+        const bool progress = ok || !child_is_leaf(s.cur);  // a lane held at a leaf by the gate does not count
         if (tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK, NCACHE>(sc, s, st, cnt, ok)) break;
-        if (++steps == budget && park()) {
+        if (progress && ++steps == budget && park()) {
             parked = true;
             break;
         }
