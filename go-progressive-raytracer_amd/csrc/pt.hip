@@ -289,16 +289,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES)))
     }
 }
 
-// The rays the last trace launch parked, one wave per ray (coop_trace): closest hits complete their hit
-// records, unoccluded shadow rays add their contribution. Their results equal the one-lane traversal's.
-template <bool MIXED>
+// The rays the last trace launch parked, 64 / G per one-wave work-group (coop_trace with groups of G
+// lanes): closest hits complete their hit records, unoccluded shadow rays add their contribution. Their
+// results equal the one-lane traversal's.
+template <bool MIXED, int G>
 __global__ __launch_bounds__(64) void k_trace_long(DevScene sc, PathBuffers pb, uint2* gstack, uint32_t gstride) {
+    constexpr uint32_t R = 64 / G;  // rays per work-group and round
     __shared__ uint32_t s_node[kLds * 64];
     __shared__ float s_t[kLds * 64];
-    __shared__ uint32_t f_node[2 * kCoopFrontier];
-    __shared__ float f_t[2 * kCoopFrontier];
+    __shared__ uint32_t f_node[kCoopEntries];
+    __shared__ float f_t[kCoopEntries];
     const uint32_t n = min(*qlen(pb, kLongQueue), pb.lcap);
-    if (blockIdx.x >= n) return;
+    if (blockIdx.x * R >= n) return;
     TStack<kLds, 64> st;
     st.s_node = s_node;
     st.s_t = s_t;
@@ -306,34 +308,43 @@ __global__ __launch_bounds__(64) void k_trace_long(DevScene sc, PathBuffers pb, 
     st.lane = threadIdx.x;
     st.gslot = blockIdx.x * 64 + threadIdx.x;
     st.gstride = gstride;
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const uint32_t v = pb.lq[i];
+    const float kInfF = __int_as_float(0x7f800000);
+    for (uint32_t base = blockIdx.x * R; base < n; base += gridDim.x * R) {  // uniform over the work-group
+        const uint32_t i = base + threadIdx.x / (uint32_t)G;
+        const bool has = i < n;
+        const uint32_t v = has ? pb.lq[i] : 0u;
         const bool any = (v & kLongAny) != 0;
         const uint32_t slot = v & ~kLongAny;
-        const float4 o = pb.o[slot];
-        const float4 d = any ? pb.sd[slot] : pb.d[slot];
-        const TRay r{o.x, o.y, o.z, 0.001f, d.x, d.y, d.z, any ? d.w : __int_as_float(0x7f800000)};
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
         THit h;
-        if (any) {
-            h.t = __int_as_float(0x7f800000);
-            h.prim = 0xFFFFFFFFu;
-            h.u = h.v = 0.0f;
-            const bool occluded = coop_trace<true, MIXED, kLds>(sc, r, h, st, f_node, f_t);
-            if (!occluded && threadIdx.x == 0) {
-                float4 c = pb.sc[slot];
-                float4 l = pb.L[slot];
-                pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+        h.t = kInfF;
+        h.prim = 0xFFFFFFFFu;
+        h.u = h.v = 0.0f;
+        if (has) {
+            o = pb.o[slot];
+            d = any ? pb.sd[slot] : pb.d[slot];
+            if (!any) {
+                const float4 hp = pb.hit[slot];
+                h.t = hp.x;
+                h.prim = __float_as_uint(hp.y);
+                h.u = hp.z;
+                h.v = hp.w;
             }
-        } else {
-            const float4 hp = pb.hit[slot];
-            h.t = hp.x;
-            h.prim = __float_as_uint(hp.y);
-            h.u = hp.z;
-            h.v = hp.w;
-            coop_trace<false, MIXED, kLds>(sc, r, h, st, f_node, f_t);
-            if (threadIdx.x == 0) pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
         }
-        __syncthreads();  // the frontier buffers are reused by the next ray
+        const TRay r{o.x, o.y, o.z, 0.001f, d.x, d.y, d.z, any ? d.w : kInfF};
+        const bool found = coop_trace<MIXED, kLds, G>(sc, r, h, st, f_node, f_t, has, any);
+        if (has && (threadIdx.x & (uint32_t)(G - 1)) == 0) {
+            if (any) {
+                if (!found) {
+                    const float4 c = pb.sc[slot];
+                    const float4 l = pb.L[slot];
+                    pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+                }
+            } else {
+                pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+            }
+        }
+        __syncthreads();  // the frontier buffers are reused by the next rays
     }
 }
 
@@ -1193,6 +1204,7 @@ struct prt_film {
     int64_t tail_paths = 0;  // the tail kernel finishes a chunk from the first bounce expected to hold this few paths
     int tail_blocks = 0;     // its grid
     int long_grid = 0;        // work-groups (one wave each) of k_trace_long
+    int coop_lanes = 64;      // its lanes per parked ray
     // BDPT: camera and light subpath stores and subpath lengths, per path slot
     bool bdpt = false;
     int bdpt_blocks = 0;
@@ -1391,6 +1403,9 @@ int film_blocks(const prt_scene* scene) {
 // grazing rays along a dense mesh) park rays after 128 steps in launches that fill the grid, after 64 in
 // smaller ones; PRT_PT_PARK_STEPS and PRT_PT_PARK_STEPS_SMALL override the budgets (0: no parking).
 constexpr int64_t kLongCapacity = (int64_t)1 << 20;
+// Lanes per parked ray: a whole wave (C3 first pass 4.1-4.4 ms against 4.5-4.7 ms with 16 lanes and 4 rays
+// per wave: a launch parks too few rays for the groups' throughput to outweigh their longer latency).
+constexpr int kDefaultCoopLanes = 64;
 uint32_t park_steps(const prt_scene* scene, bool small) {
     if (const char* e = std::getenv(small ? "PRT_PT_PARK_STEPS_SMALL" : "PRT_PT_PARK_STEPS")) {
         const long v = std::atol(e);
@@ -1419,20 +1434,36 @@ int tail_grid(const prt_scene* scene, int trace_blocks) {
 
 // Paths below which the last bounces of a chunk run as one launch (k_tail): the threads k_tail keeps
 // resident (C0's first pass 1.4 -> 0.9 ms and 64-spp render 1.52 -> 1.79 G samples/s at 2^18 on 256 CUs;
-// 2^16 gave 0.9 ms and 1.75 G, 2^20 0.85 ms and 1.32 G); none in scenes that park long rays (the tail
-// kernel traverses with one lane per ray). PRT_PT_TAIL_PATHS overrides (0: never).
-int64_t tail_paths(int tail_blocks, bool parks) {
+// 2^16 gave 0.9 ms and 1.75 G, 2^20 0.85 ms and 1.32 G; C3 with parking 5.3 -> 4.1-4.4 ms and
+// 783 -> 789 M samples/s). PRT_PT_TAIL_PATHS overrides (0: never).
+int64_t tail_paths(int tail_blocks) {
     if (const char* e = std::getenv("PRT_PT_TAIL_PATHS")) {
         const long long v = std::atoll(e);
         if (v >= 0) return (int64_t)v;
     }
-    return parks ? 0 : (int64_t)tail_blocks * kBlock;
+    return (int64_t)tail_blocks * kBlock;
+}
+
+// k_trace_long with groups of `lanes` lanes per parked ray.
+const void* long_fn(bool mixed, int lanes) {
+    if (lanes == 16) return mixed ? (const void*)k_trace_long<true, 16> : (const void*)k_trace_long<false, 16>;
+    if (lanes == 32) return mixed ? (const void*)k_trace_long<true, 32> : (const void*)k_trace_long<false, 32>;
+    return mixed ? (const void*)k_trace_long<true, 64> : (const void*)k_trace_long<false, 64>;
+}
+
+// Lanes per parked ray of k_trace_long (16, 32 or 64); PRT_COOP_LANES overrides.
+int coop_lanes() {
+    if (const char* e = std::getenv("PRT_COOP_LANES")) {
+        const int v = std::atoi(e);
+        if (v == 16 || v == 32 || v == 64) return v;
+    }
+    return kDefaultCoopLanes;
 }
 
 // Resident one-wave work-groups of k_trace_long.
-int long_grid(const prt_scene* scene) {
+int long_grid(const prt_scene* scene, int lanes) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_trace_long<true>, 64, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, long_fn(true, lanes), 64, 0) != hipSuccess ||
         per_cu <= 0)
         per_cu = 16;
     return std::max(1, scene->num_cus) * std::min(per_cu, 32);
@@ -1844,11 +1875,10 @@ int film_enqueue_chunk(prt_film* f) {
         hipError_t e = hipLaunchKernel(trace_fn(sc.mixed != 0, counting, v), dim3(f->trace_grid[v]), dim3(kBlock),
                                        args, f->trace_pad[v], st);
         if (e != hipSuccess || (budget == 0 && budget_small == 0)) return e;
-        if (sc.mixed)
-            hipLaunchKernelGGL(k_trace_long<true>, dim3(f->long_grid), dim3(64), 0, st, sc, f->pb, f->gstack, f->gslots);
-        else
-            hipLaunchKernelGGL(k_trace_long<false>, dim3(f->long_grid), dim3(64), 0, st, sc, f->pb, f->gstack,
-                               f->gslots);
+        PathBuffers pbl = f->pb;
+        void* largs[] = {(void*)&sc, (void*)&pbl, (void*)&gs, (void*)&gsl};
+        e = hipLaunchKernel(long_fn(sc.mixed != 0, f->coop_lanes), dim3(f->long_grid), dim3(64), largs, 0, st);
+        if (e != hipSuccess) return e;
         return hipGetLastError();
     };
     // per-scene leaf gate of the trace kernel, on a chunk of at least 2^20 paths other than the film's first
@@ -2102,8 +2132,9 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     f->park_steps_small = park_steps(scene, true);
     f->park_small_rays_per_lane = park_small_rays_per_lane();
     f->tail_blocks = tail_grid(scene, f->trace_grid[scene->pt_trace_variant]);
-    f->tail_paths = tail_paths(f->tail_blocks, f->park_steps != 0 || f->park_steps_small != 0);
-    f->long_grid = (int)std::min<uint32_t>(f->gslots / 64u, (uint32_t)long_grid(scene));
+    f->tail_paths = tail_paths(f->tail_blocks);
+    f->coop_lanes = coop_lanes();
+    f->long_grid = (int)std::min<uint32_t>(f->gslots / 64u, (uint32_t)long_grid(scene, f->coop_lanes));
     if ((rc = film_alloc(f, film_gstack_entries(scene) * f->gslots * sizeof(uint2), &ptr)) != PRT_OK) return bail(rc);
     f->gstack = (uint2*)ptr;
     if (p->integrator == PRT_INTEGRATOR_BDPT) {

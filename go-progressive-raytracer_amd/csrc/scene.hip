@@ -365,6 +365,7 @@ uint32_t park_budget(const prt_scene* s, int64_t n, int64_t lanes) {
     return per_lane <= 1 ? 32u : (per_lane <= 16 ? 48u : 96u);
 }
 constexpr int kLongLds = 8;  // LDS stack entries per lane of k_intersect_long (deeper entries spill)
+constexpr int kLongLanes = 64;  // lanes per parked ray in k_intersect_long (large batches park few rays)
 
 template <bool ANY, bool MIXED, int LDS, int NCACHE, int BLOCK, int GATE = 0>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1 : 6))) void k_intersect_pf(DevScene sc, const prt_ray* __restrict__ rays, int64_t n,
@@ -469,18 +470,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
     }
 }
 
-// Finishes the rays k_intersect_pf parked, one wave per ray (coop_trace), and writes their results as
-// k_intersect_pf would have.
+// Finishes the rays k_intersect_pf parked, 64 / kLongLanes per one-wave work-group (coop_trace with groups
+// of kLongLanes lanes), and writes their results as k_intersect_pf would have.
 template <bool ANY, bool MIXED>
 __global__ __launch_bounds__(64) void k_intersect_long(DevScene sc, const prt_ray* __restrict__ rays,
                                                        prt_hit* __restrict__ hits, uint8_t* __restrict__ occluded,
                                                        uint2* gstack, uint32_t gstride, ParkQueue park) {
+    constexpr uint32_t R = 64 / kLongLanes;  // rays per work-group and round
     __shared__ uint32_t s_node[kLongLds * 64];
     __shared__ float s_t[kLongLds * 64];
-    __shared__ uint32_t f_node[2 * kCoopFrontier];
-    __shared__ float f_t[2 * kCoopFrontier];
+    __shared__ uint32_t f_node[kCoopEntries];
+    __shared__ float f_t[kCoopEntries];
     const uint32_t n = min(*park.count, park.cap);
-    if (blockIdx.x >= n) return;
+    if (blockIdx.x * R >= n) return;
     TStack<kLongLds, 64> st;
     st.s_node = s_node;
     st.s_t = s_t;
@@ -488,33 +490,40 @@ __global__ __launch_bounds__(64) void k_intersect_long(DevScene sc, const prt_ra
     st.lane = threadIdx.x;
     st.gslot = blockIdx.x * 64 + threadIdx.x;
     st.gstride = gstride;
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const uint32_t ridx = park.q[i];
-        const float4* rp = reinterpret_cast<const float4*>(rays + ridx);
-        const float4 a = rp[0], b = rp[1];
-        const TRay r{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    for (uint32_t base = blockIdx.x * R; base < n; base += gridDim.x * R) {  // uniform over the work-group
+        const uint32_t i = base + threadIdx.x / (uint32_t)kLongLanes;
+        const bool has = i < n;
+        const uint32_t ridx = has ? park.q[i] : 0u;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
         THit h;
-        if (ANY) {
-            h.t = __int_as_float(0x7f800000);
-            h.prim = 0xFFFFFFFFu;
-            h.u = h.v = 0.0f;
-            const bool occ = coop_trace<true, MIXED, kLongLds>(sc, r, h, st, f_node, f_t);
-            if (threadIdx.x == 0) occluded[ridx] = occ ? 1 : 0;
-        } else {
-            const float4 hp = reinterpret_cast<const float4*>(hits)[ridx];
-            h.t = hp.x;
-            h.prim = __float_as_uint(hp.y);
-            h.u = hp.z;
-            h.v = hp.w;
-            const bool hit = coop_trace<false, MIXED, kLongLds>(sc, r, h, st, f_node, f_t);
-            if (threadIdx.x == 0) {
+        h.t = __int_as_float(0x7f800000);
+        h.prim = 0xFFFFFFFFu;
+        h.u = h.v = 0.0f;
+        if (has) {
+            const float4* rp = reinterpret_cast<const float4*>(rays + ridx);
+            a = rp[0];
+            b = rp[1];
+            if (!ANY) {
+                const float4 hp = reinterpret_cast<const float4*>(hits)[ridx];
+                h.t = hp.x;
+                h.prim = __float_as_uint(hp.y);
+                h.u = hp.z;
+                h.v = hp.w;
+            }
+        }
+        const TRay r{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        const bool hit = coop_trace<MIXED, kLongLds, kLongLanes>(sc, r, h, st, f_node, f_t, has, ANY);
+        if (has && (threadIdx.x & (uint32_t)(kLongLanes - 1)) == 0) {
+            if (ANY) {
+                occluded[ridx] = hit ? 1 : 0;
+            } else {
                 const int32_t pid = hit ? (int32_t)__float_as_uint(sc.prims[3 * (size_t)h.prim + 2].w) : -1;
                 reinterpret_cast<float4*>(hits)[ridx] = make_float4(hit ? h.t : __int_as_float(0x7f800000),
                                                                     __int_as_float(pid), hit ? h.u : 0.0f,
                                                                     hit ? h.v : 0.0f);
             }
         }
-        __syncthreads();  // the frontier buffers are reused by the next ray
+        __syncthreads();  // the frontier buffers are reused by the next rays
     }
 }
 

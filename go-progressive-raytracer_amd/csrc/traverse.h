@@ -575,72 +575,94 @@ __device__ __forceinline__ int trace_budget(const DevScene& sc, const TRay& r_in
     return parked ? kTraceParked : (s.best != 0xFFFFFFFFu ? kTraceHit : kTraceMiss);
 }
 
-// Minimum of an unsigned value over the 64 lanes of a wave; every lane must be active. DPP steps reduce
-// each row of 16 lanes (swap neighbours, swap pairs, mirror halves of 8, mirror the row), then the four
-// row minima are read as scalars.
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+// Minimum of an unsigned value over each group of G lanes (G = 16, 32 or 64, groups aligned to G); every
+// lane of the wave must be active. DPP steps reduce each row of 16 lanes (swap neighbours, swap pairs,
+// mirror halves of 8, mirror the row); rows are then combined by lane swaps (G = 32) or read as scalars
+// (G = 64). Every lane receives its group's minimum.
+template <int G>
+__device__ __forceinline__ uint32_t group_min_u32(uint32_t v) {
     v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
     v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
     v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
     v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
-    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-    return min(min(a, b), min(c, d));
+    if constexpr (G == 32) {
+        v = min(v, (uint32_t)__shfl_xor((int)v, 16, 64));
+    } else if constexpr (G == 64) {
+        const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+        v = min(min(a, b), min(c, d));
+    }
+    return v;
 }
 
-// Minimum of a non-negative float (or +inf) over the wave's lanes: such floats order like their bits.
-__device__ __forceinline__ float wave_min_nonneg(float x) { return __uint_as_float(wave_min_u32(__float_as_uint(x))); }
+// Minimum of a non-negative float (or +inf) over each group of G lanes: such floats order like their bits.
+template <int G>
+__device__ __forceinline__ float group_min_nonneg(float x) {
+    return __uint_as_float(group_min_u32<G>(__float_as_uint(x)));
+}
 
-// Exclusive prefix sum over the wave's lanes of a count in [0, 7], and the wave's total.
-__device__ __forceinline__ uint32_t wave_scan7(uint32_t c, uint32_t& total) {
-    const uint64_t lt = (1ull << (threadIdx.x & 63u)) - 1ull;
-    const uint64_t b0 = __ballot(c & 1u), b1 = __ballot(c & 2u), b2 = __ballot(c & 4u);
+// Exclusive prefix sum of a count in [0, 7] over the lanes of the group `gmask`, and the group's total.
+__device__ __forceinline__ uint32_t group_scan7(uint32_t c, uint64_t gmask, uint32_t& total) {
+    const uint64_t lt = ((1ull << (threadIdx.x & 63u)) - 1ull) & gmask;
+    const uint64_t b0 = __ballot(c & 1u) & gmask, b1 = __ballot(c & 2u) & gmask, b2 = __ballot(c & 4u) & gmask;
     total = (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
     return (uint32_t)(__popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt));
 }
 
-// Wave-cooperative traversal of ONE ray by the 64 lanes of a one-wave work-group (every lane passes
-// the same ray). For rays that visit hundreds of nodes (grazing rays along a dense mesh), whose
-// one-lane traversal is a long chain of dependent node fetches:
-//  1. the wave expands a frontier breadth-first, a node per lane per round, until it holds at least 64
-//     entries or only leaves (at most 256 entries: fewer than 64 nodes expand into at most 4 each);
+// Cooperative traversal of one ray per group of G lanes (G = 16, 32 or 64; a one-wave work-group holds
+// 64 / G groups, every lane of a group passes its group's ray). For rays that visit hundreds of nodes
+// (grazing rays along a dense mesh), whose one-lane traversal is a long chain of dependent node fetches:
+//  1. the group expands a frontier breadth-first, a node per lane per round, until it holds at least G
+//     entries or only leaves (at most 4 (G - 1): fewer than G nodes expand into at most 4 each);
 //  2. each lane traverses frontier entries depth-first with the one-lane step (its own stack), taking
-//     the next untaken entry when it finishes one; after every step the lanes share the wave's least
+//     the next untaken entry when it finishes one; after every step the group's lanes share their least
 //     hit distance to cull with.
 // `hit` holds the ray's best hit so far on entry (prim 0xFFFFFFFF: none) and the least (t, primitive
-// index) pair over all primitive hits with t in [tmin, tbest] on exit, the result of trace(). Any hit
-// (ANY): returns as soon as a lane finds one. f_node / f_t: 2 x 256 LDS entries. Returns true on a hit.
-constexpr uint32_t kCoopFrontier = 256;
-template <bool ANY, bool MIXED, int LDS_DEPTH>
+// index) pair over all primitive hits with t in [tmin, tbest] on exit, the result of trace(). An any-hit
+// ray (`any`, group-uniform) stops as soon as one of its lanes records a hit. `has`: the group has a ray.
+// Every lane of the wave must call it together. f_node / f_t: 512 LDS entries each (8 G per group).
+// Returns true on a hit.
+constexpr uint32_t kCoopEntries = 512;  // LDS frontier entries of a one-wave work-group (8 per lane)
+template <bool MIXED, int LDS_DEPTH, int G>
 __device__ __forceinline__ bool coop_trace(const DevScene& sc, const TRay& r_in, THit& hit, TStack<LDS_DEPTH, 64>& st,
-                                           uint32_t* f_node, float* f_t) {
+                                           uint32_t* f_node, float* f_t, bool has, bool any) {
+    static_assert(G == 16 || G == 32 || G == 64, "group of 16, 32 or 64 lanes");
+    constexpr uint32_t FC = 4u * G;  // frontier capacity per buffer
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gbase = lane & ~(uint32_t)(G - 1), gl = lane - gbase;
+    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+    const uint32_t fb = gbase * 8u;  // the group's two frontier buffers
     const float kInfF = __int_as_float(0x7f800000);
     TState s;
     tstate_init(s, r_in);
     const TRay& r = s.r;
-    // wave-shared bound: the best hit distance found so far by any lane (or the entry best)
+    // group-shared bound: the best hit distance found so far by any lane of the group (or the entry best)
     float bound = hit.prim != 0xFFFFFFFFu ? hit.t : s.tbest;
-    uint32_t nf = 1, cur = 0;  // frontier size, current buffer (entries at f_*[cur * 256 + i])
-    if (lane == 0) {
-        f_node[0] = 0;  // the root
-        f_t[0] = r.tmin;
+    uint32_t nf = has ? 1u : 0u, cur = 0;  // frontier size, current buffer (group-uniform)
+    if (has && gl == 0) {
+        f_node[fb] = 0;  // the root
+        f_t[fb] = r.tmin;
     }
     __syncthreads();
     // ---- 1. breadth-first frontier ----
-    while (nf < 64) {
-        const bool have = lane < nf;
-        const uint32_t node = have ? f_node[cur * kCoopFrontier + lane] : 0u;
-        const float tn = have ? f_t[cur * kCoopFrontier + lane] : kInfF;
-        const bool live = have && tn <= bound;
-        const bool inner = live && !child_is_leaf(node);
-        if (__ballot(inner) == 0) break;  // only leaves left: phase 2 takes them as they are
+    bool expanding = has && nf < (uint32_t)G;
+    while (__ballot(expanding) != 0) {
+        bool live = false, inner = false;
+        uint32_t node = 0;
+        float tn = kInfF;
+        if (expanding && gl < nf) {
+            node = f_node[fb + cur * FC + gl];
+            tn = f_t[fb + cur * FC + gl];
+            live = tn <= bound;
+            inner = live && !child_is_leaf(node);
+        }
+        const bool grow = expanding && (__ballot(inner) & gmask) != 0;  // group-uniform
         // children the ray enters before the bound (entry distances rounded down like the one-lane step)
         bool in[4] = {false, false, false, false};
         float ent[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         uint4 ch = make_uint4(0, 0, 0, 0);
-        uint32_t cnt = live ? 1u : 0u;
-        if (inner) {
+        uint32_t cnt = grow && live ? 1u : 0u;
+        if (grow && inner) {
             const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
             const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
             ch = reinterpret_cast<const uint4*>(np)[6];
@@ -663,41 +685,48 @@ __device__ __forceinline__ bool coop_trace(const DevScene& sc, const TRay& r_in,
             }
         }
         uint32_t total;
-        const uint32_t off = wave_scan7(cnt, total);
+        const uint32_t off = group_scan7(cnt, gmask, total);
         const uint32_t nxt = cur ^ 1u;
-        if (inner) {
-            uint32_t k = off;
+        if (grow) {
+            if (inner) {
+                uint32_t k = off;
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                if (in[c]) {
-                    f_node[nxt * kCoopFrontier + k] = pick4(ch, (uint32_t)c);
-                    f_t[nxt * kCoopFrontier + k] = ent[c];
-                    k++;
+                for (int c = 0; c < 4; c++) {
+                    if (in[c]) {
+                        f_node[fb + nxt * FC + k] = pick4(ch, (uint32_t)c);
+                        f_t[fb + nxt * FC + k] = ent[c];
+                        k++;
+                    }
                 }
+            } else if (live) {
+                f_node[fb + nxt * FC + off] = node;
+                f_t[fb + nxt * FC + off] = tn;
             }
-        } else if (live) {
-            f_node[nxt * kCoopFrontier + off] = node;
-            f_t[nxt * kCoopFrontier + off] = tn;
         }
         __syncthreads();
-        cur = nxt;
-        nf = total;
+        if (grow) {
+            cur = nxt;
+            nf = total;
+        }
+        // only leaves (or culled entries) left, or enough entries: phase 2 takes the frontier as it is
+        expanding = grow && nf < (uint32_t)G;
     }
     // ---- 2. depth-first traversal of the frontier entries, lanes taking entries in turn ----
-    bool active = false, owns = false;
-    uint32_t taken = 0;  // wave-uniform: entries handed out so far
+    bool active = false, owns = false, any_hit = false;
+    bool gdone = !has;   // group-uniform
+    uint32_t taken = 0;  // group-uniform: entries handed out so far
     uint32_t obest = 0xFFFFFFFFu;
     float ot = kInfF, ou = 0.0f, ov = 0.0f;
-    bool any_hit = false;
+    const uint64_t lt = ((1ull << lane) - 1ull) & gmask;
     for (;;) {
-        const uint64_t idle = __ballot(!active);
-        if (taken < nf && idle != 0) {
-            const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        const uint64_t idle = __ballot(!active) & gmask;
+        if (!gdone && taken < nf) {
+            const uint32_t rank = (uint32_t)__popcll(idle & lt);
             if (!active && taken + rank < nf) {
                 const uint32_t e = taken + rank;
-                const float tn = f_t[cur * kCoopFrontier + e];
+                const float tn = f_t[fb + cur * FC + e];
                 if (tn <= bound) {
-                    s.cur = f_node[cur * kCoopFrontier + e];
+                    s.cur = f_node[fb + cur * FC + e];
                     s.sp = 0;
                     s.tbest = bound;
                     s.best = 0xFFFFFFFFu;
@@ -706,17 +735,17 @@ __device__ __forceinline__ bool coop_trace(const DevScene& sc, const TRay& r_in,
             }
             taken = min(nf, taken + (uint32_t)__popcll(idle));
         }
-        if (__ballot(active) == 0) {
-            if (taken >= nf) break;
-            continue;
-        }
-        if (active) {
+        const uint64_t gact = __ballot(active) & gmask;
+        if (gact == 0 && taken >= nf) gdone = true;
+        if (any && (__ballot(any_hit) & gmask) != 0) gdone = true;
+        if (__ballot(!gdone) == 0) break;
+        if (!gdone && active) {
             // adopt a closer bound found by another lane (the lane's own candidate no longer wins)
             if (bound < s.tbest) {
                 s.tbest = bound;
                 s.best = 0xFFFFFFFFu;
             }
-            const bool done = tstate_step<ANY, MIXED, false, LDS_DEPTH, 64, 0>(sc, s, st, nullptr, true);
+            const bool done = tstate_step<false, MIXED, false, LDS_DEPTH, 64, 0>(sc, s, st, nullptr, true);
             if (s.best != 0xFFFFFFFFu) {  // the lane's least (t, index) pair so far
                 if (!owns || s.tbest < ot || (s.tbest == ot && s.best < obest)) {
                     ot = s.tbest;
@@ -725,32 +754,30 @@ __device__ __forceinline__ bool coop_trace(const DevScene& sc, const TRay& r_in,
                     ov = s.bv;
                 }
                 owns = true;
-                if (ANY) any_hit = true;
+                any_hit = true;
             }
             if (done) active = false;
         }
-        if (ANY) {
-            if (__ballot(any_hit) != 0) break;
-        }
-        bound = fminf(bound, wave_min_nonneg(owns ? ot : kInfF));
+        bound = fminf(bound, group_min_nonneg<G>(owns ? ot : kInfF));
     }
-    if (ANY) {
-        const bool found = __ballot(any_hit) != 0;
-        if (found) hit.prim = 0u;
+    // the least (t, index) pair over the group's lanes and the entry best
+    const float wt = group_min_nonneg<G>(owns ? ot : kInfF);
+    const uint32_t wp = group_min_u32<G>(owns && ot == wt ? obest : 0xFFFFFFFFu);
+    const uint64_t holder = __ballot(owns && ot == wt && obest == wp) & gmask;
+    const int hl = holder ? __ffsll((unsigned long long)holder) - 1 : (int)lane;
+    const float hu = __shfl(ou, hl, 64), hv = __shfl(ov, hl, 64);
+    if (any) {
+        const bool found = wp != 0xFFFFFFFFu || hit.prim != 0xFFFFFFFFu;
+        if (found) hit.prim = wp != 0xFFFFFFFFu ? wp : hit.prim;
         return found;
     }
-    // the least (t, index) pair over the lanes and the entry best
-    const float wt = wave_min_nonneg(owns ? ot : kInfF);
-    const uint32_t wp = wave_min_u32(owns && ot == wt ? obest : 0xFFFFFFFFu);
     if (wp == 0xFFFFFFFFu) return hit.prim != 0xFFFFFFFFu;  // nothing closer than the entry best
     const bool better = hit.prim == 0xFFFFFFFFu || wt < hit.t || (wt == hit.t && wp < hit.prim);
     if (!better) return true;
-    const uint64_t holder = __ballot(owns && ot == wt && obest == wp);
-    const int hl = __ffsll((unsigned long long)holder) - 1;
     hit.t = wt;
     hit.prim = wp;
-    hit.u = __shfl(ou, hl, 64);
-    hit.v = __shfl(ov, hl, 64);
+    hit.u = hu;
+    hit.v = hv;
     return true;
 }
 
