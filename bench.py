@@ -322,11 +322,14 @@ def pt_section(a, log):
         entry = {"config": name, "samples_per_s": round(gpu_sps, 1), "render_s": round(wall, 3),
                  "device_ms": round(st.device_ms, 2), "kernel_ms": kernels}
         info = scene.info()  # the choices the per-scene autotuners made on the first large chunk
+
+        def traversal(g):  # 0-8: BVH with a leaf gate of g/8 of a wave; 9: flat (every primitive tested)
+            return "flat" if g == 9 else ("bvh" if g >= 0 else "untuned")
+
+        gate = int(info.pt_trace_leaf_gate if integ == PT else info.bdpt_trace_leaf_gate)
+        entry["autotune"] = {"traversal": traversal(gate), "leaf_gate_eighths": gate if 0 <= gate <= 8 else None}
         if integ == PT:
-            entry["autotune"] = {"trace_waves_per_simd": int(info.pt_trace_waves),
-                                 "leaf_gate_eighths": int(info.pt_trace_leaf_gate)}
-        else:
-            entry["autotune"] = {"leaf_gate_eighths": int(info.bdpt_trace_leaf_gate)}
+            entry["autotune"]["trace_waves_per_simd"] = int(info.pt_trace_waves)
         if integ == PT:
             pr.reset()
             pr.render_samples(spp)
@@ -334,7 +337,10 @@ def pt_section(a, log):
             rays = c["extension_rays"] + c["shadow_rays"]
             boxes = c["extension_box_tests"] + c["shadow_box_tests"]
             prims = c["extension_prim_tests"] + c["shadow_prim_tests"]
-            tb = 48.0 * rays + 32.0 * boxes + 36.0 * prims
+            flat = traversal(gate) == "flat"
+            # a flat traversal reads each primitive record once per 64-ray wave batch (the same address in
+            # every lane), a BVH traversal per ray
+            tb = 48.0 * rays + 32.0 * boxes + (36.0 / 64.0 if flat else 36.0) * prims
             tr_s = kt["trace_ms"] * 1e-3
             entry["trace"] = {"rays": int(rays), "extension_rays": int(c["extension_rays"]),
                               "shadow_rays": int(c["shadow_rays"]), "launches": int(c["trace_launches"]),
@@ -344,8 +350,8 @@ def pt_section(a, log):
             entry["roofline"] = {"kernel": "k_trace", "bound": "hbm", "achieved": round(tb / tr_s / 1e9, 1),
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(tb / tr_s / 1e9 / HBM_PEAK_GBS, 4),
                                  "achieved_basis": "algorithmic bytes of all trace launches of the render (48 B per "
-                                                   "ray + 32 B per box test + 36 B per primitive test) / their summed "
-                                                   "device time"}
+                                                   "ray + 32 B per box test + 36 B per primitive test, per 64-ray "
+                                                   "wave batch in a flat traversal) / their summed device time"}
         # first-pass latency: one sample per pixel on a fresh film (reference pass 1, progressive.go:104-107),
         # the median over five fresh films (the film is created before the clock starts)
         fps = []

@@ -150,6 +150,7 @@ struct DevScene {
     int32_t textured;  // 1 if any material reads a texture: hits carry UVs
     float world_center[3];
     float world_radius;
+    int32_t num_prims;  // primitives in leaf order (the flat traversal tests them all)
 };
 
 }  // namespace prt

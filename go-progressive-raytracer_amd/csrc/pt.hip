@@ -66,7 +66,8 @@ struct PassParams {
     int32_t sample_base;    // first sample index (uniform films)
     int32_t max_depth, rr_min;
     uint64_t seed;
-    int32_t trace_gate;     // leaf gate of the traversals inside the BDPT kernels (eighths of a wave, 0 = none)
+    int32_t trace_gate;     // leaf gate of the traversals inside the BDPT kernels (eighths of a wave, 0 = none;
+                            // kGateFlat: flat traversal)
     int32_t tune_stride;    // the subpath kernel generates every tune_stride-th slot only (autotuning runs), else 1
     // adaptive rounds: pixels come from an active list and each pixel's next sample index is its count
     const uint32_t* act;
@@ -195,8 +196,10 @@ constexpr int kProbeCounts = 8;
 // slower over whole renders).
 constexpr int kTraceVariants = 4;
 // Leaf gates (eighths of a wave's running lanes, see leaf_gate) the autotuners choose from; 0 first.
-constexpr int kGateChoices = 3;
-constexpr int kGateChoice[kGateChoices] = {0, 4, 8};
+// Scenes of at most kFlatMaxPrims primitives also time the flat traversal (kGateFlat).
+constexpr int kGateChoices = 4;
+constexpr int kGateChoice[kGateChoices] = {0, 4, 8, kGateFlat};
+inline int gate_choices(const DevScene& sc) { return sc.num_prims <= kFlatMaxPrims ? kGateChoices : kGateChoices - 1; }
 constexpr int kTraceKernelWaves[kTraceVariants] = {7, 8, 7, 7};
 constexpr int kTraceGroupCap[kTraceVariants] = {0, 0, 6, 5};  // 0: as resources allow
 
@@ -469,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, PassParams pp, Pa
 // (emission of bounce b, shadow ray of bounce b, emission of bounce b + 1, ...), so films are identical.
 template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, PassParams pp, PathBuffers pb, int b0, uint2* gstack,
-                                                 uint32_t gstride) {
+                                                 uint32_t gstride, int gate) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
@@ -497,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, PassParams pp, Pat
             if (shadow) {
                 const float4 sd = pb.sd[slot];
                 const TRay r{o.x, o.y, o.z, 0.001f, sd.x, sd.y, sd.z, sd.w};
-                if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr)) {
+                if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr, flat_gate(sc, gate) ? gate : 0)) {
                     const float4 c = pb.sc[slot];
                     const float4 l = pb.L[slot];
                     pb.L[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
@@ -506,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, PassParams pp, Pat
             if (!cont) break;
             const float4 d = pb.d[slot];
             const TRay r{o.x, o.y, o.z, 0.001f, d.x, d.y, d.z, __int_as_float(0x7f800000)};
-            trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr);
+            trace<false, MIXED, false, kLds, kBlock, kNodeCache>(sc, r, h, st, nullptr, flat_gate(sc, gate) ? gate : 0);
             pb.hit[slot] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
         }
     }
@@ -1667,13 +1670,13 @@ static bool tune_collect(TuneJob& j, float* t) {
 }
 
 // The leaf gate of a finished tuning job whose timed launches ran every gate twice, interleaved
-// (t[k] = gate kGateChoice[k % kGateChoices]): the fastest by the shorter of its two times; gate 0 unless
+// (t[k] = gate kGateChoice[k % nch]): the fastest by the shorter of its two times; gate 0 unless
 // another is more than 3 % faster.
-static int pick_gate(const float* t) {
-    float best = std::min(t[0], t[kGateChoices]);
+static int pick_gate(const float* t, int nch) {
+    float best = std::min(t[0], t[nch]);
     int bestg = 0;
-    for (int g = 1; g < kGateChoices; g++) {
-        const float tg = std::min(t[g], t[kGateChoices + g]) * 1.03f;
+    for (int g = 1; g < nch; g++) {
+        const float tg = std::min(t[g], t[nch + g]) * 1.03f;
         if (tg < best) {
             best = tg;
             bestg = kGateChoice[g];
@@ -1794,9 +1797,10 @@ int film_enqueue_chunk(prt_film* f) {
         prt_scene* ms = const_cast<prt_scene*>(s);
         {
             std::lock_guard<std::mutex> lk(ms->bdpt_tune.mu);
+            const int nch = gate_choices(sc);
             float t[2 * kGateChoices];
             if (tune_collect(ms->bdpt_tune, t)) {
-                const int bestg = pick_gate(t);
+                const int bestg = pick_gate(t, nch);
                 int expect = -1;
                 ms->bdpt_trace_gate.compare_exchange_strong(expect, bestg);
                 ms->info.bdpt_trace_leaf_gate = ms->bdpt_trace_gate.load();
@@ -1805,11 +1809,11 @@ int film_enqueue_chunk(prt_film* f) {
                 !f->probe_flags && !f->adaptive && f->enq > 0) {
                 PassParams q = pp;
                 q.tune_stride = (int32_t)std::max<int64_t>(1, n >> 18);
-                hipError_t e = tune_begin(ms->bdpt_tune, 2 * kGateChoices, st);
+                hipError_t e = tune_begin(ms->bdpt_tune, 2 * nch, st);
                 if (e == hipSuccess) e = launch_paths(q);
                 if (e == hipSuccess) e = hipEventRecord(ms->bdpt_tune.ev[1], st);
-                for (int k = 0; k < 2 * kGateChoices && e == hipSuccess; k++) {
-                    q.trace_gate = kGateChoice[k % kGateChoices];
+                for (int k = 0; k < 2 * nch && e == hipSuccess; k++) {
+                    q.trace_gate = kGateChoice[k % nch];
                     e = launch_paths(q);
                     if (e == hipSuccess) e = hipEventRecord(ms->bdpt_tune.ev[2 + k], st);
                 }
@@ -1894,7 +1898,7 @@ int film_enqueue_chunk(prt_film* f) {
         std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
         float t[2 * kGateChoices];
         if (tune_collect(ms->pt_tune, t)) {
-            const int bestg = pick_gate(t);
+            const int bestg = pick_gate(t, gate_choices(sc));
             int expect = -1;
             ms->pt_trace_gate.compare_exchange_strong(expect, bestg);
             ms->info.pt_trace_leaf_gate = ms->pt_trace_gate.load();
@@ -1906,11 +1910,12 @@ int film_enqueue_chunk(prt_film* f) {
         std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
         if (ms->pt_trace_gate.load() >= 0 || ms->pt_tune.pending) return hipSuccess;
         const uint32_t tstride = (uint32_t)std::max<int64_t>(1, n >> 20);
-        hipError_t e = tune_begin(ms->pt_tune, 2 * kGateChoices, st);
+        const int nch = gate_choices(sc);
+        hipError_t e = tune_begin(ms->pt_tune, 2 * nch, st);
         if (e == hipSuccess) e = launch_trace(tv, 0, 1, 1, -1, tstride);
         if (e == hipSuccess) e = hipEventRecord(ms->pt_tune.ev[1], st);
-        for (int k = 0; k < 2 * kGateChoices && e == hipSuccess; k++) {
-            e = launch_trace(tv, kGateChoice[k % kGateChoices], 1, 1, -1, tstride);
+        for (int k = 0; k < 2 * nch && e == hipSuccess; k++) {
+            e = launch_trace(tv, kGateChoice[k % nch], 1, 1, -1, tstride);
             if (e == hipSuccess) e = hipEventRecord(ms->pt_tune.ev[2 + k], st);
         }
         if (e == hipSuccess) ms->pt_tune.pending = true;
@@ -1939,10 +1944,10 @@ int film_enqueue_chunk(prt_film* f) {
         PRT_HIP(trace(b0, 1, b0 > 0 ? ((b0 - 1) & 1) : -1));
         if (sc.mixed)
             hipLaunchKernelGGL(k_tail<true>, dim3(f->tail_blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b0, f->gstack,
-                               f->gslots);
+                               f->gslots, tgate);
         else
             hipLaunchKernelGGL(k_tail<false>, dim3(f->tail_blocks), dim3(kBlock), 0, st, sc, pp, f->pb, b0, f->gstack,
-                               f->gslots);
+                               f->gslots, tgate);
     } else if (pp.max_depth > 0) {
         PRT_HIP(trace(pp.max_depth, 0, (pp.max_depth - 1) & 1));
     }

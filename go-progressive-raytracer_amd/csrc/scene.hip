@@ -1179,6 +1179,7 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     s->dev.mixed = (nsph + nq + ndisc + naxial) > 0 ? 1 : 0;
     s->dev.max_stack = bvh.max_stack;
     s->dev.num_nodes = (int32_t)bvh.nodes.size();
+    s->dev.num_prims = (int32_t)nprim;
 
     // materials (float32 copies)
     std::vector<DevMaterial> mats(std::max(nm + nspot, 1));
@@ -1390,9 +1391,9 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         int v = std::atoi(fp);
         if (v >= 0 && v < 4) s->pt_trace_variant = v;
     }
-    if (const char* fg = std::getenv("PRT_TRACE_LEAF_GATE")) {  // pins the render kernels' leaf gate (0-8)
+    if (const char* fg = std::getenv("PRT_TRACE_LEAF_GATE")) {  // pins the render kernels' leaf gate (0-8, 9 flat)
         int g = std::atoi(fg);
-        if (g >= 0 && g <= 8) {
+        if ((g >= 0 && g <= 8) || (g == kGateFlat && nprim <= kFlatMaxPrims)) {
             s->pt_trace_gate = s->bdpt_trace_gate = g;
             s->info.pt_trace_leaf_gate = s->info.bdpt_trace_leaf_gate = g;
         }

@@ -531,6 +531,56 @@ __device__ __forceinline__ void load_node_cache(const DevScene& sc, float4* cach
     __syncthreads();
 }
 
+// Flat traversal for scenes of at most kFlatMaxPrims primitives: every primitive is tested in a loop that
+// is uniform across the wave (no node steps, no stack, no divergence), keeping the least (t, primitive
+// index) pair as the BVH traversal does, so hits are identical. The render kernels' traversals use it
+// when their leaf-gate argument is kGateFlat (a per-scene autotuner choice).
+constexpr int kGateFlat = 9;
+constexpr int32_t kFlatMaxPrims = 64;
+template <bool ANY, bool MIXED, bool COUNT>
+__device__ __forceinline__ void trace_flat(const DevScene& sc, TState& s, TCounts* cnt) {
+    const TRay& r = s.r;
+    for (uint32_t k = 0; k < (uint32_t)sc.num_prims; k++) {
+        const float4* pp = sc.prims + 3 * (size_t)k;
+        const float4 a = pp[0];
+        float t, u = 0.0f, v = 0.0f;
+        bool h;
+        if (COUNT) cnt->prims++;
+        if (!MIXED) {
+            const float4 b = pp[1], c = pp[2];
+            h = tri_test(r, s.w, a, b, c, s.tbest, t, u, v);
+        } else {
+            const uint32_t tag = __float_as_uint(a.w);
+            const uint32_t kind = prim_kind(tag);
+            const float4 b = pp[1];
+            if (kind == kPrimTriangle) {
+                h = tri_test(r, s.w, a, b, pp[2], s.tbest, t, u, v);
+            } else if (kind == kPrimSphere) {
+                h = sphere_test(r, a, b, s.tbest, t);
+            } else if (kind == kPrimQuad) {
+                h = quad_test(r, a, b, pp[2], sc.quads[prim_side(tag)], s.tbest, t, u, v);
+            } else if (kind == kPrimDisc) {
+                h = disc_test(r, a, b, pp[2], s.tbest, t);
+            } else if (kind == kPrimCylinder) {
+                h = cylinder_test(r, a, b, pp[2], s.tbest, t, u);
+            } else {
+                h = cone_test(r, a, b, pp[2], s.tbest, t, u);
+            }
+        }
+        if (h && (ANY || t < s.tbest || k < s.best)) {
+            s.tbest = t;
+            s.best = k;
+            s.bu = u;
+            s.bv = v;
+            if (ANY) break;
+        }
+    }
+}
+
+__device__ __forceinline__ bool flat_gate(const DevScene& sc, int gate) {
+    return gate == kGateFlat && sc.num_prims <= kFlatMaxPrims;
+}
+
 // Closest hit (ANY = false) or any hit (ANY = true) of one ray. Returns true on a hit.
 // Always inlined: its callers have one call site each, and a call would pass the ray, hit and stack
 // through scratch memory (AMDGPU's inliner gives up on large kernels by basic-block count).
@@ -540,9 +590,13 @@ __device__ __forceinline__ bool trace(const DevScene& sc, const TRay& r_in, THit
                                       TCounts* cnt, int gate = 0) {
     TState s;
     tstate_init(s, r_in);
-    for (;;) {
-        const bool ok = gate > 0 ? leaf_gate(true, s.cur, gate) : true;
-        if (tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK, NCACHE>(sc, s, st, cnt, ok)) break;
+    if (flat_gate(sc, gate)) {
+        trace_flat<ANY, MIXED, COUNT>(sc, s, cnt);
+    } else {
+        for (;;) {
+            const bool ok = gate > 0 ? leaf_gate(true, s.cur, gate) : true;
+            if (tstate_step<ANY, MIXED, COUNT, LDS_DEPTH, BLOCK, NCACHE>(sc, s, st, cnt, ok)) break;
+        }
     }
     tstate_result(s, hit);
     return s.best != 0xFFFFFFFFu;
@@ -562,6 +616,11 @@ __device__ __forceinline__ int trace_budget(const DevScene& sc, const TRay& r_in
     tstate_init(s, r_in);
     uint32_t steps = 0;
     bool parked = false;
+    if (flat_gate(sc, gate)) {  // a flat traversal has no long rays to park
+        trace_flat<ANY, MIXED, COUNT>(sc, s, cnt);
+        tstate_result(s, hit);
+        return s.best != 0xFFFFFFFFu ? kTraceHit : kTraceMiss;
+    }
     for (;;) {
         const bool ok = gate > 0 ? leaf_gate(true, s.cur, gate) : true;
         const bool progress = ok || !child_is_leaf(s.cur);  // a lane held at a leaf by the gate does not count

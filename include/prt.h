@@ -242,13 +242,14 @@ typedef struct prt_scene_info {
     int64_t reference_primitives; /* Scene.GetPrimitiveCount (scene.go:81-99): mesh triangles + 1 per
                                      other shape (a box counts once, a disc spot light's disc too) */
     int32_t pt_trace_leaf_gate;   /* leaf gate of the path tracer's traversals: a wave tests leaves once
-                                     this many eighths of its running lanes stand at one (0: always).
-                                     Picked by timing each gate on a sample of the second bounce's rays
-                                     inside a render chunk of >= 2^20 paths other than a film's first;
-                                     a later chunk reads the timings (-1: not tuned yet) */
+                                     this many eighths of its running lanes stand at one (0: always); 9:
+                                     flat traversal, every primitive tested without the BVH (scenes of at
+                                     most 64 primitives). Picked by timing each choice on a sample of the
+                                     second bounce's rays inside a render chunk of >= 2^20 paths other than
+                                     a film's first; a later chunk reads the timings (-1: not tuned yet) */
     int32_t bdpt_trace_leaf_gate; /* the same for the BDPT kernels, timed on the subpath kernel inside a
                                      BDPT chunk of >= 2^18 paths other than a film's first (-1: not tuned
-                                     yet). PRT_TRACE_LEAF_GATE=<0-8> pins both gates of new scenes */
+                                     yet). PRT_TRACE_LEAF_GATE=<0-8, 9> pins both choices of new scenes */
 } prt_scene_info;
 
 typedef struct prt_scene prt_scene;
