@@ -352,6 +352,9 @@ def pt_section(a, log):
                                  "achieved_basis": "algorithmic bytes of all trace launches of the render (48 B per "
                                                    "ray + 32 B per box test + 36 B per primitive test, per 64-ray "
                                                    "wave batch in a flat traversal) / their summed device time"}
+            if flat:
+                entry["roofline"]["limiter"] = ("flat traversal: every primitive tested by every lane in a wave-uniform "
+                                                "loop (records read once per wave), so the VALU, not HBM, bounds it")
         # first-pass latency: one sample per pixel on a fresh film (reference pass 1, progressive.go:104-107),
         # the median over five fresh films (the film is created before the clock starts)
         fps = []
