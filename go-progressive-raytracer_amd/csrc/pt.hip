@@ -1789,8 +1789,8 @@ int film_enqueue_chunk(prt_film* f) {
                                    f->lpath, f->lens, f->gstack, f->gslots);
             return hipGetLastError();
         };
-        // per-scene leaf gate of the BDPT traversals, on a chunk of at least 2^18 paths other than the film's
-        // first (which may be a latency-bound first pass): after a warm-up launch, the subpath kernel is
+        // per-scene leaf gate of the BDPT traversals, on a chunk of at least 2^18 paths that is not a one-sample
+        // first chunk (a latency-bound first pass): after a warm-up launch, the subpath kernel is
         // timed twice with each gate on about 2^18 of the chunk's samples (the regular launch writes their
         // subpaths again). Nothing waits for the timings: a later chunk of a film of the scene reads them
         // and keeps the fastest gate (gate 0 unless another is more than 3 % faster) for every BDPT kernel.
@@ -1806,7 +1806,7 @@ int film_enqueue_chunk(prt_film* f) {
                 ms->info.bdpt_trace_leaf_gate = ms->bdpt_trace_gate.load();
             }
             if (ms->bdpt_trace_gate.load() < 0 && !ms->bdpt_tune.pending && n >= ((int64_t)1 << 18) &&
-                !f->probe_flags && !f->adaptive && f->enq > 0) {
+                !f->probe_flags && !f->adaptive && (f->enq > 0 || pp.spp > 1)) {
                 PassParams q = pp;
                 q.tune_stride = (int32_t)std::max<int64_t>(1, n >> 18);
                 hipError_t e = tune_begin(ms->bdpt_tune, 2 * nch, st);
@@ -1885,8 +1885,8 @@ int film_enqueue_chunk(prt_film* f) {
         if (e != hipSuccess) return e;
         return hipGetLastError();
     };
-    // per-scene leaf gate of the trace kernel, on a chunk of at least 2^20 paths other than the film's first
-    // (which may be a latency-bound first pass): between the first bounce's shading and the second bounce's
+    // per-scene leaf gate of the trace kernel, on a chunk of at least 2^20 paths that is not a one-sample first
+    // chunk (a latency-bound first pass): between the first bounce's shading and the second bounce's
     // trace, after a warm-up launch, the second bounce's closest-hit trace of about 2^20 of its rays (it
     // only writes their hit records, which the regular trace writes again) is timed twice with each gate.
     // Nothing waits for the timings: a later chunk of a film of the scene reads them and keeps the
@@ -1905,7 +1905,7 @@ int film_enqueue_chunk(prt_film* f) {
         }
     }
     const bool tune_here = n >= ((int64_t)1 << 20) && !f->probe_flags && pp.max_depth > 1 && !f->adaptive &&
-                           f->enq > 0 && ms->pt_trace_gate.load() < 0;
+                           (f->enq > 0 || pp.spp > 1) && ms->pt_trace_gate.load() < 0;
     auto tune_bounce1 = [&]() -> hipError_t {
         std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
         if (ms->pt_trace_gate.load() >= 0 || ms->pt_tune.pending) return hipSuccess;

@@ -245,11 +245,13 @@ typedef struct prt_scene_info {
                                      this many eighths of its running lanes stand at one (0: always); 9:
                                      flat traversal, every primitive tested without the BVH (scenes of at
                                      most 64 primitives). Picked by timing each choice on a sample of the
-                                     second bounce's rays inside a render chunk of >= 2^20 paths other than
-                                     a film's first; a later chunk reads the timings (-1: not tuned yet) */
+                                     second bounce's rays inside a render chunk of >= 2^20 paths that is not
+                                     a film's one-sample first chunk; a later chunk reads the timings (-1:
+                                     not tuned yet) */
     int32_t bdpt_trace_leaf_gate; /* the same for the BDPT kernels, timed on the subpath kernel inside a
-                                     BDPT chunk of >= 2^18 paths other than a film's first (-1: not tuned
-                                     yet). PRT_TRACE_LEAF_GATE=<0-8, 9> pins both choices of new scenes */
+                                     BDPT chunk of >= 2^18 paths that is not a film's one-sample first chunk
+                                     (-1: not tuned yet). PRT_TRACE_LEAF_GATE=<0-8, 9> pins both choices of
+                                     new scenes */
 } prt_scene_info;
 
 typedef struct prt_scene prt_scene;

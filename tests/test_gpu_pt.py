@@ -233,8 +233,8 @@ def test_leaf_gate_leaves_films_unchanged(integrator, monkeypatch):
     picked by the autotuner) only reschedules lanes: films, splats included, are bit-identical for every
     gate, and so is the flat traversal (PRT_TRACE_LEAF_GATE=9: every primitive tested without the BVH,
     scenes of <= 64 primitives). The autotuner times launches inside a chunk of >= 2^20 (path tracing) or
-    2^18 (BDPT) paths other than the film's first, and a later chunk reads the timings: rendering in three
-    calls tunes the scene during the film, and a second film then renders with the tuned gate."""
+    2^18 (BDPT) paths that is not a one-sample first chunk, and a later chunk reads the timings: rendering
+    in three calls tunes the scene during the film, and a second film then renders with the tuned gate."""
     s, cfg = scenes.cornell_metre(width=128)
     w, h = s.image_size
     integ = P.INTEGRATOR_BDPT if integrator == "bdpt" else P.INTEGRATOR_PATH
