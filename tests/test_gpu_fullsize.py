@@ -151,3 +151,36 @@ def test_long_ray_parking_leaves_films_unchanged(case, mesh_sky, monkeypatch):
     for b, f in films.items():
         for x, y in zip(base, f):
             assert np.array_equal(x, y), f"film with park budget {b} differs from the film without parking"
+
+
+def test_long_rays_and_tail_in_adaptive_passes(mesh_sky, monkeypatch):
+    """Adaptive films (rounds over an active-pixel list, tile_renderer.go:50-101) run the same launches:
+    with long-ray parking at a 4-step budget and the tail kernel forced from bounce 1, the films after three
+    progressive passes equal the all-wavefront render without parking, sample counts included."""
+    desc, cfg, scene, _ = mesh_sky
+    w, h = desc.image_size
+
+    def render():
+        p = P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], 16, max_passes=4, seed=13,
+                              adaptive_min_samples=0.25, adaptive_threshold=0.05)
+        pr = P.ProgressiveRaytracer(scene, p)
+        for k in range(1, 4):
+            pr.render_pass(k)
+        out = pr.read()
+        pr.close()
+        return out
+
+    for var in ("PRT_PT_PARK_STEPS", "PRT_PT_PARK_STEPS_SMALL", "PRT_PT_TAIL_PATHS"):
+        monkeypatch.setenv(var, "0")
+    base = render()
+    monkeypatch.setenv("PRT_PT_PARK_STEPS", "4")
+    monkeypatch.setenv("PRT_PT_PARK_STEPS_SMALL", "4")
+    monkeypatch.setenv("PRT_PT_TAIL_PATHS", str(1 << 40))
+    render()  # a non-adaptive chunk has not recorded the profile: record it with a plain film
+    pr = P.ProgressiveRaytracer(scene, P.sampling_params(w, h, cfg["max_depth"], cfg["rr_min_bounces"], 2, seed=1))
+    pr.render_samples(2)
+    pr.close()
+    other = render()
+    assert base[3].min() >= 1 and base[3].max() <= 16
+    for x, y in zip(base, other):
+        assert np.array_equal(x, y)
