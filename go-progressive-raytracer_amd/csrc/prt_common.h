@@ -70,9 +70,13 @@ PRT_HD uint32_t prim_kind(uint32_t tag) { return tag >> kKindShift; }
 PRT_HD uint32_t prim_side(uint32_t tag) { return tag & kSideMask; }
 
 // Quad side data (float32 copies of float64 precomputations, NewQuad quad.go:98-121).
-struct QuadExt {
+// Quad side table: unit normal and plane offset, and the barycentric coordinates' planes: alpha = A . (p - corner),
+// beta = B . (p - corner) with A = v x W, B = W x u, W = n / (n . (u x v)).
+struct alignas(64) QuadExt {  // 64 B: a record never straddles two 128-B lines
     float nx, ny, nz, D;
-    float wx, wy, wz, pad;
+    float ax, ay, az, pad0;
+    float bx, by, bz, pad1;
+    float pad2[4];
 };
 
 // ---- materials and lights (float32 device copies) ------------------------------------

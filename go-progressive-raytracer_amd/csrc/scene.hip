@@ -1060,7 +1060,10 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
         D3 n = normalize(cross(q.u, q.v));
         double D = dot(n, q.corner);
         D3 w = n * (1.0 / dot(n, cross(q.u, q.v)));
-        qext[i] = QuadExt{(float)n.x, (float)n.y, (float)n.z, (float)D, (float)w.x, (float)w.y, (float)w.z, 0.f};
+        // alpha = W . (hv x v) = hv . (v x W), beta = W . (u x hv) = hv . (W x u) (quad.go:133-180)
+        D3 A = cross(q.v, w), B = cross(w, q.u);
+        qext[i] = QuadExt{(float)n.x, (float)n.y, (float)n.z, (float)D,   (float)A.x,  (float)A.y,
+                          (float)A.z, 0.f,        (float)B.x, (float)B.y, (float)B.z, 0.f, {0.f, 0.f, 0.f, 0.f}};
         rec[3 * gid + 0] = f4(q.corner.x, q.corner.y, q.corner.z, bits(prim_tag(kPrimQuad, (uint32_t)i)));
         rec[3 * gid + 1] = f4(q.u.x, q.u.y, q.u.z, bits((uint32_t)q.mat));
         rec[3 * gid + 2] = f4(q.v.x, q.v.y, q.v.z, bits((uint32_t)gid));

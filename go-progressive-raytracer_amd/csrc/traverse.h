@@ -144,20 +144,20 @@ __device__ __forceinline__ bool sphere_test(const TRay& r, float4 a, float4 b, f
     return true;
 }
 
-// Quad test (quad.go:133-180).
-__device__ __forceinline__ bool quad_test(const TRay& r, float4 a, float4 b, float4 c, const QuadExt& q,
-                                          float tbest, float& t_out, float& u_out, float& v_out) {
+// Quad test (quad.go:133-180): plane hit with |n.d| >= 1e-8 and t in [tmin, tbest], then the barycentric
+// coordinates alpha = W . (hv x v) and beta = W . (u x hv) of hv = p - corner, in [0, 1], as the dot
+// products hv . (v x W) and hv . (W x u) with both vectors precomputed per quad. t uses the hardware
+// reciprocal (deterministic, within 2 ulp of the quotient).
+__device__ __forceinline__ bool quad_test(const TRay& r, float4 a, const QuadExt& q, float tbest, float& t_out,
+                                          float& u_out, float& v_out) {
     PRT_EXACT_ROUNDING
     float den = r.dx * q.nx + r.dy * q.ny + r.dz * q.nz;
     if (fabsf(den) < 1e-8f) return false;
-    float t = (q.D - (r.ox * q.nx + r.oy * q.ny + r.oz * q.nz)) / den;
+    float t = (q.D - (r.ox * q.nx + r.oy * q.ny + r.oz * q.nz)) * __builtin_amdgcn_rcpf(den);
     if (t < r.tmin || t > tbest) return false;
     float px = r.ox + t * r.dx - a.x, py = r.oy + t * r.dy - a.y, pz = r.oz + t * r.dz - a.z;
-    // alpha = W . (hv x v), beta = W . (u x hv)
-    float hxv_x = py * c.z - pz * c.y, hxv_y = pz * c.x - px * c.z, hxv_z = px * c.y - py * c.x;
-    float uxh_x = b.y * pz - b.z * py, uxh_y = b.z * px - b.x * pz, uxh_z = b.x * py - b.y * px;
-    float alpha = q.wx * hxv_x + q.wy * hxv_y + q.wz * hxv_z;
-    float beta = q.wx * uxh_x + q.wy * uxh_y + q.wz * uxh_z;
+    float alpha = px * q.ax + py * q.ay + pz * q.az;
+    float beta = px * q.bx + py * q.by + pz * q.bz;
     if (alpha < 0.0f || alpha > 1.0f || beta < 0.0f || beta > 1.0f) return false;
     t_out = t;
     u_out = alpha;
@@ -491,8 +491,7 @@ __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStac
                 } else if (kind == kPrimSphere) {
                     h = sphere_test(r, a, b, s.tbest, t);
                 } else if (kind == kPrimQuad) {
-                    float4 c = pp[2];
-                    h = quad_test(r, a, b, c, sc.quads[prim_side(tag)], s.tbest, t, u, v);
+                    h = quad_test(r, a, sc.quads[prim_side(tag)], s.tbest, t, u, v);
                 } else if (kind == kPrimDisc) {
                     h = disc_test(r, a, b, pp[2], s.tbest, t);
                 } else if (kind == kPrimCylinder) {
@@ -558,7 +557,7 @@ __device__ __forceinline__ void trace_flat(const DevScene& sc, TState& s, TCount
             } else if (kind == kPrimSphere) {
                 h = sphere_test(r, a, b, s.tbest, t);
             } else if (kind == kPrimQuad) {
-                h = quad_test(r, a, b, pp[2], sc.quads[prim_side(tag)], s.tbest, t, u, v);
+                h = quad_test(r, a, sc.quads[prim_side(tag)], s.tbest, t, u, v);
             } else if (kind == kPrimDisc) {
                 h = disc_test(r, a, b, pp[2], s.tbest, t);
             } else if (kind == kPrimCylinder) {
