@@ -4,6 +4,7 @@
 
 #include <string>
 #include <atomic>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -52,6 +53,30 @@ struct PathProfile {
     int pending_upto = -1;       // bounces 0 .. pending_upto of the pending copy were recorded
 };
 
+// Device buffers and streams that a serving loop reuses across its films. Admitting a request takes
+// them from here and releasing it gives them back, so that once the pool holds buffers of the sizes the
+// load needs, neither calls hipMalloc or hipFree nor creates or destroys a stream: those calls wait on
+// the device and cost 0.4-6 ms per film, time in which the scheduler could not dispatch chunks.
+// Buffers are handed out uninitialised (a film initialises what it reads). Thread-safe.
+struct FilmPool {
+    std::mutex mu;
+    int device = 0;
+    std::multimap<size_t, void*> bufs;  // idle buffers by size
+    size_t cached = 0;                  // bytes of the idle buffers
+    std::vector<hipStream_t> streams;   // idle streams (non-blocking, default priority)
+    // The smallest idle buffer of bytes .. 1.25 * bytes, else a new one (after freeing idle buffers if the
+    // device is out of memory); *got is its size.
+    int take(size_t bytes, void** p, size_t* got);
+    void give(void* p, size_t bytes);
+    // Frees idle buffers, largest first, until at most `keep` bytes stay idle; returns the bytes freed.
+    size_t trim(size_t keep);
+    int take_stream(hipStream_t* st);
+    void give_stream(hipStream_t st);
+    // Frees every idle buffer and stream (their work must have finished).
+    void clear();
+    size_t idle_bytes();
+};
+
 }  // namespace prt
 
 struct prt_scene {
@@ -91,7 +116,9 @@ int persistent_blocks(const prt_scene* s, int block, int per_cu);
 DevCamera make_camera(const prt_camera& c, int& height, int force_height);
 
 // Film internals shared with the serving loop.
-int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t capacity_cap, prt_film** out);
+// With a pool, the film's buffers and stream come from it and go back to it when the film is destroyed.
+int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t capacity_cap, prt_film** out,
+                   FilmPool* pool = nullptr);
 size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int64_t capacity_cap);
 int film_enqueue(prt_film* f, int32_t spp);  // enqueue spp more samples per pixel, no wait
 int film_plan(prt_film* f, int32_t spp);     // split spp more samples into chunks (not enqueued)

@@ -1192,8 +1192,9 @@ struct prt_film {
     int64_t npix = 0;
     int64_t capacity = 0;  // path slots
     int32_t spp_done = 0;  // samples per pixel enqueued so far
-    std::vector<void*> bufs;
-    size_t bytes = 0;
+    std::vector<DevBuf> bufs;  // sizes as allocated (a pooled buffer may be larger than asked for)
+    size_t bytes = 0;          // bytes asked for
+    FilmPool* pool = nullptr;  // where the buffers and stream came from and go back to (nullptr: own)
     PathBuffers pb{};
     FilmBuffers fb{};
     uint2* gstack = nullptr;
@@ -1255,6 +1256,81 @@ struct prt_film {
 };
 
 namespace prt {
+
+int FilmPool::take(size_t bytes, void** p, size_t* got) {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = bufs.lower_bound(bytes);
+        if (it != bufs.end() && it->first <= bytes + bytes / 4) {
+            *p = it->second;
+            *got = it->first;
+            cached -= it->first;
+            bufs.erase(it);
+            return PRT_OK;
+        }
+    }
+    PRT_HIP(hipSetDevice(device));
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        trim(0);
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    *got = bytes;
+    return PRT_OK;
+}
+
+void FilmPool::give(void* p, size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu);
+    bufs.emplace(bytes, p);
+    cached += bytes;
+}
+
+size_t FilmPool::trim(size_t keep) {
+    std::lock_guard<std::mutex> lk(mu);
+    size_t freed = 0;
+    (void)hipSetDevice(device);
+    while (cached > keep && !bufs.empty()) {
+        auto it = std::prev(bufs.end());
+        (void)hipFree(it->second);
+        cached -= it->first;
+        freed += it->first;
+        bufs.erase(it);
+    }
+    return freed;
+}
+
+int FilmPool::take_stream(hipStream_t* st) {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!streams.empty()) {
+            *st = streams.back();
+            streams.pop_back();
+            return PRT_OK;
+        }
+    }
+    PRT_HIP(hipSetDevice(device));
+    PRT_HIP(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
+    return PRT_OK;
+}
+
+void FilmPool::give_stream(hipStream_t st) {
+    std::lock_guard<std::mutex> lk(mu);
+    streams.push_back(st);
+}
+
+size_t FilmPool::idle_bytes() {
+    std::lock_guard<std::mutex> lk(mu);
+    return cached;
+}
+
+void FilmPool::clear() {
+    trim(0);
+    std::lock_guard<std::mutex> lk(mu);
+    for (hipStream_t st : streams) (void)hipStreamDestroy(st);
+    streams.clear();
+}
 
 // GetColor: (samples + splats) / SampleCount, zero when no samples (stats.go:36-55)
 void film_mean(int64_t npix, const float4* sum, const uint32_t* cnt, const long long* spl, float* mean_rgb) {
@@ -1345,12 +1421,20 @@ void pinned_give(void* p, size_t) {
 
 int film_alloc(prt_film* f, size_t bytes, void** out) {
     void* p = nullptr;
-    PRT_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
-    f->bufs.push_back(p);
-    f->bytes += std::max<size_t>(bytes, 16);
+    bytes = std::max<size_t>(bytes, 16);
+    size_t got = bytes;
+    if (f->pool) {
+        int rc = f->pool->take(bytes, &p, &got);
+        if (rc != PRT_OK) return rc;
+    } else {
+        PRT_HIP(hipMalloc(&p, bytes));
+    }
+    f->bufs.push_back({p, got});
+    f->bytes += bytes;
     *out = p;
     return PRT_OK;
 }
+
 
 constexpr size_t kPathBytes = 7 * sizeof(float4) + 5 * sizeof(uint32_t);
 constexpr size_t kPixelBytes = 2 * sizeof(float4) + sizeof(float2) + sizeof(uint32_t);
@@ -2058,7 +2142,7 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     return (size_t)C * kPathBytes + (size_t)npix * kPixelBytes + g + b + 3 * sizeof(uint32_t) * (size_t)C;
 }
 
-int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t cap, prt_film** out) {
+int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t cap, prt_film** out, FilmPool* pool) {
     if (!scene || !p || !out) return fail(PRT_ERR_INVALID_ARGUMENT, "NULL argument");
     *out = nullptr;
     if (p->integrator != PRT_INTEGRATOR_PATH && p->integrator != PRT_INTEGRATOR_BDPT)
@@ -2076,6 +2160,7 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     PRT_HIP(hipSetDevice(scene->device));
     prt_film* f = new prt_film();
     f->scene = scene;
+    f->pool = pool;
     const_cast<prt_scene*>(scene)->users++;
     f->sc = scene->dev;
     if (w != scene->image_width || h != scene->image_height) {
@@ -2169,8 +2254,13 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
     // the device image of RGBA8 reads (allocated here, so a pass's first read does not allocate)
     if ((rc = film_alloc(f, sizeof(uchar4) * (size_t)f->npix, &ptr)) != PRT_OK) return bail(rc);
     f->rgba_dev = (uchar4*)ptr;
-    hipError_t e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    hipError_t e = hipSuccess;
+    if (pool) {
+        if ((rc = pool->take_stream(&f->stream)) != PRT_OK) return bail(rc);
+    } else {
+        e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreate"));
+    }
     e = hipEventCreate(&f->ev0);
     if (e == hipSuccess) e = hipEventCreate(&f->ev1);
     if (e != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
@@ -2287,14 +2377,24 @@ int prt_film_destroy(prt_film* f) {
         if (m.a) (void)hipEventDestroy(m.a);
         if (m.b) (void)hipEventDestroy(m.b);
     }
-    for (void* p : f->bufs) (void)hipFree(p);
+    for (const DevBuf& b : f->bufs) {
+        if (f->pool)
+            f->pool->give(b.p, b.bytes);
+        else
+            (void)hipFree(b.p);
+    }
     if (f->ev0) (void)hipEventDestroy(f->ev0);
     if (f->ev1) (void)hipEventDestroy(f->ev1);
     for (int i = 0; i < prt_film::kRing; i++) {
         if (f->rev0[i]) (void)hipEventDestroy(f->rev0[i]);
         if (f->rev1[i]) (void)hipEventDestroy(f->rev1[i]);
     }
-    if (f->stream) (void)hipStreamDestroy(f->stream);
+    if (f->stream) {
+        if (f->pool)
+            f->pool->give_stream(f->stream);
+        else
+            (void)hipStreamDestroy(f->stream);
+    }
     const_cast<prt_scene*>(f->scene)->users--;
     delete f;
     return PRT_OK;

@@ -47,6 +47,7 @@ struct Req {
         int32_t pass = 0;  // 0: empty
         int readers = 0;
         void* buf = nullptr;
+        size_t bytes = 0;  // of buf, as taken from the pool
         hipEvent_t done = nullptr;
     };
     static constexpr int kSnapSlots = 3;
@@ -99,6 +100,9 @@ struct prt_server {
     bool stop = false;
     size_t rr_next = 0;  // round-robin position of background dispatch
     hipStream_t read_stream = nullptr;  // snapshot reads (non-blocking, default priority)
+    // films' and snapshots' device buffers and streams, reused across requests; its idle buffers count
+    // against the HBM budget together with the active films
+    prt::FilmPool pool;
 
     double since(const Req* r) const {
         return std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
@@ -117,15 +121,17 @@ struct prt_server {
     }
     void free_snapshots(Req* r) {
         for (auto& sn : r->snap) {
-            if (sn.buf) (void)hipFree(sn.buf);
+            if (sn.done) (void)hipEventSynchronize(sn.done);
+            if (sn.buf) pool.give(sn.buf, sn.bytes);
             if (sn.done) (void)hipEventDestroy(sn.done);
             sn = Req::Snap{};
         }
     }
     int alloc_snapshots(Req* r) {
-        const size_t b = prt::film_snapshot_bytes(r->film);
+        const size_t b = std::max<size_t>(prt::film_snapshot_bytes(r->film), 16);
         for (auto& sn : r->snap) {
-            if (hipMalloc(&sn.buf, std::max<size_t>(b, 16)) != hipSuccess) return PRT_ERR_OUT_OF_MEMORY;
+            int rc = pool.take(b, &sn.buf, &sn.bytes);
+            if (rc != PRT_OK) return rc;
             if (hipEventCreateWithFlags(&sn.done, hipEventDisableTiming) != hipSuccess) return PRT_ERR_DEVICE;
         }
         return PRT_OK;
@@ -187,6 +193,23 @@ struct prt_server {
                 emit(r, PRT_EVENT_CANCELLED, 0, 0, 0, 0);
                 progressed = true;
             }
+            // ---- deferred releases (first: their buffers and budget go to the requests admitted next) ----
+            for (auto it = reqs.begin(); it != reqs.end();) {
+                Req* r = it->second;
+                bool finished = r->state == St::Done || r->state == St::Cancelled || r->state == St::Failed;
+                if (r->release && finished && r->readers == 0) {
+                    // a request that failed at dispatch is still listed as active until the next sweep
+                    active.erase(std::remove(active.begin(), active.end(), r), active.end());
+                    if (r->film) (void)prt::film_wait(r->film, nullptr);
+                    free_film(r);
+                    const_cast<prt_scene*>(r->scene)->users--;
+                    delete r;
+                    it = reqs.erase(it);
+                    progressed = true;
+                } else {
+                    ++it;
+                }
+            }
             // ---- admission (FIFO, HBM budget, slot count) ----
             while (!queue.empty() && (int)active.size() < max_active) {
                 Req* r = queue.front();
@@ -197,8 +220,12 @@ struct prt_server {
                     continue;
                 }
                 if (used + (int64_t)r->bytes > budget) break;  // wait for memory to free up
+                // idle pooled buffers give way to the new film when the budget would not hold both
+                if (used + (int64_t)r->bytes + (int64_t)pool.idle_bytes() > budget)
+                    pool.trim((size_t)(budget - used - (int64_t)r->bytes));
                 queue.pop_front();
-                int rc = prt::film_create_ex(r->scene, &r->params, serve_capacity(r->scene, &r->params), &r->film);
+                int rc = prt::film_create_ex(r->scene, &r->params, serve_capacity(r->scene, &r->params), &r->film,
+                                             &pool);
                 if (rc == PRT_OK) {
                     used += (int64_t)r->bytes;  // free_film returns it with the film
                     rc = prt::film_track_chunks(r->film);
@@ -314,21 +341,6 @@ struct prt_server {
                     }
                 }
             }
-            // ---- deferred releases ----
-            for (auto it = reqs.begin(); it != reqs.end();) {
-                Req* r = it->second;
-                bool finished = r->state == St::Done || r->state == St::Cancelled || r->state == St::Failed;
-                if (r->release && finished && r->readers == 0) {
-                    if (r->film) (void)prt::film_wait(r->film, nullptr);
-                    free_film(r);
-                    const_cast<prt_scene*>(r->scene)->users--;
-                    delete r;
-                    it = reqs.erase(it);
-                    progressed = true;
-                } else {
-                    ++it;
-                }
-            }
             if (!progressed) cv_work.wait_for(lk, std::chrono::microseconds(100));
         }
     }
@@ -345,6 +357,7 @@ int prt_server_create(int32_t device, int64_t budget, int32_t max_active, prt_se
         return fail(PRT_ERR_NO_DEVICE, "no such HIP device");
     prt_server* s = new prt_server();
     s->device = device;
+    s->pool.device = device;
     s->budget = budget;
     s->max_active = max_active;
     if (hipSetDevice(device) != hipSuccess ||
@@ -494,6 +507,7 @@ int prt_server_stats_get(prt_server* s, prt_server_stats* st) {
     st->completed = s->completed;
     st->budget_bytes = s->budget;
     st->used_bytes = s->used;
+    st->idle_bytes = (int64_t)s->pool.idle_bytes();
     return PRT_OK;
 }
 
@@ -512,6 +526,7 @@ int prt_server_destroy(prt_server* s) {
         const_cast<prt_scene*>(r->scene)->users--;
         delete r;
     }
+    s->pool.clear();
     if (s->read_stream) (void)hipStreamDestroy(s->read_stream);
     delete s;
     return PRT_OK;

@@ -195,7 +195,7 @@ class SsePassInfo(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("queued", C.c_int32), ("active", C.c_int32), ("completed", C.c_int64),
-                ("budget_bytes", C.c_int64), ("used_bytes", C.c_int64)]
+                ("budget_bytes", C.c_int64), ("used_bytes", C.c_int64), ("idle_bytes", C.c_int64)]
 
 _lib = None
 

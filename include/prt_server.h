@@ -45,12 +45,16 @@ typedef struct prt_server_stats {
     int32_t active;
     int64_t completed;
     int64_t budget_bytes;
-    int64_t used_bytes;
+    int64_t used_bytes;  /* film slots of admitted, unreleased requests */
+    int64_t idle_bytes;  /* device buffers kept from released films for reuse (within the budget) */
 } prt_server_stats;
 
 typedef struct prt_server prt_server;
 
-/* hbm_budget_bytes bounds the memory of all active film slots; max_active bounds concurrent requests. */
+/* hbm_budget_bytes bounds the memory of all active film slots together with the device buffers the
+ * server keeps from released films for reuse (films of later requests take them instead of allocating;
+ * a reused buffer may exceed what a film asked for by up to a quarter); max_active bounds concurrent
+ * requests. */
 int prt_server_create(int32_t device, int64_t hbm_budget_bytes, int32_t max_active, prt_server** out);
 /* Queues a progressive render of `scene` (params: integrator, width/height override, depth, RR, pass
  * schedule, seed). The scene must outlive the request: prt_scene_destroy fails while a request that

@@ -252,3 +252,35 @@ def test_concurrent_rgba8_reads_of_a_served_film(sc):
         assert np.array_equal(o, ref if k % 2 == 0 else ref[20:150, 10:200]), k
     srv.release(rid)
     srv.close()
+
+
+def test_reused_film_buffers_give_the_same_films(sc):
+    """Released films leave their device buffers and streams to the server, and films of later requests
+    take them instead of allocating: every film still matches the same request rendered on a fresh
+    server bit for bit, whatever the previous owner of its buffers rendered (other scenes, PT / BDPT,
+    other sizes)."""
+    import time
+
+    PT = P.INTEGRATOR_PATH
+    seq = [("mesh_sky", 96, 96, 6, 2, 31, BD), ("cornell", 96, 96, 12, 2, 32, PT), ("default", 96, 96, 8, 2, 33, BD),
+           ("cornell", 96, 96, 8, 2, 34, BD), ("mesh_sky", 96, 96, 8, 2, 35, PT), ("default", 80, 80, 8, 2, 36, PT)]
+    srv = P.Server(max_active=1)
+    for name, w, h, spp, passes, seed, integ in seq:
+        scene, cfg = sc[name]
+        rid = srv.submit(scene, _params(w, h, cfg, spp, passes, seed, integ))
+        _drain(srv, [rid])
+        m, c = srv.read(rid, w, h)
+        srv.release(rid)
+        t0 = time.time()
+        while srv.stats().used_bytes != 0:  # the release is carried out by the scheduler thread
+            assert time.time() - t0 < 10, "released film not returned"
+            time.sleep(0.002)
+        assert srv.stats().idle_bytes > 0
+        solo = P.Server(max_active=1)
+        sid = solo.submit(scene, _params(w, h, cfg, spp, passes, seed, integ))
+        _drain(solo, [sid])
+        ms, cs = solo.read(sid, w, h)
+        solo.close()
+        assert np.array_equal(c, cs) and np.all(c == spp)
+        assert np.array_equal(m, ms), f"{name} {w}x{h} integrator {integ}: reused buffers changed the film"
+    srv.close()
