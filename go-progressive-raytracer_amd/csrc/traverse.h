@@ -113,19 +113,21 @@ __device__ __forceinline__ bool tri_test(const TRay& r, const WRay& w, float4 a,
 // perpendicular offset l = f - (f.d / d.d) d (disc = A * (r^2 - l.l)) and the roots by the
 // cancellation-free pair q / A, C / q (Haines et al., "Precision Improvements for Ray/Sphere
 // Intersection", Ray Tracing Gems, 2019). Returns false if the ray's line misses the sphere;
-// t0 <= t1 otherwise.
+// t0 <= t1 otherwise. Quotients and the square root use the hardware reciprocal and square root
+// (deterministic, 1 ulp each) instead of the correctly rounded sequences.
 __device__ __forceinline__ bool sphere_roots(float fx, float fy, float fz, float dx, float dy, float dz, float rad,
                                              float& t0, float& t1) {
     PRT_EXACT_ROUNDING
     float A = dx * dx + dy * dy + dz * dz;
     float hb = fx * dx + fy * dy + fz * dz;
-    float k = hb / A;
+    const float inv_a = __builtin_amdgcn_rcpf(A);
+    float k = hb * inv_a;
     float lx = fx - k * dx, ly = fy - k * dy, lz = fz - k * dz;
     float disc = A * (rad * rad - (lx * lx + ly * ly + lz * lz));
     if (disc < 0.0f) return false;
     float C = fx * fx + fy * fy + fz * fz - rad * rad;
-    float q = -(hb + copysignf(sqrtf(disc), hb));
-    float a = q / A, b = q != 0.0f ? C / q : a;
+    float q = -(hb + copysignf(__builtin_amdgcn_sqrtf(disc), hb));
+    float a = q * inv_a, b = q != 0.0f ? C * __builtin_amdgcn_rcpf(q) : a;
     t0 = fminf(a, b);
     t1 = fmaxf(a, b);
     return true;
