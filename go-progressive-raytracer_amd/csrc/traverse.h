@@ -4,8 +4,9 @@
 // t in [tmin, tmax] (inclusive), the closest one is reported. Triangles are two-sided
 // (triangle.go:107-175) and use the watertight test of Woop, Benthin and Wald (JCGT 2013)
 // in float32 with a float64 fallback on exact edge hits; boxes use the robust slab test
-// with the far distance widened by 1 + 2*gamma(3) (Ize 2013) so that float32 boxes never
-// reject a ray that the exact box would accept.
+// with the far distance widened by 1 + 2*gamma(4) (Ize 2013, with one more rounding for the
+// hardware reciprocal of the direction) so that float32 boxes never reject a ray that the exact box
+// would accept.
 #pragma once
 #include "prt_common.h"
 
@@ -315,7 +316,10 @@ __device__ __forceinline__ bool cone_test(const TRay& r, float4 a, float4 b, flo
     return hit;
 }
 
-constexpr float kBoxWiden = 1.0f + 2.0f * (3.0f * 5.9604645e-08f) / (1.0f - 3.0f * 5.9604645e-08f);
+// Far-distance widening 1 + 2*gamma(4), gamma(n) = n*u / (1 - n*u) with u = 2^-24: a slab distance
+// (b - o) * (1/d) carries three roundings, the subtraction and the product (u each) and the hardware
+// reciprocal of the direction (1 ulp, at most 2u relative), so its relative error is below gamma(4).
+constexpr float kBoxWiden = 1.0f + 2.0f * (4.0f * 5.9604645e-08f) / (1.0f - 4.0f * 5.9604645e-08f);
 
 // Traversal stack: the first LDS_DEPTH entries per lane live in LDS (lane-major so each lane
 // owns one bank column), deeper entries spill to a global array indexed by the thread slot.
@@ -364,9 +368,9 @@ __device__ __forceinline__ void tstate_init(TState& s, const TRay& r_in) {
     s.r = r_in;
     if (!(s.r.tmin >= 0.0f)) s.r.tmin = 0.0f;
     // reciprocal direction; zero components become huge so that 0 * inf never occurs
-    s.idx = 1.0f / (fabsf(s.r.dx) > 1e-30f ? s.r.dx : copysignf(1e-30f, s.r.dx));
-    s.idy = 1.0f / (fabsf(s.r.dy) > 1e-30f ? s.r.dy : copysignf(1e-30f, s.r.dy));
-    s.idz = 1.0f / (fabsf(s.r.dz) > 1e-30f ? s.r.dz : copysignf(1e-30f, s.r.dz));
+    s.idx = __builtin_amdgcn_rcpf(fabsf(s.r.dx) > 1e-30f ? s.r.dx : copysignf(1e-30f, s.r.dx));
+    s.idy = __builtin_amdgcn_rcpf(fabsf(s.r.dy) > 1e-30f ? s.r.dy : copysignf(1e-30f, s.r.dy));
+    s.idz = __builtin_amdgcn_rcpf(fabsf(s.r.dz) > 1e-30f ? s.r.dz : copysignf(1e-30f, s.r.dz));
     s.w = make_wray(s.r.dx, s.r.dy, s.r.dz);
     s.tbest = s.r.tmax;
     s.best = 0xFFFFFFFFu;
