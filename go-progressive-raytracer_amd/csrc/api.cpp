@@ -1,7 +1,11 @@
 // api.cpp — error reporting and small host-only entry points of the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
 #include <string>
+#include <thread>
 
 #include "prt_internal.h"
 
@@ -9,7 +13,21 @@ namespace prt {
 
 static thread_local std::string g_last_error;
 
-void set_error(const std::string& msg) { g_last_error = msg; }
+// PRT_LOG_ERRORS=1 also prints every error to stderr with the thread that raised it: errors raised on the
+// serving loop's scheduler thread reach callers only as error codes in events.
+static const bool g_log_errors = [] {
+    const char* v = std::getenv("PRT_LOG_ERRORS");
+    return v && v[0] == '1';
+}();
+
+void set_error(const std::string& msg) {
+    g_last_error = msg;
+    if (g_log_errors) {
+        std::fprintf(stderr, "[prt error, thread %zx] %s\n", std::hash<std::thread::id>{}(std::this_thread::get_id()),
+                     msg.c_str());
+        std::fflush(stderr);
+    }
+}
 
 int fail(int code, const std::string& msg) {
     set_error(msg);

@@ -1745,10 +1745,16 @@ static bool tune_collect(TuneJob& j, float* t) {
     hipError_t q = hipEventQuery(j.ev[j.runs + 1]);
     if (q == hipErrorNotReady) return false;
     j.pending = false;
-    if (q != hipSuccess) return false;
+    if (q != hipSuccess) {
+        (void)hipGetLastError();  // handled here: not an error of the caller's next launch
+        return false;
+    }
     for (int k = 0; k < j.runs; k++) {
         t[k] = 0.0f;
-        if (hipEventElapsedTime(&t[k], j.ev[1 + k], j.ev[2 + k]) != hipSuccess) return false;
+        if (hipEventElapsedTime(&t[k], j.ev[1 + k], j.ev[2 + k]) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
     }
     return true;
 }
@@ -1771,12 +1777,54 @@ static int pick_gate(const float* t, int nch) {
 
 // Reads a completed profile copy into the scene's survival fractions (caller holds the profile lock).
 static void profile_collect(PathProfile& pf) {
-    if (!pf.pending || hipEventQuery(pf.ev) != hipSuccess) return;
+    if (!pf.pending) return;
+    const hipError_t q = hipEventQuery(pf.ev);
+    if (q == hipErrorNotReady) return;
     pf.pending = false;
+    if (q != hipSuccess) {
+        (void)hipGetLastError();  // a failed copy is dropped; the next chunk records another
+        return;
+    }
     const double n0 = (double)pf.host[0];
     if (n0 <= 0.0) return;
     if ((int)pf.frac.size() < pf.pending_upto + 1) pf.frac.resize(pf.pending_upto + 1, -1.0);
     for (int b = 0; b <= pf.pending_upto; b++) pf.frac[b] = (double)pf.host[b] / n0;
+}
+
+// A finished leaf-gate tuning job of the scene sets its gate (PT: under the job's lock; BDPT: the caller
+// holds it).
+static void collect_pt_tuning(prt_scene* ms) {
+    std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
+    float t[2 * kGateChoices];
+    if (tune_collect(ms->pt_tune, t)) {
+        const int bestg = pick_gate(t, gate_choices(ms->dev));
+        int expect = -1;
+        ms->pt_trace_gate.compare_exchange_strong(expect, bestg);
+        ms->info.pt_trace_leaf_gate = ms->pt_trace_gate.load();
+    }
+}
+static void collect_bdpt_tuning_locked(prt_scene* ms) {
+    float t[2 * kGateChoices];
+    if (tune_collect(ms->bdpt_tune, t)) {
+        const int bestg = pick_gate(t, gate_choices(ms->dev));
+        int expect = -1;
+        ms->bdpt_trace_gate.compare_exchange_strong(expect, bestg);
+        ms->info.bdpt_trace_leaf_gate = ms->bdpt_trace_gate.load();
+    }
+}
+
+// The scene's tuning jobs and survival-profile copy are recorded on the stream of the film that started
+// them, but belong to the scene, which outlives the film. Once a film's stream has drained they are
+// complete: collect them before the stream can be destroyed, since querying an event whose stream no
+// longer exists is an error.
+static void collect_scene_events(prt_scene* ms) {
+    collect_pt_tuning(ms);
+    {
+        std::lock_guard<std::mutex> lk(ms->bdpt_tune.mu);
+        collect_bdpt_tuning_locked(ms);
+    }
+    std::lock_guard<std::mutex> lk(ms->pt_profile.mu);
+    profile_collect(ms->pt_profile);
 }
 
 // First bounce b >= 1 at which a chunk of n paths is expected to trace at most `limit` paths (by the
@@ -1882,13 +1930,7 @@ int film_enqueue_chunk(prt_film* f) {
         {
             std::lock_guard<std::mutex> lk(ms->bdpt_tune.mu);
             const int nch = gate_choices(sc);
-            float t[2 * kGateChoices];
-            if (tune_collect(ms->bdpt_tune, t)) {
-                const int bestg = pick_gate(t, nch);
-                int expect = -1;
-                ms->bdpt_trace_gate.compare_exchange_strong(expect, bestg);
-                ms->info.bdpt_trace_leaf_gate = ms->bdpt_trace_gate.load();
-            }
+            collect_bdpt_tuning_locked(ms);
             if (ms->bdpt_trace_gate.load() < 0 && !ms->bdpt_tune.pending && n >= ((int64_t)1 << 18) &&
                 !f->probe_flags && !f->adaptive && (f->enq > 0 || pp.spp > 1)) {
                 PassParams q = pp;
@@ -1978,16 +2020,7 @@ int film_enqueue_chunk(prt_film* f) {
     prt_scene* ms = const_cast<prt_scene*>(s);
     const int tv = ms->pt_trace_variant;
     ms->info.pt_trace_waves = f->trace_grid[tv] / std::max(1, s->num_cus);  // work-groups of 4 waves per CU
-    {
-        std::lock_guard<std::mutex> lk(ms->pt_tune.mu);
-        float t[2 * kGateChoices];
-        if (tune_collect(ms->pt_tune, t)) {
-            const int bestg = pick_gate(t, gate_choices(sc));
-            int expect = -1;
-            ms->pt_trace_gate.compare_exchange_strong(expect, bestg);
-            ms->info.pt_trace_leaf_gate = ms->pt_trace_gate.load();
-        }
-    }
+    collect_pt_tuning(ms);
     const bool tune_here = n >= ((int64_t)1 << 20) && !f->probe_flags && pp.max_depth > 1 && !f->adaptive &&
                            (f->enq > 0 || pp.spp > 1) && ms->pt_trace_gate.load() < 0;
     auto tune_bounce1 = [&]() -> hipError_t {
@@ -2373,6 +2406,7 @@ int prt_film_destroy(prt_film* f) {
     if (!f) return PRT_OK;
     (void)hipSetDevice(f->scene->device);
     if (f->stream) (void)hipStreamSynchronize(f->stream);
+    collect_scene_events(const_cast<prt_scene*>(f->scene));
     for (auto& m : f->marks) {
         if (m.a) (void)hipEventDestroy(m.a);
         if (m.b) (void)hipEventDestroy(m.b);

@@ -581,10 +581,11 @@ const void* variant_fn(int v) {
 
 // Launches variant v; the prefetched-refill variants park long rays (park.budget > 0) and are followed
 // by k_intersect_long on `long_blocks` one-wave work-groups (at most gstride / 64).
+// Returns the launch error of the variant's kernel, then of k_intersect_long.
 template <bool ANY, bool MIXED>
-void launch_variant(int v, int blocks, hipStream_t st, const DevScene& sc, const prt_ray* rays, int64_t n,
-                    prt_hit* hits, uint8_t* occ, uint2* gs, uint32_t gstride, unsigned long long* work,
-                    ParkQueue park, int long_blocks) {
+hipError_t launch_variant(int v, int blocks, hipStream_t st, const DevScene& sc, const prt_ray* rays, int64_t n,
+                          prt_hit* hits, uint8_t* occ, uint2* gs, uint32_t gstride, unsigned long long* work,
+                          ParkQueue park, int long_blocks) {
     bool pf = false;
     switch (v) {
 #define X(id, kind, lds, refill, nc, blk)                                                                     \
@@ -601,9 +602,14 @@ void launch_variant(int v, int blocks, hipStream_t st, const DevScene& sc, const
         PRT_TRACE_VARIANTS(X)
 #undef X
     }
-    if (pf && park.budget > 0)
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (pf && park.budget > 0) {
         hipLaunchKernelGGL((k_intersect_long<ANY, MIXED>), dim3(long_blocks), dim3(64), 0, st, sc, rays, hits, occ, gs,
                            gstride, park);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 // Resident one-wave work-groups of k_intersect_long, at most `slots` / 64 (its stack slots).
@@ -611,8 +617,10 @@ int long_blocks(const prt_scene* s, uint32_t slots) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_intersect_long<false, true>, 64, 0) !=
             hipSuccess ||
-        per_cu <= 0)
+        per_cu <= 0) {
+        (void)hipGetLastError();  // the query's failure is handled here, not by the next launch check
         per_cu = 16;
+    }
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(1, s->num_cus) * std::min(per_cu, 32),
                                                        slots / 64));
 }
@@ -674,18 +682,27 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
     // park queue: count on the first 256-B line, then the ray indices
     ParkQueue park{scene->park, scene->park + 64, kParkCapacity, 0u};
     const int lb = long_blocks(scene, gstride);
+    // an error left by an earlier call on this thread (its caller saw it or chose to ignore it) is not
+    // this launch's: clear it, so that the checks below report only these launches
+    const hipError_t stale = hipGetLastError();
+    const char* failed = "intersect kernel launch";
     auto run = [&](int v, int64_t count) -> hipError_t {
         hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, st, work, park.count);
+        hipError_t e0 = hipGetLastError();
+        if (e0 != hipSuccess) {
+            failed = "counter clear launch";
+            return e0;
+        }
         int blocks = ms->variant_blocks[v];
         park.budget = park_budget(scene, count, (int64_t)blocks * kVariantBlock[v]);
         if (mode == 0) {
-            if (sc.mixed) launch_variant<false, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work, park, lb);
-            else launch_variant<false, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work, park, lb);
+            if (sc.mixed) e0 = launch_variant<false, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work, park, lb);
+            else e0 = launch_variant<false, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work, park, lb);
         } else {
-            if (sc.mixed) launch_variant<true, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work, park, lb);
-            else launch_variant<true, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work, park, lb);
+            if (sc.mixed) e0 = launch_variant<true, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work, park, lb);
+            else e0 = launch_variant<true, false>(v, blocks, st, sc, (const prt_ray*)d_rays, count, nullptr, (uint8_t*)d_out0, gs, gstride, work, park, lb);
         }
-        return hipGetLastError();
+        return e0;
     };
     if (mode == 2) {
         e = hipMemsetAsync(work, 0, sizeof(unsigned long long), st);
@@ -735,7 +752,11 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
         }
         int v = ms->forced_variant >= 0 ? ms->forced_variant : (tuned >= 0 ? tuned : kDefaultVariant);
         e = run(v, n);
-        if (e != hipSuccess) { cleanup(); return hip_fail(e, "intersect kernel launch"); }
+        if (e != hipSuccess) {
+            cleanup();
+            return fail(PRT_ERR_DEVICE, std::string(failed) + " (variant " + std::to_string(v) + "): " +
+                                            hipGetErrorString(e) + (stale != hipSuccess ? std::string(" (earlier error on this thread: ") + hipGetErrorString(stale) + ")" : std::string()));
+        }
     }
     e = hipGetLastError();
     if (e != hipSuccess) { cleanup(); return hip_fail(e, "intersect kernel launch"); }
@@ -1487,13 +1508,14 @@ int prt_intersect_closest_static(const prt_scene* scene, const prt_ray* rays, in
     uint2* gs = (uint2*)((char*)base + kStaticStackOffset);
     const uint32_t slots = (uint32_t)blocks * (uint32_t)block;
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, st, work, park.count);
     const int lb = long_blocks(scene, slots);
-    if (scene->dev.mixed)
-        launch_variant<false, true>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work, park, lb);
-    else
-        launch_variant<false, false>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work, park, lb);
+    (void)hipGetLastError();  // an earlier call's error on this thread is not this launch's
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, st, work, park.count);
     PRT_HIP(hipGetLastError());
+    if (scene->dev.mixed)
+        PRT_HIP((launch_variant<false, true>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work, park, lb)));
+    else
+        PRT_HIP((launch_variant<false, false>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work, park, lb)));
     return PRT_OK;
 }
 

@@ -407,46 +407,40 @@ __device__ __forceinline__ bool tstate_step(const DevScene& sc, TState& s, TStac
     const TRay& r = s.r;
     if (!child_is_leaf(s.cur)) {
         // ---- inner node: test the 4 child boxes (top levels from LDS when cached) ----
-        float4 lx, hx, ly, hy, lz, hz;
-        uint4 ch;
-        if (NCACHE > 0 && s.cur < (uint32_t)NCACHE) {
-            const float4* np = st.ncache + 8 * s.cur;
-            lx = np[0];
-            hx = np[1];
-            ly = np[2];
-            hy = np[3];
-            lz = np[4];
-            hz = np[5];
-            float4 c4 = np[6];
-            ch = make_uint4(__float_as_uint(c4.x), __float_as_uint(c4.y), __float_as_uint(c4.z), __float_as_uint(c4.w));
-        } else {
-            const float4* np = reinterpret_cast<const float4*>(sc.nodes + s.cur);
-            lx = np[0];
-            hx = np[1];
-            ly = np[2];
-            hy = np[3];
-            lz = np[4];
-            hz = np[5];
-            ch = reinterpret_cast<const uint4*>(np)[6];
-        }
+        // A node holds the lower and upper x planes of its four children in its first two 16-B slots,
+        // then y, then z. The near plane of an axis is the lower one when the ray's direction is
+        // positive along it: the slot is chosen by the load address (16 B further when the sign bit
+        // of the inverse direction is set) instead of selecting among loaded values.
+        const char* nb = (NCACHE > 0 && s.cur < (uint32_t)NCACHE)
+                             ? reinterpret_cast<const char*>(st.ncache + 8 * s.cur)
+                             : reinterpret_cast<const char*>(sc.nodes + s.cur);
+        const uint32_t sx = (__float_as_uint(s.idx) >> 27) & 16u;
+        const uint32_t sy = (__float_as_uint(s.idy) >> 27) & 16u;
+        const uint32_t sz = (__float_as_uint(s.idz) >> 27) & 16u;
+        const float4 nX = *reinterpret_cast<const float4*>(nb + sx);
+        const float4 fX = *reinterpret_cast<const float4*>(nb + (16u - sx));
+        const float4 nY = *reinterpret_cast<const float4*>(nb + 32u + sy);
+        const float4 fY = *reinterpret_cast<const float4*>(nb + (48u - sy));
+        const float4 nZ = *reinterpret_cast<const float4*>(nb + 64u + sz);
+        const float4 fZ = *reinterpret_cast<const float4*>(nb + (80u - sz));
+        const uint4 ch = *reinterpret_cast<const uint4*>(nb + 96);
         if (COUNT) {
             cnt->nodes++;
             cnt->boxes += 4;
         }
-        const bool px = s.idx >= 0.0f, py = s.idy >= 0.0f, pz = s.idz >= 0.0f;
         const float idx = s.idx, idy = s.idy, idz = s.idz;
-        float nx0 = ((px ? lx.x : hx.x) - r.ox) * idx, fx0 = ((px ? hx.x : lx.x) - r.ox) * idx;
-        float nx1 = ((px ? lx.y : hx.y) - r.ox) * idx, fx1 = ((px ? hx.y : lx.y) - r.ox) * idx;
-        float nx2 = ((px ? lx.z : hx.z) - r.ox) * idx, fx2 = ((px ? hx.z : lx.z) - r.ox) * idx;
-        float nx3 = ((px ? lx.w : hx.w) - r.ox) * idx, fx3 = ((px ? hx.w : lx.w) - r.ox) * idx;
-        float ny0 = ((py ? ly.x : hy.x) - r.oy) * idy, fy0 = ((py ? hy.x : ly.x) - r.oy) * idy;
-        float ny1 = ((py ? ly.y : hy.y) - r.oy) * idy, fy1 = ((py ? hy.y : ly.y) - r.oy) * idy;
-        float ny2 = ((py ? ly.z : hy.z) - r.oy) * idy, fy2 = ((py ? hy.z : ly.z) - r.oy) * idy;
-        float ny3 = ((py ? ly.w : hy.w) - r.oy) * idy, fy3 = ((py ? hy.w : ly.w) - r.oy) * idy;
-        float nz0 = ((pz ? lz.x : hz.x) - r.oz) * idz, fz0 = ((pz ? hz.x : lz.x) - r.oz) * idz;
-        float nz1 = ((pz ? lz.y : hz.y) - r.oz) * idz, fz1 = ((pz ? hz.y : lz.y) - r.oz) * idz;
-        float nz2 = ((pz ? lz.z : hz.z) - r.oz) * idz, fz2 = ((pz ? hz.z : lz.z) - r.oz) * idz;
-        float nz3 = ((pz ? lz.w : hz.w) - r.oz) * idz, fz3 = ((pz ? hz.w : lz.w) - r.oz) * idz;
+        float nx0 = (nX.x - r.ox) * idx, fx0 = (fX.x - r.ox) * idx;
+        float nx1 = (nX.y - r.ox) * idx, fx1 = (fX.y - r.ox) * idx;
+        float nx2 = (nX.z - r.ox) * idx, fx2 = (fX.z - r.ox) * idx;
+        float nx3 = (nX.w - r.ox) * idx, fx3 = (fX.w - r.ox) * idx;
+        float ny0 = (nY.x - r.oy) * idy, fy0 = (fY.x - r.oy) * idy;
+        float ny1 = (nY.y - r.oy) * idy, fy1 = (fY.y - r.oy) * idy;
+        float ny2 = (nY.z - r.oy) * idy, fy2 = (fY.z - r.oy) * idy;
+        float ny3 = (nY.w - r.oy) * idy, fy3 = (fY.w - r.oy) * idy;
+        float nz0 = (nZ.x - r.oz) * idz, fz0 = (fZ.x - r.oz) * idz;
+        float nz1 = (nZ.y - r.oz) * idz, fz1 = (fZ.y - r.oz) * idz;
+        float nz2 = (nZ.z - r.oz) * idz, fz2 = (fZ.z - r.oz) * idz;
+        float nz3 = (nZ.w - r.oz) * idz, fz3 = (fZ.w - r.oz) * idz;
         float tn0 = fmaxf(fmaxf(nx0, ny0), fmaxf(nz0, r.tmin));
         float tn1 = fmaxf(fmaxf(nx1, ny1), fmaxf(nz1, r.tmin));
         float tn2 = fmaxf(fmaxf(nx2, ny2), fmaxf(nz2, r.tmin));
