@@ -302,3 +302,23 @@ def test_tail_kernel_leaves_films_unchanged(name, monkeypatch):
     for other in films[1:]:
         for x, y in zip(films[0], other):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("integrator", [P.INTEGRATOR_PATH, P.INTEGRATOR_BDPT])
+def test_films_of_one_scene_one_after_another(integrator):
+    """A scene's leaf-gate tuning jobs and survival-profile copies run on the stream of the film that
+    started them; a film that is destroyed takes its stream with it. Films of one scene created, rendered
+    and destroyed one after another (each large enough to start a tuning job) all render without error,
+    and the same film renders the same image before and after the scene's tuning settled."""
+    s, cfg = scenes.cornell_metre(width=512)
+    scene = P.Scene(s)
+    p = P.sampling_params(1024, 1024, cfg["max_depth"], cfg["rr_min_bounces"], 2, max_passes=1, seed=3,
+                          integrator=integrator)
+    imgs = []
+    for _ in range(5):
+        pr = P.ProgressiveRaytracer(scene, p)
+        pr.render_samples(2)
+        imgs.append(pr.read()[0])
+        pr.close()
+    for img in imgs[1:]:
+        assert np.array_equal(img, imgs[0])
