@@ -331,13 +331,76 @@ __global__ __launch_bounds__(BLOCK) void k_intersect_refill(DevScene sc, const p
 // (most rays of a uniform distribution leave the root box at once, a few visit dozens of nodes), so
 // whole-wave batches idle most lanes. Here every lane holds the NEXT ray in registers, loaded while
 // the current one traverses: a lane that finishes starts its next ray in the same step with no wait,
-// and reloads its next-ray registers from the wave's ray chunk. Each wave's first chunk of kPfChunk
-// rays is static; later ones are reserved one ahead with an atomic whose result is read only when the
-// current chunk runs out, so neither the ray loads nor the work counter add latency to the steps.
+// and reloads its next-ray registers from the wave's ray chunk. Each wave's first chunk is static; later
+// ones are reserved one ahead with an atomic on its part's work counter whose result is read only when the
+// current chunk runs out, so neither the ray loads nor the work counters add latency to the steps.
 #ifndef PRT_PF_CHUNK
 #define PRT_PF_CHUNK 256
 #endif
 constexpr uint32_t kPfChunk = PRT_PF_CHUNK;
+
+// Work counters. Every chunk reservation is an atomic add on a counter, and one address sustains about
+// one such add per 14-20 ns device-wide: with one counter for the whole batch, 256-ray chunks of a
+// 16.8M-ray batch (65,536 reservations per ~1.3 ms launch) already kept it near saturation, and smaller
+// chunks made the launch slower (64 rays: 3.5 ms). The batch is split into kWorkParts contiguous parts,
+// each with its own counter on its own 256-B line (kWorkStride 64-bit words apart); work-group b starts
+// on part b % kWorkParts (work-groups are dispatched to the 8 XCDs in turn) and, once that part is used
+// up, reads every part's counter in one round trip and takes chunks of the next part with rays left.
+constexpr uint32_t kWorkParts = 8;
+constexpr uint32_t kWorkStride = 32;
+constexpr size_t kWorkBytes = 2 * (size_t)kWorkParts * kWorkStride * sizeof(unsigned long long);
+
+// Rays per chunk: `rays` / `parts` in multiples of 64, from one ray per lane up to kPfChunk.
+__device__ __forceinline__ unsigned long long pf_chunk(unsigned long long rays, unsigned long long parts) {
+    // 32-bit division: a quotient at or above kPfChunk is clamped anyway
+    const uint32_t r = (uint32_t)min(rays, 0xFFFFFFFFull);
+    const uint32_t d = (uint32_t)max(1ull, min(parts, 0xFFFFFFFFull));
+    const uint32_t c = ((r / d) + 63u) & ~63u;
+    return c < 64u ? 64ull : (c > kPfChunk ? (unsigned long long)kPfChunk : (unsigned long long)c);
+}
+
+// Part p of a batch of N rays: rays [lo, hi); its waves' first chunks (static, c0 rays each, at most half
+// the part over its waves) come first, then the chunks of its counter, from `dyn`.
+struct PfPart {
+    unsigned long long lo, hi, dyn, waves, c0;
+};
+__device__ __forceinline__ PfPart pf_part(unsigned long long N, uint32_t p, uint32_t grid, uint32_t waves_per_block) {
+    PfPart q;
+    q.lo = N * p / kWorkParts;
+    q.hi = N * (p + 1) / kWorkParts;
+    const unsigned long long blocks = grid / kWorkParts + (p < grid % kWorkParts ? 1u : 0u);
+    q.waves = blocks * waves_per_block;
+    q.c0 = pf_chunk(q.hi - q.lo, 2ull * q.waves);
+    q.dyn = min(q.lo + q.waves * q.c0, q.hi);
+    return q;
+}
+// Lane 0's value, as a wave-uniform (scalar) value.
+__device__ __forceinline__ unsigned long long lane0_u64(unsigned long long x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 0);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 0);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// The part table follows the counters, one 256-B line per part (words 1-5: dyn, hi, lo, c0, waves), so that
+// reading it does not touch the lines the counters' atomics work on.
+__device__ __forceinline__ PfPart pf_part_read(const unsigned long long* work, uint32_t p) {
+    const unsigned long long* w = work + (kWorkParts + p) * kWorkStride;
+    PfPart q;
+    q.dyn = lane0_u64(w[1]);
+    q.hi = lane0_u64(w[2]);
+    q.lo = lane0_u64(w[3]);
+    q.c0 = lane0_u64(w[4]);
+    q.waves = lane0_u64(w[5]);
+    return q;
+}
+
+// Size of the next chunk taken from a part whose counter is past `pos` (guided self-scheduling): a quarter
+// of a wave's share of the part's rays left, kPfChunk while much is left, down to one ray per lane, so
+// that the part's waves run out of rays at about the same time. (A wave runs through its chunk before it
+// looks for more; with fixed 256-ray chunks the waves ran out up to four rays per lane apart.)
+__device__ __forceinline__ unsigned long long pf_next_chunk(const PfPart& q, unsigned long long pos) {
+    return pf_chunk(q.hi > pos ? q.hi - pos : 0ull, 4ull * q.waves);
+}
 
 // Triangle-only scenes: 6 waves per SIMD (80 VGPRs, the LDS bound of the 8-entry stack and 64-node cache)
 // hide more node-fetch latency than the register count the compiler picks on its own (84, 5 waves).
@@ -389,16 +452,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long N = (unsigned long long)n;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    // wave-uniform chunk state: current chunk [cpos, cend); lane 0's pending atomic result locates
-    // the next one
-    // the first chunk of every wave is static (wave index); later chunks come from the work counter,
-    // offset by the static ones, reserved one ahead (no wave waits on an atomic to start)
-    const unsigned long long nwaves = (unsigned long long)gridDim.x * (BLOCK / 64);
-    const unsigned long long wid = (unsigned long long)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    unsigned long long cpos = wid * kPfChunk, cend = min(cpos + kPfChunk, N), nxt = 0;
-    bool exhausted = cpos >= N;
-    unsigned long long pend = 0;
-    if (!exhausted && lane == 0) pend = atomicAdd(work, (unsigned long long)kPfChunk);
+    // wave-uniform chunk state: current chunk [cpos, cend) of part `part`; lane 0's pending atomic result
+    // (psize rays of the part's counter) locates the next one
+    // a wave's first chunk is static (its index among its part's waves); later chunks come from the
+    // part's counter, reserved one ahead (no wave waits on an atomic to start), then from the next parts
+    // a wave's first chunk is static (its index among its part's waves); later chunks come from the
+    // part's counter, reserved one ahead (no wave waits on an atomic to start), then from other parts
+    uint32_t part = blockIdx.x % kWorkParts;
+    PfPart q = pf_part_read(work, part);
+    // the wave's index within its work-group, as a scalar (the chunk state is wave-uniform)
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const unsigned long long wid = (unsigned long long)(blockIdx.x / kWorkParts) * (BLOCK / 64) + wave;
+    unsigned long long cpos = min(q.lo + wid * q.c0, q.hi), cend = min(cpos + q.c0, q.hi), nxt = 0;
+    unsigned long long pend = 0, psize = pf_next_chunk(q, q.dyn);
+    bool exhausted = false;
+    if (lane == 0) pend = atomicAdd(work + part * kWorkStride, psize);
     TState s;
     bool active = false;
     int64_t ridx = 0, nidx = -1;
@@ -417,14 +485,38 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
         uint64_t need = __ballot(nidx < 0);
         while (need != 0 && !exhausted) {
             if (cpos >= cend) {
-                nxt = nwaves * kPfChunk + __shfl(pend, 0, 64);
-                if (nxt >= N) {
-                    exhausted = true;
-                    break;
+                nxt = q.dyn + lane0_u64(pend);
+                if (nxt >= q.hi) {
+                    // this part is used up: lane k reads part k's counter (all in one round trip) and
+                    // the wave takes one-ray-per-lane chunks of the next part that has rays left
+                    // (the empty asm keeps the lane-indexed addressing here instead of hoisted into
+                    // registers held across the traversal loop)
+                    uint32_t k = lane;
+                    asm volatile("" : "+v"(k));
+                    bool left = false;
+                    if (k < kWorkParts) {
+                        const unsigned long long* wk = work + k * kWorkStride;
+                        const unsigned long long taken =
+                            __hip_atomic_load(wk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        left = wk[kWorkParts * kWorkStride + 1] + taken < wk[kWorkParts * kWorkStride + 2];
+                    }
+                    const uint32_t open = (uint32_t)__ballot(left);
+                    if (open == 0u) {
+                        exhausted = true;
+                        break;
+                    }
+                    // the first open part after this one, cyclically
+                    const uint32_t rot = (open >> (part + 1)) | (open << (kWorkParts - part - 1));
+                    part = (part + 1 + (uint32_t)__builtin_ctz(rot & ((1u << kWorkParts) - 1u))) % kWorkParts;
+                    q = pf_part_read(work, part);
+                    psize = 64;
+                    if (lane == 0) pend = atomicAdd(work + part * kWorkStride, psize);
+                    continue;
                 }
                 cpos = nxt;
-                cend = min(nxt + kPfChunk, N);
-                if (lane == 0) pend = atomicAdd(work, (unsigned long long)kPfChunk);
+                cend = min(nxt + psize, q.hi);
+                psize = pf_next_chunk(q, cend);
+                if (lane == 0) pend = atomicAdd(work + part * kWorkStride, psize);
                 continue;
             }
             const uint32_t avail = (uint32_t)(cend - cpos);
@@ -527,10 +619,23 @@ __global__ __launch_bounds__(64) void k_intersect_long(DevScene sc, const prt_ra
     }
 }
 
-// Zeroes the work counter and the park count before an intersection launch.
-__global__ void k_clear_counters(unsigned long long* work, uint32_t* park_count) {
-    *work = 0ull;
-    *park_count = 0u;
+// Zeroes the work counters and the park count before an intersection launch
+// and, for k_intersect_pf, writes the part table of a batch of n rays on `grid` work-groups of
+// `waves_per_block` waves (one thread per part).
+__global__ void k_clear_counters(unsigned long long* work, uint32_t* park_count, unsigned long long n, uint32_t grid,
+                                 uint32_t waves_per_block) {
+    const uint32_t p = threadIdx.x;
+    if (p < kWorkParts) {
+        const PfPart q = pf_part(n, p, grid, waves_per_block);
+        work[p * kWorkStride] = 0ull;
+        unsigned long long* w = work + (kWorkParts + p) * kWorkStride;
+        w[1] = q.dyn;
+        w[2] = q.hi;
+        w[3] = q.lo;
+        w[4] = q.c0;
+        w[5] = q.waves;
+    }
+    if (p == 0) *park_count = 0u;
 }
 
 // Kernel variants the per-scene autotuner chooses from (identical per-ray traversal): kernel
@@ -687,13 +792,14 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
     const hipError_t stale = hipGetLastError();
     const char* failed = "intersect kernel launch";
     auto run = [&](int v, int64_t count) -> hipError_t {
-        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, st, work, park.count);
+        int blocks = ms->variant_blocks[v];
+        hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(kWorkParts), 0, st, work, park.count,
+                           (unsigned long long)count, (uint32_t)blocks, (uint32_t)(kVariantBlock[v] / 64));
         hipError_t e0 = hipGetLastError();
         if (e0 != hipSuccess) {
             failed = "counter clear launch";
             return e0;
         }
-        int blocks = ms->variant_blocks[v];
         park.budget = park_budget(scene, count, (int64_t)blocks * kVariantBlock[v]);
         if (mode == 0) {
             if (sc.mixed) e0 = launch_variant<false, true>(v, blocks, st, sc, (const prt_ray*)d_rays, count, (prt_hit*)d_out0, nullptr, gs, gstride, work, park, lb);
@@ -777,13 +883,14 @@ static int launch_intersect(const prt_scene* scene, int mode, const prt_ray* ray
 
 // ---- allocation-free closest hit (prt_intersect_closest_static) ------------------------------------
 // The work counter, the park queue and the traversal overflow stack of the intersection kernels live in
-// one static device array instead of scene memory: 256 B for the work counter, the park count (256 B)
-// and kStaticParkCapacity ray indices, then the stack, entry e of thread slot g at [e * slots + g] as in
+// one static device array instead of scene memory: kWorkBytes for the work counters, the park count
+// (256 B) and kStaticParkCapacity ray indices, then the stack, entry e of thread slot g at [e * slots + g] as in
 // scene memory. Each device holds its own copy (resolved once per device
 // with hipGetSymbolAddress); calls on one device share it, so calls that overlap on one device race.
 constexpr size_t kStaticScratchBytes = (size_t)PRT_STATIC_SCRATCH_BYTES;
 constexpr uint32_t kStaticParkCapacity = 1u << 16;
-constexpr size_t kStaticStackOffset = 512 + sizeof(uint32_t) * (size_t)kStaticParkCapacity;
+constexpr size_t kStaticParkOffset = kWorkBytes;
+constexpr size_t kStaticStackOffset = kStaticParkOffset + 256 + sizeof(uint32_t) * (size_t)kStaticParkCapacity;
 __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
 
 namespace {
@@ -1171,8 +1278,8 @@ int prt_scene_create(int32_t device, const prt_scene_desc* d, prt_scene** out) {
     PRT_HIP(hipGetDeviceProperties(&prop, device));
     s->num_cus = prop.multiProcessorCount;
     void* p;
-    // bufs[0]: work counter (kept first: the intersection launcher uses it)
-    if ((rc = scene_alloc(s, 256, nullptr, &p)) != PRT_OK) return bail(rc);
+    // bufs[0]: work counters (kept first: the intersection launcher uses them)
+    if ((rc = scene_alloc(s, kWorkBytes, nullptr, &p)) != PRT_OK) return bail(rc);
     if ((rc = scene_alloc(s, bvh.nodes.size() * sizeof(Node4), bvh.nodes.data(), &p)) != PRT_OK) return bail(rc);
     s->dev.nodes = (const Node4*)p;
     if ((rc = scene_alloc(s, ordered.size() * sizeof(float4), ordered.data(), &p)) != PRT_OK) return bail(rc);
@@ -1503,14 +1610,15 @@ int prt_intersect_closest_static(const prt_scene* scene, const prt_ray* rays, in
     // the full occupancy grid where the stack fits, else as many work-groups as it holds
     const int blocks = (int)std::min<int64_t>(scene->variant_blocks[v], fit);
     unsigned long long* work = (unsigned long long*)base;
-    uint32_t* pk = (uint32_t*)((char*)base + 256);
+    uint32_t* pk = (uint32_t*)((char*)base + kStaticParkOffset);
     const ParkQueue park{pk, pk + 64, kStaticParkCapacity, park_budget(scene, n, (int64_t)blocks * block)};
     uint2* gs = (uint2*)((char*)base + kStaticStackOffset);
     const uint32_t slots = (uint32_t)blocks * (uint32_t)block;
     hipStream_t st = (hipStream_t)stream;
     const int lb = long_blocks(scene, slots);
     (void)hipGetLastError();  // an earlier call's error on this thread is not this launch's
-    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(1), 0, st, work, park.count);
+    hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(kWorkParts), 0, st, work, park.count, (unsigned long long)n,
+                       (uint32_t)blocks, (uint32_t)(block / 64));
     PRT_HIP(hipGetLastError());
     if (scene->dev.mixed)
         PRT_HIP((launch_variant<false, true>(v, blocks, st, scene->dev, rays, n, hits, nullptr, gs, slots, work, park, lb)));
