@@ -128,8 +128,8 @@ def main():
         roofline.update({"traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel variant at this "
                                            "ray count, calibrated against a copy kernel (profiles/)",
                          "hbm_measured_GBps": round(hbm, 1), "hbm_measured_frac": round(hbm / HBM_PEAK_GBS, 4),
-                         "limiter": "dependent node fetches at 6 waves/SIMD (80 VGPRs) with the VALU ~67 % busy "
-                                    "(a 4-wide node step is ~136 VALU instructions, a leaf step ~100; leaves "
+                         "limiter": "dependent node fetches at 6 waves/SIMD (80 VGPRs) with the VALU ~68 % busy "
+                                    "(a 4-wide node step is ~125 VALU instructions, a leaf step ~100; leaves "
                                     "gated to 2/8 of a wave); "
                                     f"measured HBM traffic runs at {hbm / 1e3:.2f} TB/s"})
 
