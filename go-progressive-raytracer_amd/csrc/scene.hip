@@ -374,6 +374,8 @@ __device__ __forceinline__ PfPart pf_part(unsigned long long N, uint32_t p, uint
     q.dyn = min(q.lo + q.waves * q.c0, q.hi);
     return q;
 }
+typedef float f32x4 __attribute__((ext_vector_type(4)));  // a hit record, for vector non-temporal stores
+
 // Lane 0's value, as a wave-uniform (scalar) value.
 __device__ __forceinline__ unsigned long long lane0_u64(unsigned long long x) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 0);
@@ -540,13 +542,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 1
             if (tstate_step<ANY, MIXED, false, LDS, BLOCK, NCACHE>(sc, s, st, nullptr, leaf_ok)) {
                 active = false;
                 bool hit = s.best != 0xFFFFFFFFu;
+                // results are written with non-temporal stores: nothing here reads them again, and
+                // streaming them past the L2 leaves its lines to the nodes (16.8M-ray launch: 0.53 ->
+                // 0.47 GB written, 1.195 -> 1.144 ms)
                 if (ANY) {
-                    occluded[ridx] = hit ? 1 : 0;
+                    __builtin_nontemporal_store((uint8_t)(hit ? 1 : 0), occluded + ridx);
                 } else {
                     int32_t pid = hit ? (int32_t)__float_as_uint(sc.prims[3 * (size_t)s.best + 2].w) : -1;
-                    reinterpret_cast<float4*>(hits)[ridx] =
-                        make_float4(hit ? s.tbest : __int_as_float(0x7f800000), __int_as_float(pid),
-                                    hit ? s.bu : 0.0f, hit ? s.bv : 0.0f);
+                    const f32x4 hv = {hit ? s.tbest : __int_as_float(0x7f800000), __int_as_float(pid),
+                                      hit ? s.bu : 0.0f, hit ? s.bv : 0.0f};
+                    __builtin_nontemporal_store(hv, reinterpret_cast<f32x4*>(hits) + ridx);
                 }
             } else if (progress && ++steps == park.budget) {
                 const uint32_t k = atomicAdd(park.count, 1u);
