@@ -1024,6 +1024,151 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     }
 }
 
+// General connections through per-pair queues (the same strategies as k_bdpt_connect_general, for films
+// whose pairs fit kMaxPairs). Pair p = (s - 2) * nt + (t - 2) for s = 2 .. max_depth, t = 2 .. max_depth + 1.
+//   k_bdpt_pairs:        one thread per sample appends the sample to the queue of every pair whose light
+//                        vertex s - 1 and camera vertex t - 1 are both non-specular (the pairs the per-sample
+//                        loop would evaluate);
+//   k_bdpt_pair_connect: one lane per queued (pair, sample): contribution, MIS weight and visibility ray as in
+//                        k_bdpt_connect_general; writes the weighted contribution to res[p][sample] with w = 1
+//                        when it is added (unoccluded, non-zero), w = 0 otherwise;
+//   k_bdpt_pair_sum:     one thread per sample adds its pairs' results to pb.L in the per-sample loop's order
+//                        (s outer, t inner), so the radiance is bit-identical to the per-sample kernel's.
+// All lanes of a wave evaluate one pair, so no lane waits for another lane's longer subpaths, and a pair's
+// queue holds samples in about increasing order, so a wave's vertex reads are coalesced.
+constexpr uint32_t kMaxPairs = 256;
+
+struct PairBuffers {
+    uint32_t* queue;   // [pair][slot]: slots queued for the pair
+    float4* res;       // [pair][slot]: weighted contribution, w = 1 if it is added
+    uint32_t* count;   // queue length of pair p at count[p * kCounterStride]
+    uint32_t nt;       // camera-side pairs per light vertex (max_depth)
+    uint32_t np;       // pairs ((max_depth - 1) * max_depth)
+    uint32_t stride;   // slots per pair plane
+};
+
+__global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStore CP, VertexStore LP,
+                                                       const uint32_t* lens, PairBuffers pq) {
+    const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t lm = 0, cm = 0;  // non-specular light vertices s - 1 (bit s - 2), camera vertices t - 1 (bit t - 2)
+    if (i < n) {
+        const uint32_t len = lens[i];
+        const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
+        if (nl >= 2 && nc >= 2) {
+            for (int s = 2; s <= nl; s++)
+                if (!rec_specular(LP.rec(s - 1, i))) lm |= 1u << (s - 2);
+            for (int t = 2; t <= nc; t++)
+                if (!rec_specular(CP.rec(t - 1, i))) cm |= 1u << (t - 2);
+        }
+        if (!lm) cm = 0;
+        if (!cm) lm = 0;
+    }
+    for (uint32_t s = 0; __ballot((lm >> s) != 0u) != 0; s++) {
+        const bool ls = ((lm >> s) & 1u) != 0;
+        for (uint32_t t = 0; __ballot((cm >> t) != 0u) != 0; t++) {
+            const uint32_t p = s * pq.nt + t;
+            queue_push(pq.queue + (size_t)p * pq.stride, pq.count + p * kCounterStride, i,
+                       ls && ((cm >> t) & 1u) != 0);
+        }
+    }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_bdpt_pair_connect(DevScene sc, PassParams pp, VertexStore CP,
+                                                              VertexStore LP, PairBuffers pq, uint2* gstack,
+                                                              uint32_t gstride) {
+    __shared__ uint32_t s_node[kLds * kBlock];
+    __shared__ float s_t[kLds * kBlock];
+    __shared__ float4 s_cache[kNodeCache * 8];
+    __shared__ uint32_t s_end[kMaxPairs];  // inclusive prefix sums of the queue lengths
+    TStack<kLds, kBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kBlock + threadIdx.x;
+    st.gstride = gstride;
+    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
+    st.ncache = s_cache;
+    static_assert(kMaxPairs == kBlock, "one thread per pair in the prefix sum");
+    s_end[threadIdx.x] = threadIdx.x < pq.np ? pq.count[threadIdx.x * kCounterStride] : 0u;
+    __syncthreads();
+    for (uint32_t o = 1; o < kMaxPairs; o <<= 1) {
+        const uint32_t v = threadIdx.x >= o ? s_end[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s_end[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const DevCamera& cam = sc.cam;
+    const uint32_t total = s_end[kMaxPairs - 1];
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += stride) {
+        uint32_t lo = 0, hi = pq.np - 1;  // the pair whose range holds j
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (j < s_end[mid])
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        const uint32_t p = lo;
+        const uint32_t i = pq.queue[(size_t)p * pq.stride + (j - (p ? s_end[p - 1] : 0u))];
+        const int s = (int)(p / pq.nt) + 2, t = (int)(p % pq.nt) + 2;
+        const BV ls1 = LP.get(s - 1, i), ct1 = CP.get(t - 1, i);
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+        V3 dd = ls1.p - ct1.p;
+        const float dist = length(dd);
+        if (dist >= 0.001f) {
+            dd = dd * (1.0f / dist);
+            const float cc = dot(dd, ct1.n), cl = dot(dd * -1.0f, ls1.n);
+            if (cc > 0.0f && cl > 0.0f) {
+                const float G = (cc * cl) / (dist * dist);
+                const V3 fc = vertex_brdf(sc, ct1, dd, 0);
+                const V3 fl = vertex_brdf(sc, ls1, dd * -1.0f, 1);
+                const V3 contrib = mulv(mulv(mulv(ls1.beta, fl), fc), ct1.beta) * G;
+                if (contrib.x != 0.0f || contrib.y != 0.0f || contrib.z != 0.0f) {
+                    const BV ls2 = LP.get(s - 2, i), ct2 = CP.get(t - 2, i);
+                    const float l_tail = ((ls2.flags & VF_CONN) ? 1.0f : 0.0f) + ls2.q, l_fwd2 = remap0(ls2.pdf_fwd);
+                    const Seg c = seg_of(ls2.p - ls1.p);
+                    const V3 lw = contrib * mis_general(sc, cam, ct1, ct2, ls1, ls2, c, l_tail, l_fwd2, t);
+                    const TRay vr{ct1.p.x, ct1.p.y, ct1.p.z, 0.001f, dd.x, dd.y, dd.z, dist - 0.001f};
+                    THit vh;
+                    if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr, pp.trace_gate))
+                        r = make_float4(lw.x, lw.y, lw.z, 1.0f);
+                }
+            }
+        }
+        pq.res[(size_t)p * pq.stride + i] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bdpt_pair_sum(PassParams pp, PathBuffers pb, VertexStore CP,
+                                                          VertexStore LP, const uint32_t* lens, PairBuffers pq) {
+    const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t len = lens[i];
+    const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
+    if (nl < 2 || nc < 2) return;
+    uint32_t cm = 0;
+    for (int t = 2; t <= nc; t++)
+        if (!rec_specular(CP.rec(t - 1, i))) cm |= 1u << (t - 2);
+    if (!cm) return;
+    const float4 l4 = pb.L[i];
+    V3 total = v3(l4.x, l4.y, l4.z);
+    for (int s = 2; s <= nl; s++) {
+        if (rec_specular(LP.rec(s - 1, i))) continue;
+        const float4* row = pq.res + (size_t)(s - 2) * pq.nt * pq.stride + i;
+        for (int t = 2; t <= nc; t++) {
+            if (!((cm >> (t - 2)) & 1u)) continue;
+            const float4 r = row[(size_t)(t - 2) * pq.stride];
+            if (r.w != 0.0f) total = total + v3(r.x, r.y, r.z);
+        }
+    }
+    pb.L[i] = make_float4(total.x, total.y, total.z, 0.f);
+}
+
 // Film (PixelStats.AddSample, stats.go:41-47): per pixel, each sample is added to the running sums with
 // Kahan compensation, in sample order.
 struct FilmBuffers {
@@ -1213,6 +1358,9 @@ struct prt_film {
     bool bdpt = false;
     int bdpt_blocks = 0;
     int bdpt_general_blocks = 0;  // grid of k_bdpt_connect_general (its own occupancy)
+    bool bdpt_pairs = false;      // general connections through per-pair queues
+    int bdpt_pair_blocks = 0;     // grid of k_bdpt_pair_connect
+    PairBuffers pairs{};
     VertexStore cpath{}, lpath{};
     uint32_t* lens = nullptr;
     hipStream_t stream = nullptr;
@@ -1586,11 +1734,41 @@ size_t bdpt_store_bytes(int max_depth, int64_t slots, bool uv) {
     return (size_t)(max_depth + 1) * (uv ? 6 : 5) * sizeof(float4) * (size_t)slots;
 }
 
-// Path slots of a BDPT film: the two subpath stores stay within 2 GiB.
+// Whether a BDPT film of this depth evaluates its general connections through per-pair queues (k_bdpt_pairs,
+// k_bdpt_pair_connect, k_bdpt_pair_sum): when its (max_depth - 1) * max_depth pairs fit kMaxPairs
+// (max_depth <= 16); deeper films loop over the pairs per sample (k_bdpt_connect_general).
+// PRT_BDPT_PAIR_QUEUES=0 selects the per-sample kernel for every depth.
+bool bdpt_pairs_used(int max_depth) {
+    if (const char* e = std::getenv("PRT_BDPT_PAIR_QUEUES"))
+        if (std::atoi(e) == 0) return false;
+    return max_depth >= 2 && (int64_t)(max_depth - 1) * max_depth <= (int64_t)kMaxPairs;
+}
+
+// Bytes of the pair queues and results of `slots` path slots: a slot id and a float4 result per pair.
+size_t bdpt_pair_bytes(int max_depth, int64_t slots) {
+    if (!bdpt_pairs_used(max_depth)) return 0;
+    return (size_t)(max_depth - 1) * (size_t)max_depth * (sizeof(uint32_t) + sizeof(float4)) * (size_t)slots +
+           kMaxPairs * kCounterStride * sizeof(uint32_t);
+}
+
+// Path slots of a BDPT film: the two subpath stores stay within 2 GiB, and with the pair queues within 4 GiB.
 int64_t bdpt_capacity(int max_depth, int64_t cap, bool uv) {
-    int64_t c = ((int64_t)1 << 31) / (int64_t)(2 * bdpt_store_bytes(max_depth, 1, uv));
+    const int64_t per_slot = (int64_t)(2 * bdpt_store_bytes(max_depth, 1, uv));
+    int64_t c = ((int64_t)1 << 31) / per_slot;
+    if (bdpt_pairs_used(max_depth))
+        c = std::min<int64_t>(c, ((int64_t)1 << 32) / (per_slot + (int64_t)(max_depth - 1) * max_depth * 20));
     c = std::max<int64_t>(c, (int64_t)1 << 16) & ~(int64_t)255;
     return std::min(cap, c);
+}
+
+// Grid of k_bdpt_pair_connect: its resident blocks twice, at most the trace grid (stack slots).
+int bdpt_pair_blocks(const prt_scene* scene, int trace_blocks) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_bdpt_pair_connect<true>, kBlock, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 2;
+    return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu * 2));
 }
 
 }  // namespace
@@ -1955,17 +2133,36 @@ int film_enqueue_chunk(prt_film* f) {
         hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, true));
+        if (f->bdpt_pairs) {
+            // pair queues: filled after the subpaths, drained by the pair kernel; the per-sample strategies run
+            // first, the pair results are added to their radiance last
+            PRT_HIP(hipMemsetAsync(f->pairs.count, 0, kMaxPairs * kCounterStride * sizeof(uint32_t), st));
+            hipLaunchKernelGGL(k_bdpt_pairs, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens,
+                               f->pairs);
+        }
+        const int grid_p = (int)std::min<int64_t>(f->bdpt_pair_blocks, (n + kBlock - 1) / kBlock);
         if (sc.mixed) {
             hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
-            hipLaunchKernelGGL((k_bdpt_connect_general<true>), dim3(grid_g), dim3(kBlock), 0, st, sc,
-                               pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
+            if (f->bdpt_pairs)
+                hipLaunchKernelGGL((k_bdpt_pair_connect<true>), dim3(grid_p), dim3(kBlock), 0, st, sc, pp, f->cpath,
+                                   f->lpath, f->pairs, f->gstack, f->gslots);
+            else
+                hipLaunchKernelGGL((k_bdpt_connect_general<true>), dim3(grid_g), dim3(kBlock), 0, st, sc,
+                                   pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
         } else {
             hipLaunchKernelGGL((k_bdpt_connect<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
-            hipLaunchKernelGGL((k_bdpt_connect_general<false>), dim3(grid_g), dim3(kBlock), 0, st,
-                               sc, pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
+            if (f->bdpt_pairs)
+                hipLaunchKernelGGL((k_bdpt_pair_connect<false>), dim3(grid_p), dim3(kBlock), 0, st, sc, pp, f->cpath,
+                                   f->lpath, f->pairs, f->gstack, f->gslots);
+            else
+                hipLaunchKernelGGL((k_bdpt_connect_general<false>), dim3(grid_g), dim3(kBlock), 0, st,
+                                   sc, pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
         }
+        if (f->bdpt_pairs)
+            hipLaunchKernelGGL(k_bdpt_pair_sum, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->pb, f->cpath, f->lpath,
+                               f->lens, f->pairs);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_FILM, true));
         if (f->adaptive)
@@ -2167,7 +2364,9 @@ size_t film_device_bytes(const prt_scene* scene, const prt_render_params* p, int
     for (int v = 0; v < kTraceVariants; v++) blocks = std::max(blocks, trace_grid(scene, v));
     size_t g = film_gstack_entries(scene) * (size_t)blocks * kBlock * sizeof(uint2);
     size_t b = 0;
-    if (bd) b = 2 * bdpt_store_bytes(p->max_depth, C, uv) + sizeof(uint32_t) * C + 3 * sizeof(long long) * npix;
+    if (bd)
+        b = 2 * bdpt_store_bytes(p->max_depth, C, uv) + bdpt_pair_bytes(p->max_depth, C) + sizeof(uint32_t) * C +
+            3 * sizeof(long long) * npix;
     if (p->adaptive_min_samples < 1.0) b += 2 * sizeof(uint32_t) * npix + 6 * sizeof(uint32_t);
     b += sizeof(uchar4) * (size_t)npix;  // the device image of RGBA8 reads, allocated at the first read
     b += sizeof(uint32_t) * (size_t)std::min<int64_t>(2 * C, kLongCapacity);  // long-ray queue
@@ -2274,6 +2473,20 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
         f->lens = (uint32_t*)ptr;
         if ((rc = film_alloc(f, 3 * sizeof(long long) * (size_t)f->npix, &ptr)) != PRT_OK) return bail(rc);
         f->fb.splat = (long long*)ptr;
+        if (bdpt_pairs_used(p->max_depth)) {
+            f->bdpt_pairs = true;
+            f->bdpt_pair_blocks = bdpt_pair_blocks(scene, f->blocks);
+            PairBuffers& pq = f->pairs;
+            pq.nt = (uint32_t)p->max_depth;
+            pq.np = (uint32_t)(p->max_depth - 1) * pq.nt;
+            pq.stride = (uint32_t)C;
+            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)pq.np * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
+            pq.queue = (uint32_t*)ptr;
+            if ((rc = film_alloc(f, sizeof(float4) * (size_t)pq.np * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
+            pq.res = (float4*)ptr;
+            if ((rc = film_alloc(f, kMaxPairs * kCounterStride * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
+            pq.count = (uint32_t*)ptr;
+        }
     }
     if (p->adaptive_min_samples < 1.0) {
         f->adaptive = true;
