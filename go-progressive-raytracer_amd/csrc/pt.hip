@@ -1049,8 +1049,12 @@ struct PairBuffers {
 
 __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStore CP, VertexStore LP,
                                                        const uint32_t* lens, PairBuffers pq) {
+    // per wave: entries of each pair (then the pair queue position of the wave's first entry)
+    __shared__ uint32_t s_cnt[kBlock / 64][kMaxPairs];
     const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t lm = 0, cm = 0;  // non-specular light vertices s - 1 (bit s - 2), camera vertices t - 1 (bit t - 2)
     if (i < n) {
         const uint32_t len = lens[i];
@@ -1064,18 +1068,35 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStor
         if (!lm) cm = 0;
         if (!cm) lm = 0;
     }
-    for (uint32_t s = 0; __ballot((lm >> s) != 0u) != 0; s++) {
-        const bool ls = ((lm >> s) & 1u) != 0;
-        for (uint32_t t = 0; __ballot((cm >> t) != 0u) != 0; t++) {
-            const uint32_t p = s * pq.nt + t;
-            queue_push(pq.queue + (size_t)p * pq.stride, pq.count + p * kCounterStride, i,
-                       ls && ((cm >> t) & 1u) != 0);
+    for (uint32_t p = lane; p < pq.np; p += 64) s_cnt[w][p] = 0;
+    // the pairs any lane of the wave has: s < ns, t < nt (wave-uniform bounds)
+    uint32_t ns = 0, nt = 0;
+    while (__ballot((lm >> ns) != 0u) != 0) ns++;
+    while (__ballot((cm >> nt) != 0u) != 0) nt++;
+    for (uint32_t s = 0; s < ns; s++)
+        for (uint32_t t = 0; t < nt; t++) {
+            const uint64_t m = __ballot(((lm >> s) & (cm >> t) & 1u) != 0);
+            if (lane == 0) s_cnt[w][s * pq.nt + t] = (uint32_t)__popcll(m);
         }
+    // one queue reservation per pair and wave, all issued together
+    for (uint32_t p = lane; p < pq.np; p += 64) {
+        const uint32_t c = s_cnt[w][p];
+        if (c) s_cnt[w][p] = atomicAdd(pq.count + p * kCounterStride, c);
     }
+    for (uint32_t s = 0; s < ns; s++)
+        for (uint32_t t = 0; t < nt; t++) {
+            const bool a = ((lm >> s) & (cm >> t) & 1u) != 0;
+            const uint64_t m = __ballot(a);
+            const uint32_t p = s * pq.nt + t;
+            if (a) pq.queue[(size_t)p * pq.stride + s_cnt[w][p] + (uint32_t)__popcll(m & lt)] = i;
+        }
 }
 
+#ifndef PRT_PAIR_WAVES
+#define PRT_PAIR_WAVES 4  // waves per SIMD of k_bdpt_pair_connect (4: 127 VGPRs, no spills)
+#endif
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_bdpt_pair_connect(DevScene sc, PassParams pp, VertexStore CP,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR_WAVES))) void k_bdpt_pair_connect(DevScene sc, PassParams pp, VertexStore CP,
                                                               VertexStore LP, PairBuffers pq, uint2* gstack,
                                                               uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
