@@ -746,6 +746,132 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_walks(PassParams pp, VertexStor
     store_walk_sums(LP, i, (int)(len >> 16), true);
 }
 
+// Strategy (s, t) of sample i (key: its counter-RNG key) for s = 0 (t = nc), s = 1 or t = 1, or a general
+// connection: evaluatePathTracingStrategy (bdpt.go:316-330), evaluateDirectLightingStrategy (:333-391),
+// evaluateLightTracingStrategy (:394-471), evaluateConnectionStrategy (:489-561), each with its visibility ray
+// and calculateMISWeight. Light-tracing strategies (t = 1) splat their contribution themselves and return
+// false; the others return true with the weighted camera-side radiance in `out` when it is added.
+template <bool MIXED>
+__device__ __forceinline__ bool eval_strategy(const DevScene& sc, const PassParams& pp, const VertexStore& CP,
+                                              const VertexStore& LP, uint32_t i, uint32_t key, int s, int t,
+                                              long long* splat, TStack<kLds, kBlock>& st, V3& out) {
+    const DevCamera& cam = sc.cam;
+    V3 Lc = v3(0, 0, 0);
+    bool splatted = false;
+    int sx = 0, sy = 0;
+    BV sampled, ct1, ls1;
+    TRay vr;  // visibility ray of the strategy (one trace call site keeps the kernel call-free)
+    bool vis = false;
+    ct1 = CP.get(t - 1, i);
+    if (s > 0) ls1 = LP.get(s - 1, i);
+    if (s == 0) {  // evaluatePathTracingStrategy :316-330
+        Lc = mulv(ct1.Le, ct1.beta);
+    } else if (t == 1) {  // evaluateLightTracingStrategy :394-471
+        const BV& lv = ls1;
+        if (lv.flags & VF_SPECULAR) return false;
+        const uint32_t sp0 = (kSplatSlot + (uint32_t)s) * 16u;
+        V3 lc = sample_point_in_unit_disk(draw(key, sp0), draw(key, sp0 + 1)) * cam.lens_radius;
+        V3 lens_p = v3(cam.center) + v3(cam.u) * lc.x + v3(cam.v) * lc.y;
+        V3 dir = normalize(lv.p - lens_p);
+        float dist = length(lv.p - lens_p);
+        float cf = fabsf(dot(v3(cam.forward), dir));
+        float cpdf = (dist * dist) / (cf * cam.lens_area);
+        float ctw = dot(dir, v3(cam.forward));
+        float we = ctw < cam.cos_total_width
+                       ? 0.0f
+                       : 1.0f / (cam.plane_area * cam.lens_area * ctw * ctw * ctw * ctw);
+        if (we == 0.0f) return false;
+        V3 back = dir * -1.0f;
+        V3 f = vertex_brdf(sc, lv, back, 1);
+        V3 cbeta = v3(we, we, we) * (1.0f / cpdf);
+        float cosine = dot(back, lv.n);
+        if (cosine <= 0.0f) return false;
+        V3 contrib = mulv(mulv(f, cbeta), lv.beta);
+        if (on_surface(sc, lv)) contrib = contrib * cosine;
+        if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) return false;
+        // the splat lands outside the image: SplatQueue drops it, nothing else uses it
+        if (!cam_map(cam, lens_p, dir, sx, sy)) return false;
+        vr = TRay{lv.p.x, lv.p.y, lv.p.z, 0.001f, back.x, back.y, back.z, dist - 0.001f};
+        vis = true;
+        sampled.p = lens_p;
+        sampled.n = back;
+        sampled.wi = v3(0, 0, 0);
+        sampled.beta = cbeta;
+        sampled.Le = v3(0, 0, 0);
+        sampled.pdf_fwd = cpdf;
+        sampled.pdf_rev = total_light_pdf(sc, lens_p, back, dir);
+        sampled.mat = -1;
+        sampled.light = -1;
+        sampled.flags = VF_CAMERA;
+        Lc = contrib;
+        splatted = true;
+    } else if (s == 1) {  // evaluateDirectLightingStrategy :333-391
+        const BV& cvt = ct1;
+        if ((cvt.flags & VF_SPECULAR) || cvt.mat < 0) return false;
+        const uint32_t dp0 = (kDirectSlot + (uint32_t)t) * 16u;
+        float sel;
+        int li = select_light(sc, draw(key, dp0), sel);
+        LSample ls = light_sample(sc, sc.lights[li], cvt.p, cvt.n, draw(key, dp0 + 1), draw(key, dp0 + 2));
+        ls.pdf *= sel;
+        if ((ls.emission.x == 0.0f && ls.emission.y == 0.0f && ls.emission.z == 0.0f) || ls.pdf <= 0.0f)
+            return false;
+        float ct = fabsf(dot(ls.dir, cvt.n));
+        if (ct <= 0.0f) return false;
+        V3 f = brdf(sc, cvt.mat, cvt.wi, ls.dir, surf_of(cvt), 0);
+        V3 lbeta = ls.emission * (1.0f / ls.pdf);
+        V3 contrib = mulv(mulv(f, cvt.beta), lbeta) * ct;
+        if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) return false;
+        vr = TRay{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - 0.001f};
+        vis = true;
+        sampled.p = ls.p;
+        sampled.n = ls.n;
+        sampled.wi = v3(0, 0, 0);
+        sampled.beta = lbeta;
+        sampled.Le = ls.emission;
+        sampled.pdf_fwd = ls.pdf;
+        sampled.pdf_rev = 0.0f;
+        sampled.mat = -1;
+        sampled.light = li;
+        sampled.flags = light_vertex_flags(sc, li);
+        Lc = contrib;
+    } else {  // evaluateConnectionStrategy :489-561
+        const BV &lv = ls1, &cvt = ct1;
+        if ((lv.flags & VF_SPECULAR) || (cvt.flags & VF_SPECULAR)) return false;
+        V3 dd = lv.p - cvt.p;
+        float dist = length(dd);
+        if (dist < 0.001f) return false;
+        dd = dd * (1.0f / dist);
+        float cc = dot(dd, cvt.n), cl = dot(dd * -1.0f, lv.n);
+        if (cc <= 0.0f || cl <= 0.0f) return false;
+        float G = (cc * cl) / (dist * dist);
+        V3 fc = vertex_brdf(sc, cvt, dd, 0);
+        V3 fl = vertex_brdf(sc, lv, dd * -1.0f, 1);
+        V3 contrib = mulv(mulv(mulv(lv.beta, fl), fc), cvt.beta) * G;
+        if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) return false;
+        vr = TRay{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, dd.x, dd.y, dd.z, dist - 0.001f};
+        vis = true;
+        Lc = contrib;
+    }
+    if (vis) {
+        THit vh;
+        if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr, pp.trace_gate)) return false;
+    }
+    if (Lc.x == 0.0f && Lc.y == 0.0f && Lc.z == 0.0f && !splatted) return false;
+    float w = mis_weight(sc, cam, CP, LP, i, sampled, ct1, ls1, s, t);
+    if (splatted) {
+        V3 c = Lc * w;
+        // fixed point holds |c| < 2^36 per splat (non-finite or larger splats are dropped)
+        if (!(fabsf(c.x) < 6.8e10f && fabsf(c.y) < 6.8e10f && fabsf(c.z) < 6.8e10f)) return false;
+        long long* o3 = splat + 3 * ((size_t)sy * pp.width + sx);
+        atomicAdd((unsigned long long*)(o3 + 0), (unsigned long long)llrintf(c.x * kSplatScale));
+        atomicAdd((unsigned long long*)(o3 + 1), (unsigned long long)llrintf(c.y * kSplatScale));
+        atomicAdd((unsigned long long*)(o3 + 2), (unsigned long long)llrintf(c.z * kSplatScale));
+        return false;
+    }
+    out = Lc * w;
+    return true;
+}
+
 // Strategies with a sampled or single vertex: one thread per sample evaluates (0, nc), (1, t) and (s, 1) of
 // RayColor (bdpt.go:83-99 with evaluateBDPTStrategy :294-313), camera-side radiance to pb.L, light-tracing
 // splats to `splat`. k_bdpt_connect_general adds the general connections to pb.L afterwards.
@@ -807,119 +933,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                 s = ds;
                 t = dt;
             }
-            V3 Lc = v3(0, 0, 0);
-            bool splatted = false;
-            int sx = 0, sy = 0;
-            BV sampled, ct1, ls1;
-            TRay vr;  // visibility ray of the strategy (one trace call site keeps the kernel call-free)
-            bool vis = false;
-            ct1 = CP.get(t - 1, i);
-            if (s > 0) ls1 = LP.get(s - 1, i);
-            if (s == 0) {  // evaluatePathTracingStrategy :316-330
-                Lc = mulv(ct1.Le, ct1.beta);
-            } else if (t == 1) {  // evaluateLightTracingStrategy :394-471
-                const BV& lv = ls1;
-                if (lv.flags & VF_SPECULAR) continue;
-                const uint32_t sp0 = (kSplatSlot + (uint32_t)s) * 16u;
-                V3 lc = sample_point_in_unit_disk(draw(key, sp0), draw(key, sp0 + 1)) * cam.lens_radius;
-                V3 lens_p = v3(cam.center) + v3(cam.u) * lc.x + v3(cam.v) * lc.y;
-                V3 dir = normalize(lv.p - lens_p);
-                float dist = length(lv.p - lens_p);
-                float cf = fabsf(dot(v3(cam.forward), dir));
-                float cpdf = (dist * dist) / (cf * cam.lens_area);
-                float ctw = dot(dir, v3(cam.forward));
-                float we = ctw < cam.cos_total_width
-                               ? 0.0f
-                               : 1.0f / (cam.plane_area * cam.lens_area * ctw * ctw * ctw * ctw);
-                if (we == 0.0f) continue;
-                V3 back = dir * -1.0f;
-                V3 f = vertex_brdf(sc, lv, back, 1);
-                V3 cbeta = v3(we, we, we) * (1.0f / cpdf);
-                float cosine = dot(back, lv.n);
-                if (cosine <= 0.0f) continue;
-                V3 contrib = mulv(mulv(f, cbeta), lv.beta);
-                if (on_surface(sc, lv)) contrib = contrib * cosine;
-                if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
-                // the splat lands outside the image: SplatQueue drops it, nothing else uses it
-                if (!cam_map(cam, lens_p, dir, sx, sy)) continue;
-                vr = TRay{lv.p.x, lv.p.y, lv.p.z, 0.001f, back.x, back.y, back.z, dist - 0.001f};
-                vis = true;
-                sampled.p = lens_p;
-                sampled.n = back;
-                sampled.wi = v3(0, 0, 0);
-                sampled.beta = cbeta;
-                sampled.Le = v3(0, 0, 0);
-                sampled.pdf_fwd = cpdf;
-                sampled.pdf_rev = total_light_pdf(sc, lens_p, back, dir);
-                sampled.mat = -1;
-                sampled.light = -1;
-                sampled.flags = VF_CAMERA;
-                Lc = contrib;
-                splatted = true;
-            } else if (s == 1) {  // evaluateDirectLightingStrategy :333-391
-                const BV& cvt = ct1;
-                if ((cvt.flags & VF_SPECULAR) || cvt.mat < 0) continue;
-                const uint32_t dp0 = (kDirectSlot + (uint32_t)t) * 16u;
-                float sel;
-                int li = select_light(sc, draw(key, dp0), sel);
-                LSample ls = light_sample(sc, sc.lights[li], cvt.p, cvt.n, draw(key, dp0 + 1), draw(key, dp0 + 2));
-                ls.pdf *= sel;
-                if ((ls.emission.x == 0.0f && ls.emission.y == 0.0f && ls.emission.z == 0.0f) || ls.pdf <= 0.0f)
-                    continue;
-                float ct = fabsf(dot(ls.dir, cvt.n));
-                if (ct <= 0.0f) continue;
-                V3 f = brdf(sc, cvt.mat, cvt.wi, ls.dir, surf_of(cvt), 0);
-                V3 lbeta = ls.emission * (1.0f / ls.pdf);
-                V3 contrib = mulv(mulv(f, cvt.beta), lbeta) * ct;
-                if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
-                vr = TRay{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - 0.001f};
-                vis = true;
-                sampled.p = ls.p;
-                sampled.n = ls.n;
-                sampled.wi = v3(0, 0, 0);
-                sampled.beta = lbeta;
-                sampled.Le = ls.emission;
-                sampled.pdf_fwd = ls.pdf;
-                sampled.pdf_rev = 0.0f;
-                sampled.mat = -1;
-                sampled.light = li;
-                sampled.flags = light_vertex_flags(sc, li);
-                Lc = contrib;
-            } else {  // evaluateConnectionStrategy :489-561
-                const BV &lv = ls1, &cvt = ct1;
-                if ((lv.flags & VF_SPECULAR) || (cvt.flags & VF_SPECULAR)) continue;
-                V3 dd = lv.p - cvt.p;
-                float dist = length(dd);
-                if (dist < 0.001f) continue;
-                dd = dd * (1.0f / dist);
-                float cc = dot(dd, cvt.n), cl = dot(dd * -1.0f, lv.n);
-                if (cc <= 0.0f || cl <= 0.0f) continue;
-                float G = (cc * cl) / (dist * dist);
-                V3 fc = vertex_brdf(sc, cvt, dd, 0);
-                V3 fl = vertex_brdf(sc, lv, dd * -1.0f, 1);
-                V3 contrib = mulv(mulv(mulv(lv.beta, fl), fc), cvt.beta) * G;
-                if (contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f) continue;
-                vr = TRay{cvt.p.x, cvt.p.y, cvt.p.z, 0.001f, dd.x, dd.y, dd.z, dist - 0.001f};
-                vis = true;
-                Lc = contrib;
-            }
-            if (vis) {
-                THit vh;
-                if (trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr, pp.trace_gate)) continue;
-            }
-            if (Lc.x == 0.0f && Lc.y == 0.0f && Lc.z == 0.0f && !splatted) continue;
-            float w = mis_weight(sc, cam, CP, LP, i, sampled, ct1, ls1, s, t);
-            if (splatted) {
-                V3 c = Lc * w;
-                // fixed point holds |c| < 2^36 per splat (non-finite or larger splats are dropped)
-                if (!(fabsf(c.x) < 6.8e10f && fabsf(c.y) < 6.8e10f && fabsf(c.z) < 6.8e10f)) continue;
-                long long* o3 = splat + 3 * ((size_t)sy * pp.width + sx);
-                atomicAdd((unsigned long long*)(o3 + 0), (unsigned long long)llrintf(c.x * kSplatScale));
-                atomicAdd((unsigned long long*)(o3 + 1), (unsigned long long)llrintf(c.y * kSplatScale));
-                atomicAdd((unsigned long long*)(o3 + 2), (unsigned long long)llrintf(c.z * kSplatScale));
-            } else {
-                total = total + Lc * w;
-            }
+            V3 c;
+            if (eval_strategy<MIXED>(sc, pp, CP, LP, i, key, s, t, splat, st, c)) total = total + c;
         }
         pb.L[i] = make_float4(total.x, total.y, total.z, 0.f);
     }
@@ -1024,85 +1039,139 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     }
 }
 
-// General connections through per-pair queues (the same strategies as k_bdpt_connect_general, for films
-// whose pairs fit kMaxPairs). Pair p = (s - 2) * nt + (t - 2) for s = 2 .. max_depth, t = 2 .. max_depth + 1.
-//   k_bdpt_pairs:        one thread per sample appends the sample to the queue of every pair whose light
-//                        vertex s - 1 and camera vertex t - 1 are both non-specular (the pairs the per-sample
-//                        loop would evaluate);
-//   k_bdpt_pair_connect: one lane per queued (pair, sample): contribution, MIS weight and visibility ray as in
-//                        k_bdpt_connect_general; writes the weighted contribution to res[p][sample] with w = 1
-//                        when it is added (unoccluded, non-zero), w = 0 otherwise;
-//   k_bdpt_pair_sum:     one thread per sample adds its pairs' results to pb.L in the per-sample loop's order
-//                        (s outer, t inner), so the radiance is bit-identical to the per-sample kernel's.
-// All lanes of a wave evaluate one pair, so no lane waits for another lane's longer subpaths, and a pair's
-// queue holds samples in about increasing order, so a wave's vertex reads are coalesced.
-constexpr uint32_t kMaxPairs = 256;
+// BDPT strategies through per-strategy queues (films whose strategies fit kMaxQueues, max_depth <= 16; deeper
+// films run k_bdpt_connect and k_bdpt_connect_general, one thread per sample looping over its strategies).
+// Queues, one per strategy (s, t) of RayColor (bdpt.go:83-99), nt = max_depth:
+//   q in [0, np):            general connection s = q / nt + 2, t = q % nt + 2 (np = (nt - 1) * nt);
+//   q in [np, np + nt):      direct lighting (1, t), t = q - np + 2;
+//   q in [np + nt, nq - 1):  light tracing (s, 1), s = q - np - nt + 2;
+//   q = nq - 1:              the camera path's own emission (0, nc).
+// Result planes (the weighted camera-side radiance of a queued sample, w = 1 when it is added): one per general
+// pair and per direct-lighting strategy, then one for (0, nc); light tracing splats directly.
+//   k_bdpt_pairs:         one thread per sample queues the sample for every strategy the per-sample loops
+//                         evaluate (end vertices not specular, a material for direct lighting, an emitting last
+//                         camera vertex for (0, nc)) and records which in `mask`;
+//   k_bdpt_edge_connect:  one lane per queued sample of a direct-lighting, light-tracing or (0, nc) strategy;
+//   k_bdpt_pair_connect:  one lane per queued sample of a general pair (the contribution, MIS weight and
+//                         visibility ray of k_bdpt_connect_general);
+//   k_bdpt_pair_sum:      one thread per sample adds its results in the per-sample loops' order ((0, nc), then
+//                         (1, t) by t, then the general pairs s-major), so the radiance is that of the per-sample
+//                         kernels up to the contraction of multiply-adds.
+// All lanes of a wave evaluate one strategy, so no lane waits for another lane's longer subpaths, and a queue
+// holds samples in about increasing order, so a wave's vertex reads are coalesced.
+constexpr uint32_t kMaxQueues = 512;
 
 struct PairBuffers {
-    uint32_t* queue;   // [pair][slot]: slots queued for the pair
-    float4* res;       // [pair][slot]: weighted contribution, w = 1 if it is added
-    uint32_t* count;   // queue length of pair p at count[p * kCounterStride]
-    uint32_t nt;       // camera-side pairs per light vertex (max_depth)
-    uint32_t np;       // pairs ((max_depth - 1) * max_depth)
-    uint32_t stride;   // slots per pair plane
+    uint32_t* queue;   // [queue][slot]: slots queued for the strategy
+    float4* res;       // [plane][slot]: weighted contribution, w = 1 if it is added
+    uint2* mask;       // per slot: x = light vertices s - 1 not specular (bit s - 2) | camera vertices t - 1 not
+                       // specular (bit 16 + t - 2); y = direct-lighting strategies (bit t - 2) | (0, nc) (bit 16)
+    uint32_t* count;   // length of queue q at count[q * kCounterStride]
+    uint32_t nt;       // max_depth
+    uint32_t np;       // general pairs
+    uint32_t nq;       // queues: np + 2 * nt
+    uint32_t stride;   // slots per queue and plane
+    __device__ __forceinline__ float4* plane(uint32_t k) const { return res + (size_t)k * stride; }
 };
 
 __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStore CP, VertexStore LP,
                                                        const uint32_t* lens, PairBuffers pq) {
-    // per wave: entries of each pair (then the pair queue position of the wave's first entry)
-    __shared__ uint32_t s_cnt[kBlock / 64][kMaxPairs];
+    // per wave: entries of each queue (then the queue position of the wave's first entry)
+    __shared__ uint32_t s_cnt[kBlock / 64][kMaxQueues];
     const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t lm = 0, cm = 0;  // non-specular light vertices s - 1 (bit s - 2), camera vertices t - 1 (bit t - 2)
+    uint32_t lm = 0, cm = 0, dm = 0, zm = 0;
     if (i < n) {
         const uint32_t len = lens[i];
         const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
-        if (nl >= 2 && nc >= 2) {
-            for (int s = 2; s <= nl; s++)
-                if (!rec_specular(LP.rec(s - 1, i))) lm |= 1u << (s - 2);
-            for (int t = 2; t <= nc; t++)
-                if (!rec_specular(CP.rec(t - 1, i))) cm |= 1u << (t - 2);
+        for (int s = 2; s <= nl; s++)
+            if (!rec_specular(LP.rec(s - 1, i))) lm |= 1u << (s - 2);
+        for (int t = 2; t <= nc; t++) {
+            const uint32_t fm = __float_as_uint(CP.rec(t - 1, i).z);
+            if (fm & VF_SPECULAR) continue;
+            cm |= 1u << (t - 2);
+            if (nl >= 1 && (fm >> 8) != 0u) dm |= 1u << (t - 2);  // a material (mat >= 0)
         }
-        if (!lm) cm = 0;
-        if (!cm) lm = 0;
+        if (nc >= 1 && (__float_as_uint(CP.rec(nc - 1, i).z) & VF_LIGHT)) zm = 1;
+        pq.mask[i] = make_uint2(lm | (cm << 16), dm | (zm << 16));
     }
-    for (uint32_t p = lane; p < pq.np; p += 64) s_cnt[w][p] = 0;
-    // the pairs any lane of the wave has: s < ns, t < nt (wave-uniform bounds)
-    uint32_t ns = 0, nt = 0;
+    for (uint32_t q = lane; q < pq.nq; q += 64) s_cnt[w][q] = 0;
+    // wave-uniform bounds of the strategies any lane has
+    uint32_t ns = 0, nc2 = 0, nd = 0;
     while (__ballot((lm >> ns) != 0u) != 0) ns++;
-    while (__ballot((cm >> nt) != 0u) != 0) nt++;
+    while (__ballot((cm >> nc2) != 0u) != 0) nc2++;
+    while (__ballot((dm >> nd) != 0u) != 0) nd++;
+    const uint32_t qd = pq.np, ql = pq.np + pq.nt, qz = pq.nq - 1;
+    auto count = [&](uint32_t q, bool a) {
+        const uint64_t m = __ballot(a);
+        if (lane == 0) s_cnt[w][q] = (uint32_t)__popcll(m);
+    };
     for (uint32_t s = 0; s < ns; s++)
-        for (uint32_t t = 0; t < nt; t++) {
-            const uint64_t m = __ballot(((lm >> s) & (cm >> t) & 1u) != 0);
-            if (lane == 0) s_cnt[w][s * pq.nt + t] = (uint32_t)__popcll(m);
-        }
-    // one queue reservation per pair and wave, all issued together
-    for (uint32_t p = lane; p < pq.np; p += 64) {
-        const uint32_t c = s_cnt[w][p];
-        if (c) s_cnt[w][p] = atomicAdd(pq.count + p * kCounterStride, c);
+        for (uint32_t t = 0; t < nc2; t++) count(s * pq.nt + t, ((lm >> s) & (cm >> t) & 1u) != 0);
+    for (uint32_t t = 0; t < nd; t++) count(qd + t, ((dm >> t) & 1u) != 0);
+    for (uint32_t s = 0; s < ns; s++) count(ql + s, ((lm >> s) & 1u) != 0);
+    count(qz, zm != 0);
+    // one queue reservation per queue and wave, all issued together
+    for (uint32_t q = lane; q < pq.nq; q += 64) {
+        const uint32_t c = s_cnt[w][q];
+        if (c) s_cnt[w][q] = atomicAdd(pq.count + q * kCounterStride, c);
     }
+    auto push = [&](uint32_t q, bool a) {
+        const uint64_t m = __ballot(a);
+        if (a) pq.queue[(size_t)q * pq.stride + s_cnt[w][q] + (uint32_t)__popcll(m & lt)] = i;
+    };
     for (uint32_t s = 0; s < ns; s++)
-        for (uint32_t t = 0; t < nt; t++) {
-            const bool a = ((lm >> s) & (cm >> t) & 1u) != 0;
-            const uint64_t m = __ballot(a);
-            const uint32_t p = s * pq.nt + t;
-            if (a) pq.queue[(size_t)p * pq.stride + s_cnt[w][p] + (uint32_t)__popcll(m & lt)] = i;
-        }
+        for (uint32_t t = 0; t < nc2; t++) push(s * pq.nt + t, ((lm >> s) & (cm >> t) & 1u) != 0);
+    for (uint32_t t = 0; t < nd; t++) push(qd + t, ((dm >> t) & 1u) != 0);
+    for (uint32_t s = 0; s < ns; s++) push(ql + s, ((lm >> s) & 1u) != 0);
+    push(qz, zm != 0);
+}
+
+// Inclusive prefix sums of the lengths of queues q0 .. q1 - 1 (at most kMaxQueues) into s_end; every thread of
+// the block calls it.
+__device__ __forceinline__ void queue_prefix(uint32_t* s_end, const PairBuffers& pq, uint32_t q0, uint32_t q1) {
+    static_assert(kMaxQueues == 2 * kBlock, "two queues per thread");
+    const uint32_t k = 2 * threadIdx.x;
+    const uint32_t a = q0 + k < q1 ? pq.count[(q0 + k) * kCounterStride] : 0u;
+    const uint32_t b = q0 + k + 1 < q1 ? pq.count[(q0 + k + 1) * kCounterStride] : 0u;
+    s_end[k] = a;
+    s_end[k + 1] = a + b;
+    __syncthreads();
+    for (uint32_t o = 1; o < kBlock; o <<= 1) {  // scan of the pair sums
+        const uint32_t v = threadIdx.x >= o ? s_end[k + 1 - 2 * o] : 0u;
+        __syncthreads();
+        s_end[k + 1] += v;
+        __syncthreads();
+    }
+    if (threadIdx.x > 0) s_end[k] += s_end[k - 1];
+    __syncthreads();
+}
+
+// The queue (relative to q0) whose range of the prefix sums holds j < s_end[nq - 1].
+__device__ __forceinline__ uint32_t queue_of(const uint32_t* s_end, uint32_t nq, uint32_t j) {
+    uint32_t lo = 0, hi = nq - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (j < s_end[mid])
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
 }
 
 #ifndef PRT_PAIR_WAVES
 #define PRT_PAIR_WAVES 4  // waves per SIMD of k_bdpt_pair_connect (4: 127 VGPRs, no spills)
 #endif
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR_WAVES))) void k_bdpt_pair_connect(DevScene sc, PassParams pp, VertexStore CP,
-                                                              VertexStore LP, PairBuffers pq, uint2* gstack,
-                                                              uint32_t gstride) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR_WAVES))) void k_bdpt_pair_connect(
+    DevScene sc, PassParams pp, VertexStore CP, VertexStore LP, PairBuffers pq, uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
-    __shared__ uint32_t s_end[kMaxPairs];  // inclusive prefix sums of the queue lengths
+    __shared__ uint32_t s_end[kMaxQueues];
     TStack<kLds, kBlock> st;
     st.s_node = s_node;
     st.s_t = s_t;
@@ -1112,28 +1181,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR
     st.gstride = gstride;
     load_node_cache<kNodeCache, kBlock>(sc, s_cache);
     st.ncache = s_cache;
-    static_assert(kMaxPairs == kBlock, "one thread per pair in the prefix sum");
-    s_end[threadIdx.x] = threadIdx.x < pq.np ? pq.count[threadIdx.x * kCounterStride] : 0u;
-    __syncthreads();
-    for (uint32_t o = 1; o < kMaxPairs; o <<= 1) {
-        const uint32_t v = threadIdx.x >= o ? s_end[threadIdx.x - o] : 0u;
-        __syncthreads();
-        s_end[threadIdx.x] += v;
-        __syncthreads();
-    }
+    queue_prefix(s_end, pq, 0, pq.np);
     const DevCamera& cam = sc.cam;
-    const uint32_t total = s_end[kMaxPairs - 1];
+    const uint32_t total = s_end[pq.np - 1];
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += stride) {
-        uint32_t lo = 0, hi = pq.np - 1;  // the pair whose range holds j
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (j < s_end[mid])
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
-        const uint32_t p = lo;
+        const uint32_t p = queue_of(s_end, pq.np, j);
         const uint32_t i = pq.queue[(size_t)p * pq.stride + (j - (p ? s_end[p - 1] : 0u))];
         const int s = (int)(p / pq.nt) + 2, t = (int)(p % pq.nt) + 2;
         const BV ls1 = LP.get(s - 1, i), ct1 = CP.get(t - 1, i);
@@ -1160,33 +1213,83 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR
                 }
             }
         }
-        pq.res[(size_t)p * pq.stride + i] = r;
+        pq.plane(p)[i] = r;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bdpt_pair_sum(PassParams pp, PathBuffers pb, VertexStore CP,
-                                                          VertexStore LP, const uint32_t* lens, PairBuffers pq) {
+// Direct-lighting, light-tracing and (0, nc) strategies, one lane per queued sample.
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_bdpt_edge_connect(DevScene sc, PassParams pp, VertexStore CP,
+                                                              VertexStore LP, const uint32_t* lens, long long* splat,
+                                                              PairBuffers pq, uint2* gstack, uint32_t gstride) {
+    __shared__ uint32_t s_node[kLds * kBlock];
+    __shared__ float s_t[kLds * kBlock];
+    __shared__ float4 s_cache[kNodeCache * 8];
+    __shared__ uint32_t s_end[kMaxQueues];
+    TStack<kLds, kBlock> st;
+    st.s_node = s_node;
+    st.s_t = s_t;
+    st.g = gstack;
+    st.lane = threadIdx.x;
+    st.gslot = blockIdx.x * kBlock + threadIdx.x;
+    st.gstride = gstride;
+    load_node_cache<kNodeCache, kBlock>(sc, s_cache);
+    st.ncache = s_cache;
+    const uint32_t ne = pq.nq - pq.np;
+    queue_prefix(s_end, pq, pq.np, pq.nq);
+    const uint32_t total = s_end[ne - 1];
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += stride) {
+        const uint32_t e = queue_of(s_end, ne, j);
+        const uint32_t i = pq.queue[(size_t)(pq.np + e) * pq.stride + (j - (e ? s_end[e - 1] : 0u))];
+        int s, t;
+        uint32_t plane = 0xFFFFFFFFu;  // light tracing: no result plane
+        if (e < pq.nt) {
+            s = 1;
+            t = (int)e + 2;
+            plane = pq.np + e;
+        } else if (e < ne - 1) {
+            s = (int)(e - pq.nt) + 2;
+            t = 1;
+        } else {
+            s = 0;
+            t = (int)(lens[i] & 0xFFFFu);
+            plane = pq.np + pq.nt;
+        }
+        uint32_t pix, sample;
+        slot_sample(pp, i, pix, sample);
+        const uint32_t key = path_key(pp.seed, pix, sample);
+        V3 c;
+        const bool add = eval_strategy<MIXED>(sc, pp, CP, LP, i, key, s, t, splat, st, c);
+        if (plane != 0xFFFFFFFFu) pq.plane(plane)[i] = add ? make_float4(c.x, c.y, c.z, 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bdpt_pair_sum(PassParams pp, PathBuffers pb, PairBuffers pq) {
     const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const uint32_t len = lens[i];
-    const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
-    if (nl < 2 || nc < 2) return;
-    uint32_t cm = 0;
-    for (int t = 2; t <= nc; t++)
-        if (!rec_specular(CP.rec(t - 1, i))) cm |= 1u << (t - 2);
-    if (!cm) return;
-    const float4 l4 = pb.L[i];
-    V3 total = v3(l4.x, l4.y, l4.z);
-    for (int s = 2; s <= nl; s++) {
-        if (rec_specular(LP.rec(s - 1, i))) continue;
-        const float4* row = pq.res + (size_t)(s - 2) * pq.nt * pq.stride + i;
-        for (int t = 2; t <= nc; t++) {
-            if (!((cm >> (t - 2)) & 1u)) continue;
-            const float4 r = row[(size_t)(t - 2) * pq.stride];
-            if (r.w != 0.0f) total = total + v3(r.x, r.y, r.z);
-        }
+    const uint2 m = pq.mask[i];
+    const uint32_t lm = m.x & 0xFFFFu, cm = m.x >> 16, dm = m.y & 0xFFFFu;
+    V3 total = v3(0, 0, 0);
+    if (m.y >> 16) {
+        const float4 r = pq.plane(pq.np + pq.nt)[i];
+        if (r.w != 0.0f) total = total + v3(r.x, r.y, r.z);
     }
+    for (uint32_t t = 0; (dm >> t) != 0u; t++) {
+        if (!((dm >> t) & 1u)) continue;
+        const float4 r = pq.plane(pq.np + t)[i];
+        if (r.w != 0.0f) total = total + v3(r.x, r.y, r.z);
+    }
+    if (cm)
+        for (uint32_t s = 0; (lm >> s) != 0u; s++) {
+            if (!((lm >> s) & 1u)) continue;
+            for (uint32_t t = 0; (cm >> t) != 0u; t++) {
+                if (!((cm >> t) & 1u)) continue;
+                const float4 r = pq.plane(s * pq.nt + t)[i];
+                if (r.w != 0.0f) total = total + v3(r.x, r.y, r.z);
+            }
+        }
     pb.L[i] = make_float4(total.x, total.y, total.z, 0.f);
 }
 
@@ -1379,8 +1482,9 @@ struct prt_film {
     bool bdpt = false;
     int bdpt_blocks = 0;
     int bdpt_general_blocks = 0;  // grid of k_bdpt_connect_general (its own occupancy)
-    bool bdpt_pairs = false;      // general connections through per-pair queues
+    bool bdpt_pairs = false;      // strategies through per-strategy queues
     int bdpt_pair_blocks = 0;     // grid of k_bdpt_pair_connect
+    int bdpt_edge_blocks = 0;     // grid of k_bdpt_edge_connect
     PairBuffers pairs{};
     VertexStore cpath{}, lpath{};
     uint32_t* lens = nullptr;
@@ -1755,21 +1859,28 @@ size_t bdpt_store_bytes(int max_depth, int64_t slots, bool uv) {
     return (size_t)(max_depth + 1) * (uv ? 6 : 5) * sizeof(float4) * (size_t)slots;
 }
 
-// Whether a BDPT film of this depth evaluates its general connections through per-pair queues (k_bdpt_pairs,
-// k_bdpt_pair_connect, k_bdpt_pair_sum): when its (max_depth - 1) * max_depth pairs fit kMaxPairs
-// (max_depth <= 16); deeper films loop over the pairs per sample (k_bdpt_connect_general).
-// PRT_BDPT_PAIR_QUEUES=0 selects the per-sample kernel for every depth.
+// Whether a BDPT film of this depth evaluates its strategies through per-strategy queues (k_bdpt_pairs,
+// k_bdpt_edge_connect, k_bdpt_pair_connect, k_bdpt_pair_sum): when its (max_depth + 2) * max_depth queues fit
+// kMaxQueues and its vertex masks 16 bits (max_depth <= 16); deeper films loop over the strategies per sample
+// (k_bdpt_connect, k_bdpt_connect_general). PRT_BDPT_PAIR_QUEUES=0 selects the per-sample kernels at every
+// depth.
 bool bdpt_pairs_used(int max_depth) {
     if (const char* e = std::getenv("PRT_BDPT_PAIR_QUEUES"))
         if (std::atoi(e) == 0) return false;
-    return max_depth >= 2 && (int64_t)(max_depth - 1) * max_depth <= (int64_t)kMaxPairs;
+    return max_depth >= 2 && max_depth <= 16;
 }
 
-// Bytes of the pair queues and results of `slots` path slots: a slot id and a float4 result per pair.
+// Bytes per path slot of the strategy queues, result planes and masks.
+size_t bdpt_pair_slot_bytes(int max_depth) {
+    if (!bdpt_pairs_used(max_depth)) return 0;
+    const size_t nt = (size_t)max_depth, np = (nt - 1) * nt;
+    return (np + 2 * nt) * sizeof(uint32_t) + (np + nt + 1) * sizeof(float4) + sizeof(uint2);
+}
+
+// Bytes of the strategy queues, result planes and masks of `slots` path slots, and the queue lengths.
 size_t bdpt_pair_bytes(int max_depth, int64_t slots) {
     if (!bdpt_pairs_used(max_depth)) return 0;
-    return (size_t)(max_depth - 1) * (size_t)max_depth * (sizeof(uint32_t) + sizeof(float4)) * (size_t)slots +
-           kMaxPairs * kCounterStride * sizeof(uint32_t);
+    return bdpt_pair_slot_bytes(max_depth) * (size_t)slots + kMaxQueues * kCounterStride * sizeof(uint32_t);
 }
 
 // Path slots of a BDPT film: the two subpath stores stay within 2 GiB, and with the pair queues within 4 GiB.
@@ -1777,18 +1888,20 @@ int64_t bdpt_capacity(int max_depth, int64_t cap, bool uv) {
     const int64_t per_slot = (int64_t)(2 * bdpt_store_bytes(max_depth, 1, uv));
     int64_t c = ((int64_t)1 << 31) / per_slot;
     if (bdpt_pairs_used(max_depth))
-        c = std::min<int64_t>(c, ((int64_t)1 << 32) / (per_slot + (int64_t)(max_depth - 1) * max_depth * 20));
+        c = std::min<int64_t>(c, ((int64_t)1 << 32) / (per_slot + (int64_t)bdpt_pair_slot_bytes(max_depth)));
+    if (const char* e = std::getenv("PRT_BDPT_SLOTS")) {  // pins the path slots (chunk size) of BDPT films
+        const long long v = std::atoll(e);
+        if (v > 0) c = std::min<int64_t>(c, (int64_t)v);
+    }
     c = std::max<int64_t>(c, (int64_t)1 << 16) & ~(int64_t)255;
     return std::min(cap, c);
 }
 
-// Grid of k_bdpt_pair_connect: its resident blocks twice, at most the trace grid (stack slots).
-int bdpt_pair_blocks(const prt_scene* scene, int trace_blocks) {
+// Grids of k_bdpt_pair_connect and k_bdpt_edge_connect: their resident blocks twice, at most the trace grid
+// (stack slots).
+int bdpt_queue_blocks(const prt_scene* scene, int trace_blocks, const void* fn) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_bdpt_pair_connect<true>, kBlock, 0) !=
-            hipSuccess ||
-        per_cu <= 0)
-        per_cu = 2;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess || per_cu <= 0) per_cu = 2;
     return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu * 2));
 }
 
@@ -2155,35 +2268,35 @@ int film_enqueue_chunk(prt_film* f) {
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, true));
         if (f->bdpt_pairs) {
-            // pair queues: filled after the subpaths, drained by the pair kernel; the per-sample strategies run
-            // first, the pair results are added to their radiance last
-            PRT_HIP(hipMemsetAsync(f->pairs.count, 0, kMaxPairs * kCounterStride * sizeof(uint32_t), st));
+            // strategy queues: filled after the subpaths, drained by the strategy kernels, results added per sample
+            const int grid_p = (int)std::min<int64_t>(f->bdpt_pair_blocks, (n + kBlock - 1) / kBlock);
+            const int grid_e = (int)std::min<int64_t>(f->bdpt_edge_blocks, (n + kBlock - 1) / kBlock);
+            PRT_HIP(hipMemsetAsync(f->pairs.count, 0, kMaxQueues * kCounterStride * sizeof(uint32_t), st));
             hipLaunchKernelGGL(k_bdpt_pairs, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens,
                                f->pairs);
-        }
-        const int grid_p = (int)std::min<int64_t>(f->bdpt_pair_blocks, (n + kBlock - 1) / kBlock);
-        if (sc.mixed) {
-            hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
-                               f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
-            if (f->bdpt_pairs)
+            if (sc.mixed) {
+                hipLaunchKernelGGL((k_bdpt_edge_connect<true>), dim3(grid_e), dim3(kBlock), 0, st, sc, pp, f->cpath,
+                                   f->lpath, f->lens, f->fb.splat, f->pairs, f->gstack, f->gslots);
                 hipLaunchKernelGGL((k_bdpt_pair_connect<true>), dim3(grid_p), dim3(kBlock), 0, st, sc, pp, f->cpath,
                                    f->lpath, f->pairs, f->gstack, f->gslots);
-            else
-                hipLaunchKernelGGL((k_bdpt_connect_general<true>), dim3(grid_g), dim3(kBlock), 0, st, sc,
-                                   pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
+            } else {
+                hipLaunchKernelGGL((k_bdpt_edge_connect<false>), dim3(grid_e), dim3(kBlock), 0, st, sc, pp, f->cpath,
+                                   f->lpath, f->lens, f->fb.splat, f->pairs, f->gstack, f->gslots);
+                hipLaunchKernelGGL((k_bdpt_pair_connect<false>), dim3(grid_p), dim3(kBlock), 0, st, sc, pp, f->cpath,
+                                   f->lpath, f->pairs, f->gstack, f->gslots);
+            }
+            hipLaunchKernelGGL(k_bdpt_pair_sum, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->pb, f->pairs);
+        } else if (sc.mixed) {
+            hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
+                               f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
+            hipLaunchKernelGGL((k_bdpt_connect_general<true>), dim3(grid_g), dim3(kBlock), 0, st, sc,
+                               pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
         } else {
             hipLaunchKernelGGL((k_bdpt_connect<false>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
                                f->lpath, f->lens, f->fb.splat, f->gstack, f->gslots);
-            if (f->bdpt_pairs)
-                hipLaunchKernelGGL((k_bdpt_pair_connect<false>), dim3(grid_p), dim3(kBlock), 0, st, sc, pp, f->cpath,
-                                   f->lpath, f->pairs, f->gstack, f->gslots);
-            else
-                hipLaunchKernelGGL((k_bdpt_connect_general<false>), dim3(grid_g), dim3(kBlock), 0, st,
-                                   sc, pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
+            hipLaunchKernelGGL((k_bdpt_connect_general<false>), dim3(grid_g), dim3(kBlock), 0, st,
+                               sc, pp, f->pb, f->cpath, f->lpath, f->lens, f->gstack, f->gslots);
         }
-        if (f->bdpt_pairs)
-            hipLaunchKernelGGL(k_bdpt_pair_sum, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->pb, f->cpath, f->lpath,
-                               f->lens, f->pairs);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_FILM, true));
         if (f->adaptive)
@@ -2496,16 +2609,21 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
         f->fb.splat = (long long*)ptr;
         if (bdpt_pairs_used(p->max_depth)) {
             f->bdpt_pairs = true;
-            f->bdpt_pair_blocks = bdpt_pair_blocks(scene, f->blocks);
+            f->bdpt_pair_blocks = bdpt_queue_blocks(scene, f->blocks, (const void*)k_bdpt_pair_connect<true>);
+            f->bdpt_edge_blocks = bdpt_queue_blocks(scene, f->blocks, (const void*)k_bdpt_edge_connect<true>);
             PairBuffers& pq = f->pairs;
             pq.nt = (uint32_t)p->max_depth;
             pq.np = (uint32_t)(p->max_depth - 1) * pq.nt;
+            pq.nq = pq.np + 2 * pq.nt;
             pq.stride = (uint32_t)C;
-            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)pq.np * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
+            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)pq.nq * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
             pq.queue = (uint32_t*)ptr;
-            if ((rc = film_alloc(f, sizeof(float4) * (size_t)pq.np * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
+            if ((rc = film_alloc(f, sizeof(float4) * (size_t)(pq.np + pq.nt + 1) * (size_t)C, &ptr)) != PRT_OK)
+                return bail(rc);
             pq.res = (float4*)ptr;
-            if ((rc = film_alloc(f, kMaxPairs * kCounterStride * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
+            if ((rc = film_alloc(f, sizeof(uint2) * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
+            pq.mask = (uint2*)ptr;
+            if ((rc = film_alloc(f, kMaxQueues * kCounterStride * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
             pq.count = (uint32_t*)ptr;
         }
     }
