@@ -1039,45 +1039,74 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     }
 }
 
-// BDPT strategies through per-strategy queues (films whose strategies fit kMaxQueues, max_depth <= 16; deeper
-// films run k_bdpt_connect and k_bdpt_connect_general, one thread per sample looping over its strategies).
-// Queues, one per strategy (s, t) of RayColor (bdpt.go:83-99), nt = max_depth:
+// BDPT strategies through per-strategy work lists (films whose strategies fit kMaxQueues, max_depth <= 16;
+// deeper films run k_bdpt_connect and k_bdpt_connect_general, one thread per sample looping over its
+// strategies). Strategy q of RayColor (bdpt.go:83-99), nt = max_depth:
 //   q in [0, np):            general connection s = q / nt + 2, t = q % nt + 2 (np = (nt - 1) * nt);
 //   q in [np, np + nt):      direct lighting (1, t), t = q - np + 2;
 //   q in [np + nt, nq - 1):  light tracing (s, 1), s = q - np - nt + 2;
 //   q = nq - 1:              the camera path's own emission (0, nc).
 // Result planes (the weighted camera-side radiance of a queued sample, w = 1 when it is added): one per general
 // pair and per direct-lighting strategy, then one for (0, nc); light tracing splats directly.
-//   k_bdpt_pairs:         one thread per sample queues the sample for every strategy the per-sample loops
+//   k_bdpt_pairs:         each block of samples lists (sample, q) for every strategy the per-sample loops
 //                         evaluate (end vertices not specular, a material for direct lighting, an emitting last
-//                         camera vertex for (0, nc)) and records which in `mask`;
-//   k_bdpt_edge_connect:  one lane per queued sample of a direct-lighting, light-tracing or (0, nc) strategy;
-//   k_bdpt_pair_connect:  one lane per queued sample of a general pair (the contribution, MIS weight and
-//                         visibility ray of k_bdpt_connect_general);
+//                         camera vertex for (0, nc)) in one contiguous segment per list, ordered by q and then
+//                         sample, and records which strategies in `mask`;
+//   k_bdpt_edge_connect:  one lane per listed direct-lighting, light-tracing or (0, nc) strategy;
+//   k_bdpt_pair_connect:  one lane per listed general pair (the contribution, MIS weight and visibility ray of
+//                         k_bdpt_connect_general);
 //   k_bdpt_pair_sum:      one thread per sample adds its results in the per-sample loops' order ((0, nc), then
 //                         (1, t) by t, then the general pairs s-major), so the radiance is that of the per-sample
 //                         kernels up to the contraction of multiply-adds.
-// All lanes of a wave evaluate one strategy, so no lane waits for another lane's longer subpaths, and a queue
-// holds samples in about increasing order, so a wave's vertex reads are coalesced.
+// Lanes of a wave evaluate one strategy for consecutive samples (no lane waits for another lane's longer
+// subpaths; coalesced vertex reads), and the lists are consumed in ranges that are contiguous per XCD, so the
+// strategies of a block of samples run together on one XCD while its subpath vertices are in that XCD's L2.
 constexpr uint32_t kMaxQueues = 512;
+constexpr uint32_t kItemSlotBits = 23;  // list item: slot | q << 23 (slots per chunk < 2^23)
+constexpr uint32_t kNumXcd = 8;
 
 struct PairBuffers {
-    uint32_t* queue;   // [queue][slot]: slots queued for the strategy
+    uint32_t* items_p; // general pairs: slot | q << kItemSlotBits
+    uint32_t* items_e; // other strategies: slot | (q - np) << kItemSlotBits
     float4* res;       // [plane][slot]: weighted contribution, w = 1 if it is added
     uint2* mask;       // per slot: x = light vertices s - 1 not specular (bit s - 2) | camera vertices t - 1 not
                        // specular (bit 16 + t - 2); y = direct-lighting strategies (bit t - 2) | (0, nc) (bit 16)
-    uint32_t* count;   // length of queue q at count[q * kCounterStride]
+    uint32_t* count;   // list lengths: general pairs at count[0], others at count[kCounterStride]
     uint32_t nt;       // max_depth
     uint32_t np;       // general pairs
-    uint32_t nq;       // queues: np + 2 * nt
-    uint32_t stride;   // slots per queue and plane
+    uint32_t nq;       // strategies: np + 2 * nt
+    uint32_t stride;   // slots per result plane
     __device__ __forceinline__ float4* plane(uint32_t k) const { return res + (size_t)k * stride; }
 };
 
+// Exclusive prefix sums of v[0 .. kMaxQueues) in place (two entries per thread); returns the total. Every
+// thread of the block calls it.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t* v, uint32_t* s_tmp) {
+    static_assert(kMaxQueues == 2 * kBlock, "two entries per thread");
+    const uint32_t k = 2 * threadIdx.x;
+    const uint32_t a = v[k], b = v[k + 1];
+    s_tmp[threadIdx.x] = a + b;
+    __syncthreads();
+    for (uint32_t o = 1; o < kBlock; o <<= 1) {
+        const uint32_t x = threadIdx.x >= o ? s_tmp[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s_tmp[threadIdx.x] += x;
+        __syncthreads();
+    }
+    const uint32_t before = s_tmp[threadIdx.x] - (a + b);
+    const uint32_t total = s_tmp[kBlock - 1];
+    v[k] = before;
+    v[k + 1] = before + a;
+    __syncthreads();
+    return total;
+}
+
 __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStore CP, VertexStore LP,
                                                        const uint32_t* lens, PairBuffers pq) {
-    // per wave: entries of each queue (then the queue position of the wave's first entry)
-    __shared__ uint32_t s_cnt[kBlock / 64][kMaxQueues];
+    __shared__ uint32_t s_cnt[kBlock / 64][kMaxQueues];  // per wave and strategy: entries, then offsets
+    __shared__ uint32_t s_off[kMaxQueues];               // the block's entries per strategy, then offsets
+    __shared__ uint32_t s_tmp[kBlock];
+    __shared__ uint32_t s_base[2];
     const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -1097,7 +1126,11 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStor
         if (nc >= 1 && (__float_as_uint(CP.rec(nc - 1, i).z) & VF_LIGHT)) zm = 1;
         pq.mask[i] = make_uint2(lm | (cm << 16), dm | (zm << 16));
     }
-    for (uint32_t q = lane; q < pq.nq; q += 64) s_cnt[w][q] = 0;
+    for (uint32_t q = threadIdx.x; q < kMaxQueues; q += kBlock) {
+        s_off[q] = 0;
+        for (uint32_t v = 0; v < kBlock / 64; v++) s_cnt[v][q] = 0;
+    }
+    __syncthreads();
     // wave-uniform bounds of the strategies any lane has
     uint32_t ns = 0, nc2 = 0, nd = 0;
     while (__ballot((lm >> ns) != 0u) != 0) ns++;
@@ -1113,14 +1146,34 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStor
     for (uint32_t t = 0; t < nd; t++) count(qd + t, ((dm >> t) & 1u) != 0);
     for (uint32_t s = 0; s < ns; s++) count(ql + s, ((lm >> s) & 1u) != 0);
     count(qz, zm != 0);
-    // one queue reservation per queue and wave, all issued together
-    for (uint32_t q = lane; q < pq.nq; q += 64) {
-        const uint32_t c = s_cnt[w][q];
-        if (c) s_cnt[w][q] = atomicAdd(pq.count + q * kCounterStride, c);
+    __syncthreads();
+    // per strategy: the waves' offsets within the block's entries, and the block's count
+    for (uint32_t q = threadIdx.x; q < pq.nq; q += kBlock) {
+        uint32_t a = 0;
+        for (uint32_t v = 0; v < kBlock / 64; v++) {
+            const uint32_t c = s_cnt[v][q];
+            s_cnt[v][q] = a;
+            a += c;
+        }
+        s_off[q] = a;
     }
+    __syncthreads();
+    // the block's segment of each list: strategies in order, general pairs first
+    const uint32_t all = block_exclusive_scan(s_off, s_tmp);
+    if (threadIdx.x == 0) {
+        const uint32_t tp = s_off[pq.np];
+        s_base[0] = tp ? atomicAdd(pq.count, tp) : 0u;
+        s_base[1] = all > tp ? atomicAdd(pq.count + kCounterStride, all - tp) - tp : 0u;
+    }
+    __syncthreads();
     auto push = [&](uint32_t q, bool a) {
         const uint64_t m = __ballot(a);
-        if (a) pq.queue[(size_t)q * pq.stride + s_cnt[w][q] + (uint32_t)__popcll(m & lt)] = i;
+        if (!a) return;
+        const uint32_t pos = s_off[q] + s_cnt[w][q] + (uint32_t)__popcll(m & lt);
+        if (q < pq.np)
+            pq.items_p[s_base[0] + pos] = i | (q << kItemSlotBits);
+        else
+            pq.items_e[s_base[1] + pos] = i | ((q - pq.np) << kItemSlotBits);
     };
     for (uint32_t s = 0; s < ns; s++)
         for (uint32_t t = 0; t < nc2; t++) push(s * pq.nt + t, ((lm >> s) & (cm >> t) & 1u) != 0);
@@ -1129,38 +1182,19 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStor
     push(qz, zm != 0);
 }
 
-// Inclusive prefix sums of the lengths of queues q0 .. q1 - 1 (at most kMaxQueues) into s_end; every thread of
-// the block calls it.
-__device__ __forceinline__ void queue_prefix(uint32_t* s_end, const PairBuffers& pq, uint32_t q0, uint32_t q1) {
-    static_assert(kMaxQueues == 2 * kBlock, "two queues per thread");
-    const uint32_t k = 2 * threadIdx.x;
-    const uint32_t a = q0 + k < q1 ? pq.count[(q0 + k) * kCounterStride] : 0u;
-    const uint32_t b = q0 + k + 1 < q1 ? pq.count[(q0 + k + 1) * kCounterStride] : 0u;
-    s_end[k] = a;
-    s_end[k + 1] = a + b;
-    __syncthreads();
-    for (uint32_t o = 1; o < kBlock; o <<= 1) {  // scan of the pair sums
-        const uint32_t v = threadIdx.x >= o ? s_end[k + 1 - 2 * o] : 0u;
-        __syncthreads();
-        s_end[k + 1] += v;
-        __syncthreads();
+// The items of a list a block takes: the list is cut into kNumXcd contiguous ranges, one per XCD (work-groups
+// are dealt to the XCDs in turn, so block b runs on XCD b % kNumXcd), and the blocks of an XCD stride through
+// its range. The grid is a multiple of kNumXcd.
+struct XcdRange {
+    uint32_t j, end, step;
+    __device__ __forceinline__ XcdRange(uint32_t total) {
+        const uint32_t x = blockIdx.x % kNumXcd, y = blockIdx.x / kNumXcd, nb = gridDim.x / kNumXcd;
+        const uint32_t per = (total + kNumXcd - 1) / kNumXcd;
+        j = x * per + y * kBlock + threadIdx.x;
+        end = min(total, (x + 1) * per);
+        step = nb * kBlock;
     }
-    if (threadIdx.x > 0) s_end[k] += s_end[k - 1];
-    __syncthreads();
-}
-
-// The queue (relative to q0) whose range of the prefix sums holds j < s_end[nq - 1].
-__device__ __forceinline__ uint32_t queue_of(const uint32_t* s_end, uint32_t nq, uint32_t j) {
-    uint32_t lo = 0, hi = nq - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (j < s_end[mid])
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return lo;
-}
+};
 
 #ifndef PRT_PAIR_WAVES
 #define PRT_PAIR_WAVES 4  // waves per SIMD of k_bdpt_pair_connect (4: 127 VGPRs, no spills)
@@ -1171,7 +1205,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
-    __shared__ uint32_t s_end[kMaxQueues];
     TStack<kLds, kBlock> st;
     st.s_node = s_node;
     st.s_t = s_t;
@@ -1181,16 +1214,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR
     st.gstride = gstride;
     load_node_cache<kNodeCache, kBlock>(sc, s_cache);
     st.ncache = s_cache;
-    queue_prefix(s_end, pq, 0, pq.np);
     const DevCamera& cam = sc.cam;
-    const uint32_t total = s_end[pq.np - 1];
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += stride) {
-        const uint32_t p = queue_of(s_end, pq.np, j);
-        const uint32_t i = pq.queue[(size_t)p * pq.stride + (j - (p ? s_end[p - 1] : 0u))];
+    for (XcdRange r(*(volatile uint32_t*)pq.count); r.j < r.end; r.j += r.step) {
+        const uint32_t item = pq.items_p[r.j];
+        const uint32_t i = item & ((1u << kItemSlotBits) - 1u), p = item >> kItemSlotBits;
         const int s = (int)(p / pq.nt) + 2, t = (int)(p % pq.nt) + 2;
         const BV ls1 = LP.get(s - 1, i), ct1 = CP.get(t - 1, i);
-        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
         V3 dd = ls1.p - ct1.p;
         const float dist = length(dd);
         if (dist >= 0.001f) {
@@ -1209,11 +1239,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR
                     const TRay vr{ct1.p.x, ct1.p.y, ct1.p.z, 0.001f, dd.x, dd.y, dd.z, dist - 0.001f};
                     THit vh;
                     if (!trace<true, MIXED, false, kLds, kBlock, kNodeCache>(sc, vr, vh, st, nullptr, pp.trace_gate))
-                        r = make_float4(lw.x, lw.y, lw.z, 1.0f);
+                        res = make_float4(lw.x, lw.y, lw.z, 1.0f);
                 }
             }
         }
-        pq.plane(p)[i] = r;
+        pq.plane(p)[i] = res;
     }
 }
 
@@ -1225,7 +1255,6 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_edge_connect(DevScene sc, PassP
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
-    __shared__ uint32_t s_end[kMaxQueues];
     TStack<kLds, kBlock> st;
     st.s_node = s_node;
     st.s_t = s_t;
@@ -1236,12 +1265,9 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_edge_connect(DevScene sc, PassP
     load_node_cache<kNodeCache, kBlock>(sc, s_cache);
     st.ncache = s_cache;
     const uint32_t ne = pq.nq - pq.np;
-    queue_prefix(s_end, pq, pq.np, pq.nq);
-    const uint32_t total = s_end[ne - 1];
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < total; j += stride) {
-        const uint32_t e = queue_of(s_end, ne, j);
-        const uint32_t i = pq.queue[(size_t)(pq.np + e) * pq.stride + (j - (e ? s_end[e - 1] : 0u))];
+    for (XcdRange r(*(volatile uint32_t*)(pq.count + kCounterStride)); r.j < r.end; r.j += r.step) {
+        const uint32_t item = pq.items_e[r.j];
+        const uint32_t i = item & ((1u << kItemSlotBits) - 1u), e = item >> kItemSlotBits;
         int s, t;
         uint32_t plane = 0xFFFFFFFFu;  // light tracing: no result plane
         if (e < pq.nt) {
@@ -1870,25 +1896,27 @@ bool bdpt_pairs_used(int max_depth) {
     return max_depth >= 2 && max_depth <= 16;
 }
 
-// Bytes per path slot of the strategy queues, result planes and masks.
+// Bytes per path slot of the strategy lists, result planes and masks.
 size_t bdpt_pair_slot_bytes(int max_depth) {
     if (!bdpt_pairs_used(max_depth)) return 0;
     const size_t nt = (size_t)max_depth, np = (nt - 1) * nt;
     return (np + 2 * nt) * sizeof(uint32_t) + (np + nt + 1) * sizeof(float4) + sizeof(uint2);
 }
 
-// Bytes of the strategy queues, result planes and masks of `slots` path slots, and the queue lengths.
+// Bytes of the strategy lists, result planes and masks of `slots` path slots, and the list lengths.
 size_t bdpt_pair_bytes(int max_depth, int64_t slots) {
     if (!bdpt_pairs_used(max_depth)) return 0;
-    return bdpt_pair_slot_bytes(max_depth) * (size_t)slots + kMaxQueues * kCounterStride * sizeof(uint32_t);
+    return bdpt_pair_slot_bytes(max_depth) * (size_t)slots + 2 * kCounterStride * sizeof(uint32_t);
 }
 
 // Path slots of a BDPT film: the two subpath stores stay within 2 GiB, and with the pair queues within 4 GiB.
 int64_t bdpt_capacity(int max_depth, int64_t cap, bool uv) {
     const int64_t per_slot = (int64_t)(2 * bdpt_store_bytes(max_depth, 1, uv));
     int64_t c = ((int64_t)1 << 31) / per_slot;
-    if (bdpt_pairs_used(max_depth))
+    if (bdpt_pairs_used(max_depth)) {
         c = std::min<int64_t>(c, ((int64_t)1 << 32) / (per_slot + (int64_t)bdpt_pair_slot_bytes(max_depth)));
+        c = std::min<int64_t>(c, (int64_t)1 << kItemSlotBits);  // slot ids of the strategy lists
+    }
     if (const char* e = std::getenv("PRT_BDPT_SLOTS")) {  // pins the path slots (chunk size) of BDPT films
         const long long v = std::atoll(e);
         if (v > 0) c = std::min<int64_t>(c, (int64_t)v);
@@ -1898,11 +1926,12 @@ int64_t bdpt_capacity(int max_depth, int64_t cap, bool uv) {
 }
 
 // Grids of k_bdpt_pair_connect and k_bdpt_edge_connect: their resident blocks twice, at most the trace grid
-// (stack slots).
+// (stack slots), a multiple of kNumXcd.
 int bdpt_queue_blocks(const prt_scene* scene, int trace_blocks, const void* fn) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess || per_cu <= 0) per_cu = 2;
-    return std::max(1, std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu * 2));
+    const int g = std::min(trace_blocks, std::max(1, scene->num_cus) * per_cu * 2);
+    return std::max((int)kNumXcd, g / (int)kNumXcd * (int)kNumXcd);
 }
 
 }  // namespace
@@ -2269,9 +2298,11 @@ int film_enqueue_chunk(prt_film* f) {
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, true));
         if (f->bdpt_pairs) {
             // strategy queues: filled after the subpaths, drained by the strategy kernels, results added per sample
-            const int grid_p = (int)std::min<int64_t>(f->bdpt_pair_blocks, (n + kBlock - 1) / kBlock);
-            const int grid_e = (int)std::min<int64_t>(f->bdpt_edge_blocks, (n + kBlock - 1) / kBlock);
-            PRT_HIP(hipMemsetAsync(f->pairs.count, 0, kMaxQueues * kCounterStride * sizeof(uint32_t), st));
+            // the strategy kernels' grids: a multiple of kNumXcd, at most about one item per lane and sample
+            const int64_t cap_blocks = std::max<int64_t>(kNumXcd, ((n + kBlock - 1) / kBlock) / kNumXcd * kNumXcd);
+            const int grid_p = (int)std::min<int64_t>(f->bdpt_pair_blocks, cap_blocks);
+            const int grid_e = (int)std::min<int64_t>(f->bdpt_edge_blocks, cap_blocks);
+            PRT_HIP(hipMemsetAsync(f->pairs.count, 0, 2 * kCounterStride * sizeof(uint32_t), st));
             hipLaunchKernelGGL(k_bdpt_pairs, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens,
                                f->pairs);
             if (sc.mixed) {
@@ -2616,14 +2647,17 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
             pq.np = (uint32_t)(p->max_depth - 1) * pq.nt;
             pq.nq = pq.np + 2 * pq.nt;
             pq.stride = (uint32_t)C;
-            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)pq.nq * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
-            pq.queue = (uint32_t*)ptr;
+            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)pq.np * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
+            pq.items_p = (uint32_t*)ptr;
+            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)(pq.nq - pq.np) * (size_t)C, &ptr)) != PRT_OK)
+                return bail(rc);
+            pq.items_e = (uint32_t*)ptr;
             if ((rc = film_alloc(f, sizeof(float4) * (size_t)(pq.np + pq.nt + 1) * (size_t)C, &ptr)) != PRT_OK)
                 return bail(rc);
             pq.res = (float4*)ptr;
             if ((rc = film_alloc(f, sizeof(uint2) * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
             pq.mask = (uint2*)ptr;
-            if ((rc = film_alloc(f, kMaxQueues * kCounterStride * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
+            if ((rc = film_alloc(f, 2 * kCounterStride * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
             pq.count = (uint32_t*)ptr;
         }
     }
