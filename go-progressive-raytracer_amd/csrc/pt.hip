@@ -642,8 +642,11 @@ __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i
 }
 
 // Subpath generation: one thread per sample writes its camera and light subpaths and their lengths.
+#ifndef PRT_PATHS_WAVES
+#define PRT_PATHS_WAVES 3  // waves per SIMD of k_bdpt_paths
+#endif
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_bdpt_paths(DevScene sc, PassParams pp, VertexStore CP, VertexStore LP,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PATHS_WAVES))) void k_bdpt_paths(DevScene sc, PassParams pp, VertexStore CP, VertexStore LP,
                                                        uint32_t* lens, uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
@@ -1248,8 +1251,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR
 }
 
 // Direct-lighting, light-tracing and (0, nc) strategies, one lane per queued sample.
+#ifndef PRT_EDGE_WAVES
+#define PRT_EDGE_WAVES 3  // waves per SIMD of k_bdpt_edge_connect
+#endif
 template <bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_bdpt_edge_connect(DevScene sc, PassParams pp, VertexStore CP,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_EDGE_WAVES))) void k_bdpt_edge_connect(DevScene sc, PassParams pp, VertexStore CP,
                                                               VertexStore LP, const uint32_t* lens, long long* splat,
                                                               PairBuffers pq, uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
