@@ -140,7 +140,12 @@ def main():
         host = host_cpu()
         cores = host["threads"]
         os_ = O.OracleScene(desc)
-        reps = [timed(lambda: os_.intersect(rays, nthreads=cores)) for _ in range(a.cpu_repeats)]
+        res = {}
+
+        def run_all():
+            res["out"] = os_.intersect(rays, nthreads=cores)
+
+        reps = [timed(run_all) for _ in range(a.cpu_repeats)]
         n1 = min(a.rays, 1 << 20)
         t1 = timed(lambda: os_.intersect(rays[:n1], nthreads=1))
         tm = trimmed_mean(reps)
@@ -149,7 +154,8 @@ def main():
                          f"Moller-Trumbore) on {cores} threads, trimmed mean of {len(reps)} runs "
                          f"({', '.join(f'{t:.2f}' for t in reps)} s)",
                "one_thread_value": round(n1 / t1 / 1e6, 3), "one_thread_sample": f"first {n1} rays, {t1:.2f}s",
-               **{k: v for k, v in host.items() if k != "threads"}}
+               **{k: v for k, v in host.items() if k != "threads"},
+               "parity": hit_parity(hits, res["out"][0], res["out"][1])}
 
     secondary = {}
     if not a.no_pt:
@@ -180,6 +186,26 @@ def main():
         "secondary": secondary,
     }
     print(json.dumps(out), flush=True)
+
+
+def hit_parity(hits, ref_t, ref_p):
+    """The timed GPU hits of every ray against the float64 oracle's: primitive-id match rate, mismatches whose
+    two hits lie at the same distance (1e-4 relative: ties at shared edges and vertices), and the relative
+    hit-t error where both hit the same primitive."""
+    import numpy as np
+
+    gp, gt = hits["prim_id"].astype(np.int64), hits["t"].astype(np.float64)
+    match = gp == ref_p
+    both = (gp >= 0) & (ref_p >= 0)
+    with np.errstate(invalid="ignore"):  # misses hold t = inf on both sides
+        tie = both & ~match & (np.abs(gt - ref_t) <= 1e-4 * np.maximum(1.0, np.abs(ref_t)))
+    same = match & both
+    rel = np.abs(gt[same] - ref_t[same]) / np.maximum(np.abs(ref_t[same]), 1e-30)
+    return {"rays": int(len(gp)), "prim_id_match": round(float(match.mean()), 7),
+            "mismatches": int((~match).sum()), "mismatches_at_equal_t": int(tie.sum()),
+            "hit_t_rel_err_max": float(rel.max()) if rel.size else 0.0,
+            "hit_t_rel_err_mean": float(rel.mean()) if rel.size else 0.0,
+            "reference": "float64 oracle (Moller-Trumbore over a midpoint BVH) on the same rays"}
 
 
 def host_cpu():
