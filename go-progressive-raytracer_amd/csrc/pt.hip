@@ -1051,7 +1051,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 //   q = nq - 1:              the camera path's own emission (0, nc).
 // Result planes (the weighted camera-side radiance of a queued sample, w = 1 when it is added): one per general
 // pair and per direct-lighting strategy, then one for (0, nc); light tracing splats directly.
-//   k_bdpt_pairs:         each block of samples lists (sample, q) for every strategy the per-sample loops
+//   k_bdpt_pairs:         stores the MIS walk sums of both subpaths (as k_bdpt_walks), then each block of
+//                         samples lists (sample, q) for every strategy the per-sample loops
 //                         evaluate (end vertices not specular, a material for direct lighting, an emitting last
 //                         camera vertex for (0, nc)) in one contiguous segment per list, ordered by q and then
 //                         sample, and records which strategies in `mask`;
@@ -1118,6 +1119,9 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStor
     if (i < n) {
         const uint32_t len = lens[i];
         const int nc = (int)(len & 0xFFFFu), nl = (int)(len >> 16);
+        // the MIS walk sums of both subpaths first (k_bdpt_walks of the per-sample kernels)
+        store_walk_sums(CP, i, nc, false);
+        store_walk_sums(LP, i, nl, true);
         for (int s = 2; s <= nl; s++)
             if (!rec_specular(LP.rec(s - 1, i))) lm |= 1u << (s - 2);
         for (int t = 2; t <= nc; t++) {
@@ -2299,7 +2303,8 @@ int film_enqueue_chunk(prt_film* f) {
         PRT_HIP(launch_paths(pp));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_PATHS, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, true));
-        hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
+        if (!f->bdpt_pairs)  // the strategy lists store the walk sums in k_bdpt_pairs
+            hipLaunchKernelGGL(k_bdpt_walks, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens);
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_CONNECT, true));
         if (f->bdpt_pairs) {
