@@ -556,6 +556,11 @@ def serve_section(a, log):
            "makespan_s": round(makespan, 2),
            "samples_per_s": round(sum(pix_samples.values()) / makespan, 1),
            "integrators": "path tracing and BDPT, 1:1" if a.serve_bdpt else "path tracing only"}
+    names = ["default", "cornell_metre", "mesh_sky"]
+    slow = sorted(ttfp, key=lambda r: -ttfp[r])[:5]
+    out["ttfp_slowest"] = [{"scene": names[picks[meta[r][0]]], "size": "x".join(map(str, sizes[res[meta[r][0]]])),
+                            "integrator": "bdpt" if integs[meta[r][0]] else "pt", "ttfp_ms": round(ttfp[r], 2),
+                            "post_setup_ms": round(ttfp[r] - admitted.get(r, 0.0), 2)} for r in slow]
     kcpu = 0 if a.no_cpu else min(a.serve_cpu_requests, n)
     if kcpu > 0:
         import oracle as O
