@@ -754,10 +754,18 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_walks(PassParams pp, VertexStor
 // evaluateLightTracingStrategy (:394-471), evaluateConnectionStrategy (:489-561), each with its visibility ray
 // and calculateMISWeight. Light-tracing strategies (t = 1) splat their contribution themselves and return
 // false; the others return true with the weighted camera-side radiance in `out` when it is added.
-template <bool MIXED>
+// Strategy lists of the BDPT work lists (k_bdpt_pairs): general pairs, direct lighting (1, t), light tracing
+// (s, 1), the camera path's own emission (0, nc). A kernel compiled for one kind evaluates only its branch.
+constexpr int kKindAny = -1, kKindGeneral = 0, kKindDirect = 1, kKindLightTrace = 2, kKindEmission = 3;
+constexpr int kNumLists = 4;
+
+template <bool MIXED, int KIND = kKindAny>
 __device__ __forceinline__ bool eval_strategy(const DevScene& sc, const PassParams& pp, const VertexStore& CP,
                                               const VertexStore& LP, uint32_t i, uint32_t key, int s, int t,
                                               long long* splat, TStack<kLds, kBlock>& st, V3& out) {
+    if constexpr (KIND == kKindDirect) s = 1;
+    if constexpr (KIND == kKindLightTrace) t = 1;
+    if constexpr (KIND == kKindEmission) s = 0;
     const DevCamera& cam = sc.cam;
     V3 Lc = v3(0, 0, 0);
     bool splatted = false;
@@ -1070,12 +1078,11 @@ constexpr uint32_t kItemSlotBits = 23;  // list item: slot | q << 23 (slots per 
 constexpr uint32_t kNumXcd = 8;
 
 struct PairBuffers {
-    uint32_t* items_p; // general pairs: slot | q << kItemSlotBits
-    uint32_t* items_e; // other strategies: slot | (q - np) << kItemSlotBits
+    uint32_t* items[kNumLists];  // per strategy kind: slot | (strategy index within the kind) << kItemSlotBits
     float4* res;       // [plane][slot]: weighted contribution, w = 1 if it is added
     uint2* mask;       // per slot: x = light vertices s - 1 not specular (bit s - 2) | camera vertices t - 1 not
                        // specular (bit 16 + t - 2); y = direct-lighting strategies (bit t - 2) | (0, nc) (bit 16)
-    uint32_t* count;   // list lengths: general pairs at count[0], others at count[kCounterStride]
+    uint32_t* count;   // length of list k at count[k * kCounterStride]
     uint32_t nt;       // max_depth
     uint32_t np;       // general pairs
     uint32_t nq;       // strategies: np + 2 * nt
@@ -1110,7 +1117,7 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStor
     __shared__ uint32_t s_cnt[kBlock / 64][kMaxQueues];  // per wave and strategy: entries, then offsets
     __shared__ uint32_t s_off[kMaxQueues];               // the block's entries per strategy, then offsets
     __shared__ uint32_t s_tmp[kBlock];
-    __shared__ uint32_t s_base[2];
+    __shared__ uint32_t s_base[kNumLists];
     const uint32_t n = (uint32_t)pp.npix * (uint32_t)pp.spp;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -1165,22 +1172,23 @@ __global__ __launch_bounds__(kBlock) void k_bdpt_pairs(PassParams pp, VertexStor
         s_off[q] = a;
     }
     __syncthreads();
-    // the block's segment of each list: strategies in order, general pairs first
+    // the block's segment of each list: strategies in order (general pairs, direct lighting, light tracing,
+    // emission), one reservation per list
     const uint32_t all = block_exclusive_scan(s_off, s_tmp);
-    if (threadIdx.x == 0) {
-        const uint32_t tp = s_off[pq.np];
-        s_base[0] = tp ? atomicAdd(pq.count, tp) : 0u;
-        s_base[1] = all > tp ? atomicAdd(pq.count + kCounterStride, all - tp) - tp : 0u;
+    if (threadIdx.x < (uint32_t)kNumLists) {
+        const uint32_t q0[kNumLists + 1] = {0u, pq.np, pq.np + pq.nt, pq.nq - 1, pq.nq};
+        const uint32_t k = threadIdx.x;
+        const uint32_t b0 = s_off[q0[k]], b1 = k + 1 < (uint32_t)kNumLists ? s_off[q0[k + 1]] : all;
+        s_base[k] = (b1 > b0 ? atomicAdd(pq.count + k * kCounterStride, b1 - b0) : 0u) - b0;
     }
     __syncthreads();
     auto push = [&](uint32_t q, bool a) {
         const uint64_t m = __ballot(a);
         if (!a) return;
         const uint32_t pos = s_off[q] + s_cnt[w][q] + (uint32_t)__popcll(m & lt);
-        if (q < pq.np)
-            pq.items_p[s_base[0] + pos] = i | (q << kItemSlotBits);
-        else
-            pq.items_e[s_base[1] + pos] = i | ((q - pq.np) << kItemSlotBits);
+        const uint32_t k = q < qd ? 0u : q < ql ? 1u : q < qz ? 2u : 3u;
+        const uint32_t e = q - (k == 0 ? 0u : k == 1 ? qd : k == 2 ? ql : qz);
+        pq.items[k][s_base[k] + pos] = i | (e << kItemSlotBits);
     };
     for (uint32_t s = 0; s < ns; s++)
         for (uint32_t t = 0; t < nc2; t++) push(s * pq.nt + t, ((lm >> s) & (cm >> t) & 1u) != 0);
@@ -1223,7 +1231,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR
     st.ncache = s_cache;
     const DevCamera& cam = sc.cam;
     for (XcdRange r(*(volatile uint32_t*)pq.count); r.j < r.end; r.j += r.step) {
-        const uint32_t item = pq.items_p[r.j];
+        const uint32_t item = pq.items[kKindGeneral][r.j];
         const uint32_t i = item & ((1u << kItemSlotBits) - 1u), p = item >> kItemSlotBits;
         const int s = (int)(p / pq.nt) + 2, t = (int)(p % pq.nt) + 2;
         const BV ls1 = LP.get(s - 1, i), ct1 = CP.get(t - 1, i);
@@ -1254,14 +1262,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PAIR
     }
 }
 
-// Direct-lighting, light-tracing and (0, nc) strategies, one lane per queued sample.
+// Direct-lighting (KIND = kKindDirect), light-tracing (kKindLightTrace) or (0, nc) (kKindEmission) strategies,
+// one lane per listed sample; a kernel per kind, each compiled for its branch of the strategy.
 #ifndef PRT_EDGE_WAVES
-#define PRT_EDGE_WAVES 3  // waves per SIMD of k_bdpt_edge_connect
+#define PRT_EDGE_WAVES 4  // waves per SIMD of k_bdpt_edge_connect
 #endif
-template <bool MIXED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_EDGE_WAVES))) void k_bdpt_edge_connect(DevScene sc, PassParams pp, VertexStore CP,
-                                                              VertexStore LP, const uint32_t* lens, long long* splat,
-                                                              PairBuffers pq, uint2* gstack, uint32_t gstride) {
+template <bool MIXED, int KIND>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_EDGE_WAVES))) void k_bdpt_edge_connect(
+    DevScene sc, PassParams pp, VertexStore CP, VertexStore LP, const uint32_t* lens, long long* splat,
+    PairBuffers pq, uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
     __shared__ float s_t[kLds * kBlock];
     __shared__ float4 s_cache[kNodeCache * 8];
@@ -1274,18 +1283,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_EDGE
     st.gstride = gstride;
     load_node_cache<kNodeCache, kBlock>(sc, s_cache);
     st.ncache = s_cache;
-    const uint32_t ne = pq.nq - pq.np;
-    for (XcdRange r(*(volatile uint32_t*)(pq.count + kCounterStride)); r.j < r.end; r.j += r.step) {
-        const uint32_t item = pq.items_e[r.j];
+    const uint32_t* items = pq.items[KIND];
+    for (XcdRange r(*(volatile uint32_t*)(pq.count + KIND * kCounterStride)); r.j < r.end; r.j += r.step) {
+        const uint32_t item = items[r.j];
         const uint32_t i = item & ((1u << kItemSlotBits) - 1u), e = item >> kItemSlotBits;
-        int s, t;
-        uint32_t plane = 0xFFFFFFFFu;  // light tracing: no result plane
-        if (e < pq.nt) {
+        int s = 0, t = 0;
+        uint32_t plane = 0;
+        if constexpr (KIND == kKindDirect) {
             s = 1;
             t = (int)e + 2;
             plane = pq.np + e;
-        } else if (e < ne - 1) {
-            s = (int)(e - pq.nt) + 2;
+        } else if constexpr (KIND == kKindLightTrace) {
+            s = (int)e + 2;
             t = 1;
         } else {
             s = 0;
@@ -1296,8 +1305,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_EDGE
         slot_sample(pp, i, pix, sample);
         const uint32_t key = path_key(pp.seed, pix, sample);
         V3 c;
-        const bool add = eval_strategy<MIXED>(sc, pp, CP, LP, i, key, s, t, splat, st, c);
-        if (plane != 0xFFFFFFFFu) pq.plane(plane)[i] = add ? make_float4(c.x, c.y, c.z, 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool add = eval_strategy<MIXED, KIND>(sc, pp, CP, LP, i, key, s, t, splat, st, c);
+        if constexpr (KIND != kKindLightTrace)
+            pq.plane(plane)[i] = add ? make_float4(c.x, c.y, c.z, 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
@@ -1520,7 +1530,7 @@ struct prt_film {
     int bdpt_general_blocks = 0;  // grid of k_bdpt_connect_general (its own occupancy)
     bool bdpt_pairs = false;      // strategies through per-strategy queues
     int bdpt_pair_blocks = 0;     // grid of k_bdpt_pair_connect
-    int bdpt_edge_blocks = 0;     // grid of k_bdpt_edge_connect
+    int bdpt_edge_blocks[kNumLists] = {0, 0, 0, 0};  // grids of k_bdpt_edge_connect per strategy kind
     PairBuffers pairs{};
     VertexStore cpath{}, lpath{};
     uint32_t* lens = nullptr;
@@ -1916,7 +1926,7 @@ size_t bdpt_pair_slot_bytes(int max_depth) {
 // Bytes of the strategy lists, result planes and masks of `slots` path slots, and the list lengths.
 size_t bdpt_pair_bytes(int max_depth, int64_t slots) {
     if (!bdpt_pairs_used(max_depth)) return 0;
-    return bdpt_pair_slot_bytes(max_depth) * (size_t)slots + 2 * kCounterStride * sizeof(uint32_t);
+    return bdpt_pair_slot_bytes(max_depth) * (size_t)slots + kNumLists * kCounterStride * sizeof(uint32_t);
 }
 
 // Path slots of a BDPT film: the two subpath stores stay within 2 GiB, and with the pair queues within 4 GiB.
@@ -2312,21 +2322,27 @@ int film_enqueue_chunk(prt_film* f) {
             // the strategy kernels' grids: a multiple of kNumXcd, at most about one item per lane and sample
             const int64_t cap_blocks = std::max<int64_t>(kNumXcd, ((n + kBlock - 1) / kBlock) / kNumXcd * kNumXcd);
             const int grid_p = (int)std::min<int64_t>(f->bdpt_pair_blocks, cap_blocks);
-            const int grid_e = (int)std::min<int64_t>(f->bdpt_edge_blocks, cap_blocks);
-            PRT_HIP(hipMemsetAsync(f->pairs.count, 0, 2 * kCounterStride * sizeof(uint32_t), st));
+            auto grid_e = [&](int k) { return (int)std::min<int64_t>(f->bdpt_edge_blocks[k], cap_blocks); };
+            PRT_HIP(hipMemsetAsync(f->pairs.count, 0, kNumLists * kCounterStride * sizeof(uint32_t), st));
             hipLaunchKernelGGL(k_bdpt_pairs, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->cpath, f->lpath, f->lens,
                                f->pairs);
+#define PRT_EDGE_LAUNCH(M, K)                                                                                    \
+    hipLaunchKernelGGL((k_bdpt_edge_connect<M, K>), dim3(grid_e(K)), dim3(kBlock), 0, st, sc, pp, f->cpath, f->lpath, \
+                       f->lens, f->fb.splat, f->pairs, f->gstack, f->gslots)
             if (sc.mixed) {
-                hipLaunchKernelGGL((k_bdpt_edge_connect<true>), dim3(grid_e), dim3(kBlock), 0, st, sc, pp, f->cpath,
-                                   f->lpath, f->lens, f->fb.splat, f->pairs, f->gstack, f->gslots);
+                PRT_EDGE_LAUNCH(true, kKindEmission);
+                PRT_EDGE_LAUNCH(true, kKindDirect);
+                PRT_EDGE_LAUNCH(true, kKindLightTrace);
                 hipLaunchKernelGGL((k_bdpt_pair_connect<true>), dim3(grid_p), dim3(kBlock), 0, st, sc, pp, f->cpath,
                                    f->lpath, f->pairs, f->gstack, f->gslots);
             } else {
-                hipLaunchKernelGGL((k_bdpt_edge_connect<false>), dim3(grid_e), dim3(kBlock), 0, st, sc, pp, f->cpath,
-                                   f->lpath, f->lens, f->fb.splat, f->pairs, f->gstack, f->gslots);
+                PRT_EDGE_LAUNCH(false, kKindEmission);
+                PRT_EDGE_LAUNCH(false, kKindDirect);
+                PRT_EDGE_LAUNCH(false, kKindLightTrace);
                 hipLaunchKernelGGL((k_bdpt_pair_connect<false>), dim3(grid_p), dim3(kBlock), 0, st, sc, pp, f->cpath,
                                    f->lpath, f->pairs, f->gstack, f->gslots);
             }
+#undef PRT_EDGE_LAUNCH
             hipLaunchKernelGGL(k_bdpt_pair_sum, dim3(walk_blocks), dim3(kBlock), 0, st, pp, f->pb, f->pairs);
         } else if (sc.mixed) {
             hipLaunchKernelGGL((k_bdpt_connect<true>), dim3(grid), dim3(kBlock), 0, st, sc, pp, f->pb, f->cpath,
@@ -2652,23 +2668,29 @@ int film_create_ex(const prt_scene* scene, const prt_render_params* p, int64_t c
         if (bdpt_pairs_used(p->max_depth)) {
             f->bdpt_pairs = true;
             f->bdpt_pair_blocks = bdpt_queue_blocks(scene, f->blocks, (const void*)k_bdpt_pair_connect<true>);
-            f->bdpt_edge_blocks = bdpt_queue_blocks(scene, f->blocks, (const void*)k_bdpt_edge_connect<true>);
+            f->bdpt_edge_blocks[kKindDirect] =
+                bdpt_queue_blocks(scene, f->blocks, (const void*)k_bdpt_edge_connect<true, kKindDirect>);
+            f->bdpt_edge_blocks[kKindLightTrace] =
+                bdpt_queue_blocks(scene, f->blocks, (const void*)k_bdpt_edge_connect<true, kKindLightTrace>);
+            f->bdpt_edge_blocks[kKindEmission] =
+                bdpt_queue_blocks(scene, f->blocks, (const void*)k_bdpt_edge_connect<true, kKindEmission>);
             PairBuffers& pq = f->pairs;
             pq.nt = (uint32_t)p->max_depth;
             pq.np = (uint32_t)(p->max_depth - 1) * pq.nt;
             pq.nq = pq.np + 2 * pq.nt;
             pq.stride = (uint32_t)C;
-            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)pq.np * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
-            pq.items_p = (uint32_t*)ptr;
-            if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)(pq.nq - pq.np) * (size_t)C, &ptr)) != PRT_OK)
-                return bail(rc);
-            pq.items_e = (uint32_t*)ptr;
+            const uint32_t list_len[kNumLists] = {pq.np, pq.nt, pq.nt - 1, 1};  // strategies per kind
+            for (int k = 0; k < kNumLists; k++) {
+                if ((rc = film_alloc(f, sizeof(uint32_t) * (size_t)list_len[k] * (size_t)C, &ptr)) != PRT_OK)
+                    return bail(rc);
+                pq.items[k] = (uint32_t*)ptr;
+            }
             if ((rc = film_alloc(f, sizeof(float4) * (size_t)(pq.np + pq.nt + 1) * (size_t)C, &ptr)) != PRT_OK)
                 return bail(rc);
             pq.res = (float4*)ptr;
             if ((rc = film_alloc(f, sizeof(uint2) * (size_t)C, &ptr)) != PRT_OK) return bail(rc);
             pq.mask = (uint2*)ptr;
-            if ((rc = film_alloc(f, 2 * kCounterStride * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
+            if ((rc = film_alloc(f, kNumLists * kCounterStride * sizeof(uint32_t), &ptr)) != PRT_OK) return bail(rc);
             pq.count = (uint32_t*)ptr;
         }
     }
