@@ -645,7 +645,9 @@ __device__ int bdpt_extend(const DevScene& sc, const VertexStore& VS, uint32_t i
 #ifndef PRT_PATHS_WAVES
 #define PRT_PATHS_WAVES 3  // waves per SIMD of k_bdpt_paths
 #endif
-template <bool MIXED>
+// SIDES: 3 both subpaths, 1 the camera subpath only, 2 the light subpath only (a kernel per side is smaller; each
+// writes its half of the packed lengths).
+template <bool MIXED, int SIDES = 3>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PATHS_WAVES))) void k_bdpt_paths(DevScene sc, PassParams pp, VertexStore CP, VertexStore LP,
                                                        uint32_t* lens, uint2* gstack, uint32_t gstride) {
     __shared__ uint32_t s_node[kLds * kBlock];
@@ -670,60 +672,69 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PRT_PATH
         slot_sample(pp, i, pix, sample);
         int px = (int)(pix % (uint32_t)pp.width), py = (int)(pix / (uint32_t)pp.width);
         uint32_t key = path_key(pp.seed, pix, sample);
-        V3 o, d;
-        camera_ray(cam, px, py, draw(key, 0), draw(key, 1), draw(key, 2), draw(key, 3), o, d);
-        // camera subpath (generateCameraPath, bdpt.go:106-135)
-        BV cv;
-        cv.p = o;
-        cv.n = d * -1.0f;
-        cv.wi = v3(0, 0, 0);
-        cv.beta = v3(1, 1, 1);
-        cv.Le = v3(0, 0, 0);
-        cv.pdf_fwd = cv.pdf_rev = 0.0f;
-        cv.mat = -1;
-        cv.light = -1;
-        cv.flags = VF_CAMERA;
-        CP.put(0, i, cv);
-        const uint32_t nc = (uint32_t)bdpt_extend<MIXED>(sc, CP, i, 1, o, d, v3(1, 1, 1), cam_dir_pdf(cam, o, d),
-                                                         pp.max_depth, true, key, 1, st, nullptr, pp.trace_gate);
+        uint32_t nc = 0, nl = 0;
+        if constexpr ((SIDES & 1) != 0) {
+            V3 o, d;
+            camera_ray(cam, px, py, draw(key, 0), draw(key, 1), draw(key, 2), draw(key, 3), o, d);
+            // camera subpath (generateCameraPath, bdpt.go:106-135)
+            BV cv;
+            cv.p = o;
+            cv.n = d * -1.0f;
+            cv.wi = v3(0, 0, 0);
+            cv.beta = v3(1, 1, 1);
+            cv.Le = v3(0, 0, 0);
+            cv.pdf_fwd = cv.pdf_rev = 0.0f;
+            cv.mat = -1;
+            cv.light = -1;
+            cv.flags = VF_CAMERA;
+            CP.put(0, i, cv);
+            nc = (uint32_t)bdpt_extend<MIXED>(sc, CP, i, 1, o, d, v3(1, 1, 1), cam_dir_pdf(cam, o, d),
+                                                             pp.max_depth, true, key, 1, st, nullptr, pp.trace_gate);
+        }
         // light subpath (generateLightPath, bdpt.go:139-203)
-        uint32_t nl = 0;
-        if (sc.num_lights > 0) {
-            const uint32_t lp0 = kLightSlot * 16u;
-            float sel;
-            int li = select_light(sc, draw(key, lp0), sel);
-            const DevLight& L = sc.lights[li];
-            ESample es = light_sample_emission(sc, L, draw(key, lp0 + 1), draw(key, lp0 + 2), draw(key, lp0 + 3),
-                                               draw(key, lp0 + 4));
-            float ct = fabsf(dot(es.dir, es.n));
-            BV lv;
-            lv.p = es.p;
-            lv.n = es.n;
-            lv.wi = v3(0, 0, 0);
-            lv.beta = es.emission;
-            lv.Le = es.emission;
-            lv.pdf_fwd = es.area_pdf * sel;
-            lv.pdf_rev = 0.0f;
-            lv.mat = -1;
-            lv.light = li;
-            bool inf = light_is_infinite(L.type);
-            lv.flags = light_vertex_flags(sc, li);
-            LP.put(0, i, lv);
-            V3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
-            nl = (uint32_t)bdpt_extend<MIXED>(sc, LP, i, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false,
-                                              key, kLightSlot + 1, st,
-                                              L.type == PRT_LIGHT_POINT_SPOT ? L.from_lo : nullptr, pp.trace_gate);
-            if (inf) {  // infinite-light density fix-ups (bdpt.go:182-199)
-                if (nl > 1) {
-                    BV fb = LP.get(1, i);
-                    float pf = es.area_pdf * sel;
-                    if (fb.mat >= 0) pf *= fabsf(dot(es.dir, fb.n));
-                    LP.set_pdf_fwd(1, i, pf);
+        if constexpr ((SIDES & 2) != 0) {
+            if (sc.num_lights > 0) {
+                const uint32_t lp0 = kLightSlot * 16u;
+                float sel;
+                int li = select_light(sc, draw(key, lp0), sel);
+                const DevLight& L = sc.lights[li];
+                ESample es = light_sample_emission(sc, L, draw(key, lp0 + 1), draw(key, lp0 + 2), draw(key, lp0 + 3),
+                                                   draw(key, lp0 + 4));
+                float ct = fabsf(dot(es.dir, es.n));
+                BV lv;
+                lv.p = es.p;
+                lv.n = es.n;
+                lv.wi = v3(0, 0, 0);
+                lv.beta = es.emission;
+                lv.Le = es.emission;
+                lv.pdf_fwd = es.area_pdf * sel;
+                lv.pdf_rev = 0.0f;
+                lv.mat = -1;
+                lv.light = li;
+                bool inf = light_is_infinite(L.type);
+                lv.flags = light_vertex_flags(sc, li);
+                LP.put(0, i, lv);
+                V3 beta = es.emission * (ct / (sel * es.area_pdf * es.dir_pdf));
+                nl = (uint32_t)bdpt_extend<MIXED>(sc, LP, i, 1, es.p, es.dir, beta, es.dir_pdf, pp.max_depth - 1, false,
+                                                  key, kLightSlot + 1, st,
+                                                  L.type == PRT_LIGHT_POINT_SPOT ? L.from_lo : nullptr, pp.trace_gate);
+                if (inf) {  // infinite-light density fix-ups (bdpt.go:182-199)
+                    if (nl > 1) {
+                        BV fb = LP.get(1, i);
+                        float pf = es.area_pdf * sel;
+                        if (fb.mat >= 0) pf *= fabsf(dot(es.dir, fb.n));
+                        LP.set_pdf_fwd(1, i, pf);
+                    }
+                    LP.set_pdf_fwd(0, i, infinite_density(sc, es.p, es.n, es.dir));
                 }
-                LP.set_pdf_fwd(0, i, infinite_density(sc, es.p, es.n, es.dir));
             }
         }
-        lens[i] = nc | (nl << 16);
+        if constexpr (SIDES == 3)
+            lens[i] = nc | (nl << 16);
+        else if constexpr (SIDES == 1)
+            reinterpret_cast<uint16_t*>(lens)[2 * i] = (uint16_t)nc;
+        else
+            reinterpret_cast<uint16_t*>(lens)[2 * i + 1] = (uint16_t)nl;
     }
 }
 
@@ -2273,13 +2284,23 @@ int film_enqueue_chunk(prt_film* f) {
         int grid = (int)std::min<int64_t>(f->bdpt_blocks, (n + kBlock - 1) / kBlock);
         int grid_g = (int)std::min<int64_t>(f->bdpt_general_blocks, (n + kBlock - 1) / kBlock);
         const int walk_blocks = (int)((n + kBlock - 1) / kBlock);
-        auto launch_paths = [&](const PassParams& q) {
-            if (sc.mixed)
-                hipLaunchKernelGGL((k_bdpt_paths<true>), dim3(grid), dim3(kBlock), 0, st, sc, q, f->cpath, f->lpath,
-                                   f->lens, f->gstack, f->gslots);
-            else
-                hipLaunchKernelGGL((k_bdpt_paths<false>), dim3(grid), dim3(kBlock), 0, st, sc, q, f->cpath,
-                                   f->lpath, f->lens, f->gstack, f->gslots);
+        // tuning runs time both subpaths in one kernel; regular launches run a kernel per side
+        auto launch_paths = [&](const PassParams& q, bool split = false) {
+#define PRT_PATHS_LAUNCH(M, S)                                                                                  \
+    hipLaunchKernelGGL((k_bdpt_paths<M, S>), dim3(grid), dim3(kBlock), 0, st, sc, q, f->cpath, f->lpath, f->lens, \
+                       f->gstack, f->gslots)
+            if (split && sc.mixed) {
+                PRT_PATHS_LAUNCH(true, 1);
+                PRT_PATHS_LAUNCH(true, 2);
+            } else if (split) {
+                PRT_PATHS_LAUNCH(false, 1);
+                PRT_PATHS_LAUNCH(false, 2);
+            } else if (sc.mixed) {
+                PRT_PATHS_LAUNCH(true, 3);
+            } else {
+                PRT_PATHS_LAUNCH(false, 3);
+            }
+#undef PRT_PATHS_LAUNCH
             return hipGetLastError();
         };
         // per-scene leaf gate of the BDPT traversals, on a chunk of at least 2^18 paths that is not a one-sample
@@ -2310,7 +2331,7 @@ int film_enqueue_chunk(prt_film* f) {
         }
         pp.trace_gate = std::max(0, ms->bdpt_trace_gate.load());
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_PATHS, true));
-        PRT_HIP(launch_paths(pp));
+        PRT_HIP(launch_paths(pp, true));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_PATHS, false));
         PRT_HIP(probe_mark(f, PRT_PROBE_KIND_BDPT_WALKS, true));
         if (!f->bdpt_pairs)  // the strategy lists store the walk sums in k_bdpt_pairs
