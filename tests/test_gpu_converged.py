@@ -10,9 +10,11 @@ seeds). Bars:
   independent oracle render at n samples gives (heavy-tailed pixels such as caustics exceed the normal
   tail at low sample counts; measured per configuration when the fixture was made), plus 4 binomial
   standard errors;
-* the mean z over the image is within +-0.02 (a dropped or misweighted estimator term shifts it by
-  many times that);
-* the spread of z is at most 1.2 x the calibration's plus 0.05.
+* the mean of z clipped to +-10 over the image is within 0.02 plus 3 standard deviations of the calibration
+  renders' values of their average (a dropped or misweighted estimator term shifts it by many times that);
+* the spread of the clipped z is at most 1.2 x the calibration's plus 0.05.
+z is clipped for these two because a pixel that catches a rare bright path (a sun caustic through the glass
+mesh) at a few hundred samples has a z in the hundreds, which alone would set the unclipped mean and spread.
 """
 import hashlib
 import json
@@ -63,11 +65,15 @@ def test_gpu_image_within_3_sigma_of_converged(name):
     cal = man["calibration"]
     p0 = max(0.0027, max(c["frac_abs_z_gt_3"] for c in cal))
     allow = 1.25 * p0 + 4 * math.sqrt(p0 * (1 - p0) / z.size)
-    std_allow = 1.2 * max(c["std_z"] for c in cal) + 0.05
+    zc = np.clip(z, -10.0, 10.0)
+    std_allow = 1.2 * max(c["std_zc"] for c in cal) + 0.05
+    mz = [c["mean_zc"] for c in cal]
+    mz_mid, mz_sd = float(np.mean(mz)), float(np.std(mz, ddof=1))
     print(f"{name}: |z|>3 fraction {frac:.5f} (allowed {allow:.5f}; oracle at {n} spp "
-          f"{[round(c['frac_abs_z_gt_3'], 5) for c in cal]}), mean z {z.mean():+.4f}, std z {z.std():.3f} "
-          f"(allowed {std_allow:.3f}); RMSE {np.sqrt(((g - mean) ** 2).mean()):.5f}, "
+          f"{[round(c['frac_abs_z_gt_3'], 5) for c in cal]}), clipped z mean {zc.mean():+.4f} (allowed "
+          f"{mz_mid:+.4f} +- {0.02 + 3 * mz_sd:.4f}), std {zc.std():.3f} (allowed {std_allow:.3f}); unclipped "
+          f"z mean {z.mean():+.4f} std {z.std():.3f}; RMSE {np.sqrt(((g - mean) ** 2).mean()):.5f}, "
           f"image mean gpu {g.mean():.5f} fixture {mean.mean():.5f}")
     assert frac <= allow
-    assert abs(float(z.mean())) <= 0.02
-    assert float(z.std()) <= std_allow
+    assert abs(float(zc.mean()) - mz_mid) <= 0.02 + 3 * mz_sd
+    assert float(zc.std()) <= std_allow
